@@ -1,0 +1,130 @@
+#!/usr/bin/env python3
+"""Throughput of the MPM substep on one MI355X.
+
+Workload (the single-GPU configuration the metric is quoted on): 4,194,304
+Drucker-Prager snow particles (E=1.4e5, nu=0.2, rho0=400, M=tan 30deg,
+cohesion 500) in an 8-cell layer over a seeded six-mode sine terrain given as
+a dense SDF, 256^3 grid on the unit box, APIC, gravity -9.8, dt=5e-5.  A step
+is one full substep (bin, P2G, grid update with SDF BC, G2P) over all
+particles, inputs resident in HBM.  `value` = particles x steps / seconds.
+
+Also printed in the JSON line:
+  roofline      the dominant kernel's algorithmic bytes per launch over its
+                average HIP-event duration on the engine's stream, against the
+                8 TB/s HBM peak (bytes per particle / per active node as in the
+                substep model B = 288 N + 68 G_act: bin 20 N + 8 G, P2G 92 N +
+                16 G, grid 28 G, G2P 176 N + 16 G)
+  cpu_baseline  the FP32 CPU oracle (a restatement of the reference's shaders,
+                OpenMP over the host cores) timed on a bounded sample: a few
+                substeps of the same scene
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-substeps/s at 4M DP particles (1 GPU); fwd+adjoint / fwd time"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def kernel_bytes(name, n, g_act):
+    return {
+        "p2g": 92 * n + 16 * g_act,
+        "g2p": 176 * n + 16 * g_act,
+        "grid_update": 28 * g_act,
+    }.get(name)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--profile-steps", type=int, default=20, help="substeps timed per kernel with HIP events")
+    ap.add_argument("--cpu-steps", type=int, default=5, help="substeps of the CPU oracle sample (0 = skip)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the single-scene substep is not sharded across GPUs)", file=sys.stderr)
+        sys.exit(2)
+
+    from nebula_amd import mpm
+    from nebula_amd.scenes import make_engine, snow_terrain_256
+
+    log("[bench] building scene snow_terrain_256")
+    sc = snow_terrain_256()
+    n = sc.n
+    eng = make_engine(sc)
+    log(f"[bench] {n} particles, grid {sc.grid_res()}, warmup {args.warmup}")
+    eng.substep(sc.dt, args.warmup)
+    eng.sync()
+    log(f"[bench] timing {args.steps} substeps")
+    t0 = time.perf_counter()
+    eng.substep(sc.dt, args.steps)
+    eng.sync()
+    elapsed = time.perf_counter() - t0
+    value = n * args.steps / elapsed
+    ms_per_step = elapsed / args.steps * 1e3
+    log(f"[bench] {ms_per_step:.4f} ms/substep, {value:.4e} particle-substeps/s")
+
+    g_act = eng.activeNodeCount()
+    tiles = eng.activeTileCount()
+    log(f"[bench] profiling {args.profile_steps} substeps per kernel")
+    prof = eng.profileSubsteps(sc.dt, args.profile_steps)
+    avg_ms = {k: v / args.profile_steps for k, v in prof.items()}
+    dom = max(("p2g", "g2p", "grid_update"), key=lambda k: avg_ms.get(k, 0.0))
+    dbytes = kernel_bytes(dom, n, g_act)
+    achieved = dbytes / (avg_ms[dom] * 1e-3) / 1e9
+    b_alg = 288 * n + 68 * g_act
+    roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "bytes_per_launch": dbytes, "avg_launch_ms": round(avg_ms[dom], 5)}
+    substep_rate = b_alg / (ms_per_step * 1e-3) / 1e9
+
+    cpu = None
+    if args.cpu_steps > 0:
+        import oracle
+
+        if not os.path.exists(oracle.LIB_PATH):
+            oracle.build()
+        log(f"[bench] CPU oracle sample: {args.cpu_steps} substeps of the same scene")
+        scn = oracle.make_scene(sc.dt, sc.cfg.cellSize, sc.grid_res(), (0.0, 0.0, 0.0), tuple(sc.cfg.domainSize),
+                                sc.flip_ratio)
+        sim = oracle.Sim(scn, mpm.materials_bytes(sc.materials), sc.colliders.to_bytes(),
+                         mpm.forces_bytes(sc.forces), sc.sdf, 32)
+        sim.set_state(sc.fields)
+        secs = sim.substeps(args.cpu_steps)
+        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        cpu = {"value": round(n * args.cpu_steps / secs, 1), "unit": "particle-substeps/s", "cores": cores,
+               "kind": "port",
+               "sample": f"FP32 CPU oracle (C++/OpenMP restatement of the reference shaders), "
+                         f"{args.cpu_steps} substeps of the full {n}-particle scene, {secs:.2f} s"}
+        log(f"[bench] cpu {cpu['value']:.4e} particle-substeps/s on {cores} threads")
+
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "particle-substeps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic: seeded six-mode sine terrain SDF (seed 3), 8 jittered particles per cell",
+        "config": {"workload": "snow_terrain_256: 4,194,304 Drucker-Prager snow particles over a terrain SDF, "
+                               "256^3 grid, APIC, dt 5e-5", "particles": n, "grid": list(sc.grid_res()),
+                   "active_nodes": g_act, "active_tiles": tiles, "parallelism": "single GPU"},
+        "roofline": roofline,
+        "substep_roofline": {"bytes_per_step": b_alg, "achieved": round(substep_rate, 2), "peak": HBM_PEAK_GBS,
+                             "unit": "GB/s", "frac": round(substep_rate / HBM_PEAK_GBS, 4)},
+        "kernels_avg_ms": {k: round(v, 5) for k, v in avg_ms.items()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,262 @@
+/*
+ * nebula_mpm.h — C ABI of the MI355X-native MPM substep.
+ *
+ * This is the drop-in boundary for the reference's `class MPMEngine`
+ * (src/engine/MPMEngine.h:59-186, src/engine/EngineBase.h:23-66).  Every entry
+ * point below names the reference member it replaces.  The ABI is plain C:
+ * opaque handle, POD structs, host or device pointers plus sizes, an `int`
+ * status (0 = ok, negative = error) and a thread-local error string.  No C++
+ * exception crosses it and no torch type appears in it.
+ *
+ * Differences from the Vulkan surface, by design:
+ *   - `step(VkCommandBuffer, dt)` becomes `nb_mpm_step(h, dt)`: work is enqueued
+ *     on the engine's HIP stream (its own, or the one handed to create); the
+ *     caller synchronises with `nb_mpm_sync` or its own stream wait.
+ *   - the runtime shaderc rebuild of the grid-update shader on every force change
+ *     (src/core/ForceShaderCompiler.cpp:139-166) becomes a compiled device-side
+ *     switch over the 32-byte force records; `nb_mpm_set_forces` only uploads.
+ *   - setters that the reference leaves unchecked (setColliders writes past its
+ *     64-slot buffer, src/engine/MPMEngine.cpp:260-265) return NB_ERR_RANGE.
+ *
+ * Particle state uses the reference's layout (src/engine/MPMEngine.cpp:46-53,
+ * shaders/mpm_common.glsl:98-122): eight float4 arrays of length max_particles
+ *   P  = (x, y, z, Vp)            v  = (vx, vy, vz, bits(material id))
+ *   F0 = (F col 0, tau_xx)        B0 = (B col 0, tau_xy)
+ *   F1 = (F col 1, tau_yy)        B1 = (B col 1, tau_xz)
+ *   F2 = (F col 2, tau_zz)        B2 = (B col 2, tau_yz)
+ * The material id is a uint32 stored bit-for-bit in v.w.
+ */
+#ifndef NEBULA_MPM_H
+#define NEBULA_MPM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NB_MPM_ABI_VERSION 1
+
+/* status codes */
+#define NB_OK 0
+#define NB_ERR_ARG (-1)      /* null handle/pointer or malformed argument     */
+#define NB_ERR_RANGE (-2)    /* count or index beyond a documented capacity   */
+#define NB_ERR_HIP (-3)      /* a HIP runtime call failed (message has detail) */
+#define NB_ERR_STATE (-4)    /* call not valid in the engine's current state  */
+#define NB_ERR_UNSUPPORTED (-5)
+
+/* capacities, as allocated by the reference (src/engine/MPMEngine.cpp:75,82;
+ * src/engine/EngineBase.h:34) */
+#define NB_MAX_MATERIALS 16u
+#define NB_MAX_COLLIDERS 64u
+#define NB_MAX_FORCES 32u
+
+/* material models — src/core/MaterialParams.h:5-12 */
+#define NB_MAT_ELASTIC 0u
+#define NB_MAT_VON_MISES 1u
+#define NB_MAT_DRUCKER_PRAGER 2u
+#define NB_MAT_GRANULAR_POWDER 3u
+#define NB_MAT_FLUID 4u
+#define NB_MAT_VISCOPLASTIC_MUD 5u
+
+/* analytic collider types — src/core/Collider.h:8-13 */
+#define NB_COLL_PLANE 0u
+#define NB_COLL_SPHERE 1u
+#define NB_COLL_BOX 2u
+#define NB_COLL_CAPSULE 3u
+
+/* force types — src/core/Force.h:13-19 */
+#define NB_FORCE_GRAVITY 0u
+#define NB_FORCE_CONSTANT_WIND 1u
+#define NB_FORCE_TURBULENCE 2u
+#define NB_FORCE_NOISE 3u
+#define NB_FORCE_Z_CLAMP 4u
+
+/* particle fields (the eight float4 arrays above) */
+#define NB_FIELD_P 0
+#define NB_FIELD_V 1
+#define NB_FIELD_F0 2
+#define NB_FIELD_F1 3
+#define NB_FIELD_F2 4
+#define NB_FIELD_B0 5
+#define NB_FIELD_B1 6
+#define NB_FIELD_B2 7
+#define NB_NUM_FIELDS 8
+
+/* 64-byte material record, layout of MaterialParams (src/core/MaterialParams.h:16-38) */
+typedef struct nb_material {
+  float mu;          /* 0  Lame mu = E/(2(1+nu))                 */
+  float lambda;      /* 4  Lame lambda = E nu/((1+nu)(1-2nu))    */
+  float rho0;        /* 8  rest density; particle mass = rho0*Vp */
+  uint32_t model;    /* 12 NB_MAT_*                               */
+  float M_friction;  /* 16 Drucker-Prager M = tan(friction angle) */
+  float q_cohesion;  /* 20 Drucker-Prager cohesion               */
+  float q_max;       /* 24 Von Mises yield (on Hencky dev strain) */
+  float bulkK;       /* 28 fluid bulk modulus                     */
+  float fluidGamma;  /* 32 */
+  float viscosity;   /* 36 viscoplastic mud viscosity            */
+  float hardening;   /* 40 carried, unread (as in the reference)  */
+  float xi;          /* 44 carried, unread                        */
+  float pad[4];      /* 48..63 */
+} nb_material;
+
+/* 64-byte analytic collider record, layout of ColliderPrimitive (src/core/Collider.h:19-29) */
+typedef struct nb_collider {
+  uint32_t type;      /* NB_COLL_*                                     */
+  float px, py, pz;   /* point on plane / centre / capsule start        */
+  float nx, ny, nz;   /* plane normal / box half extents / capsule axis */
+  float radius;       /* sphere or capsule radius                       */
+  float vx, vy, vz;   /* collider velocity                              */
+  float restitution;
+  float friction;
+  float pad[3];
+} nb_collider;
+
+/* 32-byte force record, layout of ForceGPU (src/core/Force.h:27-32):
+ * data[] meaning per type follows Force.h (gravity/wind: dir xyz in data[0..2],
+ * strength in data[3]). */
+typedef struct nb_force {
+  uint16_t type;        /* NB_FORCE_* */
+  uint16_t affect_mask; /* 0 = applies to every node */
+  float data[7];
+} nb_force;
+
+/* MPMConfig (src/engine/MPMEngine.h:22-55).  Particles of the default block are
+ * an nx*ny*nz lattice placed as MPMEngine::init does (src/engine/MPMEngine.cpp:86-137). */
+typedef struct nb_mpm_config {
+  uint32_t nx, ny, nz;
+  float domain_size[3]; /* physical size [m]; worldMin = 0, worldMax = domain_size */
+  float cell_size;      /* h, shared by all axes */
+  uint32_t max_particles; /* 0 = nx*ny*nz */
+  float E, nu, rho0;    /* default material (presetJelly) */
+} nb_mpm_config;
+
+/* The public knobs of MPMEngine (src/engine/MPMEngine.h:72-80). */
+typedef struct nb_mpm_knobs {
+  float restitution;     /* carried, unread (as in the reference) */
+  float wall_friction;   /* carried, unread */
+  int32_t num_substeps;  /* substeps per nb_mpm_step; dt_sub = dt/max(1,n) */
+  uint32_t plastic_model;/* used only when no material table is set (never, as in the reference) */
+  float q_max;
+  float M_friction;
+  float q_cohesion;
+  float flip_ratio;      /* 0 = PIC, < -0.5 = APIC, > 0.01 = FLIP blend */
+} nb_mpm_knobs;
+
+typedef struct nb_mpm nb_mpm; /* opaque engine handle */
+
+/* Thread-local description of the last failed call on this thread. */
+const char* nb_last_error(void);
+int nb_mpm_abi_version(void);
+
+/* Fill cfg with MPMConfig's member defaults (src/engine/MPMEngine.h:25-39). */
+void nb_mpm_config_default(nb_mpm_config* cfg);
+/* Fill knobs with MPMEngine's member defaults (src/engine/MPMEngine.h:72-80). */
+void nb_mpm_knobs_default(nb_mpm_knobs* knobs);
+
+/* MPMEngine::init (src/engine/MPMEngine.cpp:34-156).  hip_stream: a hipStream_t
+ * to enqueue on, or NULL for an engine-owned stream of default priority. */
+int nb_mpm_create(const nb_mpm_config* cfg, void* hip_stream, nb_mpm** out);
+/* MPMEngine::cleanup (src/engine/MPMEngine.cpp:160-163). */
+int nb_mpm_destroy(nb_mpm* h);
+
+/* MPMEngine::step (src/engine/MPMEngine.cpp:404-474): forces, then
+ * knobs.num_substeps substeps of dt/num_substeps.  Asynchronous on the stream. */
+int nb_mpm_step(nb_mpm* h, float dt);
+/* One substep of size dt_sub (the body of the reference's substep loop). */
+int nb_mpm_substep(nb_mpm* h, float dt_sub);
+/* Enqueue n substeps of size dt_sub. */
+int nb_mpm_substeps(nb_mpm* h, float dt_sub, uint32_t n);
+/* Wait for all work enqueued on the engine's stream. */
+int nb_mpm_sync(nb_mpm* h);
+
+int nb_mpm_set_knobs(nb_mpm* h, const nb_mpm_knobs* knobs);
+int nb_mpm_get_knobs(const nb_mpm* h, nb_mpm_knobs* knobs);
+int nb_mpm_get_config(const nb_mpm* h, nb_mpm_config* cfg);
+
+/* Domain helpers — src/core/Domain.h:14-39 and MPMConfig::totalCells. */
+int nb_mpm_grid_res(const nb_mpm* h, uint32_t res[3]);
+int nb_mpm_total_cells(const nb_mpm* h, uint32_t* total_cells);
+/* MPMEngine::liveParticleCount. */
+int nb_mpm_live_particle_count(const nb_mpm* h, uint32_t* n);
+
+/* MPMEngine::setMaterials (src/engine/MPMEngine.cpp:239-243): n == 0 is a no-op,
+ * n > NB_MAX_MATERIALS is NB_ERR_RANGE. */
+int nb_mpm_set_materials(nb_mpm* h, const nb_material* mats, uint32_t n);
+/* MPMEngine::setParticleMaterialIds (src/engine/MPMEngine.cpp:245-256): overwrites
+ * v[0..n) with (0, 0, 0, bits(id)). */
+int nb_mpm_set_particle_material_ids(nb_mpm* h, const uint32_t* ids, uint32_t n);
+/* MPMEngine::setColliders / clearAnalyticColliders (src/engine/MPMEngine.cpp:260-267). */
+int nb_mpm_set_colliders(nb_mpm* h, const nb_collider* cols, uint32_t n);
+int nb_mpm_clear_analytic_colliders(nb_mpm* h);
+/* MPMEngine::setColliderSDF (src/engine/MPMEngine.cpp:204-209): a dense float
+ * SDF indexed by Morton cell id; n must equal the total cell count. */
+int nb_mpm_set_collider_sdf(nb_mpm* h, const float* morton_sdf, uint64_t n);
+/* MPMEngine::setColliderSphere / clearCollider (src/engine/MPMEngine.cpp:180-202). */
+int nb_mpm_set_collider_sphere(nb_mpm* h, float radius, float cx, float cy, float cz);
+int nb_mpm_clear_collider(nb_mpm* h);
+/* EngineBase::setForces / clearForces (src/engine/EngineBase.cpp:22-40).
+ * Gravity, constant wind and z-clamp (a no-op on a grid, as in the reference)
+ * are supported; turbulence and noise return NB_ERR_UNSUPPORTED. */
+int nb_mpm_set_forces(nb_mpm* h, const nb_force* forces, uint32_t n);
+
+/* MPMEngine::appendParticles (src/engine/MPMEngine.cpp:211-235): host arrays of
+ * n float4 (pos = (x, Vp), vel = (v, bits(id))); F = I, B = 0, tau = 0; silently
+ * clamped to capacity.  *appended (may be NULL) receives the count written. */
+int nb_mpm_append_particles(nb_mpm* h, const float* pos4, const float* vel4, uint32_t n, uint32_t* appended);
+/* MPMEngine::resetParticles (src/engine/MPMEngine.cpp:281-287). */
+int nb_mpm_reset_particles(nb_mpm* h);
+
+/* Bulk state access in the reference's float4 layout (host pointers,
+ * synchronous).  Rows [first, first+count) of field NB_FIELD_*. */
+int nb_mpm_get_field(nb_mpm* h, int field, float* out4, uint32_t first, uint32_t count);
+int nb_mpm_set_field(nb_mpm* h, int field, const float* in4, uint32_t first, uint32_t count);
+/* Device pointer of a field (what getPositionBuffer/getVelocityBuffer return). */
+int nb_mpm_field_device_ptr(nb_mpm* h, int field, void** dptr);
+/* Set the live particle count directly (<= capacity); used with set_field. */
+int nb_mpm_set_live_particle_count(nb_mpm* h, uint32_t n);
+
+/* ---- pass-level access, the counterpart of tests/helpers/MPMHarness.h:56-59 ----
+ * These run one stage of the substep on the current state so tests can isolate
+ * it, and export the intermediate buffers in the reference's dense formats. */
+
+/* Binning (K2-K7: zero_cells, mpm_hash_count, hash_scan_*, mpm_hash_sort). */
+int nb_mpm_run_bin(nb_mpm* h);
+/* After nb_mpm_run_bin: keys[N] (Morton cell id per particle), sorted[N]
+ * (particle indices, stable by index within a cell), cell_count[NC] and
+ * cell_end[NC] (END offsets, the reference's cellOffset after mpm_hash_sort).
+ * Any pointer may be NULL. */
+int nb_mpm_get_bins(nb_mpm* h, uint32_t* keys, uint32_t* sorted, uint32_t* cell_count, uint32_t* cell_end);
+/* P2G (K1+K8) on the current bins; requires nb_mpm_run_bin first. */
+int nb_mpm_run_p2g(nb_mpm* h, float dt_sub);
+/* Grid update + dense-SDF BC (K9+K10) on the current grid. */
+int nb_mpm_run_grid_update(nb_mpm* h, float dt_sub);
+/* G2P (K11) on the current grid. */
+int nb_mpm_run_g2p(nb_mpm* h, float dt_sub);
+/* Dense grid export in the reference format: mom4 = 2*NC float4 ([0,NC) momentum
+ * or velocity, [NC,2NC) saved pre-force velocity), mass = NC floats.  Nodes no
+ * particle reaches read as zero. */
+int nb_mpm_get_grid(nb_mpm* h, float* mom4, float* mass);
+/* Dense grid import (MPMHarness::uploadGrid, tests/helpers/MPMHarness.cpp:182-188):
+ * mom4 = NC float4 (the [0,NC) half), mass = NC floats, old4 = NC float4 of
+ * saved pre-force velocity (the [NC,2NC) half) or NULL for zeros.  Makes the
+ * whole grid visible to the next nb_mpm_run_grid_update / nb_mpm_run_g2p. */
+int nb_mpm_set_grid(nb_mpm* h, const float* mom4, const float* mass, const float* old4);
+
+/* ---- measurement ----
+ * Run n substeps of dt_sub, recording HIP events around every kernel launch on
+ * the engine's stream.  kernel_ms[i] receives the summed duration of kernel i,
+ * names are written (comma separated) into names (capacity names_cap). */
+int nb_mpm_profile_substeps(nb_mpm* h, float dt_sub, uint32_t n, float* kernel_ms, uint32_t kernel_cap,
+                            uint32_t* n_kernels, char* names, uint32_t names_cap);
+/* Count of grid nodes with mass >= 1e-10 in the current grid (G_act). */
+int nb_mpm_active_node_count(nb_mpm* h, uint64_t* g_act);
+/* Number of occupied 8^3 cell tiles after the last bin. */
+int nb_mpm_active_tile_count(nb_mpm* h, uint32_t* tiles);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NEBULA_MPM_H */

@@ -1,0 +1,726 @@
+// mpm_engine.hip — host side of the MI355X MPM engine: lifecycle, state
+// setters/getters and substep orchestration behind the C ABI declared in
+// include/nebula_mpm.h.  Counterpart of the reference's MPMEngine
+// (src/engine/MPMEngine.cpp) with HIP allocations and launches in place of the
+// bindless Vulkan buffer table and pipelines.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/nebula_mpm.h"
+#include "nb_device_math.hpp"
+#include "nb_engine.hpp"
+
+using namespace nb;
+
+static_assert(sizeof(nb_material) == 64, "nb_material mirrors the 64-byte MaterialParams");
+static_assert(sizeof(nb_collider) == 64, "nb_collider mirrors the 64-byte ColliderPrimitive");
+static_assert(sizeof(nb_force) == 32, "nb_force mirrors the 32-byte ForceGPU");
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define NB_HIP(call)                                                                          \
+  do {                                                                                        \
+    hipError_t e_ = (call);                                                                   \
+    if (e_ != hipSuccess) return fail(NB_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+enum class GridMode { NONE, SPARSE, DENSE };
+
+}  // namespace
+
+struct nb_mpm {
+  nb_mpm_config cfg{};
+  nb_mpm_knobs knobs{};
+  uint32_t grid_res[3] = {1, 1, 1};
+  uint32_t cube_res = 1, NC = 1, tile_dim = 1, NT = 1, cap = 0, cap_tiles = 1;
+  uint32_t n_live = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  DevState d{};
+  uint32_t material_count = 1, collider_count = 0, force_count = 0;
+  bool has_sdf = false;
+  bool bins_valid = false;
+  GridMode grid_mode = GridMode::NONE;
+  bool old_valid = false;
+  std::vector<void*> allocs;
+
+  template <class T>
+  int alloc(T** p, size_t count) {
+    void* q = nullptr;
+    size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess) return fail(NB_ERR_HIP, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    allocs.push_back(q);
+    *p = reinterpret_cast<T*>(q);
+    return NB_OK;
+  }
+
+  // device pointers for a launch: the SDF only while a collider SDF is set
+  DevState dev() const {
+    DevState v = d;
+    if (!has_sdf) v.sdf = nullptr;
+    return v;
+  }
+
+  SubParams params(float dt, bool write_old) const {
+    SubParams p{};
+    p.dt = dt;
+    p.h = cfg.cell_size;
+    for (int a = 0; a < 3; a++) {
+      p.wmin[a] = 0.f;  // worldMin = 0, worldMax = domainSize (src/engine/MPMEngine.cpp:373-374)
+      p.wmax[a] = cfg.domain_size[a];
+      p.grid_res[a] = int32_t(grid_res[a]);
+    }
+    p.cube_res = cube_res;
+    p.tile_dim = tile_dim;
+    p.n = n_live;
+    p.flip_ratio = knobs.flip_ratio;
+    p.material_count = material_count;
+    p.collider_count = collider_count;
+    p.force_count = force_count;
+    p.write_old = write_old ? 1u : 0u;
+    return p;
+  }
+};
+
+namespace {
+
+// src/core/Domain.h:14-37
+void domain_dims(const nb_mpm_config& c, uint32_t res[3], uint32_t& cube) {
+  const float h = std::max(c.cell_size, 1e-6f);
+  uint32_t m = 1;
+  for (int a = 0; a < 3; a++) {
+    float r = std::ceil(c.domain_size[a] / h);
+    res[a] = r < 1.f ? 1u : uint32_t(r);
+    m = std::max(m, res[a]);
+  }
+  cube = 1;
+  while (cube < m) cube <<= 1;
+}
+
+nb_material preset_jelly(float E, float nu, float rho0) {  // src/core/MaterialParams.h:42-58
+  nb_material m{};
+  m.mu = E / (2.0f * (1.0f + nu));
+  m.lambda = E * nu / ((1.0f + nu) * (1.0f - 2.0f * nu));
+  m.rho0 = rho0;
+  m.model = NB_MAT_ELASTIC;
+  return m;
+}
+
+int upload(nb_mpm* h, void* dst, const void* src, size_t bytes) {
+  if (!bytes) return NB_OK;
+  NB_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, h->stream));
+  NB_HIP(hipStreamSynchronize(h->stream));
+  return NB_OK;
+}
+int download(nb_mpm* h, void* dst, const void* src, size_t bytes) {
+  if (!bytes) return NB_OK;
+  NB_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, h->stream));
+  NB_HIP(hipStreamSynchronize(h->stream));
+  return NB_OK;
+}
+
+int enqueue_substep(nb_mpm* h, float dt, Prof* pf) {
+  if (h->n_live == 0) return NB_OK;
+  const bool flip = h->knobs.flip_ratio > 0.01f;
+  SubParams p = h->params(dt, flip);
+  launch_bin(h->dev(), p, h->NT, h->stream, pf);
+  launch_p2g(h->dev(), p, h->NT, h->stream, pf);
+  launch_grid(h->dev(), p, h->NT, true, true, h->d.node_tiles, h->d.counters + CNT_NODE_TILES, h->stream, pf);
+  launch_g2p(h->dev(), p, true, h->stream, pf);
+  NB_HIP(hipGetLastError());
+  h->bins_valid = true;
+  h->grid_mode = GridMode::SPARSE;
+  h->old_valid = flip;
+  return NB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* nb_last_error(void) { return g_err.c_str(); }
+int nb_mpm_abi_version(void) { return NB_MPM_ABI_VERSION; }
+
+void nb_mpm_config_default(nb_mpm_config* c) {
+  if (!c) return;
+  std::memset(c, 0, sizeof(*c));
+  c->nx = c->ny = c->nz = 32;
+  c->domain_size[0] = c->domain_size[1] = c->domain_size[2] = 10.0f;
+  c->cell_size = 10.0f / 64.0f;
+  c->max_particles = 0;
+  c->E = 1e4f;
+  c->nu = 0.3f;
+  c->rho0 = 1000.0f;
+}
+
+void nb_mpm_knobs_default(nb_mpm_knobs* k) {
+  if (!k) return;
+  std::memset(k, 0, sizeof(*k));
+  k->restitution = 0.3f;
+  k->wall_friction = 0.0f;
+  k->num_substeps = 20;
+  k->plastic_model = 0;
+  k->q_max = 1e5f;
+  k->M_friction = 0.577f;
+  k->q_cohesion = 0.0f;
+  k->flip_ratio = 0.0f;
+}
+
+int nb_mpm_create(const nb_mpm_config* cfg, void* hip_stream, nb_mpm** out) {
+  if (!cfg || !out) return fail(NB_ERR_ARG, "nb_mpm_create: null argument");
+  *out = nullptr;
+  if (!(cfg->cell_size > 0.f)) return fail(NB_ERR_ARG, "cell_size must be > 0");
+  for (int a = 0; a < 3; a++)
+    if (!(cfg->domain_size[a] >= 0.f)) return fail(NB_ERR_ARG, "domain_size must be >= 0");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(NB_ERR_HIP, "no HIP device available");
+
+  nb_mpm* h = new nb_mpm();
+  h->cfg = *cfg;
+  nb_mpm_knobs_default(&h->knobs);
+  domain_dims(*cfg, h->grid_res, h->cube_res);
+  if (h->cube_res > 1024u) {
+    const uint32_t cr = h->cube_res;
+    delete h;
+    return fail(NB_ERR_RANGE, "grid of %u cells per axis exceeds the 10-bit Morton range", cr);
+  }
+  h->NC = h->cube_res * h->cube_res * h->cube_res;
+  h->tile_dim = std::max(h->cube_res, 8u) / 8u;
+  h->NT = h->tile_dim * h->tile_dim * h->tile_dim;
+  const uint64_t lattice = uint64_t(cfg->nx) * cfg->ny * cfg->nz;
+  const uint64_t cap = cfg->max_particles ? cfg->max_particles : lattice;
+  if (cap > 0xFFFFFFF0ull || lattice > cap) {
+    delete h;
+    return fail(NB_ERR_RANGE, "particle capacity %llu invalid (lattice %llu)", (unsigned long long)cap,
+                (unsigned long long)lattice);
+  }
+  h->cap = uint32_t(cap);
+  h->cap_tiles = std::max<uint32_t>(1u, std::min<uint32_t>(h->NT, std::max<uint32_t>(h->cap, 1u)));
+
+  auto cleanup_fail = [&](int rc) {
+    std::string keep = g_err;
+    nb_mpm_destroy(h);
+    g_err = keep;
+    return rc;
+  };
+
+  if (hip_stream) {
+    h->stream = reinterpret_cast<hipStream_t>(hip_stream);
+  } else {
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
+      return cleanup_fail(fail(NB_ERR_HIP, "hipStreamCreate failed"));
+    h->own_stream = true;
+  }
+
+  DevState& d = h->d;
+  const size_t cells = size_t(h->NT) * kTileCells;
+  int rc = NB_OK;
+  for (int f = 0; f < NB_NUM_FIELDS && rc == NB_OK; f++) rc = h->alloc(&d.field[f], h->cap);
+  if (rc == NB_OK) rc = h->alloc(&d.keys, h->cap);
+  if (rc == NB_OK) rc = h->alloc(&d.rank, h->cap);
+  if (rc == NB_OK) rc = h->alloc(&d.sorted, h->cap);
+  if (rc == NB_OK) rc = h->alloc(&d.cell_count, cells);
+  if (rc == NB_OK) rc = h->alloc(&d.cell_end, cells);
+  if (rc == NB_OK) rc = h->alloc(&d.tile_flag, h->NT);
+  if (rc == NB_OK) rc = h->alloc(&d.tile_mark, h->NT);
+  if (rc == NB_OK) rc = h->alloc(&d.tile_act, h->NT);
+  if (rc == NB_OK) rc = h->alloc(&d.act_tiles, h->NT);
+  if (rc == NB_OK) rc = h->alloc(&d.node_tiles, h->NT);
+  if (rc == NB_OK) rc = h->alloc(&d.all_tiles, h->NT);
+  if (rc == NB_OK) rc = h->alloc(&d.tile_total, h->NT);
+  if (rc == NB_OK) rc = h->alloc(&d.tile_start, h->NT);
+  if (rc == NB_OK) rc = h->alloc(&d.counters, CNT_SLOTS);
+  if (rc == NB_OK) rc = h->alloc(&d.partial, size_t(h->cap_tiles) * kBoxNodes);
+  if (rc == NB_OK) rc = h->alloc(&d.grid_node, cells);
+  if (rc == NB_OK) rc = h->alloc(&d.grid_old, cells);
+  if (rc == NB_OK) rc = h->alloc(&d.materials, NB_MAX_MATERIALS);
+  if (rc == NB_OK) rc = h->alloc(&d.colliders, NB_MAX_COLLIDERS);
+  if (rc == NB_OK) rc = h->alloc(&d.forces, NB_MAX_FORCES);
+  if (rc != NB_OK) return cleanup_fail(rc);
+  d.sdf = nullptr;
+
+  auto zero = [&](void* p, size_t bytes) { return hipMemsetAsync(p, 0, bytes, h->stream); };
+  if (zero(d.cell_count, cells * 4) != hipSuccess || zero(d.cell_end, cells * 4) != hipSuccess ||
+      zero(d.tile_flag, h->NT) != hipSuccess || zero(d.counters, CNT_SLOTS * 4) != hipSuccess ||
+      zero(d.grid_node, cells * 16) != hipSuccess || zero(d.grid_old, cells * 16) != hipSuccess ||
+      zero(d.materials, NB_MAX_MATERIALS * 64) != hipSuccess || zero(d.colliders, NB_MAX_COLLIDERS * 64) != hipSuccess ||
+      zero(d.forces, NB_MAX_FORCES * 32) != hipSuccess)
+    return cleanup_fail(fail(NB_ERR_HIP, "hipMemsetAsync failed"));
+  for (int f = 0; f < NB_NUM_FIELDS; f++)
+    if (zero(d.field[f], size_t(std::max(h->cap, 1u)) * 16) != hipSuccess)
+      return cleanup_fail(fail(NB_ERR_HIP, "hipMemsetAsync failed"));
+  if (hipMemsetAsync(d.tile_act, 0xFF, size_t(h->NT) * 4, h->stream) != hipSuccess)
+    return cleanup_fail(fail(NB_ERR_HIP, "hipMemsetAsync failed"));
+  launch_fill_identity(d.all_tiles, h->NT, h->stream);
+  {
+    uint32_t all = h->NT;
+    if (hipMemcpyAsync(d.counters + CNT_ALL_TILES, &all, 4, hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+        hipStreamSynchronize(h->stream) != hipSuccess)
+      return cleanup_fail(fail(NB_ERR_HIP, "counter upload failed"));
+  }
+
+  // default material (src/engine/MPMEngine.cpp:67-78)
+  {
+    nb_material m = preset_jelly(cfg->E, cfg->nu, cfg->rho0);
+    m.model = NB_MAT_ELASTIC;
+    m.M_friction = h->knobs.M_friction;
+    m.q_cohesion = h->knobs.q_cohesion;
+    m.q_max = h->knobs.q_max;
+    if ((rc = upload(h, d.materials, &m, sizeof(m))) != NB_OK) return cleanup_fail(rc);
+    h->material_count = 1;
+  }
+
+  // default particle block (src/engine/MPMEngine.cpp:86-137)
+  h->n_live = uint32_t(lattice);
+  if (lattice) {
+    const float block_frac = 0.40f;
+    const float min_dom = std::min({cfg->domain_size[0], cfg->domain_size[1], cfg->domain_size[2]});
+    const float sp = min_dom * block_frac / float(std::max({cfg->nx, cfg->ny, cfg->nz}));
+    const float Vp = sp * sp * sp;
+    const float cx = cfg->domain_size[0] * 0.5f, cz = cfg->domain_size[2] * 0.5f, cy = cfg->domain_size[1] * 0.70f;
+    const float hx = sp * float(cfg->nx - 1) / 2.0f, hy = sp * float(cfg->ny - 1) / 2.0f,
+                hz = sp * float(cfg->nz - 1) / 2.0f;
+    std::vector<float> pos(size_t(lattice) * 4);
+    size_t idx = 0;
+    for (uint32_t iz = 0; iz < cfg->nz; iz++)
+      for (uint32_t iy = 0; iy < cfg->ny; iy++)
+        for (uint32_t ix = 0; ix < cfg->nx; ix++, idx++) {
+          pos[4 * idx + 0] = cx - hx + ix * sp;
+          pos[4 * idx + 1] = cy - hy + iy * sp;
+          pos[4 * idx + 2] = cz - hz + iz * sp;
+          pos[4 * idx + 3] = Vp;
+        }
+    if ((rc = upload(h, d.field[NB_FIELD_P], pos.data(), pos.size() * 4)) != NB_OK) return cleanup_fail(rc);
+    launch_init_particles(d, 0, h->n_live, h->stream);
+    if (hipStreamSynchronize(h->stream) != hipSuccess) return cleanup_fail(fail(NB_ERR_HIP, "init failed"));
+  }
+  *out = h;
+  return NB_OK;
+}
+
+int nb_mpm_destroy(nb_mpm* h) {
+  if (!h) return NB_OK;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  for (void* p : h->allocs) (void)hipFree(p);
+  if (h->d.sdf) (void)hipFree(h->d.sdf);
+  if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return NB_OK;
+}
+
+int nb_mpm_step(nb_mpm* h, float dt) {
+  if (!h) return fail(NB_ERR_ARG, "null handle");
+  const int ns = std::max(1, h->knobs.num_substeps);
+  const float sub = dt / float(ns);
+  for (int i = 0; i < ns; i++) {
+    int rc = enqueue_substep(h, sub, nullptr);
+    if (rc != NB_OK) return rc;
+  }
+  return NB_OK;
+}
+
+int nb_mpm_substep(nb_mpm* h, float dt_sub) {
+  if (!h) return fail(NB_ERR_ARG, "null handle");
+  return enqueue_substep(h, dt_sub, nullptr);
+}
+
+int nb_mpm_substeps(nb_mpm* h, float dt_sub, uint32_t n) {
+  if (!h) return fail(NB_ERR_ARG, "null handle");
+  for (uint32_t i = 0; i < n; i++) {
+    int rc = enqueue_substep(h, dt_sub, nullptr);
+    if (rc != NB_OK) return rc;
+  }
+  return NB_OK;
+}
+
+int nb_mpm_sync(nb_mpm* h) {
+  if (!h) return fail(NB_ERR_ARG, "null handle");
+  NB_HIP(hipStreamSynchronize(h->stream));
+  NB_HIP(hipGetLastError());
+  return NB_OK;
+}
+
+int nb_mpm_set_knobs(nb_mpm* h, const nb_mpm_knobs* k) {
+  if (!h || !k) return fail(NB_ERR_ARG, "null argument");
+  h->knobs = *k;
+  return NB_OK;
+}
+int nb_mpm_get_knobs(const nb_mpm* h, nb_mpm_knobs* k) {
+  if (!h || !k) return fail(NB_ERR_ARG, "null argument");
+  *k = h->knobs;
+  return NB_OK;
+}
+int nb_mpm_get_config(const nb_mpm* h, nb_mpm_config* c) {
+  if (!h || !c) return fail(NB_ERR_ARG, "null argument");
+  *c = h->cfg;
+  return NB_OK;
+}
+int nb_mpm_grid_res(const nb_mpm* h, uint32_t res[3]) {
+  if (!h || !res) return fail(NB_ERR_ARG, "null argument");
+  for (int a = 0; a < 3; a++) res[a] = h->grid_res[a];
+  return NB_OK;
+}
+int nb_mpm_total_cells(const nb_mpm* h, uint32_t* nc) {
+  if (!h || !nc) return fail(NB_ERR_ARG, "null argument");
+  *nc = h->NC;
+  return NB_OK;
+}
+int nb_mpm_live_particle_count(const nb_mpm* h, uint32_t* n) {
+  if (!h || !n) return fail(NB_ERR_ARG, "null argument");
+  *n = h->n_live;
+  return NB_OK;
+}
+
+int nb_mpm_set_materials(nb_mpm* h, const nb_material* mats, uint32_t n) {
+  if (!h) return fail(NB_ERR_ARG, "null handle");
+  if (n == 0) return NB_OK;  // src/engine/MPMEngine.cpp:240
+  if (!mats) return fail(NB_ERR_ARG, "null materials");
+  if (n > NB_MAX_MATERIALS) return fail(NB_ERR_RANGE, "%u materials exceed the %u slots", n, NB_MAX_MATERIALS);
+  int rc = upload(h, h->d.materials, mats, sizeof(nb_material) * n);
+  if (rc == NB_OK) h->material_count = n;
+  return rc;
+}
+
+int nb_mpm_set_particle_material_ids(nb_mpm* h, const uint32_t* ids, uint32_t n) {
+  if (!h) return fail(NB_ERR_ARG, "null handle");
+  if (n == 0) return NB_OK;
+  if (!ids) return fail(NB_ERR_ARG, "null ids");
+  if (n > h->cap) return fail(NB_ERR_RANGE, "%u ids exceed capacity %u", n, h->cap);
+  std::vector<float> v(size_t(n) * 4, 0.f);
+  for (uint32_t i = 0; i < n; i++) std::memcpy(&v[4 * size_t(i) + 3], &ids[i], 4);
+  return upload(h, h->d.field[NB_FIELD_V], v.data(), v.size() * 4);
+}
+
+int nb_mpm_set_colliders(nb_mpm* h, const nb_collider* cols, uint32_t n) {
+  if (!h) return fail(NB_ERR_ARG, "null handle");
+  if (n > NB_MAX_COLLIDERS) return fail(NB_ERR_RANGE, "%u colliders exceed the %u slots", n, NB_MAX_COLLIDERS);
+  if (n && !cols) return fail(NB_ERR_ARG, "null colliders");
+  int rc = upload(h, h->d.colliders, cols, sizeof(nb_collider) * n);
+  if (rc == NB_OK) h->collider_count = n;
+  return rc;
+}
+int nb_mpm_clear_analytic_colliders(nb_mpm* h) {
+  if (!h) return fail(NB_ERR_ARG, "null handle");
+  h->collider_count = 0;
+  return NB_OK;
+}
+
+static int ensure_sdf(nb_mpm* h) {
+  if (h->d.sdf) return NB_OK;
+  NB_HIP(hipMalloc(&h->d.sdf, size_t(h->NC) * 4));
+  return NB_OK;
+}
+
+int nb_mpm_set_collider_sdf(nb_mpm* h, const float* sdf, uint64_t n) {
+  if (!h) return fail(NB_ERR_ARG, "null handle");
+  if (!sdf) return fail(NB_ERR_ARG, "null sdf");
+  if (n != h->NC) return fail(NB_ERR_RANGE, "sdf has %llu values, the Morton cube has %u cells", (unsigned long long)n, h->NC);
+  int rc = ensure_sdf(h);
+  if (rc != NB_OK) return rc;
+  rc = upload(h, h->d.sdf, sdf, size_t(n) * 4);
+  if (rc == NB_OK) h->has_sdf = true;
+  return rc;
+}
+
+int nb_mpm_set_collider_sphere(nb_mpm* h, float r, float cx, float cy, float cz) {
+  if (!h) return fail(NB_ERR_ARG, "null handle");
+  int rc = ensure_sdf(h);
+  if (rc != NB_OK) return rc;
+  launch_sdf_sphere(h->d.sdf, h->params(0.f, false), r, cx, cy, cz, h->stream);
+  NB_HIP(hipGetLastError());
+  NB_HIP(hipStreamSynchronize(h->stream));
+  h->has_sdf = true;
+  return NB_OK;
+}
+
+int nb_mpm_clear_collider(nb_mpm* h) {
+  if (!h) return fail(NB_ERR_ARG, "null handle");
+  h->has_sdf = false;
+  return NB_OK;
+}
+
+int nb_mpm_set_forces(nb_mpm* h, const nb_force* forces, uint32_t n) {
+  if (!h) return fail(NB_ERR_ARG, "null handle");
+  if (n > NB_MAX_FORCES) return fail(NB_ERR_RANGE, "%u forces exceed the %u slots", n, NB_MAX_FORCES);
+  if (n && !forces) return fail(NB_ERR_ARG, "null forces");
+  for (uint32_t i = 0; i < n; i++) {
+    uint32_t t = forces[i].type;
+    if (t == NB_FORCE_TURBULENCE || t == NB_FORCE_NOISE)
+      return fail(NB_ERR_UNSUPPORTED, "force type %u (curl/fBm noise) is not implemented", t);
+    if (t > NB_FORCE_Z_CLAMP) return fail(NB_ERR_ARG, "unknown force type %u", t);
+  }
+  int rc = upload(h, h->d.forces, forces, sizeof(nb_force) * n);
+  if (rc == NB_OK) h->force_count = n;
+  return rc;
+}
+
+int nb_mpm_append_particles(nb_mpm* h, const float* pos4, const float* vel4, uint32_t n, uint32_t* appended) {
+  if (!h) return fail(NB_ERR_ARG, "null handle");
+  if (appended) *appended = 0;
+  const uint32_t nn = std::min(n, h->cap - h->n_live);
+  if (nn == 0) return NB_OK;
+  if (!pos4 || !vel4) return fail(NB_ERR_ARG, "null particle arrays");
+  int rc = upload(h, h->d.field[NB_FIELD_P] + h->n_live, pos4, size_t(nn) * 16);
+  if (rc == NB_OK) rc = upload(h, h->d.field[NB_FIELD_V] + h->n_live, vel4, size_t(nn) * 16);
+  if (rc != NB_OK) return rc;
+  launch_init_particles(h->d, h->n_live, nn, h->stream);
+  NB_HIP(hipGetLastError());
+  NB_HIP(hipStreamSynchronize(h->stream));
+  h->n_live += nn;
+  h->bins_valid = false;
+  if (appended) *appended = nn;
+  return NB_OK;
+}
+
+int nb_mpm_reset_particles(nb_mpm* h) {
+  if (!h) return fail(NB_ERR_ARG, "null handle");
+  NB_HIP(hipStreamSynchronize(h->stream));
+  h->n_live = 0;
+  h->bins_valid = false;
+  return NB_OK;
+}
+
+int nb_mpm_get_field(nb_mpm* h, int field, float* out4, uint32_t first, uint32_t count) {
+  if (!h || !out4) return fail(NB_ERR_ARG, "null argument");
+  if (field < 0 || field >= NB_NUM_FIELDS) return fail(NB_ERR_ARG, "bad field %d", field);
+  if (uint64_t(first) + count > h->cap) return fail(NB_ERR_RANGE, "rows beyond capacity");
+  return download(h, out4, h->d.field[field] + first, size_t(count) * 16);
+}
+
+int nb_mpm_set_field(nb_mpm* h, int field, const float* in4, uint32_t first, uint32_t count) {
+  if (!h || !in4) return fail(NB_ERR_ARG, "null argument");
+  if (field < 0 || field >= NB_NUM_FIELDS) return fail(NB_ERR_ARG, "bad field %d", field);
+  if (uint64_t(first) + count > h->cap) return fail(NB_ERR_RANGE, "rows beyond capacity");
+  h->bins_valid = false;
+  return upload(h, h->d.field[field] + first, in4, size_t(count) * 16);
+}
+
+int nb_mpm_field_device_ptr(nb_mpm* h, int field, void** dptr) {
+  if (!h || !dptr) return fail(NB_ERR_ARG, "null argument");
+  if (field < 0 || field >= NB_NUM_FIELDS) return fail(NB_ERR_ARG, "bad field %d", field);
+  *dptr = h->d.field[field];
+  return NB_OK;
+}
+
+int nb_mpm_set_live_particle_count(nb_mpm* h, uint32_t n) {
+  if (!h) return fail(NB_ERR_ARG, "null handle");
+  if (n > h->cap) return fail(NB_ERR_RANGE, "%u exceeds capacity %u", n, h->cap);
+  h->n_live = n;
+  h->bins_valid = false;
+  return NB_OK;
+}
+
+// ---- pass-level access ----
+int nb_mpm_run_bin(nb_mpm* h) {
+  if (!h) return fail(NB_ERR_ARG, "null handle");
+  SubParams p = h->params(0.f, false);
+  if (h->n_live) launch_bin(h->dev(), p, h->NT, h->stream, nullptr);
+  NB_HIP(hipGetLastError());
+  NB_HIP(hipStreamSynchronize(h->stream));
+  h->bins_valid = true;
+  return NB_OK;
+}
+
+int nb_mpm_get_bins(nb_mpm* h, uint32_t* keys, uint32_t* sorted, uint32_t* cell_count, uint32_t* cell_end) {
+  if (!h) return fail(NB_ERR_ARG, "null handle");
+  if (!h->bins_valid) return fail(NB_ERR_STATE, "no bins: call nb_mpm_run_bin first");
+  const uint32_t n = h->n_live;
+  int rc = NB_OK;
+  if (keys && n) rc = download(h, keys, h->d.keys, size_t(n) * 4);
+  if (rc == NB_OK && sorted && n) rc = download(h, sorted, h->d.sorted, size_t(n) * 4);
+  if (rc != NB_OK) return rc;
+  if (cell_count || cell_end) {
+    std::vector<uint32_t> cnt(h->NC), dev_end(h->NC);
+    if (n) {
+      if ((rc = download(h, cnt.data(), h->d.cell_count, size_t(h->NC) * 4)) != NB_OK) return rc;
+      if ((rc = download(h, dev_end.data(), h->d.cell_end, size_t(h->NC) * 4)) != NB_OK) return rc;
+    }
+    if (cell_count) std::memcpy(cell_count, cnt.data(), size_t(h->NC) * 4);
+    if (cell_end) {
+      // dense END offsets: occupied cells carry the device value; an empty cell
+      // holds the running prefix (the reference's cellOffset for count 0)
+      uint32_t acc = 0;
+      for (uint32_t c = 0; c < h->NC; c++) {
+        acc += cnt[c];
+        cell_end[c] = cnt[c] ? dev_end[c] : acc;
+      }
+    }
+  }
+  return NB_OK;
+}
+
+int nb_mpm_run_p2g(nb_mpm* h, float dt_sub) {
+  if (!h) return fail(NB_ERR_ARG, "null handle");
+  if (!h->bins_valid) return fail(NB_ERR_STATE, "no bins: call nb_mpm_run_bin first");
+  SubParams p = h->params(dt_sub, false);
+  if (h->n_live) {
+    launch_p2g(h->dev(), p, h->NT, h->stream, nullptr);
+    launch_grid(h->dev(), p, h->NT, true, false, h->d.node_tiles, h->d.counters + CNT_NODE_TILES, h->stream, nullptr);
+  }
+  NB_HIP(hipGetLastError());
+  NB_HIP(hipStreamSynchronize(h->stream));
+  h->grid_mode = h->n_live ? GridMode::SPARSE : GridMode::NONE;
+  h->old_valid = false;
+  return NB_OK;
+}
+
+int nb_mpm_run_grid_update(nb_mpm* h, float dt_sub) {
+  if (!h) return fail(NB_ERR_ARG, "null handle");
+  if (h->grid_mode == GridMode::NONE) return NB_OK;
+  SubParams p = h->params(dt_sub, true);
+  if (h->grid_mode == GridMode::DENSE)
+    launch_grid(h->dev(), p, h->NT, false, true, h->d.all_tiles, h->d.counters + CNT_ALL_TILES, h->stream, nullptr);
+  else
+    launch_grid(h->dev(), p, h->NT, false, true, h->d.node_tiles, h->d.counters + CNT_NODE_TILES, h->stream, nullptr);
+  NB_HIP(hipGetLastError());
+  NB_HIP(hipStreamSynchronize(h->stream));
+  h->old_valid = true;
+  return NB_OK;
+}
+
+int nb_mpm_run_g2p(nb_mpm* h, float dt_sub) {
+  if (!h) return fail(NB_ERR_ARG, "null handle");
+  SubParams p = h->params(dt_sub, false);
+  if (h->n_live) launch_g2p(h->dev(), p, h->bins_valid, h->stream, nullptr);
+  NB_HIP(hipGetLastError());
+  NB_HIP(hipStreamSynchronize(h->stream));
+  h->bins_valid = false;  // positions moved
+  return NB_OK;
+}
+
+int nb_mpm_get_grid(nb_mpm* h, float* mom4, float* mass) {
+  if (!h) return fail(NB_ERR_ARG, "null handle");
+  const size_t NC = h->NC;
+  if (mom4) std::memset(mom4, 0, NC * 2 * 16);
+  if (mass) std::memset(mass, 0, NC * 4);
+  if (h->grid_mode == GridMode::NONE) return NB_OK;
+  const size_t cells = size_t(h->NT) * kTileCells;
+  std::vector<float> node(cells * 4), old(cells * 4);
+  int rc = download(h, node.data(), h->d.grid_node, cells * 16);
+  if (rc == NB_OK && h->old_valid) rc = download(h, old.data(), h->d.grid_old, cells * 16);
+  if (rc != NB_OK) return rc;
+  std::vector<uint32_t> tiles;
+  if (h->grid_mode == GridMode::DENSE) {
+    tiles.resize(h->NT);
+    for (uint32_t t = 0; t < h->NT; t++) tiles[t] = t;
+  } else {
+    uint32_t cnt[CNT_SLOTS];
+    if ((rc = download(h, cnt, h->d.counters, sizeof(cnt))) != NB_OK) return rc;
+    tiles.resize(cnt[CNT_NODE_TILES]);
+    if (!tiles.empty() && (rc = download(h, tiles.data(), h->d.node_tiles, tiles.size() * 4)) != NB_OK) return rc;
+  }
+  for (uint32_t T : tiles) {
+    for (uint32_t j = 0; j < uint32_t(kTileCells); j++) {
+      const size_t nid = (size_t(T) << kTileBits) | j;
+      if (nid >= NC) continue;
+      const float* g = &node[nid * 4];
+      if (mom4) {
+        mom4[nid * 4 + 0] = g[0];
+        mom4[nid * 4 + 1] = g[1];
+        mom4[nid * 4 + 2] = g[2];
+        if (h->old_valid && g[3] >= 1e-10f) {
+          mom4[(NC + nid) * 4 + 0] = old[nid * 4 + 0];
+          mom4[(NC + nid) * 4 + 1] = old[nid * 4 + 1];
+          mom4[(NC + nid) * 4 + 2] = old[nid * 4 + 2];
+        }
+      }
+      if (mass) mass[nid] = g[3];
+    }
+  }
+  return NB_OK;
+}
+
+int nb_mpm_set_grid(nb_mpm* h, const float* mom4, const float* mass, const float* old4) {
+  if (!h || !mom4 || !mass) return fail(NB_ERR_ARG, "null argument");
+  const size_t cells = size_t(h->NT) * kTileCells;
+  std::vector<float> node(cells * 4, 0.f);
+  for (size_t c = 0; c < h->NC; c++) {
+    node[4 * c + 0] = mom4[4 * c + 0];
+    node[4 * c + 1] = mom4[4 * c + 1];
+    node[4 * c + 2] = mom4[4 * c + 2];
+    node[4 * c + 3] = mass[c];
+  }
+  int rc = upload(h, h->d.grid_node, node.data(), cells * 16);
+  if (rc != NB_OK) return rc;
+  if (old4) {
+    std::fill(node.begin(), node.end(), 0.f);
+    for (size_t c = 0; c < h->NC; c++)
+      for (int k = 0; k < 3; k++) node[4 * c + k] = old4[4 * c + k];
+    if ((rc = upload(h, h->d.grid_old, node.data(), cells * 16)) != NB_OK) return rc;
+  } else {
+    NB_HIP(hipMemsetAsync(h->d.grid_old, 0, cells * 16, h->stream));
+    NB_HIP(hipStreamSynchronize(h->stream));
+  }
+  h->grid_mode = GridMode::DENSE;
+  h->old_valid = true;
+  return NB_OK;
+}
+
+int nb_mpm_profile_substeps(nb_mpm* h, float dt_sub, uint32_t n, float* kernel_ms, uint32_t kernel_cap,
+                            uint32_t* n_kernels, char* names, uint32_t names_cap) {
+  if (!h) return fail(NB_ERR_ARG, "null handle");
+  Prof pf{};
+  for (int i = 0; i < 2 * kNumProfiledKernels; i++) NB_HIP(hipEventCreate(&pf.ev[i]));
+  int rc = NB_OK;
+  for (uint32_t i = 0; i < n && rc == NB_OK; i++) rc = enqueue_substep(h, dt_sub, &pf);
+  (void)hipStreamSynchronize(h->stream);
+  for (int i = 0; i < 2 * kNumProfiledKernels; i++) (void)hipEventDestroy(pf.ev[i]);
+  if (rc != NB_OK) return rc;
+  if (n_kernels) *n_kernels = kNumProfiledKernels;
+  if (kernel_ms)
+    for (uint32_t i = 0; i < kernel_cap && i < uint32_t(kNumProfiledKernels); i++) kernel_ms[i] = float(pf.ms[i]);
+  if (names && names_cap) {
+    std::string s;
+    for (int i = 0; i < kNumProfiledKernels; i++) {
+      if (i) s += ",";
+      s += kKernelNames[i];
+    }
+    std::snprintf(names, names_cap, "%s", s.c_str());
+  }
+  return NB_OK;
+}
+
+int nb_mpm_active_node_count(nb_mpm* h, uint64_t* g_act) {
+  if (!h || !g_act) return fail(NB_ERR_ARG, "null argument");
+  *g_act = 0;
+  if (h->grid_mode == GridMode::NONE) return NB_OK;
+  const size_t NC = h->NC;
+  std::vector<float> mass(NC);
+  int rc = nb_mpm_get_grid(h, nullptr, mass.data());
+  if (rc != NB_OK) return rc;
+  uint64_t c = 0;
+  for (size_t i = 0; i < NC; i++) c += mass[i] >= 1e-10f;
+  *g_act = c;
+  return NB_OK;
+}
+
+int nb_mpm_active_tile_count(nb_mpm* h, uint32_t* tiles) {
+  if (!h || !tiles) return fail(NB_ERR_ARG, "null argument");
+  uint32_t cnt[CNT_SLOTS];
+  int rc = download(h, cnt, h->d.counters, sizeof(cnt));
+  if (rc != NB_OK) return rc;
+  *tiles = cnt[CNT_ACTIVE];
+  return NB_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,890 @@
+// mpm_kernels.hip — the MPM substep as hand-written gfx950 kernels.
+//
+// One substep (the reference's 11 dispatches, src/engine/MPMEngine.cpp:415-473)
+// runs here as:
+//   bin   k_bin_clear -> k_bin_keys -> k_bin_mark -> k_bin_compact ->
+//         k_bin_tile_scan -> k_bin_tile_offsets -> k_bin_scatter -> k_bin_stabilize
+//         (replaces zero_cells, mpm_hash_count, hash_scan_local/global,
+//          hash_add_base, mpm_hash_sort; output is bit-identical to the stable
+//          counting sort with END offsets of tests/helpers/MPMHarness.cpp:132-178,
+//          with no group-count limit)
+//   P2G   k_p2g: one block per occupied 8^3 tile, one thread per cell; the cell's
+//         particles are accumulated in registers into the 27 nodes they reach,
+//         then merged into an LDS node box in 27 barrier-separated phases (no
+//         atomics, fixed order) and written as a per-tile partial
+//         (replaces mpm_zero_grid + mpm_p2g.comp's node-centric gather)
+//   grid  k_grid: one block per node tile; sums the <= 8 partials covering each
+//         node in a fixed order, then velocity, cap, forces, walls, analytic
+//         colliders and the dense-SDF BC in registers
+//         (replaces mpm_grid_update.comp + mpm_nanovdb_bc.comp)
+//   G2P   k_g2p: one thread per particle in bin order; 27-node gather, F update,
+//         register SVD, return map, Hencky stress, guards, advection
+//         (replaces mpm_g2p.comp)
+// Every kernel is deterministic: identical inputs give bit-identical outputs.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "nb_device_math.hpp"
+#include "nb_engine.hpp"
+
+namespace nb {
+
+const char* const kKernelNames[kNumProfiledKernels] = {
+    "bin_clear", "bin_keys", "bin_mark", "bin_compact", "bin_tile_scan",
+    "bin_tile_offsets", "bin_scatter", "bin_stabilize", "p2g", "grid_update", "g2p"};
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void tile_coords(uint32_t T, int& tx, int& ty, int& tz) {
+  tx = int(morton_compact(T));
+  ty = int(morton_compact(T >> 1));
+  tz = int(morton_compact(T >> 2));
+}
+
+// Exclusive scan across a block of BS threads (BS a multiple of 64).
+// s_wave must hold BS/64 + 1 words.  Returns the exclusive prefix; *total is
+// the block sum.  Contains barriers: call from uniform control flow.
+template <int BS>
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_wave, uint32_t& total) {
+  constexpr int NW = BS / 64;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    uint32_t y = __shfl_up(x, d);
+    if (lane >= d) x += y;
+  }
+  if (lane == 63) s_wave[wave] = x;
+  __syncthreads();
+  if (wave == 0) {
+    uint32_t w = lane < NW ? s_wave[lane] : 0u;
+    uint32_t ws = w;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      uint32_t y = __shfl_up(ws, d);
+      if (lane >= d) ws += y;
+    }
+    if (lane < NW) s_wave[lane] = ws - w;
+    if (lane == NW - 1) s_wave[NW] = ws;
+  }
+  __syncthreads();
+  uint32_t excl = x - v + s_wave[wave];
+  total = s_wave[NW];
+  __syncthreads();
+  return excl;
+}
+
+// cell key of a position: shaders/mpm_common.glsl:190-195 (IEEE quotient,
+// clamp to [0, gridRes-1], truncation; NaN maps to 0 as fmaxf returns the
+// non-NaN operand)
+__device__ __forceinline__ uint32_t cell_key(const SubParams& p, float x, float y, float z) {
+  float lx = fminf(fmaxf((x - p.wmin[0]) / p.h, 0.f), float(p.grid_res[0]) - 1.f);
+  float ly = fminf(fmaxf((y - p.wmin[1]) / p.h, 0.f), float(p.grid_res[1]) - 1.f);
+  float lz = fminf(fmaxf((z - p.wmin[2]) / p.h, 0.f), float(p.grid_res[2]) - 1.f);
+  return morton3(uint32_t(lx), uint32_t(ly), uint32_t(lz));
+}
+
+// ---------------------------------------------------------------------------
+// BIN
+// ---------------------------------------------------------------------------
+// Zero the dense cell counters of the tiles the previous bin occupied; outside
+// those tiles the counters are already zero.
+__global__ __launch_bounds__(512) void k_bin_clear(uint32_t* __restrict__ cell_count,
+                                                   const uint32_t* __restrict__ act_tiles,
+                                                   const uint32_t* __restrict__ counters) {
+  const uint32_t na = counters[CNT_ACTIVE];
+  for (uint32_t a = blockIdx.x; a < na; a += gridDim.x) {
+    cell_count[(size_t(act_tiles[a]) << kTileBits) + threadIdx.x] = 0u;
+  }
+}
+
+// Key of every particle, its rank among equal keys (wave-aggregated atomics:
+// one atomic per distinct key per wave), and the occupied-tile flags.
+__global__ __launch_bounds__(256) void k_bin_keys(SubParams p, const float4* __restrict__ P,
+                                                  uint32_t* __restrict__ keys, uint32_t* __restrict__ rank,
+                                                  uint32_t* __restrict__ cell_count, uint8_t* __restrict__ tile_flag) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = i < p.n;
+  uint32_t key = 0xFFFFFFFFu;
+  if (valid) {
+    float4 x = P[i];
+    key = cell_key(p, x.x, x.y, x.z);
+    keys[i] = key;
+  }
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  uint64_t pending = __ballot(valid);
+  uint32_t my_rank = 0;
+  while (pending) {
+    const int leader = __ffsll((unsigned long long)pending) - 1;
+    const uint32_t lkey = __shfl(key, leader);
+    const uint64_t peers = __ballot(valid && key == lkey);
+    uint32_t base = 0;
+    if (int(lane) == leader) {
+      base = atomicAdd(&cell_count[lkey], uint32_t(__popcll(peers)));
+      tile_flag[lkey >> kTileBits] = 1;
+    }
+    base = __shfl(base, leader);
+    if (valid && key == lkey) my_rank = base + uint32_t(__popcll(peers & lt_mask));
+    pending &= ~peers;
+  }
+  if (valid) rank[i] = my_rank;
+}
+
+// Per tile: bit0 = occupied, bit1 = some tile in its 3x3x3 neighbourhood is
+// occupied (its nodes can receive mass).
+__global__ __launch_bounds__(256) void k_bin_mark(const uint8_t* __restrict__ tile_flag,
+                                                  uint8_t* __restrict__ tile_mark, uint32_t tile_dim, uint32_t nt) {
+  const uint32_t T = blockIdx.x * blockDim.x + threadIdx.x;
+  if (T >= nt) return;
+  int tx, ty, tz;
+  tile_coords(T, tx, ty, tz);
+  const int td = int(tile_dim);
+  uint8_t node = 0;
+  for (int dz = -1; dz <= 1; dz++)
+    for (int dy = -1; dy <= 1; dy++)
+      for (int dx = -1; dx <= 1; dx++) {
+        int x = tx + dx, y = ty + dy, z = tz + dz;
+        if (x < 0 || y < 0 || z < 0 || x >= td || y >= td || z >= td) continue;
+        node |= tile_flag[morton3(x, y, z)];
+      }
+  tile_mark[T] = uint8_t((tile_flag[T] ? 1 : 0) | (node ? 2 : 0));
+}
+
+// Single block: ordered compaction of occupied tiles and node tiles; resets the
+// occupied flags for the next bin.
+__global__ __launch_bounds__(1024) void k_bin_compact(const uint8_t* __restrict__ tile_mark,
+                                                      uint8_t* __restrict__ tile_flag, int32_t* __restrict__ tile_act,
+                                                      uint32_t* __restrict__ act_tiles,
+                                                      uint32_t* __restrict__ node_tiles,
+                                                      uint32_t* __restrict__ counters, uint32_t nt) {
+  __shared__ uint32_t s_wave[1024 / 64 + 1];
+  const uint32_t chunk = (nt + 1023u) / 1024u;
+  const uint32_t lo = min(nt, threadIdx.x * chunk), hi = min(nt, lo + chunk);
+  uint32_t na = 0, nn = 0;
+  for (uint32_t T = lo; T < hi; T++) {
+    uint8_t m = tile_mark[T];
+    na += m & 1u;
+    nn += (m >> 1) & 1u;
+  }
+  uint32_t tot_a, tot_n;
+  uint32_t oa = block_exclusive_scan<1024>(na, s_wave, tot_a);
+  uint32_t on = block_exclusive_scan<1024>(nn, s_wave, tot_n);
+  for (uint32_t T = lo; T < hi; T++) {
+    uint8_t m = tile_mark[T];
+    if (m & 1u) {
+      act_tiles[oa] = T;
+      tile_act[T] = int32_t(oa);
+      oa++;
+    } else {
+      tile_act[T] = -1;
+    }
+    if (m & 2u) node_tiles[on++] = T;
+    tile_flag[T] = 0;
+  }
+  if (threadIdx.x == 0) {
+    counters[CNT_ACTIVE] = tot_a;
+    counters[CNT_NODE_TILES] = tot_n;
+  }
+}
+
+// Per occupied tile: exclusive scan of its 512 cell counts (local cell start,
+// kept in cell_end until the scatter) and the tile total.
+__global__ __launch_bounds__(512) void k_bin_tile_scan(const uint32_t* __restrict__ cell_count,
+                                                       uint32_t* __restrict__ cell_end,
+                                                       const uint32_t* __restrict__ act_tiles,
+                                                       uint32_t* __restrict__ tile_total,
+                                                       const uint32_t* __restrict__ counters) {
+  __shared__ uint32_t s_wave[512 / 64 + 1];
+  const uint32_t na = counters[CNT_ACTIVE];
+  for (uint32_t a = blockIdx.x; a < na; a += gridDim.x) {
+    const size_t key = (size_t(act_tiles[a]) << kTileBits) + threadIdx.x;
+    uint32_t c = cell_count[key];
+    uint32_t tot;
+    uint32_t ex = block_exclusive_scan<512>(c, s_wave, tot);
+    cell_end[key] = ex;
+    if (threadIdx.x == 0) tile_total[a] = tot;
+  }
+}
+
+// Single block: exclusive scan of the tile totals in tile order.
+__global__ __launch_bounds__(1024) void k_bin_tile_offsets(const uint32_t* __restrict__ tile_total,
+                                                           uint32_t* __restrict__ tile_start,
+                                                           const uint32_t* __restrict__ counters) {
+  __shared__ uint32_t s_wave[1024 / 64 + 1];
+  const uint32_t na = counters[CNT_ACTIVE];
+  const uint32_t chunk = (na + 1023u) / 1024u;
+  const uint32_t lo = min(na, threadIdx.x * chunk), hi = min(na, lo + chunk);
+  uint32_t s = 0;
+  for (uint32_t a = lo; a < hi; a++) s += tile_total[a];
+  uint32_t tot;
+  uint32_t o = block_exclusive_scan<1024>(s, s_wave, tot);
+  for (uint32_t a = lo; a < hi; a++) {
+    tile_start[a] = o;
+    o += tile_total[a];
+  }
+}
+
+// Place every particle at tile start + cell start + its rank.
+__global__ __launch_bounds__(256) void k_bin_scatter(uint32_t n, const uint32_t* __restrict__ keys,
+                                                     const uint32_t* __restrict__ rank,
+                                                     const uint32_t* __restrict__ cell_end,
+                                                     const int32_t* __restrict__ tile_act,
+                                                     const uint32_t* __restrict__ tile_start,
+                                                     uint32_t* __restrict__ sorted) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t key = keys[i];
+  const int32_t a = tile_act[key >> kTileBits];
+  const uint32_t slot = tile_start[a] + cell_end[key] + rank[i];
+  if (slot < n) sorted[slot] = i;  // always true when the counters started at zero
+}
+
+// Per cell: order its particles by index (the ranks of different waves arrive
+// in atomic order), then store the END offset, the reference's cellOffset.
+__global__ __launch_bounds__(512) void k_bin_stabilize(uint32_t n, const uint32_t* __restrict__ cell_count,
+                                                       uint32_t* __restrict__ cell_end,
+                                                       const uint32_t* __restrict__ act_tiles,
+                                                       const uint32_t* __restrict__ tile_start,
+                                                       uint32_t* __restrict__ sorted,
+                                                       const uint32_t* __restrict__ counters) {
+  const uint32_t na = counters[CNT_ACTIVE];
+  for (uint32_t a = blockIdx.x; a < na; a += gridDim.x) {
+    const size_t key = (size_t(act_tiles[a]) << kTileBits) + threadIdx.x;
+    const uint32_t start = tile_start[a] + cell_end[key];
+    const uint32_t stop = min(start + cell_count[key], n);
+    for (uint32_t k = start + 1; k < stop; k++) {
+      uint32_t v = sorted[k];
+      uint32_t j = k;
+      while (j > start && sorted[j - 1] > v) {
+        sorted[j] = sorted[j - 1];
+        j--;
+      }
+      sorted[j] = v;
+    }
+    cell_end[key] = start + cell_count[key];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// P2G
+// ---------------------------------------------------------------------------
+// Per-axis stencil data of a particle relative to the three nodes c-1, c, c+1.
+struct Axis3 {
+  float w[3], g[3], dx[3];  // weight, weight derivative / h, x_node - x_p
+};
+
+__device__ __forceinline__ void axis_stencil(float xp, float wmin, float h, int first_node, Axis3& a) {
+  const float f = (xp - wmin) / h;
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    const float nf = float(first_node + i) + 0.5f;
+    const float d = f - nf;
+    a.w[i] = bspline2(d);
+    a.g[i] = bspline2g(d) / h;
+    a.dx[i] = nf * h + wmin - xp;
+  }
+}
+
+__global__ __launch_bounds__(512) void k_p2g(SubParams p, const float4* __restrict__ P, const float4* __restrict__ V,
+                                             const float4* __restrict__ F0, const float4* __restrict__ F1,
+                                             const float4* __restrict__ F2, const float4* __restrict__ B0,
+                                             const float4* __restrict__ B1, const float4* __restrict__ B2,
+                                             const nb_material* __restrict__ mats,
+                                             const uint32_t* __restrict__ cell_count,
+                                             const uint32_t* __restrict__ cell_end,
+                                             const uint32_t* __restrict__ sorted,
+                                             const uint32_t* __restrict__ act_tiles,
+                                             const uint32_t* __restrict__ counters, float4* __restrict__ partial) {
+  __shared__ float s_m[kBoxNodes], s_x[kBoxNodes], s_y[kBoxNodes], s_z[kBoxNodes];
+  const uint32_t na = counters[CNT_ACTIVE];
+  const int lx = threadIdx.x & 7, ly = (threadIdx.x >> 3) & 7, lz = threadIdx.x >> 6;
+  const bool apic = p.flip_ratio < -0.5f;
+  const float D_inv = 4.f / (p.h * p.h);
+  const uint32_t mlast = p.material_count - 1u;
+
+  for (uint32_t a = blockIdx.x; a < na; a += gridDim.x) {
+    const uint32_t T = act_tiles[a];
+    int tx, ty, tz;
+    tile_coords(T, tx, ty, tz);
+    const int cx = tx * 8 + lx, cy = ty * 8 + ly, cz = tz * 8 + lz;
+    for (int e = threadIdx.x; e < kBoxNodes; e += 512) {
+      s_m[e] = 0.f; s_x[e] = 0.f; s_y[e] = 0.f; s_z[e] = 0.f;
+    }
+    const uint32_t key = (T << kTileBits) | morton3(lx, ly, lz);
+    const uint32_t cnt = cell_count[key];
+    const uint32_t end = cell_end[key];
+
+    float am[27], ax[27], ay[27], az[27];
+#pragma unroll
+    for (int o = 0; o < 27; o++) { am[o] = 0.f; ax[o] = 0.f; ay[o] = 0.f; az[o] = 0.f; }
+
+    for (uint32_t k = end - cnt; k < end; k++) {
+      const uint32_t q = sorted[k];
+      const float4 pp = P[q];
+      const float4 vv = V[q];
+      uint32_t mid = __float_as_uint(vv.w);
+      mid = mid > mlast ? mlast : mid;
+      const float mp = mats[mid].rho0 * pp.w;
+      Axis3 sx, sy, sz;
+      axis_stencil(pp.x, p.wmin[0], p.h, cx - 1, sx);
+      axis_stencil(pp.y, p.wmin[1], p.h, cy - 1, sy);
+      axis_stencil(pp.z, p.wmin[2], p.h, cz - 1, sz);
+      // affine momentum matrix C = mp * D_inv * B (zero unless APIC)
+      m3 C = m3_zero();
+      if (apic) {
+        const float4 b0 = B0[q], b1 = B1[q], b2 = B2[q];
+        const float s = mp * D_inv;
+        C.m[0][0] = s * b0.x; C.m[0][1] = s * b0.y; C.m[0][2] = s * b0.z;
+        C.m[1][0] = s * b1.x; C.m[1][1] = s * b1.y; C.m[1][2] = s * b1.z;
+        C.m[2][0] = s * b2.x; C.m[2][1] = s * b2.y; C.m[2][2] = s * b2.z;
+      }
+      // stress term dt*Vp*tau, tau symmetric in the w lanes (shaders/mpm_common.glsl:104-113)
+      const float st = p.dt * pp.w;
+      const float t00 = st * F0[q].w, t11 = st * F1[q].w, t22 = st * F2[q].w;
+      const float t01 = st * B0[q].w, t02 = st * B1[q].w, t12 = st * B2[q].w;
+      const float mvx = mp * vv.x, mvy = mp * vv.y, mvz = mp * vv.z;
+#pragma unroll
+      for (int kz = 0; kz < 3; kz++) {
+#pragma unroll
+        for (int jy = 0; jy < 3; jy++) {
+          const float wyz = sy.w[jy] * sz.w[kz];
+          const float gy_wz = sy.g[jy] * sz.w[kz];
+          const float wy_gz = sy.w[jy] * sz.g[kz];
+          // mp*v + C*(0, dy, dz)
+          const float Ax = mvx + C.m[1][0] * sy.dx[jy] + C.m[2][0] * sz.dx[kz];
+          const float Ay = mvy + C.m[1][1] * sy.dx[jy] + C.m[2][1] * sz.dx[kz];
+          const float Az = mvz + C.m[1][2] * sy.dx[jy] + C.m[2][2] * sz.dx[kz];
+          // tau*(0, gy wz, wy gz) (the wx factor applied per node)
+          const float Tx = t01 * gy_wz + t02 * wy_gz;
+          const float Ty = t11 * gy_wz + t12 * wy_gz;
+          const float Tz = t12 * gy_wz + t22 * wy_gz;
+#pragma unroll
+          for (int ix = 0; ix < 3; ix++) {
+            const int o = (kz * 3 + jy) * 3 + ix;
+            float w = sx.w[ix] * wyz;
+            const bool live = w >= 1e-12f;  // shaders/mpm_p2g.comp:61
+            w = live ? w : 0.f;
+            const float wx = live ? sx.w[ix] : 0.f;
+            const float wd = w * sx.dx[ix];
+            const float gw = live ? sx.g[ix] * wyz : 0.f;
+            am[o] += w * mp;
+            ax[o] += w * Ax + wd * C.m[0][0] - gw * t00 - wx * Tx;
+            ay[o] += w * Ay + wd * C.m[0][1] - gw * t01 - wx * Ty;
+            az[o] += w * Az + wd * C.m[0][2] - gw * t02 - wx * Tz;
+          }
+        }
+      }
+    }
+    // merge the 27 per-cell node sums into the box, one offset per phase: in a
+    // phase every thread owns a distinct node, so plain read-add-write is exact
+#pragma unroll
+    for (int kz = 0; kz < 3; kz++)
+#pragma unroll
+      for (int jy = 0; jy < 3; jy++)
+#pragma unroll
+        for (int ix = 0; ix < 3; ix++) {
+          const int o = (kz * 3 + jy) * 3 + ix;
+          const int e = ((lz + kz) * kBox + (ly + jy)) * kBox + (lx + ix);
+          __syncthreads();
+          s_m[e] += am[o];
+          s_x[e] += ax[o];
+          s_y[e] += ay[o];
+          s_z[e] += az[o];
+        }
+    __syncthreads();
+    float4* out = partial + size_t(a) * kBoxNodes;
+    for (int e = threadIdx.x; e < kBoxNodes; e += 512) out[e] = make_float4(s_x[e], s_y[e], s_z[e], s_m[e]);
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// GRID UPDATE
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float sample_sdf(const float* sdf, const SubParams& p, int x, int y, int z) {
+  if (x < 0 || y < 0 || z < 0 || x >= p.grid_res[0] || y >= p.grid_res[1] || z >= p.grid_res[2]) return 1e6f;
+  return sdf[morton3(x, y, z)];
+}
+
+__device__ __forceinline__ float len3(float x, float y, float z) { return sqrtf(x * x + y * y + z * z); }
+
+// applyColliderBC — shaders/collider_common.glsl:62-73
+__device__ __forceinline__ void collider_bc(float& vx, float& vy, float& vz, float nx, float ny, float nz,
+                                            const nb_collider& c) {
+  float rx = vx - c.vx, ry = vy - c.vy, rz = vz - c.vz;
+  float vn = rx * nx + ry * ny + rz * nz;
+  if (vn < 0.f) {
+    float nvx = vn * nx, nvy = vn * ny, nvz = vn * nz;
+    float tx = rx - nvx, ty = ry - nvy, tz = rz - nvz;
+    float tl = len3(tx, ty, tz);
+    if (tl > 1e-8f) {
+      float sc = fmaxf(0.f, 1.f - c.friction * fabsf(vn) / tl);
+      tx *= sc; ty *= sc; tz *= sc;
+    }
+    vx = c.vx + tx - c.restitution * nvx;
+    vy = c.vy + ty - c.restitution * nvy;
+    vz = c.vz + tz - c.restitution * nvz;
+  }
+}
+
+// analytic SDFs — shaders/collider_common.glsl:22-58; returns sdf, writes normal
+__device__ __forceinline__ float collider_sdf(const nb_collider& c, float x, float y, float z, float& nx, float& ny,
+                                              float& nz) {
+  if (c.type == NB_COLL_SPHERE) {
+    float dx = x - c.px, dy = y - c.py, dz = z - c.pz;
+    float dist = len3(dx, dy, dz);
+    if (dist > 1e-8f) { nx = dx / dist; ny = dy / dist; nz = dz / dist; } else { nx = 0.f; ny = 1.f; nz = 0.f; }
+    return dist - c.radius;
+  } else if (c.type == NB_COLL_BOX) {
+    float rx = x - c.px, ry = y - c.py, rz = z - c.pz;
+    float qx = fabsf(rx) - c.nx, qy = fabsf(ry) - c.ny, qz = fabsf(rz) - c.nz;
+    float px = fmaxf(qx, 0.f), py = fmaxf(qy, 0.f), pz = fmaxf(qz, 0.f);
+    float ql = len3(px, py, pz);
+    float sdf = ql + fminf(fmaxf(qx, fmaxf(qy, qz)), 0.f);
+    if (sdf < 0.f) {
+      float ex = c.nx - fabsf(rx), ey = c.ny - fabsf(ry), ez = c.nz - fabsf(rz);
+      nx = ny = nz = 0.f;
+      if (ex < ey && ex < ez) nx = glsl_sign(rx + 1e-8f);
+      else if (ey < ez) ny = glsl_sign(ry + 1e-8f);
+      else nz = glsl_sign(rz + 1e-8f);
+    } else {
+      float sx = glsl_sign(rx + 1e-8f), sy = glsl_sign(ry + 1e-8f), sz = glsl_sign(rz + 1e-8f);
+      float ux, uy, uz;
+      if (ql > 1e-8f) { ux = px * sx; uy = py * sy; uz = pz * sz; } else { ux = sx; uy = sy; uz = sz; }
+      float ul = len3(ux, uy, uz);
+      nx = ux / ul; ny = uy / ul; nz = uz / ul;
+    }
+    return sdf;
+  } else if (c.type == NB_COLL_CAPSULE) {
+    float al = len3(c.nx, c.ny, c.nz);
+    float adx, ady, adz;
+    if (al > 1e-8f) { adx = c.nx / al; ady = c.ny / al; adz = c.nz / al; } else { adx = 0.f; ady = 1.f; adz = 0.f; }
+    float t = fminf(fmaxf((x - c.px) * adx + (y - c.py) * ady + (z - c.pz) * adz, 0.f), al);
+    float dx = x - (c.px + t * adx), dy = y - (c.py + t * ady), dz = z - (c.pz + t * adz);
+    float dist = len3(dx, dy, dz);
+    if (dist > 1e-8f) { nx = dx / dist; ny = dy / dist; nz = dz / dist; } else { nx = 0.f; ny = 1.f; nz = 0.f; }
+    return dist - c.radius;
+  }
+  // plane
+  float l = len3(c.nx, c.ny, c.nz);
+  nx = c.nx / l; ny = c.ny / l; nz = c.nz / l;
+  return (x - c.px) * nx + (y - c.py) * ny + (z - c.pz) * nz;
+}
+
+template <bool FROM_PARTIALS, bool UPDATE>
+__global__ __launch_bounds__(512) void k_grid(SubParams p, const float4* __restrict__ partial,
+                                              const int32_t* __restrict__ tile_act, const uint32_t* __restrict__ tiles,
+                                              const uint32_t* __restrict__ count, float4* __restrict__ grid_node,
+                                              float4* __restrict__ grid_old, const float* __restrict__ sdf,
+                                              const nb_collider* __restrict__ cols, const nb_force* __restrict__ forces) {
+  const uint32_t nb_tiles = *count;
+  const int lx = threadIdx.x & 7, ly = (threadIdx.x >> 3) & 7, lz = threadIdx.x >> 6;
+  const int td = int(p.tile_dim);
+  for (uint32_t b = blockIdx.x; b < nb_tiles; b += gridDim.x) {
+    const uint32_t N = tiles[b];
+    int tx, ty, tz;
+    tile_coords(N, tx, ty, tz);
+    const int nx = tx * 8 + lx, ny = ty * 8 + ly, nz = tz * 8 + lz;
+    const size_t nid = (size_t(N) << kTileBits) | morton3(lx, ly, lz);
+    float mx, my, mz, mass;
+    if (FROM_PARTIALS) {
+      mx = my = mz = mass = 0.f;
+      // the tiles whose 10^3 box contains this node, in a fixed order
+      for (int dz = -1; dz <= 1; dz++) {
+        if ((dz < 0 && lz != 0) || (dz > 0 && lz != 7)) continue;
+        const int sz = tz + dz;
+        if (sz < 0 || sz >= td) continue;
+        for (int dy = -1; dy <= 1; dy++) {
+          if ((dy < 0 && ly != 0) || (dy > 0 && ly != 7)) continue;
+          const int sy = ty + dy;
+          if (sy < 0 || sy >= td) continue;
+          for (int dx = -1; dx <= 1; dx++) {
+            if ((dx < 0 && lx != 0) || (dx > 0 && lx != 7)) continue;
+            const int sx = tx + dx;
+            if (sx < 0 || sx >= td) continue;
+            const int32_t a = tile_act[morton3(sx, sy, sz)];
+            if (a < 0) continue;
+            const int bx = lx - 8 * dx + 1, by = ly - 8 * dy + 1, bz = lz - 8 * dz + 1;
+            const float4 q = partial[size_t(a) * kBoxNodes + (bz * kBox + by) * kBox + bx];
+            mx += q.x; my += q.y; mz += q.z; mass += q.w;
+          }
+        }
+      }
+    } else {
+      const float4 g = grid_node[nid];
+      mx = g.x; my = g.y; mz = g.z; mass = g.w;
+    }
+    if (!UPDATE || mass < 1e-10f) {
+      // massless nodes keep their momentum (shaders/mpm_grid_update.comp:109-110)
+      grid_node[nid] = make_float4(mx, my, mz, mass);
+      continue;
+    }
+    float vx = mx / mass, vy = my / mass, vz = mz / mass;
+    {
+      const float sp = len3(vx, vy, vz);
+      const float vmax = 15.f;
+      if (sp > vmax) {
+        const float k = vmax / sp;
+        vx *= k; vy *= k; vz *= k;
+      }
+    }
+    if (p.write_old) grid_old[nid] = make_float4(vx, vy, vz, 0.f);
+    const float px = (float(nx) + 0.5f) * p.h + p.wmin[0];
+    const float py = (float(ny) + 0.5f) * p.h + p.wmin[1];
+    const float pz = (float(nz) + 0.5f) * p.h + p.wmin[2];
+    // force block generated by src/core/ForceShaderCompiler.cpp:72-93 (typeFlag 0)
+    for (uint32_t fi = 0; fi < p.force_count; fi++) {
+      const nb_force f = forces[fi];
+      const uint32_t mask = f.affect_mask;
+      if (mask != 0u && (mask & 1u) == 0u) continue;
+      if (f.type == NB_FORCE_GRAVITY || f.type == NB_FORCE_CONSTANT_WIND) {
+        const float s = f.data[3];
+        vx += (f.data[0] * s) * p.dt;
+        vy += (f.data[1] * s) * p.dt;
+        vz += (f.data[2] * s) * p.dt;
+      }
+    }
+    // separating walls on the two outer node layers (shaders/mpm_grid_update.comp:54-60)
+    const int Nx = p.grid_res[0] - 1, Ny = p.grid_res[1] - 1, Nz = p.grid_res[2] - 1;
+    if (nx <= 1 && vx < 0.f) vx = 0.f;
+    if (nx >= Nx - 1 && vx > 0.f) vx = 0.f;
+    if (ny <= 1 && vy < 0.f) vy = 0.f;
+    if (ny >= Ny - 1 && vy > 0.f) vy = 0.f;
+    if (nz <= 1 && vz < 0.f) vz = 0.f;
+    if (nz >= Nz - 1 && vz > 0.f) vz = 0.f;
+    for (uint32_t ci = 0; ci < p.collider_count; ci++) {
+      const nb_collider c = cols[ci];
+      float cnx, cny, cnz;
+      const float sd = collider_sdf(c, px, py, pz, cnx, cny, cnz);
+      if (sd <= 0.f) collider_bc(vx, vy, vz, cnx, cny, cnz, c);
+    }
+    // dense-SDF BC (shaders/mpm_nanovdb_bc.comp:23-54)
+    if (sdf) {
+      const float sd = sample_sdf(sdf, p, nx, ny, nz);
+      if (sd < 0.f) {
+        vx = vy = vz = 0.f;
+      } else if (sd < p.h * 1.5f) {
+        const float gx = sample_sdf(sdf, p, nx + 1, ny, nz) - sample_sdf(sdf, p, nx - 1, ny, nz);
+        const float gy = sample_sdf(sdf, p, nx, ny + 1, nz) - sample_sdf(sdf, p, nx, ny - 1, nz);
+        const float gz = sample_sdf(sdf, p, nx, ny, nz + 1) - sample_sdf(sdf, p, nx, ny, nz - 1);
+        const float gl = len3(gx, gy, gz);
+        if (gl > 1e-6f) {
+          const float ux = gx / gl, uy = gy / gl, uz = gz / gl;
+          const float vn = vx * ux + vy * uy + vz * uz;
+          if (vn < 0.f) {
+            vx -= vn * ux; vy -= vn * uy; vz -= vn * uz;
+          }
+        }
+      }
+    }
+    grid_node[nid] = make_float4(vx, vy, vz, mass);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// G2P
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_g2p(SubParams p, float4* __restrict__ P, float4* __restrict__ V,
+                                             float4* __restrict__ F0, float4* __restrict__ F1,
+                                             float4* __restrict__ F2, float4* __restrict__ B0,
+                                             float4* __restrict__ B1, float4* __restrict__ B2,
+                                             const nb_material* __restrict__ mats,
+                                             const uint32_t* __restrict__ sorted,
+                                             const float4* __restrict__ grid_node,
+                                             const float4* __restrict__ grid_old, const float* __restrict__ sdf) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= p.n) return;
+  const uint32_t q = sorted ? sorted[k] : k;
+  const float4 pp = P[q];
+  const float xp = pp.x, yp = pp.y, zp = pp.z, Vp = pp.w;
+  const bool flip = p.flip_ratio > 0.01f;
+
+  // stencil base = floor((x - wmin)/h) - 1; the quotient is clamped to a range
+  // that keeps every integer below in bounds without changing which nodes are valid
+  int base[3];
+  Axis3 ax3[3];
+  const float xs[3] = {xp, yp, zp};
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    float f = (xs[a] - p.wmin[a]) / p.h;
+    f = fminf(fmaxf(f, -4.f), float(p.grid_res[a]) + 4.f);
+    base[a] = int(floorf(f)) - 1;
+    axis_stencil(xs[a], p.wmin[a], p.h, base[a], ax3[a]);
+  }
+
+  float vpx = 0.f, vpy = 0.f, vpz = 0.f, vox = 0.f, voy = 0.f, voz = 0.f;
+  m3 vg = m3_zero(), Bn = m3_zero();
+#pragma unroll
+  for (int iz = 0; iz < 3; iz++) {
+    const int nz = base[2] + iz;
+    if (nz < 0 || nz >= p.grid_res[2]) continue;
+#pragma unroll
+    for (int iy = 0; iy < 3; iy++) {
+      const int ny = base[1] + iy;
+      if (ny < 0 || ny >= p.grid_res[1]) continue;
+#pragma unroll
+      for (int ix = 0; ix < 3; ix++) {
+        const int nx = base[0] + ix;
+        if (nx < 0 || nx >= p.grid_res[0]) continue;
+        const uint32_t nid = morton3(nx, ny, nz);
+        const float4 g = grid_node[nid];
+        if (g.w < 1e-10f) continue;
+        const float wx = ax3[0].w[ix], wy = ax3[1].w[iy], wz = ax3[2].w[iz];
+        const float w = wx * wy * wz;
+        if (w < 1e-12f) continue;
+        const float dwx = ax3[0].g[ix] * wy * wz;
+        const float dwy = wx * ax3[1].g[iy] * wz;
+        const float dwz = wx * wy * ax3[2].g[iz];
+        const float dxx = ax3[0].dx[ix], dxy = ax3[1].dx[iy], dxz = ax3[2].dx[iz];
+        vpx += w * g.x; vpy += w * g.y; vpz += w * g.z;
+        // outerProduct(vi, dw)[j][i] = vi[i]*dw[j]
+        vg.m[0][0] += g.x * dwx; vg.m[0][1] += g.y * dwx; vg.m[0][2] += g.z * dwx;
+        vg.m[1][0] += g.x * dwy; vg.m[1][1] += g.y * dwy; vg.m[1][2] += g.z * dwy;
+        vg.m[2][0] += g.x * dwz; vg.m[2][1] += g.y * dwz; vg.m[2][2] += g.z * dwz;
+        const float wgx = w * g.x, wgy = w * g.y, wgz = w * g.z;
+        Bn.m[0][0] += wgx * dxx; Bn.m[0][1] += wgy * dxx; Bn.m[0][2] += wgz * dxx;
+        Bn.m[1][0] += wgx * dxy; Bn.m[1][1] += wgy * dxy; Bn.m[1][2] += wgz * dxy;
+        Bn.m[2][0] += wgx * dxz; Bn.m[2][1] += wgy * dxz; Bn.m[2][2] += wgz * dxz;
+        if (flip) {
+          const float4 o = grid_old[nid];
+          vox += w * o.x; voy += w * o.y; voz += w * o.z;
+        }
+      }
+    }
+  }
+
+  const float4 vel = V[q];
+  uint32_t mid = __float_as_uint(vel.w);
+  mid = mid > p.material_count - 1u ? p.material_count - 1u : mid;
+  const nb_material mat = mats[mid];
+
+  const float4 f0 = F0[q], f1 = F1[q], f2 = F2[q];
+  m3 F;
+  F.m[0][0] = f0.x; F.m[0][1] = f0.y; F.m[0][2] = f0.z;
+  F.m[1][0] = f1.x; F.m[1][1] = f1.y; F.m[1][2] = f1.z;
+  F.m[2][0] = f2.x; F.m[2][1] = f2.y; F.m[2][2] = f2.z;
+  m3 L;
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int r = 0; r < 3; r++) L.m[c][r] = (c == r ? 1.f : 0.f) + p.dt * vg.m[c][r];
+  m3 Fn = m3_mul(L, F);
+
+  m3 tau;
+  if (mat.model == NB_MAT_FLUID) {
+    float J = fmaxf(m3_det(Fn), 1e-4f);
+    const float cr = powf(J, 1.f / 3.f);
+    Fn = m3_diag(cr, cr, cr);
+    const float pr = mat.bulkK * (J - 1.f);
+    tau = m3_diag(pr, pr, pr);
+  } else if (mat.model == NB_MAT_VON_MISES) {
+    Fn = project_von_mises(Fn, mat.mu, mat.q_max);
+    tau = hencky_stress(Fn, mat.mu, mat.lambda);
+  } else if (mat.model == NB_MAT_DRUCKER_PRAGER || mat.model == NB_MAT_GRANULAR_POWDER) {
+    Fn = project_drucker_prager(Fn, mat.mu, mat.lambda, mat.M_friction, mat.q_cohesion);
+    tau = hencky_stress(Fn, mat.mu, mat.lambda);
+  } else if (mat.model == NB_MAT_VISCOPLASTIC_MUD) {
+    Fn = project_drucker_prager(Fn, mat.mu, mat.lambda, mat.M_friction, mat.q_cohesion);
+    tau = hencky_stress(Fn, mat.mu, mat.lambda);
+    m3 D;
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+      for (int r = 0; r < 3; r++) D.m[c][r] = 0.5f * (vg.m[c][r] + vg.m[r][c]);
+    const float trD = D.m[0][0] + D.m[1][1] + D.m[2][2];
+    const float k2 = 2.f * mat.viscosity;
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+      for (int r = 0; r < 3; r++) tau.m[c][r] += k2 * (D.m[c][r] - (c == r ? trD / 3.f : 0.f));
+  } else {
+    tau = hencky_stress(Fn, mat.mu, mat.lambda);
+  }
+
+  bool jreset = false;
+  {
+    const float Jd = fabsf(m3_det(Fn));
+    if (Jd < 0.15f || Jd > 10.f) {
+      Fn = m3_ident();
+      tau = m3_zero();
+      jreset = true;
+    }
+  }
+  if (p.flip_ratio > -0.5f) Bn = m3_zero();
+
+  float vnx, vny, vnz;
+  if (flip) {
+    const float r = p.flip_ratio;
+    const float fx = vel.x + (vpx - vox), fy = vel.y + (vpy - voy), fz = vel.z + (vpz - voz);
+    vnx = vpx * (1.f - r) + fx * r;
+    vny = vpy * (1.f - r) + fy * r;
+    vnz = vpz * (1.f - r) + fz * r;
+  } else {
+    vnx = vpx; vny = vpy; vnz = vpz;
+  }
+  if (jreset) { vnx *= 0.1f; vny *= 0.1f; vnz *= 0.1f; }
+  float xn = xp + p.dt * vnx, yn = yp + p.dt * vny, zn = zp + p.dt * vnz;
+
+  // particle-level SDF push-out (shaders/mpm_g2p.comp:220-252)
+  if (sdf) {
+    const int gx = p.grid_res[0] - 1, gy = p.grid_res[1] - 1, gz = p.grid_res[2] - 1;
+    float fx = fminf(fmaxf((xn - p.wmin[0]) / p.h, -4.f), float(p.grid_res[0]) + 4.f);
+    float fy = fminf(fmaxf((yn - p.wmin[1]) / p.h, -4.f), float(p.grid_res[1]) + 4.f);
+    float fz = fminf(fmaxf((zn - p.wmin[2]) / p.h, -4.f), float(p.grid_res[2]) + 4.f);
+    const int ix = min(max(int(floorf(fx)), 0), gx);
+    const int iy = min(max(int(floorf(fy)), 0), gy);
+    const int iz = min(max(int(floorf(fz)), 0), gz);
+    const float psdf = sdf[morton3(ix, iy, iz)];
+    if (psdf < 0.f) {
+      const float gnx = sdf[morton3(min(ix + 1, gx), iy, iz)] - sdf[morton3(max(ix - 1, 0), iy, iz)];
+      const float gny = sdf[morton3(ix, min(iy + 1, gy), iz)] - sdf[morton3(ix, max(iy - 1, 0), iz)];
+      const float gnz = sdf[morton3(ix, iy, min(iz + 1, gz))] - sdf[morton3(ix, iy, max(iz - 1, 0))];
+      const float gl = len3(gnx, gny, gnz);
+      if (gl > 1e-6f) {
+        const float ux = gnx / gl, uy = gny / gl, uz = gnz / gl;
+        xn -= psdf * ux; yn -= psdf * uy; zn -= psdf * uz;
+        const float vn = vnx * ux + vny * uy + vnz * uz;
+        if (vn < 0.f) { vnx -= vn * ux; vny -= vn * uy; vnz -= vn * uz; }
+      } else {
+        xn = xp; yn = yp; zn = zp;
+        vnx = vny = vnz = 0.f;
+      }
+      Fn = m3_ident();
+      tau = m3_zero();
+    }
+  }
+  // world clamp at a 1.5h margin (shaders/mpm_g2p.comp:255-263)
+  {
+    const float mg = p.h * 1.5f;
+    const float lox = p.wmin[0] + mg, loy = p.wmin[1] + mg, loz = p.wmin[2] + mg;
+    const float hix = p.wmax[0] - mg, hiy = p.wmax[1] - mg, hiz = p.wmax[2] - mg;
+    if (xn < lox) { xn = lox; vnx = fmaxf(vnx, 0.f); }
+    if (xn > hix) { xn = hix; vnx = fminf(vnx, 0.f); }
+    if (yn < loy) { yn = loy; vny = fmaxf(vny, 0.f); }
+    if (yn > hiy) { yn = hiy; vny = fminf(vny, 0.f); }
+    if (zn < loz) { zn = loz; vnz = fmaxf(vnz, 0.f); }
+    if (zn > hiz) { zn = hiz; vnz = fminf(vnz, 0.f); }
+  }
+  P[q] = make_float4(xn, yn, zn, Vp);
+  V[q] = make_float4(vnx, vny, vnz, vel.w);
+  F0[q] = make_float4(Fn.m[0][0], Fn.m[0][1], Fn.m[0][2], tau.m[0][0]);
+  F1[q] = make_float4(Fn.m[1][0], Fn.m[1][1], Fn.m[1][2], tau.m[1][1]);
+  F2[q] = make_float4(Fn.m[2][0], Fn.m[2][1], Fn.m[2][2], tau.m[2][2]);
+  B0[q] = make_float4(Bn.m[0][0], Bn.m[0][1], Bn.m[0][2], tau.m[0][1]);
+  B1[q] = make_float4(Bn.m[1][0], Bn.m[1][1], Bn.m[1][2], tau.m[0][2]);
+  B2[q] = make_float4(Bn.m[2][0], Bn.m[2][1], Bn.m[2][2], tau.m[1][2]);
+}
+
+// ---------------------------------------------------------------------------
+// utilities
+// ---------------------------------------------------------------------------
+__global__ void k_sdf_sphere(float* __restrict__ sdf, SubParams p, float r, float cx, float cy, float cz) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t nc = p.cube_res * p.cube_res * p.cube_res;
+  if (c >= nc) return;
+  const int x = int(morton_compact(c)), y = int(morton_compact(c >> 1)), z = int(morton_compact(c >> 2));
+  if (x >= p.grid_res[0] || y >= p.grid_res[1] || z >= p.grid_res[2]) {
+    sdf[c] = 0.f;  // cells outside gridRes stay unwritten (zero) as in the reference's host loop
+    return;
+  }
+  // src/engine/MPMEngine.cpp:186-193 (cell centres from the origin)
+  const float wx = (float(x) + 0.5f) * p.h, wy = (float(y) + 0.5f) * p.h, wz = (float(z) + 0.5f) * p.h;
+  const float d = sqrtf((wx - cx) * (wx - cx) + (wy - cy) * (wy - cy) + (wz - cz) * (wz - cz));
+  sdf[c] = d - r;
+}
+
+__global__ void k_fill_identity(uint32_t* __restrict__ a, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = i;
+}
+
+// F = I, B = 0, tau = 0 for rows [first, first+count) (appendParticles)
+__global__ void k_init_FB(float4* F0, float4* F1, float4* F2, float4* B0, float4* B1, float4* B2, uint32_t first,
+                          uint32_t count) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const uint32_t q = first + i;
+  F0[q] = make_float4(1.f, 0.f, 0.f, 0.f);
+  F1[q] = make_float4(0.f, 1.f, 0.f, 0.f);
+  F2[q] = make_float4(0.f, 0.f, 1.f, 0.f);
+  B0[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+  B1[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+  B2[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static inline uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+// blocks for a tile-stride kernel: enough to fill the chip, never more than tiles
+static inline uint32_t tile_grid(uint32_t n_tiles) { return n_tiles < 8192u ? (n_tiles ? n_tiles : 1u) : 8192u; }
+
+struct ProfScope {
+  Prof* pf;
+  int id;
+  hipStream_t s;
+  ProfScope(Prof* pf_, int id_, hipStream_t s_) : pf(pf_), id(id_), s(s_) {
+    if (pf) (void)hipEventRecord(pf->ev[2 * id], s);
+  }
+  ~ProfScope() {
+    if (pf) {
+      (void)hipEventRecord(pf->ev[2 * id + 1], s);
+      pf->record(id);
+    }
+  }
+};
+
+void launch_bin(const DevState& d, const SubParams& p, uint32_t n_tiles, hipStream_t s, Prof* pf) {
+  const uint32_t tg = tile_grid(n_tiles);
+  { ProfScope ps(pf, 0, s); k_bin_clear<<<tg, 512, 0, s>>>(d.cell_count, d.act_tiles, d.counters); }
+  { ProfScope ps(pf, 1, s); k_bin_keys<<<cdiv(p.n, 256), 256, 0, s>>>(p, d.field[NB_FIELD_P], d.keys, d.rank, d.cell_count, d.tile_flag); }
+  { ProfScope ps(pf, 2, s); k_bin_mark<<<cdiv(n_tiles, 256), 256, 0, s>>>(d.tile_flag, d.tile_mark, p.tile_dim, n_tiles); }
+  { ProfScope ps(pf, 3, s); k_bin_compact<<<1, 1024, 0, s>>>(d.tile_mark, d.tile_flag, d.tile_act, d.act_tiles, d.node_tiles, d.counters, n_tiles); }
+  { ProfScope ps(pf, 4, s); k_bin_tile_scan<<<tg, 512, 0, s>>>(d.cell_count, d.cell_end, d.act_tiles, d.tile_total, d.counters); }
+  { ProfScope ps(pf, 5, s); k_bin_tile_offsets<<<1, 1024, 0, s>>>(d.tile_total, d.tile_start, d.counters); }
+  { ProfScope ps(pf, 6, s); k_bin_scatter<<<cdiv(p.n, 256), 256, 0, s>>>(p.n, d.keys, d.rank, d.cell_end, d.tile_act, d.tile_start, d.sorted); }
+  { ProfScope ps(pf, 7, s); k_bin_stabilize<<<tg, 512, 0, s>>>(p.n, d.cell_count, d.cell_end, d.act_tiles, d.tile_start, d.sorted, d.counters); }
+}
+
+void launch_p2g(const DevState& d, const SubParams& p, uint32_t n_tiles, hipStream_t s, Prof* pf) {
+  ProfScope ps(pf, 8, s);
+  k_p2g<<<tile_grid(n_tiles), 512, 0, s>>>(p, d.field[0], d.field[1], d.field[2], d.field[3], d.field[4], d.field[5],
+                                           d.field[6], d.field[7], d.materials, d.cell_count, d.cell_end, d.sorted,
+                                           d.act_tiles, d.counters, d.partial);
+}
+
+void launch_grid(const DevState& d, const SubParams& p, uint32_t n_tiles, bool from_partials, bool update,
+                 const uint32_t* tiles, const uint32_t* count, hipStream_t s, Prof* pf) {
+  ProfScope ps(pf, 9, s);
+  const uint32_t g = tile_grid(n_tiles);
+  if (from_partials && update)
+    k_grid<true, true><<<g, 512, 0, s>>>(p, d.partial, d.tile_act, tiles, count, d.grid_node, d.grid_old, d.sdf, d.colliders, d.forces);
+  else if (from_partials)
+    k_grid<true, false><<<g, 512, 0, s>>>(p, d.partial, d.tile_act, tiles, count, d.grid_node, d.grid_old, d.sdf, d.colliders, d.forces);
+  else
+    k_grid<false, true><<<g, 512, 0, s>>>(p, d.partial, d.tile_act, tiles, count, d.grid_node, d.grid_old, d.sdf, d.colliders, d.forces);
+}
+
+void launch_g2p(const DevState& d, const SubParams& p, bool use_sorted, hipStream_t s, Prof* pf) {
+  ProfScope ps(pf, 10, s);
+  k_g2p<<<cdiv(p.n, 256), 256, 0, s>>>(p, d.field[0], d.field[1], d.field[2], d.field[3], d.field[4], d.field[5],
+                                       d.field[6], d.field[7], d.materials, use_sorted ? d.sorted : nullptr,
+                                       d.grid_node, d.grid_old, d.sdf);
+}
+
+void launch_sdf_sphere(float* sdf, const SubParams& p, float r, float cx, float cy, float cz, hipStream_t s) {
+  const uint32_t nc = p.cube_res * p.cube_res * p.cube_res;
+  k_sdf_sphere<<<cdiv(nc, 256), 256, 0, s>>>(sdf, p, r, cx, cy, cz);
+}
+
+void launch_fill_identity(uint32_t* a, uint32_t n, hipStream_t s) { k_fill_identity<<<cdiv(n, 256), 256, 0, s>>>(a, n); }
+
+void launch_init_particles(const DevState& d, uint32_t first, uint32_t count, hipStream_t s) {
+  if (!count) return;
+  k_init_FB<<<cdiv(count, 256), 256, 0, s>>>(d.field[2], d.field[3], d.field[4], d.field[5], d.field[6], d.field[7],
+                                             first, count);
+}
+
+}  // namespace nb

@@ -1,0 +1,282 @@
+// nb_device_math.hpp — gfx950 device math for the MPM substep.
+//
+// Register-resident 3x3 algebra (column-major, m[col][row] as in GLSL), Morton
+// keys, quadratic B-splines, the Jacobi/F^T F SVD, Hencky stress and the two
+// plasticity return maps.  The algorithms follow the reference's GLSL
+// (shaders/mpm_common.glsl:161-302, shaders/mpm_g2p.comp:19-68) step for step,
+// because the particle state they produce is compared with it; the code is
+// written for wave64 VALU execution: every loop has a compile-time trip count
+// so the compiler keeps the matrices in VGPRs.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace nb {
+
+struct m3 {
+  float m[3][3];  // m[col][row]
+};
+
+__device__ __forceinline__ m3 m3_zero() {
+  m3 r;
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int k = 0; k < 3; k++) r.m[c][k] = 0.f;
+  return r;
+}
+__device__ __forceinline__ m3 m3_ident() {
+  m3 r = m3_zero();
+  r.m[0][0] = r.m[1][1] = r.m[2][2] = 1.f;
+  return r;
+}
+__device__ __forceinline__ m3 m3_diag(float a, float b, float c) {
+  m3 r = m3_zero();
+  r.m[0][0] = a;
+  r.m[1][1] = b;
+  r.m[2][2] = c;
+  return r;
+}
+// (A*B)[col][row] = sum_k A[k][row] * B[col][k]
+__device__ __forceinline__ m3 m3_mul(const m3& A, const m3& B) {
+  m3 r;
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int k = 0; k < 3; k++) r.m[c][k] = A.m[0][k] * B.m[c][0] + A.m[1][k] * B.m[c][1] + A.m[2][k] * B.m[c][2];
+  return r;
+}
+__device__ __forceinline__ m3 m3_transpose(const m3& A) {
+  m3 r;
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int k = 0; k < 3; k++) r.m[c][k] = A.m[k][c];
+  return r;
+}
+__device__ __forceinline__ float m3_det(const m3& m) {
+  return m.m[0][0] * (m.m[1][1] * m.m[2][2] - m.m[2][1] * m.m[1][2]) -
+         m.m[1][0] * (m.m[0][1] * m.m[2][2] - m.m[2][1] * m.m[0][2]) +
+         m.m[2][0] * (m.m[0][1] * m.m[1][2] - m.m[1][1] * m.m[0][2]);
+}
+// U * diag(d) * V^T
+__device__ __forceinline__ m3 m3_udvt(const m3& U, float d0, float d1, float d2, const m3& V) {
+  m3 r;
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int k = 0; k < 3; k++) r.m[c][k] = U.m[0][k] * d0 * V.m[0][c] + U.m[1][k] * d1 * V.m[1][c] + U.m[2][k] * d2 * V.m[2][c];
+  return r;
+}
+
+__device__ __forceinline__ float glsl_sign(float x) { return x > 0.f ? 1.f : (x < 0.f ? -1.f : 0.f); }
+
+// ---- Morton keys (shaders/mpm_common.glsl:161-187) ----
+__host__ __device__ __forceinline__ uint32_t morton_expand(uint32_t v) {
+  v = (v | (v << 16u)) & 0x030000FFu;
+  v = (v | (v << 8u)) & 0x0300F00Fu;
+  v = (v | (v << 4u)) & 0x030C30C3u;
+  v = (v | (v << 2u)) & 0x09249249u;
+  return v;
+}
+__host__ __device__ __forceinline__ uint32_t morton_compact(uint32_t v) {
+  v &= 0x09249249u;
+  v = (v | (v >> 2u)) & 0x030C30C3u;
+  v = (v | (v >> 4u)) & 0x0300F00Fu;
+  v = (v | (v >> 8u)) & 0x030000FFu;
+  v = (v | (v >> 16u)) & 0x000003FFu;
+  return v;
+}
+__host__ __device__ __forceinline__ uint32_t morton3(uint32_t x, uint32_t y, uint32_t z) {
+  return morton_expand(x) | (morton_expand(y) << 1u) | (morton_expand(z) << 2u);
+}
+
+// ---- quadratic B-spline (shaders/mpm_common.glsl:198-210) ----
+__device__ __forceinline__ float bspline2(float d) {
+  float ad = fabsf(d);
+  if (ad < 0.5f) return 0.75f - ad * ad;
+  if (ad < 1.5f) {
+    float t = 1.5f - ad;
+    return 0.5f * t * t;
+  }
+  return 0.f;
+}
+__device__ __forceinline__ float bspline2g(float d) {
+  float ad = fabsf(d);
+  if (ad < 0.5f) return -2.f * d;
+  if (ad < 1.5f) return -glsl_sign(d) * (1.5f - ad);
+  return 0.f;
+}
+
+// ---- Jacobi eigen-decomposition of symmetric A (shaders/mpm_common.glsl:213-247) ----
+// Same pivot rule, rotation and stopping test as the GLSL.  The matrix is held
+// as its six independent entries and the three pivot cases are resolved with
+// selects, so every lane runs one instruction stream and nothing is indexed
+// dynamically (no scratch).  V keeps the GLSL indexing V[col][row]: the GLSL
+// rotates V[r][p0], V[r][p1], i.e. here V.m[r][p0], V.m[r][p1].
+__device__ __forceinline__ void jacobi_eigen3(float a00, float a11, float a22, float a01, float a02, float a12,
+                                              float D[3], m3& V) {
+  V = m3_ident();
+  for (int iter = 0; iter < 20; iter++) {
+    float b01 = fabsf(a01), b02 = fabsf(a02), b12 = fabsf(a12);
+    if (fmaxf(b01, fmaxf(b02, b12)) < 1e-8f) break;
+    // pivot case: 0 -> (0,1), 1 -> (0,2), 2 -> (1,2)
+    int cs = (b01 >= fmaxf(b02, b12)) ? 0 : ((b02 >= b12) ? 1 : 2);
+    float app = (cs == 2) ? a11 : a00;
+    float aqq = (cs == 0) ? a11 : a22;
+    float apq = (cs == 0) ? a01 : ((cs == 1) ? a02 : a12);
+    float arp = (cs == 0) ? a02 : a01;  // A[r][p0]
+    float arq = (cs == 2) ? a02 : a12;  // A[r][p1]
+    float tau = (aqq - app) / (2.f * apq);
+    float t = glsl_sign(tau) / (fabsf(tau) + sqrtf(1.f + tau * tau));
+    float c = 1.f / sqrtf(1.f + t * t);
+    float s = t * c;
+    float napp = app - t * apq, naqq = aqq + t * apq;
+    float nrp = c * arp - s * arq;
+    float nrq = s * arp + c * arq;
+    a00 = (cs == 2) ? a00 : napp;
+    a11 = (cs == 0) ? naqq : ((cs == 2) ? napp : a11);
+    a22 = (cs == 0) ? a22 : naqq;
+    a01 = (cs == 0) ? 0.f : nrp;
+    a02 = (cs == 0) ? nrp : ((cs == 1) ? 0.f : nrq);
+    a12 = (cs == 2) ? 0.f : nrq;
+#pragma unroll
+    for (int rr = 0; rr < 3; rr++) {
+      float vp = (cs == 2) ? V.m[rr][1] : V.m[rr][0];
+      float vq = (cs == 0) ? V.m[rr][1] : V.m[rr][2];
+      float nvp = c * vp - s * vq;
+      float nvq = s * vp + c * vq;
+      V.m[rr][0] = (cs == 2) ? V.m[rr][0] : nvp;
+      V.m[rr][1] = (cs == 0) ? nvq : ((cs == 2) ? nvp : V.m[rr][1]);
+      V.m[rr][2] = (cs == 0) ? V.m[rr][2] : nvq;
+    }
+  }
+  D[0] = a00;
+  D[1] = a11;
+  D[2] = a22;
+}
+
+// ---- SVD via F^T F (shaders/mpm_common.glsl:250-281) ----
+__device__ __forceinline__ void svd3(const m3& F, m3& U, float sig[3], m3& V) {
+  // FtF = transpose(F) * F: FtF[c][k] = dot(F col k, F col c)
+  float f00 = F.m[0][0] * F.m[0][0] + F.m[0][1] * F.m[0][1] + F.m[0][2] * F.m[0][2];
+  float f11 = F.m[1][0] * F.m[1][0] + F.m[1][1] * F.m[1][1] + F.m[1][2] * F.m[1][2];
+  float f22 = F.m[2][0] * F.m[2][0] + F.m[2][1] * F.m[2][1] + F.m[2][2] * F.m[2][2];
+  float f01 = F.m[0][0] * F.m[1][0] + F.m[0][1] * F.m[1][1] + F.m[0][2] * F.m[1][2];
+  float f02 = F.m[0][0] * F.m[2][0] + F.m[0][1] * F.m[2][1] + F.m[0][2] * F.m[2][2];
+  float f12 = F.m[1][0] * F.m[2][0] + F.m[1][1] * F.m[2][1] + F.m[1][2] * F.m[2][2];
+  float D[3];
+  jacobi_eigen3(f00, f11, f22, f01, f02, f12, D, V);
+  sig[0] = sqrtf(fmaxf(D[0], 0.f));
+  sig[1] = sqrtf(fmaxf(D[1], 0.f));
+  sig[2] = sqrtf(fmaxf(D[2], 0.f));
+  // selection sort, descending; swap V columns i<->j (GLSL V[0][i] = column 0
+  // row i: it swaps ROWS of our m[col][row] storage at index i, j)
+#define NB_SWAP_SV(i, j)                                                        \
+  if (sig[j] > sig[i]) {                                                        \
+    float ts = sig[i]; sig[i] = sig[j]; sig[j] = ts;                            \
+    _Pragma("unroll") for (int cc = 0; cc < 3; cc++) {                          \
+      float tv = V.m[cc][i]; V.m[cc][i] = V.m[cc][j]; V.m[cc][j] = tv;          \
+    }                                                                           \
+  }
+  NB_SWAP_SV(0, 1)
+  NB_SWAP_SV(0, 2)
+  NB_SWAP_SV(1, 2)
+#undef NB_SWAP_SV
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    // GLSL vi = vec3(V[0][i], V[1][i], V[2][i]); U[r][i] = (F*vi)[r]/sigma_i,
+    // or the unit vector e_i when sigma_i <= 1e-8 (U starts as zero there)
+    const float v0 = V.m[0][i], v1 = V.m[1][i], v2 = V.m[2][i];
+    const float fv0 = F.m[0][0] * v0 + F.m[1][0] * v1 + F.m[2][0] * v2;
+    const float fv1 = F.m[0][1] * v0 + F.m[1][1] * v1 + F.m[2][1] * v2;
+    const float fv2 = F.m[0][2] * v0 + F.m[1][2] * v1 + F.m[2][2] * v2;
+    const bool big = sig[i] > 1e-8f;
+    const float inv_ok = big ? sig[i] : 1.f;
+    U.m[0][i] = big ? fv0 / inv_ok : (i == 0 ? 1.f : 0.f);
+    U.m[1][i] = big ? fv1 / inv_ok : (i == 1 ? 1.f : 0.f);
+    U.m[2][i] = big ? fv2 / inv_ok : (i == 2 ? 1.f : 0.f);
+  }
+  if (m3_det(U) < 0.f) {
+    U.m[0][2] = -U.m[0][2];
+    U.m[1][2] = -U.m[1][2];
+    U.m[2][2] = -U.m[2][2];
+    sig[2] = -sig[2];
+  }
+}
+
+
+// ---- Hencky Kirchhoff stress (shaders/mpm_common.glsl:284-293) ----
+// tau = U diag(lambda tr(eps) + 2 mu eps) U^T with U as the GLSL stores it.
+__device__ __forceinline__ m3 hencky_from_svd(const m3& U, const float sig[3], float mu, float lam) {
+  float s0 = fmaxf(fabsf(sig[0]), 1e-6f), s1 = fmaxf(fabsf(sig[1]), 1e-6f), s2 = fmaxf(fabsf(sig[2]), 1e-6f);
+  float e0 = logf(s0), e1 = logf(s1), e2 = logf(s2);
+  float tr = e0 + e1 + e2;
+  float lt = lam * tr, tm = 2.f * mu;
+  return m3_udvt(U, lt + tm * e0, lt + tm * e1, lt + tm * e2, U);
+}
+__device__ __forceinline__ m3 hencky_stress(const m3& F, float mu, float lam) {
+  m3 U, V;
+  float sig[3];
+  svd3(F, U, sig, V);
+  return hencky_from_svd(U, sig, mu, lam);
+}
+
+// ---- Von Mises radial return (shaders/mpm_g2p.comp:19-38) ----
+__device__ __forceinline__ m3 project_von_mises(const m3& F, float mu, float q_max) {
+  m3 U, V;
+  float sig[3];
+  svd3(F, U, sig, V);
+  float e0 = logf(fmaxf(fabsf(sig[0]), 1e-6f));
+  float e1 = logf(fmaxf(fabsf(sig[1]), 1e-6f));
+  float e2 = logf(fmaxf(fabsf(sig[2]), 1e-6f));
+  float tr = e0 + e1 + e2;
+  float m = tr / 3.f;
+  float d0 = e0 - m, d1 = e1 - m, d2 = e2 - m;
+  float nd = sqrtf(d0 * d0 + d1 * d1 + d2 * d2);
+  float delta = nd - q_max / (2.f * mu);
+  if (delta > 0.f && nd > 1e-8f) {
+    float k = delta / nd;
+    d0 -= k * d0;
+    d1 -= k * d1;
+    d2 -= k * d2;
+    return m3_udvt(U, expf(d0 + m), expf(d1 + m), expf(d2 + m), V);
+  }
+  return F;
+}
+
+// ---- Drucker-Prager return map (shaders/mpm_g2p.comp:41-68): always recomposes ----
+__device__ __forceinline__ m3 project_drucker_prager(const m3& F, float mu, float lam, float M, float q_c) {
+  m3 U, V;
+  float sig[3];
+  svd3(F, U, sig, V);
+  float e0 = logf(fmaxf(fabsf(sig[0]), 1e-6f));
+  float e1 = logf(fmaxf(fabsf(sig[1]), 1e-6f));
+  float e2 = logf(fmaxf(fabsf(sig[2]), 1e-6f));
+  float tr = e0 + e1 + e2;
+  float m = tr / 3.f;
+  float d0 = e0 - m, d1 = e1 - m, d2 = e2 - m;
+  float nd = sqrtf(d0 * d0 + d1 * d1 + d2 * d2);
+  float K = lam + 2.f * mu / 3.f;
+  float p = -K * tr;
+  float q_yield = M * p + q_c;
+  if (q_yield <= 1e-8f) {
+    float p_proj = -q_c / fmaxf(M, 1e-8f);
+    float tr2 = -p_proj / K;
+    float m2 = tr2 / 3.f;
+    e0 = m2; e1 = m2; e2 = m2;
+  } else {
+    float delta = nd - q_yield / (2.f * mu);
+    if (delta > 0.f && nd > 1e-8f) {
+      float k = delta / nd;
+      d0 -= k * d0;
+      d1 -= k * d1;
+      d2 -= k * d2;
+      e0 = d0 + m; e1 = d1 + m; e2 = d2 + m;
+    }
+  }
+  return m3_udvt(U, expf(e0), expf(e1), expf(e2), V);
+}
+
+}  // namespace nb

@@ -1,0 +1,108 @@
+// nb_engine.hpp — internal state of the MI355X MPM engine and the launcher
+// interface between the host translation unit (mpm_engine.hip) and the kernel
+// translation unit (mpm_kernels.hip).
+//
+// HBM layout (one GPU):
+//   particles  8 x float4 x capacity, the reference's SoA (P, v, F0-2, B0-2)
+//   bins       keys[N], rank[N], sorted[N]; cell_count/cell_end dense over the
+//              Morton cube (only occupied 8^3 tiles are ever touched)
+//   tiles      the Morton cube cut into 8^3-cell tiles (tile id = key >> 9):
+//              per-tile flags, active-tile list, dilated node-tile list
+//   partials   per active tile, the 10^3 node box its particles reach
+//              (float4 = momentum xyz, mass), written without atomics
+//   grid       grid_node[NC] float4 = (velocity or momentum, mass),
+//              grid_old[NC] float4 = pre-force velocity (FLIP)
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/nebula_mpm.h"
+
+namespace nb {
+
+constexpr int kTileBits = 9;          // 8^3 cells per tile
+constexpr int kTileCells = 512;
+constexpr int kBox = 10;              // node box edge of a tile (8 + 2 halo)
+constexpr int kBoxNodes = 1000;
+
+// counters[] slots (device)
+enum : int { CNT_ACTIVE = 0, CNT_NODE_TILES = 1, CNT_ALL_TILES = 2, CNT_SLOTS = 4 };
+
+// Substep constants, passed by value to every kernel (the role MPMSimPC plays
+// for the reference's shaders, src/core/MPMSimPC.h:14-73).
+struct SubParams {
+  float dt;
+  float h;            // cell size
+  float wmin[3];
+  float wmax[3];
+  int32_t grid_res[3];
+  uint32_t cube_res;
+  uint32_t tile_dim;  // tiles per axis = max(cube_res, 8) / 8
+  uint32_t n;         // live particles
+  float flip_ratio;
+  uint32_t material_count;
+  uint32_t collider_count;
+  uint32_t force_count;
+  uint32_t write_old;  // write the pre-force velocity (FLIP, or pass-level export)
+};
+
+struct DevState {
+  float4* field[NB_NUM_FIELDS];
+  uint32_t* keys;
+  uint32_t* rank;
+  uint32_t* sorted;
+  uint32_t* cell_count;   // NT*512
+  uint32_t* cell_end;     // NT*512
+  uint8_t* tile_flag;     // NT: tile holds a particle (set by the key pass)
+  uint8_t* tile_mark;     // NT: bit0 active, bit1 node tile (dilation)
+  int32_t* tile_act;      // NT: index in act_tiles or -1
+  uint32_t* act_tiles;    // NT
+  uint32_t* node_tiles;   // NT
+  uint32_t* all_tiles;    // NT: 0..NT-1 (dense pass-level mode)
+  uint32_t* tile_total;   // NT
+  uint32_t* tile_start;   // NT
+  uint32_t* counters;     // CNT_SLOTS
+  float4* partial;        // cap_tiles * 1000
+  float4* grid_node;      // NT*512
+  float4* grid_old;       // NT*512
+  float* sdf;             // NC (null when no SDF collider)
+  nb_material* materials;
+  nb_collider* colliders;
+  nb_force* forces;
+};
+
+// names of the kernels, in the order the profiler indexes them
+constexpr int kNumProfiledKernels = 11;
+extern const char* const kKernelNames[kNumProfiledKernels];
+
+// Optional per-kernel timing: HIP events recorded around each launch on the
+// launch stream; record() waits for the end event and accumulates.
+struct Prof {
+  hipEvent_t ev[2 * kNumProfiledKernels];
+  double ms[kNumProfiledKernels];
+  uint32_t calls[kNumProfiledKernels];
+  void record(int id) {
+    (void)hipEventSynchronize(ev[2 * id + 1]);
+    float t = 0.f;
+    (void)hipEventElapsedTime(&t, ev[2 * id], ev[2 * id + 1]);
+    ms[id] += t;
+    calls[id]++;
+  }
+};
+
+// ---- launchers (mpm_kernels.hip); pf may be null ----
+void launch_bin(const DevState& d, const SubParams& p, uint32_t n_tiles, hipStream_t s, Prof* pf);
+void launch_p2g(const DevState& d, const SubParams& p, uint32_t n_tiles, hipStream_t s, Prof* pf);
+// Sum partials into grid_node and apply the grid update (+ dense-SDF BC).
+// update=false only sums (pass-level P2G export).  tiles/count select the
+// node-tile list (the dilated list, or all tiles in dense mode).
+void launch_grid(const DevState& d, const SubParams& p, uint32_t n_tiles, bool from_partials, bool update,
+                 const uint32_t* tiles, const uint32_t* count, hipStream_t s, Prof* pf);
+// use_sorted: visit particles in bin order (else index order)
+void launch_g2p(const DevState& d, const SubParams& p, bool use_sorted, hipStream_t s, Prof* pf);
+void launch_sdf_sphere(float* sdf, const SubParams& p, float r, float cx, float cy, float cz, hipStream_t s);
+void launch_fill_identity(uint32_t* a, uint32_t n, hipStream_t s);
+void launch_init_particles(const DevState& d, uint32_t first, uint32_t count, hipStream_t s);
+
+}  // namespace nb

@@ -1,0 +1,694 @@
+"""Python mirror of the reference's MPMEngine surface over the C ABI.
+
+``MPMEngine`` keeps the reference's method names and argument meaning
+(src/engine/MPMEngine.h:59-186, src/engine/EngineBase.h:23-31) so that code
+written against the reference reads the same; every call goes through
+``libnebula_mpm.so`` (include/nebula_mpm.h).  There is no CPU fallback: if the
+HIP library is missing or no GPU is present the calls raise.
+
+Also here: the POD records (MaterialParams, ColliderPrimitive, ForceGPU) with
+the reference's byte layouts, the material presets of
+src/core/MaterialParams.h:42-103, ColliderSet (src/core/Collider.h:31-120),
+the force classes of src/core/Force.h and the domain helpers of
+src/core/Domain.h.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+from typing import Iterable, List, Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libnebula_mpm.so")
+
+NB_OK = 0
+FIELD_P, FIELD_V, FIELD_F0, FIELD_F1, FIELD_F2, FIELD_B0, FIELD_B1, FIELD_B2 = range(8)
+FIELD_NAMES = ("P", "v", "F0", "F1", "F2", "B0", "B1", "B2")
+MAX_MATERIALS, MAX_COLLIDERS, MAX_FORCES = 16, 64, 32
+
+
+class MaterialModel:
+    """src/core/MaterialParams.h:5-12"""
+
+    ELASTIC = 0
+    VON_MISES = 1
+    DRUCKER_PRAGER = 2
+    GRANULAR_POWDER = 3
+    FLUID = 4
+    VISCOPLASTIC_MUD = 5
+
+
+class ColliderType:
+    """src/core/Collider.h:8-13"""
+
+    PLANE = 0
+    SPHERE = 1
+    BOX = 2
+    CAPSULE = 3
+
+
+class ForceType:
+    """src/core/Force.h:13-19"""
+
+    GRAVITY = 0
+    CONSTANT_WIND = 1
+    TURBULENCE = 2
+    NOISE = 3
+    Z_CLAMP = 4
+
+
+class MaterialParams(C.Structure):
+    """64-byte material record (src/core/MaterialParams.h:16-38)."""
+
+    _fields_ = [
+        ("mu", C.c_float), ("lambda_", C.c_float), ("rho0", C.c_float), ("model", C.c_uint32),
+        ("M_friction", C.c_float), ("q_cohesion", C.c_float), ("q_max", C.c_float), ("bulkK", C.c_float),
+        ("fluidGamma", C.c_float), ("viscosity", C.c_float), ("hardening", C.c_float), ("xi", C.c_float),
+        ("pad", C.c_float * 4),
+    ]
+
+
+class ColliderPrimitive(C.Structure):
+    """64-byte analytic collider record (src/core/Collider.h:19-29)."""
+
+    _fields_ = [
+        ("type", C.c_uint32), ("px", C.c_float), ("py", C.c_float), ("pz", C.c_float),
+        ("nx", C.c_float), ("ny", C.c_float), ("nz", C.c_float), ("radius", C.c_float),
+        ("vx", C.c_float), ("vy", C.c_float), ("vz", C.c_float), ("restitution", C.c_float),
+        ("friction", C.c_float), ("pad", C.c_float * 3),
+    ]
+
+
+class ForceGPU(C.Structure):
+    """32-byte force record (src/core/Force.h:27-32)."""
+
+    _fields_ = [("type", C.c_uint16), ("affectMask", C.c_uint16), ("data", C.c_float * 7)]
+
+
+class MPMConfig(C.Structure):
+    """MPMConfig (src/engine/MPMEngine.h:22-55); defaults from its member initialisers."""
+
+    _fields_ = [
+        ("nx", C.c_uint32), ("ny", C.c_uint32), ("nz", C.c_uint32),
+        ("domainSize", C.c_float * 3), ("cellSize", C.c_float), ("maxParticles", C.c_uint32),
+        ("E", C.c_float), ("nu", C.c_float), ("rho0", C.c_float),
+    ]
+
+    def __init__(self, **kw):
+        super().__init__()
+        self.nx = self.ny = self.nz = 32
+        self.domainSize[:] = (10.0, 10.0, 10.0)
+        self.cellSize = 10.0 / 64.0
+        self.maxParticles = 0
+        self.E, self.nu, self.rho0 = 1e4, 0.3, 1000.0
+        for k, v in kw.items():
+            if k == "domainSize":
+                self.domainSize[:] = tuple(float(x) for x in v)
+            else:
+                setattr(self, k, v)
+
+    def particleCount(self):
+        return self.nx * self.ny * self.nz
+
+    def maxParticleCount(self):
+        return self.maxParticles if self.maxParticles > 0 else self.particleCount()
+
+    def gridRes(self):
+        return gridRes(tuple(self.domainSize), self.cellSize)
+
+    def totalCells(self):
+        return hashCells(self.gridRes())
+
+    def nGroups(self):
+        return nGroups(self.totalCells())
+
+    def mu(self):
+        return calcMu(self.E, self.nu)
+
+    def lame(self):
+        return calcLambda(self.E, self.nu)
+
+
+class _Knobs(C.Structure):
+    _fields_ = [
+        ("restitution", C.c_float), ("wall_friction", C.c_float), ("num_substeps", C.c_int32),
+        ("plastic_model", C.c_uint32), ("q_max", C.c_float), ("M_friction", C.c_float),
+        ("q_cohesion", C.c_float), ("flip_ratio", C.c_float),
+    ]
+
+
+# ---------------------------------------------------------------------------
+# domain helpers — src/core/Domain.h:14-39
+# ---------------------------------------------------------------------------
+def gridRes(domainSize: Sequence[float], cellSize: float):
+    h = np.float32(max(np.float32(cellSize), np.float32(1e-6)))
+    out = []
+    for d in domainSize:
+        r = math.ceil(float(np.float32(d) / h))
+        out.append(max(int(r), 1))
+    return tuple(out)
+
+
+def mortonCubeRes(res: Sequence[int]) -> int:
+    m = max(max(res), 1)
+    p = 1
+    while p < m:
+        p <<= 1
+    return p
+
+
+def hashCells(res: Sequence[int]) -> int:
+    c = mortonCubeRes(res)
+    return c * c * c
+
+
+def nGroups(cells: int) -> int:
+    return (cells + 255) // 256
+
+
+def mortonEncode(x: int, y: int, z: int) -> int:
+    """Morton cell id (shaders/mpm_common.glsl:161-181)."""
+
+    def expand(v):
+        v &= 0x3FF
+        v = (v | (v << 16)) & 0x030000FF
+        v = (v | (v << 8)) & 0x0300F00F
+        v = (v | (v << 4)) & 0x030C30C3
+        v = (v | (v << 2)) & 0x09249249
+        return v
+
+    return expand(x) | (expand(y) << 1) | (expand(z) << 2)
+
+
+def mortonEncodeArray(x, y, z):
+    def expand(v):
+        v = np.asarray(v, np.uint32) & np.uint32(0x3FF)
+        v = (v | (v << np.uint32(16))) & np.uint32(0x030000FF)
+        v = (v | (v << np.uint32(8))) & np.uint32(0x0300F00F)
+        v = (v | (v << np.uint32(4))) & np.uint32(0x030C30C3)
+        v = (v | (v << np.uint32(2))) & np.uint32(0x09249249)
+        return v
+
+    return expand(x) | (expand(y) << np.uint32(1)) | (expand(z) << np.uint32(2))
+
+
+# ---------------------------------------------------------------------------
+# materials — src/core/MaterialParams.h:40-103
+# ---------------------------------------------------------------------------
+def calcMu(E, nu):
+    E, nu = np.float32(E), np.float32(nu)
+    return float(E / (np.float32(2.0) * (np.float32(1.0) + nu)))
+
+
+def calcLambda(E, nu):
+    E, nu = np.float32(E), np.float32(nu)
+    return float(E * nu / ((np.float32(1.0) + nu) * (np.float32(1.0) - np.float32(2.0) * nu)))
+
+
+def _mat(**kw) -> MaterialParams:
+    m = MaterialParams()
+    for k, v in kw.items():
+        setattr(m, "lambda_" if k == "lambda" else k, v)
+    return m
+
+
+def presetJelly(E=1e4, nu=0.4, rho0=1000.0):
+    return _mat(mu=calcMu(E, nu), lambda_=calcLambda(E, nu), rho0=rho0, model=MaterialModel.ELASTIC)
+
+
+def presetSand(E=1e5, nu=0.3, rho0=1600.0):
+    return _mat(mu=calcMu(E, nu), lambda_=calcLambda(E, nu), rho0=rho0, model=MaterialModel.DRUCKER_PRAGER,
+                M_friction=0.577, q_cohesion=0.0)
+
+
+def presetWater(rho0=1000.0, bulkK=5e3):
+    return _mat(mu=0.0, lambda_=0.0, rho0=rho0, model=MaterialModel.FLUID, bulkK=bulkK, fluidGamma=7.0,
+                viscosity=0.01)
+
+
+def presetMud(E=8e4, nu=0.35, rho0=1800.0):
+    return _mat(mu=calcMu(E, nu), lambda_=calcLambda(E, nu), rho0=rho0, model=MaterialModel.VISCOPLASTIC_MUD,
+                M_friction=0.4, q_cohesion=5e2, viscosity=0.1)
+
+
+def presetPowder(E=5e4, nu=0.25, rho0=1200.0):
+    return _mat(mu=calcMu(E, nu), lambda_=calcLambda(E, nu), rho0=rho0, model=MaterialModel.GRANULAR_POWDER,
+                M_friction=0.8, q_cohesion=50.0)
+
+
+def materials_bytes(mats: Iterable[MaterialParams]) -> bytes:
+    return b"".join(bytes(m) for m in mats)
+
+
+# ---------------------------------------------------------------------------
+# colliders — src/core/Collider.h:31-120
+# ---------------------------------------------------------------------------
+class ColliderSet:
+    def __init__(self):
+        self._p: List[ColliderPrimitive] = []
+
+    def _add(self, t, pos, n, radius, restitution, friction, vel):
+        c = ColliderPrimitive()
+        c.type = t
+        c.px, c.py, c.pz = (float(x) for x in pos)
+        c.nx, c.ny, c.nz = (float(x) for x in n)
+        c.radius = radius
+        c.vx, c.vy, c.vz = (float(x) for x in vel)
+        c.restitution, c.friction = restitution, friction
+        self._p.append(c)
+
+    def addPlane(self, point, normal, restitution=0.3, friction=0.0, vel=(0, 0, 0)):
+        n = np.asarray(normal, np.float32)
+        n = n / np.float32(np.sqrt(np.float32(np.dot(n, n))))
+        self._add(ColliderType.PLANE, point, n, 0.0, restitution, friction, vel)
+
+    def addSphere(self, center, radius, restitution=0.3, friction=0.0, vel=(0, 0, 0)):
+        self._add(ColliderType.SPHERE, center, (0, 0, 0), radius, restitution, friction, vel)
+
+    def addBox(self, center, halfExt, restitution=0.3, friction=0.0, vel=(0, 0, 0)):
+        self._add(ColliderType.BOX, center, halfExt, 0.0, restitution, friction, vel)
+
+    def addCapsule(self, start, axisVec, radius, restitution=0.3, friction=0.0, vel=(0, 0, 0)):
+        self._add(ColliderType.CAPSULE, start, axisVec, radius, restitution, friction, vel)
+
+    def clear(self):
+        self._p.clear()
+
+    def count(self):
+        return len(self._p)
+
+    def empty(self):
+        return not self._p
+
+    def data(self):
+        return list(self._p)
+
+    def to_bytes(self) -> bytes:
+        return b"".join(bytes(c) for c in self._p)
+
+
+# ---------------------------------------------------------------------------
+# forces — src/core/Force.h:48-181 (gravity and constant wind act on the grid)
+# ---------------------------------------------------------------------------
+class Force:
+    type_id = -1
+    affectMask = 0
+
+    def pack_data(self, data):
+        pass
+
+    def advanceTime(self, dt):
+        pass
+
+    def pack(self) -> ForceGPU:
+        g = ForceGPU()
+        g.type = self.type_id
+        g.affectMask = self.affectMask & 0xFFFF
+        self.pack_data(g.data)
+        return g
+
+
+class GravityForce(Force):
+    type_id = ForceType.GRAVITY
+
+    def __init__(self, direction=(0.0, 0.0, -1.0), strength=9.8):
+        self.direction = tuple(direction)
+        self.strength = strength
+        self.affectMask = 0
+
+    def pack_data(self, data):
+        data[0], data[1], data[2] = self.direction
+        data[3] = self.strength
+
+    @staticmethod
+    def FromDirection(direction, magnitude):
+        return GravityForce(direction, magnitude)
+
+
+class ConstantWindForce(GravityForce):
+    type_id = ForceType.CONSTANT_WIND
+
+    def __init__(self, direction=(1.0, 0.0, 0.0), strength=1.0):
+        super().__init__(direction, strength)
+
+    @staticmethod
+    def FromDirection(direction, magnitude):
+        return ConstantWindForce(direction, magnitude)
+
+
+class ZClampForce(Force):
+    """Position constraint; no effect on the MPM grid (the reference's grid
+    shader has no post-force marker, src/core/Force.h:75-82)."""
+
+    type_id = ForceType.Z_CLAMP
+
+    def __init__(self, z=0.0):
+        self.zValue = z
+        self.affectMask = 0
+
+    def pack_data(self, data):
+        data[0] = self.zValue
+
+
+def forces_bytes(forces: Iterable[Force]) -> bytes:
+    return b"".join(bytes(f.pack()) for f in forces)
+
+
+# ---------------------------------------------------------------------------
+# the library
+# ---------------------------------------------------------------------------
+_lib = None
+
+EXPORTED_SYMBOLS = (
+    "nb_last_error", "nb_mpm_abi_version", "nb_mpm_config_default", "nb_mpm_knobs_default", "nb_mpm_create",
+    "nb_mpm_destroy", "nb_mpm_step", "nb_mpm_substep", "nb_mpm_substeps", "nb_mpm_sync", "nb_mpm_set_knobs",
+    "nb_mpm_get_knobs", "nb_mpm_get_config", "nb_mpm_grid_res", "nb_mpm_total_cells",
+    "nb_mpm_live_particle_count", "nb_mpm_set_materials", "nb_mpm_set_particle_material_ids",
+    "nb_mpm_set_colliders", "nb_mpm_clear_analytic_colliders", "nb_mpm_set_collider_sdf",
+    "nb_mpm_set_collider_sphere", "nb_mpm_clear_collider", "nb_mpm_set_forces", "nb_mpm_append_particles",
+    "nb_mpm_reset_particles", "nb_mpm_get_field", "nb_mpm_set_field", "nb_mpm_field_device_ptr",
+    "nb_mpm_set_live_particle_count", "nb_mpm_run_bin", "nb_mpm_get_bins", "nb_mpm_run_p2g",
+    "nb_mpm_run_grid_update", "nb_mpm_run_g2p", "nb_mpm_get_grid", "nb_mpm_set_grid",
+    "nb_mpm_profile_substeps", "nb_mpm_active_node_count", "nb_mpm_active_tile_count",
+)
+
+
+class NebulaError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libnebula_mpm.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise NebulaError(f"HIP library missing: {LIB_PATH} — build it with __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        vp, fp, up = C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_uint32)
+        L.nb_last_error.restype = C.c_char_p
+        L.nb_mpm_create.argtypes = [C.POINTER(MPMConfig), vp, C.POINTER(vp)]
+        L.nb_mpm_config_default.argtypes = [C.POINTER(MPMConfig)]
+        L.nb_mpm_config_default.restype = None
+        L.nb_mpm_knobs_default.argtypes = [C.POINTER(_Knobs)]
+        L.nb_mpm_knobs_default.restype = None
+        for nm in ("nb_mpm_destroy", "nb_mpm_sync", "nb_mpm_clear_analytic_colliders", "nb_mpm_clear_collider",
+                   "nb_mpm_reset_particles", "nb_mpm_run_bin"):
+            getattr(L, nm).argtypes = [vp]
+        for nm in ("nb_mpm_step", "nb_mpm_substep", "nb_mpm_run_p2g", "nb_mpm_run_grid_update", "nb_mpm_run_g2p"):
+            getattr(L, nm).argtypes = [vp, C.c_float]
+        L.nb_mpm_substeps.argtypes = [vp, C.c_float, C.c_uint32]
+        L.nb_mpm_set_knobs.argtypes = [vp, C.POINTER(_Knobs)]
+        L.nb_mpm_get_knobs.argtypes = [vp, C.POINTER(_Knobs)]
+        L.nb_mpm_get_config.argtypes = [vp, C.POINTER(MPMConfig)]
+        L.nb_mpm_grid_res.argtypes = [vp, up]
+        L.nb_mpm_total_cells.argtypes = [vp, up]
+        L.nb_mpm_live_particle_count.argtypes = [vp, up]
+        L.nb_mpm_set_materials.argtypes = [vp, vp, C.c_uint32]
+        L.nb_mpm_set_particle_material_ids.argtypes = [vp, up, C.c_uint32]
+        L.nb_mpm_set_colliders.argtypes = [vp, vp, C.c_uint32]
+        L.nb_mpm_set_collider_sdf.argtypes = [vp, fp, C.c_uint64]
+        L.nb_mpm_set_collider_sphere.argtypes = [vp, C.c_float, C.c_float, C.c_float, C.c_float]
+        L.nb_mpm_set_forces.argtypes = [vp, vp, C.c_uint32]
+        L.nb_mpm_append_particles.argtypes = [vp, fp, fp, C.c_uint32, up]
+        L.nb_mpm_get_field.argtypes = [vp, C.c_int, fp, C.c_uint32, C.c_uint32]
+        L.nb_mpm_set_field.argtypes = [vp, C.c_int, fp, C.c_uint32, C.c_uint32]
+        L.nb_mpm_field_device_ptr.argtypes = [vp, C.c_int, C.POINTER(vp)]
+        L.nb_mpm_set_live_particle_count.argtypes = [vp, C.c_uint32]
+        L.nb_mpm_get_bins.argtypes = [vp, up, up, up, up]
+        L.nb_mpm_get_grid.argtypes = [vp, fp, fp]
+        L.nb_mpm_set_grid.argtypes = [vp, fp, fp, fp]
+        L.nb_mpm_profile_substeps.argtypes = [vp, C.c_float, C.c_uint32, fp, C.c_uint32, up, C.c_char_p, C.c_uint32]
+        L.nb_mpm_active_node_count.argtypes = [vp, C.POINTER(C.c_uint64)]
+        L.nb_mpm_active_tile_count.argtypes = [vp, up]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != NB_OK:
+        raise NebulaError(f"nebula_mpm error {rc}: {lib().nb_last_error().decode()}")
+
+
+def _f(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _u(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint32)) if a is not None else None
+
+
+class MPMEngine:
+    """The reference's MPMEngine over the HIP library.
+
+    ``init(cfg)`` allocates and seeds the default particle block exactly as
+    src/engine/MPMEngine.cpp:34-156 does; ``step(dt)`` enqueues
+    ``numSubsteps`` substeps on the engine's HIP stream (call ``sync()`` to
+    wait).  Public knobs are properties with the reference's names.
+    """
+
+    def __init__(self, cfg: Optional[MPMConfig] = None, stream: Optional[int] = None):
+        self._h = None
+        self._forces: List[Force] = []
+        if cfg is not None:
+            self.init(cfg, stream)
+
+    # lifecycle -----------------------------------------------------------
+    def init(self, cfg: MPMConfig, stream: Optional[int] = None):
+        h = C.c_void_p()
+        _check(lib().nb_mpm_create(C.byref(cfg), C.c_void_p(stream) if stream else None, C.byref(h)))
+        self._h = h
+        self._cfg = cfg
+        self._n_cells = self.totalCells()
+
+    def cleanup(self):
+        if self._h is not None:
+            lib().nb_mpm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.cleanup()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    # stepping ------------------------------------------------------------
+    def step(self, dt: float):
+        for f in self._forces:
+            f.advanceTime(dt)
+        if self._forces:
+            self._upload_forces()
+        _check(lib().nb_mpm_step(self._h, dt))
+
+    def substep(self, dt_sub: float, n: int = 1):
+        _check(lib().nb_mpm_substeps(self._h, dt_sub, n))
+
+    def sync(self):
+        _check(lib().nb_mpm_sync(self._h))
+
+    # knobs ---------------------------------------------------------------
+    def _knobs(self) -> _Knobs:
+        k = _Knobs()
+        _check(lib().nb_mpm_get_knobs(self._h, C.byref(k)))
+        return k
+
+    def _set_knob(self, name, value):
+        k = self._knobs()
+        setattr(k, name, value)
+        _check(lib().nb_mpm_set_knobs(self._h, C.byref(k)))
+
+    numSubsteps = property(lambda s: s._knobs().num_substeps, lambda s, v: s._set_knob("num_substeps", int(v)))
+    flip_ratio = property(lambda s: s._knobs().flip_ratio, lambda s, v: s._set_knob("flip_ratio", float(v)))
+    restitution = property(lambda s: s._knobs().restitution, lambda s, v: s._set_knob("restitution", float(v)))
+    wall_friction = property(lambda s: s._knobs().wall_friction, lambda s, v: s._set_knob("wall_friction", float(v)))
+    plasticModel = property(lambda s: s._knobs().plastic_model, lambda s, v: s._set_knob("plastic_model", int(v)))
+    q_max = property(lambda s: s._knobs().q_max, lambda s, v: s._set_knob("q_max", float(v)))
+    M_friction = property(lambda s: s._knobs().M_friction, lambda s, v: s._set_knob("M_friction", float(v)))
+    q_cohesion = property(lambda s: s._knobs().q_cohesion, lambda s, v: s._set_knob("q_cohesion", float(v)))
+
+    # queries -------------------------------------------------------------
+    def config(self) -> MPMConfig:
+        c = MPMConfig()
+        _check(lib().nb_mpm_get_config(self._h, C.byref(c)))
+        return c
+
+    def liveParticleCount(self) -> int:
+        n = C.c_uint32()
+        _check(lib().nb_mpm_live_particle_count(self._h, C.byref(n)))
+        return n.value
+
+    def gridRes(self):
+        r = (C.c_uint32 * 3)()
+        _check(lib().nb_mpm_grid_res(self._h, r))
+        return tuple(r)
+
+    def totalCells(self) -> int:
+        n = C.c_uint32()
+        _check(lib().nb_mpm_total_cells(self._h, C.byref(n)))
+        return n.value
+
+    # scene data ----------------------------------------------------------
+    def setMaterials(self, mats: Sequence[MaterialParams]):
+        arr = (MaterialParams * max(len(mats), 1))(*mats)
+        _check(lib().nb_mpm_set_materials(self._h, C.cast(arr, C.c_void_p), len(mats)))
+
+    def setParticleMaterialIds(self, ids):
+        a = np.ascontiguousarray(ids, np.uint32)
+        _check(lib().nb_mpm_set_particle_material_ids(self._h, _u(a), a.size))
+
+    def setColliders(self, cols: ColliderSet):
+        data = cols.data()
+        arr = (ColliderPrimitive * max(len(data), 1))(*data)
+        _check(lib().nb_mpm_set_colliders(self._h, C.cast(arr, C.c_void_p), len(data)))
+
+    def clearAnalyticColliders(self):
+        _check(lib().nb_mpm_clear_analytic_colliders(self._h))
+
+    def setColliderSDF(self, mortonSDF):
+        a = np.ascontiguousarray(mortonSDF, np.float32)
+        _check(lib().nb_mpm_set_collider_sdf(self._h, _f(a), a.size))
+
+    def setColliderSphere(self, radius, cx, cy, cz):
+        _check(lib().nb_mpm_set_collider_sphere(self._h, radius, cx, cy, cz))
+
+    def clearCollider(self):
+        _check(lib().nb_mpm_clear_collider(self._h))
+
+    def _upload_forces(self):
+        packed = [f.pack() for f in self._forces]
+        arr = (ForceGPU * max(len(packed), 1))(*packed)
+        _check(lib().nb_mpm_set_forces(self._h, C.cast(arr, C.c_void_p), len(packed)))
+
+    def addForce(self, f: Force):
+        self._forces.append(f)
+        self._upload_forces()
+
+    def removeForce(self, f: Force):
+        self._forces = [x for x in self._forces if x is not f]
+        self._upload_forces()
+
+    def setForces(self, forces: Sequence[Force]):
+        self._forces = list(forces)
+        self._upload_forces()
+
+    def clearForces(self):
+        self._forces = []
+        self._upload_forces()
+
+    def forces(self):
+        return list(self._forces)
+
+    def appendParticles(self, pos, vel) -> int:
+        p = np.ascontiguousarray(pos, np.float32).reshape(-1, 4)
+        v = np.ascontiguousarray(vel, np.float32).reshape(-1, 4)
+        if p.shape != v.shape:
+            raise ValueError("pos and vel must have the same length")
+        n = C.c_uint32()
+        _check(lib().nb_mpm_append_particles(self._h, _f(p), _f(v), p.shape[0], C.byref(n)))
+        return n.value
+
+    def resetParticles(self):
+        _check(lib().nb_mpm_reset_particles(self._h))
+
+    # state access ----------------------------------------------------------
+    def getField(self, field: int, first: int = 0, count: Optional[int] = None):
+        if count is None:
+            count = self.liveParticleCount() - first
+        out = np.empty((count, 4), np.float32)
+        if count:
+            _check(lib().nb_mpm_get_field(self._h, field, _f(out), first, count))
+        return out
+
+    def setField(self, field: int, data, first: int = 0):
+        a = np.ascontiguousarray(data, np.float32).reshape(-1, 4)
+        _check(lib().nb_mpm_set_field(self._h, field, _f(a), first, a.shape[0]))
+
+    def getState(self):
+        """(8, n, 4) float32: P, v, F0, F1, F2, B0, B1, B2."""
+        return np.stack([self.getField(f) for f in range(8)])
+
+    def setState(self, fields):
+        f = np.ascontiguousarray(fields, np.float32)
+        n = f.shape[1]
+        _check(lib().nb_mpm_set_live_particle_count(self._h, n))
+        for i in range(8):
+            self.setField(i, f[i])
+
+    def getPositionBuffer(self) -> int:
+        p = C.c_void_p()
+        _check(lib().nb_mpm_field_device_ptr(self._h, FIELD_P, C.byref(p)))
+        return p.value
+
+    def getVelocityBuffer(self) -> int:
+        p = C.c_void_p()
+        _check(lib().nb_mpm_field_device_ptr(self._h, FIELD_V, C.byref(p)))
+        return p.value
+
+    # pass-level access (tests/helpers/MPMHarness.h:56-59) ----------------------
+    def runBin(self):
+        _check(lib().nb_mpm_run_bin(self._h))
+
+    def getBins(self):
+        n = self.liveParticleCount()
+        NC = self._n_cells
+        keys = np.empty(n, np.uint32)
+        srt = np.empty(n, np.uint32)
+        cnt = np.empty(NC, np.uint32)
+        end = np.empty(NC, np.uint32)
+        _check(lib().nb_mpm_get_bins(self._h, _u(keys), _u(srt), _u(cnt), _u(end)))
+        return keys, srt, cnt, end
+
+    def runP2G(self, dt):
+        _check(lib().nb_mpm_run_p2g(self._h, dt))
+
+    def runGridUpdate(self, dt):
+        _check(lib().nb_mpm_run_grid_update(self._h, dt))
+
+    def runG2P(self, dt):
+        _check(lib().nb_mpm_run_g2p(self._h, dt))
+
+    def getGrid(self):
+        NC = self._n_cells
+        mom = np.empty((2 * NC, 4), np.float32)
+        mass = np.empty(NC, np.float32)
+        _check(lib().nb_mpm_get_grid(self._h, _f(mom), _f(mass)))
+        return mom, mass
+
+    def setGrid(self, mom4, mass, old4=None):
+        """mom4: NC float4 (or the full 2*NC export, whose second half is then
+        taken as the saved velocity unless old4 is given), mass: NC floats."""
+        NC = self._n_cells
+        full = np.asarray(mom4, np.float32).reshape(-1, 4)
+        m = np.ascontiguousarray(full[:NC])
+        if old4 is None and full.shape[0] >= 2 * NC:
+            old4 = full[NC:2 * NC]
+        ms = np.ascontiguousarray(mass, np.float32)
+        o = np.ascontiguousarray(np.asarray(old4, np.float32).reshape(-1, 4)[:NC]) if old4 is not None else None
+        _check(lib().nb_mpm_set_grid(self._h, _f(m), _f(ms), _f(o) if o is not None else None))
+
+    # measurement -----------------------------------------------------------
+    def profileSubsteps(self, dt_sub: float, n: int):
+        """Run n substeps with HIP events around every kernel; returns
+        {kernel name: total ms}."""
+        ms = np.zeros(32, np.float32)
+        nk = C.c_uint32()
+        names = C.create_string_buffer(1024)
+        _check(lib().nb_mpm_profile_substeps(self._h, dt_sub, n, _f(ms), 32, C.byref(nk), names, 1024))
+        keys = names.value.decode().split(",")
+        return {k: float(ms[i]) for i, k in enumerate(keys[: nk.value])}
+
+    def activeNodeCount(self) -> int:
+        g = C.c_uint64()
+        _check(lib().nb_mpm_active_node_count(self._h, C.byref(g)))
+        return g.value
+
+    def activeTileCount(self) -> int:
+        t = C.c_uint32()
+        _check(lib().nb_mpm_active_tile_count(self._h, C.byref(t)))
+        return t.value

@@ -1,0 +1,323 @@
+"""CPU oracle of the MPM substep — TEST INFRASTRUCTURE ONLY.
+
+This package is the checker.  Only ``tests/``, ``__graft_entry__.smoke()`` and
+the ``cpu_baseline`` leg of ``bench.py`` may import it; the product library
+(``nebula_amd``) never does, and its kernels never route through it.
+
+``mpm_oracle.hpp`` restates the reference's GLSL substep (shaders/mpm_*.comp,
+mpm_common.glsl, collider_common.glsl, mpm_nanovdb_bc.comp) in C++, templated
+on the arithmetic type: ``precision=32`` is the reference-semantics FP32 oracle,
+``precision=64`` the FP64 twin.  ``make -C oracle`` builds
+``oracle/build/liboracle_mpm.so``, which this module loads through ctypes.
+
+Parity status: every function is pinned by a test in ``tests/test_oracle*.py``
+(known-answer tests restated from the reference's tests/test_mpm.cpp and
+tests/test_domain.cpp, closed forms, numpy LAPACK, brute force), except the
+degenerate SVD branches (the ``sign(0)`` Jacobi stall and the ``sigma <= 1e-8``
+identity columns), which follow the GLSL text only: parity unpinned there.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "build", "liboracle_mpm.so")
+
+_lib = None
+
+
+class OrcScene(C.Structure):
+    _fields_ = [
+        ("dt", C.c_float),
+        ("cell_size", C.c_float),
+        ("grid_res", C.c_uint32 * 3),
+        ("world_min", C.c_float * 3),
+        ("world_max", C.c_float * 3),
+        ("flip_ratio", C.c_float),
+        ("cube_res", C.c_uint32),
+        ("material_count", C.c_uint32),
+        ("collider_count", C.c_uint32),
+        ("force_count", C.c_uint32),
+        ("has_sdf", C.c_uint32),
+    ]
+
+
+def build():
+    """Compile the oracle with its Makefile (g++)."""
+    import subprocess
+
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle library missing: {LIB_PATH} (run `make -C oracle`)")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        fp = C.POINTER(C.c_float)
+        dp = C.POINTER(C.c_double)
+        up = C.POINTER(C.c_uint32)
+        L.orc_sim_create.restype = vp
+        L.orc_sim_create.argtypes = [C.POINTER(OrcScene), vp, vp, vp, fp, C.c_int]
+        L.orc_sim_destroy.argtypes = [vp]
+        L.orc_sim_set_dt.argtypes = [vp, C.c_float]
+        L.orc_sim_set_flip_ratio.argtypes = [vp, C.c_float]
+        L.orc_sim_set_colliders.argtypes = [vp, vp, C.c_uint32]
+        L.orc_sim_set_state.argtypes = [vp, fp, C.c_uint32]
+        L.orc_sim_get_state.argtypes = [vp, fp]
+        L.orc_sim_get_state_f64.argtypes = [vp, dp]
+        for nm in ("orc_sim_bin", "orc_sim_p2g", "orc_sim_grid_update", "orc_sim_sdf_bc", "orc_sim_g2p"):
+            getattr(L, nm).argtypes = [vp]
+        L.orc_sim_substeps.argtypes = [vp, C.c_uint32]
+        L.orc_sim_substeps.restype = C.c_double
+        L.orc_sim_get_bins.argtypes = [vp, up, up, up, up]
+        L.orc_sim_set_bins.argtypes = [vp, up, up, up]
+        L.orc_sim_get_grid.argtypes = [vp, fp, fp]
+        L.orc_sim_get_grid_f64.argtypes = [vp, dp, dp]
+        L.orc_sim_set_grid.argtypes = [vp, fp, fp]
+        L.orc_morton_encode.restype = C.c_uint32
+        L.orc_morton_encode.argtypes = [C.c_int, C.c_int, C.c_int]
+        L.orc_morton_decode.argtypes = [C.c_uint32, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.orc_cell_id.restype = C.c_uint32
+        L.orc_cell_id.argtypes = [C.POINTER(OrcScene), C.c_float, C.c_float, C.c_float]
+        for nm, t in (("orc_bspline2_f32", C.c_float), ("orc_bspline2g_f32", C.c_float),
+                      ("orc_bspline2_f64", C.c_double), ("orc_bspline2g_f64", C.c_double)):
+            getattr(L, nm).restype = t
+            getattr(L, nm).argtypes = [t]
+        L.orc_svd3_f32.argtypes = [fp, fp, fp, fp]
+        L.orc_svd3_f64.argtypes = [dp, dp, dp, dp]
+        L.orc_hencky_f32.argtypes = [fp, C.c_float, C.c_float, fp]
+        L.orc_hencky_f64.argtypes = [dp, C.c_double, C.c_double, dp]
+        L.orc_project_vm_f32.argtypes = [fp, C.c_float, C.c_float, C.c_float, fp]
+        L.orc_project_vm_f64.argtypes = [dp, C.c_double, C.c_double, C.c_double, dp]
+        L.orc_project_dp_f32.argtypes = [fp, C.c_float, C.c_float, C.c_float, C.c_float, fp]
+        L.orc_project_dp_f64.argtypes = [dp, C.c_double, C.c_double, C.c_double, C.c_double, dp]
+        _lib = L
+    return _lib
+
+
+def _fptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _uptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint32)) if a is not None else None
+
+
+def cube_res_of(grid_res):
+    m = max(max(grid_res), 1)
+    p = 1
+    while p < m:
+        p <<= 1
+    return p
+
+
+def make_scene(dt, cell_size, grid_res, world_min=(0.0, 0.0, 0.0), world_max=None, flip_ratio=0.0):
+    sc = OrcScene()
+    sc.dt = dt
+    sc.cell_size = cell_size
+    for a in range(3):
+        sc.grid_res[a] = int(grid_res[a])
+        sc.world_min[a] = float(world_min[a])
+        sc.world_max[a] = float(world_max[a]) if world_max is not None else float(world_min[a] + grid_res[a] * cell_size)
+    sc.flip_ratio = flip_ratio
+    sc.cube_res = cube_res_of(grid_res)
+    return sc
+
+
+class Sim:
+    """One oracle simulation: the reference's substep on host arrays.
+
+    ``materials``/``colliders``/``forces`` are raw bytes of the 64/64/32-byte
+    records (the reference's MaterialParams / ColliderPrimitive / ForceGPU).
+    ``fields`` is an (8, n, 4) float32 array: P, v, F0, F1, F2, B0, B1, B2.
+    """
+
+    def __init__(self, scene, materials: bytes, colliders: bytes = b"", forces: bytes = b"", sdf=None,
+                 precision=32):
+        L = lib()
+        sc = OrcScene.from_buffer_copy(scene)
+        sc.material_count = len(materials) // 64
+        sc.collider_count = len(colliders) // 64
+        sc.force_count = len(forces) // 32
+        sc.has_sdf = 1 if sdf is not None else 0
+        self.scene = sc
+        self.precision = precision
+        self.NC = sc.cube_res ** 3
+        self._mats = C.create_string_buffer(materials, max(len(materials), 1))
+        self._cols = C.create_string_buffer(colliders, max(len(colliders), 1))
+        self._forces = C.create_string_buffer(forces, max(len(forces), 1))
+        self._sdf = np.ascontiguousarray(sdf, dtype=np.float32) if sdf is not None else None
+        self.h = L.orc_sim_create(C.byref(sc), self._mats, self._cols, self._forces,
+                                  _fptr(self._sdf) if self._sdf is not None else None, precision)
+        self.n = 0
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_sim_destroy(self.h)
+            self.h = None
+
+    def set_dt(self, dt):
+        lib().orc_sim_set_dt(self.h, dt)
+
+    def set_flip_ratio(self, r):
+        lib().orc_sim_set_flip_ratio(self.h, r)
+
+    def set_colliders(self, colliders: bytes):
+        buf = C.create_string_buffer(colliders, max(len(colliders), 1))
+        lib().orc_sim_set_colliders(self.h, buf, len(colliders) // 64)
+
+    def set_state(self, fields):
+        f = np.ascontiguousarray(fields, dtype=np.float32)
+        assert f.ndim == 3 and f.shape[0] == 8 and f.shape[2] == 4
+        self.n = f.shape[1]
+        lib().orc_sim_set_state(self.h, _fptr(f), self.n)
+
+    def get_state(self):
+        out = np.empty((8, self.n, 4), np.float32)
+        lib().orc_sim_get_state(self.h, _fptr(out))
+        return out
+
+    def get_state_f64(self):
+        assert self.precision == 64
+        out = np.empty((8, self.n, 4), np.float64)
+        lib().orc_sim_get_state_f64(self.h, _dptr(out))
+        return out
+
+    def bin(self):
+        lib().orc_sim_bin(self.h)
+
+    def get_bins(self):
+        keys = np.empty(self.n, np.uint32)
+        srt = np.empty(self.n, np.uint32)
+        cnt = np.empty(self.NC, np.uint32)
+        end = np.empty(self.NC, np.uint32)
+        lib().orc_sim_get_bins(self.h, _uptr(keys), _uptr(srt), _uptr(cnt), _uptr(end))
+        return keys, srt, cnt, end
+
+    def set_bins(self, sorted_idx, cell_count, cell_end):
+        s = np.ascontiguousarray(sorted_idx, np.uint32)
+        c = np.ascontiguousarray(cell_count, np.uint32)
+        e = np.ascontiguousarray(cell_end, np.uint32)
+        lib().orc_sim_set_bins(self.h, _uptr(s), _uptr(c), _uptr(e))
+
+    def p2g(self):
+        lib().orc_sim_p2g(self.h)
+
+    def grid_update(self):
+        lib().orc_sim_grid_update(self.h)
+
+    def sdf_bc(self):
+        lib().orc_sim_sdf_bc(self.h)
+
+    def g2p(self):
+        lib().orc_sim_g2p(self.h)
+
+    def substeps(self, n):
+        """Run n full substeps; returns the wall time in seconds."""
+        return lib().orc_sim_substeps(self.h, n)
+
+    def get_grid(self):
+        mom = np.empty((2 * self.NC, 4), np.float32)
+        mass = np.empty(self.NC, np.float32)
+        lib().orc_sim_get_grid(self.h, _fptr(mom), _fptr(mass))
+        return mom, mass
+
+    def get_grid_f64(self):
+        mom = np.empty((2 * self.NC, 4), np.float64)
+        mass = np.empty(self.NC, np.float64)
+        lib().orc_sim_get_grid_f64(self.h, _dptr(mom), _dptr(mass))
+        return mom, mass
+
+    def set_grid(self, mom4, mass):
+        m = np.ascontiguousarray(mom4, np.float32).reshape(-1, 4)[: self.NC]
+        ms = np.ascontiguousarray(mass, np.float32)
+        lib().orc_sim_set_grid(self.h, _fptr(np.ascontiguousarray(m)), _fptr(ms))
+
+
+# ---- scalar helpers (property tests) ----
+
+def morton_encode(x, y, z):
+    return lib().orc_morton_encode(x, y, z)
+
+
+def morton_decode(c):
+    x, y, z = C.c_int(), C.c_int(), C.c_int()
+    lib().orc_morton_decode(c, C.byref(x), C.byref(y), C.byref(z))
+    return x.value, y.value, z.value
+
+
+def cell_id(scene, x, y, z):
+    return lib().orc_cell_id(C.byref(scene), x, y, z)
+
+
+def bspline2(d, precision=32):
+    return lib().orc_bspline2_f32(d) if precision == 32 else lib().orc_bspline2_f64(d)
+
+
+def bspline2g(d, precision=32):
+    return lib().orc_bspline2g_f32(d) if precision == 32 else lib().orc_bspline2g_f64(d)
+
+
+def _m(a, dt):
+    """numpy (3,3) row-major matrix -> column-major flat array."""
+    return np.ascontiguousarray(np.asarray(a, dt).T.reshape(9))
+
+
+def _unm(a):
+    return a.reshape(3, 3).T.copy()
+
+
+def svd3(F, precision=32):
+    dt = np.float32 if precision == 32 else np.float64
+    f = _m(F, dt)
+    U = np.empty(9, dt)
+    V = np.empty(9, dt)
+    s = np.empty(3, dt)
+    if precision == 32:
+        lib().orc_svd3_f32(_fptr(f), _fptr(U), _fptr(s), _fptr(V))
+    else:
+        lib().orc_svd3_f64(_dptr(f), _dptr(U), _dptr(s), _dptr(V))
+    return _unm(U), s, _unm(V)
+
+
+def hencky_stress(F, mu, lam, precision=32):
+    dt = np.float32 if precision == 32 else np.float64
+    f = _m(F, dt)
+    t = np.empty(9, dt)
+    if precision == 32:
+        lib().orc_hencky_f32(_fptr(f), mu, lam, _fptr(t))
+    else:
+        lib().orc_hencky_f64(_dptr(f), mu, lam, _dptr(t))
+    return _unm(t)
+
+
+def project_von_mises(F, mu, lam, q_max, precision=32):
+    dt = np.float32 if precision == 32 else np.float64
+    f = _m(F, dt)
+    o = np.empty(9, dt)
+    if precision == 32:
+        lib().orc_project_vm_f32(_fptr(f), mu, lam, q_max, _fptr(o))
+    else:
+        lib().orc_project_vm_f64(_dptr(f), mu, lam, q_max, _dptr(o))
+    return _unm(o)
+
+
+def project_drucker_prager(F, mu, lam, M, q_c, precision=32):
+    dt = np.float32 if precision == 32 else np.float64
+    f = _m(F, dt)
+    o = np.empty(9, dt)
+    if precision == 32:
+        lib().orc_project_dp_f32(_fptr(f), mu, lam, M, q_c, _fptr(o))
+    else:
+        lib().orc_project_dp_f64(_dptr(f), mu, lam, M, q_c, _dptr(o))
+    return _unm(o)

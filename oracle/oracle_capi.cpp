@@ -1,0 +1,273 @@
+// oracle_capi.cpp — C entry points of the CPU oracle, loaded by the Python
+// test harness through ctypes.  TEST INFRASTRUCTURE ONLY: the product library
+// never links this file, and only tests/, smoke() and the cpu_baseline leg of
+// bench.py load it.
+#include <chrono>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "mpm_oracle.hpp"
+
+using namespace oracle;
+
+namespace {
+template <class R> static M3<R> load_m3(const R* a) {
+  M3<R> m;
+  for (int c = 0; c < 3; c++) m.c[c] = {a[3 * c + 0], a[3 * c + 1], a[3 * c + 2]};
+  return m;
+}
+template <class R> static void store_m3(const M3<R>& m, R* a) {
+  for (int c = 0; c < 3; c++)
+    for (int r = 0; r < 3; r++) a[3 * c + r] = m.at(c, r);
+}
+}  // namespace
+
+extern "C" {
+
+// Scene description handed in by the harness (a flat POD copy of Params).
+struct orc_scene {
+  float dt;
+  float cell_size;
+  uint32_t grid_res[3];
+  float world_min[3];
+  float world_max[3];
+  float flip_ratio;
+  uint32_t cube_res;
+  uint32_t material_count;
+  uint32_t collider_count;
+  uint32_t force_count;
+  uint32_t has_sdf;
+};
+
+struct orc_sim {
+  int precision;  // 32 or 64
+  orc_scene sc;
+  std::vector<Material> mats;
+  std::vector<Collider> cols;
+  std::vector<Force> forces;
+  std::vector<float> sdf;
+  State<float> s32;
+  State<double> s64;
+
+  Params params() const {
+    Params p{};
+    p.dt = sc.dt;
+    p.cell_size = sc.cell_size;
+    for (int a = 0; a < 3; a++) {
+      p.grid_res[a] = sc.grid_res[a];
+      p.world_min[a] = sc.world_min[a];
+      p.world_max[a] = sc.world_max[a];
+    }
+    p.flip_ratio = sc.flip_ratio;
+    p.cube_res = sc.cube_res;
+    p.materials = mats.data();
+    p.material_count = uint32_t(mats.size());
+    p.colliders = cols.data();
+    p.collider_count = uint32_t(cols.size());
+    p.forces = forces.data();
+    p.force_count = uint32_t(forces.size());
+    p.sdf = sc.has_sdf ? sdf.data() : nullptr;
+    return p;
+  }
+};
+
+orc_sim* orc_sim_create(const orc_scene* sc, const void* mats, const void* cols, const void* forces, const float* sdf,
+                        int precision) {
+  orc_sim* s = new orc_sim();
+  s->precision = precision;
+  s->sc = *sc;
+  s->mats.resize(sc->material_count);
+  if (sc->material_count) std::memcpy(s->mats.data(), mats, sizeof(Material) * sc->material_count);
+  s->cols.resize(sc->collider_count);
+  if (sc->collider_count) std::memcpy(s->cols.data(), cols, sizeof(Collider) * sc->collider_count);
+  s->forces.resize(sc->force_count);
+  if (sc->force_count) std::memcpy(s->forces.data(), forces, sizeof(Force) * sc->force_count);
+  uint32_t NC = sc->cube_res * sc->cube_res * sc->cube_res;
+  if (sc->has_sdf) s->sdf.assign(sdf, sdf + NC);
+  return s;
+}
+
+void orc_sim_destroy(orc_sim* s) { delete s; }
+
+void orc_sim_set_dt(orc_sim* s, float dt) { s->sc.dt = dt; }
+void orc_sim_set_flip_ratio(orc_sim* s, float r) { s->sc.flip_ratio = r; }
+void orc_sim_set_colliders(orc_sim* s, const void* cols, uint32_t n) {
+  s->cols.resize(n);
+  if (n) std::memcpy(s->cols.data(), cols, sizeof(Collider) * n);
+  s->sc.collider_count = n;
+}
+
+// fields: 8 arrays of n float4 concatenated (P, v, F0, F1, F2, B0, B1, B2)
+void orc_sim_set_state(orc_sim* s, const float* fields, uint32_t n) {
+  uint32_t NC = s->sc.cube_res * s->sc.cube_res * s->sc.cube_res;
+  if (s->precision == 64) {
+    s->s64.resize(n, NC);
+    for (int f = 0; f < 8; f++)
+      for (size_t i = 0; i < size_t(n) * 4; i++) s->s64.f[f][i] = double(fields[size_t(f) * n * 4 + i]);
+    for (uint32_t i = 0; i < n; i++) std::memcpy(&s->s64.matid[i], &fields[size_t(1) * n * 4 + 4 * i + 3], 4);
+  } else {
+    s->s32.resize(n, NC);
+    for (int f = 0; f < 8; f++) std::memcpy(s->s32.f[f].data(), fields + size_t(f) * n * 4, sizeof(float) * n * 4);
+    for (uint32_t i = 0; i < n; i++) std::memcpy(&s->s32.matid[i], &fields[size_t(1) * n * 4 + 4 * i + 3], 4);
+  }
+}
+
+void orc_sim_get_state(orc_sim* s, float* fields) {
+  uint32_t n = s->precision == 64 ? s->s64.n : s->s32.n;
+  for (int f = 0; f < 8; f++)
+    for (size_t i = 0; i < size_t(n) * 4; i++)
+      fields[size_t(f) * n * 4 + i] = s->precision == 64 ? float(s->s64.f[f][i]) : s->s32.f[f][i];
+  for (uint32_t i = 0; i < n; i++) {
+    uint32_t id = s->precision == 64 ? s->s64.matid[i] : s->s32.matid[i];
+    std::memcpy(&fields[size_t(1) * n * 4 + 4 * i + 3], &id, 4);
+  }
+}
+
+// double-precision state export (FP64 twin only); matid lane holds the id as a number
+void orc_sim_get_state_f64(orc_sim* s, double* fields) {
+  uint32_t n = s->s64.n;
+  for (int f = 0; f < 8; f++)
+    for (size_t i = 0; i < size_t(n) * 4; i++) fields[size_t(f) * n * 4 + i] = s->s64.f[f][i];
+  for (uint32_t i = 0; i < n; i++) fields[size_t(1) * n * 4 + 4 * i + 3] = double(s->s64.matid[i]);
+}
+
+void orc_sim_bin(orc_sim* s) {
+  Params p = s->params();
+  if (s->precision == 64)
+    bin<double>(p, s->s64.f[0].data(), s->s64.n, s->s64.keys.data(), s->s64.sorted.data(), s->s64.cell_count.data(),
+                s->s64.cell_end.data());
+  else
+    bin<float>(p, s->s32.f[0].data(), s->s32.n, s->s32.keys.data(), s->s32.sorted.data(), s->s32.cell_count.data(),
+               s->s32.cell_end.data());
+}
+
+void orc_sim_get_bins(orc_sim* s, uint32_t* keys, uint32_t* sorted, uint32_t* cell_count, uint32_t* cell_end) {
+  auto cp = [](const std::vector<uint32_t>& v, uint32_t* out) {
+    if (out) std::memcpy(out, v.data(), v.size() * 4);
+  };
+  if (s->precision == 64) {
+    cp(s->s64.keys, keys); cp(s->s64.sorted, sorted); cp(s->s64.cell_count, cell_count); cp(s->s64.cell_end, cell_end);
+  } else {
+    cp(s->s32.keys, keys); cp(s->s32.sorted, sorted); cp(s->s32.cell_count, cell_count); cp(s->s32.cell_end, cell_end);
+  }
+}
+
+void orc_sim_p2g(orc_sim* s) {
+  Params p = s->params();
+  if (s->precision == 64) p2g<double>(p, s->s64); else p2g<float>(p, s->s32);
+}
+void orc_sim_grid_update(orc_sim* s) {
+  Params p = s->params();
+  if (s->precision == 64) grid_update<double>(p, s->s64); else grid_update<float>(p, s->s32);
+}
+void orc_sim_sdf_bc(orc_sim* s) {
+  Params p = s->params();
+  if (s->precision == 64) sdf_bc<double>(p, s->s64); else sdf_bc<float>(p, s->s32);
+}
+void orc_sim_g2p(orc_sim* s) {
+  Params p = s->params();
+  if (s->precision == 64) g2p<double>(p, s->s64); else g2p<float>(p, s->s32);
+}
+// n substeps; returns the wall time in seconds
+double orc_sim_substeps(orc_sim* s, uint32_t n) {
+  Params p = s->params();
+  auto t0 = std::chrono::steady_clock::now();
+  for (uint32_t i = 0; i < n; i++) {
+    if (s->precision == 64) substep<double>(p, s->s64); else substep<float>(p, s->s32);
+  }
+  auto t1 = std::chrono::steady_clock::now();
+  return std::chrono::duration<double>(t1 - t0).count();
+}
+
+// grid: mom4 = 2*NC float4, mass = NC
+void orc_sim_get_grid(orc_sim* s, float* mom4, float* mass) {
+  if (s->precision == 64) {
+    for (size_t i = 0; i < s->s64.grid_mom.size(); i++) mom4[i] = float(s->s64.grid_mom[i]);
+    for (size_t i = 0; i < s->s64.grid_mass.size(); i++) mass[i] = float(s->s64.grid_mass[i]);
+  } else {
+    std::memcpy(mom4, s->s32.grid_mom.data(), s->s32.grid_mom.size() * 4);
+    std::memcpy(mass, s->s32.grid_mass.data(), s->s32.grid_mass.size() * 4);
+  }
+}
+void orc_sim_get_grid_f64(orc_sim* s, double* mom4, double* mass) {
+  std::memcpy(mom4, s->s64.grid_mom.data(), s->s64.grid_mom.size() * 8);
+  std::memcpy(mass, s->s64.grid_mass.data(), s->s64.grid_mass.size() * 8);
+}
+// mom4 = NC float4 (first half only), mass = NC; the second half is zeroed
+void orc_sim_set_grid(orc_sim* s, const float* mom4, const float* mass) {
+  uint32_t NC = s->sc.cube_res * s->sc.cube_res * s->sc.cube_res;
+  if (s->precision == 64) {
+    std::fill(s->s64.grid_mom.begin(), s->s64.grid_mom.end(), 0.0);
+    for (size_t i = 0; i < size_t(NC) * 4; i++) s->s64.grid_mom[i] = mom4[i];
+    for (size_t i = 0; i < NC; i++) s->s64.grid_mass[i] = mass[i];
+  } else {
+    std::fill(s->s32.grid_mom.begin(), s->s32.grid_mom.end(), 0.0f);
+    std::memcpy(s->s32.grid_mom.data(), mom4, size_t(NC) * 16);
+    std::memcpy(s->s32.grid_mass.data(), mass, size_t(NC) * 4);
+  }
+}
+// Set the bins directly (tests inject host-built bins, as MPMHarness::buildHashGridCPU does).
+void orc_sim_set_bins(orc_sim* s, const uint32_t* sorted, const uint32_t* cell_count, const uint32_t* cell_end) {
+  uint32_t NC = s->sc.cube_res * s->sc.cube_res * s->sc.cube_res;
+  if (s->precision == 64) {
+    std::memcpy(s->s64.sorted.data(), sorted, s->s64.n * 4);
+    std::memcpy(s->s64.cell_count.data(), cell_count, NC * 4);
+    std::memcpy(s->s64.cell_end.data(), cell_end, NC * 4);
+  } else {
+    std::memcpy(s->s32.sorted.data(), sorted, s->s32.n * 4);
+    std::memcpy(s->s32.cell_count.data(), cell_count, NC * 4);
+    std::memcpy(s->s32.cell_end.data(), cell_end, NC * 4);
+  }
+}
+
+// ---- scalar kernels, for property tests ----
+uint32_t orc_morton_encode(int x, int y, int z) { return morton_encode(x, y, z); }
+void orc_morton_decode(uint32_t c, int* x, int* y, int* z) { morton_decode(c, *x, *y, *z); }
+uint32_t orc_cell_id(const orc_scene* sc, float x, float y, float z) {
+  Params p{};
+  p.cell_size = sc->cell_size;
+  for (int a = 0; a < 3; a++) {
+    p.grid_res[a] = sc->grid_res[a];
+    p.world_min[a] = sc->world_min[a];
+  }
+  return cell_id_from_pos<float>(p, x, y, z);
+}
+float orc_bspline2_f32(float d) { return bspline2<float>(d); }
+float orc_bspline2g_f32(float d) { return bspline2g<float>(d); }
+double orc_bspline2_f64(double d) { return bspline2<double>(d); }
+double orc_bspline2g_f64(double d) { return bspline2g<double>(d); }
+
+// matrices are column-major 9-arrays
+void orc_svd3_f32(const float* F, float* U, float* sigma, float* V) {
+  M3<float> u, v;
+  V3<float> s;
+  svd3(load_m3(F), u, s, v);
+  store_m3(u, U);
+  store_m3(v, V);
+  sigma[0] = s.x; sigma[1] = s.y; sigma[2] = s.z;
+}
+void orc_svd3_f64(const double* F, double* U, double* sigma, double* V) {
+  M3<double> u, v;
+  V3<double> s;
+  svd3(load_m3(F), u, s, v);
+  store_m3(u, U);
+  store_m3(v, V);
+  sigma[0] = s.x; sigma[1] = s.y; sigma[2] = s.z;
+}
+void orc_hencky_f32(const float* F, float mu, float lam, float* tau) { store_m3(hencky_stress(load_m3(F), mu, lam), tau); }
+void orc_hencky_f64(const double* F, double mu, double lam, double* tau) { store_m3(hencky_stress(load_m3(F), mu, lam), tau); }
+void orc_project_vm_f32(const float* F, float mu, float lam, float qmax, float* out) {
+  store_m3(project_von_mises(load_m3(F), mu, lam, qmax), out);
+}
+void orc_project_vm_f64(const double* F, double mu, double lam, double qmax, double* out) {
+  store_m3(project_von_mises(load_m3(F), mu, lam, qmax), out);
+}
+void orc_project_dp_f32(const float* F, float mu, float lam, float M, float qc, float* out) {
+  store_m3(project_drucker_prager(load_m3(F), mu, lam, M, qc), out);
+}
+void orc_project_dp_f64(const double* F, double mu, double lam, double M, double qc, double* out) {
+  store_m3(project_drucker_prager(load_m3(F), mu, lam, M, qc), out);
+}
+
+}  // extern "C"

@@ -1,0 +1,503 @@
+"""Pins the CPU oracle (CPU-only tests).
+
+Each check is independent of the oracle's own formulas: known-answer tests
+restated from the reference's tests/test_mpm.cpp and tests/test_domain.cpp,
+closed forms, numpy/LAPACK, brute force on tiny inputs, and invariants chosen
+so that a dropped term, a wrong sign or index, or a transposed operand fails.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from nbtest import F_of, harness_scene, oracle_sim, stress_of
+from nebula_amd import mpm
+from nebula_amd.scenes import elastic_block_64
+
+MU, LAM = 5000.0, 11538.0
+
+
+# ---------------------------------------------------------------------------
+# domain (tests/test_domain.cpp:8-49)
+# ---------------------------------------------------------------------------
+def test_domain_grid_res():
+    assert mpm.gridRes((10.0, 10.0, 10.0), 10.0 / 64.0) == (64, 64, 64)
+    assert mpm.gridRes((10.0, 10.0, 10.0), 3.0) == (4, 4, 4)
+    assert mpm.gridRes((40.0, 10.0, 10.0), 5.0) == (8, 2, 2)
+    assert mpm.gridRes((0.01, 0.01, 0.01), 5.0) == (1, 1, 1)
+    assert mpm.gridRes((0.0, 0.0, 0.0), 5.0) == (1, 1, 1)
+
+
+def test_domain_cube_and_cells():
+    assert mpm.mortonCubeRes((64, 64, 64)) == 64
+    assert mpm.mortonCubeRes((5, 5, 5)) == 8
+    assert mpm.mortonCubeRes((40, 10, 10)) == 64
+    assert mpm.mortonCubeRes((1, 1, 1)) == 1
+    assert mpm.hashCells((64, 64, 64)) == 64 ** 3
+    hc = mpm.hashCells((40, 10, 10))
+    assert hc == 64 ** 3 and hc > 40 * 10 * 10
+    assert mpm.nGroups(256) == 1 and mpm.nGroups(257) == 2 and mpm.nGroups(0) == 0
+
+
+# ---------------------------------------------------------------------------
+# Morton keys and cell ids
+# ---------------------------------------------------------------------------
+def _interleave(x, y, z):
+    k = 0
+    for b in range(10):
+        k |= ((x >> b) & 1) << (3 * b) | ((y >> b) & 1) << (3 * b + 1) | ((z >> b) & 1) << (3 * b + 2)
+    return k
+
+
+def test_morton_brute_force_and_variants():
+    rng = np.random.default_rng(0)
+    pts = rng.integers(0, 1024, size=(2000, 3))
+    for x, y, z in pts:
+        k = oracle.morton_encode(int(x), int(y), int(z))
+        assert k == _interleave(int(x), int(y), int(z))
+        assert k == mpm.mortonEncode(int(x), int(y), int(z))  # the host (C++ ^-variant) mirror
+        assert oracle.morton_decode(k) == (x, y, z)
+    # vectorised host helper agrees too
+    k = mpm.mortonEncodeArray(pts[:, 0], pts[:, 1], pts[:, 2])
+    assert all(int(k[i]) == _interleave(*map(int, pts[i])) for i in range(50))
+
+
+def test_cell_id_truncates_clamped_quotient():
+    sc = oracle.make_scene(1e-3, 10.0 / 64.0, (64, 40, 64), (0.0, 0.0, 0.0), (10.0, 6.25, 10.0))
+    rng = np.random.default_rng(1)
+    x = rng.uniform(-2.0, 12.0, size=(3000, 3)).astype(np.float32)
+    h = np.float32(10.0 / 64.0)
+    hi = np.array([63, 39, 63], np.float32)
+    q = np.clip(x / h, np.float32(0), hi).astype(np.uint32)
+    want = mpm.mortonEncodeArray(q[:, 0], q[:, 1], q[:, 2])
+    got = np.array([oracle.cell_id(sc, *map(float, p)) for p in x], np.uint32)
+    assert np.array_equal(got, want)
+    # exact cell boundary: x = k*h lands in cell k
+    assert oracle.cell_id(sc, float(5 * h), 0.0, 0.0) == mpm.mortonEncode(5, 0, 0)
+
+
+def test_binning_is_stable_counting_sort_with_end_offsets():
+    rng = np.random.default_rng(2)
+    n = 5000
+    pos = rng.uniform(-0.1, 1.1, size=(n, 3)).astype(np.float32)  # some outside: clamped
+    pos[:200] = pos[0]  # a crowded cell
+    f = np.zeros((8, n, 4), np.float32)
+    f[0, :, :3] = pos
+    sc = oracle.make_scene(1e-4, 1.0 / 16, (16, 16, 16))
+    m = mpm.presetJelly()
+    sim = oracle.Sim(sc, bytes(m), precision=32)
+    sim.set_state(f)
+    sim.bin()
+    keys, srt, cnt, end = sim.get_bins()
+    h = np.float32(1.0 / 16)
+    q = np.clip(pos / h, 0, 15).astype(np.uint32)
+    want_keys = mpm.mortonEncodeArray(q[:, 0], q[:, 1], q[:, 2])
+    assert np.array_equal(keys, want_keys)
+    assert np.array_equal(srt, np.argsort(want_keys, kind="stable").astype(np.uint32))
+    want_cnt = np.bincount(want_keys, minlength=16 ** 3).astype(np.uint32)
+    assert np.array_equal(cnt, want_cnt)
+    assert np.array_equal(end, np.cumsum(want_cnt).astype(np.uint32))
+
+
+# ---------------------------------------------------------------------------
+# B-splines (tests/test_mpm.cpp:59-132)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("precision", [32, 64])
+def test_bspline_partition_of_unity(precision):
+    h = 1.25
+    for ox in (0.1, 0.3, 0.6, 0.9, 1.1):
+        for oy in (0.1, 0.6, 1.1):
+            for oz in (0.3, 0.9):
+                xp = np.array([(2 + ox) * h, (2 + oy) * h, (2 + oz) * h])
+                base = np.floor(xp / h).astype(int) - 1
+                s = 0.0
+                for k in range(3):
+                    for j in range(3):
+                        for i in range(3):
+                            d = (xp - (base + np.array([i, j, k]) + 0.5) * h) / h
+                            s += (oracle.bspline2(d[0], precision) * oracle.bspline2(d[1], precision)
+                                  * oracle.bspline2(d[2], precision))
+                assert abs(s - 1.0) < 1e-5
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_bspline_gradient_sum_zero_and_linear_completeness(precision):
+    h = 1.25
+    for ox in (0.2, 0.5, 0.8, 1.2):
+        xp = (3 + ox) * h
+        base = math.floor(xp / h) - 1
+        g = 0.0
+        lin = 0.0
+        for i in range(3):
+            xi = (base + i + 0.5) * h
+            d = (xp - xi) / h
+            g += oracle.bspline2g(d, precision)
+            lin += (xi - xp) * oracle.bspline2g(d, precision) / h  # sum (x_i - x_p) dw_i = 1
+        assert abs(g) < 1e-5
+        assert abs(lin - 1.0) < 1e-5
+
+
+def test_bspline_closed_form():
+    for d in np.linspace(-1.7, 1.7, 69):
+        a = abs(d)
+        w = 0.75 - a * a if a < 0.5 else (0.5 * (1.5 - a) ** 2 if a < 1.5 else 0.0)
+        g = -2 * d if a < 0.5 else (-np.sign(d) * (1.5 - a) if a < 1.5 else 0.0)
+        assert abs(oracle.bspline2(d, 64) - w) < 1e-15
+        assert abs(oracle.bspline2g(d, 64) - g) < 1e-15
+
+
+# ---------------------------------------------------------------------------
+# SVD, Hencky, return maps
+# ---------------------------------------------------------------------------
+def test_svd_matches_lapack_and_reference_row_convention():
+    rng = np.random.default_rng(3)
+    for _ in range(200):
+        F = np.eye(3) + 0.3 * rng.standard_normal((3, 3))
+        U, s, V = oracle.svd3(F, 64)
+        ref = np.linalg.svd(F, compute_uv=False)
+        # |sigma| sorted descending equals LAPACK's singular values
+        assert np.allclose(np.abs(s), ref, rtol=1e-9, atol=1e-12)
+        # the GLSL keeps singular vectors in the rows of its mat3:
+        # F = U^T diag(s) V with the matrices as the GLSL stores them
+        assert np.allclose(U.T @ np.diag(s) @ V, F, atol=1e-10)
+        # orthogonal to the Jacobi stopping threshold (off-diagonal of F^T F < 1e-8)
+        assert np.allclose(U @ U.T, np.eye(3), atol=1e-6)
+        assert np.allclose(V @ V.T, np.eye(3), atol=1e-6)
+        assert np.linalg.det(U) > 0
+
+
+def test_svd_fp32_close_to_fp64():
+    rng = np.random.default_rng(4)
+    for _ in range(200):
+        F = (np.eye(3) + 0.2 * rng.standard_normal((3, 3))).astype(np.float32)
+        _, s32, _ = oracle.svd3(F, 32)
+        _, s64, _ = oracle.svd3(F.astype(np.float64), 64)
+        assert np.allclose(s32, s64, rtol=2e-5, atol=2e-6)
+
+
+def test_hencky_identity_is_stress_free():  # tests/test_mpm.cpp:137-144
+    tau = oracle.hencky_stress(np.eye(3), MU, LAM, 32)
+    assert np.abs(tau).max() < 1e-4
+
+
+def test_hencky_uniform_compression():  # tests/test_mpm.cpp:149-168
+    tau = oracle.hencky_stress(0.9 * np.eye(3), MU, LAM, 32)
+    assert (np.diag(tau) < -1).all()
+    assert abs(tau[0, 0] - tau[1, 1]) < 1 and abs(tau[1, 1] - tau[2, 2]) < 1
+    assert abs(tau[0, 1]) < 1e-4 and abs(tau[0, 2]) < 1e-4 and abs(tau[1, 2]) < 1e-4
+    expected = (3 * LAM + 2 * MU) * math.log(0.9)
+    assert abs(tau[0, 0] - expected) < abs(expected) * 0.01
+
+
+def test_hencky_closed_form_diagonal_and_transposed_rotation():
+    # diagonal F: tau = diag(lam tr(eps) + 2 mu eps) exactly (U = V = I)
+    a = np.array([1.1, 0.95, 1.02])
+    tau = oracle.hencky_stress(np.diag(a), MU, LAM, 64)
+    eps = np.log(a)
+    assert np.allclose(np.diag(tau), LAM * eps.sum() + 2 * MU * eps, rtol=1e-12)
+    # F = R S: the reference's row-stored U makes tau = R^T k R (not R k R^T)
+    th = 0.3
+    R = np.array([[math.cos(th), -math.sin(th), 0], [math.sin(th), math.cos(th), 0], [0, 0, 1.0]])
+    S = np.diag([1.1, 1.0, 1.0])
+    tau = oracle.hencky_stress(R @ S, MU, LAM, 64)
+    k = np.diag(LAM * np.log(1.1) + 2 * MU * np.log([1.1, 1.0, 1.0]))
+    assert np.allclose(tau, R.T @ k @ R, atol=1e-8)
+    assert not np.allclose(tau, R @ k @ R.T, atol=1.0)
+
+
+def test_von_mises_projection_invariants():
+    rng = np.random.default_rng(5)
+    q_max = 300.0
+    for _ in range(100):
+        F = np.eye(3) + 0.2 * rng.standard_normal((3, 3))
+        if np.linalg.det(F) < 0.2:
+            continue
+        Fp = oracle.project_von_mises(F, MU, LAM, q_max, 64)
+        s0 = np.linalg.svd(F, compute_uv=False)
+        s1 = np.linalg.svd(Fp, compute_uv=False)
+        e0, e1 = np.log(s0), np.log(s1)
+        # tolerances: the F^T F Jacobi stops at off-diagonals < 1e-8
+        assert abs(e0.sum() - e1.sum()) < 1e-7  # volume preserved
+        dev1 = e1 - e1.mean()
+        assert np.linalg.norm(dev1) <= q_max / (2 * MU) + 1e-7
+        dev0 = e0 - e0.mean()
+        if np.linalg.norm(dev0) <= q_max / (2 * MU):
+            assert np.allclose(Fp, F)  # elastic: untouched
+
+
+def test_drucker_prager_projection_invariants():
+    rng = np.random.default_rng(6)
+    M, qc = 0.577, 50.0
+    K = LAM + 2 * MU / 3
+    for _ in range(200):
+        F = np.eye(3) + 0.15 * rng.standard_normal((3, 3))
+        if np.linalg.det(F) < 0.2:
+            continue
+        Fp = oracle.project_drucker_prager(F, MU, LAM, M, qc, 64)
+        e = np.log(np.linalg.svd(Fp, compute_uv=False))
+        e0 = np.log(np.linalg.svd(F, compute_uv=False))
+        p = -K * e0.sum()
+        if M * p + qc <= 1e-8:  # apex: purely volumetric, tr eps = q_c/(K M)
+            assert np.allclose(e, e.mean(), atol=1e-7)
+            assert abs(e.sum() - qc / (K * M)) < 1e-7
+        else:
+            assert abs(e.sum() - e0.sum()) < 1e-7
+            assert np.linalg.norm(e - e.mean()) <= (M * p + qc) / (2 * MU) + 1e-7
+
+
+# ---------------------------------------------------------------------------
+# per-pass KATs of tests/test_mpm.cpp, through the oracle
+# ---------------------------------------------------------------------------
+def test_p2g_mass_and_momentum_conservation():  # 2-1
+    sc = harness_scene(8, 10.0, [dict(pos=(5.0, 5.0, 5.0), Vp=1.5, vel=(2.0, -1.0, 0.0))], 0.001, 1.0, MU, LAM, 0.0)
+    sim = oracle_sim(sc)
+    sim.bin()
+    sim.p2g()
+    mom, mass = sim.get_grid()
+    assert abs(mass.sum() - 1.5) < 1e-3
+    assert np.allclose(mom[: sim.NC, :3].sum(0), (3.0, -1.5, 0.0), atol=1e-2)
+
+
+def test_grid_update_gravity_and_wall():  # 3-1, 3-2
+    sc = harness_scene(8, 10.0, [], 0.01, 1000.0, MU, LAM, -9.8)
+    sim = oracle_sim(sc)
+    NC = sim.NC
+    mom = np.zeros((NC, 4), np.float32)
+    mass = np.zeros(NC, np.float32)
+    t = mpm.mortonEncode(4, 4, 4)
+    mass[t] = 1.0
+    sim.set_grid(mom, mass)
+    sim.grid_update()
+    m2, _ = sim.get_grid()
+    assert np.allclose(m2[t, :3], (0.0, -0.098, 0.0), atol=1e-4)
+
+    sc = harness_scene(8, 10.0, [], 0.01, 1000.0, MU, LAM, 0.0)
+    sim = oracle_sim(sc)
+    w = mpm.mortonEncode(1, 4, 4)
+    mom[:] = 0
+    mass[:] = 0
+    mass[w] = 1.0
+    mom[w] = (-5.0, 2.0, 0.0, 0.0)
+    sim.set_grid(mom, mass)
+    sim.grid_update()
+    m2, _ = sim.get_grid()
+    assert np.allclose(m2[w, :3], (0.0, 2.0, 0.0), atol=1e-4)
+
+
+def _stencil_grid(NC, xp, h, vel_fn):
+    mom = np.zeros((NC, 4), np.float32)
+    mass = np.zeros(NC, np.float32)
+    base = np.floor(np.asarray(xp) / h).astype(int) - 1
+    for k in range(3):
+        for j in range(3):
+            for i in range(3):
+                n = base + (i, j, k)
+                m = mpm.mortonEncode(*map(int, n))
+                mass[m] = 1.0
+                mom[m, :3] = vel_fn((n + 0.5) * h)
+    return mom, mass
+
+
+def test_g2p_uniform_field():  # 4-1
+    sc = harness_scene(8, 10.0, [dict(pos=(5.1, 5.1, 5.1), Vp=1.0)], 0.001, 1000.0, MU, LAM, 0.0)
+    sim = oracle_sim(sc)
+    mom, mass = _stencil_grid(sim.NC, (5.1, 5.1, 5.1), 1.25, lambda x: (1.5, 3.0, 0.0))
+    sim.set_grid(mom, mass)
+    sim.g2p()
+    st = sim.get_state()
+    assert np.allclose(st[1, 0, :3], (1.5, 3.0, 0.0), atol=1e-3)
+
+
+def test_g2p_velocity_gradient_updates_F():  # 7-2
+    A, dt = 10.0, 0.01
+    sc = harness_scene(8, 10.0, [dict(pos=(5.0, 5.0, 5.0), Vp=1.0)], dt, 1000.0, MU, LAM, 0.0)
+    sim = oracle_sim(sc)
+    mom, mass = _stencil_grid(sim.NC, (5.0, 5.0, 5.0), 1.25, lambda x: (A * (x[0] - 5.0), 0.0, 0.0))
+    sim.set_grid(mom, mass)
+    sim.g2p()
+    st = sim.get_state()
+    F = F_of(st, 0)
+    assert abs(F[0, 0] - (1 + A * dt)) < 2e-3
+    assert abs(F[1, 1] - 1) < 1e-4 and abs(F[2, 2] - 1) < 1e-4
+    assert abs(F[0, 1]) < 1e-4 and abs(F[1, 0]) < 1e-4 and abs(F[0, 2]) < 1e-4 and abs(F[2, 0]) < 1e-4
+    tau = stress_of(st, 0)
+    assert tau[0, 0] > 0 and tau[1, 1] > 0 and tau[0, 0] > tau[1, 1] + 100 and abs(tau[0, 1]) < 1
+
+
+def test_full_step_gravity_one_step():  # 7-1
+    h = 1.25
+    dt, g = 0.01, -9.8
+    sc = harness_scene(8, 10.0, [dict(pos=(5.0, 5.0, 5.0), Vp=h ** 3)], dt, 1000.0, MU, LAM, g)
+    sim = oracle_sim(sc)
+    sim.substeps(1)
+    st = sim.get_state()
+    assert np.allclose(st[1, 0, :3], (0.0, g * dt, 0.0), atol=1e-4)
+    assert np.allclose(st[0, 0, :3], (5.0, 5.0 + dt * g * dt, 5.0), atol=1e-4)
+
+
+def test_full_step_single_particle_constant_velocity():  # 6-2
+    sc = harness_scene(8, 10.0, [dict(pos=(4.2, 5.0, 5.0), Vp=1.0, vel=(1.0, 0.0, 0.0))], 0.01, 1.0, MU, LAM, 0.0)
+    sim = oracle_sim(sc)
+    for _ in range(20):
+        sim.substeps(1)
+        st = sim.get_state()
+        assert abs(st[1, 0, 0] - 1.0) < 1e-3 and abs(st[1, 0, 1]) < 1e-3
+    assert abs(st[0, 0, 0] - 4.4) < 0.05
+
+
+def test_full_step_momentum_conservation():  # 7-3
+    ps = [dict(pos=(4.0, 5.0, 5.0), vel=(2.0, 0, 0)), dict(pos=(6.0, 5.0, 5.0), vel=(-2.0, 0, 0)),
+          dict(pos=(5.0, 4.0, 5.0), vel=(0, 2.0, 0)), dict(pos=(5.0, 6.0, 5.0), vel=(0, -2.0, 0))]
+    for p in ps:
+        p["Vp"] = 0.3
+    sc = harness_scene(8, 10.0, ps, 0.01, 1.0, MU, LAM, 0.0)
+    sim = oracle_sim(sc)
+    sim.substeps(1)
+    st = sim.get_state()
+    assert np.abs((0.3 * st[1, :, :3]).sum(0)).max() < 1e-3
+
+
+def test_full_step_no_nan_in_bounds():  # 6-1
+    sp = 0.3
+    ps = []
+    for iz in range(2):
+        for iy in range(2):
+            for ix in range(2):
+                ps.append(dict(pos=(5 + (ix - 0.5) * sp, 7 + (iy - 0.5) * sp, 5 + (iz - 0.5) * sp), Vp=sp ** 3))
+    sc = harness_scene(16, 10.0, ps, 1.0 / 240.0, 1000.0, 1e4, 2.3e4, -9.8)
+    sim = oracle_sim(sc)
+    sim.substeps(50)
+    st = sim.get_state()
+    lo, hi = 0.625 * 1.5, 10 - 0.625 * 1.5
+    assert np.isfinite(st[:2]).all()
+    assert (st[0, :, :3] >= lo).all() and (st[0, :, :3] <= hi).all()
+
+
+def test_full_step_wall_clamp():  # 7-4
+    h = 1.25
+    sc = harness_scene(8, 10.0, [dict(pos=(2.5, 5.0, 5.0), Vp=h ** 3, vel=(-5.0, 0.0, 0.0))], 0.01, 1000.0, 1e4,
+                       2.3e4, 0.0)
+    sim = oracle_sim(sc)
+    for _ in range(30):
+        sim.substeps(1)
+        assert sim.get_state()[0, 0, 0] >= 1.5 * h - 1e-4
+    v = sim.get_state()[1, 0, :3]
+    assert v[0] >= -0.5 and np.isfinite(v).all()
+
+
+def test_rectangular_domain_short_axis_clamp():  # 8-1
+    from nebula_amd.mpm import MPMConfig
+    from nebula_amd.scenes import Scene
+
+    cfg = MPMConfig(nx=0, ny=0, nz=0, domainSize=(20.0, 5.0, 20.0), cellSize=1.25, maxParticles=4)
+    Vp = 1.25 ** 3
+    f = np.zeros((8, 1, 4), np.float32)
+    f[0, 0] = (10.0, 2.5, 10.0, Vp)
+    f[1, 0] = (0.0, 50.0, 0.0, 0.0)
+    f[2, 0, 0] = f[3, 0, 1] = f[4, 0, 2] = 1.0
+    jelly = mpm.presetJelly(cfg.E, cfg.nu, cfg.rho0)
+    sc = Scene("rect", cfg, [jelly], f, dt=(1.0 / 60.0) / 20, flip_ratio=0.0, substeps=600)
+    sim = oracle_sim(sc)
+    sim.substeps(30 * 20)
+    p = sim.get_state()[0, 0]
+    assert np.isfinite(p[1]) and p[1] <= 5.0 and p[1] > 1.5
+    assert abs(p[0] - 10.0) < 2.0 and abs(p[2] - 10.0) < 2.0
+
+
+# ---------------------------------------------------------------------------
+# brute force and closed forms for P2G / G2P
+# ---------------------------------------------------------------------------
+def _brute_p2g(fields, h, res, dt, rho0, apic):
+    """Direct particle->node scatter in float64 (the math of mpm_p2g.comp)."""
+    NCc = mpm.mortonCubeRes((res, res, res)) ** 3
+    mass = np.zeros(NCc)
+    mom = np.zeros((NCc, 3))
+    for q in range(fields.shape[1]):
+        xp = fields[0, q, :3].astype(np.float64)
+        Vp = float(fields[0, q, 3])
+        v = fields[1, q, :3].astype(np.float64)
+        B = np.stack([fields[5, q, :3], fields[6, q, :3], fields[7, q, :3]], axis=1).astype(np.float64)
+        tau = stress_of(fields, q).astype(np.float64)
+        c = np.clip(np.floor(xp / h), 0, res - 1).astype(int)
+        for dz in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dx in (-1, 0, 1):
+                    n = c + (dx, dy, dz)
+                    if (n < 0).any() or (n >= res).any():
+                        continue
+                    xi = (n + 0.5) * h
+                    d = (xp - xi) / h
+                    w1 = [oracle.bspline2(t, 64) for t in d]
+                    g1 = [oracle.bspline2g(t, 64) for t in d]
+                    w = w1[0] * w1[1] * w1[2]
+                    if w < 1e-12:
+                        continue
+                    dw = np.array([g1[0] * w1[1] * w1[2], w1[0] * g1[1] * w1[2], w1[0] * w1[1] * g1[2]]) / h
+                    mp = rho0 * Vp
+                    veff = v + (4 / h ** 2) * (B @ (xi - xp)) if apic else v
+                    k = mpm.mortonEncode(*map(int, n))
+                    mass[k] += w * mp
+                    mom[k] += w * mp * veff - dt * Vp * (tau @ dw)
+    return mom, mass
+
+
+@pytest.mark.parametrize("apic", [False, True])
+def test_p2g_matches_brute_force_scatter(apic):
+    rng = np.random.default_rng(7)
+    n, res, h, dt, rho0 = 48, 8, 0.125, 1e-3, 3.0
+    f = np.zeros((8, n, 4), np.float32)
+    f[0, :, :3] = rng.uniform(0.2, 0.8, (n, 3))
+    f[0, :, 3] = rng.uniform(0.5, 1.5, n) * h ** 3
+    f[1, :, :3] = rng.normal(0, 1, (n, 3))
+    for c in range(3):
+        f[2 + c, :, c] = 1.0
+        f[5 + c, :, :3] = rng.normal(0, 0.1, (n, 3))
+    f[2:8, :, 3] = rng.normal(0, 100, (6, n))  # stress w lanes
+    m = mpm.presetJelly(rho0=rho0)
+    sc = oracle.make_scene(dt, h, (res, res, res), flip_ratio=-1.0 if apic else 0.0)
+    sim = oracle.Sim(sc, bytes(m), precision=64)
+    sim.set_state(f)
+    sim.bin()
+    sim.p2g()
+    mom, mass = sim.get_grid_f64()
+    bm, bmass = _brute_p2g(f, h, res, float(np.float32(dt)), rho0, apic)  # the scene carries dt as float32
+    assert np.allclose(mass, bmass, rtol=1e-12, atol=1e-15)
+    assert np.allclose(mom[: sim.NC, :3], bm, rtol=1e-10, atol=1e-10)
+
+
+def test_apic_transfers_reproduce_affine_field():
+    """G2P of an affine grid field v(x) = a + A x gives B with 4/h^2 B = A (the
+    quadratic B-spline's D^-1), and P2G of (v_p, B) reproduces a + A x_i at the
+    nodes: the APIC pair is exact on affine fields (closed form)."""
+    h, res = 0.125, 8
+    a = np.array([0.3, -0.2, 0.1])
+    A = np.array([[0.5, -1.0, 0.2], [0.3, 0.1, -0.4], [0.0, 0.7, -0.2]])
+    xp = np.array([0.52, 0.47, 0.55])
+    sc = oracle.make_scene(0.0, h, (res, res, res), flip_ratio=-1.0)
+    m = mpm.presetJelly(rho0=1.0)
+    sim = oracle.Sim(sc, bytes(m), precision=64)
+    f = np.zeros((8, 1, 4), np.float32)
+    f[0, 0] = (*xp, h ** 3)
+    f[2, 0, 0] = f[3, 0, 1] = f[4, 0, 2] = 1.0
+    sim.set_state(f)
+    mom, mass = _stencil_grid(sim.NC, xp, h, lambda x: a + A @ x)
+    sim.set_grid(mom, mass)
+    sim.g2p()
+    st = sim.get_state_f64()
+    vp = st[1, 0, :3]
+    B = np.stack([st[5, 0, :3], st[6, 0, :3], st[7, 0, :3]], axis=1)
+    assert np.allclose(vp, a + A @ xp, atol=2e-6)
+    assert np.allclose(4 / h ** 2 * B, A, atol=2e-5)
+
+
+def test_fp32_oracle_tracks_fp64_twin():
+    sc = elastic_block_64(res=16, ppc=8)
+    s32 = oracle_sim(sc, 32)
+    s64 = oracle_sim(sc, 64)
+    s32.substeps(20)
+    s64.substeps(20)
+    a, b = s32.get_state(), s64.get_state_f64()
+    h = 1.0 / 16
+    assert np.abs(a[0, :, :3] - b[0, :, :3]).max() < 1e-4 * h
+    assert np.abs(a[1, :, :3] - b[1, :, :3]).max() < 1e-4
