@@ -142,6 +142,14 @@ typedef struct nb_mpm_knobs {
   float M_friction;
   float q_cohesion;
   float flip_ratio;      /* 0 = PIC, < -0.5 = APIC, > 0.01 = FLIP blend */
+  /* 0 (default): the reference's SVD convention.  Its F^T F Jacobi leaves the
+   * singular vectors in the rows of the GLSL mat3 while the Hencky stress and
+   * the return-map recomposition use them as columns (shaders/mpm_common.glsl:
+   * 250-293, shaders/mpm_g2p.comp:35,67), i.e. tau = U^T k U and
+   * F <- U^T exp(eps) V with U, V the true singular-vector matrices.
+   * 1: singular vectors used as columns (tau = U k U^T, F <- U exp(eps) V^T);
+   * an extension with no counterpart in the reference. */
+  uint32_t corrected_svd;
 } nb_mpm_knobs;
 
 typedef struct nb_mpm nb_mpm; /* opaque engine handle */

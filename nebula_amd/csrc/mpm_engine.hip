@@ -98,6 +98,7 @@ struct nb_mpm {
     p.collider_count = collider_count;
     p.force_count = force_count;
     p.write_old = write_old ? 1u : 0u;
+    p.corrected_svd = knobs.corrected_svd ? 1u : 0u;
     return p;
   }
 };
@@ -184,6 +185,7 @@ void nb_mpm_knobs_default(nb_mpm_knobs* k) {
   k->M_friction = 0.577f;
   k->q_cohesion = 0.0f;
   k->flip_ratio = 0.0f;
+  k->corrected_svd = 0;
 }
 
 int nb_mpm_create(const nb_mpm_config* cfg, void* hip_stream, nb_mpm** out) {

@@ -673,6 +673,7 @@ __global__ __launch_bounds__(256) void k_g2p(SubParams p, float4* __restrict__ P
   m3 Fn = m3_mul(L, F);
 
   m3 tau;
+  const bool cs = p.corrected_svd != 0u;
   if (mat.model == NB_MAT_FLUID) {
     float J = fmaxf(m3_det(Fn), 1e-4f);
     const float cr = powf(J, 1.f / 3.f);
@@ -680,14 +681,14 @@ __global__ __launch_bounds__(256) void k_g2p(SubParams p, float4* __restrict__ P
     const float pr = mat.bulkK * (J - 1.f);
     tau = m3_diag(pr, pr, pr);
   } else if (mat.model == NB_MAT_VON_MISES) {
-    Fn = project_von_mises(Fn, mat.mu, mat.q_max);
-    tau = hencky_stress(Fn, mat.mu, mat.lambda);
+    Fn = project_von_mises(Fn, mat.mu, mat.q_max, cs);
+    tau = hencky_stress(Fn, mat.mu, mat.lambda, cs);
   } else if (mat.model == NB_MAT_DRUCKER_PRAGER || mat.model == NB_MAT_GRANULAR_POWDER) {
-    Fn = project_drucker_prager(Fn, mat.mu, mat.lambda, mat.M_friction, mat.q_cohesion);
-    tau = hencky_stress(Fn, mat.mu, mat.lambda);
+    Fn = project_drucker_prager(Fn, mat.mu, mat.lambda, mat.M_friction, mat.q_cohesion, cs);
+    tau = hencky_stress(Fn, mat.mu, mat.lambda, cs);
   } else if (mat.model == NB_MAT_VISCOPLASTIC_MUD) {
-    Fn = project_drucker_prager(Fn, mat.mu, mat.lambda, mat.M_friction, mat.q_cohesion);
-    tau = hencky_stress(Fn, mat.mu, mat.lambda);
+    Fn = project_drucker_prager(Fn, mat.mu, mat.lambda, mat.M_friction, mat.q_cohesion, cs);
+    tau = hencky_stress(Fn, mat.mu, mat.lambda, cs);
     m3 D;
 #pragma unroll
     for (int c = 0; c < 3; c++)
@@ -700,7 +701,7 @@ __global__ __launch_bounds__(256) void k_g2p(SubParams p, float4* __restrict__ P
 #pragma unroll
       for (int r = 0; r < 3; r++) tau.m[c][r] += k2 * (D.m[c][r] - (c == r ? trD / 3.f : 0.f));
   } else {
-    tau = hencky_stress(Fn, mat.mu, mat.lambda);
+    tau = hencky_stress(Fn, mat.mu, mat.lambda, cs);
   }
 
   bool jreset = false;

@@ -158,7 +158,10 @@ __device__ __forceinline__ void jacobi_eigen3(float a00, float a11, float a22, f
 }
 
 // ---- SVD via F^T F (shaders/mpm_common.glsl:250-281) ----
-__device__ __forceinline__ void svd3(const m3& F, m3& U, float sig[3], m3& V) {
+// corrected = false keeps the GLSL storage (singular vectors in the rows of
+// U, V as the GLSL mat3 holds them); corrected = true transposes both so the
+// downstream U diag U^T / U diag V^T are the true spectral forms.
+__device__ __forceinline__ void svd3(const m3& F, m3& U, float sig[3], m3& V, bool corrected) {
   // FtF = transpose(F) * F: FtF[c][k] = dot(F col k, F col c)
   float f00 = F.m[0][0] * F.m[0][0] + F.m[0][1] * F.m[0][1] + F.m[0][2] * F.m[0][2];
   float f11 = F.m[1][0] * F.m[1][0] + F.m[1][1] * F.m[1][1] + F.m[1][2] * F.m[1][2];
@@ -204,6 +207,10 @@ __device__ __forceinline__ void svd3(const m3& F, m3& U, float sig[3], m3& V) {
     U.m[2][2] = -U.m[2][2];
     sig[2] = -sig[2];
   }
+  if (corrected) {
+    U = m3_transpose(U);
+    V = m3_transpose(V);
+  }
 }
 
 
@@ -216,18 +223,18 @@ __device__ __forceinline__ m3 hencky_from_svd(const m3& U, const float sig[3], f
   float lt = lam * tr, tm = 2.f * mu;
   return m3_udvt(U, lt + tm * e0, lt + tm * e1, lt + tm * e2, U);
 }
-__device__ __forceinline__ m3 hencky_stress(const m3& F, float mu, float lam) {
+__device__ __forceinline__ m3 hencky_stress(const m3& F, float mu, float lam, bool corrected) {
   m3 U, V;
   float sig[3];
-  svd3(F, U, sig, V);
+  svd3(F, U, sig, V, corrected);
   return hencky_from_svd(U, sig, mu, lam);
 }
 
 // ---- Von Mises radial return (shaders/mpm_g2p.comp:19-38) ----
-__device__ __forceinline__ m3 project_von_mises(const m3& F, float mu, float q_max) {
+__device__ __forceinline__ m3 project_von_mises(const m3& F, float mu, float q_max, bool corrected) {
   m3 U, V;
   float sig[3];
-  svd3(F, U, sig, V);
+  svd3(F, U, sig, V, corrected);
   float e0 = logf(fmaxf(fabsf(sig[0]), 1e-6f));
   float e1 = logf(fmaxf(fabsf(sig[1]), 1e-6f));
   float e2 = logf(fmaxf(fabsf(sig[2]), 1e-6f));
@@ -247,10 +254,11 @@ __device__ __forceinline__ m3 project_von_mises(const m3& F, float mu, float q_m
 }
 
 // ---- Drucker-Prager return map (shaders/mpm_g2p.comp:41-68): always recomposes ----
-__device__ __forceinline__ m3 project_drucker_prager(const m3& F, float mu, float lam, float M, float q_c) {
+__device__ __forceinline__ m3 project_drucker_prager(const m3& F, float mu, float lam, float M, float q_c,
+                                                    bool corrected) {
   m3 U, V;
   float sig[3];
-  svd3(F, U, sig, V);
+  svd3(F, U, sig, V, corrected);
   float e0 = logf(fmaxf(fabsf(sig[0]), 1e-6f));
   float e1 = logf(fmaxf(fabsf(sig[1]), 1e-6f));
   float e2 = logf(fmaxf(fabsf(sig[2]), 1e-6f));

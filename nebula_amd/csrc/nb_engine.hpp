@@ -45,6 +45,7 @@ struct SubParams {
   uint32_t collider_count;
   uint32_t force_count;
   uint32_t write_old;  // write the pre-force velocity (FLIP, or pass-level export)
+  uint32_t corrected_svd;  // 1: singular vectors as columns (see nb_mpm_knobs)
 };
 
 struct DevState {

@@ -136,7 +136,7 @@ class _Knobs(C.Structure):
     _fields_ = [
         ("restitution", C.c_float), ("wall_friction", C.c_float), ("num_substeps", C.c_int32),
         ("plastic_model", C.c_uint32), ("q_max", C.c_float), ("M_friction", C.c_float),
-        ("q_cohesion", C.c_float), ("flip_ratio", C.c_float),
+        ("q_cohesion", C.c_float), ("flip_ratio", C.c_float), ("corrected_svd", C.c_uint32),
     ]
 
 
@@ -511,6 +511,9 @@ class MPMEngine:
     q_max = property(lambda s: s._knobs().q_max, lambda s, v: s._set_knob("q_max", float(v)))
     M_friction = property(lambda s: s._knobs().M_friction, lambda s, v: s._set_knob("M_friction", float(v)))
     q_cohesion = property(lambda s: s._knobs().q_cohesion, lambda s, v: s._set_knob("q_cohesion", float(v)))
+    # extension: singular vectors used as columns (see include/nebula_mpm.h)
+    corrected_svd = property(lambda s: bool(s._knobs().corrected_svd),
+                             lambda s, v: s._set_knob("corrected_svd", 1 if v else 0))
 
     # queries -------------------------------------------------------------
     def config(self) -> MPMConfig:

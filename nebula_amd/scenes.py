@@ -49,6 +49,7 @@ class Scene:
     sdf: Optional[np.ndarray] = None
     seed: int = 0
     collider_velocity: Optional[tuple] = None  # moving analytic collider (index 0)
+    corrected_svd: bool = False  # singular vectors as columns (an extension; see include/nebula_mpm.h)
 
     @property
     def n(self):
@@ -235,5 +236,6 @@ def make_engine(scene: Scene, stream=None):
         eng.setColliderSDF(scene.sdf)
     eng.setState(scene.fields)
     eng.flip_ratio = scene.flip_ratio
+    eng.corrected_svd = scene.corrected_svd
     eng.numSubsteps = 1
     return eng

@@ -42,6 +42,7 @@ class OrcScene(C.Structure):
         ("collider_count", C.c_uint32),
         ("force_count", C.c_uint32),
         ("has_sdf", C.c_uint32),
+        ("corrected_svd", C.c_uint32),
     ]
 
 
@@ -67,6 +68,7 @@ def lib():
         L.orc_sim_destroy.argtypes = [vp]
         L.orc_sim_set_dt.argtypes = [vp, C.c_float]
         L.orc_sim_set_flip_ratio.argtypes = [vp, C.c_float]
+        L.orc_sim_set_corrected_svd.argtypes = [vp, C.c_int]
         L.orc_sim_set_colliders.argtypes = [vp, vp, C.c_uint32]
         L.orc_sim_set_state.argtypes = [vp, fp, C.c_uint32]
         L.orc_sim_get_state.argtypes = [vp, fp]
@@ -171,6 +173,9 @@ class Sim:
 
     def set_flip_ratio(self, r):
         lib().orc_sim_set_flip_ratio(self.h, r)
+
+    def set_corrected_svd(self, on: bool):
+        lib().orc_sim_set_corrected_svd(self.h, 1 if on else 0)
 
     def set_colliders(self, colliders: bytes):
         buf = C.create_string_buffer(colliders, max(len(colliders), 1))

@@ -175,6 +175,7 @@ struct Params {
   const Force* forces;
   uint32_t force_count;
   const float* sdf;  // dense Morton SDF (NC floats) or null
+  bool corrected_svd;  // see svd3
 };
 
 inline uint32_t total_cells(const Params& p) { return p.cube_res * p.cube_res * p.cube_res; }
@@ -284,7 +285,10 @@ template <class R> inline void jacobi_eigen3(M3<R> A, V3<R>& D, M3<R>& V) {
   D = {A.at(0, 0), A.at(1, 1), A.at(2, 2)};
 }
 
-template <class R> inline void svd3(const M3<R>& F, M3<R>& U, V3<R>& sigma, M3<R>& V) {
+// corrected = false: the GLSL result as stored (singular vectors in the rows of
+// U and V).  corrected = true: both transposed, so that U diag U^T and
+// U diag V^T are the true spectral forms (an extension, not reference semantics).
+template <class R> inline void svd3(const M3<R>& F, M3<R>& U, V3<R>& sigma, M3<R>& V, bool corrected = false) {
   M3<R> FtF = transpose(F) * F;
   V3<R> sig2;
   jacobi_eigen3(FtF, sig2, V);
@@ -314,13 +318,17 @@ template <class R> inline void svd3(const M3<R>& F, M3<R>& U, V3<R>& sigma, M3<R
     U.at(2, 2) = -U.at(2, 2);
     sigma[2] = -sigma[2];
   }
+  if (corrected) {
+    U = transpose(U);
+    V = transpose(V);
+  }
 }
 
 // Hencky Kirchhoff stress — shaders/mpm_common.glsl:284-293
-template <class R> inline M3<R> hencky_stress(const M3<R>& F, R mu, R lam) {
+template <class R> inline M3<R> hencky_stress(const M3<R>& F, R mu, R lam, bool corrected = false) {
   M3<R> U, V;
   V3<R> s;
-  svd3(F, U, s, V);
+  svd3(F, U, s, V, corrected);
   s = {std::max(std::fabs(s.x), R(1e-6)), std::max(std::fabs(s.y), R(1e-6)), std::max(std::fabs(s.z), R(1e-6))};
   V3<R> eps(std::log(s.x), std::log(s.y), std::log(s.z));
   R tr = eps.x + eps.y + eps.z;
@@ -329,11 +337,11 @@ template <class R> inline M3<R> hencky_stress(const M3<R>& F, R mu, R lam) {
 }
 
 // Von Mises projection — shaders/mpm_g2p.comp:19-38
-template <class R> inline M3<R> project_von_mises(M3<R> F, R mu, R lam, R q_max) {
+template <class R> inline M3<R> project_von_mises(M3<R> F, R mu, R lam, R q_max, bool corrected = false) {
   (void)lam;
   M3<R> U, V;
   V3<R> s;
-  svd3(F, U, s, V);
+  svd3(F, U, s, V, corrected);
   s = {std::max(std::fabs(s.x), R(1e-6)), std::max(std::fabs(s.y), R(1e-6)), std::max(std::fabs(s.z), R(1e-6))};
   V3<R> eps(std::log(s.x), std::log(s.y), std::log(s.z));
   R tr = eps.x + eps.y + eps.z;
@@ -351,10 +359,11 @@ template <class R> inline M3<R> project_von_mises(M3<R> F, R mu, R lam, R q_max)
 }
 
 // Drucker-Prager projection — shaders/mpm_g2p.comp:41-68
-template <class R> inline M3<R> project_drucker_prager(const M3<R>& F, R mu, R lam, R M, R q_c) {
+template <class R> inline M3<R> project_drucker_prager(const M3<R>& F, R mu, R lam, R M, R q_c,
+                                                      bool corrected = false) {
   M3<R> U, V;
   V3<R> s;
-  svd3(F, U, s, V);
+  svd3(F, U, s, V, corrected);
   s = {std::max(std::fabs(s.x), R(1e-6)), std::max(std::fabs(s.y), R(1e-6)), std::max(std::fabs(s.z), R(1e-6))};
   V3<R> eps(std::log(s.x), std::log(s.y), std::log(s.z));
   R tr = eps.x + eps.y + eps.z;
@@ -709,6 +718,7 @@ template <class R> void g2p(const Params& p, State<R>& s) {
         }
     V3<R> vold(s.f[1][4 * q + 0], s.f[1][4 * q + 1], s.f[1][4 * q + 2]);
     const Material& mat = material_of<R>(p, s.matid[q]);
+    const bool cs = p.corrected_svd;
     R mu = R(mat.mu), lam = R(mat.lambda);
     M3<R> F = s.read_mat(2, q);
     M3<R> Fn = (M3<R>::ident() + dt * vgrad) * F;
@@ -719,20 +729,20 @@ template <class R> void g2p(const Params& p, State<R>& s) {
       Fn = std::pow(J, R(1) / R(3)) * M3<R>::ident();
       tau = (R(mat.bulkK) * (J - R(1))) * M3<R>::ident();
     } else if (mat.model == 1u) {  // VON_MISES
-      Fn = project_von_mises(Fn, mu, lam, R(mat.q_max));
-      tau = hencky_stress(Fn, mu, lam);
+      Fn = project_von_mises(Fn, mu, lam, R(mat.q_max), cs);
+      tau = hencky_stress(Fn, mu, lam, cs);
     } else if (mat.model == 2u || mat.model == 3u) {  // DRUCKER_PRAGER / GRANULAR_POWDER
-      Fn = project_drucker_prager(Fn, mu, lam, R(mat.M_friction), R(mat.q_cohesion));
-      tau = hencky_stress(Fn, mu, lam);
+      Fn = project_drucker_prager(Fn, mu, lam, R(mat.M_friction), R(mat.q_cohesion), cs);
+      tau = hencky_stress(Fn, mu, lam, cs);
     } else if (mat.model == 5u) {  // VISCOPLASTIC_MUD
-      Fn = project_drucker_prager(Fn, mu, lam, R(mat.M_friction), R(mat.q_cohesion));
-      tau = hencky_stress(Fn, mu, lam);
+      Fn = project_drucker_prager(Fn, mu, lam, R(mat.M_friction), R(mat.q_cohesion), cs);
+      tau = hencky_stress(Fn, mu, lam, cs);
       M3<R> D = R(0.5) * (vgrad + transpose(vgrad));
       R trD = D.at(0, 0) + D.at(1, 1) + D.at(2, 2);
       M3<R> devD = D - (trD / R(3)) * M3<R>::ident();
       tau = tau + (R(2) * R(mat.viscosity)) * devD;
     } else {
-      tau = hencky_stress(Fn, mu, lam);
+      tau = hencky_stress(Fn, mu, lam, cs);
     }
     bool jreset = false;
     {

@@ -38,6 +38,7 @@ struct orc_scene {
   uint32_t collider_count;
   uint32_t force_count;
   uint32_t has_sdf;
+  uint32_t corrected_svd;
 };
 
 struct orc_sim {
@@ -68,6 +69,7 @@ struct orc_sim {
     p.forces = forces.data();
     p.force_count = uint32_t(forces.size());
     p.sdf = sc.has_sdf ? sdf.data() : nullptr;
+    p.corrected_svd = sc.corrected_svd != 0;
     return p;
   }
 };
@@ -92,6 +94,7 @@ void orc_sim_destroy(orc_sim* s) { delete s; }
 
 void orc_sim_set_dt(orc_sim* s, float dt) { s->sc.dt = dt; }
 void orc_sim_set_flip_ratio(orc_sim* s, float r) { s->sc.flip_ratio = r; }
+void orc_sim_set_corrected_svd(orc_sim* s, int on) { s->sc.corrected_svd = on ? 1u : 0u; }
 void orc_sim_set_colliders(orc_sim* s, const void* cols, uint32_t n) {
   s->cols.resize(n);
   if (n) std::memcpy(s->cols.data(), cols, sizeof(Collider) * n);

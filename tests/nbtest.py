@@ -18,6 +18,7 @@ def oracle_sim(scene: Scene, precision=32, t=0.0):
                            scene.flip_ratio)
     sim = oracle.Sim(sc, materials_bytes(scene.materials), scene.colliders_at(t).to_bytes(),
                      forces_bytes(scene.forces), scene.sdf, precision)
+    sim.set_corrected_svd(scene.corrected_svd)
     sim.set_state(scene.fields)
     return sim
 

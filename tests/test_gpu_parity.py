@@ -305,14 +305,11 @@ def test_grid_update_matches_oracle_with_colliders_sdf_forces(gpu_lib):
     assert np.array_equal(gms, oms2)
 
 
-@pytest.mark.parametrize("model", ["elastic", "von_mises", "drucker_prager", "fluid", "mud"])
-@pytest.mark.parametrize("mode", [-1.0, 0.0, 0.95])
-def test_g2p_matches_oracle(gpu_lib, model, mode):
+def _g2p_case(model, mode, corrected):
     from nebula_amd.scenes import terrain_params, terrain_sdf
 
     sc = elastic_block_64()
     m = sc.materials[0]
-    E, nu = 1e4, 0.2
     if model == "von_mises":
         m.model, m.q_max = MaterialModel.VON_MISES, 30.0
     elif model == "drucker_prager":
@@ -322,71 +319,151 @@ def test_g2p_matches_oracle(gpu_lib, model, mode):
     elif model == "mud":
         m.model, m.M_friction, m.q_cohesion, m.viscosity = MaterialModel.VISCOPLASTIC_MUD, 0.4, 5.0, 0.1
     sc.flip_ratio = mode
+    sc.corrected_svd = corrected
     sc.sdf = terrain_sdf(64, terrain_params(3)) + np.float32(0.3)  # surface crosses the block
-    st = _evolved(elastic_block_64(), 30)
-    sim = oracle_sim(sc)
-    sim.set_state(st)
-    sim.bin()
-    sim.p2g()
-    sim.grid_update()
-    sim.sdf_bc()
-    om, oms = sim.get_grid()
-    NC = sim.NC
+    src = elastic_block_64()
+    src.corrected_svd = True  # a smooth, non-trivial starting state (F, tau, B, v)
+    return sc, _evolved(src, 30)
+
+
+def _g2p_all(sc, st):
+    """The same G2P input (the FP32 oracle's updated grid) through the GPU, the
+    FP32 oracle and the FP64 twin."""
+    s32 = oracle_sim(sc, 32)
+    s32.set_state(st)
+    s32.bin()
+    s32.p2g()
+    s32.grid_update()
+    s32.sdf_bc()
+    om, oms = s32.get_grid()
+    NC = s32.NC
     e = make_engine(sc)
     e.setState(st)
     e.setGrid(om, oms)  # both halves: FLIP reads the saved pre-force velocity
-    sim.g2p()
     e.runBin()
     e.runG2P(sc.dt)
-    g = e.getState()
-    o = sim.get_state()
-    names = ["x", "v", "F", "tau_d", "B", "tau_o"]
-    errs = {
-        "x": rel_err(g[0, :, :3], o[0, :, :3]),
-        "v": rel_err(g[1, :, :3], o[1, :, :3]),
-        "F": rel_err(g[2:5, :, :3], o[2:5, :, :3]),
-        "tau": rel_err(g[2:8, :, 3], o[2:8, :, 3]),
-        "B": rel_err(g[5:8, :, :3], o[5:8, :, :3], floor=1e-12),
-    }
-    say(f"  g2p {model} mode={mode}: " + " ".join(f"{k} {v:.1e}" for k, v in errs.items()))
-    assert np.array_equal(g[1, :, 3].view(np.uint32), o[1, :, 3].view(np.uint32))  # material ids untouched
-    assert errs["x"] < 1e-6 and errs["v"] < 1e-4 and errs["F"] < 1e-5 and errs["B"] < 1e-4
-    assert errs["tau"] < 1e-3
+    s64 = oracle_sim(sc, 64)
+    s64.set_state(st)
+    s64.bin()
+    s64.p2g()
+    s64.grid_update()
+    s64.sdf_bc()
+    s32.g2p()
+    # the FP64 twin's own grid differs from the FP32 one by FP32 noise; G2P on
+    # its grid is what the FP32-vs-FP64 spread of the reference algorithm means
+    s64.g2p()
+    return e.getState(), s32.get_state(), s64.get_state_f64()
+
+
+FIELDS = {"x": (0, slice(0, 3)), "v": (1, slice(0, 3)), "F": (slice(2, 5), slice(0, 3)),
+          "tau": (slice(2, 8), 3), "B": (slice(5, 8), slice(0, 3))}
+# absolute floors (relative to each field's scale) for fields the FP32/FP64
+# spread can leave at zero
+FLOORS = {"x": 2e-7, "v": 2e-6, "F": 2e-6, "tau": 2e-5, "B": 2e-6}
+
+
+def _field(a, k, absval=False):
+    i, j = FIELDS[k]
+    x = a[i][..., j] if not isinstance(i, int) else a[i][:, j]
+    x = np.asarray(x, np.float64)
+    return np.abs(x) if absval else x
+
+
+@pytest.mark.parametrize("corrected", [False, True], ids=["refsvd", "corrsvd"])
+@pytest.mark.parametrize("model", ["elastic", "von_mises", "drucker_prager", "fluid", "mud"])
+@pytest.mark.parametrize("mode", [-1.0, 0.0, 0.95], ids=["apic", "pic", "flip"])
+def test_g2p_matches_oracle(gpu_lib, model, mode, corrected):
+    """GPU G2P vs the FP32 oracle on one input.  Tolerance: the GPU-vs-FP32
+    error of each field is at most 4x the FP32-vs-FP64 spread of the reference
+    algorithm on the same input (plus a floor of a few ulp of the field's scale).
+    With the reference SVD convention the signs of the singular vectors are a
+    by-product of the Jacobi sweep and enter F and tau (tau = U^T k U), so F and
+    tau are compared entrywise in magnitude (diagonals signed) and the full
+    tensors must agree on at least 99% of the particles."""
+    sc, st = _g2p_case(model, mode, corrected)
+    g, o32, o64 = _g2p_all(sc, st)
+    assert np.array_equal(g[1, :, 3].view(np.uint32), o32[1, :, 3].view(np.uint32))  # material ids untouched
+    rep = []
+    for k in FIELDS:
+        signless = (not corrected) and k in ("F", "tau")
+        gg, aa, bb = (_field(x, k, signless) for x in (g, o32, o64))
+        scale = max(np.abs(bb).max(), 1e-30)
+        e_g = np.abs(gg - aa).max() / scale
+        e_o = np.abs(aa - bb).max() / scale
+        rep.append(f"{k} {e_g:.1e}/{e_o:.1e}")
+        assert e_g <= 4 * e_o + FLOORS[k], f"{k}: gpu-vs-fp32 {e_g:.2e} > 4 x fp32-vs-fp64 {e_o:.2e}"
+    if not corrected:
+        for k in ("F", "tau"):
+            gg, aa, bb = (_field(x, k) for x in (g, o32, o64))
+            scale = max(np.abs(bb).max(), 1e-30)
+            tol = 4 * np.abs(np.abs(aa) - np.abs(bb)).max() / scale + FLOORS[k]
+            per = np.abs(gg - aa)
+            per = per.max(axis=(0, 2)) if per.ndim == 3 else per.max(axis=0)  # per particle
+            frac = float(np.mean(per / scale <= tol))
+            rep.append(f"{k} full-match {frac:.4f}")
+            assert frac >= 0.99
+    say(f"  g2p {model} {sc.flip_ratio:+.2f} {'corr' if corrected else 'ref'}: " + " ".join(rep))
 
 
 # ---------------------------------------------------------------------------
 # trajectories
 # ---------------------------------------------------------------------------
-def test_elastic_block_trajectory_1000_substeps(gpu_lib):
-    """elastic_block_64 (32,768 particles, 64^3, APIC, dt=1e-4): positions
-    within 1e-4*dx of the FP32 oracle after 1000 substeps; per-field error
-    against the FP32 oracle and the FP64 twin printed."""
+def _traj_errors(g, o32, o64, dx):
+    out = {}
+    for k in ("x", "v", "F", "B"):
+        a, b, c = (_field(x, k) for x in (g, o32, o64))
+        out[k] = (np.abs(a - b).max(), np.abs(a - c).max(), np.abs(b - c).max())
+    return out
+
+
+def test_elastic_block_trajectory_reference_convention(gpu_lib):
+    """elastic_block_64 (32,768 particles, 64^3, APIC, dt=1e-4) with the
+    reference's SVD convention.  Its transposed stress rotation makes this
+    scene unstable within ~100 substeps (the FP32 and FP64 oracles separate by
+    tens of cells), so parity is stated as: at each checkpoint the GPU is no
+    farther from the FP64 twin than 3x the FP32 oracle is (+0.01 dx)."""
     sc = elastic_block_64()
+    dx = sc.cfg.cellSize
+    e = make_engine(sc)
+    s32, s64 = oracle_sim(sc, 32), oracle_sim(sc, 64)
+    done = 0
+    for cp in (10, 25, 50, 100):
+        e.substep(sc.dt, cp - done)
+        s32.substeps(cp - done)
+        s64.substeps(cp - done)
+        done = cp
+        er = _traj_errors(e.getState(), s32.get_state(), s64.get_state_f64(), dx)
+        say(f"  ref-svd {cp:4d} substeps: " + "; ".join(
+            f"{k} gpu-32 {a:.1e} gpu-64 {b:.1e} 32-64 {c:.1e}" for k, (a, b, c) in er.items()))
+        a, b, c = er["x"]
+        assert b <= 3 * c + 0.01 * dx
+
+
+def test_elastic_block_trajectory_1000_substeps_corrected(gpu_lib):
+    """The same scene with singular vectors used as columns (corrected_svd): a
+    stable fall.  After 1000 substeps the per-field errors against the FP32
+    oracle and the FP64 twin are printed; the GPU must be as close to FP64 as
+    the FP32 oracle is (2x + 1e-5 dx) and within 1e-3 dx of the FP32 oracle."""
+    sc = elastic_block_64()
+    sc.corrected_svd = True
     dx = sc.cfg.cellSize
     e = make_engine(sc)
     t0 = time.time()
     e.substep(sc.dt, 1000)
     e.sync()
-    g = e.getState()
     say(f"  gpu 1000 substeps {time.time() - t0:.2f}s")
-    s32 = oracle_sim(sc, 32)
+    s32, s64 = oracle_sim(sc, 32), oracle_sim(sc, 64)
     t0 = time.time()
     s32.substeps(1000)
     say(f"  fp32 oracle 1000 substeps {time.time() - t0:.1f}s")
-    o32 = s32.get_state()
-    s64 = oracle_sim(sc, 64)
     s64.substeps(1000)
-    o64 = s64.get_state_f64()
-    rep = []
-    for nm, sl in (("x", (0, slice(None), slice(0, 3))), ("v", (1, slice(None), slice(0, 3))),
-                   ("F", (slice(2, 5), slice(None), slice(0, 3))), ("C", (slice(5, 8), slice(None), slice(0, 3)))):
-        d32 = np.abs(g[sl].astype(np.float64) - o32[sl]).max()
-        d64 = np.abs(g[sl].astype(np.float64) - o64[sl]).max()
-        o3264 = np.abs(o32[sl].astype(np.float64) - o64[sl]).max()
-        rep.append(f"{nm}: |gpu-fp32| {d32:.2e} |gpu-fp64| {d64:.2e} |fp32-fp64| {o3264:.2e}")
-    say("  " + "; ".join(rep))
-    dxmax = np.abs(g[0, :, :3] - o32[0, :, :3]).max()
-    assert dxmax < 1e-4 * dx, f"max |x_gpu - x_fp32| = {dxmax / dx:.2e} dx"
+    er = _traj_errors(e.getState(), s32.get_state(), s64.get_state_f64(), dx)
+    say("  corr-svd 1000 substeps: " + "; ".join(
+        f"{k} gpu-32 {a:.2e} gpu-64 {b:.2e} 32-64 {c:.2e}" for k, (a, b, c) in er.items()))
+    a, b, c = er["x"]
+    say(f"  positions: |gpu-fp32| {a / dx:.2e} dx, |gpu-fp64| {b / dx:.2e} dx, |fp32-fp64| {c / dx:.2e} dx")
+    assert b <= 2 * c + 1e-5 * dx
+    assert a <= 1e-3 * dx
 
 
 def test_substeps_are_deterministic(gpu_lib):
@@ -420,18 +497,27 @@ def test_multi_material_moving_sphere_matches_oracle(gpu_lib):
     assert ex < 1e-4 and ev < 1e-3
 
 
-def test_snow_terrain_properties(gpu_lib):
-    """The 4,194,304-particle DP snow scene: mass conservation of P2G (partition
-    of unity), finiteness and confinement over a few substeps."""
+def test_snow_terrain_p2g_and_properties(gpu_lib):
+    """The 4,194,304-particle DP snow scene at full size: P2G grid against the
+    oracle's (mass and momentum at every node), total mass, then finiteness
+    and confinement over a few substeps."""
     sc = snow_terrain_256()
     e = make_engine(sc)
     e.runBin()
     e.runP2G(sc.dt)
-    _, mass = e.getGrid()
-    m_p = float(sc.materials[0].rho0) * sc.fields[0, :, 3].astype(np.float64)
-    tot = mass.astype(np.float64).sum()
-    say(f"  snow: grid mass {tot:.6e} particle mass {m_p.sum():.6e} active tiles {e.activeTileCount()}")
-    assert abs(tot - m_p.sum()) < 1e-4 * m_p.sum()
+    gm, gms = e.getGrid()
+    sim = oracle_sim(sc)
+    sim.bin()
+    sim.p2g()
+    om, oms = sim.get_grid()
+    NC = sim.NC
+    em = rel_err(gms, oms)
+    ep = rel_err(gm[:NC, :3], om[:NC, :3])
+    tot, otot = gms.astype(np.float64).sum(), oms.astype(np.float64).sum()
+    say(f"  snow p2g: mass rel {em:.2e} momentum rel {ep:.2e}; grid mass {tot:.6e} oracle {otot:.6e}; "
+        f"active tiles {e.activeTileCount()}")
+    assert em < 1e-5 and ep < 1e-4
+    assert abs(tot - otot) < 1e-6 * otot
     e.substep(sc.dt, 5)
     st = e.getState()
     assert np.isfinite(st).all()
