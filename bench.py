@@ -50,6 +50,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--profile-steps", type=int, default=20, help="substeps timed per kernel with HIP events")
     ap.add_argument("--cpu-steps", type=int, default=5, help="substeps of the CPU oracle sample (0 = skip)")
+    ap.add_argument("--calibration-copy-mb", type=int, default=0,
+                    help="also run a plain copy kernel of this many MiB (HBM byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -88,6 +90,9 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "bytes_per_launch": dbytes, "avg_launch_ms": round(avg_ms[dom], 5)}
     substep_rate = b_alg / (ms_per_step * 1e-3) / 1e9
+
+    if args.calibration_copy_mb > 0:
+        eng.calibrationCopy(args.calibration_copy_mb << 20, 3)
 
     cpu = None
     if args.cpu_steps > 0:

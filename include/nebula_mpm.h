@@ -147,8 +147,9 @@ typedef struct nb_mpm_knobs {
    * the return-map recomposition use them as columns (shaders/mpm_common.glsl:
    * 250-293, shaders/mpm_g2p.comp:35,67), i.e. tau = U^T k U and
    * F <- U^T exp(eps) V with U, V the true singular-vector matrices.
-   * 1: singular vectors used as columns (tau = U k U^T, F <- U exp(eps) V^T);
-   * an extension with no counterpart in the reference. */
+   * 1: a proper polar SVD (U, V rotations, singular vectors as columns:
+   * tau = U k U^T, F <- U exp(eps) V^T); an extension with no counterpart in
+   * the reference. */
   uint32_t corrected_svd;
 } nb_mpm_knobs;
 
@@ -258,6 +259,11 @@ int nb_mpm_set_grid(nb_mpm* h, const float* mom4, const float* mass, const float
  * names are written (comma separated) into names (capacity names_cap). */
 int nb_mpm_profile_substeps(nb_mpm* h, float dt_sub, uint32_t n, float* kernel_ms, uint32_t kernel_cap,
                             uint32_t* n_kernels, char* names, uint32_t names_cap);
+/* Calibration for HBM byte counters: copies `bytes` (rounded down to 16) from
+ * one scratch buffer to another with a plain float4 copy kernel (k_copy),
+ * `repeats` times, on the engine's stream; the buffers are allocated and freed
+ * inside the call.  Diagnostic only; no counterpart in the reference. */
+int nb_mpm_calibration_copy(nb_mpm* h, uint64_t bytes, uint32_t repeats);
 /* Count of grid nodes with mass >= 1e-10 in the current grid (G_act). */
 int nb_mpm_active_node_count(nb_mpm* h, uint64_t* g_act);
 /* Number of occupied 8^3 cell tiles after the last bin. */

@@ -702,6 +702,28 @@ int nb_mpm_profile_substeps(nb_mpm* h, float dt_sub, uint32_t n, float* kernel_m
   return NB_OK;
 }
 
+int nb_mpm_calibration_copy(nb_mpm* h, uint64_t bytes, uint32_t repeats) {
+  if (!h) return fail(NB_ERR_ARG, "null handle");
+  const uint64_t n4 = bytes / 16;
+  if (!n4) return NB_OK;
+  void *a = nullptr, *b = nullptr;
+  NB_HIP(hipMalloc(&a, n4 * 16));
+  if (hipMalloc(&b, n4 * 16) != hipSuccess) {
+    (void)hipFree(a);
+    return fail(NB_ERR_HIP, "hipMalloc failed");
+  }
+  hipError_t e = hipMemsetAsync(a, 0, n4 * 16, h->stream);
+  for (uint32_t r = 0; r < repeats && e == hipSuccess; r++) {
+    launch_copy(reinterpret_cast<float4*>(b), reinterpret_cast<const float4*>(a), n4, h->stream);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  if (e != hipSuccess) return fail(NB_ERR_HIP, "calibration copy failed: %s", hipGetErrorString(e));
+  return NB_OK;
+}
+
 int nb_mpm_active_node_count(nb_mpm* h, uint64_t* g_act) {
   if (!h || !g_act) return fail(NB_ERR_ARG, "null argument");
   *g_act = 0;

@@ -815,6 +815,12 @@ __global__ void k_init_FB(float4* F0, float4* F1, float4* F2, float4* B0, float4
   B2[q] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
+// plain streaming copy (HBM byte-counter calibration)
+__global__ __launch_bounds__(256) void k_copy(float4* __restrict__ dst, const float4* __restrict__ src, uint64_t n4) {
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n4; i += uint64_t(gridDim.x) * blockDim.x)
+    dst[i] = src[i];
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -881,6 +887,10 @@ void launch_sdf_sphere(float* sdf, const SubParams& p, float r, float cx, float 
 }
 
 void launch_fill_identity(uint32_t* a, uint32_t n, hipStream_t s) { k_fill_identity<<<cdiv(n, 256), 256, 0, s>>>(a, n); }
+
+void launch_copy(float4* dst, const float4* src, uint64_t n4, hipStream_t s) {
+  k_copy<<<8192, 256, 0, s>>>(dst, src, n4);
+}
 
 void launch_init_particles(const DevState& d, uint32_t first, uint32_t count, hipStream_t s) {
   if (!count) return;

@@ -159,8 +159,9 @@ __device__ __forceinline__ void jacobi_eigen3(float a00, float a11, float a22, f
 
 // ---- SVD via F^T F (shaders/mpm_common.glsl:250-281) ----
 // corrected = false keeps the GLSL storage (singular vectors in the rows of
-// U, V as the GLSL mat3 holds them); corrected = true transposes both so the
-// downstream U diag U^T / U diag V^T are the true spectral forms.
+// U, V as the GLSL mat3 holds them); corrected = true makes V a rotation before
+// U is formed and transposes both, so F = U diag(sigma) V^T with U, V in SO(3)
+// and the downstream U diag U^T / U diag V^T are the true spectral forms.
 __device__ __forceinline__ void svd3(const m3& F, m3& U, float sig[3], m3& V, bool corrected) {
   // FtF = transpose(F) * F: FtF[c][k] = dot(F col k, F col c)
   float f00 = F.m[0][0] * F.m[0][0] + F.m[0][1] * F.m[0][1] + F.m[0][2] * F.m[0][2];
@@ -187,6 +188,11 @@ __device__ __forceinline__ void svd3(const m3& F, m3& U, float sig[3], m3& V, bo
   NB_SWAP_SV(0, 2)
   NB_SWAP_SV(1, 2)
 #undef NB_SWAP_SV
+  if (corrected && m3_det(V) < 0.f) {  // keep V a rotation: flip the third singular vector
+    V.m[0][2] = -V.m[0][2];
+    V.m[1][2] = -V.m[1][2];
+    V.m[2][2] = -V.m[2][2];
+  }
 #pragma unroll
   for (int i = 0; i < 3; i++) {
     // GLSL vi = vec3(V[0][i], V[1][i], V[2][i]); U[r][i] = (F*vi)[r]/sigma_i,

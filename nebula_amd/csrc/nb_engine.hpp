@@ -105,5 +105,6 @@ void launch_g2p(const DevState& d, const SubParams& p, bool use_sorted, hipStrea
 void launch_sdf_sphere(float* sdf, const SubParams& p, float r, float cx, float cy, float cz, hipStream_t s);
 void launch_fill_identity(uint32_t* a, uint32_t n, hipStream_t s);
 void launch_init_particles(const DevState& d, uint32_t first, uint32_t count, hipStream_t s);
+void launch_copy(float4* dst, const float4* src, uint64_t n4, hipStream_t s);
 
 }  // namespace nb

@@ -372,7 +372,7 @@ EXPORTED_SYMBOLS = (
     "nb_mpm_reset_particles", "nb_mpm_get_field", "nb_mpm_set_field", "nb_mpm_field_device_ptr",
     "nb_mpm_set_live_particle_count", "nb_mpm_run_bin", "nb_mpm_get_bins", "nb_mpm_run_p2g",
     "nb_mpm_run_grid_update", "nb_mpm_run_g2p", "nb_mpm_get_grid", "nb_mpm_set_grid",
-    "nb_mpm_profile_substeps", "nb_mpm_active_node_count", "nb_mpm_active_tile_count",
+    "nb_mpm_profile_substeps", "nb_mpm_calibration_copy", "nb_mpm_active_node_count", "nb_mpm_active_tile_count",
 )
 
 
@@ -422,6 +422,7 @@ def lib():
         L.nb_mpm_set_grid.argtypes = [vp, fp, fp, fp]
         L.nb_mpm_profile_substeps.argtypes = [vp, C.c_float, C.c_uint32, fp, C.c_uint32, up, C.c_char_p, C.c_uint32]
         L.nb_mpm_active_node_count.argtypes = [vp, C.POINTER(C.c_uint64)]
+        L.nb_mpm_calibration_copy.argtypes = [vp, C.c_uint64, C.c_uint32]
         L.nb_mpm_active_tile_count.argtypes = [vp, up]
         _lib = L
     return _lib
@@ -685,6 +686,10 @@ class MPMEngine:
         _check(lib().nb_mpm_profile_substeps(self._h, dt_sub, n, _f(ms), 32, C.byref(nk), names, 1024))
         keys = names.value.decode().split(",")
         return {k: float(ms[i]) for i, k in enumerate(keys[: nk.value])}
+
+    def calibrationCopy(self, nbytes: int, repeats: int = 1):
+        """Plain copy kernel of nbytes (HBM byte-counter calibration)."""
+        _check(lib().nb_mpm_calibration_copy(self._h, nbytes, repeats))
 
     def activeNodeCount(self) -> int:
         g = C.c_uint64()

@@ -286,8 +286,10 @@ template <class R> inline void jacobi_eigen3(M3<R> A, V3<R>& D, M3<R>& V) {
 }
 
 // corrected = false: the GLSL result as stored (singular vectors in the rows of
-// U and V).  corrected = true: both transposed, so that U diag U^T and
-// U diag V^T are the true spectral forms (an extension, not reference semantics).
+// U and V).  corrected = true (an extension, not reference semantics): V is
+// made a rotation before U is formed and both are transposed, so that
+// F = U diag(sigma) V^T with U, V in SO(3) and U diag U^T, U diag V^T are the
+// true spectral forms.
 template <class R> inline void svd3(const M3<R>& F, M3<R>& U, V3<R>& sigma, M3<R>& V, bool corrected = false) {
   M3<R> FtF = transpose(F) * F;
   V3<R> sig2;
@@ -300,6 +302,11 @@ template <class R> inline void svd3(const M3<R>& F, M3<R>& U, V3<R>& sigma, M3<R
         for (int r = 0; r < 3; r++) std::swap(V.at(r, i), V.at(r, j));
       }
     }
+  if (corrected && determinant(V) < R(0)) {  // keep V a rotation: flip the third singular vector
+    V.at(0, 2) = -V.at(0, 2);
+    V.at(1, 2) = -V.at(1, 2);
+    V.at(2, 2) = -V.at(2, 2);
+  }
   U = M3<R>();
   for (int i = 0; i < 3; i++) {
     V3<R> vi(V.at(0, i), V.at(1, i), V.at(2, i));

@@ -322,8 +322,13 @@ def _g2p_case(model, mode, corrected):
     sc.corrected_svd = corrected
     sc.sdf = terrain_sdf(64, terrain_params(3)) + np.float32(0.3)  # surface crosses the block
     src = elastic_block_64()
-    src.corrected_svd = True  # a smooth, non-trivial starting state (F, tau, B, v)
-    return sc, _evolved(src, 30)
+    src.corrected_svd = True  # a smooth, non-trivial starting state (tau, B, v)
+    st = _evolved(src, 30)
+    # finite strains (|F - I| ~ 0.05), so that log(sigma) in the Hencky stress is
+    # resolved in FP32 and the return maps are active
+    rng = np.random.default_rng(21)
+    st[2:5, :, :3] += rng.normal(0.0, 0.05, st[2:5, :, :3].shape).astype(np.float32)
+    return sc, st
 
 
 def _g2p_all(sc, st):
