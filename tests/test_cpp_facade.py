@@ -1,0 +1,34 @@
+"""The C++ drop-in facade (include/nebula_mpm.hpp): compiles against the C ABI
+with g++ (CPU) and runs the reference-style C++ tests on the GPU."""
+import os
+import subprocess
+
+import pytest
+
+from nebula_amd import mpm
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+SRC = os.path.join(ROOT, "tests", "cpp", "engine_cpp_test.cpp")
+OUT = os.path.join(ROOT, "_build", "engine_cpp_test")
+
+
+def _build():
+    os.makedirs(os.path.dirname(OUT), exist_ok=True)
+    libdir = os.path.dirname(mpm.LIB_PATH)
+    subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-I", os.path.join(ROOT, "include"), SRC, "-o", OUT,
+                    "-L", libdir, "-lnebula_mpm", f"-Wl,-rpath,{libdir}"], check=True)
+
+
+def test_facade_compiles_and_links():
+    if not os.path.exists(mpm.LIB_PATH):
+        pytest.skip("HIP library not built")
+    _build()
+    assert os.path.exists(OUT)
+
+
+@pytest.mark.gpu
+def test_facade_runs_reference_style_tests(gpu_lib):
+    _build()
+    r = subprocess.run([OUT], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
