@@ -59,6 +59,14 @@ struct nb_mpm {
   uint32_t material_count = 1, collider_count = 0, force_count = 0;
   bool has_sdf = false;
   bool bins_valid = false;
+  // The kernels work on d.rec (one record per particle); the ABI exposes the
+  // reference layout d.field.  rec_stale: the API wrote d.field since the
+  // records were last filled.  ptr_exported: device pointers to d.field were
+  // handed out, so the caller may write them between calls.  keys_valid: the
+  // last G2P wrote every live particle's cell key and nothing moved since.
+  bool rec_stale = true;
+  bool ptr_exported = false;
+  bool keys_valid = false;
   GridMode grid_mode = GridMode::NONE;
   bool old_valid = false;
   std::vector<void*> allocs;
@@ -140,19 +148,51 @@ int download(nb_mpm* h, void* dst, const void* src, size_t bytes) {
   return NB_OK;
 }
 
+// Refresh the particle records from the reference-layout arrays when those may
+// have changed since the records were last written back.
+int sync_in(nb_mpm* h) {
+  if ((h->rec_stale || h->ptr_exported) && h->n_live) {
+    launch_soa_to_aos(h->d, h->n_live, h->stream);
+    NB_HIP(hipGetLastError());
+    h->keys_valid = false;
+  }
+  h->rec_stale = false;
+  return NB_OK;
+}
+
+// Write the particle records back to the reference-layout arrays.
+int sync_out(nb_mpm* h) {
+  if (h->n_live) {
+    launch_aos_to_soa(h->d, h->n_live, h->stream);
+    NB_HIP(hipGetLastError());
+  }
+  return NB_OK;
+}
+
+// one substep on the records (the caller brackets a run of them with
+// sync_in / sync_out)
 int enqueue_substep(nb_mpm* h, float dt, Prof* pf) {
   if (h->n_live == 0) return NB_OK;
   const bool flip = h->knobs.flip_ratio > 0.01f;
   SubParams p = h->params(dt, flip);
-  launch_bin(h->dev(), p, h->NT, h->stream, pf);
+  launch_bin(h->dev(), p, h->NT, h->keys_valid, h->stream, pf);
   launch_p2g(h->dev(), p, h->NT, h->stream, pf);
   launch_grid(h->dev(), p, h->NT, true, true, h->d.node_tiles, h->d.counters + CNT_NODE_TILES, h->stream, pf);
   launch_g2p(h->dev(), p, true, h->stream, pf);
   NB_HIP(hipGetLastError());
   h->bins_valid = true;
+  h->keys_valid = true;
   h->grid_mode = GridMode::SPARSE;
   h->old_valid = flip;
   return NB_OK;
+}
+
+int run_substeps(nb_mpm* h, float dt, uint32_t n, Prof* pf) {
+  if (n == 0 || h->n_live == 0) return NB_OK;
+  int rc = sync_in(h);
+  for (uint32_t i = 0; i < n && rc == NB_OK; i++) rc = enqueue_substep(h, dt, pf);
+  if (rc == NB_OK) rc = sync_out(h);
+  return rc;
 }
 
 }  // namespace
@@ -238,6 +278,7 @@ int nb_mpm_create(const nb_mpm_config* cfg, void* hip_stream, nb_mpm** out) {
   const size_t cells = size_t(h->NT) * kTileCells;
   int rc = NB_OK;
   for (int f = 0; f < NB_NUM_FIELDS && rc == NB_OK; f++) rc = h->alloc(&d.field[f], h->cap);
+  if (rc == NB_OK) rc = h->alloc(&d.rec, size_t(h->cap) * kRec);
   if (rc == NB_OK) rc = h->alloc(&d.keys, h->cap);
   if (rc == NB_OK) rc = h->alloc(&d.rank, h->cap);
   if (rc == NB_OK) rc = h->alloc(&d.sorted, h->cap);
@@ -333,26 +374,17 @@ int nb_mpm_destroy(nb_mpm* h) {
 int nb_mpm_step(nb_mpm* h, float dt) {
   if (!h) return fail(NB_ERR_ARG, "null handle");
   const int ns = std::max(1, h->knobs.num_substeps);
-  const float sub = dt / float(ns);
-  for (int i = 0; i < ns; i++) {
-    int rc = enqueue_substep(h, sub, nullptr);
-    if (rc != NB_OK) return rc;
-  }
-  return NB_OK;
+  return run_substeps(h, dt / float(ns), uint32_t(ns), nullptr);
 }
 
 int nb_mpm_substep(nb_mpm* h, float dt_sub) {
   if (!h) return fail(NB_ERR_ARG, "null handle");
-  return enqueue_substep(h, dt_sub, nullptr);
+  return run_substeps(h, dt_sub, 1, nullptr);
 }
 
 int nb_mpm_substeps(nb_mpm* h, float dt_sub, uint32_t n) {
   if (!h) return fail(NB_ERR_ARG, "null handle");
-  for (uint32_t i = 0; i < n; i++) {
-    int rc = enqueue_substep(h, dt_sub, nullptr);
-    if (rc != NB_OK) return rc;
-  }
-  return NB_OK;
+  return run_substeps(h, dt_sub, n, nullptr);
 }
 
 int nb_mpm_sync(nb_mpm* h) {
@@ -410,6 +442,7 @@ int nb_mpm_set_particle_material_ids(nb_mpm* h, const uint32_t* ids, uint32_t n)
   if (n > h->cap) return fail(NB_ERR_RANGE, "%u ids exceed capacity %u", n, h->cap);
   std::vector<float> v(size_t(n) * 4, 0.f);
   for (uint32_t i = 0; i < n; i++) std::memcpy(&v[4 * size_t(i) + 3], &ids[i], 4);
+  h->rec_stale = true;
   return upload(h, h->d.field[NB_FIELD_V], v.data(), v.size() * 4);
 }
 
@@ -490,6 +523,7 @@ int nb_mpm_append_particles(nb_mpm* h, const float* pos4, const float* vel4, uin
   NB_HIP(hipStreamSynchronize(h->stream));
   h->n_live += nn;
   h->bins_valid = false;
+  h->rec_stale = true;
   if (appended) *appended = nn;
   return NB_OK;
 }
@@ -514,6 +548,7 @@ int nb_mpm_set_field(nb_mpm* h, int field, const float* in4, uint32_t first, uin
   if (field < 0 || field >= NB_NUM_FIELDS) return fail(NB_ERR_ARG, "bad field %d", field);
   if (uint64_t(first) + count > h->cap) return fail(NB_ERR_RANGE, "rows beyond capacity");
   h->bins_valid = false;
+  h->rec_stale = true;
   return upload(h, h->d.field[field] + first, in4, size_t(count) * 16);
 }
 
@@ -521,6 +556,7 @@ int nb_mpm_field_device_ptr(nb_mpm* h, int field, void** dptr) {
   if (!h || !dptr) return fail(NB_ERR_ARG, "null argument");
   if (field < 0 || field >= NB_NUM_FIELDS) return fail(NB_ERR_ARG, "bad field %d", field);
   *dptr = h->d.field[field];
+  h->ptr_exported = true;
   return NB_OK;
 }
 
@@ -529,6 +565,7 @@ int nb_mpm_set_live_particle_count(nb_mpm* h, uint32_t n) {
   if (n > h->cap) return fail(NB_ERR_RANGE, "%u exceeds capacity %u", n, h->cap);
   h->n_live = n;
   h->bins_valid = false;
+  h->rec_stale = true;
   return NB_OK;
 }
 
@@ -536,7 +573,9 @@ int nb_mpm_set_live_particle_count(nb_mpm* h, uint32_t n) {
 int nb_mpm_run_bin(nb_mpm* h) {
   if (!h) return fail(NB_ERR_ARG, "null handle");
   SubParams p = h->params(0.f, false);
-  if (h->n_live) launch_bin(h->dev(), p, h->NT, h->stream, nullptr);
+  int rc = sync_in(h);
+  if (rc != NB_OK) return rc;
+  if (h->n_live) launch_bin(h->dev(), p, h->NT, false, h->stream, nullptr);
   NB_HIP(hipGetLastError());
   NB_HIP(hipStreamSynchronize(h->stream));
   h->bins_valid = true;
@@ -575,6 +614,8 @@ int nb_mpm_run_p2g(nb_mpm* h, float dt_sub) {
   if (!h) return fail(NB_ERR_ARG, "null handle");
   if (!h->bins_valid) return fail(NB_ERR_STATE, "no bins: call nb_mpm_run_bin first");
   SubParams p = h->params(dt_sub, false);
+  int rc = sync_in(h);
+  if (rc != NB_OK) return rc;
   if (h->n_live) {
     launch_p2g(h->dev(), p, h->NT, h->stream, nullptr);
     launch_grid(h->dev(), p, h->NT, true, false, h->d.node_tiles, h->d.counters + CNT_NODE_TILES, h->stream, nullptr);
@@ -603,10 +644,14 @@ int nb_mpm_run_grid_update(nb_mpm* h, float dt_sub) {
 int nb_mpm_run_g2p(nb_mpm* h, float dt_sub) {
   if (!h) return fail(NB_ERR_ARG, "null handle");
   SubParams p = h->params(dt_sub, false);
+  int rc = sync_in(h);
+  if (rc != NB_OK) return rc;
   if (h->n_live) launch_g2p(h->dev(), p, h->bins_valid, h->stream, nullptr);
   NB_HIP(hipGetLastError());
+  if ((rc = sync_out(h)) != NB_OK) return rc;
   NB_HIP(hipStreamSynchronize(h->stream));
   h->bins_valid = false;  // positions moved
+  h->keys_valid = h->n_live != 0;
   return NB_OK;
 }
 
@@ -683,8 +728,7 @@ int nb_mpm_profile_substeps(nb_mpm* h, float dt_sub, uint32_t n, float* kernel_m
   if (!h) return fail(NB_ERR_ARG, "null handle");
   Prof pf{};
   for (int i = 0; i < 2 * kNumProfiledKernels; i++) NB_HIP(hipEventCreate(&pf.ev[i]));
-  int rc = NB_OK;
-  for (uint32_t i = 0; i < n && rc == NB_OK; i++) rc = enqueue_substep(h, dt_sub, &pf);
+  int rc = run_substeps(h, dt_sub, n, &pf);
   (void)hipStreamSynchronize(h->stream);
   for (int i = 0; i < 2 * kNumProfiledKernels; i++) (void)hipEventDestroy(pf.ev[i]);
   if (rc != NB_OK) return rc;

@@ -8,18 +8,25 @@
 //          hash_add_base, mpm_hash_sort; output is bit-identical to the stable
 //          counting sort with END offsets of tests/helpers/MPMHarness.cpp:132-178,
 //          with no group-count limit)
-//   P2G   k_p2g: one block per occupied 8^3 tile, one thread per cell; the cell's
-//         particles are accumulated in registers into the 27 nodes they reach,
-//         then merged into an LDS node box in 27 barrier-separated phases (no
-//         atomics, fixed order) and written as a per-tile partial
+//   P2G   k_p2g: one block per occupied 8^3 tile; its occupied cells are
+//         compacted onto threads, each thread walks its cell's particles (one
+//         128-byte record each) and sums their 27 node contributions in
+//         registers; the sums are merged into the tile's 10^3 LDS node box in 27
+//         barrier-separated phases (every thread owns a distinct node in a
+//         phase: no atomics, fixed order) and written as a per-tile partial
 //         (replaces mpm_zero_grid + mpm_p2g.comp's node-centric gather)
 //   grid  k_grid: one block per node tile; sums the <= 8 partials covering each
 //         node in a fixed order, then velocity, cap, forces, walls, analytic
 //         colliders and the dense-SDF BC in registers
 //         (replaces mpm_grid_update.comp + mpm_nanovdb_bc.comp)
 //   G2P   k_g2p: one thread per particle in bin order; 27-node gather, F update,
-//         register SVD, return map, Hencky stress, guards, advection
+//         register SVD, return map, Hencky stress, guards, advection; also
+//         writes the particle's cell key for the next substep's bin
 //         (replaces mpm_g2p.comp)
+// Particle state lives in an interleaved record per particle (the eight float4
+// of the reference's layout side by side: 128 bytes, one cache line), so the
+// gathers of P2G and G2P read whole lines; k_soa_to_aos / k_aos_to_soa move it
+// to and from the reference's eight arrays at the API boundary.
 // Every kernel is deterministic: identical inputs give bit-identical outputs.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -99,18 +106,24 @@ __global__ __launch_bounds__(512) void k_bin_clear(uint32_t* __restrict__ cell_c
   }
 }
 
-// Key of every particle, its rank among equal keys (wave-aggregated atomics:
-// one atomic per distinct key per wave), and the occupied-tile flags.
-__global__ __launch_bounds__(256) void k_bin_keys(SubParams p, const float4* __restrict__ P,
+// Key of every particle (computed from the record, or read when the previous
+// G2P wrote it), its rank among equal keys (wave-aggregated atomics: one
+// atomic per distinct key per wave), and the occupied-tile flags.
+template <bool FROM_REC>
+__global__ __launch_bounds__(256) void k_bin_keys(SubParams p, const float4* __restrict__ rec,
                                                   uint32_t* __restrict__ keys, uint32_t* __restrict__ rank,
                                                   uint32_t* __restrict__ cell_count, uint8_t* __restrict__ tile_flag) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i < p.n;
   uint32_t key = 0xFFFFFFFFu;
   if (valid) {
-    float4 x = P[i];
-    key = cell_key(p, x.x, x.y, x.z);
-    keys[i] = key;
+    if (FROM_REC) {
+      const float4 x = rec[size_t(i) * kRec];
+      key = cell_key(p, x.x, x.y, x.z);
+      keys[i] = key;
+    } else {
+      key = keys[i];
+    }
   }
   const uint32_t lane = threadIdx.x & 63;
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -153,7 +166,14 @@ __global__ __launch_bounds__(256) void k_bin_mark(const uint8_t* __restrict__ ti
 }
 
 // Single block: ordered compaction of occupied tiles and node tiles; resets the
-// occupied flags for the next bin.
+// occupied flags for the next bin.  Each thread owns a contiguous chunk of
+// tiles; when the chunk is a multiple of 16 its marks are read with 16-byte
+// loads so the loads of a thread are independent.
+__device__ __forceinline__ uint8_t byte_of(const uint4& v, int i) {
+  const uint32_t w = i < 4 ? v.x : (i < 8 ? v.y : (i < 12 ? v.z : v.w));
+  return uint8_t((w >> (8 * (i & 3))) & 0xFFu);
+}
+
 __global__ __launch_bounds__(1024) void k_bin_compact(const uint8_t* __restrict__ tile_mark,
                                                       uint8_t* __restrict__ tile_flag, int32_t* __restrict__ tile_act,
                                                       uint32_t* __restrict__ act_tiles,
@@ -162,26 +182,55 @@ __global__ __launch_bounds__(1024) void k_bin_compact(const uint8_t* __restrict_
   __shared__ uint32_t s_wave[1024 / 64 + 1];
   const uint32_t chunk = (nt + 1023u) / 1024u;
   const uint32_t lo = min(nt, threadIdx.x * chunk), hi = min(nt, lo + chunk);
+  const bool vec = (chunk % 16u) == 0u;  // then nt is a multiple of 16 * 1024 and lo is 16-aligned
   uint32_t na = 0, nn = 0;
-  for (uint32_t T = lo; T < hi; T++) {
-    uint8_t m = tile_mark[T];
-    na += m & 1u;
-    nn += (m >> 1) & 1u;
+  if (vec) {
+    for (uint32_t T = lo; T < hi; T += 16) {
+      const uint4 v = *reinterpret_cast<const uint4*>(tile_mark + T);
+      const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        na += __popc(w[k] & 0x01010101u);
+        nn += __popc(w[k] & 0x02020202u);
+      }
+    }
+  } else {
+    for (uint32_t T = lo; T < hi; T++) {
+      const uint8_t m = tile_mark[T];
+      na += m & 1u;
+      nn += (m >> 1) & 1u;
+    }
   }
   uint32_t tot_a, tot_n;
   uint32_t oa = block_exclusive_scan<1024>(na, s_wave, tot_a);
   uint32_t on = block_exclusive_scan<1024>(nn, s_wave, tot_n);
-  for (uint32_t T = lo; T < hi; T++) {
-    uint8_t m = tile_mark[T];
-    if (m & 1u) {
-      act_tiles[oa] = T;
-      tile_act[T] = int32_t(oa);
-      oa++;
-    } else {
-      tile_act[T] = -1;
+  if (vec) {
+    for (uint32_t T0 = lo; T0 < hi; T0 += 16) {
+      const uint4 v = *reinterpret_cast<const uint4*>(tile_mark + T0);
+#pragma unroll
+      for (int i = 0; i < 16; i++) {
+        const uint8_t m = byte_of(v, i);
+        const uint32_t T = T0 + i;
+        if (m & 1u) act_tiles[oa] = T;
+        tile_act[T] = (m & 1u) ? int32_t(oa) : -1;
+        oa += m & 1u;
+        if (m & 2u) node_tiles[on++] = T;
+      }
+      *reinterpret_cast<uint4*>(tile_flag + T0) = make_uint4(0u, 0u, 0u, 0u);
     }
-    if (m & 2u) node_tiles[on++] = T;
-    tile_flag[T] = 0;
+  } else {
+    for (uint32_t T = lo; T < hi; T++) {
+      const uint8_t m = tile_mark[T];
+      if (m & 1u) {
+        act_tiles[oa] = T;
+        tile_act[T] = int32_t(oa);
+        oa++;
+      } else {
+        tile_act[T] = -1;
+      }
+      if (m & 2u) node_tiles[on++] = T;
+      tile_flag[T] = 0;
+    }
   }
   if (threadIdx.x == 0) {
     counters[CNT_ACTIVE] = tot_a;
@@ -243,6 +292,11 @@ __global__ __launch_bounds__(256) void k_bin_scatter(uint32_t n, const uint32_t*
 
 // Per cell: order its particles by index (the ranks of different waves arrive
 // in atomic order), then store the END offset, the reference's cellOffset.
+// Cells of up to 16 particles are sorted in registers (independent loads,
+// odd-even transposition network, written back only if the order changed);
+// larger cells use an in-place insertion sort.
+constexpr int kRegSort = 16;
+
 __global__ __launch_bounds__(512) void k_bin_stabilize(uint32_t n, const uint32_t* __restrict__ cell_count,
                                                        uint32_t* __restrict__ cell_end,
                                                        const uint32_t* __restrict__ act_tiles,
@@ -253,17 +307,47 @@ __global__ __launch_bounds__(512) void k_bin_stabilize(uint32_t n, const uint32_
   for (uint32_t a = blockIdx.x; a < na; a += gridDim.x) {
     const size_t key = (size_t(act_tiles[a]) << kTileBits) + threadIdx.x;
     const uint32_t start = tile_start[a] + cell_end[key];
-    const uint32_t stop = min(start + cell_count[key], n);
-    for (uint32_t k = start + 1; k < stop; k++) {
-      uint32_t v = sorted[k];
-      uint32_t j = k;
-      while (j > start && sorted[j - 1] > v) {
-        sorted[j] = sorted[j - 1];
-        j--;
+    const uint32_t cnt = cell_count[key];
+    const uint32_t stop = min(start + cnt, n);
+    const uint32_t m = stop > start ? stop - start : 0u;
+    if (m <= uint32_t(kRegSort)) {
+      uint32_t v[kRegSort];
+#pragma unroll
+      for (int i = 0; i < kRegSort; i++) v[i] = uint32_t(i) < m ? sorted[start + i] : 0xFFFFFFFFu;
+      // cells whose particles all came from one wave are already in order
+      bool ordered = true;
+#pragma unroll
+      for (int i = 0; i + 1 < kRegSort; i++) ordered &= v[i] <= v[i + 1];
+      bool changed = false;
+      if (!__all(ordered)) {
+#pragma unroll
+        for (int round = 0; round < kRegSort; round++) {
+#pragma unroll
+          for (int i = round & 1; i + 1 < kRegSort; i += 2) {
+            const uint32_t lo = min(v[i], v[i + 1]), hi = max(v[i], v[i + 1]);
+            changed |= lo != v[i];
+            v[i] = lo;
+            v[i + 1] = hi;
+          }
+        }
       }
-      sorted[j] = v;
+      if (changed) {
+#pragma unroll
+        for (int i = 0; i < kRegSort; i++)
+          if (uint32_t(i) < m) sorted[start + i] = v[i];
+      }
+    } else {
+      for (uint32_t k = start + 1; k < stop; k++) {
+        uint32_t v = sorted[k];
+        uint32_t j = k;
+        while (j > start && sorted[j - 1] > v) {
+          sorted[j] = sorted[j - 1];
+          j--;
+        }
+        sorted[j] = v;
+      }
     }
-    cell_end[key] = start + cell_count[key];
+    cell_end[key] = start + cnt;
   }
 }
 
@@ -275,127 +359,164 @@ struct Axis3 {
   float w[3], g[3], dx[3];  // weight, weight derivative / h, x_node - x_p
 };
 
-__device__ __forceinline__ void axis_stencil(float xp, float wmin, float h, int first_node, Axis3& a) {
+// Quadratic B-spline weight and derivative (shaders/mpm_common.glsl:198-210),
+// evaluated with selects instead of branches; inv_h = 1/h replaces the
+// division of the derivative by h.
+__device__ __forceinline__ void axis_stencil(float xp, float wmin, float h, float inv_h, int first_node, Axis3& a) {
   const float f = (xp - wmin) / h;
 #pragma unroll
   for (int i = 0; i < 3; i++) {
     const float nf = float(first_node + i) + 0.5f;
     const float d = f - nf;
-    a.w[i] = bspline2(d);
-    a.g[i] = bspline2g(d) / h;
+    const float ad = fabsf(d);
+    const float t = 1.5f - ad;
+    const bool in = ad < 0.5f, mid = ad < 1.5f;
+    a.w[i] = in ? 0.75f - ad * ad : (mid ? 0.5f * t * t : 0.f);
+    a.g[i] = (in ? -2.f * d : (mid ? (d > 0.f ? -t : t) : 0.f)) * inv_h;
     a.dx[i] = nf * h + wmin - xp;
   }
 }
 
-__global__ __launch_bounds__(512) void k_p2g(SubParams p, const float4* __restrict__ P, const float4* __restrict__ V,
-                                             const float4* __restrict__ F0, const float4* __restrict__ F1,
-                                             const float4* __restrict__ F2, const float4* __restrict__ B0,
-                                             const float4* __restrict__ B1, const float4* __restrict__ B2,
-                                             const nb_material* __restrict__ mats,
-                                             const uint32_t* __restrict__ cell_count,
-                                             const uint32_t* __restrict__ cell_end,
-                                             const uint32_t* __restrict__ sorted,
-                                             const uint32_t* __restrict__ act_tiles,
-                                             const uint32_t* __restrict__ counters, float4* __restrict__ partial) {
+// P2G: one block per occupied tile.  The tile's occupied cells are compacted
+// (in Morton order) onto the block's threads; a thread walks its cell's
+// particles in bin order, reading one 128-byte record each, and sums their
+// contributions to the 27 nodes the cell reaches in registers.  The sums are
+// then merged into the tile's 10^3 LDS node box in 27 barrier-separated
+// phases: within a phase every thread adds to a distinct node, so plain
+// read-add-write is exact and the order is fixed.
+constexpr int kP2GThreads = 256;
+
+__global__ __launch_bounds__(kP2GThreads) void k_p2g(SubParams p, const float4* __restrict__ rec,
+                                                     const nb_material* __restrict__ mats,
+                                                     const uint32_t* __restrict__ cell_count,
+                                                     const uint32_t* __restrict__ cell_end,
+                                                     const uint32_t* __restrict__ sorted,
+                                                     const uint32_t* __restrict__ act_tiles,
+                                                     const uint32_t* __restrict__ counters,
+                                                     float4* __restrict__ partial) {
   __shared__ float s_m[kBoxNodes], s_x[kBoxNodes], s_y[kBoxNodes], s_z[kBoxNodes];
+  __shared__ uint16_t s_cell[kTileCells];
+  __shared__ uint32_t s_wave[kP2GThreads / 64 + 1];
   const uint32_t na = counters[CNT_ACTIVE];
-  const int lx = threadIdx.x & 7, ly = (threadIdx.x >> 3) & 7, lz = threadIdx.x >> 6;
   const bool apic = p.flip_ratio < -0.5f;
   const float D_inv = 4.f / (p.h * p.h);
+  const float inv_h = 1.f / p.h;
   const uint32_t mlast = p.material_count - 1u;
 
   for (uint32_t a = blockIdx.x; a < na; a += gridDim.x) {
     const uint32_t T = act_tiles[a];
     int tx, ty, tz;
     tile_coords(T, tx, ty, tz);
-    const int cx = tx * 8 + lx, cy = ty * 8 + ly, cz = tz * 8 + lz;
-    for (int e = threadIdx.x; e < kBoxNodes; e += 512) {
+    const size_t tbase = size_t(T) << kTileBits;
+    for (int e = threadIdx.x; e < kBoxNodes; e += kP2GThreads) {
       s_m[e] = 0.f; s_x[e] = 0.f; s_y[e] = 0.f; s_z[e] = 0.f;
     }
-    const uint32_t key = (T << kTileBits) | morton3(lx, ly, lz);
-    const uint32_t cnt = cell_count[key];
-    const uint32_t end = cell_end[key];
+    // compact the occupied cells (thread t looks at cells 2t, 2t+1)
+    const uint32_t ca = 2u * threadIdx.x, cb = ca + 1u;
+    const bool oa = cell_count[tbase + ca] != 0u, ob = cell_count[tbase + cb] != 0u;
+    uint32_t n_occ;
+    uint32_t pos = block_exclusive_scan<kP2GThreads>(uint32_t(oa) + uint32_t(ob), s_wave, n_occ);
+    if (oa) s_cell[pos++] = uint16_t(ca);
+    if (ob) s_cell[pos] = uint16_t(cb);
+    __syncthreads();
 
-    float am[27], ax[27], ay[27], az[27];
-#pragma unroll
-    for (int o = 0; o < 27; o++) { am[o] = 0.f; ax[o] = 0.f; ay[o] = 0.f; az[o] = 0.f; }
-
-    for (uint32_t k = end - cnt; k < end; k++) {
-      const uint32_t q = sorted[k];
-      const float4 pp = P[q];
-      const float4 vv = V[q];
-      uint32_t mid = __float_as_uint(vv.w);
-      mid = mid > mlast ? mlast : mid;
-      const float mp = mats[mid].rho0 * pp.w;
-      Axis3 sx, sy, sz;
-      axis_stencil(pp.x, p.wmin[0], p.h, cx - 1, sx);
-      axis_stencil(pp.y, p.wmin[1], p.h, cy - 1, sy);
-      axis_stencil(pp.z, p.wmin[2], p.h, cz - 1, sz);
-      // affine momentum matrix C = mp * D_inv * B (zero unless APIC)
-      m3 C = m3_zero();
-      if (apic) {
-        const float4 b0 = B0[q], b1 = B1[q], b2 = B2[q];
-        const float s = mp * D_inv;
-        C.m[0][0] = s * b0.x; C.m[0][1] = s * b0.y; C.m[0][2] = s * b0.z;
-        C.m[1][0] = s * b1.x; C.m[1][1] = s * b1.y; C.m[1][2] = s * b1.z;
-        C.m[2][0] = s * b2.x; C.m[2][1] = s * b2.y; C.m[2][2] = s * b2.z;
+    for (uint32_t base = 0; base < n_occ; base += kP2GThreads) {
+      const uint32_t j = base + threadIdx.x;
+      const bool have = j < n_occ;
+      const uint32_t c = have ? uint32_t(s_cell[j]) : 0u;
+      const int lx = int(morton_compact(c)), ly = int(morton_compact(c >> 1)), lz = int(morton_compact(c >> 2));
+      const int cx = tx * 8 + lx, cy = ty * 8 + ly, cz = tz * 8 + lz;
+      uint32_t kbeg = 0, kend = 0;
+      if (have) {
+        kend = cell_end[tbase + c];
+        kbeg = kend - cell_count[tbase + c];
       }
-      // stress term dt*Vp*tau, tau symmetric in the w lanes (shaders/mpm_common.glsl:104-113)
-      const float st = p.dt * pp.w;
-      const float t00 = st * F0[q].w, t11 = st * F1[q].w, t22 = st * F2[q].w;
-      const float t01 = st * B0[q].w, t02 = st * B1[q].w, t12 = st * B2[q].w;
-      const float mvx = mp * vv.x, mvy = mp * vv.y, mvz = mp * vv.z;
+      float am[27], ax[27], ay[27], az[27];
 #pragma unroll
-      for (int kz = 0; kz < 3; kz++) {
+      for (int o = 0; o < 27; o++) { am[o] = 0.f; ax[o] = 0.f; ay[o] = 0.f; az[o] = 0.f; }
+
+      // the next particle's record is loaded while this one is scattered
+      float4 npp, nvv, nb0, nb1, nb2;
+      float nf0w, nf1w, nf2w;
+      auto load_rec = [&](uint32_t k) {
+        const float4* r = rec + size_t(sorted[k]) * kRec;
+        npp = r[kP]; nvv = r[kV];
+        nf0w = r[kF0].w; nf1w = r[kF1].w; nf2w = r[kF2].w;
+        nb0 = r[kB0]; nb1 = r[kB1]; nb2 = r[kB2];
+      };
+      if (kbeg < kend) load_rec(kbeg);
+      for (uint32_t k = kbeg; k < kend; k++) {
+        const float4 pp = npp, vv = nvv, b0 = nb0, b1 = nb1, b2 = nb2;
+        const float f0w = nf0w, f1w = nf1w, f2w = nf2w;
+        if (k + 1 < kend) load_rec(k + 1);
+        uint32_t mid = __float_as_uint(vv.w);
+        mid = mid > mlast ? mlast : mid;
+        const float mp = mats[mid].rho0 * pp.w;
+        Axis3 sx, sy, sz;
+        axis_stencil(pp.x, p.wmin[0], p.h, inv_h, cx - 1, sx);
+        axis_stencil(pp.y, p.wmin[1], p.h, inv_h, cy - 1, sy);
+        axis_stencil(pp.z, p.wmin[2], p.h, inv_h, cz - 1, sz);
+        const float sc = apic ? mp * D_inv : 0.f;  // C = m (4/h^2) B, zero unless APIC
+        const float c00 = sc * b0.x, c01 = sc * b0.y, c02 = sc * b0.z;
+        const float c10 = sc * b1.x, c11 = sc * b1.y, c12 = sc * b1.z;
+        const float c20 = sc * b2.x, c21 = sc * b2.y, c22 = sc * b2.z;
+        // dt Vp tau, tau symmetric in the w lanes (shaders/mpm_common.glsl:104-113)
+        const float st = p.dt * pp.w;
+        const float t00 = st * f0w, t11 = st * f1w, t22 = st * f2w;
+        const float t01 = st * b0.w, t02 = st * b1.w, t12 = st * b2.w;
+        const float mvx = mp * vv.x, mvy = mp * vv.y, mvz = mp * vv.z;
 #pragma unroll
-        for (int jy = 0; jy < 3; jy++) {
-          const float wyz = sy.w[jy] * sz.w[kz];
-          const float gy_wz = sy.g[jy] * sz.w[kz];
-          const float wy_gz = sy.w[jy] * sz.g[kz];
-          // mp*v + C*(0, dy, dz)
-          const float Ax = mvx + C.m[1][0] * sy.dx[jy] + C.m[2][0] * sz.dx[kz];
-          const float Ay = mvy + C.m[1][1] * sy.dx[jy] + C.m[2][1] * sz.dx[kz];
-          const float Az = mvz + C.m[1][2] * sy.dx[jy] + C.m[2][2] * sz.dx[kz];
-          // tau*(0, gy wz, wy gz) (the wx factor applied per node)
-          const float Tx = t01 * gy_wz + t02 * wy_gz;
-          const float Ty = t11 * gy_wz + t12 * wy_gz;
-          const float Tz = t12 * gy_wz + t22 * wy_gz;
+        for (int kz = 0; kz < 3; kz++) {
 #pragma unroll
-          for (int ix = 0; ix < 3; ix++) {
-            const int o = (kz * 3 + jy) * 3 + ix;
-            float w = sx.w[ix] * wyz;
-            const bool live = w >= 1e-12f;  // shaders/mpm_p2g.comp:61
-            w = live ? w : 0.f;
-            const float wx = live ? sx.w[ix] : 0.f;
-            const float wd = w * sx.dx[ix];
-            const float gw = live ? sx.g[ix] * wyz : 0.f;
-            am[o] += w * mp;
-            ax[o] += w * Ax + wd * C.m[0][0] - gw * t00 - wx * Tx;
-            ay[o] += w * Ay + wd * C.m[0][1] - gw * t01 - wx * Ty;
-            az[o] += w * Az + wd * C.m[0][2] - gw * t02 - wx * Tz;
+          for (int jy = 0; jy < 3; jy++) {
+            const float wyz = sy.w[jy] * sz.w[kz];
+            const float gy_wz = sy.g[jy] * sz.w[kz];
+            const float wy_gz = sy.w[jy] * sz.g[kz];
+            // m v + C (0, dy, dz)
+            const float Ax = mvx + c10 * sy.dx[jy] + c20 * sz.dx[kz];
+            const float Ay = mvy + c11 * sy.dx[jy] + c21 * sz.dx[kz];
+            const float Az = mvz + c12 * sy.dx[jy] + c22 * sz.dx[kz];
+            // tau (0, gy wz, wy gz); the wx factor is applied per node
+            const float Tx = t01 * gy_wz + t02 * wy_gz;
+            const float Ty = t11 * gy_wz + t12 * wy_gz;
+            const float Tz = t12 * gy_wz + t22 * wy_gz;
+#pragma unroll
+            for (int ix = 0; ix < 3; ix++) {
+              const int o = (kz * 3 + jy) * 3 + ix;
+              float w = sx.w[ix] * wyz;
+              const bool live = w >= 1e-12f;  // shaders/mpm_p2g.comp:61
+              w = live ? w : 0.f;
+              const float wx = live ? sx.w[ix] : 0.f;
+              const float wd = w * sx.dx[ix];
+              const float gw = live ? sx.g[ix] * wyz : 0.f;
+              am[o] += w * mp;
+              ax[o] += w * Ax + wd * c00 - gw * t00 - wx * Tx;
+              ay[o] += w * Ay + wd * c01 - gw * t01 - wx * Ty;
+              az[o] += w * Az + wd * c02 - gw * t02 - wx * Tz;
+            }
           }
         }
       }
+#pragma unroll
+      for (int kz = 0; kz < 3; kz++)
+#pragma unroll
+        for (int jy = 0; jy < 3; jy++)
+#pragma unroll
+          for (int ix = 0; ix < 3; ix++) {
+            const int o = (kz * 3 + jy) * 3 + ix;
+            const int e = ((lz + kz) * kBox + (ly + jy)) * kBox + (lx + ix);
+            __syncthreads();
+            if (have) {
+              s_m[e] += am[o];
+              s_x[e] += ax[o];
+              s_y[e] += ay[o];
+              s_z[e] += az[o];
+            }
+          }
+      __syncthreads();
     }
-    // merge the 27 per-cell node sums into the box, one offset per phase: in a
-    // phase every thread owns a distinct node, so plain read-add-write is exact
-#pragma unroll
-    for (int kz = 0; kz < 3; kz++)
-#pragma unroll
-      for (int jy = 0; jy < 3; jy++)
-#pragma unroll
-        for (int ix = 0; ix < 3; ix++) {
-          const int o = (kz * 3 + jy) * 3 + ix;
-          const int e = ((lz + kz) * kBox + (ly + jy)) * kBox + (lx + ix);
-          __syncthreads();
-          s_m[e] += am[o];
-          s_x[e] += ax[o];
-          s_y[e] += ay[o];
-          s_z[e] += az[o];
-        }
-    __syncthreads();
     float4* out = partial + size_t(a) * kBoxNodes;
-    for (int e = threadIdx.x; e < kBoxNodes; e += 512) out[e] = make_float4(s_x[e], s_y[e], s_z[e], s_m[e]);
+    for (int e = threadIdx.x; e < kBoxNodes; e += kP2GThreads) out[e] = make_float4(s_x[e], s_y[e], s_z[e], s_m[e]);
     __syncthreads();
   }
 }
@@ -586,10 +707,8 @@ __global__ __launch_bounds__(512) void k_grid(SubParams p, const float4* __restr
 // ---------------------------------------------------------------------------
 // G2P
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_g2p(SubParams p, float4* __restrict__ P, float4* __restrict__ V,
-                                             float4* __restrict__ F0, float4* __restrict__ F1,
-                                             float4* __restrict__ F2, float4* __restrict__ B0,
-                                             float4* __restrict__ B1, float4* __restrict__ B2,
+template <bool FLIP>
+__global__ __launch_bounds__(256) void k_g2p(SubParams p, float4* __restrict__ rec, uint32_t* __restrict__ keys,
                                              const nb_material* __restrict__ mats,
                                              const uint32_t* __restrict__ sorted,
                                              const float4* __restrict__ grid_node,
@@ -597,70 +716,169 @@ __global__ __launch_bounds__(256) void k_g2p(SubParams p, float4* __restrict__ P
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= p.n) return;
   const uint32_t q = sorted ? sorted[k] : k;
-  const float4 pp = P[q];
+  float4* r = rec + size_t(q) * kRec;
+  const float4 pp = r[kP];
   const float xp = pp.x, yp = pp.y, zp = pp.z, Vp = pp.w;
-  const bool flip = p.flip_ratio > 0.01f;
+  constexpr bool flip = FLIP;  // flip_ratio > 0.01
 
   // stencil base = floor((x - wmin)/h) - 1; the quotient is clamped to a range
   // that keeps every integer below in bounds without changing which nodes are valid
   int base[3];
   Axis3 ax3[3];
   const float xs[3] = {xp, yp, zp};
+  const float inv_h = 1.f / p.h;
 #pragma unroll
   for (int a = 0; a < 3; a++) {
     float f = (xs[a] - p.wmin[a]) / p.h;
     f = fminf(fmaxf(f, -4.f), float(p.grid_res[a]) + 4.f);
     base[a] = int(floorf(f)) - 1;
-    axis_stencil(xs[a], p.wmin[a], p.h, base[a], ax3[a]);
+    axis_stencil(xs[a], p.wmin[a], p.h, inv_h, base[a], ax3[a]);
   }
+  // Node (i,j,k) takes part when it lies in the grid and carries mass >= 1e-10
+  // and w = (wx*wy)*wz >= 1e-12 (shaders/mpm_g2p.comp:93-101).  Out-of-grid and
+  // massless nodes are folded in as zero velocity.  If no node of the wave has
+  // 0 < w < 1e-12 (the smallest positive per-axis weights bound every w from
+  // below), the weight cut removes nothing and the sums are contracted axis by
+  // axis; otherwise every node is visited with the cut applied.
+  bool tiny;
+  {
+    float mn[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      mn[a] = INFINITY;
+#pragma unroll
+      for (int i = 0; i < 3; i++) mn[a] = fminf(mn[a], ax3[a].w[i] > 0.f ? ax3[a].w[i] : INFINITY);
+    }
+    tiny = (mn[0] * mn[1]) * mn[2] < 1e-12f;
+  }
+  // out-of-grid nodes read a clamped (valid) address and are zeroed by select
+  auto node_vel = [&](const float4* __restrict__ grid, int ix, int iy, int iz) -> float4 {
+    const int nx = base[0] + ix, ny = base[1] + iy, nz = base[2] + iz;
+    const bool inside = nx >= 0 && nx < p.grid_res[0] && ny >= 0 && ny < p.grid_res[1] && nz >= 0 &&
+                        nz < p.grid_res[2];
+    const uint32_t cx = uint32_t(min(max(nx, 0), p.grid_res[0] - 1));
+    const uint32_t cy = uint32_t(min(max(ny, 0), p.grid_res[1] - 1));
+    const uint32_t cz = uint32_t(min(max(nz, 0), p.grid_res[2] - 1));
+    const float4 g = grid[morton3(cx, cy, cz)];
+    return inside ? g : make_float4(0.f, 0.f, 0.f, 0.f);
+  };
 
   float vpx = 0.f, vpy = 0.f, vpz = 0.f, vox = 0.f, voy = 0.f, voz = 0.f;
   m3 vg = m3_zero(), Bn = m3_zero();
+  // One z slab of 9 nodes per iteration (the loop is kept rolled so only one
+  // slab's loads are live); the slab's z stencil values sit in position 0 of
+  // zw/zg/zd/zwd and are rotated at the end of the iteration.
+  const Axis3 &X = ax3[0], &Y = ax3[1];
+  float zw[3] = {ax3[2].w[0], ax3[2].w[1], ax3[2].w[2]};
+  float zg[3] = {ax3[2].g[0], ax3[2].g[1], ax3[2].g[2]};
+  float zd[3] = {ax3[2].dx[0], ax3[2].dx[1], ax3[2].dx[2]};
+  if (!__any(tiny)) {
+    // axis-by-axis contraction: x, then y, then z
+    float wdx[3], wdy[3];
 #pragma unroll
-  for (int iz = 0; iz < 3; iz++) {
-    const int nz = base[2] + iz;
-    if (nz < 0 || nz >= p.grid_res[2]) continue;
+    for (int i = 0; i < 3; i++) {
+      wdx[i] = X.w[i] * X.dx[i];
+      wdy[i] = Y.w[i] * Y.dx[i];
+    }
+#pragma unroll 1
+    for (int iz = 0; iz < 3; iz++) {
+      float Y00[3] = {0.f, 0.f, 0.f}, Y01[3] = {0.f, 0.f, 0.f}, Y02[3] = {0.f, 0.f, 0.f};
+      float Y1[3] = {0.f, 0.f, 0.f}, Y2[3] = {0.f, 0.f, 0.f}, YO[3] = {0.f, 0.f, 0.f};
 #pragma unroll
-    for (int iy = 0; iy < 3; iy++) {
-      const int ny = base[1] + iy;
-      if (ny < 0 || ny >= p.grid_res[1]) continue;
+      for (int iy = 0; iy < 3; iy++) {
+        float X0[3] = {0.f, 0.f, 0.f}, X1[3] = {0.f, 0.f, 0.f}, X2[3] = {0.f, 0.f, 0.f}, XO[3] = {0.f, 0.f, 0.f};
 #pragma unroll
-      for (int ix = 0; ix < 3; ix++) {
-        const int nx = base[0] + ix;
-        if (nx < 0 || nx >= p.grid_res[0]) continue;
-        const uint32_t nid = morton3(nx, ny, nz);
-        const float4 g = grid_node[nid];
-        if (g.w < 1e-10f) continue;
-        const float wx = ax3[0].w[ix], wy = ax3[1].w[iy], wz = ax3[2].w[iz];
-        const float w = wx * wy * wz;
-        if (w < 1e-12f) continue;
-        const float dwx = ax3[0].g[ix] * wy * wz;
-        const float dwy = wx * ax3[1].g[iy] * wz;
-        const float dwz = wx * wy * ax3[2].g[iz];
-        const float dxx = ax3[0].dx[ix], dxy = ax3[1].dx[iy], dxz = ax3[2].dx[iz];
-        vpx += w * g.x; vpy += w * g.y; vpz += w * g.z;
-        // outerProduct(vi, dw)[j][i] = vi[i]*dw[j]
-        vg.m[0][0] += g.x * dwx; vg.m[0][1] += g.y * dwx; vg.m[0][2] += g.z * dwx;
-        vg.m[1][0] += g.x * dwy; vg.m[1][1] += g.y * dwy; vg.m[1][2] += g.z * dwy;
-        vg.m[2][0] += g.x * dwz; vg.m[2][1] += g.y * dwz; vg.m[2][2] += g.z * dwz;
-        const float wgx = w * g.x, wgy = w * g.y, wgz = w * g.z;
-        Bn.m[0][0] += wgx * dxx; Bn.m[0][1] += wgy * dxx; Bn.m[0][2] += wgz * dxx;
-        Bn.m[1][0] += wgx * dxy; Bn.m[1][1] += wgy * dxy; Bn.m[1][2] += wgz * dxy;
-        Bn.m[2][0] += wgx * dxz; Bn.m[2][1] += wgy * dxz; Bn.m[2][2] += wgz * dxz;
-        if (flip) {
-          const float4 o = grid_old[nid];
-          vox += w * o.x; voy += w * o.y; voz += w * o.z;
+        for (int ix = 0; ix < 3; ix++) {
+          const float4 g = node_vel(grid_node, ix, iy, iz);
+          const bool live = g.w >= 1e-10f;
+          const float gv[3] = {live ? g.x : 0.f, live ? g.y : 0.f, live ? g.z : 0.f};
+#pragma unroll
+          for (int c = 0; c < 3; c++) {
+            X0[c] += X.w[ix] * gv[c];
+            X1[c] += X.g[ix] * gv[c];
+            X2[c] += wdx[ix] * gv[c];
+          }
+          if (flip) {
+            const float4 o = node_vel(grid_old, ix, iy, iz);
+            XO[0] += X.w[ix] * (live ? o.x : 0.f);
+            XO[1] += X.w[ix] * (live ? o.y : 0.f);
+            XO[2] += X.w[ix] * (live ? o.z : 0.f);
+          }
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          Y00[c] += Y.w[iy] * X0[c];
+          Y01[c] += Y.g[iy] * X0[c];
+          Y02[c] += wdy[iy] * X0[c];
+          Y1[c] += Y.w[iy] * X1[c];
+          Y2[c] += Y.w[iy] * X2[c];
+          YO[c] += Y.w[iy] * XO[c];
         }
       }
+      const float wz = zw[0], gz = zg[0], wdz = zw[0] * zd[0];
+      float* vp3[3] = {&vpx, &vpy, &vpz};
+      float* vo3[3] = {&vox, &voy, &voz};
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        *vp3[c] += wz * Y00[c];
+        *vo3[c] += wz * YO[c];
+        // vg = sum outer(v_i, grad w_i): vg.m[col = grad axis][row = velocity component]
+        vg.m[0][c] += wz * Y1[c];
+        vg.m[1][c] += wz * Y01[c];
+        vg.m[2][c] += gz * Y00[c];
+        // Bn = sum outer(w_i v_i, x_i - x_p)
+        Bn.m[0][c] += wz * Y2[c];
+        Bn.m[1][c] += wz * Y02[c];
+        Bn.m[2][c] += wdz * Y00[c];
+      }
+      zw[0] = zw[1]; zw[1] = zw[2];
+      zg[0] = zg[1]; zg[1] = zg[2];
+      zd[0] = zd[1]; zd[1] = zd[2];
+    }
+  } else {
+#pragma unroll 1
+    for (int iz = 0; iz < 3; iz++) {
+      const float wz = zw[0], gz = zg[0], dxz = zd[0];
+#pragma unroll
+      for (int iy = 0; iy < 3; iy++) {
+#pragma unroll
+        for (int ix = 0; ix < 3; ix++) {
+          const float4 g = node_vel(grid_node, ix, iy, iz);
+          const float wx = X.w[ix], wy = Y.w[iy];
+          const float w0 = wx * wy * wz;
+          const bool live = g.w >= 1e-10f && w0 >= 1e-12f;
+          const float w = live ? w0 : 0.f;
+          const float dwx = live ? X.g[ix] * wy * wz : 0.f;
+          const float dwy = live ? wx * Y.g[iy] * wz : 0.f;
+          const float dwz = live ? wx * wy * gz : 0.f;
+          const float dxx = X.dx[ix], dxy = Y.dx[iy];
+          vpx += w * g.x; vpy += w * g.y; vpz += w * g.z;
+          // outerProduct(vi, dw)[j][i] = vi[i]*dw[j]
+          vg.m[0][0] += g.x * dwx; vg.m[0][1] += g.y * dwx; vg.m[0][2] += g.z * dwx;
+          vg.m[1][0] += g.x * dwy; vg.m[1][1] += g.y * dwy; vg.m[1][2] += g.z * dwy;
+          vg.m[2][0] += g.x * dwz; vg.m[2][1] += g.y * dwz; vg.m[2][2] += g.z * dwz;
+          const float wgx = w * g.x, wgy = w * g.y, wgz = w * g.z;
+          Bn.m[0][0] += wgx * dxx; Bn.m[0][1] += wgy * dxx; Bn.m[0][2] += wgz * dxx;
+          Bn.m[1][0] += wgx * dxy; Bn.m[1][1] += wgy * dxy; Bn.m[1][2] += wgz * dxy;
+          Bn.m[2][0] += wgx * dxz; Bn.m[2][1] += wgy * dxz; Bn.m[2][2] += wgz * dxz;
+          if (flip) {
+            const float4 o = node_vel(grid_old, ix, iy, iz);
+            vox += w * o.x; voy += w * o.y; voz += w * o.z;
+          }
+        }
+      }
+      zw[0] = zw[1]; zw[1] = zw[2];
+      zg[0] = zg[1]; zg[1] = zg[2];
+      zd[0] = zd[1]; zd[1] = zd[2];
     }
   }
 
-  const float4 vel = V[q];
+  const float4 vel = r[kV];
   uint32_t mid = __float_as_uint(vel.w);
   mid = mid > p.material_count - 1u ? p.material_count - 1u : mid;
   const nb_material mat = mats[mid];
 
-  const float4 f0 = F0[q], f1 = F1[q], f2 = F2[q];
+  const float4 f0 = r[kF0], f1 = r[kF1], f2 = r[kF2];
   m3 F;
   F.m[0][0] = f0.x; F.m[0][1] = f0.y; F.m[0][2] = f0.z;
   F.m[1][0] = f1.x; F.m[1][1] = f1.y; F.m[1][2] = f1.z;
@@ -680,28 +898,33 @@ __global__ __launch_bounds__(256) void k_g2p(SubParams p, float4* __restrict__ P
     Fn = m3_diag(cr, cr, cr);
     const float pr = mat.bulkK * (J - 1.f);
     tau = m3_diag(pr, pr, pr);
-  } else if (mat.model == NB_MAT_VON_MISES) {
-    Fn = project_von_mises(Fn, mat.mu, mat.q_max, cs);
-    tau = hencky_stress(Fn, mat.mu, mat.lambda, cs);
-  } else if (mat.model == NB_MAT_DRUCKER_PRAGER || mat.model == NB_MAT_GRANULAR_POWDER) {
-    Fn = project_drucker_prager(Fn, mat.mu, mat.lambda, mat.M_friction, mat.q_cohesion, cs);
-    tau = hencky_stress(Fn, mat.mu, mat.lambda, cs);
-  } else if (mat.model == NB_MAT_VISCOPLASTIC_MUD) {
-    Fn = project_drucker_prager(Fn, mat.mu, mat.lambda, mat.M_friction, mat.q_cohesion, cs);
-    tau = hencky_stress(Fn, mat.mu, mat.lambda, cs);
-    m3 D;
-#pragma unroll
-    for (int c = 0; c < 3; c++)
-#pragma unroll
-      for (int r = 0; r < 3; r++) D.m[c][r] = 0.5f * (vg.m[c][r] + vg.m[r][c]);
-    const float trD = D.m[0][0] + D.m[1][1] + D.m[2][2];
-    const float k2 = 2.f * mat.viscosity;
-#pragma unroll
-    for (int c = 0; c < 3; c++)
-#pragma unroll
-      for (int r = 0; r < 3; r++) tau.m[c][r] += k2 * (D.m[c][r] - (c == r ? trD / 3.f : 0.f));
   } else {
-    tau = hencky_stress(Fn, mat.mu, mat.lambda, cs);
+    // elastic: Hencky stress of F; Von Mises / Drucker-Prager (also granular
+    // powder and viscoplastic mud): return map, then Hencky stress of the
+    // projected F.  In corrected mode the return map's SVD is the true SVD of
+    // the projected F, so its factors give the stress without a second SVD.
+    const bool dp = mat.model == NB_MAT_DRUCKER_PRAGER || mat.model == NB_MAT_GRANULAR_POWDER ||
+                    mat.model == NB_MAT_VISCOPLASTIC_MUD;
+    const bool plastic = dp || mat.model == NB_MAT_VON_MISES;
+    if (plastic) {
+      const PlasticOut po = project_plastic(Fn, dp, mat.mu, mat.lambda, mat.q_max, mat.M_friction, mat.q_cohesion, cs);
+      Fn = po.F;
+      if (cs) tau = hencky_from_log(po.U, po.eps[0], po.eps[1], po.eps[2], mat.mu, mat.lambda);
+    }
+    if (!(plastic && cs)) tau = hencky_stress(Fn, mat.mu, mat.lambda, cs);
+    if (mat.model == NB_MAT_VISCOPLASTIC_MUD) {
+      m3 D;
+#pragma unroll
+      for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int r = 0; r < 3; r++) D.m[c][r] = 0.5f * (vg.m[c][r] + vg.m[r][c]);
+      const float trD = D.m[0][0] + D.m[1][1] + D.m[2][2];
+      const float k2 = 2.f * mat.viscosity;
+#pragma unroll
+      for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int r = 0; r < 3; r++) tau.m[c][r] += k2 * (D.m[c][r] - (c == r ? trD / 3.f : 0.f));
+    }
   }
 
   bool jreset = false;
@@ -768,14 +991,15 @@ __global__ __launch_bounds__(256) void k_g2p(SubParams p, float4* __restrict__ P
     if (zn < loz) { zn = loz; vnz = fmaxf(vnz, 0.f); }
     if (zn > hiz) { zn = hiz; vnz = fminf(vnz, 0.f); }
   }
-  P[q] = make_float4(xn, yn, zn, Vp);
-  V[q] = make_float4(vnx, vny, vnz, vel.w);
-  F0[q] = make_float4(Fn.m[0][0], Fn.m[0][1], Fn.m[0][2], tau.m[0][0]);
-  F1[q] = make_float4(Fn.m[1][0], Fn.m[1][1], Fn.m[1][2], tau.m[1][1]);
-  F2[q] = make_float4(Fn.m[2][0], Fn.m[2][1], Fn.m[2][2], tau.m[2][2]);
-  B0[q] = make_float4(Bn.m[0][0], Bn.m[0][1], Bn.m[0][2], tau.m[0][1]);
-  B1[q] = make_float4(Bn.m[1][0], Bn.m[1][1], Bn.m[1][2], tau.m[0][2]);
-  B2[q] = make_float4(Bn.m[2][0], Bn.m[2][1], Bn.m[2][2], tau.m[1][2]);
+  r[kP] = make_float4(xn, yn, zn, Vp);
+  r[kV] = make_float4(vnx, vny, vnz, vel.w);
+  r[kF0] = make_float4(Fn.m[0][0], Fn.m[0][1], Fn.m[0][2], tau.m[0][0]);
+  r[kF1] = make_float4(Fn.m[1][0], Fn.m[1][1], Fn.m[1][2], tau.m[1][1]);
+  r[kF2] = make_float4(Fn.m[2][0], Fn.m[2][1], Fn.m[2][2], tau.m[2][2]);
+  r[kB0] = make_float4(Bn.m[0][0], Bn.m[0][1], Bn.m[0][2], tau.m[0][1]);
+  r[kB1] = make_float4(Bn.m[1][0], Bn.m[1][1], Bn.m[1][2], tau.m[0][2]);
+  r[kB2] = make_float4(Bn.m[2][0], Bn.m[2][1], Bn.m[2][2], tau.m[1][2]);
+  keys[q] = cell_key(p, xn, yn, zn);  // the next substep's bin key
 }
 
 // ---------------------------------------------------------------------------
@@ -815,6 +1039,20 @@ __global__ void k_init_FB(float4* F0, float4* F1, float4* F2, float4* B0, float4
   B2[q] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
+// reference layout (eight float4 arrays) <-> one 128-byte record per particle
+__global__ __launch_bounds__(256) void k_soa_to_aos(FieldPtrs f, float4* __restrict__ rec, uint32_t n) {
+  const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= uint64_t(n) * kRec) return;
+  const uint32_t i = uint32_t(t >> 3), c = uint32_t(t & 7u);
+  rec[t] = f.f[c][i];
+}
+__global__ __launch_bounds__(256) void k_aos_to_soa(const float4* __restrict__ rec, FieldPtrs f, uint32_t n) {
+  const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= uint64_t(n) * kRec) return;
+  const uint32_t i = uint32_t(t >> 3), c = uint32_t(t & 7u);
+  f.f[c][i] = rec[t];
+}
+
 // plain streaming copy (HBM byte-counter calibration)
 __global__ __launch_bounds__(256) void k_copy(float4* __restrict__ dst, const float4* __restrict__ src, uint64_t n4) {
   for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n4; i += uint64_t(gridDim.x) * blockDim.x)
@@ -843,10 +1081,16 @@ struct ProfScope {
   }
 };
 
-void launch_bin(const DevState& d, const SubParams& p, uint32_t n_tiles, hipStream_t s, Prof* pf) {
+void launch_bin(const DevState& d, const SubParams& p, uint32_t n_tiles, bool keys_valid, hipStream_t s, Prof* pf) {
   const uint32_t tg = tile_grid(n_tiles);
   { ProfScope ps(pf, 0, s); k_bin_clear<<<tg, 512, 0, s>>>(d.cell_count, d.act_tiles, d.counters); }
-  { ProfScope ps(pf, 1, s); k_bin_keys<<<cdiv(p.n, 256), 256, 0, s>>>(p, d.field[NB_FIELD_P], d.keys, d.rank, d.cell_count, d.tile_flag); }
+  {
+    ProfScope ps(pf, 1, s);
+    if (keys_valid)
+      k_bin_keys<false><<<cdiv(p.n, 256), 256, 0, s>>>(p, d.rec, d.keys, d.rank, d.cell_count, d.tile_flag);
+    else
+      k_bin_keys<true><<<cdiv(p.n, 256), 256, 0, s>>>(p, d.rec, d.keys, d.rank, d.cell_count, d.tile_flag);
+  }
   { ProfScope ps(pf, 2, s); k_bin_mark<<<cdiv(n_tiles, 256), 256, 0, s>>>(d.tile_flag, d.tile_mark, p.tile_dim, n_tiles); }
   { ProfScope ps(pf, 3, s); k_bin_compact<<<1, 1024, 0, s>>>(d.tile_mark, d.tile_flag, d.tile_act, d.act_tiles, d.node_tiles, d.counters, n_tiles); }
   { ProfScope ps(pf, 4, s); k_bin_tile_scan<<<tg, 512, 0, s>>>(d.cell_count, d.cell_end, d.act_tiles, d.tile_total, d.counters); }
@@ -857,9 +1101,8 @@ void launch_bin(const DevState& d, const SubParams& p, uint32_t n_tiles, hipStre
 
 void launch_p2g(const DevState& d, const SubParams& p, uint32_t n_tiles, hipStream_t s, Prof* pf) {
   ProfScope ps(pf, 8, s);
-  k_p2g<<<tile_grid(n_tiles), 512, 0, s>>>(p, d.field[0], d.field[1], d.field[2], d.field[3], d.field[4], d.field[5],
-                                           d.field[6], d.field[7], d.materials, d.cell_count, d.cell_end, d.sorted,
-                                           d.act_tiles, d.counters, d.partial);
+  k_p2g<<<tile_grid(n_tiles), kP2GThreads, 0, s>>>(p, d.rec, d.materials, d.cell_count, d.cell_end, d.sorted,
+                                                   d.act_tiles, d.counters, d.partial);
 }
 
 void launch_grid(const DevState& d, const SubParams& p, uint32_t n_tiles, bool from_partials, bool update,
@@ -876,9 +1119,11 @@ void launch_grid(const DevState& d, const SubParams& p, uint32_t n_tiles, bool f
 
 void launch_g2p(const DevState& d, const SubParams& p, bool use_sorted, hipStream_t s, Prof* pf) {
   ProfScope ps(pf, 10, s);
-  k_g2p<<<cdiv(p.n, 256), 256, 0, s>>>(p, d.field[0], d.field[1], d.field[2], d.field[3], d.field[4], d.field[5],
-                                       d.field[6], d.field[7], d.materials, use_sorted ? d.sorted : nullptr,
-                                       d.grid_node, d.grid_old, d.sdf);
+  const uint32_t* srt = use_sorted ? d.sorted : nullptr;
+  if (p.flip_ratio > 0.01f)
+    k_g2p<true><<<cdiv(p.n, 256), 256, 0, s>>>(p, d.rec, d.keys, d.materials, srt, d.grid_node, d.grid_old, d.sdf);
+  else
+    k_g2p<false><<<cdiv(p.n, 256), 256, 0, s>>>(p, d.rec, d.keys, d.materials, srt, d.grid_node, d.grid_old, d.sdf);
 }
 
 void launch_sdf_sphere(float* sdf, const SubParams& p, float r, float cx, float cy, float cz, hipStream_t s) {
@@ -887,6 +1132,22 @@ void launch_sdf_sphere(float* sdf, const SubParams& p, float r, float cx, float 
 }
 
 void launch_fill_identity(uint32_t* a, uint32_t n, hipStream_t s) { k_fill_identity<<<cdiv(n, 256), 256, 0, s>>>(a, n); }
+
+void launch_soa_to_aos(const DevState& d, uint32_t n, hipStream_t s) {
+  if (!n) return;
+  FieldPtrs f;
+  for (int i = 0; i < NB_NUM_FIELDS; i++) f.f[i] = d.field[i];
+  const uint64_t t = uint64_t(n) * kRec;
+  k_soa_to_aos<<<uint32_t((t + 255) / 256), 256, 0, s>>>(f, d.rec, n);
+}
+
+void launch_aos_to_soa(const DevState& d, uint32_t n, hipStream_t s) {
+  if (!n) return;
+  FieldPtrs f;
+  for (int i = 0; i < NB_NUM_FIELDS; i++) f.f[i] = d.field[i];
+  const uint64_t t = uint64_t(n) * kRec;
+  k_aos_to_soa<<<uint32_t((t + 255) / 256), 256, 0, s>>>(d.rec, f, n);
+}
 
 void launch_copy(float4* dst, const float4* src, uint64_t n4, hipStream_t s) {
   k_copy<<<8192, 256, 0, s>>>(dst, src, n4);

@@ -92,22 +92,18 @@ __host__ __device__ __forceinline__ uint32_t morton3(uint32_t x, uint32_t y, uin
   return morton_expand(x) | (morton_expand(y) << 1u) | (morton_expand(z) << 2u);
 }
 
-// ---- quadratic B-spline (shaders/mpm_common.glsl:198-210) ----
-__device__ __forceinline__ float bspline2(float d) {
-  float ad = fabsf(d);
-  if (ad < 0.5f) return 0.75f - ad * ad;
-  if (ad < 1.5f) {
-    float t = 1.5f - ad;
-    return 0.5f * t * t;
-  }
-  return 0.f;
-}
-__device__ __forceinline__ float bspline2g(float d) {
-  float ad = fabsf(d);
-  if (ad < 0.5f) return -2.f * d;
-  if (ad < 1.5f) return -glsl_sign(d) * (1.5f - ad);
-  return 0.f;
-}
+// ---- hardware reciprocal, square root and transcendentals ----
+// One VALU instruction each, accurate to about 1 ulp (log/exp: base-2
+// instructions with the base change folded into one multiply).  GLSL only
+// bounds division (2.5 ulp), sqrt/inversesqrt, log and exp (3 ulp; exp
+// 3 + 2|x| ulp) by ulps, so the reference's shaders run with accuracy of this
+// order.  Cell keys and stencil bases keep the IEEE quotient: they must land in
+// the same cell as the reference's.
+__device__ __forceinline__ float hw_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float hw_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float hw_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float hw_log(float x) { return __builtin_amdgcn_logf(x) * 0.693147180559945309f; }
+__device__ __forceinline__ float hw_exp(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504088896341f); }
 
 // ---- Jacobi eigen-decomposition of symmetric A (shaders/mpm_common.glsl:213-247) ----
 // Same pivot rule, rotation and stopping test as the GLSL.  The matrix is held
@@ -128,9 +124,11 @@ __device__ __forceinline__ void jacobi_eigen3(float a00, float a11, float a22, f
     float apq = (cs == 0) ? a01 : ((cs == 1) ? a02 : a12);
     float arp = (cs == 0) ? a02 : a01;  // A[r][p0]
     float arq = (cs == 2) ? a02 : a12;  // A[r][p1]
-    float tau = (aqq - app) / (2.f * apq);
-    float t = glsl_sign(tau) / (fabsf(tau) + sqrtf(1.f + tau * tau));
-    float c = 1.f / sqrtf(1.f + t * t);
+    // tau = (aqq - app) / (2 apq); t = sign(tau) / (|tau| + sqrt(1 + tau^2));
+    // c = 1 / sqrt(1 + t^2); s = t c
+    float tau = (aqq - app) * hw_rcp(2.f * apq);
+    float t = glsl_sign(tau) * hw_rcp(fabsf(tau) + hw_sqrt(1.f + tau * tau));
+    float c = hw_rsq(1.f + t * t);
     float s = t * c;
     float napp = app - t * apq, naqq = aqq + t * apq;
     float nrp = c * arp - s * arq;
@@ -172,9 +170,9 @@ __device__ __forceinline__ void svd3(const m3& F, m3& U, float sig[3], m3& V, bo
   float f12 = F.m[1][0] * F.m[2][0] + F.m[1][1] * F.m[2][1] + F.m[1][2] * F.m[2][2];
   float D[3];
   jacobi_eigen3(f00, f11, f22, f01, f02, f12, D, V);
-  sig[0] = sqrtf(fmaxf(D[0], 0.f));
-  sig[1] = sqrtf(fmaxf(D[1], 0.f));
-  sig[2] = sqrtf(fmaxf(D[2], 0.f));
+  sig[0] = hw_sqrt(fmaxf(D[0], 0.f));
+  sig[1] = hw_sqrt(fmaxf(D[1], 0.f));
+  sig[2] = hw_sqrt(fmaxf(D[2], 0.f));
   // selection sort, descending; swap V columns i<->j (GLSL V[0][i] = column 0
   // row i: it swaps ROWS of our m[col][row] storage at index i, j)
 #define NB_SWAP_SV(i, j)                                                        \
@@ -202,10 +200,10 @@ __device__ __forceinline__ void svd3(const m3& F, m3& U, float sig[3], m3& V, bo
     const float fv1 = F.m[0][1] * v0 + F.m[1][1] * v1 + F.m[2][1] * v2;
     const float fv2 = F.m[0][2] * v0 + F.m[1][2] * v1 + F.m[2][2] * v2;
     const bool big = sig[i] > 1e-8f;
-    const float inv_ok = big ? sig[i] : 1.f;
-    U.m[0][i] = big ? fv0 / inv_ok : (i == 0 ? 1.f : 0.f);
-    U.m[1][i] = big ? fv1 / inv_ok : (i == 1 ? 1.f : 0.f);
-    U.m[2][i] = big ? fv2 / inv_ok : (i == 2 ? 1.f : 0.f);
+    const float inv = hw_rcp(big ? sig[i] : 1.f);
+    U.m[0][i] = big ? fv0 * inv : (i == 0 ? 1.f : 0.f);
+    U.m[1][i] = big ? fv1 * inv : (i == 1 ? 1.f : 0.f);
+    U.m[2][i] = big ? fv2 * inv : (i == 2 ? 1.f : 0.f);
   }
   if (m3_det(U) < 0.f) {
     U.m[0][2] = -U.m[0][2];
@@ -219,78 +217,82 @@ __device__ __forceinline__ void svd3(const m3& F, m3& U, float sig[3], m3& V, bo
   }
 }
 
-
 // ---- Hencky Kirchhoff stress (shaders/mpm_common.glsl:284-293) ----
-// tau = U diag(lambda tr(eps) + 2 mu eps) U^T with U as the GLSL stores it.
-__device__ __forceinline__ m3 hencky_from_svd(const m3& U, const float sig[3], float mu, float lam) {
-  float s0 = fmaxf(fabsf(sig[0]), 1e-6f), s1 = fmaxf(fabsf(sig[1]), 1e-6f), s2 = fmaxf(fabsf(sig[2]), 1e-6f);
-  float e0 = logf(s0), e1 = logf(s1), e2 = logf(s2);
-  float tr = e0 + e1 + e2;
-  float lt = lam * tr, tm = 2.f * mu;
+// tau = U diag(lambda tr(eps) + 2 mu eps) U^T with U as the GLSL stores it;
+// eps_i = log(max(|sigma_i|, 1e-6)).
+__device__ __forceinline__ m3 hencky_from_log(const m3& U, float e0, float e1, float e2, float mu, float lam) {
+  const float lt = lam * (e0 + e1 + e2), tm = 2.f * mu;
   return m3_udvt(U, lt + tm * e0, lt + tm * e1, lt + tm * e2, U);
 }
+__device__ __forceinline__ float log_strain(float s) { return hw_log(fmaxf(fabsf(s), 1e-6f)); }
 __device__ __forceinline__ m3 hencky_stress(const m3& F, float mu, float lam, bool corrected) {
   m3 U, V;
   float sig[3];
   svd3(F, U, sig, V, corrected);
-  return hencky_from_svd(U, sig, mu, lam);
+  return hencky_from_log(U, log_strain(sig[0]), log_strain(sig[1]), log_strain(sig[2]), mu, lam);
 }
 
-// ---- Von Mises radial return (shaders/mpm_g2p.comp:19-38) ----
-__device__ __forceinline__ m3 project_von_mises(const m3& F, float mu, float q_max, bool corrected) {
+// ---- plastic return maps (shaders/mpm_g2p.comp:19-68) ----
+// One SVD serves both maps.  Von Mises (dp = false) scales the deviatoric log
+// strain back to the yield radius q_max / 2mu and keeps F untouched when
+// inside it; Drucker-Prager (dp = true) projects onto the cone
+// q <= M p + q_c (apex: volumetric strain only) and always recomposes F.
+// Returns the projected F; eps[] receives its log strains and U its left
+// singular vectors, which give the Hencky stress of the result directly when
+// U, V are the true SVD factors (corrected mode).
+struct PlasticOut {
+  m3 F;
+  m3 U;
+  float eps[3];
+};
+__device__ __forceinline__ PlasticOut project_plastic(const m3& F, bool dp, float mu, float lam, float q_max, float M,
+                                                      float q_c, bool corrected) {
   m3 U, V;
   float sig[3];
   svd3(F, U, sig, V, corrected);
-  float e0 = logf(fmaxf(fabsf(sig[0]), 1e-6f));
-  float e1 = logf(fmaxf(fabsf(sig[1]), 1e-6f));
-  float e2 = logf(fmaxf(fabsf(sig[2]), 1e-6f));
-  float tr = e0 + e1 + e2;
-  float m = tr / 3.f;
+  float e0 = log_strain(sig[0]), e1 = log_strain(sig[1]), e2 = log_strain(sig[2]);
+  const float tr = e0 + e1 + e2;
+  const float m = tr / 3.f;
   float d0 = e0 - m, d1 = e1 - m, d2 = e2 - m;
-  float nd = sqrtf(d0 * d0 + d1 * d1 + d2 * d2);
-  float delta = nd - q_max / (2.f * mu);
-  if (delta > 0.f && nd > 1e-8f) {
-    float k = delta / nd;
-    d0 -= k * d0;
-    d1 -= k * d1;
-    d2 -= k * d2;
-    return m3_udvt(U, expf(d0 + m), expf(d1 + m), expf(d2 + m), V);
-  }
-  return F;
-}
-
-// ---- Drucker-Prager return map (shaders/mpm_g2p.comp:41-68): always recomposes ----
-__device__ __forceinline__ m3 project_drucker_prager(const m3& F, float mu, float lam, float M, float q_c,
-                                                    bool corrected) {
-  m3 U, V;
-  float sig[3];
-  svd3(F, U, sig, V, corrected);
-  float e0 = logf(fmaxf(fabsf(sig[0]), 1e-6f));
-  float e1 = logf(fmaxf(fabsf(sig[1]), 1e-6f));
-  float e2 = logf(fmaxf(fabsf(sig[2]), 1e-6f));
-  float tr = e0 + e1 + e2;
-  float m = tr / 3.f;
-  float d0 = e0 - m, d1 = e1 - m, d2 = e2 - m;
-  float nd = sqrtf(d0 * d0 + d1 * d1 + d2 * d2);
-  float K = lam + 2.f * mu / 3.f;
-  float p = -K * tr;
-  float q_yield = M * p + q_c;
-  if (q_yield <= 1e-8f) {
-    float p_proj = -q_c / fmaxf(M, 1e-8f);
-    float tr2 = -p_proj / K;
-    float m2 = tr2 / 3.f;
-    e0 = m2; e1 = m2; e2 = m2;
-  } else {
-    float delta = nd - q_yield / (2.f * mu);
-    if (delta > 0.f && nd > 1e-8f) {
-      float k = delta / nd;
-      d0 -= k * d0;
-      d1 -= k * d1;
-      d2 -= k * d2;
-      e0 = d0 + m; e1 = d1 + m; e2 = d2 + m;
+  const float nd = hw_sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+  bool recompose = dp;
+  float q_lim = q_max;
+  bool apex = false;
+  if (dp) {
+    const float K = lam + 2.f * mu / 3.f;
+    const float q_yield = M * (-K * tr) + q_c;
+    apex = q_yield <= 1e-8f;
+    q_lim = q_yield;
+    if (apex) {
+      const float m2 = (q_c / fmaxf(M, 1e-8f) / K) / 3.f;  // tr2 = -p_proj / K, p_proj = -q_c / M
+      e0 = m2; e1 = m2; e2 = m2;
     }
   }
-  return m3_udvt(U, expf(e0), expf(e1), expf(e2), V);
+  if (!apex) {
+    const float delta = nd - q_lim / (2.f * mu);
+    if (delta > 0.f && nd > 1e-8f) {
+      const float k = delta * hw_rcp(nd);
+      e0 = d0 - k * d0 + m;
+      e1 = d1 - k * d1 + m;
+      e2 = d2 - k * d2 + m;
+      recompose = true;
+    }
+  }
+  PlasticOut o;
+  o.U = U;
+  if (recompose) {
+    const float x0 = hw_exp(e0), x1 = hw_exp(e1), x2 = hw_exp(e2);
+    o.F = m3_udvt(U, x0, x1, x2, V);
+    o.eps[0] = log_strain(x0);
+    o.eps[1] = log_strain(x1);
+    o.eps[2] = log_strain(x2);
+  } else {
+    o.F = F;
+    o.eps[0] = e0;
+    o.eps[1] = e1;
+    o.eps[2] = e2;
+  }
+  return o;
 }
 
 }  // namespace nb

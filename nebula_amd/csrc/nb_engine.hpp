@@ -3,7 +3,10 @@
 // translation unit (mpm_kernels.hip).
 //
 // HBM layout (one GPU):
-//   particles  8 x float4 x capacity, the reference's SoA (P, v, F0-2, B0-2)
+//   particles  8 x float4 x capacity, the reference's SoA (P, v, F0-2, B0-2),
+//              which the ABI exposes, and the working copy the kernels use:
+//              one 128-byte record per particle (the same eight float4 side
+//              by side), converted at the API boundary
 //   bins       keys[N], rank[N], sorted[N]; cell_count/cell_end dense over the
 //              Morton cube (only occupied 8^3 tiles are ever touched)
 //   tiles      the Morton cube cut into 8^3-cell tiles (tile id = key >> 9):
@@ -25,6 +28,13 @@ constexpr int kTileBits = 9;          // 8^3 cells per tile
 constexpr int kTileCells = 512;
 constexpr int kBox = 10;              // node box edge of a tile (8 + 2 halo)
 constexpr int kBoxNodes = 1000;
+
+// interleaved particle record: the eight float4 of the reference's layout
+constexpr int kRec = 8;
+enum : int { kP = 0, kV = 1, kF0 = 2, kF1 = 3, kF2 = 4, kB0 = 5, kB1 = 6, kB2 = 7 };
+struct FieldPtrs {
+  float4* f[NB_NUM_FIELDS];
+};
 
 // counters[] slots (device)
 enum : int { CNT_ACTIVE = 0, CNT_NODE_TILES = 1, CNT_ALL_TILES = 2, CNT_SLOTS = 4 };
@@ -49,7 +59,8 @@ struct SubParams {
 };
 
 struct DevState {
-  float4* field[NB_NUM_FIELDS];
+  float4* field[NB_NUM_FIELDS];  // reference layout, what the ABI exposes
+  float4* rec;                   // capacity * kRec: working particle records
   uint32_t* keys;
   uint32_t* rank;
   uint32_t* sorted;
@@ -93,7 +104,8 @@ struct Prof {
 };
 
 // ---- launchers (mpm_kernels.hip); pf may be null ----
-void launch_bin(const DevState& d, const SubParams& p, uint32_t n_tiles, hipStream_t s, Prof* pf);
+// keys_valid: the previous G2P wrote every particle's key (else compute them)
+void launch_bin(const DevState& d, const SubParams& p, uint32_t n_tiles, bool keys_valid, hipStream_t s, Prof* pf);
 void launch_p2g(const DevState& d, const SubParams& p, uint32_t n_tiles, hipStream_t s, Prof* pf);
 // Sum partials into grid_node and apply the grid update (+ dense-SDF BC).
 // update=false only sums (pass-level P2G export).  tiles/count select the
@@ -106,5 +118,7 @@ void launch_sdf_sphere(float* sdf, const SubParams& p, float r, float cx, float 
 void launch_fill_identity(uint32_t* a, uint32_t n, hipStream_t s);
 void launch_init_particles(const DevState& d, uint32_t first, uint32_t count, hipStream_t s);
 void launch_copy(float4* dst, const float4* src, uint64_t n4, hipStream_t s);
+void launch_soa_to_aos(const DevState& d, uint32_t n, hipStream_t s);
+void launch_aos_to_soa(const DevState& d, uint32_t n, hipStream_t s);
 
 }  // namespace nb

@@ -472,6 +472,7 @@ def test_elastic_block_trajectory_1000_substeps_corrected(gpu_lib):
 
 
 def test_substeps_are_deterministic(gpu_lib):
+    """Atomic-free kernels: bit-identical results run to run."""
     sc = multi_material_128()
     outs = []
     for _ in range(2):
