@@ -118,31 +118,38 @@ __global__ __launch_bounds__(256) void k_bin_keys(SubParams p, const float4* __r
   uint32_t key = 0xFFFFFFFFu;
   if (valid) {
     if (FROM_REC) {
-      const float4 x = rec[size_t(i) * kRec];
+      const float4 x = rec[size_t(i) * kRec + R_P];
       key = cell_key(p, x.x, x.y, x.z);
       keys[i] = key;
     } else {
       key = keys[i];
     }
   }
+  // group the wave's lanes by key (ballots only), then one atomic per group,
+  // all groups in the same instruction
   const uint32_t lane = threadIdx.x & 63;
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   uint64_t pending = __ballot(valid);
-  uint32_t my_rank = 0;
+  int my_leader = 0;
+  uint32_t my_pos = 0, my_cnt = 0;
   while (pending) {
     const int leader = __ffsll((unsigned long long)pending) - 1;
     const uint32_t lkey = __shfl(key, leader);
     const uint64_t peers = __ballot(valid && key == lkey);
-    uint32_t base = 0;
-    if (int(lane) == leader) {
-      base = atomicAdd(&cell_count[lkey], uint32_t(__popcll(peers)));
-      tile_flag[lkey >> kTileBits] = 1;
+    if (valid && key == lkey) {
+      my_leader = leader;
+      my_pos = uint32_t(__popcll(peers & lt_mask));
     }
-    base = __shfl(base, leader);
-    if (valid && key == lkey) my_rank = base + uint32_t(__popcll(peers & lt_mask));
+    if (int(lane) == leader) my_cnt = uint32_t(__popcll(peers));
     pending &= ~peers;
   }
-  if (valid) rank[i] = my_rank;
+  uint32_t base = 0;
+  if (valid && my_pos == 0u) {
+    base = atomicAdd(&cell_count[key], my_cnt);
+    tile_flag[key >> kTileBits] = 1;
+  }
+  base = __shfl(base, my_leader);
+  if (valid) rank[i] = base + my_pos;
 }
 
 // Per tile: bit0 = occupied, bit1 = some tile in its 3x3x3 neighbourhood is
@@ -292,10 +299,15 @@ __global__ __launch_bounds__(256) void k_bin_scatter(uint32_t n, const uint32_t*
 
 // Per cell: order its particles by index (the ranks of different waves arrive
 // in atomic order), then store the END offset, the reference's cellOffset.
-// Cells of up to 16 particles are sorted in registers (independent loads,
-// odd-even transposition network, written back only if the order changed);
-// larger cells use an in-place insertion sort.
-constexpr int kRegSort = 16;
+// Cells of up to 32 particles are sorted in registers by their thread:
+// independent loads, an odd-even transposition network that waves whose
+// cells are all in order skip, write-back only if the order changed.  Larger
+// cells (the SDF push-out piles particles up) are listed and then sorted by
+// the whole block: staged in LDS, each element's place is the number of
+// smaller elements (indices are distinct).  Cells beyond the LDS stage fall
+// back to an in-place insertion sort by their thread.
+constexpr int kRegSort = 32;
+constexpr uint32_t kBlockSort = 2048;
 
 __global__ __launch_bounds__(512) void k_bin_stabilize(uint32_t n, const uint32_t* __restrict__ cell_count,
                                                        uint32_t* __restrict__ cell_end,
@@ -303,8 +315,13 @@ __global__ __launch_bounds__(512) void k_bin_stabilize(uint32_t n, const uint32_
                                                        const uint32_t* __restrict__ tile_start,
                                                        uint32_t* __restrict__ sorted,
                                                        const uint32_t* __restrict__ counters) {
+  __shared__ uint32_t s_big_start[kTileCells], s_big_m[kTileCells];
+  __shared__ uint32_t s_vals[kBlockSort];
+  __shared__ uint32_t s_nbig;
   const uint32_t na = counters[CNT_ACTIVE];
   for (uint32_t a = blockIdx.x; a < na; a += gridDim.x) {
+    if (threadIdx.x == 0) s_nbig = 0u;
+    __syncthreads();
     const size_t key = (size_t(act_tiles[a]) << kTileBits) + threadIdx.x;
     const uint32_t start = tile_start[a] + cell_end[key];
     const uint32_t cnt = cell_count[key];
@@ -336,6 +353,10 @@ __global__ __launch_bounds__(512) void k_bin_stabilize(uint32_t n, const uint32_
         for (int i = 0; i < kRegSort; i++)
           if (uint32_t(i) < m) sorted[start + i] = v[i];
       }
+    } else if (m <= kBlockSort) {
+      const uint32_t slot = atomicAdd(&s_nbig, 1u);
+      s_big_start[slot] = start;
+      s_big_m[slot] = m;
     } else {
       for (uint32_t k = start + 1; k < stop; k++) {
         uint32_t v = sorted[k];
@@ -348,6 +369,20 @@ __global__ __launch_bounds__(512) void k_bin_stabilize(uint32_t n, const uint32_
       }
     }
     cell_end[key] = start + cnt;
+    __syncthreads();
+    const uint32_t nbig = s_nbig;
+    for (uint32_t b = 0; b < nbig; b++) {
+      const uint32_t bs = s_big_start[b], bm = s_big_m[b];
+      for (uint32_t e = threadIdx.x; e < bm; e += blockDim.x) s_vals[e] = sorted[bs + e];
+      __syncthreads();
+      for (uint32_t e = threadIdx.x; e < bm; e += blockDim.x) {
+        const uint32_t v = s_vals[e];
+        uint32_t r = 0;
+        for (uint32_t j = 0; j < bm; j++) r += s_vals[j] < v ? 1u : 0u;
+        sorted[bs + r] = v;
+      }
+      __syncthreads();
+    }
   }
 }
 
@@ -358,6 +393,12 @@ __global__ __launch_bounds__(512) void k_bin_stabilize(uint32_t n, const uint32_
 struct Axis3 {
   float w[3], g[3], dx[3];  // weight, weight derivative / h, x_node - x_p
 };
+
+// smallest positive entry (+inf if none)
+__device__ __forceinline__ float min_positive3(const float w[3]) {
+  const float a = w[0] > 0.f ? w[0] : INFINITY, b = w[1] > 0.f ? w[1] : INFINITY, c = w[2] > 0.f ? w[2] : INFINITY;
+  return fminf(a, fminf(b, c));
+}
 
 // Quadratic B-spline weight and derivative (shaders/mpm_common.glsl:198-210),
 // evaluated with selects instead of branches; inv_h = 1/h replaces the
@@ -378,13 +419,24 @@ __device__ __forceinline__ void axis_stencil(float xp, float wmin, float h, floa
 }
 
 // P2G: one block per occupied tile.  The tile's occupied cells are compacted
-// (in Morton order) onto the block's threads; a thread walks its cell's
-// particles in bin order, reading one 128-byte record each, and sums their
+// (in Morton order) onto the block's threads as work items: a cell of up to
+// kChunk particles is one item, a more crowded cell is split into up to
+// kMaxChunks items of consecutive particles (so one crowded cell does not hold
+// its wave for the whole walk).  A thread walks its item's particles in bin
+// order, reading the first 96 bytes of each record, and sums their
 // contributions to the 27 nodes the cell reaches in registers.  The sums are
-// then merged into the tile's 10^3 LDS node box in 27 barrier-separated
-// phases: within a phase every thread adds to a distinct node, so plain
-// read-add-write is exact and the order is fixed.
+// merged into the tile's 10^3 LDS node box chunk layer by chunk layer, each in
+// 27 barrier-separated phases: within a phase the writing threads hold
+// distinct cells, so every one adds to a distinct node, plain read-add-write
+// is exact and the order of the sums is fixed.
 constexpr int kP2GThreads = 256;
+constexpr uint32_t kIdxStage = 4096;  // particle indices staged in LDS per pass
+constexpr uint32_t kChunk = 12;
+constexpr uint32_t kMaxChunks = 4;
+
+__device__ __forceinline__ uint32_t p2g_chunks(uint32_t count) {
+  return count == 0u ? 0u : min((count + kChunk - 1u) / kChunk, kMaxChunks);
+}
 
 __global__ __launch_bounds__(kP2GThreads) void k_p2g(SubParams p, const float4* __restrict__ rec,
                                                      const nb_material* __restrict__ mats,
@@ -395,8 +447,13 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g(SubParams p, const float4* 
                                                      const uint32_t* __restrict__ counters,
                                                      float4* __restrict__ partial) {
   __shared__ float s_m[kBoxNodes], s_x[kBoxNodes], s_y[kBoxNodes], s_z[kBoxNodes];
-  __shared__ uint16_t s_cell[kTileCells];
+  __shared__ uint16_t s_item[kTileCells * kMaxChunks];  // cell | chunk << 9
   __shared__ uint32_t s_wave[kP2GThreads / 64 + 1];
+  __shared__ uint32_t s_layers;
+  __shared__ uint32_t s_idx[kIdxStage];
+  __shared__ uint32_t s_lo, s_hi;
+  __shared__ float s_rho0[NB_MAX_MATERIALS];
+  if (threadIdx.x < p.material_count) s_rho0[threadIdx.x] = mats[threadIdx.x].rho0;
   const uint32_t na = counters[CNT_ACTIVE];
   const bool apic = p.flip_ratio < -0.5f;
   const float D_inv = 4.f / (p.h * p.h);
@@ -411,60 +468,127 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g(SubParams p, const float4* 
     for (int e = threadIdx.x; e < kBoxNodes; e += kP2GThreads) {
       s_m[e] = 0.f; s_x[e] = 0.f; s_y[e] = 0.f; s_z[e] = 0.f;
     }
-    // compact the occupied cells (thread t looks at cells 2t, 2t+1)
+    // work items of the occupied cells (thread t looks at cells 2t, 2t+1)
+    if (threadIdx.x == 0) s_layers = 0u;
     const uint32_t ca = 2u * threadIdx.x, cb = ca + 1u;
-    const bool oa = cell_count[tbase + ca] != 0u, ob = cell_count[tbase + cb] != 0u;
-    uint32_t n_occ;
-    uint32_t pos = block_exclusive_scan<kP2GThreads>(uint32_t(oa) + uint32_t(ob), s_wave, n_occ);
-    if (oa) s_cell[pos++] = uint16_t(ca);
-    if (ob) s_cell[pos] = uint16_t(cb);
+    const uint32_t qa = p2g_chunks(cell_count[tbase + ca]), qb = p2g_chunks(cell_count[tbase + cb]);
+    uint32_t n_items;
+    uint32_t pos = block_exclusive_scan<kP2GThreads>(qa + qb, s_wave, n_items);
+    for (uint32_t q = 0; q < qa; q++) s_item[pos++] = uint16_t(ca | (q << 9));
+    for (uint32_t q = 0; q < qb; q++) s_item[pos++] = uint16_t(cb | (q << 9));
+    if (max(qa, qb) > 1u) atomicMax(&s_layers, max(qa, qb) - 1u);
     __syncthreads();
+    const uint32_t n_layers = s_layers + 1u;
 
-    for (uint32_t base = 0; base < n_occ; base += kP2GThreads) {
+    for (uint32_t base = 0; base < n_items; base += kP2GThreads) {
       const uint32_t j = base + threadIdx.x;
-      const bool have = j < n_occ;
-      const uint32_t c = have ? uint32_t(s_cell[j]) : 0u;
+      const bool have = j < n_items;
+      const uint32_t it = have ? uint32_t(s_item[j]) : 0u;
+      const uint32_t c = it & 511u, chunk = it >> 9;
       const int lx = int(morton_compact(c)), ly = int(morton_compact(c >> 1)), lz = int(morton_compact(c >> 2));
       const int cx = tx * 8 + lx, cy = ty * 8 + ly, cz = tz * 8 + lz;
       uint32_t kbeg = 0, kend = 0;
       if (have) {
-        kend = cell_end[tbase + c];
-        kbeg = kend - cell_count[tbase + c];
+        const uint32_t cnt = cell_count[tbase + c], end = cell_end[tbase + c];
+        const uint32_t per = (cnt + p2g_chunks(cnt) - 1u) / p2g_chunks(cnt);
+        kbeg = end - cnt + chunk * per;
+        kend = min(kbeg + per, end);
       }
+      // The pass's items are cells in Morton order, so their particles are one
+      // contiguous range of the bin order: its indices are staged in LDS with
+      // coalesced loads (ranges beyond the stage read the bin order directly).
+      if (have && j == base) s_lo = kbeg;
+      if (have && (j + 1 == n_items || threadIdx.x == kP2GThreads - 1)) s_hi = kend;
+      __syncthreads();
+      const uint32_t lo = s_lo, hi = s_hi;
+      const bool staged = hi - lo <= kIdxStage;
+      if (staged)
+        for (uint32_t e = threadIdx.x; e < hi - lo; e += kP2GThreads) s_idx[e] = sorted[lo + e];
+      __syncthreads();
+      auto idx_of = [&](uint32_t k) -> uint32_t { return staged ? s_idx[k - lo] : sorted[k]; };
       float am[27], ax[27], ay[27], az[27];
 #pragma unroll
       for (int o = 0; o < 27; o++) { am[o] = 0.f; ax[o] = 0.f; ay[o] = 0.f; az[o] = 0.f; }
 
-      // the next particle's record is loaded while this one is scattered
-      float4 npp, nvv, nb0, nb1, nb2;
-      float nf0w, nf1w, nf2w;
-      auto load_rec = [&](uint32_t k) {
-        const float4* r = rec + size_t(sorted[k]) * kRec;
-        npp = r[kP]; nvv = r[kV];
-        nf0w = r[kF0].w; nf1w = r[kF1].w; nf2w = r[kF2].w;
-        nb0 = r[kB0]; nb1 = r[kB1]; nb2 = r[kB2];
+      // Per particle (shaders/mpm_p2g.comp:40-90): mass m = rho0 Vp, affine
+      // C = m (4/h^2) B (APIC only), dt Vp tau from the w lanes
+      // (shaders/mpm_common.glsl:104-113), and the stencil of its cell.
+      struct Src {
+        Axis3 sx, sy, sz;
+        float mp, mvx, mvy, mvz;
+        float c00, c01, c02, c10, c11, c12, c20, c21, c22;
+        float t00, t11, t22, t01, t02, t12;
       };
-      if (kbeg < kend) load_rec(kbeg);
-      for (uint32_t k = kbeg; k < kend; k++) {
-        const float4 pp = npp, vv = nvv, b0 = nb0, b1 = nb1, b2 = nb2;
-        const float f0w = nf0w, f1w = nf1w, f2w = nf2w;
-        if (k + 1 < kend) load_rec(k + 1);
+      auto setup = [&](const float4& pp, const float4& vv, const float4& b0, const float4& b1, const float4& b2,
+                       const float4& tt, Src& q) {
         uint32_t mid = __float_as_uint(vv.w);
         mid = mid > mlast ? mlast : mid;
-        const float mp = mats[mid].rho0 * pp.w;
-        Axis3 sx, sy, sz;
-        axis_stencil(pp.x, p.wmin[0], p.h, inv_h, cx - 1, sx);
-        axis_stencil(pp.y, p.wmin[1], p.h, inv_h, cy - 1, sy);
-        axis_stencil(pp.z, p.wmin[2], p.h, inv_h, cz - 1, sz);
-        const float sc = apic ? mp * D_inv : 0.f;  // C = m (4/h^2) B, zero unless APIC
-        const float c00 = sc * b0.x, c01 = sc * b0.y, c02 = sc * b0.z;
-        const float c10 = sc * b1.x, c11 = sc * b1.y, c12 = sc * b1.z;
-        const float c20 = sc * b2.x, c21 = sc * b2.y, c22 = sc * b2.z;
-        // dt Vp tau, tau symmetric in the w lanes (shaders/mpm_common.glsl:104-113)
+        q.mp = s_rho0[mid] * pp.w;
+        axis_stencil(pp.x, p.wmin[0], p.h, inv_h, cx - 1, q.sx);
+        axis_stencil(pp.y, p.wmin[1], p.h, inv_h, cy - 1, q.sy);
+        axis_stencil(pp.z, p.wmin[2], p.h, inv_h, cz - 1, q.sz);
+        const float sc = apic ? q.mp * D_inv : 0.f;
+        q.c00 = sc * b0.x; q.c01 = sc * b0.y; q.c02 = sc * b0.z;
+        q.c10 = sc * b1.x; q.c11 = sc * b1.y; q.c12 = sc * b1.z;
+        q.c20 = sc * b2.x; q.c21 = sc * b2.y; q.c22 = sc * b2.z;
         const float st = p.dt * pp.w;
-        const float t00 = st * f0w, t11 = st * f1w, t22 = st * f2w;
-        const float t01 = st * b0.w, t02 = st * b1.w, t12 = st * b2.w;
-        const float mvx = mp * vv.x, mvy = mp * vv.y, mvz = mp * vv.z;
+        q.t00 = st * tt.x; q.t11 = st * tt.y; q.t22 = st * tt.z;
+        q.t01 = st * b0.w; q.t02 = st * b1.w; q.t12 = st * b2.w;
+        q.mvx = q.mp * vv.x; q.mvy = q.mp * vv.y; q.mvz = q.mp * vv.z;
+      };
+      // The reference skips nodes with w = wx wy wz < 1e-12
+      // (shaders/mpm_p2g.comp:61).  The smallest positive per-axis weights
+      // bound every w from below (1.0001e-12 covers the rounding of the
+      // product order): particles with no such node take the fast loop.
+      auto has_tiny = [&](const Src& q) {
+        return (min_positive3(q.sx.w) * min_positive3(q.sy.w)) * min_positive3(q.sz.w) < 1.0001e-12f;
+      };
+
+      // Fast loop.  Node contributions factor as
+      //   m w = (m wx) wyz,  mom = wx P(y,z) + wyz Q(x)
+      // with P = wyz (m v + C (0, dy, dz)) - tau (0, gy wz, wy gz) and
+      // Q = wx dx C e_x - gx tau e_x: 7 FMAs per node (explicit FMA chains;
+      // strict evaluation of acc + (a b + c d) costs a multiply, an FMA and an
+      // add).  A particle with a sub-cutoff node is scaled to zero here and
+      // added by the exact loop below.  The next particle's record is loaded
+      // while this one is scattered.
+      bool any_tiny = false;
+      float4 npp, nvv, nb0, nb1, nb2, ntt;
+      auto load_rec = [&](uint32_t q) {
+        const float4* r = rec + size_t(q) * kRec;
+        npp = r[R_P]; nvv = r[R_V];
+        nb0 = r[R_B0]; nb1 = r[R_B1]; nb2 = r[R_B2];
+        ntt = r[R_T];
+      };
+      if (kbeg < kend) load_rec(idx_of(kbeg));
+      for (uint32_t k = kbeg; k < kend; k++) {
+        const float4 pp = npp, vv = nvv, b0 = nb0, b1 = nb1, b2 = nb2, tt = ntt;
+        if (k + 1 < kend) load_rec(idx_of(k + 1));
+        Src q;
+        setup(pp, vv, b0, b1, b2, tt, q);
+        const bool tiny = has_tiny(q);
+        any_tiny |= tiny;
+        const float keep = tiny ? 0.f : 1.f;
+        const float mp = q.mp * keep, st = keep;
+        const float mvx = q.mvx * keep, mvy = q.mvy * keep, mvz = q.mvz * keep;
+        const float c00 = q.c00 * keep, c01 = q.c01 * keep, c02 = q.c02 * keep;
+        const float c10 = q.c10 * keep, c11 = q.c11 * keep, c12 = q.c12 * keep;
+        const float c20 = q.c20 * keep, c21 = q.c21 * keep, c22 = q.c22 * keep;
+        const float t00 = q.t00 * st, t11 = q.t11 * st, t22 = q.t22 * st;
+        const float t01 = q.t01 * st, t02 = q.t02 * st, t12 = q.t12 * st;
+        const Axis3 &sx = q.sx, &sy = q.sy, &sz = q.sz;
+        float mwx[3], Qx[3], Qy[3], Qz[3], Ayx[3], Ayy[3], Ayz[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+          mwx[i] = mp * sx.w[i];
+          const float wdx = sx.w[i] * sx.dx[i];
+          Qx[i] = fmaf(-sx.g[i], t00, wdx * c00);
+          Qy[i] = fmaf(-sx.g[i], t01, wdx * c01);
+          Qz[i] = fmaf(-sx.g[i], t02, wdx * c02);
+          Ayx[i] = fmaf(c10, sy.dx[i], mvx);
+          Ayy[i] = fmaf(c11, sy.dx[i], mvy);
+          Ayz[i] = fmaf(c12, sy.dx[i], mvz);
+        }
 #pragma unroll
         for (int kz = 0; kz < 3; kz++) {
 #pragma unroll
@@ -472,47 +596,79 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g(SubParams p, const float4* 
             const float wyz = sy.w[jy] * sz.w[kz];
             const float gy_wz = sy.g[jy] * sz.w[kz];
             const float wy_gz = sy.w[jy] * sz.g[kz];
-            // m v + C (0, dy, dz)
-            const float Ax = mvx + c10 * sy.dx[jy] + c20 * sz.dx[kz];
-            const float Ay = mvy + c11 * sy.dx[jy] + c21 * sz.dx[kz];
-            const float Az = mvz + c12 * sy.dx[jy] + c22 * sz.dx[kz];
-            // tau (0, gy wz, wy gz); the wx factor is applied per node
-            const float Tx = t01 * gy_wz + t02 * wy_gz;
-            const float Ty = t11 * gy_wz + t12 * wy_gz;
-            const float Tz = t12 * gy_wz + t22 * wy_gz;
+            const float Px = fmaf(-t02, wy_gz, fmaf(wyz, fmaf(c20, sz.dx[kz], Ayx[jy]), -(t01 * gy_wz)));
+            const float Py = fmaf(-t12, wy_gz, fmaf(wyz, fmaf(c21, sz.dx[kz], Ayy[jy]), -(t11 * gy_wz)));
+            const float Pz = fmaf(-t22, wy_gz, fmaf(wyz, fmaf(c22, sz.dx[kz], Ayz[jy]), -(t12 * gy_wz)));
 #pragma unroll
             for (int ix = 0; ix < 3; ix++) {
               const int o = (kz * 3 + jy) * 3 + ix;
-              float w = sx.w[ix] * wyz;
-              const bool live = w >= 1e-12f;  // shaders/mpm_p2g.comp:61
-              w = live ? w : 0.f;
-              const float wx = live ? sx.w[ix] : 0.f;
-              const float wd = w * sx.dx[ix];
-              const float gw = live ? sx.g[ix] * wyz : 0.f;
-              am[o] += w * mp;
-              ax[o] += w * Ax + wd * c00 - gw * t00 - wx * Tx;
-              ay[o] += w * Ay + wd * c01 - gw * t01 - wx * Ty;
-              az[o] += w * Az + wd * c02 - gw * t02 - wx * Tz;
+              am[o] = fmaf(mwx[ix], wyz, am[o]);
+              ax[o] = fmaf(wyz, Qx[ix], fmaf(sx.w[ix], Px, ax[o]));
+              ay[o] = fmaf(wyz, Qy[ix], fmaf(sx.w[ix], Py, ay[o]));
+              az[o] = fmaf(wyz, Qz[ix], fmaf(sx.w[ix], Pz, az[o]));
             }
           }
         }
       }
+      // Exact loop: the particles skipped above, node by node with the cutoff.
+      if (__any(any_tiny)) {
+        for (uint32_t k = kbeg; k < kend; k++) {
+          const float4* r = rec + size_t(idx_of(k)) * kRec;
+          Src q;
+          setup(r[R_P], r[R_V], r[R_B0], r[R_B1], r[R_B2], r[R_T], q);
+          if (!has_tiny(q)) continue;
 #pragma unroll
-      for (int kz = 0; kz < 3; kz++)
+          for (int kz = 0; kz < 3; kz++) {
 #pragma unroll
-        for (int jy = 0; jy < 3; jy++)
+            for (int jy = 0; jy < 3; jy++) {
+              const float wyz = q.sy.w[jy] * q.sz.w[kz];
+              const float gy_wz = q.sy.g[jy] * q.sz.w[kz];
+              const float wy_gz = q.sy.w[jy] * q.sz.g[kz];
+              // m v + C (0, dy, dz)
+              const float Ax = q.mvx + q.c10 * q.sy.dx[jy] + q.c20 * q.sz.dx[kz];
+              const float Ay = q.mvy + q.c11 * q.sy.dx[jy] + q.c21 * q.sz.dx[kz];
+              const float Az = q.mvz + q.c12 * q.sy.dx[jy] + q.c22 * q.sz.dx[kz];
+              // tau (0, gy wz, wy gz); the wx factor is applied per node
+              const float Tx = q.t01 * gy_wz + q.t02 * wy_gz;
+              const float Ty = q.t11 * gy_wz + q.t12 * wy_gz;
+              const float Tz = q.t12 * gy_wz + q.t22 * wy_gz;
 #pragma unroll
-          for (int ix = 0; ix < 3; ix++) {
-            const int o = (kz * 3 + jy) * 3 + ix;
-            const int e = ((lz + kz) * kBox + (ly + jy)) * kBox + (lx + ix);
-            __syncthreads();
-            if (have) {
-              s_m[e] += am[o];
-              s_x[e] += ax[o];
-              s_y[e] += ay[o];
-              s_z[e] += az[o];
+              for (int ix = 0; ix < 3; ix++) {
+                const int o = (kz * 3 + jy) * 3 + ix;
+                float w = q.sx.w[ix] * wyz;
+                const bool live = w >= 1e-12f;
+                w = live ? w : 0.f;
+                const float wx = live ? q.sx.w[ix] : 0.f;
+                const float wd = w * q.sx.dx[ix];
+                const float gw = live ? q.sx.g[ix] * wyz : 0.f;
+                am[o] += w * q.mp;
+                ax[o] += w * Ax + wd * q.c00 - gw * q.t00 - wx * Tx;
+                ay[o] += w * Ay + wd * q.c01 - gw * q.t01 - wx * Ty;
+                az[o] += w * Az + wd * q.c02 - gw * q.t02 - wx * Tz;
+              }
             }
           }
+        }
+      }
+      for (uint32_t layer = 0; layer < n_layers; layer++) {
+        const bool mine = have && chunk == layer;
+#pragma unroll
+        for (int kz = 0; kz < 3; kz++)
+#pragma unroll
+          for (int jy = 0; jy < 3; jy++)
+#pragma unroll
+            for (int ix = 0; ix < 3; ix++) {
+              const int o = (kz * 3 + jy) * 3 + ix;
+              const int e = ((lz + kz) * kBox + (ly + jy)) * kBox + (lx + ix);
+              __syncthreads();
+              if (mine) {
+                s_m[e] += am[o];
+                s_x[e] += ax[o];
+                s_y[e] += ay[o];
+                s_z[e] += az[o];
+              }
+            }
+      }
       __syncthreads();
     }
     float4* out = partial + size_t(a) * kBoxNodes;
@@ -713,11 +869,19 @@ __global__ __launch_bounds__(256) void k_g2p(SubParams p, float4* __restrict__ r
                                              const uint32_t* __restrict__ sorted,
                                              const float4* __restrict__ grid_node,
                                              const float4* __restrict__ grid_old, const float* __restrict__ sdf) {
+  __shared__ nb_material s_mats[NB_MAX_MATERIALS];
+  {
+    float* dst = reinterpret_cast<float*>(s_mats);
+    const float* src = reinterpret_cast<const float*>(mats);
+    const uint32_t nf = p.material_count * uint32_t(sizeof(nb_material) / 4);
+    for (uint32_t e = threadIdx.x; e < nf; e += blockDim.x) dst[e] = src[e];
+    __syncthreads();
+  }
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= p.n) return;
   const uint32_t q = sorted ? sorted[k] : k;
   float4* r = rec + size_t(q) * kRec;
-  const float4 pp = r[kP];
+  const float4 pp = r[R_P];
   const float xp = pp.x, yp = pp.y, zp = pp.z, Vp = pp.w;
   constexpr bool flip = FLIP;  // flip_ratio > 0.01
 
@@ -740,27 +904,18 @@ __global__ __launch_bounds__(256) void k_g2p(SubParams p, float4* __restrict__ r
   // 0 < w < 1e-12 (the smallest positive per-axis weights bound every w from
   // below), the weight cut removes nothing and the sums are contracted axis by
   // axis; otherwise every node is visited with the cut applied.
-  bool tiny;
-  {
-    float mn[3];
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-      mn[a] = INFINITY;
-#pragma unroll
-      for (int i = 0; i < 3; i++) mn[a] = fminf(mn[a], ax3[a].w[i] > 0.f ? ax3[a].w[i] : INFINITY);
-    }
-    tiny = (mn[0] * mn[1]) * mn[2] < 1e-12f;
-  }
-  // out-of-grid nodes read a clamped (valid) address and are zeroed by select
-  auto node_vel = [&](const float4* __restrict__ grid, int ix, int iy, int iz) -> float4 {
+  const bool tiny = (min_positive3(ax3[0].w) * min_positive3(ax3[1].w)) * min_positive3(ax3[2].w) < 1e-12f;
+  // out-of-grid nodes read a clamped (valid) address; the caller drops them
+  // together with the massless ones
+  auto node_in = [&](int ix, int iy, int iz) -> bool {
     const int nx = base[0] + ix, ny = base[1] + iy, nz = base[2] + iz;
-    const bool inside = nx >= 0 && nx < p.grid_res[0] && ny >= 0 && ny < p.grid_res[1] && nz >= 0 &&
-                        nz < p.grid_res[2];
-    const uint32_t cx = uint32_t(min(max(nx, 0), p.grid_res[0] - 1));
-    const uint32_t cy = uint32_t(min(max(ny, 0), p.grid_res[1] - 1));
-    const uint32_t cz = uint32_t(min(max(nz, 0), p.grid_res[2] - 1));
-    const float4 g = grid[morton3(cx, cy, cz)];
-    return inside ? g : make_float4(0.f, 0.f, 0.f, 0.f);
+    return nx >= 0 && nx < p.grid_res[0] && ny >= 0 && ny < p.grid_res[1] && nz >= 0 && nz < p.grid_res[2];
+  };
+  auto node_vel = [&](const float4* __restrict__ grid, int ix, int iy, int iz) -> float4 {
+    const uint32_t cx = uint32_t(min(max(base[0] + ix, 0), p.grid_res[0] - 1));
+    const uint32_t cy = uint32_t(min(max(base[1] + iy, 0), p.grid_res[1] - 1));
+    const uint32_t cz = uint32_t(min(max(base[2] + iz, 0), p.grid_res[2] - 1));
+    return grid[morton3(cx, cy, cz)];
   };
 
   float vpx = 0.f, vpy = 0.f, vpz = 0.f, vox = 0.f, voy = 0.f, voz = 0.f;
@@ -782,37 +937,50 @@ __global__ __launch_bounds__(256) void k_g2p(SubParams p, float4* __restrict__ r
     }
 #pragma unroll 1
     for (int iz = 0; iz < 3; iz++) {
-      float Y00[3] = {0.f, 0.f, 0.f}, Y01[3] = {0.f, 0.f, 0.f}, Y02[3] = {0.f, 0.f, 0.f};
-      float Y1[3] = {0.f, 0.f, 0.f}, Y2[3] = {0.f, 0.f, 0.f}, YO[3] = {0.f, 0.f, 0.f};
+      // sums start from their first term (no zero-initialised accumulators)
+      float Y00[3], Y01[3], Y02[3], Y1[3], Y2[3], YO[3];
 #pragma unroll
       for (int iy = 0; iy < 3; iy++) {
-        float X0[3] = {0.f, 0.f, 0.f}, X1[3] = {0.f, 0.f, 0.f}, X2[3] = {0.f, 0.f, 0.f}, XO[3] = {0.f, 0.f, 0.f};
+        float gv[3][3], go[3][3];
 #pragma unroll
         for (int ix = 0; ix < 3; ix++) {
           const float4 g = node_vel(grid_node, ix, iy, iz);
-          const bool live = g.w >= 1e-10f;
-          const float gv[3] = {live ? g.x : 0.f, live ? g.y : 0.f, live ? g.z : 0.f};
-#pragma unroll
-          for (int c = 0; c < 3; c++) {
-            X0[c] += X.w[ix] * gv[c];
-            X1[c] += X.g[ix] * gv[c];
-            X2[c] += wdx[ix] * gv[c];
-          }
+          const bool live = node_in(ix, iy, iz) && g.w >= 1e-10f;
+          gv[ix][0] = live ? g.x : 0.f;
+          gv[ix][1] = live ? g.y : 0.f;
+          gv[ix][2] = live ? g.z : 0.f;
           if (flip) {
             const float4 o = node_vel(grid_old, ix, iy, iz);
-            XO[0] += X.w[ix] * (live ? o.x : 0.f);
-            XO[1] += X.w[ix] * (live ? o.y : 0.f);
-            XO[2] += X.w[ix] * (live ? o.z : 0.f);
+            go[ix][0] = live ? o.x : 0.f;
+            go[ix][1] = live ? o.y : 0.f;
+            go[ix][2] = live ? o.z : 0.f;
           }
+        }
+        float X0[3], X1[3], X2[3], XO[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          X0[c] = X.w[0] * gv[0][c] + X.w[1] * gv[1][c] + X.w[2] * gv[2][c];
+          X1[c] = X.g[0] * gv[0][c] + X.g[1] * gv[1][c] + X.g[2] * gv[2][c];
+          X2[c] = wdx[0] * gv[0][c] + wdx[1] * gv[1][c] + wdx[2] * gv[2][c];
+          if (flip) XO[c] = X.w[0] * go[0][c] + X.w[1] * go[1][c] + X.w[2] * go[2][c];
         }
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-          Y00[c] += Y.w[iy] * X0[c];
-          Y01[c] += Y.g[iy] * X0[c];
-          Y02[c] += wdy[iy] * X0[c];
-          Y1[c] += Y.w[iy] * X1[c];
-          Y2[c] += Y.w[iy] * X2[c];
-          YO[c] += Y.w[iy] * XO[c];
+          if (iy == 0) {
+            Y00[c] = Y.w[0] * X0[c];
+            Y01[c] = Y.g[0] * X0[c];
+            Y02[c] = wdy[0] * X0[c];
+            Y1[c] = Y.w[0] * X1[c];
+            Y2[c] = Y.w[0] * X2[c];
+            if (flip) YO[c] = Y.w[0] * XO[c];
+          } else {
+            Y00[c] += Y.w[iy] * X0[c];
+            Y01[c] += Y.g[iy] * X0[c];
+            Y02[c] += wdy[iy] * X0[c];
+            Y1[c] += Y.w[iy] * X1[c];
+            Y2[c] += Y.w[iy] * X2[c];
+            if (flip) YO[c] += Y.w[iy] * XO[c];
+          }
         }
       }
       const float wz = zw[0], gz = zg[0], wdz = zw[0] * zd[0];
@@ -821,7 +989,7 @@ __global__ __launch_bounds__(256) void k_g2p(SubParams p, float4* __restrict__ r
 #pragma unroll
       for (int c = 0; c < 3; c++) {
         *vp3[c] += wz * Y00[c];
-        *vo3[c] += wz * YO[c];
+        if (flip) *vo3[c] += wz * YO[c];
         // vg = sum outer(v_i, grad w_i): vg.m[col = grad axis][row = velocity component]
         vg.m[0][c] += wz * Y1[c];
         vg.m[1][c] += wz * Y01[c];
@@ -843,10 +1011,13 @@ __global__ __launch_bounds__(256) void k_g2p(SubParams p, float4* __restrict__ r
       for (int iy = 0; iy < 3; iy++) {
 #pragma unroll
         for (int ix = 0; ix < 3; ix++) {
-          const float4 g = node_vel(grid_node, ix, iy, iz);
+          float4 g = node_vel(grid_node, ix, iy, iz);
           const float wx = X.w[ix], wy = Y.w[iy];
           const float w0 = wx * wy * wz;
-          const bool live = g.w >= 1e-10f && w0 >= 1e-12f;
+          const bool live = node_in(ix, iy, iz) && g.w >= 1e-10f && w0 >= 1e-12f;
+          g.x = live ? g.x : 0.f;
+          g.y = live ? g.y : 0.f;
+          g.z = live ? g.z : 0.f;
           const float w = live ? w0 : 0.f;
           const float dwx = live ? X.g[ix] * wy * wz : 0.f;
           const float dwy = live ? wx * Y.g[iy] * wz : 0.f;
@@ -873,16 +1044,16 @@ __global__ __launch_bounds__(256) void k_g2p(SubParams p, float4* __restrict__ r
     }
   }
 
-  const float4 vel = r[kV];
+  const float4 vel = r[R_V];
   uint32_t mid = __float_as_uint(vel.w);
   mid = mid > p.material_count - 1u ? p.material_count - 1u : mid;
-  const nb_material mat = mats[mid];
+  const nb_material mat = s_mats[mid];
 
-  const float4 f0 = r[kF0], f1 = r[kF1], f2 = r[kF2];
+  const float4 rt = r[R_T], fa = r[R_FA], fb = r[R_FB];
   m3 F;
-  F.m[0][0] = f0.x; F.m[0][1] = f0.y; F.m[0][2] = f0.z;
-  F.m[1][0] = f1.x; F.m[1][1] = f1.y; F.m[1][2] = f1.z;
-  F.m[2][0] = f2.x; F.m[2][1] = f2.y; F.m[2][2] = f2.z;
+  F.m[0][0] = rt.w; F.m[0][1] = fa.x; F.m[0][2] = fa.y;
+  F.m[1][0] = fa.z; F.m[1][1] = fa.w; F.m[1][2] = fb.x;
+  F.m[2][0] = fb.y; F.m[2][1] = fb.z; F.m[2][2] = fb.w;
   m3 L;
 #pragma unroll
   for (int c = 0; c < 3; c++)
@@ -991,14 +1162,14 @@ __global__ __launch_bounds__(256) void k_g2p(SubParams p, float4* __restrict__ r
     if (zn < loz) { zn = loz; vnz = fmaxf(vnz, 0.f); }
     if (zn > hiz) { zn = hiz; vnz = fminf(vnz, 0.f); }
   }
-  r[kP] = make_float4(xn, yn, zn, Vp);
-  r[kV] = make_float4(vnx, vny, vnz, vel.w);
-  r[kF0] = make_float4(Fn.m[0][0], Fn.m[0][1], Fn.m[0][2], tau.m[0][0]);
-  r[kF1] = make_float4(Fn.m[1][0], Fn.m[1][1], Fn.m[1][2], tau.m[1][1]);
-  r[kF2] = make_float4(Fn.m[2][0], Fn.m[2][1], Fn.m[2][2], tau.m[2][2]);
-  r[kB0] = make_float4(Bn.m[0][0], Bn.m[0][1], Bn.m[0][2], tau.m[0][1]);
-  r[kB1] = make_float4(Bn.m[1][0], Bn.m[1][1], Bn.m[1][2], tau.m[0][2]);
-  r[kB2] = make_float4(Bn.m[2][0], Bn.m[2][1], Bn.m[2][2], tau.m[1][2]);
+  r[R_P] = make_float4(xn, yn, zn, Vp);
+  r[R_V] = make_float4(vnx, vny, vnz, vel.w);
+  r[R_B0] = make_float4(Bn.m[0][0], Bn.m[0][1], Bn.m[0][2], tau.m[0][1]);
+  r[R_B1] = make_float4(Bn.m[1][0], Bn.m[1][1], Bn.m[1][2], tau.m[0][2]);
+  r[R_B2] = make_float4(Bn.m[2][0], Bn.m[2][1], Bn.m[2][2], tau.m[1][2]);
+  r[R_T] = make_float4(tau.m[0][0], tau.m[1][1], tau.m[2][2], Fn.m[0][0]);
+  r[R_FA] = make_float4(Fn.m[0][1], Fn.m[0][2], Fn.m[1][0], Fn.m[1][1]);
+  r[R_FB] = make_float4(Fn.m[1][2], Fn.m[2][0], Fn.m[2][1], Fn.m[2][2]);
   keys[q] = cell_key(p, xn, yn, zn);  // the next substep's bin key
 }
 
@@ -1041,16 +1212,32 @@ __global__ void k_init_FB(float4* F0, float4* F1, float4* F2, float4* B0, float4
 
 // reference layout (eight float4 arrays) <-> one 128-byte record per particle
 __global__ __launch_bounds__(256) void k_soa_to_aos(FieldPtrs f, float4* __restrict__ rec, uint32_t n) {
-  const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= uint64_t(n) * kRec) return;
-  const uint32_t i = uint32_t(t >> 3), c = uint32_t(t & 7u);
-  rec[t] = f.f[c][i];
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 F0 = f.f[NB_FIELD_F0][i], F1 = f.f[NB_FIELD_F1][i], F2 = f.f[NB_FIELD_F2][i];
+  float4* r = rec + size_t(i) * kRec;
+  r[R_P] = f.f[NB_FIELD_P][i];
+  r[R_V] = f.f[NB_FIELD_V][i];
+  r[R_B0] = f.f[NB_FIELD_B0][i];
+  r[R_B1] = f.f[NB_FIELD_B1][i];
+  r[R_B2] = f.f[NB_FIELD_B2][i];
+  r[R_T] = make_float4(F0.w, F1.w, F2.w, F0.x);
+  r[R_FA] = make_float4(F0.y, F0.z, F1.x, F1.y);
+  r[R_FB] = make_float4(F1.z, F2.x, F2.y, F2.z);
 }
 __global__ __launch_bounds__(256) void k_aos_to_soa(const float4* __restrict__ rec, FieldPtrs f, uint32_t n) {
-  const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= uint64_t(n) * kRec) return;
-  const uint32_t i = uint32_t(t >> 3), c = uint32_t(t & 7u);
-  f.f[c][i] = rec[t];
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4* r = rec + size_t(i) * kRec;
+  const float4 t = r[R_T], a = r[R_FA], b = r[R_FB];
+  f.f[NB_FIELD_P][i] = r[R_P];
+  f.f[NB_FIELD_V][i] = r[R_V];
+  f.f[NB_FIELD_B0][i] = r[R_B0];
+  f.f[NB_FIELD_B1][i] = r[R_B1];
+  f.f[NB_FIELD_B2][i] = r[R_B2];
+  f.f[NB_FIELD_F0][i] = make_float4(t.w, a.x, a.y, t.x);
+  f.f[NB_FIELD_F1][i] = make_float4(a.z, a.w, b.x, t.y);
+  f.f[NB_FIELD_F2][i] = make_float4(b.y, b.z, b.w, t.z);
 }
 
 // plain streaming copy (HBM byte-counter calibration)
@@ -1137,16 +1324,14 @@ void launch_soa_to_aos(const DevState& d, uint32_t n, hipStream_t s) {
   if (!n) return;
   FieldPtrs f;
   for (int i = 0; i < NB_NUM_FIELDS; i++) f.f[i] = d.field[i];
-  const uint64_t t = uint64_t(n) * kRec;
-  k_soa_to_aos<<<uint32_t((t + 255) / 256), 256, 0, s>>>(f, d.rec, n);
+  k_soa_to_aos<<<cdiv(n, 256), 256, 0, s>>>(f, d.rec, n);
 }
 
 void launch_aos_to_soa(const DevState& d, uint32_t n, hipStream_t s) {
   if (!n) return;
   FieldPtrs f;
   for (int i = 0; i < NB_NUM_FIELDS; i++) f.f[i] = d.field[i];
-  const uint64_t t = uint64_t(n) * kRec;
-  k_aos_to_soa<<<uint32_t((t + 255) / 256), 256, 0, s>>>(d.rec, f, n);
+  k_aos_to_soa<<<cdiv(n, 256), 256, 0, s>>>(d.rec, f, n);
 }
 
 void launch_copy(float4* dst, const float4* src, uint64_t n4, hipStream_t s) {

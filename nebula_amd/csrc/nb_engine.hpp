@@ -5,8 +5,8 @@
 // HBM layout (one GPU):
 //   particles  8 x float4 x capacity, the reference's SoA (P, v, F0-2, B0-2),
 //              which the ABI exposes, and the working copy the kernels use:
-//              one 128-byte record per particle (the same eight float4 side
-//              by side), converted at the API boundary
+//              one 128-byte record per particle (the same 32 floats,
+//              reordered), converted at the API boundary
 //   bins       keys[N], rank[N], sorted[N]; cell_count/cell_end dense over the
 //              Morton cube (only occupied 8^3 tiles are ever touched)
 //   tiles      the Morton cube cut into 8^3-cell tiles (tile id = key >> 9):
@@ -29,9 +29,18 @@ constexpr int kTileCells = 512;
 constexpr int kBox = 10;              // node box edge of a tile (8 + 2 halo)
 constexpr int kBoxNodes = 1000;
 
-// interleaved particle record: the eight float4 of the reference's layout
+// Working particle record, 8 float4 = 128 bytes (one cache line), holding the
+// same 32 floats as the reference's eight arrays P, v, F0-2, B0-2 (tau packed
+// in their w lanes) in an order that puts everything P2G reads in the first
+// 96 bytes:
+//   R_P  = P                        (x, y, z, Vp)
+//   R_V  = v                        (vx, vy, vz, material id bits)
+//   R_B0 = B0, R_B1 = B1, R_B2 = B2 (affine columns; tau_xy, tau_xz, tau_yz)
+//   R_T  = (F0.w, F1.w, F2.w, F0.x) (tau_xx, tau_yy, tau_zz, F col 0 x)
+//   R_FA = (F0.y, F0.z, F1.x, F1.y)
+//   R_FB = (F1.z, F2.x, F2.y, F2.z)
 constexpr int kRec = 8;
-enum : int { kP = 0, kV = 1, kF0 = 2, kF1 = 3, kF2 = 4, kB0 = 5, kB1 = 6, kB2 = 7 };
+enum : int { R_P = 0, R_V = 1, R_B0 = 2, R_B1 = 3, R_B2 = 4, R_T = 5, R_FA = 6, R_FB = 7 };
 struct FieldPtrs {
   float4* f[NB_NUM_FIELDS];
 };

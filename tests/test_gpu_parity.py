@@ -211,12 +211,24 @@ def test_bins_edge_cases(gpu_lib):
     f = sc.fields.copy()
     n = f.shape[1]
     f[0, : n // 8, :3] = rng.uniform(-0.5, 1.5, (n // 8, 3))  # outside the domain: clamped keys
-    f[0, n // 8: n // 8 + 700, :3] = (0.51, 0.52, 0.53)  # one crowded cell (700 particles)
+    f[0, n // 8: n // 8 + 300, :3] = (0.51, 0.52, 0.53)  # one crowded cell (300 particles)
     f[0, -1, :3] = (1.0, 1.0, 1.0)  # exactly on the upper boundary
+    # cells of 17..40 particles (register sort and its fallback); the
+    # permutation below scatters each cell over many waves
+    lo = n // 8 + 300
+    assert lo + 146 < n - 1
+    for j, m in enumerate((17, 24, 32, 33, 40)):
+        f[0, lo: lo + m, :3] = (0.11 + 0.1 * j, 0.27, 0.33)
+        lo += m
     f = np.ascontiguousarray(f[:, rng.permutation(n)])
     _bins_equal(sc, f)
     # a single particle and one particle per tile corner
     _bins_equal(sc, np.ascontiguousarray(f[:, :1]))
+    # a cell beyond the block-wide sort's stage (2048): thread insertion sort
+    big = elastic_block_64()
+    g = big.fields.copy()
+    g[0, 1000:3100, :3] = (0.5, 0.5, 0.5)
+    _bins_equal(big, np.ascontiguousarray(g[:, rng.permutation(g.shape[1])]))
     # re-binning after motion reuses and clears the previous tiles
     e = make_engine(sc)
     for _ in range(3):
@@ -229,6 +241,27 @@ def test_bins_edge_cases(gpu_lib):
     sim.bin()
     ok, os_, oc, oe = sim.get_bins()
     assert np.array_equal(keys, ok) and np.array_equal(srt, os_) and np.array_equal(cnt, oc)
+
+
+def test_full_substep_bins_match_oracle(gpu_lib):
+    """Inside a full substep the bin reads the cell keys the previous G2P
+    wrote.  After 40 substeps of the snow scene (cells piled up to 30+
+    particles by the SDF push-out) the bins a substep builds equal the
+    oracle's bins of the state it started from."""
+    sc = snow_terrain_256()
+    e = make_engine(sc)
+    e.substep(sc.dt, 40)
+    st = e.getState()
+    e.substep(sc.dt, 1)
+    _, srt, cnt, end = e.getBins()
+    sim = oracle_sim(sc)
+    sim.set_state(st)
+    sim.bin()
+    _, os_, oc, oe = sim.get_bins()
+    say(f"  max cell {int(cnt.max())}, cells > 32: {int((cnt > 32).sum())}")
+    assert np.array_equal(cnt, oc), "cell counts differ"
+    assert np.array_equal(end, oe), "END offsets differ"
+    assert np.array_equal(srt, os_), "sorted order differs"
 
 
 # ---------------------------------------------------------------------------
