@@ -49,34 +49,35 @@ __device__ __forceinline__ void tile_coords(uint32_t T, int& tx, int& ty, int& t
   tz = int(morton_compact(T >> 2));
 }
 
-// Exclusive scan across a block of BS threads (BS a multiple of 64).
-// s_wave must hold BS/64 + 1 words.  Returns the exclusive prefix; *total is
-// the block sum.  Contains barriers: call from uniform control flow.
-template <int BS>
-__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_wave, uint32_t& total) {
+// Exclusive scan across a block of BS threads (BS a multiple of 64) of a
+// 32- or 64-bit unsigned value.  s_wave must hold BS/64 + 1 values.  Returns
+// the exclusive prefix; *total is the block sum.  Contains barriers: call from
+// uniform control flow.
+template <int BS, class T>
+__device__ __forceinline__ T block_exclusive_scan(T v, T* s_wave, T& total) {
   constexpr int NW = BS / 64;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  uint32_t x = v;
+  T x = v;
 #pragma unroll
   for (int d = 1; d < 64; d <<= 1) {
-    uint32_t y = __shfl_up(x, d);
+    T y = __shfl_up(x, d);
     if (lane >= d) x += y;
   }
   if (lane == 63) s_wave[wave] = x;
   __syncthreads();
   if (wave == 0) {
-    uint32_t w = lane < NW ? s_wave[lane] : 0u;
-    uint32_t ws = w;
+    T w = lane < NW ? s_wave[lane] : T(0);
+    T ws = w;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
-      uint32_t y = __shfl_up(ws, d);
+      T y = __shfl_up(ws, d);
       if (lane >= d) ws += y;
     }
     if (lane < NW) s_wave[lane] = ws - w;
     if (lane == NW - 1) s_wave[NW] = ws;
   }
   __syncthreads();
-  uint32_t excl = x - v + s_wave[wave];
+  T excl = x - v + s_wave[wave];
   total = s_wave[NW];
   __syncthreads();
   return excl;
@@ -403,8 +404,8 @@ __device__ __forceinline__ float min_positive3(const float w[3]) {
 // Quadratic B-spline weight and derivative (shaders/mpm_common.glsl:198-210),
 // evaluated with selects instead of branches; inv_h = 1/h replaces the
 // division of the derivative by h.
-__device__ __forceinline__ void axis_stencil(float xp, float wmin, float h, float inv_h, int first_node, Axis3& a) {
-  const float f = (xp - wmin) / h;
+__device__ __forceinline__ void axis_stencil_f(float f, float xp, float wmin, float h, float inv_h, int first_node,
+                                               Axis3& a) {
 #pragma unroll
   for (int i = 0; i < 3; i++) {
     const float nf = float(first_node + i) + 0.5f;
@@ -416,6 +417,10 @@ __device__ __forceinline__ void axis_stencil(float xp, float wmin, float h, floa
     a.g[i] = (in ? -2.f * d : (mid ? (d > 0.f ? -t : t) : 0.f)) * inv_h;
     a.dx[i] = nf * h + wmin - xp;
   }
+}
+// f = (x - wmin) / h, the IEEE quotient the cell keys use
+__device__ __forceinline__ void axis_stencil(float xp, float wmin, float h, float inv_h, int first_node, Axis3& a) {
+  axis_stencil_f((xp - wmin) / h, xp, wmin, h, inv_h, first_node, a);
 }
 
 // P2G: one block per occupied tile.  The tile's occupied cells are compacted
@@ -437,21 +442,29 @@ constexpr uint32_t kMaxChunks = 4;
 __device__ __forceinline__ uint32_t p2g_chunks(uint32_t count) {
   return count == 0u ? 0u : min((count + kChunk - 1u) / kChunk, kMaxChunks);
 }
+// item length groups, longest first: >= 12, 10-11, 8-9, 6-7, <= 5 particles
+constexpr uint32_t kGroups = 5;
+__device__ __forceinline__ uint32_t p2g_group(uint32_t count, uint32_t chunks) {
+  if (chunks == 0u) return 0u;
+  const uint32_t per = (count + chunks - 1u) / chunks;
+  return per >= 12u ? 0u : (per >= 10u ? 1u : (per >= 8u ? 2u : (per >= 6u ? 3u : 4u)));
+}
 
-__global__ __launch_bounds__(kP2GThreads) void k_p2g(SubParams p, const float4* __restrict__ rec,
+__global__ __launch_bounds__(kP2GThreads, 2) void k_p2g(SubParams p, const float4* __restrict__ rec,
                                                      const nb_material* __restrict__ mats,
                                                      const uint32_t* __restrict__ cell_count,
                                                      const uint32_t* __restrict__ cell_end,
                                                      const uint32_t* __restrict__ sorted,
                                                      const uint32_t* __restrict__ act_tiles,
+                                                     const uint32_t* __restrict__ tile_start,
+                                                     const uint32_t* __restrict__ tile_total,
                                                      const uint32_t* __restrict__ counters,
                                                      float4* __restrict__ partial) {
   __shared__ float s_m[kBoxNodes], s_x[kBoxNodes], s_y[kBoxNodes], s_z[kBoxNodes];
   __shared__ uint16_t s_item[kTileCells * kMaxChunks];  // cell | chunk << 9
-  __shared__ uint32_t s_wave[kP2GThreads / 64 + 1];
   __shared__ uint32_t s_layers;
   __shared__ uint32_t s_idx[kIdxStage];
-  __shared__ uint32_t s_lo, s_hi;
+  __shared__ uint64_t s_scan64[kP2GThreads / 64 + 1];
   __shared__ float s_rho0[NB_MAX_MATERIALS];
   if (threadIdx.x < p.material_count) s_rho0[threadIdx.x] = mats[threadIdx.x].rho0;
   const uint32_t na = counters[CNT_ACTIVE];
@@ -468,17 +481,50 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g(SubParams p, const float4* 
     for (int e = threadIdx.x; e < kBoxNodes; e += kP2GThreads) {
       s_m[e] = 0.f; s_x[e] = 0.f; s_y[e] = 0.f; s_z[e] = 0.f;
     }
-    // work items of the occupied cells (thread t looks at cells 2t, 2t+1)
+    // Work items of the occupied cells (thread t looks at cells 2t, 2t+1),
+    // grouped by particle count (largest first, cell order within a group) so
+    // that the lanes of a wave walk similar lengths: a stable counting sort
+    // over kGroups groups, whose per-group counts (< 4096) are packed 12 bits
+    // each into one 64-bit block scan.
     if (threadIdx.x == 0) s_layers = 0u;
     const uint32_t ca = 2u * threadIdx.x, cb = ca + 1u;
-    const uint32_t qa = p2g_chunks(cell_count[tbase + ca]), qb = p2g_chunks(cell_count[tbase + cb]);
-    uint32_t n_items;
-    uint32_t pos = block_exclusive_scan<kP2GThreads>(qa + qb, s_wave, n_items);
-    for (uint32_t q = 0; q < qa; q++) s_item[pos++] = uint16_t(ca | (q << 9));
-    for (uint32_t q = 0; q < qb; q++) s_item[pos++] = uint16_t(cb | (q << 9));
+    const uint32_t na_ = cell_count[tbase + ca], nb_ = cell_count[tbase + cb];
+    const uint32_t qa = p2g_chunks(na_), qb = p2g_chunks(nb_);
+    const uint32_t ga = p2g_group(na_, qa), gb = p2g_group(nb_, qb);
+    const uint64_t mine = (uint64_t(qa) << (12 * ga)) + (uint64_t(qb) << (12 * gb));
+    uint64_t tot64;
+    const uint64_t pre = block_exclusive_scan<kP2GThreads>(mine, s_scan64, tot64);
     if (max(qa, qb) > 1u) atomicMax(&s_layers, max(qa, qb) - 1u);
+    uint32_t gbase[kGroups];
+    {
+      uint32_t acc = 0;
+#pragma unroll
+      for (int g = 0; g < int(kGroups); g++) {
+        gbase[g] = acc;
+        acc += uint32_t(tot64 >> (12 * g)) & 4095u;
+      }
+    }
+    uint32_t pa = 0, pb = 0;
+#pragma unroll
+    for (int g = 0; g < int(kGroups); g++) {
+      if (ga == uint32_t(g)) pa = gbase[g] + (uint32_t(pre >> (12 * g)) & 4095u);
+      if (gb == uint32_t(g)) pb = gbase[g] + (uint32_t(pre >> (12 * g)) & 4095u) + (ga == gb ? qa : 0u);
+    }
+    for (uint32_t q = 0; q < qa; q++) s_item[pa + q] = uint16_t(ca | (q << 9));
+    for (uint32_t q = 0; q < qb; q++) s_item[pb + q] = uint16_t(cb | (q << 9));
+    // the tile's particles are one contiguous range of the bin order: their
+    // indices are staged in LDS with coalesced loads (a tile beyond the stage
+    // reads the bin order directly)
+    const uint32_t lo = tile_start[a], tile_n = tile_total[a];
+    const bool staged = tile_n <= kIdxStage;
+    if (staged)
+      for (uint32_t e = threadIdx.x; e < tile_n; e += kP2GThreads) s_idx[e] = sorted[lo + e];
     __syncthreads();
+    uint32_t n_items = 0;
+#pragma unroll
+    for (int g = 0; g < int(kGroups); g++) n_items += uint32_t(tot64 >> (12 * g)) & 4095u;
     const uint32_t n_layers = s_layers + 1u;
+    auto idx_of = [&](uint32_t k) -> uint32_t { return staged ? s_idx[k - lo] : sorted[k]; };
 
     for (uint32_t base = 0; base < n_items; base += kP2GThreads) {
       const uint32_t j = base + threadIdx.x;
@@ -494,18 +540,6 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g(SubParams p, const float4* 
         kbeg = end - cnt + chunk * per;
         kend = min(kbeg + per, end);
       }
-      // The pass's items are cells in Morton order, so their particles are one
-      // contiguous range of the bin order: its indices are staged in LDS with
-      // coalesced loads (ranges beyond the stage read the bin order directly).
-      if (have && j == base) s_lo = kbeg;
-      if (have && (j + 1 == n_items || threadIdx.x == kP2GThreads - 1)) s_hi = kend;
-      __syncthreads();
-      const uint32_t lo = s_lo, hi = s_hi;
-      const bool staged = hi - lo <= kIdxStage;
-      if (staged)
-        for (uint32_t e = threadIdx.x; e < hi - lo; e += kP2GThreads) s_idx[e] = sorted[lo + e];
-      __syncthreads();
-      auto idx_of = [&](uint32_t k) -> uint32_t { return staged ? s_idx[k - lo] : sorted[k]; };
       float am[27], ax[27], ay[27], az[27];
 #pragma unroll
       for (int o = 0; o < 27; o++) { am[o] = 0.f; ax[o] = 0.f; ay[o] = 0.f; az[o] = 0.f; }
@@ -519,19 +553,24 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g(SubParams p, const float4* 
         float c00, c01, c02, c10, c11, c12, c20, c21, c22;
         float t00, t11, t22, t01, t02, t12;
       };
+      // (the stencil's quotient (x - wmin)/h is taken as a product with 1/h:
+      // the cell is the bin's, only the weights use it)
+      auto stencil = [&](const float4& pp, Src& q) {
+        axis_stencil_f((pp.x - p.wmin[0]) * inv_h, pp.x, p.wmin[0], p.h, inv_h, cx - 1, q.sx);
+        axis_stencil_f((pp.y - p.wmin[1]) * inv_h, pp.y, p.wmin[1], p.h, inv_h, cy - 1, q.sy);
+        axis_stencil_f((pp.z - p.wmin[2]) * inv_h, pp.z, p.wmin[2], p.h, inv_h, cz - 1, q.sz);
+      };
+      // keep = 0 scales the particle's contribution to exactly zero
       auto setup = [&](const float4& pp, const float4& vv, const float4& b0, const float4& b1, const float4& b2,
-                       const float4& tt, Src& q) {
+                       const float4& tt, float keep, Src& q) {
         uint32_t mid = __float_as_uint(vv.w);
         mid = mid > mlast ? mlast : mid;
-        q.mp = s_rho0[mid] * pp.w;
-        axis_stencil(pp.x, p.wmin[0], p.h, inv_h, cx - 1, q.sx);
-        axis_stencil(pp.y, p.wmin[1], p.h, inv_h, cy - 1, q.sy);
-        axis_stencil(pp.z, p.wmin[2], p.h, inv_h, cz - 1, q.sz);
+        q.mp = s_rho0[mid] * pp.w * keep;
         const float sc = apic ? q.mp * D_inv : 0.f;
         q.c00 = sc * b0.x; q.c01 = sc * b0.y; q.c02 = sc * b0.z;
         q.c10 = sc * b1.x; q.c11 = sc * b1.y; q.c12 = sc * b1.z;
         q.c20 = sc * b2.x; q.c21 = sc * b2.y; q.c22 = sc * b2.z;
-        const float st = p.dt * pp.w;
+        const float st = p.dt * pp.w * keep;
         q.t00 = st * tt.x; q.t11 = st * tt.y; q.t22 = st * tt.z;
         q.t01 = st * b0.w; q.t02 = st * b1.w; q.t12 = st * b2.w;
         q.mvx = q.mp * vv.x; q.mvy = q.mp * vv.y; q.mvz = q.mp * vv.z;
@@ -565,17 +604,14 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g(SubParams p, const float4* 
         const float4 pp = npp, vv = nvv, b0 = nb0, b1 = nb1, b2 = nb2, tt = ntt;
         if (k + 1 < kend) load_rec(idx_of(k + 1));
         Src q;
-        setup(pp, vv, b0, b1, b2, tt, q);
+        stencil(pp, q);
         const bool tiny = has_tiny(q);
         any_tiny |= tiny;
-        const float keep = tiny ? 0.f : 1.f;
-        const float mp = q.mp * keep, st = keep;
-        const float mvx = q.mvx * keep, mvy = q.mvy * keep, mvz = q.mvz * keep;
-        const float c00 = q.c00 * keep, c01 = q.c01 * keep, c02 = q.c02 * keep;
-        const float c10 = q.c10 * keep, c11 = q.c11 * keep, c12 = q.c12 * keep;
-        const float c20 = q.c20 * keep, c21 = q.c21 * keep, c22 = q.c22 * keep;
-        const float t00 = q.t00 * st, t11 = q.t11 * st, t22 = q.t22 * st;
-        const float t01 = q.t01 * st, t02 = q.t02 * st, t12 = q.t12 * st;
+        setup(pp, vv, b0, b1, b2, tt, tiny ? 0.f : 1.f, q);
+        const float mp = q.mp, mvx = q.mvx, mvy = q.mvy, mvz = q.mvz;
+        const float c00 = q.c00, c01 = q.c01, c02 = q.c02, c10 = q.c10, c11 = q.c11, c12 = q.c12;
+        const float c20 = q.c20, c21 = q.c21, c22 = q.c22;
+        const float t00 = q.t00, t11 = q.t11, t22 = q.t22, t01 = q.t01, t02 = q.t02, t12 = q.t12;
         const Axis3 &sx = q.sx, &sy = q.sy, &sz = q.sz;
         float mwx[3], Qx[3], Qy[3], Qz[3], Ayx[3], Ayy[3], Ayz[3];
 #pragma unroll
@@ -615,8 +651,10 @@ __global__ __launch_bounds__(kP2GThreads) void k_p2g(SubParams p, const float4* 
         for (uint32_t k = kbeg; k < kend; k++) {
           const float4* r = rec + size_t(idx_of(k)) * kRec;
           Src q;
-          setup(r[R_P], r[R_V], r[R_B0], r[R_B1], r[R_B2], r[R_T], q);
+          const float4 pp = r[R_P];
+          stencil(pp, q);
           if (!has_tiny(q)) continue;
+          setup(pp, r[R_V], r[R_B0], r[R_B1], r[R_B2], r[R_T], 1.f, q);
 #pragma unroll
           for (int kz = 0; kz < 3; kz++) {
 #pragma unroll
@@ -1289,7 +1327,8 @@ void launch_bin(const DevState& d, const SubParams& p, uint32_t n_tiles, bool ke
 void launch_p2g(const DevState& d, const SubParams& p, uint32_t n_tiles, hipStream_t s, Prof* pf) {
   ProfScope ps(pf, 8, s);
   k_p2g<<<tile_grid(n_tiles), kP2GThreads, 0, s>>>(p, d.rec, d.materials, d.cell_count, d.cell_end, d.sorted,
-                                                   d.act_tiles, d.counters, d.partial);
+                                                   d.act_tiles, d.tile_start, d.tile_total, d.counters,
+                                                   d.partial);
 }
 
 void launch_grid(const DevState& d, const SubParams& p, uint32_t n_tiles, bool from_partials, bool update,
