@@ -286,6 +286,7 @@ int nb_mpm_create(const nb_mpm_config* cfg, void* hip_stream, nb_mpm** out) {
   if (rc == NB_OK) rc = h->alloc(&d.cell_end, cells);
   if (rc == NB_OK) rc = h->alloc(&d.tile_flag, h->NT);
   if (rc == NB_OK) rc = h->alloc(&d.tile_mark, h->NT);
+  if (rc == NB_OK) rc = h->alloc(&d.blk_cnt, (h->NT + 255) / 256);
   if (rc == NB_OK) rc = h->alloc(&d.tile_act, h->NT);
   if (rc == NB_OK) rc = h->alloc(&d.act_tiles, h->NT);
   if (rc == NB_OK) rc = h->alloc(&d.node_tiles, h->NT);
@@ -667,6 +668,7 @@ int nb_mpm_get_grid(nb_mpm* h, float* mom4, float* mass) {
   if (rc == NB_OK && h->old_valid) rc = download(h, old.data(), h->d.grid_old, cells * 16);
   if (rc != NB_OK) return rc;
   std::vector<uint32_t> tiles;
+  std::vector<int32_t> tile_act;
   if (h->grid_mode == GridMode::DENSE) {
     tiles.resize(h->NT);
     for (uint32_t t = 0; t < h->NT; t++) tiles[t] = t;
@@ -675,11 +677,34 @@ int nb_mpm_get_grid(nb_mpm* h, float* mom4, float* mass) {
     if ((rc = download(h, cnt, h->d.counters, sizeof(cnt))) != NB_OK) return rc;
     tiles.resize(cnt[CNT_NODE_TILES]);
     if (!tiles.empty() && (rc = download(h, tiles.data(), h->d.node_tiles, tiles.size() * 4)) != NB_OK) return rc;
+    tile_act.resize(h->NT);
+    if ((rc = download(h, tile_act.data(), h->d.tile_act, size_t(h->NT) * 4)) != NB_OK) return rc;
   }
+  // sparse grids hold only the nodes inside some occupied tile's 10^3 box
+  const int td = int(h->tile_dim);
+  auto covered = [&](uint32_t T, uint32_t j) {
+    if (tile_act.empty()) return true;
+    const int t[3] = {int(morton_compact(T)), int(morton_compact(T >> 1)), int(morton_compact(T >> 2))};
+    const int l[3] = {int(morton_compact(j)), int(morton_compact(j >> 1)), int(morton_compact(j >> 2))};
+    for (int dz = -1; dz <= 1; dz++)
+      for (int dy = -1; dy <= 1; dy++)
+        for (int dx = -1; dx <= 1; dx++) {
+          const int dd[3] = {dx, dy, dz};
+          bool ok = true;
+          int s3[3];
+          for (int a = 0; a < 3; a++) {
+            if ((dd[a] < 0 && l[a] != 0) || (dd[a] > 0 && l[a] != 7)) ok = false;
+            s3[a] = t[a] + dd[a];
+            if (s3[a] < 0 || s3[a] >= td) ok = false;
+          }
+          if (ok && tile_act[morton3(uint32_t(s3[0]), uint32_t(s3[1]), uint32_t(s3[2]))] >= 0) return true;
+        }
+    return false;
+  };
   for (uint32_t T : tiles) {
     for (uint32_t j = 0; j < uint32_t(kTileCells); j++) {
       const size_t nid = (size_t(T) << kTileBits) | j;
-      if (nid >= NC) continue;
+      if (nid >= NC || !covered(T, j)) continue;
       const float* g = &node[nid * 4];
       if (mom4) {
         mom4[nid * 4 + 0] = g[0];

@@ -154,95 +154,62 @@ __global__ __launch_bounds__(256) void k_bin_keys(SubParams p, const float4* __r
 }
 
 // Per tile: bit0 = occupied, bit1 = some tile in its 3x3x3 neighbourhood is
-// occupied (its nodes can receive mass).
+// occupied (its nodes can receive mass).  Each block of 256 tiles also stores
+// its counts of both kinds (occupied in the low, node tiles in the high half).
 __global__ __launch_bounds__(256) void k_bin_mark(const uint8_t* __restrict__ tile_flag,
-                                                  uint8_t* __restrict__ tile_mark, uint32_t tile_dim, uint32_t nt) {
+                                                  uint8_t* __restrict__ tile_mark, uint64_t* __restrict__ blk_cnt,
+                                                  uint32_t tile_dim, uint32_t nt) {
+  __shared__ uint64_t s_scan[256 / 64 + 1];
   const uint32_t T = blockIdx.x * blockDim.x + threadIdx.x;
-  if (T >= nt) return;
-  int tx, ty, tz;
-  tile_coords(T, tx, ty, tz);
-  const int td = int(tile_dim);
-  uint8_t node = 0;
-  for (int dz = -1; dz <= 1; dz++)
-    for (int dy = -1; dy <= 1; dy++)
-      for (int dx = -1; dx <= 1; dx++) {
-        int x = tx + dx, y = ty + dy, z = tz + dz;
-        if (x < 0 || y < 0 || z < 0 || x >= td || y >= td || z >= td) continue;
-        node |= tile_flag[morton3(x, y, z)];
-      }
-  tile_mark[T] = uint8_t((tile_flag[T] ? 1 : 0) | (node ? 2 : 0));
+  uint32_t m = 0;
+  if (T < nt) {
+    int tx, ty, tz;
+    tile_coords(T, tx, ty, tz);
+    const int td = int(tile_dim);
+    uint8_t node = 0;
+    for (int dz = -1; dz <= 1; dz++)
+      for (int dy = -1; dy <= 1; dy++)
+        for (int dx = -1; dx <= 1; dx++) {
+          int x = tx + dx, y = ty + dy, z = tz + dz;
+          if (x < 0 || y < 0 || z < 0 || x >= td || y >= td || z >= td) continue;
+          node |= tile_flag[morton3(x, y, z)];
+        }
+    m = (tile_flag[T] ? 1u : 0u) | (node ? 2u : 0u);
+    tile_mark[T] = uint8_t(m);
+  }
+  uint64_t tot;
+  (void)block_exclusive_scan<256>(uint64_t(m & 1u) | (uint64_t(m >> 1) << 32), s_scan, tot);
+  if (threadIdx.x == 0) blk_cnt[blockIdx.x] = tot;
 }
 
-// Single block: ordered compaction of occupied tiles and node tiles; resets the
-// occupied flags for the next bin.  Each thread owns a contiguous chunk of
-// tiles; when the chunk is a multiple of 16 its marks are read with 16-byte
-// loads so the loads of a thread are independent.
-__device__ __forceinline__ uint8_t byte_of(const uint4& v, int i) {
-  const uint32_t w = i < 4 ? v.x : (i < 8 ? v.y : (i < 12 ? v.z : v.w));
-  return uint8_t((w >> (8 * (i & 3))) & 0xFFu);
-}
-
-__global__ __launch_bounds__(1024) void k_bin_compact(const uint8_t* __restrict__ tile_mark,
-                                                      uint8_t* __restrict__ tile_flag, int32_t* __restrict__ tile_act,
-                                                      uint32_t* __restrict__ act_tiles,
-                                                      uint32_t* __restrict__ node_tiles,
-                                                      uint32_t* __restrict__ counters, uint32_t nt) {
-  __shared__ uint32_t s_wave[1024 / 64 + 1];
-  const uint32_t chunk = (nt + 1023u) / 1024u;
-  const uint32_t lo = min(nt, threadIdx.x * chunk), hi = min(nt, lo + chunk);
-  const bool vec = (chunk % 16u) == 0u;  // then nt is a multiple of 16 * 1024 and lo is 16-aligned
-  uint32_t na = 0, nn = 0;
-  if (vec) {
-    for (uint32_t T = lo; T < hi; T += 16) {
-      const uint4 v = *reinterpret_cast<const uint4*>(tile_mark + T);
-      const uint32_t w[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-      for (int k = 0; k < 4; k++) {
-        na += __popc(w[k] & 0x01010101u);
-        nn += __popc(w[k] & 0x02020202u);
-      }
-    }
-  } else {
-    for (uint32_t T = lo; T < hi; T++) {
-      const uint8_t m = tile_mark[T];
-      na += m & 1u;
-      nn += (m >> 1) & 1u;
-    }
+// Ordered compaction of occupied tiles and node tiles, one block per 256
+// tiles: the block's offsets are the sums of the counts of the blocks before
+// it; resets the occupied flags for the next bin.
+__global__ __launch_bounds__(256) void k_bin_compact(const uint8_t* __restrict__ tile_mark,
+                                                     const uint64_t* __restrict__ blk_cnt,
+                                                     uint8_t* __restrict__ tile_flag, int32_t* __restrict__ tile_act,
+                                                     uint32_t* __restrict__ act_tiles,
+                                                     uint32_t* __restrict__ node_tiles,
+                                                     uint32_t* __restrict__ counters, uint32_t nt) {
+  __shared__ uint64_t s_scan[256 / 64 + 1];
+  uint64_t before = 0;
+  for (uint32_t i = threadIdx.x; i < blockIdx.x; i += blockDim.x) before += blk_cnt[i];
+  uint64_t off;
+  (void)block_exclusive_scan<256>(before, s_scan, off);
+  const uint32_t T = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t m = T < nt ? tile_mark[T] : 0u;
+  uint64_t tot;
+  const uint64_t pre = block_exclusive_scan<256>(uint64_t(m & 1u) | (uint64_t(m >> 1) << 32), s_scan, tot);
+  const uint32_t oa = uint32_t(off) + uint32_t(pre), on = uint32_t(off >> 32) + uint32_t(pre >> 32);
+  if (T < nt) {
+    if (m & 1u) act_tiles[oa] = T;
+    tile_act[T] = (m & 1u) ? int32_t(oa) : -1;
+    if (m & 2u) node_tiles[on] = T;
+    tile_flag[T] = 0;
   }
-  uint32_t tot_a, tot_n;
-  uint32_t oa = block_exclusive_scan<1024>(na, s_wave, tot_a);
-  uint32_t on = block_exclusive_scan<1024>(nn, s_wave, tot_n);
-  if (vec) {
-    for (uint32_t T0 = lo; T0 < hi; T0 += 16) {
-      const uint4 v = *reinterpret_cast<const uint4*>(tile_mark + T0);
-#pragma unroll
-      for (int i = 0; i < 16; i++) {
-        const uint8_t m = byte_of(v, i);
-        const uint32_t T = T0 + i;
-        if (m & 1u) act_tiles[oa] = T;
-        tile_act[T] = (m & 1u) ? int32_t(oa) : -1;
-        oa += m & 1u;
-        if (m & 2u) node_tiles[on++] = T;
-      }
-      *reinterpret_cast<uint4*>(tile_flag + T0) = make_uint4(0u, 0u, 0u, 0u);
-    }
-  } else {
-    for (uint32_t T = lo; T < hi; T++) {
-      const uint8_t m = tile_mark[T];
-      if (m & 1u) {
-        act_tiles[oa] = T;
-        tile_act[T] = int32_t(oa);
-        oa++;
-      } else {
-        tile_act[T] = -1;
-      }
-      if (m & 2u) node_tiles[on++] = T;
-      tile_flag[T] = 0;
-    }
-  }
-  if (threadIdx.x == 0) {
-    counters[CNT_ACTIVE] = tot_a;
-    counters[CNT_NODE_TILES] = tot_n;
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+    counters[CNT_ACTIVE] = uint32_t(off) + uint32_t(tot);
+    counters[CNT_NODE_TILES] = uint32_t(off >> 32) + uint32_t(tot >> 32);
   }
 }
 
@@ -795,17 +762,20 @@ __global__ __launch_bounds__(512) void k_grid(SubParams p, const float4* __restr
                                               float4* __restrict__ grid_old, const float* __restrict__ sdf,
                                               const nb_collider* __restrict__ cols, const nb_force* __restrict__ forces) {
   const uint32_t nb_tiles = *count;
-  const int lx = threadIdx.x & 7, ly = (threadIdx.x >> 3) & 7, lz = threadIdx.x >> 6;
+  // thread j holds the tile's j-th node in Morton order (contiguous in HBM)
+  const int lx = int(morton_compact(threadIdx.x)), ly = int(morton_compact(threadIdx.x >> 1)),
+            lz = int(morton_compact(threadIdx.x >> 2));
   const int td = int(p.tile_dim);
   for (uint32_t b = blockIdx.x; b < nb_tiles; b += gridDim.x) {
     const uint32_t N = tiles[b];
     int tx, ty, tz;
     tile_coords(N, tx, ty, tz);
     const int nx = tx * 8 + lx, ny = ty * 8 + ly, nz = tz * 8 + lz;
-    const size_t nid = (size_t(N) << kTileBits) | morton3(lx, ly, lz);
+    const size_t nid = (size_t(N) << kTileBits) | threadIdx.x;
     float mx, my, mz, mass;
     if (FROM_PARTIALS) {
       mx = my = mz = mass = 0.f;
+      bool covered = false;
       // the tiles whose 10^3 box contains this node, in a fixed order
       for (int dz = -1; dz <= 1; dz++) {
         if ((dz < 0 && lz != 0) || (dz > 0 && lz != 7)) continue;
@@ -824,9 +794,13 @@ __global__ __launch_bounds__(512) void k_grid(SubParams p, const float4* __restr
             const int bx = lx - 8 * dx + 1, by = ly - 8 * dy + 1, bz = lz - 8 * dz + 1;
             const float4 q = partial[size_t(a) * kBoxNodes + (bz * kBox + by) * kBox + bx];
             mx += q.x; my += q.y; mz += q.z; mass += q.w;
+            covered = true;
           }
         }
       }
+      // A node outside every occupied tile's box receives nothing and lies in
+      // no particle's stencil: it is not written (readers treat it as empty).
+      if (!covered) continue;
     } else {
       const float4 g = grid_node[nid];
       mx = g.x; my = g.y; mz = g.z; mass = g.w;
@@ -1316,8 +1290,15 @@ void launch_bin(const DevState& d, const SubParams& p, uint32_t n_tiles, bool ke
     else
       k_bin_keys<true><<<cdiv(p.n, 256), 256, 0, s>>>(p, d.rec, d.keys, d.rank, d.cell_count, d.tile_flag);
   }
-  { ProfScope ps(pf, 2, s); k_bin_mark<<<cdiv(n_tiles, 256), 256, 0, s>>>(d.tile_flag, d.tile_mark, p.tile_dim, n_tiles); }
-  { ProfScope ps(pf, 3, s); k_bin_compact<<<1, 1024, 0, s>>>(d.tile_mark, d.tile_flag, d.tile_act, d.act_tiles, d.node_tiles, d.counters, n_tiles); }
+  {
+    ProfScope ps(pf, 2, s);
+    k_bin_mark<<<cdiv(n_tiles, 256), 256, 0, s>>>(d.tile_flag, d.tile_mark, d.blk_cnt, p.tile_dim, n_tiles);
+  }
+  {
+    ProfScope ps(pf, 3, s);
+    k_bin_compact<<<cdiv(n_tiles, 256), 256, 0, s>>>(d.tile_mark, d.blk_cnt, d.tile_flag, d.tile_act, d.act_tiles,
+                                                     d.node_tiles, d.counters, n_tiles);
+  }
   { ProfScope ps(pf, 4, s); k_bin_tile_scan<<<tg, 512, 0, s>>>(d.cell_count, d.cell_end, d.act_tiles, d.tile_total, d.counters); }
   { ProfScope ps(pf, 5, s); k_bin_tile_offsets<<<1, 1024, 0, s>>>(d.tile_total, d.tile_start, d.counters); }
   { ProfScope ps(pf, 6, s); k_bin_scatter<<<cdiv(p.n, 256), 256, 0, s>>>(p.n, d.keys, d.rank, d.cell_end, d.tile_act, d.tile_start, d.sorted); }

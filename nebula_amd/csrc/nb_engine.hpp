@@ -77,6 +77,7 @@ struct DevState {
   uint32_t* cell_end;     // NT*512
   uint8_t* tile_flag;     // NT: tile holds a particle (set by the key pass)
   uint8_t* tile_mark;     // NT: bit0 active, bit1 node tile (dilation)
+  uint64_t* blk_cnt;      // NT/256: per 256 tiles, active (low) and node (high) counts
   int32_t* tile_act;      // NT: index in act_tiles or -1
   uint32_t* act_tiles;    // NT
   uint32_t* node_tiles;   // NT
