@@ -268,14 +268,44 @@ __global__ __launch_bounds__(256) void k_bin_scatter(uint32_t n, const uint32_t*
 // Per cell: order its particles by index (the ranks of different waves arrive
 // in atomic order), then store the END offset, the reference's cellOffset.
 // Cells of up to 32 particles are sorted in registers by their thread:
-// independent loads, an odd-even transposition network that waves whose
-// cells are all in order skip, write-back only if the order changed.  Larger
+// independent loads, an odd-even transposition network as wide as the wave's
+// largest cell (8, 16 or 32) that stops once a round pair swaps nothing in the
+// wave, write-back only if the order changed.  Larger
 // cells (the SDF push-out piles particles up) are listed and then sorted by
 // the whole block: staged in LDS, each element's place is the number of
 // smaller elements (indices are distinct).  Cells beyond the LDS stage fall
 // back to an in-place insertion sort by their thread.
 constexpr int kRegSort = 32;
 constexpr uint32_t kBlockSort = 2048;
+
+template <int M>
+__device__ __forceinline__ void cell_reg_sort(uint32_t* __restrict__ sorted, uint32_t start, uint32_t m) {
+  uint32_t v[M];
+#pragma unroll
+  for (int i = 0; i < M; i++) v[i] = uint32_t(i) < m ? sorted[start + i] : 0xFFFFFFFFu;
+  bool changed = false;
+#pragma unroll
+  for (int round = 0; round < M; round += 2) {
+    bool swapped = false;
+#pragma unroll
+    for (int half = 0; half < 2; half++) {
+#pragma unroll
+      for (int i = half; i + 1 < M; i += 2) {
+        const uint32_t lo = min(v[i], v[i + 1]), hi = max(v[i], v[i + 1]);
+        swapped |= lo != v[i];
+        v[i] = lo;
+        v[i + 1] = hi;
+      }
+    }
+    changed |= swapped;
+    if (!__any(swapped)) break;
+  }
+  if (changed) {
+#pragma unroll
+    for (int i = 0; i < M; i++)
+      if (uint32_t(i) < m) sorted[start + i] = v[i];
+  }
+}
 
 __global__ __launch_bounds__(512) void k_bin_stabilize(uint32_t n, const uint32_t* __restrict__ cell_count,
                                                        uint32_t* __restrict__ cell_end,
@@ -295,32 +325,12 @@ __global__ __launch_bounds__(512) void k_bin_stabilize(uint32_t n, const uint32_
     const uint32_t cnt = cell_count[key];
     const uint32_t stop = min(start + cnt, n);
     const uint32_t m = stop > start ? stop - start : 0u;
-    if (m <= uint32_t(kRegSort)) {
-      uint32_t v[kRegSort];
-#pragma unroll
-      for (int i = 0; i < kRegSort; i++) v[i] = uint32_t(i) < m ? sorted[start + i] : 0xFFFFFFFFu;
-      // cells whose particles all came from one wave are already in order
-      bool ordered = true;
-#pragma unroll
-      for (int i = 0; i + 1 < kRegSort; i++) ordered &= v[i] <= v[i + 1];
-      bool changed = false;
-      if (!__all(ordered)) {
-#pragma unroll
-        for (int round = 0; round < kRegSort; round++) {
-#pragma unroll
-          for (int i = round & 1; i + 1 < kRegSort; i += 2) {
-            const uint32_t lo = min(v[i], v[i + 1]), hi = max(v[i], v[i + 1]);
-            changed |= lo != v[i];
-            v[i] = lo;
-            v[i + 1] = hi;
-          }
-        }
-      }
-      if (changed) {
-#pragma unroll
-        for (int i = 0; i < kRegSort; i++)
-          if (uint32_t(i) < m) sorted[start + i] = v[i];
-      }
+    const bool reg = m <= uint32_t(kRegSort);
+    if (reg) {
+      const uint32_t mr = m;
+      if (__any(mr > 16u)) cell_reg_sort<32>(sorted, start, mr);
+      else if (__any(mr > 8u)) cell_reg_sort<16>(sorted, start, mr);
+      else cell_reg_sort<8>(sorted, start, mr);
     } else if (m <= kBlockSort) {
       const uint32_t slot = atomicAdd(&s_nbig, 1u);
       s_big_start[slot] = start;
@@ -1263,7 +1273,7 @@ __global__ __launch_bounds__(256) void k_copy(float4* __restrict__ dst, const fl
 // ---------------------------------------------------------------------------
 static inline uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 // blocks for a tile-stride kernel: enough to fill the chip, never more than tiles
-static inline uint32_t tile_grid(uint32_t n_tiles) { return n_tiles < 8192u ? (n_tiles ? n_tiles : 1u) : 8192u; }
+static inline uint32_t tile_grid(uint32_t n_tiles) { return n_tiles < 2048u ? (n_tiles ? n_tiles : 1u) : 2048u; }
 
 struct ProfScope {
   Prof* pf;
