@@ -83,6 +83,16 @@ __device__ __forceinline__ T block_exclusive_scan(T v, T* s_wave, T& total) {
   return excl;
 }
 
+// Workgroups are dealt to the 8 XCDs (each with its own L2) round-robin by
+// id.  Block b is given chunk xcd_chunk(b, nb) of the work, so that each XCD
+// walks one contiguous eighth of the chunks: neighbouring particles, and the
+// grid nodes they share, stay in one L2.  A bijection of [0, nb).
+constexpr uint32_t kXCDs = 8;
+__device__ __forceinline__ uint32_t xcd_chunk(uint32_t b, uint32_t nb) {
+  const uint32_t x = b % kXCDs, i = b / kXCDs, q = nb / kXCDs, r = nb % kXCDs;
+  return x < r ? x * (q + 1u) + i : r * (q + 1u) + (x - r) * q + i;
+}
+
 // cell key of a position: shaders/mpm_common.glsl:190-195 (IEEE quotient,
 // clamp to [0, gridRes-1], truncation; NaN maps to 0 as fmaxf returns the
 // non-NaN operand)
@@ -899,7 +909,7 @@ __global__ __launch_bounds__(256) void k_g2p(SubParams p, float4* __restrict__ r
     for (uint32_t e = threadIdx.x; e < nf; e += blockDim.x) dst[e] = src[e];
     __syncthreads();
   }
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t k = xcd_chunk(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (k >= p.n) return;
   const uint32_t q = sorted ? sorted[k] : k;
   float4* r = rec + size_t(q) * kRec;
