@@ -33,75 +33,13 @@
 
 #include "nb_device_math.hpp"
 #include "nb_engine.hpp"
+#include "nb_kernel_util.hpp"
 
 namespace nb {
 
 const char* const kKernelNames[kNumProfiledKernels] = {
     "bin_clear", "bin_keys", "bin_mark", "bin_compact", "bin_tile_scan",
     "bin_tile_offsets", "bin_scatter", "bin_stabilize", "p2g", "grid_update", "g2p"};
-
-// ---------------------------------------------------------------------------
-// helpers
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ void tile_coords(uint32_t T, int& tx, int& ty, int& tz) {
-  tx = int(morton_compact(T));
-  ty = int(morton_compact(T >> 1));
-  tz = int(morton_compact(T >> 2));
-}
-
-// Exclusive scan across a block of BS threads (BS a multiple of 64) of a
-// 32- or 64-bit unsigned value.  s_wave must hold BS/64 + 1 values.  Returns
-// the exclusive prefix; *total is the block sum.  Contains barriers: call from
-// uniform control flow.
-template <int BS, class T>
-__device__ __forceinline__ T block_exclusive_scan(T v, T* s_wave, T& total) {
-  constexpr int NW = BS / 64;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  T x = v;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    T y = __shfl_up(x, d);
-    if (lane >= d) x += y;
-  }
-  if (lane == 63) s_wave[wave] = x;
-  __syncthreads();
-  if (wave == 0) {
-    T w = lane < NW ? s_wave[lane] : T(0);
-    T ws = w;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      T y = __shfl_up(ws, d);
-      if (lane >= d) ws += y;
-    }
-    if (lane < NW) s_wave[lane] = ws - w;
-    if (lane == NW - 1) s_wave[NW] = ws;
-  }
-  __syncthreads();
-  T excl = x - v + s_wave[wave];
-  total = s_wave[NW];
-  __syncthreads();
-  return excl;
-}
-
-// Workgroups are dealt to the 8 XCDs (each with its own L2) round-robin by
-// id.  Block b is given chunk xcd_chunk(b, nb) of the work, so that each XCD
-// walks one contiguous eighth of the chunks: neighbouring particles, and the
-// grid nodes they share, stay in one L2.  A bijection of [0, nb).
-constexpr uint32_t kXCDs = 8;
-__device__ __forceinline__ uint32_t xcd_chunk(uint32_t b, uint32_t nb) {
-  const uint32_t x = b % kXCDs, i = b / kXCDs, q = nb / kXCDs, r = nb % kXCDs;
-  return x < r ? x * (q + 1u) + i : r * (q + 1u) + (x - r) * q + i;
-}
-
-// cell key of a position: shaders/mpm_common.glsl:190-195 (IEEE quotient,
-// clamp to [0, gridRes-1], truncation; NaN maps to 0 as fmaxf returns the
-// non-NaN operand)
-__device__ __forceinline__ uint32_t cell_key(const SubParams& p, float x, float y, float z) {
-  float lx = fminf(fmaxf((x - p.wmin[0]) / p.h, 0.f), float(p.grid_res[0]) - 1.f);
-  float ly = fminf(fmaxf((y - p.wmin[1]) / p.h, 0.f), float(p.grid_res[1]) - 1.f);
-  float lz = fminf(fmaxf((z - p.wmin[2]) / p.h, 0.f), float(p.grid_res[2]) - 1.f);
-  return morton3(uint32_t(lx), uint32_t(ly), uint32_t(lz));
-}
 
 // ---------------------------------------------------------------------------
 // BIN
@@ -377,39 +315,6 @@ __global__ __launch_bounds__(512) void k_bin_stabilize(uint32_t n, const uint32_
 // ---------------------------------------------------------------------------
 // P2G
 // ---------------------------------------------------------------------------
-// Per-axis stencil data of a particle relative to the three nodes c-1, c, c+1.
-struct Axis3 {
-  float w[3], g[3], dx[3];  // weight, weight derivative / h, x_node - x_p
-};
-
-// smallest positive entry (+inf if none)
-__device__ __forceinline__ float min_positive3(const float w[3]) {
-  const float a = w[0] > 0.f ? w[0] : INFINITY, b = w[1] > 0.f ? w[1] : INFINITY, c = w[2] > 0.f ? w[2] : INFINITY;
-  return fminf(a, fminf(b, c));
-}
-
-// Quadratic B-spline weight and derivative (shaders/mpm_common.glsl:198-210),
-// evaluated with selects instead of branches; inv_h = 1/h replaces the
-// division of the derivative by h.
-__device__ __forceinline__ void axis_stencil_f(float f, float xp, float wmin, float h, float inv_h, int first_node,
-                                               Axis3& a) {
-#pragma unroll
-  for (int i = 0; i < 3; i++) {
-    const float nf = float(first_node + i) + 0.5f;
-    const float d = f - nf;
-    const float ad = fabsf(d);
-    const float t = 1.5f - ad;
-    const bool in = ad < 0.5f, mid = ad < 1.5f;
-    a.w[i] = in ? 0.75f - ad * ad : (mid ? 0.5f * t * t : 0.f);
-    a.g[i] = (in ? -2.f * d : (mid ? (d > 0.f ? -t : t) : 0.f)) * inv_h;
-    a.dx[i] = nf * h + wmin - xp;
-  }
-}
-// f = (x - wmin) / h, the IEEE quotient the cell keys use
-__device__ __forceinline__ void axis_stencil(float xp, float wmin, float h, float inv_h, int first_node, Axis3& a) {
-  axis_stencil_f((xp - wmin) / h, xp, wmin, h, inv_h, first_node, a);
-}
-
 // P2G: one block per occupied tile.  The tile's occupied cells are compacted
 // (in Morton order) onto the block's threads as work items: a cell of up to
 // kChunk particles is one item, a more crowded cell is split into up to
@@ -705,76 +610,6 @@ __global__ __launch_bounds__(kP2GThreads, 2) void k_p2g(SubParams p, const float
 // ---------------------------------------------------------------------------
 // GRID UPDATE
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float sample_sdf(const float* sdf, const SubParams& p, int x, int y, int z) {
-  if (x < 0 || y < 0 || z < 0 || x >= p.grid_res[0] || y >= p.grid_res[1] || z >= p.grid_res[2]) return 1e6f;
-  return sdf[morton3(x, y, z)];
-}
-
-__device__ __forceinline__ float len3(float x, float y, float z) { return sqrtf(x * x + y * y + z * z); }
-
-// applyColliderBC — shaders/collider_common.glsl:62-73
-__device__ __forceinline__ void collider_bc(float& vx, float& vy, float& vz, float nx, float ny, float nz,
-                                            const nb_collider& c) {
-  float rx = vx - c.vx, ry = vy - c.vy, rz = vz - c.vz;
-  float vn = rx * nx + ry * ny + rz * nz;
-  if (vn < 0.f) {
-    float nvx = vn * nx, nvy = vn * ny, nvz = vn * nz;
-    float tx = rx - nvx, ty = ry - nvy, tz = rz - nvz;
-    float tl = len3(tx, ty, tz);
-    if (tl > 1e-8f) {
-      float sc = fmaxf(0.f, 1.f - c.friction * fabsf(vn) / tl);
-      tx *= sc; ty *= sc; tz *= sc;
-    }
-    vx = c.vx + tx - c.restitution * nvx;
-    vy = c.vy + ty - c.restitution * nvy;
-    vz = c.vz + tz - c.restitution * nvz;
-  }
-}
-
-// analytic SDFs — shaders/collider_common.glsl:22-58; returns sdf, writes normal
-__device__ __forceinline__ float collider_sdf(const nb_collider& c, float x, float y, float z, float& nx, float& ny,
-                                              float& nz) {
-  if (c.type == NB_COLL_SPHERE) {
-    float dx = x - c.px, dy = y - c.py, dz = z - c.pz;
-    float dist = len3(dx, dy, dz);
-    if (dist > 1e-8f) { nx = dx / dist; ny = dy / dist; nz = dz / dist; } else { nx = 0.f; ny = 1.f; nz = 0.f; }
-    return dist - c.radius;
-  } else if (c.type == NB_COLL_BOX) {
-    float rx = x - c.px, ry = y - c.py, rz = z - c.pz;
-    float qx = fabsf(rx) - c.nx, qy = fabsf(ry) - c.ny, qz = fabsf(rz) - c.nz;
-    float px = fmaxf(qx, 0.f), py = fmaxf(qy, 0.f), pz = fmaxf(qz, 0.f);
-    float ql = len3(px, py, pz);
-    float sdf = ql + fminf(fmaxf(qx, fmaxf(qy, qz)), 0.f);
-    if (sdf < 0.f) {
-      float ex = c.nx - fabsf(rx), ey = c.ny - fabsf(ry), ez = c.nz - fabsf(rz);
-      nx = ny = nz = 0.f;
-      if (ex < ey && ex < ez) nx = glsl_sign(rx + 1e-8f);
-      else if (ey < ez) ny = glsl_sign(ry + 1e-8f);
-      else nz = glsl_sign(rz + 1e-8f);
-    } else {
-      float sx = glsl_sign(rx + 1e-8f), sy = glsl_sign(ry + 1e-8f), sz = glsl_sign(rz + 1e-8f);
-      float ux, uy, uz;
-      if (ql > 1e-8f) { ux = px * sx; uy = py * sy; uz = pz * sz; } else { ux = sx; uy = sy; uz = sz; }
-      float ul = len3(ux, uy, uz);
-      nx = ux / ul; ny = uy / ul; nz = uz / ul;
-    }
-    return sdf;
-  } else if (c.type == NB_COLL_CAPSULE) {
-    float al = len3(c.nx, c.ny, c.nz);
-    float adx, ady, adz;
-    if (al > 1e-8f) { adx = c.nx / al; ady = c.ny / al; adz = c.nz / al; } else { adx = 0.f; ady = 1.f; adz = 0.f; }
-    float t = fminf(fmaxf((x - c.px) * adx + (y - c.py) * ady + (z - c.pz) * adz, 0.f), al);
-    float dx = x - (c.px + t * adx), dy = y - (c.py + t * ady), dz = z - (c.pz + t * adz);
-    float dist = len3(dx, dy, dz);
-    if (dist > 1e-8f) { nx = dx / dist; ny = dy / dist; nz = dz / dist; } else { nx = 0.f; ny = 1.f; nz = 0.f; }
-    return dist - c.radius;
-  }
-  // plane
-  float l = len3(c.nx, c.ny, c.nz);
-  nx = c.nx / l; ny = c.ny / l; nz = c.nz / l;
-  return (x - c.px) * nx + (y - c.py) * ny + (z - c.pz) * nz;
-}
-
 template <bool FROM_PARTIALS, bool UPDATE>
 __global__ __launch_bounds__(512) void k_grid(SubParams p, const float4* __restrict__ partial,
                                               const int32_t* __restrict__ tile_act, const uint32_t* __restrict__ tiles,
