@@ -269,6 +269,35 @@ int nb_mpm_active_node_count(nb_mpm* h, uint64_t* g_act);
 /* Number of occupied 8^3 cell tiles after the last bin. */
 int nb_mpm_active_tile_count(nb_mpm* h, uint32_t* tiles);
 
+/* ---- Reverse mode (adjoint) ----
+ * The reference has no differentiation, so these entry points have no
+ * counterpart in it.  Semantics: the substep
+ * with corrected_svd = 1 (the reference's transposed-SVD stress is not a
+ * differentiable function of F), PIC or APIC, elastic / Von Mises /
+ * Drucker-Prager (granular powder) materials, forces, walls and the dense
+ * SDF boundary; FLIP, analytic colliders, fluid and mud return
+ * NB_ERR_UNSUPPORTED.  The J reset, the SDF push-out and the world clamp are
+ * treated as constants where they act.
+ *
+ * Use: nb_mpm_adjoint_begin(h, n, segment) -> nb_mpm_forward_recorded(h, dt)
+ * runs n substeps from the current state, keeping a checkpoint every `segment`
+ * substeps -> fill the adjoint of the final state, dL/d(state), with
+ * nb_mpm_set_adjoint (same eight float4 arrays and packing as the state:
+ * dL/dx in P.xyz, dL/dv in v.xyz, dL/dF columns in F0..2.xyz, dL/dtau in the
+ * w lanes, dL/dB columns in B0..2.xyz) -> nb_mpm_backward replaces it with
+ * dL/d(state at the start of the run), recomputing each segment from its
+ * checkpoint, and returns dL/d(mu, lambda, M_friction, q_cohesion) of every
+ * material (material_count x 4 floats) in d_params.  The engine is left in
+ * the state the recorded run ended in.  Memory: (ceil(n/segment) + segment
+ * + 3) x 128 B x capacity. */
+int nb_mpm_adjoint_begin(nb_mpm* h, uint32_t n_substeps, uint32_t segment);
+int nb_mpm_forward_recorded(nb_mpm* h, float dt_sub);
+int nb_mpm_set_adjoint(nb_mpm* h, int field, const float* in4, uint32_t first, uint32_t count);
+int nb_mpm_get_adjoint(nb_mpm* h, int field, float* out4, uint32_t first, uint32_t count);
+int nb_mpm_adjoint_device_ptr(nb_mpm* h, int field, void** dptr);
+int nb_mpm_backward(nb_mpm* h, float* d_params, uint32_t params_cap);
+int nb_mpm_adjoint_end(nb_mpm* h);
+
 #ifdef __cplusplus
 }
 #endif

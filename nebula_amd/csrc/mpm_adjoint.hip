@@ -1,0 +1,700 @@
+// mpm_adjoint.hip — reverse mode of the MPM substep (G2P, grid update, P2G),
+// for the proper polar SVD semantics, PIC/APIC transfers, elastic, Von Mises
+// and Drucker-Prager materials, forces, walls and the dense-SDF boundary.
+//
+// Per substep s (state x, v, F, B, tau -> x', v', F', B', tau'), given the
+// adjoint of the state after the substep, the kernels run in this order on
+// the recomputed forward (bins, P2G partials and grid velocities of s):
+//   k_g2p_bwd      per particle: the G2P chain backwards (advection, clamps,
+//                  push-out, J reset, return map + Hencky adjoint, F update);
+//                  writes d/dF, the position part from G2P's weights, and the
+//                  coefficients of dL/dv_node: w A + w M_B d + M_G grad w
+//   k_adj_scatter  the coefficients scattered to a grid adjoint, tile by
+//                  tile like P2G (register sums, fixed-order LDS merge)
+//   k_grid_bwd     per node: the grid update backwards -> dL/dp, dL/dm
+//   k_p2g_bwd      per particle: P2G backwards as a 27-node gather
+//                  -> d/dv, d/dB, d/dtau and the position part from P2G
+//   k_pgrad_reduce per-wave material-parameter partials summed in a fixed order
+// Positions enter every weight, so both transfers contribute the weight
+// gradients and Hessians.  Everything is deterministic.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "nb_adjoint_math.hpp"
+#include "nb_device_math.hpp"
+#include "nb_engine.hpp"
+#include "nb_kernel_util.hpp"
+
+namespace nb {
+
+// Per-axis stencil with second derivatives: weight, d/dx, d2/dx2, x_node - x_p.
+struct AxisD {
+  float w[3], g[3], hh[3], dx[3];
+};
+
+__device__ __forceinline__ void axis_stencil_d(float xp, float wmin, float h, int first_node, AxisD& a) {
+  const float f = (xp - wmin) / h;
+  const float inv_h = 1.f / h;
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    const float nf = float(first_node + i) + 0.5f;
+    const float d = f - nf;
+    const float ad = fabsf(d);
+    const float t = 1.5f - ad;
+    const bool in = ad < 0.5f, mid = ad < 1.5f;
+    a.w[i] = in ? 0.75f - ad * ad : (mid ? 0.5f * t * t : 0.f);
+    a.g[i] = (in ? -2.f * d : (mid ? (d > 0.f ? -t : t) : 0.f)) * inv_h;
+    a.hh[i] = (in ? -2.f : (mid ? 1.f : 0.f)) * inv_h * inv_h;
+    a.dx[i] = nf * h + wmin - xp;
+  }
+}
+
+// weight, gradient and Hessian of node (ix, iy, iz)
+struct NodeW {
+  float w, dw[3], H[3][3], d[3];
+};
+__device__ __forceinline__ void node_w(const AxisD& X, const AxisD& Y, const AxisD& Z, int ix, int iy, int iz, NodeW& n) {
+  const float wx = X.w[ix], wy = Y.w[iy], wz = Z.w[iz];
+  const float gx = X.g[ix], gy = Y.g[iy], gz = Z.g[iz];
+  n.w = wx * wy * wz;
+  n.dw[0] = gx * wy * wz;
+  n.dw[1] = wx * gy * wz;
+  n.dw[2] = wx * wy * gz;
+  n.H[0][0] = X.hh[ix] * wy * wz;
+  n.H[1][1] = wx * Y.hh[iy] * wz;
+  n.H[2][2] = wx * wy * Z.hh[iz];
+  n.H[0][1] = n.H[1][0] = gx * gy * wz;
+  n.H[0][2] = n.H[2][0] = gx * wy * gz;
+  n.H[1][2] = n.H[2][1] = wx * gy * gz;
+  n.d[0] = X.dx[ix];
+  n.d[1] = Y.dx[iy];
+  n.d[2] = Z.dx[iz];
+}
+
+__device__ __forceinline__ int adj_model(uint32_t m) {
+  return m == NB_MAT_VON_MISES ? 1 : ((m == NB_MAT_DRUCKER_PRAGER || m == NB_MAT_GRANULAR_POWDER) ? 2 : 0);
+}
+
+// record <-> matrices (m[col][row]; F columns packed as in the forward record)
+__device__ __forceinline__ m3 rec_F(const float4* r) {
+  const float4 t = r[R_T], a = r[R_FA], b = r[R_FB];
+  m3 F;
+  F.m[0][0] = t.w; F.m[0][1] = a.x; F.m[0][2] = a.y;
+  F.m[1][0] = a.z; F.m[1][1] = a.w; F.m[1][2] = b.x;
+  F.m[2][0] = b.y; F.m[2][1] = b.z; F.m[2][2] = b.w;
+  return F;
+}
+__device__ __forceinline__ m3 rec_B(const float4* r) {
+  const float4 b0 = r[R_B0], b1 = r[R_B1], b2 = r[R_B2];
+  m3 B;
+  B.m[0][0] = b0.x; B.m[0][1] = b0.y; B.m[0][2] = b0.z;
+  B.m[1][0] = b1.x; B.m[1][1] = b1.y; B.m[1][2] = b1.z;
+  B.m[2][0] = b2.x; B.m[2][1] = b2.y; B.m[2][2] = b2.z;
+  return B;
+}
+// tau as the full symmetric matrix of its six packed values
+__device__ __forceinline__ m3 rec_tau(const float4* r) {
+  const float4 t = r[R_T], b0 = r[R_B0], b1 = r[R_B1], b2 = r[R_B2];
+  m3 T;
+  T.m[0][0] = t.x; T.m[1][1] = t.y; T.m[2][2] = t.z;
+  T.m[1][0] = T.m[0][1] = b0.w;
+  T.m[2][0] = T.m[0][2] = b1.w;
+  T.m[2][1] = T.m[1][2] = b2.w;
+  return T;
+}
+// The adjoint of a packed symmetric value enters both of its matrix slots:
+// dL/dT_rc = dL/dT_cr = g_packed / 2 off the diagonal.
+__device__ __forceinline__ m3 adj_tau_full(const float4* g) {
+  const float4 t = g[R_T], b0 = g[R_B0], b1 = g[R_B1], b2 = g[R_B2];
+  m3 T;
+  T.m[0][0] = t.x; T.m[1][1] = t.y; T.m[2][2] = t.z;
+  T.m[1][0] = T.m[0][1] = 0.5f * b0.w;
+  T.m[2][0] = T.m[0][2] = 0.5f * b1.w;
+  T.m[2][1] = T.m[1][2] = 0.5f * b2.w;
+  return T;
+}
+
+constexpr int kCoef = 6;  // float4 per particle: A (3), M_B (9), M_G (9)
+
+// ---------------------------------------------------------------------------
+// G2P backward
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_g2p_bwd(SubParams p, const float4* __restrict__ rec,
+                                                 const float4* __restrict__ adj_in, float4* __restrict__ adj_out,
+                                                 float4* __restrict__ coef, float* __restrict__ pgrad_w,
+                                                 const nb_material* __restrict__ mats,
+                                                 const uint32_t* __restrict__ sorted,
+                                                 const float4* __restrict__ grid_node, const float* __restrict__ sdf) {
+  const uint32_t k = xcd_chunk(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const bool valid = k < p.n;
+  const uint32_t q = valid ? sorted[k] : 0u;
+  const float4* r = rec + size_t(q) * kRec;
+  float g4[4] = {0.f, 0.f, 0.f, 0.f};
+  uint32_t mid = 0;
+  if (valid) {
+    const float4 pp = r[R_P];
+    const float xs[3] = {pp.x, pp.y, pp.z};
+    mid = __float_as_uint(r[R_V].w);
+    mid = mid > p.material_count - 1u ? p.material_count - 1u : mid;
+    const nb_material mat = mats[mid];
+    const int model = adj_model(mat.model);
+    int base[3];
+    AxisD ax[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      float f = (xs[a] - p.wmin[a]) / p.h;
+      f = fminf(fmaxf(f, -4.f), float(p.grid_res[a]) + 4.f);
+      base[a] = int(floorf(f)) - 1;
+      axis_stencil_d(xs[a], p.wmin[a], p.h, base[a], ax[a]);
+    }
+    auto node_at = [&](int ix, int iy, int iz, float4& g) -> bool {
+      const int nx = base[0] + ix, ny = base[1] + iy, nz = base[2] + iz;
+      if (nx < 0 || nx >= p.grid_res[0] || ny < 0 || ny >= p.grid_res[1] || nz < 0 || nz >= p.grid_res[2])
+        return false;
+      g = grid_node[morton3(nx, ny, nz)];
+      return g.w >= 1e-10f;
+    };
+    // forward recompute (shaders/mpm_g2p.comp:87-118)
+    float vp[3] = {0.f, 0.f, 0.f};
+    m3 vg = m3_zero(), Bn = m3_zero();
+#pragma unroll
+    for (int iz = 0; iz < 3; iz++)
+#pragma unroll
+      for (int iy = 0; iy < 3; iy++)
+#pragma unroll
+        for (int ix = 0; ix < 3; ix++) {
+          float4 g;
+          if (!node_at(ix, iy, iz, g)) continue;
+          NodeW n;
+          node_w(ax[0], ax[1], ax[2], ix, iy, iz, n);
+          if (n.w < 1e-12f) continue;
+          const float v[3] = {g.x, g.y, g.z};
+#pragma unroll
+          for (int rr = 0; rr < 3; rr++) {
+            vp[rr] += n.w * v[rr];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+              vg.m[c][rr] += v[rr] * n.dw[c];
+              Bn.m[c][rr] += n.w * v[rr] * n.d[c];
+            }
+          }
+        }
+    const m3 F = rec_F(r);
+    m3 L;
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+      for (int rr = 0; rr < 3; rr++) L.m[c][rr] = (c == rr ? 1.f : 0.f) + p.dt * vg.m[c][rr];
+    const m3 Ft = m3_mul(L, F);
+    const PlasticOut po = project_plastic(Ft, model == 2, mat.mu, mat.lambda, mat.q_max, mat.M_friction,
+                                          mat.q_cohesion, true);
+    const m3 Fp = model == 0 ? Ft : po.F;
+    const float Jd = fabsf(m3_det(Fp));
+    const bool jreset = Jd < 0.15f || Jd > 10.f;
+    const bool apic = p.flip_ratio < -0.5f;
+    float v1[3] = {vp[0], vp[1], vp[2]};
+    if (jreset) { v1[0] *= 0.1f; v1[1] *= 0.1f; v1[2] *= 0.1f; }
+    float x2[3] = {xs[0] + p.dt * v1[0], xs[1] + p.dt * v1[1], xs[2] + p.dt * v1[2]};
+    float v2[3] = {v1[0], v1[1], v1[2]};
+    bool pushed = false, push_proj = false, push_stuck = false;
+    float u[3] = {0.f, 0.f, 0.f};
+    if (sdf) {
+      const int gmx[3] = {p.grid_res[0] - 1, p.grid_res[1] - 1, p.grid_res[2] - 1};
+      int ic[3];
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+        const float f = fminf(fmaxf((x2[a] - p.wmin[a]) / p.h, -4.f), float(p.grid_res[a]) + 4.f);
+        ic[a] = min(max(int(floorf(f)), 0), gmx[a]);
+      }
+      const float psdf = sdf[morton3(ic[0], ic[1], ic[2])];
+      if (psdf < 0.f) {
+        pushed = true;
+        const float gnx = sdf[morton3(min(ic[0] + 1, gmx[0]), ic[1], ic[2])] - sdf[morton3(max(ic[0] - 1, 0), ic[1], ic[2])];
+        const float gny = sdf[morton3(ic[0], min(ic[1] + 1, gmx[1]), ic[2])] - sdf[morton3(ic[0], max(ic[1] - 1, 0), ic[2])];
+        const float gnz = sdf[morton3(ic[0], ic[1], min(ic[2] + 1, gmx[2]))] - sdf[morton3(ic[0], ic[1], max(ic[2] - 1, 0))];
+        const float gl = len3(gnx, gny, gnz);
+        if (gl > 1e-6f) {
+          u[0] = gnx / gl; u[1] = gny / gl; u[2] = gnz / gl;
+#pragma unroll
+          for (int a = 0; a < 3; a++) x2[a] -= psdf * u[a];
+          const float vn = v2[0] * u[0] + v2[1] * u[1] + v2[2] * u[2];
+          if (vn < 0.f) {
+            push_proj = true;
+#pragma unroll
+            for (int a = 0; a < 3; a++) v2[a] -= vn * u[a];
+          }
+        } else {
+          push_stuck = true;
+#pragma unroll
+          for (int a = 0; a < 3; a++) { x2[a] = xs[a]; v2[a] = 0.f; }
+        }
+      }
+    }
+    // clamp flags (shaders/mpm_g2p.comp:255-263)
+    bool xcl[3], vcl[3];
+    {
+      const float mg = p.h * 1.5f;
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+        const float lo = p.wmin[a] + mg, hi = p.wmax[a] - mg;
+        xcl[a] = false;
+        vcl[a] = false;
+        if (x2[a] < lo) { x2[a] = lo; xcl[a] = true; if (v2[a] < 0.f) { vcl[a] = true; v2[a] = 0.f; } }
+        if (x2[a] > hi) { x2[a] = hi; xcl[a] = true; if (v2[a] > 0.f) { vcl[a] = true; v2[a] = 0.f; } }
+      }
+    }
+
+    // ---- backward ----
+    const float4* ga = adj_in + size_t(q) * kRec;
+    const float4 gP = ga[R_P], gV = ga[R_V];
+    float gx2[3] = {gP.x, gP.y, gP.z}, gv2[3] = {gV.x, gV.y, gV.z};
+    m3 gB2 = rec_B(ga);
+    m3 gT = adj_tau_full(ga);
+    m3 gF2 = rec_F(ga);
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      if (xcl[a]) gx2[a] = 0.f;
+      if (vcl[a]) gv2[a] = 0.f;
+    }
+    float gx[3] = {0.f, 0.f, 0.f};  // d/dx of the state before the substep
+    if (pushed) {
+      gF2 = m3_zero();
+      gT = m3_zero();
+      if (push_stuck) {
+#pragma unroll
+        for (int a = 0; a < 3; a++) { gx[a] += gx2[a]; gx2[a] = 0.f; gv2[a] = 0.f; }
+      } else if (push_proj) {
+        const float un = gv2[0] * u[0] + gv2[1] * u[1] + gv2[2] * u[2];
+#pragma unroll
+        for (int a = 0; a < 3; a++) gv2[a] -= un * u[a];
+      }
+    }
+    // x' = x + dt v'
+#pragma unroll
+    for (int a = 0; a < 3; a++) { gx[a] += gx2[a]; gv2[a] += p.dt * gx2[a]; }
+    float A[3];
+    if (jreset) {
+      gF2 = m3_zero();
+      gT = m3_zero();
+#pragma unroll
+      for (int a = 0; a < 3; a++) A[a] = 0.1f * gv2[a];
+    } else {
+#pragma unroll
+      for (int a = 0; a < 3; a++) A[a] = gv2[a];
+    }
+    const m3 MB = apic ? gB2 : m3_zero();
+    m3 gFt = m3_zero();
+    if (!jreset && !pushed)
+      gFt = plastic_hencky_adjoint(Ft, model, mat.mu, mat.lambda, mat.q_max, mat.M_friction, mat.q_cohesion, gF2,
+                                   gT, g4);
+    // F_tmp = L F,  L = I + dt grad v
+    m3 MG = m3_mult(gFt, F);
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+      for (int rr = 0; rr < 3; rr++) MG.m[c][rr] *= p.dt;
+    const m3 gF = m3_tmul(L, gFt);
+    // position through G2P's weights
+#pragma unroll
+    for (int iz = 0; iz < 3; iz++)
+#pragma unroll
+      for (int iy = 0; iy < 3; iy++)
+#pragma unroll
+        for (int ix = 0; ix < 3; ix++) {
+          float4 g;
+          if (!node_at(ix, iy, iz, g)) continue;
+          NodeW n;
+          node_w(ax[0], ax[1], ax[2], ix, iy, iz, n);
+          if (n.w < 1e-12f) continue;
+          const float v[3] = {g.x, g.y, g.z};
+          const float vA = v[0] * A[0] + v[1] * A[1] + v[2] * A[2];
+          float vMBd = 0.f, MBtv[3], MGtv[3];
+#pragma unroll
+          for (int c = 0; c < 3; c++) {
+            MBtv[c] = v[0] * MB.m[c][0] + v[1] * MB.m[c][1] + v[2] * MB.m[c][2];
+            MGtv[c] = v[0] * MG.m[c][0] + v[1] * MG.m[c][1] + v[2] * MG.m[c][2];
+            vMBd += MBtv[c] * n.d[c];
+          }
+#pragma unroll
+          for (int b = 0; b < 3; b++)
+            gx[b] += (vA + vMBd) * n.dw[b] - n.w * MBtv[b] + MGtv[0] * n.H[0][b] + MGtv[1] * n.H[1][b] +
+                     MGtv[2] * n.H[2][b];
+        }
+    float4* go = adj_out + size_t(q) * kRec;
+    go[R_P] = make_float4(gx[0], gx[1], gx[2], 0.f);
+    go[R_T] = make_float4(0.f, 0.f, 0.f, gF.m[0][0]);
+    go[R_FA] = make_float4(gF.m[0][1], gF.m[0][2], gF.m[1][0], gF.m[1][1]);
+    go[R_FB] = make_float4(gF.m[1][2], gF.m[2][0], gF.m[2][1], gF.m[2][2]);
+    float cf[kCoef * 4];
+    cf[0] = A[0]; cf[1] = A[1]; cf[2] = A[2];
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+      for (int rr = 0; rr < 3; rr++) {
+        cf[3 + 3 * c + rr] = MB.m[c][rr];
+        cf[12 + 3 * c + rr] = MG.m[c][rr];
+      }
+    cf[21] = cf[22] = cf[23] = 0.f;
+    float4* co = coef + size_t(q) * kCoef;
+#pragma unroll
+    for (int j = 0; j < kCoef; j++) co[j] = make_float4(cf[4 * j], cf[4 * j + 1], cf[4 * j + 2], cf[4 * j + 3]);
+  }
+  // per-wave material-parameter partials: slot (material, parameter), 64 slots
+  const uint32_t lane = threadIdx.x & 63u;
+  float* slot = pgrad_w + size_t(k >> 6) * (NB_MAX_MATERIALS * 4);
+  slot[lane] = 0.f;
+  uint64_t pending = __ballot(valid);
+  while (pending) {
+    const int leader = __ffsll((unsigned long long)pending) - 1;
+    const uint32_t lmid = __shfl(mid, leader);
+    const bool mine = valid && mid == lmid;
+    float s4[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      float x = mine ? g4[j] : 0.f;
+#pragma unroll
+      for (int o = 32; o >= 1; o >>= 1) x += __shfl_xor(x, o);
+      s4[j] = x;
+    }
+    if (lane == 0) {
+#pragma unroll
+      for (int j = 0; j < 4; j++) slot[lmid * 4 + j] = s4[j];
+    }
+    pending &= ~__ballot(mine);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// scatter of the G2P-backward coefficients to the grid adjoint
+// ---------------------------------------------------------------------------
+constexpr int kAdjThreads = 256;
+
+__global__ __launch_bounds__(kAdjThreads) void k_adj_scatter(SubParams p, const float4* __restrict__ rec,
+                                                             const float4* __restrict__ coef,
+                                                             const uint32_t* __restrict__ cell_count,
+                                                             const uint32_t* __restrict__ cell_end,
+                                                             const uint32_t* __restrict__ sorted,
+                                                             const uint32_t* __restrict__ act_tiles,
+                                                             const uint32_t* __restrict__ counters,
+                                                             float4* __restrict__ partial_adj) {
+  __shared__ float s_x[kBoxNodes], s_y[kBoxNodes], s_z[kBoxNodes];
+  __shared__ uint16_t s_cell[kTileCells];
+  __shared__ uint32_t s_wave[kAdjThreads / 64 + 1];
+  const uint32_t na = counters[CNT_ACTIVE];
+  for (uint32_t a = blockIdx.x; a < na; a += gridDim.x) {
+    const uint32_t T = act_tiles[a];
+    int tx, ty, tz;
+    tile_coords(T, tx, ty, tz);
+    const size_t tbase = size_t(T) << kTileBits;
+    for (int e = threadIdx.x; e < kBoxNodes; e += kAdjThreads) { s_x[e] = 0.f; s_y[e] = 0.f; s_z[e] = 0.f; }
+    const uint32_t ca = 2u * threadIdx.x, cb = ca + 1u;
+    const bool oa = cell_count[tbase + ca] != 0u, ob = cell_count[tbase + cb] != 0u;
+    uint32_t n_occ;
+    uint32_t pos = block_exclusive_scan<kAdjThreads>(uint32_t(oa) + uint32_t(ob), s_wave, n_occ);
+    if (oa) s_cell[pos++] = uint16_t(ca);
+    if (ob) s_cell[pos] = uint16_t(cb);
+    __syncthreads();
+    for (uint32_t base = 0; base < n_occ; base += kAdjThreads) {
+      const uint32_t j = base + threadIdx.x;
+      const bool have = j < n_occ;
+      const uint32_t c = have ? uint32_t(s_cell[j]) : 0u;
+      const int lx = int(morton_compact(c)), ly = int(morton_compact(c >> 1)), lz = int(morton_compact(c >> 2));
+      const int cx = tx * 8 + lx, cy = ty * 8 + ly, cz = tz * 8 + lz;
+      uint32_t kbeg = 0, kend = 0;
+      if (have) {
+        kend = cell_end[tbase + c];
+        kbeg = kend - cell_count[tbase + c];
+      }
+      float ax[27], ay[27], az[27];
+#pragma unroll
+      for (int o = 0; o < 27; o++) { ax[o] = 0.f; ay[o] = 0.f; az[o] = 0.f; }
+      for (uint32_t kk = kbeg; kk < kend; kk++) {
+        const uint32_t q = sorted[kk];
+        const float4 pp = rec[size_t(q) * kRec + R_P];
+        float cf[kCoef * 4];
+        const float4* co = coef + size_t(q) * kCoef;
+#pragma unroll
+        for (int jj = 0; jj < kCoef; jj++) {
+          const float4 v4 = co[jj];
+          cf[4 * jj] = v4.x; cf[4 * jj + 1] = v4.y; cf[4 * jj + 2] = v4.z; cf[4 * jj + 3] = v4.w;
+        }
+        AxisD X, Y, Z;
+        axis_stencil_d(pp.x, p.wmin[0], p.h, cx - 1, X);
+        axis_stencil_d(pp.y, p.wmin[1], p.h, cy - 1, Y);
+        axis_stencil_d(pp.z, p.wmin[2], p.h, cz - 1, Z);
+#pragma unroll
+        for (int kz = 0; kz < 3; kz++)
+#pragma unroll
+          for (int jy = 0; jy < 3; jy++)
+#pragma unroll
+            for (int ix = 0; ix < 3; ix++) {
+              const int o = (kz * 3 + jy) * 3 + ix;
+              const float w = X.w[ix] * Y.w[jy] * Z.w[kz];
+              if (w < 1e-12f) continue;  // G2P's weight cut
+              const float dw[3] = {X.g[ix] * Y.w[jy] * Z.w[kz], X.w[ix] * Y.g[jy] * Z.w[kz], X.w[ix] * Y.w[jy] * Z.g[kz]};
+              const float d[3] = {X.dx[ix], Y.dx[jy], Z.dx[kz]};
+              float out[3];
+#pragma unroll
+              for (int rr = 0; rr < 3; rr++) {
+                float s = w * cf[rr];
+#pragma unroll
+                for (int cc = 0; cc < 3; cc++) s += w * cf[3 + 3 * cc + rr] * d[cc] + cf[12 + 3 * cc + rr] * dw[cc];
+                out[rr] = s;
+              }
+              ax[o] += out[0];
+              ay[o] += out[1];
+              az[o] += out[2];
+            }
+      }
+#pragma unroll
+      for (int kz = 0; kz < 3; kz++)
+#pragma unroll
+        for (int jy = 0; jy < 3; jy++)
+#pragma unroll
+          for (int ix = 0; ix < 3; ix++) {
+            const int o = (kz * 3 + jy) * 3 + ix;
+            const int e = ((lz + kz) * kBox + (ly + jy)) * kBox + (lx + ix);
+            __syncthreads();
+            if (have) { s_x[e] += ax[o]; s_y[e] += ay[o]; s_z[e] += az[o]; }
+          }
+      __syncthreads();
+    }
+    float4* out = partial_adj + size_t(a) * kBoxNodes;
+    for (int e = threadIdx.x; e < kBoxNodes; e += kAdjThreads) out[e] = make_float4(s_x[e], s_y[e], s_z[e], 0.f);
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// grid update backward
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(512) void k_grid_bwd(SubParams p, const float4* __restrict__ partial,
+                                                  const float4* __restrict__ partial_adj,
+                                                  const int32_t* __restrict__ tile_act,
+                                                  const uint32_t* __restrict__ tiles, const uint32_t* __restrict__ count,
+                                                  float4* __restrict__ grid_adj, const float* __restrict__ sdf,
+                                                  const nb_force* __restrict__ forces) {
+  const uint32_t nb_tiles = *count;
+  const int lx = int(morton_compact(threadIdx.x)), ly = int(morton_compact(threadIdx.x >> 1)),
+            lz = int(morton_compact(threadIdx.x >> 2));
+  const int td = int(p.tile_dim);
+  for (uint32_t b = blockIdx.x; b < nb_tiles; b += gridDim.x) {
+    const uint32_t N = tiles[b];
+    int tx, ty, tz;
+    tile_coords(N, tx, ty, tz);
+    const int nx = tx * 8 + lx, ny = ty * 8 + ly, nz = tz * 8 + lz;
+    const size_t nid = (size_t(N) << kTileBits) | threadIdx.x;
+    float mx = 0.f, my = 0.f, mz = 0.f, mass = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
+    bool covered = false;
+    for (int dz = -1; dz <= 1; dz++) {
+      if ((dz < 0 && lz != 0) || (dz > 0 && lz != 7)) continue;
+      const int sz = tz + dz;
+      if (sz < 0 || sz >= td) continue;
+      for (int dy = -1; dy <= 1; dy++) {
+        if ((dy < 0 && ly != 0) || (dy > 0 && ly != 7)) continue;
+        const int sy = ty + dy;
+        if (sy < 0 || sy >= td) continue;
+        for (int dx = -1; dx <= 1; dx++) {
+          if ((dx < 0 && lx != 0) || (dx > 0 && lx != 7)) continue;
+          const int sx = tx + dx;
+          if (sx < 0 || sx >= td) continue;
+          const int32_t a = tile_act[morton3(sx, sy, sz)];
+          if (a < 0) continue;
+          const int bx = lx - 8 * dx + 1, by = ly - 8 * dy + 1, bz = lz - 8 * dz + 1;
+          const size_t e = size_t(a) * kBoxNodes + (bz * kBox + by) * kBox + bx;
+          const float4 q = partial[e];
+          mx += q.x; my += q.y; mz += q.z; mass += q.w;
+          const float4 g = partial_adj[e];
+          gx += g.x; gy += g.y; gz += g.z;
+          covered = true;
+        }
+      }
+    }
+    if (!covered) continue;
+    if (mass < 1e-10f) {
+      grid_adj[nid] = make_float4(0.f, 0.f, 0.f, 0.f);
+      continue;
+    }
+    // forward recompute (shaders/mpm_grid_update.comp:17-101, mpm_nanovdb_bc.comp:23-54)
+    const float u0x = mx / mass, u0y = my / mass, u0z = mz / mass;
+    const float sp = len3(u0x, u0y, u0z);
+    const bool capped = sp > 15.f;
+    float vx = u0x, vy = u0y, vz = u0z;
+    if (capped) { const float kk = 15.f / sp; vx *= kk; vy *= kk; vz *= kk; }
+    for (uint32_t fi = 0; fi < p.force_count; fi++) {
+      const nb_force f = forces[fi];
+      const uint32_t mask = f.affect_mask;
+      if (mask != 0u && (mask & 1u) == 0u) continue;
+      if (f.type == NB_FORCE_GRAVITY || f.type == NB_FORCE_CONSTANT_WIND) {
+        const float s = f.data[3];
+        vx += (f.data[0] * s) * p.dt;
+        vy += (f.data[1] * s) * p.dt;
+        vz += (f.data[2] * s) * p.dt;
+      }
+    }
+    const int Nx = p.grid_res[0] - 1, Ny = p.grid_res[1] - 1, Nz = p.grid_res[2] - 1;
+    bool wx = false, wy = false, wz = false;
+    if (nx <= 1 && vx < 0.f) { vx = 0.f; wx = true; }
+    if (nx >= Nx - 1 && vx > 0.f) { vx = 0.f; wx = true; }
+    if (ny <= 1 && vy < 0.f) { vy = 0.f; wy = true; }
+    if (ny >= Ny - 1 && vy > 0.f) { vy = 0.f; wy = true; }
+    if (nz <= 1 && vz < 0.f) { vz = 0.f; wz = true; }
+    if (nz >= Nz - 1 && vz > 0.f) { vz = 0.f; wz = true; }
+    bool sdf_zero = false, sdf_proj = false;
+    float ux = 0.f, uy = 0.f, uz = 0.f;
+    if (sdf) {
+      const float sd = sample_sdf(sdf, p, nx, ny, nz);
+      if (sd < 0.f) {
+        sdf_zero = true;
+      } else if (sd < p.h * 1.5f) {
+        const float ggx = sample_sdf(sdf, p, nx + 1, ny, nz) - sample_sdf(sdf, p, nx - 1, ny, nz);
+        const float ggy = sample_sdf(sdf, p, nx, ny + 1, nz) - sample_sdf(sdf, p, nx, ny - 1, nz);
+        const float ggz = sample_sdf(sdf, p, nx, ny, nz + 1) - sample_sdf(sdf, p, nx, ny, nz - 1);
+        const float gl = len3(ggx, ggy, ggz);
+        if (gl > 1e-6f) {
+          ux = ggx / gl; uy = ggy / gl; uz = ggz / gl;
+          if (vx * ux + vy * uy + vz * uz < 0.f) sdf_proj = true;
+        }
+      }
+    }
+    // backward
+    if (sdf_zero) { gx = gy = gz = 0.f; }
+    if (sdf_proj) {
+      const float gn = gx * ux + gy * uy + gz * uz;
+      gx -= gn * ux; gy -= gn * uy; gz -= gn * uz;
+    }
+    if (wx) gx = 0.f;
+    if (wy) gy = 0.f;
+    if (wz) gz = 0.f;
+    if (capped) {
+      const float hx = u0x / sp, hy = u0y / sp, hz = u0z / sp;
+      const float gh = gx * hx + gy * hy + gz * hz;
+      const float kk = 15.f / sp;
+      gx = kk * (gx - gh * hx); gy = kk * (gy - gh * hy); gz = kk * (gz - gh * hz);
+    }
+    const float im = 1.f / mass;
+    grid_adj[nid] = make_float4(gx * im, gy * im, gz * im, -(gx * u0x + gy * u0y + gz * u0z) * im);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// P2G backward
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_p2g_bwd(SubParams p, const float4* __restrict__ rec,
+                                                 float4* __restrict__ adj_out, const nb_material* __restrict__ mats,
+                                                 const uint32_t* __restrict__ sorted,
+                                                 const float4* __restrict__ grid_adj) {
+  const uint32_t k = xcd_chunk(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (k >= p.n) return;
+  const uint32_t q = sorted[k];
+  const float4* r = rec + size_t(q) * kRec;
+  const float4 pp = r[R_P], vv = r[R_V];
+  uint32_t mid = __float_as_uint(vv.w);
+  mid = mid > p.material_count - 1u ? p.material_count - 1u : mid;
+  const float mp = mats[mid].rho0 * pp.w;
+  const bool apic = p.flip_ratio < -0.5f;
+  const float sc = apic ? mp * (4.f / (p.h * p.h)) : 0.f;
+  const m3 B = rec_B(r);
+  const m3 T = rec_tau(r);
+  const float st = p.dt * pp.w;
+  const float xs[3] = {pp.x, pp.y, pp.z}, v[3] = {vv.x, vv.y, vv.z};
+  int cc[3];
+  AxisD ax[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    // the bin's cell (shaders/mpm_common.glsl:190-195); P2G's stencil is the cell's 27 nodes
+    const float f = fminf(fmaxf((xs[a] - p.wmin[a]) / p.h, 0.f), float(p.grid_res[a]) - 1.f);
+    cc[a] = int(f);
+    axis_stencil_d(xs[a], p.wmin[a], p.h, cc[a] - 1, ax[a]);
+  }
+  const int cube = int(p.cube_res);
+  float gv[3] = {0.f, 0.f, 0.f}, gx[3] = {0.f, 0.f, 0.f};
+  m3 gC = m3_zero(), gT = m3_zero();
+#pragma unroll
+  for (int iz = 0; iz < 3; iz++)
+#pragma unroll
+    for (int iy = 0; iy < 3; iy++)
+#pragma unroll
+      for (int ix = 0; ix < 3; ix++) {
+        const int nx = cc[0] - 1 + ix, ny = cc[1] - 1 + iy, nz = cc[2] - 1 + iz;
+        if (nx < 0 || ny < 0 || nz < 0 || nx >= cube || ny >= cube || nz >= cube) continue;
+        NodeW n;
+        node_w(ax[0], ax[1], ax[2], ix, iy, iz, n);
+        if (n.w < 1e-12f) continue;  // shaders/mpm_p2g.comp:61
+        const float4 g4 = grid_adj[morton3(nx, ny, nz)];
+        const float gp[3] = {g4.x, g4.y, g4.z};
+        const float gm = g4.w;
+        float mom[3];  // m v + C d
+#pragma unroll
+        for (int rr = 0; rr < 3; rr++)
+          mom[rr] = mp * v[rr] + sc * (B.m[0][rr] * n.d[0] + B.m[1][rr] * n.d[1] + B.m[2][rr] * n.d[2]);
+        const float gpm = gp[0] * mom[0] + gp[1] * mom[1] + gp[2] * mom[2];
+        float Ctg[3], TgH[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+        for (int c = 0; c < 3; c++) Ctg[c] = sc * (B.m[c][0] * gp[0] + B.m[c][1] * gp[1] + B.m[c][2] * gp[2]);
+        // (tau^T gp)_c
+        float Ttg[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) Ttg[c] = T.m[c][0] * gp[0] + T.m[c][1] * gp[1] + T.m[c][2] * gp[2];
+#pragma unroll
+        for (int b = 0; b < 3; b++) TgH[b] = Ttg[0] * n.H[0][b] + Ttg[1] * n.H[1][b] + Ttg[2] * n.H[2][b];
+#pragma unroll
+        for (int rr = 0; rr < 3; rr++) {
+          gv[rr] += mp * n.w * gp[rr];
+#pragma unroll
+          for (int c = 0; c < 3; c++) {
+            gC.m[c][rr] += n.w * gp[rr] * n.d[c];
+            gT.m[c][rr] += -st * gp[rr] * n.dw[c];
+          }
+        }
+#pragma unroll
+        for (int b = 0; b < 3; b++) gx[b] += (mp * gm + gpm) * n.dw[b] - n.w * Ctg[b] - st * TgH[b];
+      }
+  float4* go = adj_out + size_t(q) * kRec;
+  const float4 g0 = go[R_P];
+  const float4 gt = go[R_T];
+  go[R_P] = make_float4(g0.x + gx[0], g0.y + gx[1], g0.z + gx[2], 0.f);
+  go[R_V] = make_float4(gv[0], gv[1], gv[2], 0.f);
+  // packed tau: diagonal, and off-diagonal values that fill two slots each
+  go[R_B0] = make_float4(sc * gC.m[0][0], sc * gC.m[0][1], sc * gC.m[0][2], gT.m[1][0] + gT.m[0][1]);
+  go[R_B1] = make_float4(sc * gC.m[1][0], sc * gC.m[1][1], sc * gC.m[1][2], gT.m[2][0] + gT.m[0][2]);
+  go[R_B2] = make_float4(sc * gC.m[2][0], sc * gC.m[2][1], sc * gC.m[2][2], gT.m[2][1] + gT.m[1][2]);
+  go[R_T] = make_float4(gT.m[0][0], gT.m[1][1], gT.m[2][2], gt.w);
+}
+
+// fixed-order sum of the per-wave parameter partials, added to the running total
+__global__ __launch_bounds__(256) void k_pgrad_reduce(const float* __restrict__ pgrad_w, uint32_t n_waves,
+                                                      float* __restrict__ pgrad) {
+  __shared__ float s[256];
+  const uint32_t j = blockIdx.x;  // slot
+  float acc = 0.f;
+  for (uint32_t w = threadIdx.x; w < n_waves; w += 256) acc += pgrad_w[size_t(w) * (NB_MAX_MATERIALS * 4) + j];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (int(threadIdx.x) < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) pgrad[j] += s[0];
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static inline uint32_t cdiv_a(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+static inline uint32_t tile_grid_a(uint32_t n) { return n < 2048u ? (n ? n : 1u) : 2048u; }
+
+void launch_adjoint_substep(const DevState& d, const AdjState& a, const SubParams& p, uint32_t n_tiles,
+                            const float4* adj_in, float4* adj_out, hipStream_t s) {
+  const uint32_t nblk = cdiv_a(p.n, 256);
+  k_g2p_bwd<<<nblk, 256, 0, s>>>(p, d.rec, adj_in, adj_out, a.coef, a.pgrad_w, d.materials, d.sorted, d.grid_node,
+                                  d.sdf);
+  k_adj_scatter<<<tile_grid_a(n_tiles), kAdjThreads, 0, s>>>(p, d.rec, a.coef, d.cell_count, d.cell_end, d.sorted,
+                                                             d.act_tiles, d.counters, a.partial_adj);
+  k_grid_bwd<<<tile_grid_a(n_tiles), 512, 0, s>>>(p, d.partial, a.partial_adj, d.tile_act, d.node_tiles,
+                                                   d.counters + CNT_NODE_TILES, a.grid_adj, d.sdf, d.forces);
+  k_p2g_bwd<<<nblk, 256, 0, s>>>(p, d.rec, adj_out, d.materials, d.sorted, a.grid_adj);
+  k_pgrad_reduce<<<NB_MAX_MATERIALS * 4, 256, 0, s>>>(a.pgrad_w, nblk * 4u, a.pgrad);
+}
+
+}  // namespace nb

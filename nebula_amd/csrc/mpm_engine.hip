@@ -57,6 +57,7 @@ struct nb_mpm {
   bool own_stream = false;
   DevState d{};
   uint32_t material_count = 1, collider_count = 0, force_count = 0;
+  std::vector<nb_material> mats_host;  // host copy of the material table
   bool has_sdf = false;
   bool bins_valid = false;
   // The kernels work on d.rec (one record per particle); the ABI exposes the
@@ -70,6 +71,22 @@ struct nb_mpm {
   GridMode grid_mode = GridMode::NONE;
   bool old_valid = false;
   std::vector<void*> allocs;
+
+  // reverse mode: checkpoints of a recorded run of n substeps every `seg`,
+  // the substep states of one segment, the adjoint in both layouts
+  struct Adjoint {
+    bool on = false, recorded = false;
+    uint32_t n = 0, seg = 0, n_ckpt = 0;
+    float dt = 0.f;
+    std::vector<void*> allocs;
+    float4* ckpt = nullptr;   // n_ckpt * cap * kRec
+    float4* tape = nullptr;   // seg * cap * kRec
+    float4* final_rec = nullptr;
+    float4* adj_a = nullptr;  // cap * kRec
+    float4* adj_b = nullptr;
+    float4* field[NB_NUM_FIELDS] = {};  // adjoint in the reference layout
+    AdjState a{};
+  } adj;
 
   template <class T>
   int alloc(T** p, size_t count) {
@@ -137,13 +154,13 @@ nb_material preset_jelly(float E, float nu, float rho0) {  // src/core/MaterialP
 
 int upload(nb_mpm* h, void* dst, const void* src, size_t bytes) {
   if (!bytes) return NB_OK;
-  NB_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, h->stream));
+  NB_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, h->stream));
   NB_HIP(hipStreamSynchronize(h->stream));
   return NB_OK;
 }
 int download(nb_mpm* h, void* dst, const void* src, size_t bytes) {
   if (!bytes) return NB_OK;
-  NB_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, h->stream));
+  NB_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, h->stream));
   NB_HIP(hipStreamSynchronize(h->stream));
   return NB_OK;
 }
@@ -332,6 +349,7 @@ int nb_mpm_create(const nb_mpm_config* cfg, void* hip_stream, nb_mpm** out) {
     m.q_max = h->knobs.q_max;
     if ((rc = upload(h, d.materials, &m, sizeof(m))) != NB_OK) return cleanup_fail(rc);
     h->material_count = 1;
+    h->mats_host.assign(1, m);
   }
 
   // default particle block (src/engine/MPMEngine.cpp:86-137)
@@ -366,6 +384,7 @@ int nb_mpm_destroy(nb_mpm* h) {
   if (!h) return NB_OK;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   for (void* p : h->allocs) (void)hipFree(p);
+  for (void* p : h->adj.allocs) (void)hipFree(p);
   if (h->d.sdf) (void)hipFree(h->d.sdf);
   if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
@@ -432,7 +451,10 @@ int nb_mpm_set_materials(nb_mpm* h, const nb_material* mats, uint32_t n) {
   if (!mats) return fail(NB_ERR_ARG, "null materials");
   if (n > NB_MAX_MATERIALS) return fail(NB_ERR_RANGE, "%u materials exceed the %u slots", n, NB_MAX_MATERIALS);
   int rc = upload(h, h->d.materials, mats, sizeof(nb_material) * n);
-  if (rc == NB_OK) h->material_count = n;
+  if (rc == NB_OK) {
+    h->material_count = n;
+    h->mats_host.assign(mats, mats + n);
+  }
   return rc;
 }
 
@@ -813,6 +835,182 @@ int nb_mpm_active_tile_count(nb_mpm* h, uint32_t* tiles) {
   int rc = download(h, cnt, h->d.counters, sizeof(cnt));
   if (rc != NB_OK) return rc;
   *tiles = cnt[CNT_ACTIVE];
+  return NB_OK;
+}
+
+// ---- reverse mode ----
+namespace {
+FieldPtrs adj_fields(nb_mpm* h) {
+  FieldPtrs f;
+  for (int i = 0; i < NB_NUM_FIELDS; i++) f.f[i] = h->adj.field[i];
+  return f;
+}
+int adj_alloc(nb_mpm* h, float4** p, size_t count) {
+  void* q = nullptr;
+  const size_t bytes = std::max<size_t>(count, 1) * sizeof(float4);
+  hipError_t e = hipMalloc(&q, bytes);
+  if (e != hipSuccess) return fail(NB_ERR_HIP, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  h->adj.allocs.push_back(q);
+  *p = reinterpret_cast<float4*>(q);
+  return NB_OK;
+}
+void adj_free(nb_mpm* h) {
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  for (void* p : h->adj.allocs) (void)hipFree(p);
+  h->adj = nb_mpm::Adjoint{};
+}
+int copy_rec(nb_mpm* h, float4* dst, const float4* src) {
+  if (!h->n_live) return NB_OK;
+  NB_HIP(hipMemcpyAsync(dst, src, size_t(h->n_live) * kRec * sizeof(float4), hipMemcpyDeviceToDevice, h->stream));
+  return NB_OK;
+}
+}  // namespace
+
+int nb_mpm_adjoint_begin(nb_mpm* h, uint32_t n_substeps, uint32_t segment) {
+  if (!h) return fail(NB_ERR_ARG, "null handle");
+  if (n_substeps == 0 || segment == 0) return fail(NB_ERR_ARG, "n_substeps and segment must be > 0");
+  if (!h->knobs.corrected_svd)
+    return fail(NB_ERR_STATE, "the adjoint is defined for corrected_svd = 1 (proper polar SVD)");
+  if (h->knobs.flip_ratio > 0.01f) return fail(NB_ERR_UNSUPPORTED, "the adjoint supports PIC and APIC, not FLIP");
+  if (h->collider_count) return fail(NB_ERR_UNSUPPORTED, "the adjoint does not support analytic colliders");
+  for (uint32_t i = 0; i < h->material_count; i++) {
+    const uint32_t m = h->mats_host[i].model;
+    if (m == NB_MAT_FLUID || m == NB_MAT_VISCOPLASTIC_MUD)
+      return fail(NB_ERR_UNSUPPORTED, "material %u: the adjoint supports elastic, Von Mises and Drucker-Prager", i);
+  }
+  adj_free(h);
+  auto& A = h->adj;
+  A.n = n_substeps;
+  A.seg = std::min(segment, n_substeps);
+  A.n_ckpt = (n_substeps + A.seg - 1) / A.seg;
+  const size_t rec = size_t(h->cap) * kRec;
+  const uint32_t n_waves = ((h->cap + 255u) / 256u) * 4u;
+  int rc = NB_OK;
+  if (rc == NB_OK) rc = adj_alloc(h, &A.ckpt, rec * A.n_ckpt);
+  if (rc == NB_OK) rc = adj_alloc(h, &A.tape, rec * A.seg);
+  if (rc == NB_OK) rc = adj_alloc(h, &A.final_rec, rec);
+  if (rc == NB_OK) rc = adj_alloc(h, &A.adj_a, rec);
+  if (rc == NB_OK) rc = adj_alloc(h, &A.adj_b, rec);
+  for (int f = 0; f < NB_NUM_FIELDS && rc == NB_OK; f++) rc = adj_alloc(h, &A.field[f], h->cap);
+  if (rc == NB_OK) rc = adj_alloc(h, &A.a.coef, size_t(h->cap) * 6);
+  if (rc == NB_OK) rc = adj_alloc(h, &A.a.partial_adj, size_t(h->cap_tiles) * kBoxNodes);
+  if (rc == NB_OK) rc = adj_alloc(h, &A.a.grid_adj, size_t(h->NT) * kTileCells);
+  float4* pw = nullptr;
+  float4* pg = nullptr;
+  if (rc == NB_OK) rc = adj_alloc(h, &pw, size_t(n_waves) * NB_MAX_MATERIALS);
+  if (rc == NB_OK) rc = adj_alloc(h, &pg, NB_MAX_MATERIALS);
+  if (rc != NB_OK) {
+    std::string keep = g_err;
+    adj_free(h);
+    g_err = keep;
+    return rc;
+  }
+  A.a.pgrad_w = reinterpret_cast<float*>(pw);
+  A.a.pgrad = reinterpret_cast<float*>(pg);
+  for (int f = 0; f < NB_NUM_FIELDS; f++) NB_HIP(hipMemsetAsync(A.field[f], 0, size_t(h->cap) * 16, h->stream));
+  NB_HIP(hipStreamSynchronize(h->stream));
+  A.on = true;
+  return NB_OK;
+}
+
+int nb_mpm_adjoint_end(nb_mpm* h) {
+  if (!h) return fail(NB_ERR_ARG, "null handle");
+  adj_free(h);
+  return NB_OK;
+}
+
+int nb_mpm_forward_recorded(nb_mpm* h, float dt_sub) {
+  if (!h) return fail(NB_ERR_ARG, "null handle");
+  auto& A = h->adj;
+  if (!A.on) return fail(NB_ERR_STATE, "call nb_mpm_adjoint_begin first");
+  int rc = sync_in(h);
+  for (uint32_t s = 0; s < A.n && rc == NB_OK; s++) {
+    if (s % A.seg == 0) rc = copy_rec(h, A.ckpt + size_t(s / A.seg) * h->cap * kRec, h->d.rec);
+    if (rc == NB_OK) rc = enqueue_substep(h, dt_sub, nullptr);
+  }
+  if (rc == NB_OK) rc = copy_rec(h, A.final_rec, h->d.rec);
+  if (rc == NB_OK) rc = sync_out(h);
+  if (rc != NB_OK) return rc;
+  NB_HIP(hipStreamSynchronize(h->stream));
+  A.dt = dt_sub;
+  A.recorded = true;
+  return NB_OK;
+}
+
+int nb_mpm_set_adjoint(nb_mpm* h, int field, const float* in4, uint32_t first, uint32_t count) {
+  if (!h || !in4) return fail(NB_ERR_ARG, "null argument");
+  if (!h->adj.on) return fail(NB_ERR_STATE, "call nb_mpm_adjoint_begin first");
+  if (field < 0 || field >= NB_NUM_FIELDS) return fail(NB_ERR_ARG, "bad field %d", field);
+  if (uint64_t(first) + count > h->cap) return fail(NB_ERR_RANGE, "rows beyond capacity");
+  return upload(h, h->adj.field[field] + first, in4, size_t(count) * 16);
+}
+
+int nb_mpm_get_adjoint(nb_mpm* h, int field, float* out4, uint32_t first, uint32_t count) {
+  if (!h || !out4) return fail(NB_ERR_ARG, "null argument");
+  if (!h->adj.on) return fail(NB_ERR_STATE, "call nb_mpm_adjoint_begin first");
+  if (field < 0 || field >= NB_NUM_FIELDS) return fail(NB_ERR_ARG, "bad field %d", field);
+  if (uint64_t(first) + count > h->cap) return fail(NB_ERR_RANGE, "rows beyond capacity");
+  return download(h, out4, h->adj.field[field] + first, size_t(count) * 16);
+}
+
+int nb_mpm_adjoint_device_ptr(nb_mpm* h, int field, void** dptr) {
+  if (!h || !dptr) return fail(NB_ERR_ARG, "null argument");
+  if (!h->adj.on) return fail(NB_ERR_STATE, "call nb_mpm_adjoint_begin first");
+  if (field < 0 || field >= NB_NUM_FIELDS) return fail(NB_ERR_ARG, "bad field %d", field);
+  *dptr = h->adj.field[field];
+  return NB_OK;
+}
+
+int nb_mpm_backward(nb_mpm* h, float* d_params, uint32_t params_cap) {
+  if (!h) return fail(NB_ERR_ARG, "null handle");
+  auto& A = h->adj;
+  if (!A.on || !A.recorded) return fail(NB_ERR_STATE, "call nb_mpm_forward_recorded first");
+  const uint32_t cap = h->cap;
+  const size_t rec = size_t(cap) * kRec;
+  NB_HIP(hipMemsetAsync(A.a.pgrad, 0, NB_MAX_MATERIALS * 4 * sizeof(float), h->stream));
+  launch_soa_to_aos_ptrs(adj_fields(h), A.adj_a, h->n_live, h->stream);
+  float4* in = A.adj_a;
+  float4* out = A.adj_b;
+  int rc = NB_OK;
+  for (int sg = int(A.n_ckpt) - 1; sg >= 0 && rc == NB_OK; sg--) {
+    const uint32_t s0 = uint32_t(sg) * A.seg;
+    const uint32_t len = std::min(A.seg, A.n - s0);
+    // recompute the segment, keeping the state before each substep
+    rc = copy_rec(h, h->d.rec, A.ckpt + size_t(sg) * rec);
+    h->keys_valid = false;
+    for (uint32_t j = 0; j < len && rc == NB_OK; j++) {
+      rc = copy_rec(h, A.tape + size_t(j) * rec, h->d.rec);
+      if (rc == NB_OK) rc = enqueue_substep(h, A.dt, nullptr);
+    }
+    for (int j = int(len) - 1; j >= 0 && rc == NB_OK; j--) {
+      rc = copy_rec(h, h->d.rec, A.tape + size_t(j) * rec);
+      if (rc != NB_OK) break;
+      SubParams p = h->params(A.dt, false);
+      const DevState dv = h->dev();
+      launch_bin(dv, p, h->NT, false, h->stream, nullptr);
+      launch_p2g(dv, p, h->NT, h->stream, nullptr);
+      launch_grid(dv, p, h->NT, true, true, h->d.node_tiles, h->d.counters + CNT_NODE_TILES, h->stream, nullptr);
+      launch_adjoint_substep(dv, A.a, p, h->NT, in, out, h->stream);
+      NB_HIP(hipGetLastError());
+      std::swap(in, out);
+    }
+  }
+  if (rc != NB_OK) return rc;
+  launch_aos_to_soa_ptrs(in, adj_fields(h), h->n_live, h->stream);
+  // back to the state the recorded run ended in
+  rc = copy_rec(h, h->d.rec, A.final_rec);
+  if (rc != NB_OK) return rc;
+  h->keys_valid = false;
+  h->bins_valid = false;
+  h->grid_mode = GridMode::NONE;
+  if ((rc = sync_out(h)) != NB_OK) return rc;
+  if (d_params) {
+    float pg[NB_MAX_MATERIALS * 4];
+    if ((rc = download(h, pg, A.a.pgrad, sizeof(pg))) != NB_OK) return rc;
+    const uint32_t n = std::min(params_cap, h->material_count * 4u);
+    std::memcpy(d_params, pg, n * sizeof(float));
+  }
+  NB_HIP(hipStreamSynchronize(h->stream));
   return NB_OK;
 }
 

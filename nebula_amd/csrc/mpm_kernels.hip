@@ -1209,6 +1209,13 @@ void launch_aos_to_soa(const DevState& d, uint32_t n, hipStream_t s) {
   k_aos_to_soa<<<cdiv(n, 256), 256, 0, s>>>(d.rec, f, n);
 }
 
+void launch_soa_to_aos_ptrs(const FieldPtrs& f, float4* rec, uint32_t n, hipStream_t s) {
+  if (n) k_soa_to_aos<<<cdiv(n, 256), 256, 0, s>>>(f, rec, n);
+}
+void launch_aos_to_soa_ptrs(const float4* rec, const FieldPtrs& f, uint32_t n, hipStream_t s) {
+  if (n) k_aos_to_soa<<<cdiv(n, 256), 256, 0, s>>>(rec, f, n);
+}
+
 void launch_copy(float4* dst, const float4* src, uint64_t n4, hipStream_t s) {
   k_copy<<<8192, 256, 0, s>>>(dst, src, n4);
 }

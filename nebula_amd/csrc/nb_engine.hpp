@@ -130,5 +130,21 @@ void launch_init_particles(const DevState& d, uint32_t first, uint32_t count, hi
 void launch_copy(float4* dst, const float4* src, uint64_t n4, hipStream_t s);
 void launch_soa_to_aos(const DevState& d, uint32_t n, hipStream_t s);
 void launch_aos_to_soa(const DevState& d, uint32_t n, hipStream_t s);
+void launch_soa_to_aos_ptrs(const FieldPtrs& f, float4* rec, uint32_t n, hipStream_t s);
+void launch_aos_to_soa_ptrs(const float4* rec, const FieldPtrs& f, uint32_t n, hipStream_t s);
+
+// Buffers of the reverse-mode substep (mpm_adjoint.hip).
+struct AdjState {
+  float4* coef;         // capacity * 6: per-particle grid-adjoint coefficients
+  float4* partial_adj;  // cap_tiles * kBoxNodes: per-tile grid-adjoint boxes
+  float4* grid_adj;     // NT * 512: (dL/dp, dL/dm) per node
+  float* pgrad_w;       // per wave of the G2P backward: NB_MAX_MATERIALS x 4 partials
+  float* pgrad;         // NB_MAX_MATERIALS x 4: running d/d(mu, lambda, M, q_c)
+};
+// One substep backwards on the recomputed forward of state d.rec (bins, P2G
+// partials and grid velocities already in d): adj_in is the adjoint of the
+// state after the substep, adj_out receives the adjoint of d.rec.
+void launch_adjoint_substep(const DevState& d, const AdjState& a, const SubParams& p, uint32_t n_tiles,
+                            const float4* adj_in, float4* adj_out, hipStream_t s);
 
 }  // namespace nb

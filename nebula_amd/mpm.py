@@ -373,6 +373,8 @@ EXPORTED_SYMBOLS = (
     "nb_mpm_set_live_particle_count", "nb_mpm_run_bin", "nb_mpm_get_bins", "nb_mpm_run_p2g",
     "nb_mpm_run_grid_update", "nb_mpm_run_g2p", "nb_mpm_get_grid", "nb_mpm_set_grid",
     "nb_mpm_profile_substeps", "nb_mpm_calibration_copy", "nb_mpm_active_node_count", "nb_mpm_active_tile_count",
+    "nb_mpm_adjoint_begin", "nb_mpm_forward_recorded", "nb_mpm_set_adjoint", "nb_mpm_get_adjoint",
+    "nb_mpm_adjoint_device_ptr", "nb_mpm_backward", "nb_mpm_adjoint_end",
 )
 
 
@@ -424,6 +426,13 @@ def lib():
         L.nb_mpm_active_node_count.argtypes = [vp, C.POINTER(C.c_uint64)]
         L.nb_mpm_calibration_copy.argtypes = [vp, C.c_uint64, C.c_uint32]
         L.nb_mpm_active_tile_count.argtypes = [vp, up]
+        L.nb_mpm_adjoint_begin.argtypes = [vp, C.c_uint32, C.c_uint32]
+        L.nb_mpm_forward_recorded.argtypes = [vp, C.c_float]
+        L.nb_mpm_set_adjoint.argtypes = [vp, C.c_int, vp, C.c_uint32, C.c_uint32]
+        L.nb_mpm_get_adjoint.argtypes = [vp, C.c_int, vp, C.c_uint32, C.c_uint32]
+        L.nb_mpm_adjoint_device_ptr.argtypes = [vp, C.c_int, C.POINTER(vp)]
+        L.nb_mpm_backward.argtypes = [vp, fp, C.c_uint32]
+        L.nb_mpm_adjoint_end.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -700,3 +709,45 @@ class MPMEngine:
         t = C.c_uint32()
         _check(lib().nb_mpm_active_tile_count(self._h, C.byref(t)))
         return t.value
+
+    # ---- reverse mode (no counterpart in the reference) ----
+    def adjointBegin(self, n_substeps: int, segment: int):
+        """Reserve checkpoints for a recorded run of n_substeps, one every
+        `segment` substeps (needs corrected_svd = 1; PIC/APIC; elastic, Von
+        Mises, Drucker-Prager)."""
+        _check(lib().nb_mpm_adjoint_begin(self._h, n_substeps, segment))
+
+    def forwardRecorded(self, dt_sub: float):
+        _check(lib().nb_mpm_forward_recorded(self._h, dt_sub))
+
+    def setAdjoint(self, grads):
+        """dL/d(final state): (8, n, 4) float32 in the state's layout, a host
+        array, or an int device pointer base per field via setAdjointField."""
+        g = np.ascontiguousarray(grads, np.float32)
+        for i in range(8):
+            _check(lib().nb_mpm_set_adjoint(self._h, i, g[i].ctypes.data, 0, g.shape[1]))
+
+    def setAdjointField(self, field: int, ptr: int, count: int, first: int = 0):
+        """Copy `count` float4 rows from a host or device address."""
+        _check(lib().nb_mpm_set_adjoint(self._h, field, ptr, first, count))
+
+    def getAdjointField(self, field: int, ptr: int, count: int, first: int = 0):
+        _check(lib().nb_mpm_get_adjoint(self._h, field, ptr, first, count))
+
+    def getAdjoint(self):
+        n = self.liveParticleCount()
+        out = np.empty((8, n, 4), np.float32)
+        for i in range(8):
+            _check(lib().nb_mpm_get_adjoint(self._h, i, out[i].ctypes.data, 0, n))
+        return out
+
+    def backward(self):
+        """Adjoint of the recorded run: returns dL/d(mu, lambda, M_friction,
+        q_cohesion) per material, (material_count, 4)."""
+        cap = 4 * 16
+        buf = np.zeros(cap, np.float32)
+        _check(lib().nb_mpm_backward(self._h, _f(buf), cap))
+        return buf.reshape(16, 4)
+
+    def adjointEnd(self):
+        _check(lib().nb_mpm_adjoint_end(self._h))

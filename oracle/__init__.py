@@ -73,6 +73,7 @@ def lib():
         L.orc_sim_set_state.argtypes = [vp, fp, C.c_uint32]
         L.orc_sim_get_state.argtypes = [vp, fp]
         L.orc_sim_get_state_f64.argtypes = [vp, dp]
+        L.orc_sim_set_state_f64.argtypes = [vp, dp, C.POINTER(C.c_uint32), C.c_uint32]
         for nm in ("orc_sim_bin", "orc_sim_p2g", "orc_sim_grid_update", "orc_sim_sdf_bc", "orc_sim_g2p"):
             getattr(L, nm).argtypes = [vp]
         L.orc_sim_substeps.argtypes = [vp, C.c_uint32]
@@ -186,6 +187,14 @@ class Sim:
         assert f.ndim == 3 and f.shape[0] == 8 and f.shape[2] == 4
         self.n = f.shape[1]
         lib().orc_sim_set_state(self.h, _fptr(f), self.n)
+
+    def set_state_f64(self, fields, matid):
+        """FP64 state (precision 64): fields (8, n, 4) float64, matid (n,) uint32."""
+        assert self.precision == 64
+        f = np.ascontiguousarray(fields, dtype=np.float64)
+        m = np.ascontiguousarray(matid, dtype=np.uint32)
+        self.n = f.shape[1]
+        lib().orc_sim_set_state_f64(self.h, _dptr(f), m.ctypes.data_as(C.POINTER(C.c_uint32)), self.n)
 
     def get_state(self):
         out = np.empty((8, self.n, 4), np.float32)

@@ -116,6 +116,16 @@ void orc_sim_set_state(orc_sim* s, const float* fields, uint32_t n) {
   }
 }
 
+// FP64 state (precision 64 only), e.g. for finite differences; material ids
+// come separately since they are bit patterns in the float layout
+void orc_sim_set_state_f64(orc_sim* s, const double* fields, const uint32_t* matid, uint32_t n) {
+  uint32_t NC = s->sc.cube_res * s->sc.cube_res * s->sc.cube_res;
+  s->s64.resize(n, NC);
+  for (int f = 0; f < 8; f++)
+    for (size_t i = 0; i < size_t(n) * 4; i++) s->s64.f[f][i] = fields[size_t(f) * n * 4 + i];
+  for (uint32_t i = 0; i < n; i++) s->s64.matid[i] = matid[i];
+}
+
 void orc_sim_get_state(orc_sim* s, float* fields) {
   uint32_t n = s->precision == 64 ? s->s64.n : s->s32.n;
   for (int f = 0; f < 8; f++)

@@ -1,0 +1,153 @@
+"""Reverse mode of the substep (nb_mpm_backward) against finite differences of
+the FP64 oracle.
+
+The reference has no adjoint; what fixes the gradient is the forward map
+itself.  For L = <W, state after n substeps> (W random), the GPU adjoint's
+directional derivative <dL/d(input), D> along random directions D is compared
+with the FP64 oracle's central difference (L(in + eps D) - L(in - eps D)) /
+(2 eps), for the particle positions, velocities, F, tau and B at the start
+and for each material's mu, lambda, M and q_c.  The scene (corrected SVD,
+elastic + Drucker-Prager with particles on both sides of the yield surface
+and at the apex, two checkpointed segments) keeps every particle away from the
+walls and the J-reset band, where the map is not differentiable.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from nbtest import oracle_sim
+from nebula_amd import mpm
+from nebula_amd.mpm import GravityForce, MaterialModel, MPMConfig, calcLambda, calcMu
+from nebula_amd.scenes import Scene, _fields, _material, _seed_cells, make_engine
+
+pytestmark = pytest.mark.gpu
+
+N_SUB, SEG = 4, 2
+
+
+def _scene(flip_ratio=-1.0, seed=11):
+    rng = np.random.default_rng(seed)
+    res, h = 16, 1.0 / 16
+    g = np.arange(6, 10)
+    cx, cy, cz = np.meshgrid(g, g, g, indexing="ij")
+    cells = np.stack([cx.ravel(), cy.ravel(), cz.ravel()], axis=1)
+    pos = _seed_cells(cells, h, 8, rng)
+    n = pos.shape[0]
+    vel = (0.1 * rng.standard_normal((n, 3))).astype(np.float32)
+    f = _fields(pos, vel, h ** 3 / 8)
+    # deformed start: F = I + noise, small affine field and stress
+    for c in range(3):
+        f[2 + c, :, :3] = np.eye(3, dtype=np.float32)[c] + 0.03 * rng.standard_normal((n, 3))
+    f[5:8, :, :3] = 0.02 * rng.standard_normal((3, n, 3))
+    f[2:8, :, 3] = 5.0 * rng.standard_normal((6, n))
+    mid = (pos[:, 0] >= 0.5).astype(np.uint32)  # elastic half, Drucker-Prager half
+    f[1, :, 3] = mid.view(np.float32)
+    cfg = MPMConfig(nx=0, ny=0, nz=0, domainSize=(1.0, 1.0, 1.0), cellSize=h, maxParticles=n,
+                    E=2e3, nu=0.3, rho0=1e3)
+    mats = [_material(2e3, 0.3, 1e3, MaterialModel.ELASTIC),
+            _material(2e3, 0.3, 1e3, MaterialModel.DRUCKER_PRAGER, M=math.tan(math.radians(30)), qc=2.0)]
+    return Scene("adjoint_block", cfg, mats, f, dt=2e-3, flip_ratio=flip_ratio, substeps=N_SUB,
+                 forces=[GravityForce((0.0, 1.0, 0.0), -9.8)], corrected_svd=True)
+
+
+# lanes of the eight float4 arrays that carry each input group
+GROUPS = {
+    "x": [(0, slice(0, 3))],
+    "v": [(1, slice(0, 3))],
+    "F": [(2, slice(0, 3)), (3, slice(0, 3)), (4, slice(0, 3))],
+    "tau": [(k, slice(3, 4)) for k in range(2, 8)],
+    "B": [(5, slice(0, 3)), (6, slice(0, 3)), (7, slice(0, 3))],
+}
+SCALE = {"x": 1e-4, "v": 1e-2, "F": 1e-3, "tau": 1.0, "B": 1e-3}  # x: kinks of the yield surface lie ~1e-7 away
+PARAMS = ("mu", "lambda_", "M_friction", "q_cohesion")
+
+
+def _mask(n):
+    m = np.zeros((8, n, 4), np.float64)
+    for grp in GROUPS.values():
+        for k, sl in grp:
+            m[k, :, sl] = 1.0
+    return m
+
+
+def _loss(sc, fields, mats, W):
+    sim = oracle_sim(Scene(sc.name, sc.cfg, mats, sc.fields, sc.dt, sc.flip_ratio, sc.substeps,
+                           forces=sc.forces, corrected_svd=True), precision=64)
+    sim.set_state_f64(fields, sc.fields[1, :, 3].view(np.uint32))
+    sim.substeps(N_SUB)
+    return float(np.sum(W * sim.get_state_f64()))
+
+
+def _fields64(sc):
+    return sc.fields.astype(np.float64)
+
+
+def _check_scene(sc):
+    n = sc.n
+    rng = np.random.default_rng(5)
+    W = rng.standard_normal((8, n, 4)) * _mask(n)
+    e = make_engine(sc)
+    e.adjointBegin(N_SUB, SEG)
+    e.forwardRecorded(sc.dt)
+    e.setAdjoint(W.astype(np.float32))
+    dpar = e.backward()[:len(sc.materials)]
+    grad = e.getAdjoint().astype(np.float64)
+    e.adjointEnd()
+
+    f0 = _fields64(sc)
+    f0[1, :, 3] = 0.0  # the material ids travel separately (bit patterns)
+    fails = []
+    for name, grp in GROUPS.items():
+        D = np.zeros_like(f0)
+        for k, sl in grp:
+            D[k, :, sl] = rng.standard_normal(D[k, :, sl].shape)
+        eps = 1e-4 * SCALE[name]
+        fp, fm = f0 + eps * D, f0 - eps * D
+        fd = (_loss(sc, fp, sc.materials, W) - _loss(sc, fm, sc.materials, W)) / (2 * eps)
+        ad = float(np.sum(grad * D))
+        err = abs(ad - fd) / max(abs(fd), 1e-8)
+        print(f"  d/d{name}: adjoint {ad:+.6e}  fd {fd:+.6e}  rel {err:.2e}", flush=True)
+        if err > 1e-3:
+            fails.append(name)
+    for mi in range(len(sc.materials)):
+        for j, pname in enumerate(PARAMS):
+            base = getattr(sc.materials[mi], pname)
+            if base == 0.0:
+                continue
+
+            def mats_with(value):
+                ms = [mpm.MaterialParams.from_buffer_copy(bytes(m)) for m in sc.materials]
+                setattr(ms[mi], pname, value)
+                return ms, getattr(ms[mi], pname)  # the float32 value actually used
+
+            mp_, vp_ = mats_with(base * (1 + 1e-3))
+            mm_, vm_ = mats_with(base * (1 - 1e-3))
+            fd = (_loss(sc, f0, mp_, W) - _loss(sc, f0, mm_, W)) / (vp_ - vm_)
+            ad = float(dpar[mi, j])
+            err = abs(ad - fd) / max(abs(fd), 1e-6 * max(1.0, abs(ad)))
+            print(f"  mat {mi} d/d{pname}: adjoint {ad:+.6e}  fd {fd:+.6e}  rel {err:.2e}", flush=True)
+            if err > 1e-3 and abs(ad - fd) > 1e-6:
+                fails.append(f"{mi}:{pname}")
+    assert not fails, f"gradient mismatch: {fails}"
+
+
+def test_adjoint_apic_matches_fp64_finite_differences(gpu_lib):
+    _check_scene(_scene(flip_ratio=-1.0))
+
+
+def test_adjoint_pic_matches_fp64_finite_differences(gpu_lib):
+    _check_scene(_scene(flip_ratio=0.0))
+
+
+def test_adjoint_rejects_unsupported_configurations(gpu_lib):
+    sc = _scene()
+    e = make_engine(sc)
+    e.corrected_svd = False
+    with pytest.raises(mpm.NebulaError):
+        e.adjointBegin(2, 1)
+    e.corrected_svd = True
+    e.flip_ratio = 0.5
+    with pytest.raises(mpm.NebulaError):
+        e.adjointBegin(2, 1)
