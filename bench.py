@@ -14,6 +14,12 @@ Also printed in the JSON line:
                 8 TB/s HBM peak (bytes per particle / per active node as in the
                 substep model B = 288 N + 68 G_act: bin 20 N + 8 G, P2G 92 N +
                 16 G, grid 28 G, G2P 176 N + 16 G)
+  adjoint       the reverse-mode leg of the metric on the same scene with the
+                proper polar SVD (the mode the adjoint is defined for):
+                forward-only time of K substeps against a recorded forward of
+                K substeps plus the backward (dL/d initial state and
+                dL/d material parameters), keeping all K states and with
+                sqrt(K)-substep checkpoint segments
   cpu_baseline  the FP32 CPU oracle (a restatement of the reference's shaders,
                 OpenMP over the host cores) timed on a bounded sample: a few
                 substeps of the same scene
@@ -43,6 +49,44 @@ def kernel_bytes(name, n, g_act):
     }.get(name)
 
 
+def adjoint_leg(sc, k):
+    """Forward+adjoint over forward time for k substeps of the scene (proper
+    polar SVD), with every state kept and with sqrt(k)-substep segments."""
+    import math
+
+    import numpy as np
+
+    from nebula_amd.scenes import make_engine
+
+    sc.corrected_svd = True
+    eng = make_engine(sc)
+    n = sc.n
+    eng.substep(sc.dt, 4)
+    eng.sync()
+    t0 = time.perf_counter()
+    eng.substep(sc.dt, k)
+    eng.sync()
+    t_fwd = time.perf_counter() - t0
+    seed = np.random.default_rng(0).standard_normal((8, n, 4), dtype=np.float32)
+    res = {"substeps": k, "fwd_ms_per_substep": round(t_fwd / k * 1e3, 4)}
+    for tag, seg in (("all_states", k), ("checkpointed", max(1, math.isqrt(k)))):
+        eng.adjointBegin(k, seg)
+        t0 = time.perf_counter()
+        eng.forwardRecorded(sc.dt)
+        t_rec = time.perf_counter() - t0
+        eng.setAdjoint(seed)
+        t0 = time.perf_counter()
+        eng.backward()
+        t_bwd = time.perf_counter() - t0
+        eng.adjointEnd()
+        res[tag] = {"segment": seg, "recorded_fwd_ms_per_substep": round(t_rec / k * 1e3, 4),
+                    "bwd_ms_per_substep": round(t_bwd / k * 1e3, 4), "ratio": round((t_rec + t_bwd) / t_fwd, 3)}
+        log(f"[bench] adjoint ({tag}, segment {seg}): fwd {t_fwd / k * 1e3:.3f} ms, recorded fwd "
+            f"{t_rec / k * 1e3:.3f} ms, bwd {t_bwd / k * 1e3:.3f} ms per substep; ratio {res[tag]['ratio']:.3f}")
+    res["ratio"] = res["all_states"]["ratio"]
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -50,6 +94,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--profile-steps", type=int, default=20, help="substeps timed per kernel with HIP events")
     ap.add_argument("--cpu-steps", type=int, default=5, help="substeps of the CPU oracle sample (0 = skip)")
+    ap.add_argument("--adjoint-substeps", type=int, default=16, help="substeps of the reverse-mode leg (0 = skip)")
     ap.add_argument("--calibration-copy-mb", type=int, default=0,
                     help="also run a plain copy kernel of this many MiB (HBM byte-counter calibration)")
     args = ap.parse_args()
@@ -94,6 +139,8 @@ def main():
     if args.calibration_copy_mb > 0:
         eng.calibrationCopy(args.calibration_copy_mb << 20, 3)
 
+    adjoint = adjoint_leg(sc, args.adjoint_substeps) if args.adjoint_substeps > 0 else None
+
     cpu = None
     if args.cpu_steps > 0:
         import oracle
@@ -126,6 +173,7 @@ def main():
         "substep_roofline": {"bytes_per_step": b_alg, "achieved": round(substep_rate, 2), "peak": HBM_PEAK_GBS,
                              "unit": "GB/s", "frac": round(substep_rate / HBM_PEAK_GBS, 4)},
         "kernels_avg_ms": {k: round(v, 5) for k, v in avg_ms.items()},
+        "adjoint": adjoint,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -285,11 +285,13 @@ int nb_mpm_active_tile_count(nb_mpm* h, uint32_t* tiles);
  * nb_mpm_set_adjoint (same eight float4 arrays and packing as the state:
  * dL/dx in P.xyz, dL/dv in v.xyz, dL/dF columns in F0..2.xyz, dL/dtau in the
  * w lanes, dL/dB columns in B0..2.xyz) -> nb_mpm_backward replaces it with
- * dL/d(state at the start of the run), recomputing each segment from its
- * checkpoint, and returns dL/d(mu, lambda, M_friction, q_cohesion) of every
- * material (material_count x 4 floats) in d_params.  The engine is left in
- * the state the recorded run ended in.  Memory: (ceil(n/segment) + segment
- * + 3) x 128 B x capacity. */
+ * dL/d(state at the start of the run), recomputing the inner states of every
+ * segment but the last from its checkpoint (segment = 1 or segment = n keeps
+ * all n states and recomputes nothing), and returns dL/d(mu, lambda,
+ * M_friction, q_cohesion) of every material (material_count x 4 floats) in
+ * d_params.  The engine is left in the state the recorded run ended in.
+ * Memory: (ceil(n/segment) + segment + 1) x 128 B + 224 B per particle of
+ * capacity, plus two grid-sized buffers. */
 int nb_mpm_adjoint_begin(nb_mpm* h, uint32_t n_substeps, uint32_t segment);
 int nb_mpm_forward_recorded(nb_mpm* h, float dt_sub);
 int nb_mpm_set_adjoint(nb_mpm* h, int field, const float* in4, uint32_t first, uint32_t count);

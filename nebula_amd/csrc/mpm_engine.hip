@@ -79,9 +79,13 @@ struct nb_mpm {
     uint32_t n = 0, seg = 0, n_ckpt = 0;
     float dt = 0.f;
     std::vector<void*> allocs;
+    // The state before substep s = sg * seg + j is slot(sg, j): ckpt[sg] for
+    // j = 0, tape[j - 1] otherwise.  The recorded run writes every slot; the
+    // tape then holds the last segment, and the backward recomputes the tape
+    // for each earlier segment from its checkpoint.
     float4* ckpt = nullptr;   // n_ckpt * cap * kRec
-    float4* tape = nullptr;   // seg * cap * kRec
-    float4* final_rec = nullptr;
+    float4* tape = nullptr;   // (seg - 1) * cap * kRec
+    int tape_seg = -1;        // the segment whose records the tape holds
     float4* adj_a = nullptr;  // cap * kRec
     float4* adj_b = nullptr;
     float4* field[NB_NUM_FIELDS] = {};  // adjoint in the reference layout
@@ -188,14 +192,18 @@ int sync_out(nb_mpm* h) {
 
 // one substep on the records (the caller brackets a run of them with
 // sync_in / sync_out)
-int enqueue_substep(nb_mpm* h, float dt, Prof* pf) {
+// One substep.  rec_in / rec_out (default: the engine's records, in place)
+// let the recorded run move the state along its tape without copies.
+int enqueue_substep(nb_mpm* h, float dt, Prof* pf, float4* rec_in = nullptr, float4* rec_out = nullptr) {
   if (h->n_live == 0) return NB_OK;
   const bool flip = h->knobs.flip_ratio > 0.01f;
   SubParams p = h->params(dt, flip);
-  launch_bin(h->dev(), p, h->NT, h->keys_valid, h->stream, pf);
-  launch_p2g(h->dev(), p, h->NT, h->stream, pf);
-  launch_grid(h->dev(), p, h->NT, true, true, h->d.node_tiles, h->d.counters + CNT_NODE_TILES, h->stream, pf);
-  launch_g2p(h->dev(), p, true, h->stream, pf);
+  DevState dv = h->dev();
+  if (rec_in) dv.rec = rec_in;
+  launch_bin(dv, p, h->NT, h->keys_valid, h->stream, pf);
+  launch_p2g(dv, p, h->NT, h->stream, pf);
+  launch_grid(dv, p, h->NT, true, true, h->d.node_tiles, h->d.counters + CNT_NODE_TILES, h->stream, pf);
+  launch_g2p(dv, p, true, h->stream, pf, rec_out);
   NB_HIP(hipGetLastError());
   h->bins_valid = true;
   h->keys_valid = true;
@@ -864,6 +872,15 @@ int copy_rec(nb_mpm* h, float4* dst, const float4* src) {
   NB_HIP(hipMemcpyAsync(dst, src, size_t(h->n_live) * kRec * sizeof(float4), hipMemcpyDeviceToDevice, h->stream));
   return NB_OK;
 }
+// the records before substep s (see nb_mpm::Adjoint)
+float4* adj_slot(nb_mpm* h, uint32_t s) {
+  const auto& A = h->adj;
+  const size_t rec = size_t(h->cap) * kRec;
+  const uint32_t sg = s / A.seg, j = s % A.seg;
+  return j == 0 ? A.ckpt + sg * rec : A.tape + (j - 1) * rec;
+}
+// where substep s writes its result: the next slot, or the engine's records after the last substep
+float4* adj_next(nb_mpm* h, uint32_t s) { return s + 1 == h->adj.n ? h->d.rec : adj_slot(h, s + 1); }
 }  // namespace
 
 int nb_mpm_adjoint_begin(nb_mpm* h, uint32_t n_substeps, uint32_t segment) {
@@ -887,8 +904,7 @@ int nb_mpm_adjoint_begin(nb_mpm* h, uint32_t n_substeps, uint32_t segment) {
   const uint32_t n_waves = ((h->cap + 255u) / 256u) * 4u;
   int rc = NB_OK;
   if (rc == NB_OK) rc = adj_alloc(h, &A.ckpt, rec * A.n_ckpt);
-  if (rc == NB_OK) rc = adj_alloc(h, &A.tape, rec * A.seg);
-  if (rc == NB_OK) rc = adj_alloc(h, &A.final_rec, rec);
+  if (rc == NB_OK) rc = adj_alloc(h, &A.tape, rec * (A.seg - 1));
   if (rc == NB_OK) rc = adj_alloc(h, &A.adj_a, rec);
   if (rc == NB_OK) rc = adj_alloc(h, &A.adj_b, rec);
   for (int f = 0; f < NB_NUM_FIELDS && rc == NB_OK; f++) rc = adj_alloc(h, &A.field[f], h->cap);
@@ -924,16 +940,14 @@ int nb_mpm_forward_recorded(nb_mpm* h, float dt_sub) {
   auto& A = h->adj;
   if (!A.on) return fail(NB_ERR_STATE, "call nb_mpm_adjoint_begin first");
   int rc = sync_in(h);
-  for (uint32_t s = 0; s < A.n && rc == NB_OK; s++) {
-    if (s % A.seg == 0) rc = copy_rec(h, A.ckpt + size_t(s / A.seg) * h->cap * kRec, h->d.rec);
-    if (rc == NB_OK) rc = enqueue_substep(h, dt_sub, nullptr);
-  }
-  if (rc == NB_OK) rc = copy_rec(h, A.final_rec, h->d.rec);
+  if (rc == NB_OK) rc = copy_rec(h, A.ckpt, h->d.rec);
+  for (uint32_t s = 0; s < A.n && rc == NB_OK; s++) rc = enqueue_substep(h, dt_sub, nullptr, adj_slot(h, s), adj_next(h, s));
   if (rc == NB_OK) rc = sync_out(h);
   if (rc != NB_OK) return rc;
   NB_HIP(hipStreamSynchronize(h->stream));
   A.dt = dt_sub;
   A.recorded = true;
+  A.tape_seg = int(A.n_ckpt) - 1;
   return NB_OK;
 }
 
@@ -965,8 +979,6 @@ int nb_mpm_backward(nb_mpm* h, float* d_params, uint32_t params_cap) {
   if (!h) return fail(NB_ERR_ARG, "null handle");
   auto& A = h->adj;
   if (!A.on || !A.recorded) return fail(NB_ERR_STATE, "call nb_mpm_forward_recorded first");
-  const uint32_t cap = h->cap;
-  const size_t rec = size_t(cap) * kRec;
   NB_HIP(hipMemsetAsync(A.a.pgrad, 0, NB_MAX_MATERIALS * 4 * sizeof(float), h->stream));
   launch_soa_to_aos_ptrs(adj_fields(h), A.adj_a, h->n_live, h->stream);
   float4* in = A.adj_a;
@@ -975,18 +987,17 @@ int nb_mpm_backward(nb_mpm* h, float* d_params, uint32_t params_cap) {
   for (int sg = int(A.n_ckpt) - 1; sg >= 0 && rc == NB_OK; sg--) {
     const uint32_t s0 = uint32_t(sg) * A.seg;
     const uint32_t len = std::min(A.seg, A.n - s0);
-    // recompute the segment, keeping the state before each substep
-    rc = copy_rec(h, h->d.rec, A.ckpt + size_t(sg) * rec);
-    h->keys_valid = false;
-    for (uint32_t j = 0; j < len && rc == NB_OK; j++) {
-      rc = copy_rec(h, A.tape + size_t(j) * rec, h->d.rec);
-      if (rc == NB_OK) rc = enqueue_substep(h, A.dt, nullptr);
+    if (A.tape_seg != sg) {
+      // recompute the segment's tape from its checkpoint
+      h->keys_valid = false;
+      for (uint32_t j = 0; j + 1 < len && rc == NB_OK; j++)
+        rc = enqueue_substep(h, A.dt, nullptr, adj_slot(h, s0 + j), adj_slot(h, s0 + j + 1));
+      A.tape_seg = sg;
     }
     for (int j = int(len) - 1; j >= 0 && rc == NB_OK; j--) {
-      rc = copy_rec(h, h->d.rec, A.tape + size_t(j) * rec);
-      if (rc != NB_OK) break;
       SubParams p = h->params(A.dt, false);
-      const DevState dv = h->dev();
+      DevState dv = h->dev();
+      dv.rec = adj_slot(h, s0 + uint32_t(j));
       launch_bin(dv, p, h->NT, false, h->stream, nullptr);
       launch_p2g(dv, p, h->NT, h->stream, nullptr);
       launch_grid(dv, p, h->NT, true, true, h->d.node_tiles, h->d.counters + CNT_NODE_TILES, h->stream, nullptr);
@@ -997,13 +1008,11 @@ int nb_mpm_backward(nb_mpm* h, float* d_params, uint32_t params_cap) {
   }
   if (rc != NB_OK) return rc;
   launch_aos_to_soa_ptrs(in, adj_fields(h), h->n_live, h->stream);
-  // back to the state the recorded run ended in
-  rc = copy_rec(h, h->d.rec, A.final_rec);
-  if (rc != NB_OK) return rc;
+  // the engine's records still hold the state the recorded run ended in; the
+  // bins, keys and grid are those of an earlier substep
   h->keys_valid = false;
   h->bins_valid = false;
   h->grid_mode = GridMode::NONE;
-  if ((rc = sync_out(h)) != NB_OK) return rc;
   if (d_params) {
     float pg[NB_MAX_MATERIALS * 4];
     if ((rc = download(h, pg, A.a.pgrad, sizeof(pg))) != NB_OK) return rc;

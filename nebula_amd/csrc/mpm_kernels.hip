@@ -730,8 +730,11 @@ __global__ __launch_bounds__(512) void k_grid(SubParams p, const float4* __restr
 // ---------------------------------------------------------------------------
 // G2P
 // ---------------------------------------------------------------------------
+// rec_out may equal rec_in: every record slot written is data-dependent on the
+// load of the same slot (or not read at all), so updating in place is safe.
 template <bool FLIP>
-__global__ __launch_bounds__(256) void k_g2p(SubParams p, float4* __restrict__ rec, uint32_t* __restrict__ keys,
+__global__ __launch_bounds__(256) void k_g2p(SubParams p, const float4* rec_in, float4* rec_out,
+                                             uint32_t* __restrict__ keys,
                                              const nb_material* __restrict__ mats,
                                              const uint32_t* __restrict__ sorted,
                                              const float4* __restrict__ grid_node,
@@ -747,7 +750,8 @@ __global__ __launch_bounds__(256) void k_g2p(SubParams p, float4* __restrict__ r
   const uint32_t k = xcd_chunk(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (k >= p.n) return;
   const uint32_t q = sorted ? sorted[k] : k;
-  float4* r = rec + size_t(q) * kRec;
+  const float4* r = rec_in + size_t(q) * kRec;
+  float4* ro = rec_out + size_t(q) * kRec;
   const float4 pp = r[R_P];
   const float xp = pp.x, yp = pp.y, zp = pp.z, Vp = pp.w;
   constexpr bool flip = FLIP;  // flip_ratio > 0.01
@@ -1029,14 +1033,14 @@ __global__ __launch_bounds__(256) void k_g2p(SubParams p, float4* __restrict__ r
     if (zn < loz) { zn = loz; vnz = fmaxf(vnz, 0.f); }
     if (zn > hiz) { zn = hiz; vnz = fminf(vnz, 0.f); }
   }
-  r[R_P] = make_float4(xn, yn, zn, Vp);
-  r[R_V] = make_float4(vnx, vny, vnz, vel.w);
-  r[R_B0] = make_float4(Bn.m[0][0], Bn.m[0][1], Bn.m[0][2], tau.m[0][1]);
-  r[R_B1] = make_float4(Bn.m[1][0], Bn.m[1][1], Bn.m[1][2], tau.m[0][2]);
-  r[R_B2] = make_float4(Bn.m[2][0], Bn.m[2][1], Bn.m[2][2], tau.m[1][2]);
-  r[R_T] = make_float4(tau.m[0][0], tau.m[1][1], tau.m[2][2], Fn.m[0][0]);
-  r[R_FA] = make_float4(Fn.m[0][1], Fn.m[0][2], Fn.m[1][0], Fn.m[1][1]);
-  r[R_FB] = make_float4(Fn.m[1][2], Fn.m[2][0], Fn.m[2][1], Fn.m[2][2]);
+  ro[R_P] = make_float4(xn, yn, zn, Vp);
+  ro[R_V] = make_float4(vnx, vny, vnz, vel.w);
+  ro[R_B0] = make_float4(Bn.m[0][0], Bn.m[0][1], Bn.m[0][2], tau.m[0][1]);
+  ro[R_B1] = make_float4(Bn.m[1][0], Bn.m[1][1], Bn.m[1][2], tau.m[0][2]);
+  ro[R_B2] = make_float4(Bn.m[2][0], Bn.m[2][1], Bn.m[2][2], tau.m[1][2]);
+  ro[R_T] = make_float4(tau.m[0][0], tau.m[1][1], tau.m[2][2], Fn.m[0][0]);
+  ro[R_FA] = make_float4(Fn.m[0][1], Fn.m[0][2], Fn.m[1][0], Fn.m[1][1]);
+  ro[R_FB] = make_float4(Fn.m[1][2], Fn.m[2][0], Fn.m[2][1], Fn.m[2][2]);
   keys[q] = cell_key(p, xn, yn, zn);  // the next substep's bin key
 }
 
@@ -1179,13 +1183,14 @@ void launch_grid(const DevState& d, const SubParams& p, uint32_t n_tiles, bool f
     k_grid<false, true><<<g, 512, 0, s>>>(p, d.partial, d.tile_act, tiles, count, d.grid_node, d.grid_old, d.sdf, d.colliders, d.forces);
 }
 
-void launch_g2p(const DevState& d, const SubParams& p, bool use_sorted, hipStream_t s, Prof* pf) {
+void launch_g2p(const DevState& d, const SubParams& p, bool use_sorted, hipStream_t s, Prof* pf, float4* rec_out) {
   ProfScope ps(pf, 10, s);
   const uint32_t* srt = use_sorted ? d.sorted : nullptr;
+  float4* out = rec_out ? rec_out : d.rec;
   if (p.flip_ratio > 0.01f)
-    k_g2p<true><<<cdiv(p.n, 256), 256, 0, s>>>(p, d.rec, d.keys, d.materials, srt, d.grid_node, d.grid_old, d.sdf);
+    k_g2p<true><<<cdiv(p.n, 256), 256, 0, s>>>(p, d.rec, out, d.keys, d.materials, srt, d.grid_node, d.grid_old, d.sdf);
   else
-    k_g2p<false><<<cdiv(p.n, 256), 256, 0, s>>>(p, d.rec, d.keys, d.materials, srt, d.grid_node, d.grid_old, d.sdf);
+    k_g2p<false><<<cdiv(p.n, 256), 256, 0, s>>>(p, d.rec, out, d.keys, d.materials, srt, d.grid_node, d.grid_old, d.sdf);
 }
 
 void launch_sdf_sphere(float* sdf, const SubParams& p, float r, float cx, float cy, float cz, hipStream_t s) {

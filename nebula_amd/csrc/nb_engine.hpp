@@ -123,7 +123,9 @@ void launch_p2g(const DevState& d, const SubParams& p, uint32_t n_tiles, hipStre
 void launch_grid(const DevState& d, const SubParams& p, uint32_t n_tiles, bool from_partials, bool update,
                  const uint32_t* tiles, const uint32_t* count, hipStream_t s, Prof* pf);
 // use_sorted: visit particles in bin order (else index order)
-void launch_g2p(const DevState& d, const SubParams& p, bool use_sorted, hipStream_t s, Prof* pf);
+// rec_out: where the updated records go (nullptr: in place over d.rec)
+void launch_g2p(const DevState& d, const SubParams& p, bool use_sorted, hipStream_t s, Prof* pf,
+                float4* rec_out = nullptr);
 void launch_sdf_sphere(float* sdf, const SubParams& p, float r, float cx, float cy, float cz, hipStream_t s);
 void launch_fill_identity(uint32_t* a, uint32_t n, hipStream_t s);
 void launch_init_particles(const DevState& d, uint32_t first, uint32_t count, hipStream_t s);
