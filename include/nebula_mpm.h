@@ -290,8 +290,9 @@ int nb_mpm_active_tile_count(nb_mpm* h, uint32_t* tiles);
  * all n states and recomputes nothing), and returns dL/d(mu, lambda,
  * M_friction, q_cohesion) of every material (material_count x 4 floats) in
  * d_params.  The engine is left in the state the recorded run ended in.
- * Memory: (ceil(n/segment) + segment + 1) x 128 B + 224 B per particle of
- * capacity, plus two grid-sized buffers. */
+ * Memory per particle of capacity: (ceil(n/segment) + segment - 1) x 132 B
+ * (records and bin keys of the kept states) + 480 B, plus two grid-sized
+ * buffers. */
 int nb_mpm_adjoint_begin(nb_mpm* h, uint32_t n_substeps, uint32_t segment);
 int nb_mpm_forward_recorded(nb_mpm* h, float dt_sub);
 int nb_mpm_set_adjoint(nb_mpm* h, int field, const float* in4, uint32_t first, uint32_t count);

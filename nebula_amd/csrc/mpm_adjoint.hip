@@ -14,7 +14,8 @@
 //   k_grid_bwd     per node: the grid update backwards -> dL/dp, dL/dm
 //   k_p2g_bwd      per particle: P2G backwards as a 27-node gather
 //                  -> d/dv, d/dB, d/dtau and the position part from P2G
-//   k_pgrad_reduce per-wave material-parameter partials summed in a fixed order
+//   k_pgrad_chunks / k_pgrad_total  per-block material-parameter partials
+//                  summed in a fixed order
 // Positions enter every weight, so both transfers contribute the weight
 // gradients and Hessians.  Everything is deterministic.
 #include <hip/hip_runtime.h>
@@ -186,10 +187,10 @@ __global__ __launch_bounds__(256) void k_g2p_bwd(SubParams p, const float4* __re
 #pragma unroll
       for (int rr = 0; rr < 3; rr++) L.m[c][rr] = (c == rr ? 1.f : 0.f) + p.dt * vg.m[c][rr];
     const m3 Ft = m3_mul(L, F);
-    const PlasticOut po = project_plastic(Ft, model == 2, mat.mu, mat.lambda, mat.q_max, mat.M_friction,
-                                          mat.q_cohesion, true);
-    const m3 Fp = model == 0 ? Ft : po.F;
-    const float Jd = fabsf(m3_det(Fp));
+    PlasticFwd pf;
+    plastic_forward(Ft, model, mat.mu, mat.lambda, mat.q_max, mat.M_friction, mat.q_cohesion, pf);
+    // det F' = ss0 ss1 ss2 when recomposed from rotations, det F_tmp otherwise
+    const float Jd = pf.rm.recompose ? pf.ss[0] * pf.ss[1] * pf.ss[2] : fabsf(m3_det(Ft));
     const bool jreset = Jd < 0.15f || Jd > 10.f;
     const bool apic = p.flip_ratio < -0.5f;
     float v1[3] = {vp[0], vp[1], vp[2]};
@@ -284,9 +285,7 @@ __global__ __launch_bounds__(256) void k_g2p_bwd(SubParams p, const float4* __re
     }
     const m3 MB = apic ? gB2 : m3_zero();
     m3 gFt = m3_zero();
-    if (!jreset && !pushed)
-      gFt = plastic_hencky_adjoint(Ft, model, mat.mu, mat.lambda, mat.q_max, mat.M_friction, mat.q_cohesion, gF2,
-                                   gT, g4);
+    if (!jreset && !pushed) gFt = plastic_hencky_adjoint(pf, mat.mu, mat.lambda, gF2, gT, g4);
     // F_tmp = L F,  L = I + dt grad v
     m3 MG = m3_mult(gFt, F);
 #pragma unroll
@@ -339,10 +338,12 @@ __global__ __launch_bounds__(256) void k_g2p_bwd(SubParams p, const float4* __re
 #pragma unroll
     for (int j = 0; j < kCoef; j++) co[j] = make_float4(cf[4 * j], cf[4 * j + 1], cf[4 * j + 2], cf[4 * j + 3]);
   }
-  // per-wave material-parameter partials: slot (material, parameter), 64 slots
-  const uint32_t lane = threadIdx.x & 63u;
-  float* slot = pgrad_w + size_t(k >> 6) * (NB_MAX_MATERIALS * 4);
-  slot[lane] = 0.f;
+  // material-parameter partials: slot (material, parameter), 64 per wave,
+  // summed over the block's waves in a fixed order, one row per block
+  static_assert(NB_MAX_MATERIALS * 4 == 64, "one slot per lane");
+  __shared__ float s_pg[4][64];
+  const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+  s_pg[wv][lane] = 0.f;
   uint64_t pending = __ballot(valid);
   while (pending) {
     const int leader = __ffsll((unsigned long long)pending) - 1;
@@ -358,10 +359,13 @@ __global__ __launch_bounds__(256) void k_g2p_bwd(SubParams p, const float4* __re
     }
     if (lane == 0) {
 #pragma unroll
-      for (int j = 0; j < 4; j++) slot[lmid * 4 + j] = s4[j];
+      for (int j = 0; j < 4; j++) s_pg[wv][lmid * 4 + j] = s4[j];
     }
     pending &= ~__ballot(mine);
   }
+  __syncthreads();
+  if (threadIdx.x < 64)
+    pgrad_w[size_t(blockIdx.x) * 64 + lane] = ((s_pg[0][lane] + s_pg[1][lane]) + s_pg[2][lane]) + s_pg[3][lane];
 }
 
 // ---------------------------------------------------------------------------
@@ -662,20 +666,27 @@ __global__ __launch_bounds__(256) void k_p2g_bwd(SubParams p, const float4* __re
   go[R_T] = make_float4(gT.m[0][0], gT.m[1][1], gT.m[2][2], gt.w);
 }
 
-// fixed-order sum of the per-wave parameter partials, added to the running total
-__global__ __launch_bounds__(256) void k_pgrad_reduce(const float* __restrict__ pgrad_w, uint32_t n_waves,
-                                                      float* __restrict__ pgrad) {
-  __shared__ float s[256];
-  const uint32_t j = blockIdx.x;  // slot
+// Fixed-order sum of the per-block parameter rows (64 floats each): every
+// block of 256 threads sums kPgChunk consecutive rows, lane = slot, the four
+// waves taking interleaved rows; then one block adds the chunk sums, in
+// order, to the running total.
+constexpr uint32_t kPgChunk = 256;
+__global__ __launch_bounds__(256) void k_pgrad_chunks(const float* __restrict__ rows, uint32_t n_rows,
+                                                      float* __restrict__ chunks) {
+  __shared__ float s[4][64];
+  const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+  const uint32_t r0 = blockIdx.x * kPgChunk, r1 = min(r0 + kPgChunk, n_rows);
   float acc = 0.f;
-  for (uint32_t w = threadIdx.x; w < n_waves; w += 256) acc += pgrad_w[size_t(w) * (NB_MAX_MATERIALS * 4) + j];
-  s[threadIdx.x] = acc;
+  for (uint32_t r = r0 + wv; r < r1; r += 4) acc += rows[size_t(r) * 64 + lane];
+  s[wv][lane] = acc;
   __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (int(threadIdx.x) < o) s[threadIdx.x] += s[threadIdx.x + o];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) pgrad[j] += s[0];
+  if (threadIdx.x < 64) chunks[size_t(blockIdx.x) * 64 + lane] = ((s[0][lane] + s[1][lane]) + s[2][lane]) + s[3][lane];
+}
+__global__ __launch_bounds__(64) void k_pgrad_total(const float* __restrict__ chunks, uint32_t n_chunks,
+                                                    float* __restrict__ pgrad) {
+  float acc = 0.f;
+  for (uint32_t c = 0; c < n_chunks; c++) acc += chunks[size_t(c) * 64 + threadIdx.x];
+  pgrad[threadIdx.x] += acc;
 }
 
 // ---------------------------------------------------------------------------
@@ -694,7 +705,10 @@ void launch_adjoint_substep(const DevState& d, const AdjState& a, const SubParam
   k_grid_bwd<<<tile_grid_a(n_tiles), 512, 0, s>>>(p, d.partial, a.partial_adj, d.tile_act, d.node_tiles,
                                                    d.counters + CNT_NODE_TILES, a.grid_adj, d.sdf, d.forces);
   k_p2g_bwd<<<nblk, 256, 0, s>>>(p, d.rec, adj_out, d.materials, d.sorted, a.grid_adj);
-  k_pgrad_reduce<<<NB_MAX_MATERIALS * 4, 256, 0, s>>>(a.pgrad_w, nblk * 4u, a.pgrad);
+  const uint32_t n_chunks = cdiv_a(nblk, kPgChunk);
+  float* chunks = a.pgrad_w + size_t(nblk) * 64;
+  k_pgrad_chunks<<<n_chunks, 256, 0, s>>>(a.pgrad_w, nblk, chunks);
+  k_pgrad_total<<<1, 64, 0, s>>>(chunks, n_chunks, a.pgrad);
 }
 
 }  // namespace nb

@@ -85,6 +85,8 @@ struct nb_mpm {
     // for each earlier segment from its checkpoint.
     float4* ckpt = nullptr;   // n_ckpt * cap * kRec
     float4* tape = nullptr;   // (seg - 1) * cap * kRec
+    uint32_t* ckpt_keys = nullptr;  // bin keys of the same states: n_ckpt * cap
+    uint32_t* tape_keys = nullptr;  // (seg - 1) * cap
     int tape_seg = -1;        // the segment whose records the tape holds
     float4* adj_a = nullptr;  // cap * kRec
     float4* adj_b = nullptr;
@@ -192,17 +194,21 @@ int sync_out(nb_mpm* h) {
 
 // one substep on the records (the caller brackets a run of them with
 // sync_in / sync_out)
-// One substep.  rec_in / rec_out (default: the engine's records, in place)
-// let the recorded run move the state along its tape without copies.
-int enqueue_substep(nb_mpm* h, float dt, Prof* pf, float4* rec_in = nullptr, float4* rec_out = nullptr) {
+// One substep.  rec_in / rec_out and keys_in / keys_out (default: the
+// engine's records and bin keys, in place) let the recorded run move the
+// state along its tape without copies.
+int enqueue_substep(nb_mpm* h, float dt, Prof* pf, float4* rec_in = nullptr, float4* rec_out = nullptr,
+                    uint32_t* keys_in = nullptr, uint32_t* keys_out = nullptr) {
   if (h->n_live == 0) return NB_OK;
   const bool flip = h->knobs.flip_ratio > 0.01f;
   SubParams p = h->params(dt, flip);
   DevState dv = h->dev();
   if (rec_in) dv.rec = rec_in;
+  if (keys_in) dv.keys = keys_in;
   launch_bin(dv, p, h->NT, h->keys_valid, h->stream, pf);
   launch_p2g(dv, p, h->NT, h->stream, pf);
   launch_grid(dv, p, h->NT, true, true, h->d.node_tiles, h->d.counters + CNT_NODE_TILES, h->stream, pf);
+  dv.keys = keys_out ? keys_out : h->d.keys;  // G2P writes the next substep's keys
   launch_g2p(dv, p, true, h->stream, pf, rec_out);
   NB_HIP(hipGetLastError());
   h->bins_valid = true;
@@ -881,6 +887,13 @@ float4* adj_slot(nb_mpm* h, uint32_t s) {
 }
 // where substep s writes its result: the next slot, or the engine's records after the last substep
 float4* adj_next(nb_mpm* h, uint32_t s) { return s + 1 == h->adj.n ? h->d.rec : adj_slot(h, s + 1); }
+// the bin keys of the records in adj_slot(h, s) / adj_next(h, s)
+uint32_t* key_slot(nb_mpm* h, uint32_t s) {
+  const auto& A = h->adj;
+  const uint32_t sg = s / A.seg, j = s % A.seg;
+  return j == 0 ? A.ckpt_keys + size_t(sg) * h->cap : A.tape_keys + size_t(j - 1) * h->cap;
+}
+uint32_t* key_next(nb_mpm* h, uint32_t s) { return s + 1 == h->adj.n ? h->d.keys : key_slot(h, s + 1); }
 }  // namespace
 
 int nb_mpm_adjoint_begin(nb_mpm* h, uint32_t n_substeps, uint32_t segment) {
@@ -901,10 +914,15 @@ int nb_mpm_adjoint_begin(nb_mpm* h, uint32_t n_substeps, uint32_t segment) {
   A.seg = std::min(segment, n_substeps);
   A.n_ckpt = (n_substeps + A.seg - 1) / A.seg;
   const size_t rec = size_t(h->cap) * kRec;
-  const uint32_t n_waves = ((h->cap + 255u) / 256u) * 4u;
+  const uint32_t n_blocks = (h->cap + 255u) / 256u;  // blocks of the G2P backward
+  const uint32_t n_chunks = (n_blocks + 255u) / 256u;
   int rc = NB_OK;
   if (rc == NB_OK) rc = adj_alloc(h, &A.ckpt, rec * A.n_ckpt);
   if (rc == NB_OK) rc = adj_alloc(h, &A.tape, rec * (A.seg - 1));
+  float4* kc = nullptr;
+  float4* kt = nullptr;
+  if (rc == NB_OK) rc = adj_alloc(h, &kc, (size_t(h->cap) * A.n_ckpt + 3) / 4);
+  if (rc == NB_OK) rc = adj_alloc(h, &kt, (size_t(h->cap) * (A.seg - 1) + 3) / 4);
   if (rc == NB_OK) rc = adj_alloc(h, &A.adj_a, rec);
   if (rc == NB_OK) rc = adj_alloc(h, &A.adj_b, rec);
   for (int f = 0; f < NB_NUM_FIELDS && rc == NB_OK; f++) rc = adj_alloc(h, &A.field[f], h->cap);
@@ -913,7 +931,7 @@ int nb_mpm_adjoint_begin(nb_mpm* h, uint32_t n_substeps, uint32_t segment) {
   if (rc == NB_OK) rc = adj_alloc(h, &A.a.grid_adj, size_t(h->NT) * kTileCells);
   float4* pw = nullptr;
   float4* pg = nullptr;
-  if (rc == NB_OK) rc = adj_alloc(h, &pw, size_t(n_waves) * NB_MAX_MATERIALS);
+  if (rc == NB_OK) rc = adj_alloc(h, &pw, size_t(n_blocks + n_chunks) * NB_MAX_MATERIALS);
   if (rc == NB_OK) rc = adj_alloc(h, &pg, NB_MAX_MATERIALS);
   if (rc != NB_OK) {
     std::string keep = g_err;
@@ -922,6 +940,8 @@ int nb_mpm_adjoint_begin(nb_mpm* h, uint32_t n_substeps, uint32_t segment) {
     return rc;
   }
   A.a.pgrad_w = reinterpret_cast<float*>(pw);
+  A.ckpt_keys = reinterpret_cast<uint32_t*>(kc);
+  A.tape_keys = reinterpret_cast<uint32_t*>(kt);
   A.a.pgrad = reinterpret_cast<float*>(pg);
   for (int f = 0; f < NB_NUM_FIELDS; f++) NB_HIP(hipMemsetAsync(A.field[f], 0, size_t(h->cap) * 16, h->stream));
   NB_HIP(hipStreamSynchronize(h->stream));
@@ -941,7 +961,9 @@ int nb_mpm_forward_recorded(nb_mpm* h, float dt_sub) {
   if (!A.on) return fail(NB_ERR_STATE, "call nb_mpm_adjoint_begin first");
   int rc = sync_in(h);
   if (rc == NB_OK) rc = copy_rec(h, A.ckpt, h->d.rec);
-  for (uint32_t s = 0; s < A.n && rc == NB_OK; s++) rc = enqueue_substep(h, dt_sub, nullptr, adj_slot(h, s), adj_next(h, s));
+  h->keys_valid = false;  // the first substep writes its keys into the first key slot
+  for (uint32_t s = 0; s < A.n && rc == NB_OK; s++)
+    rc = enqueue_substep(h, dt_sub, nullptr, adj_slot(h, s), adj_next(h, s), key_slot(h, s), key_next(h, s));
   if (rc == NB_OK) rc = sync_out(h);
   if (rc != NB_OK) return rc;
   NB_HIP(hipStreamSynchronize(h->stream));
@@ -989,16 +1011,18 @@ int nb_mpm_backward(nb_mpm* h, float* d_params, uint32_t params_cap) {
     const uint32_t len = std::min(A.seg, A.n - s0);
     if (A.tape_seg != sg) {
       // recompute the segment's tape from its checkpoint
-      h->keys_valid = false;
+      h->keys_valid = true;
       for (uint32_t j = 0; j + 1 < len && rc == NB_OK; j++)
-        rc = enqueue_substep(h, A.dt, nullptr, adj_slot(h, s0 + j), adj_slot(h, s0 + j + 1));
+        rc = enqueue_substep(h, A.dt, nullptr, adj_slot(h, s0 + j), adj_slot(h, s0 + j + 1), key_slot(h, s0 + j),
+                             key_slot(h, s0 + j + 1));
       A.tape_seg = sg;
     }
     for (int j = int(len) - 1; j >= 0 && rc == NB_OK; j--) {
       SubParams p = h->params(A.dt, false);
       DevState dv = h->dev();
       dv.rec = adj_slot(h, s0 + uint32_t(j));
-      launch_bin(dv, p, h->NT, false, h->stream, nullptr);
+      dv.keys = key_slot(h, s0 + uint32_t(j));
+      launch_bin(dv, p, h->NT, true, h->stream, nullptr);
       launch_p2g(dv, p, h->NT, h->stream, nullptr);
       launch_grid(dv, p, h->NT, true, true, h->d.node_tiles, h->d.counters + CNT_NODE_TILES, h->stream, nullptr);
       launch_adjoint_substep(dv, A.a, p, h->NT, in, out, h->stream);

@@ -135,25 +135,39 @@ __device__ __forceinline__ void return_map(const float e[3], int model, float mu
   o.s = sc;
 }
 
+// The forward of the particle-local map from one SVD of F_tmp: singular
+// frame, return map and the singular values of F' (ss; F' = U diag(ss) V^T
+// when rm.recompose, F' = F_tmp otherwise).
+struct PlasticFwd {
+  m3 U, V;
+  float sig[3], ss[3];
+  ReturnMap rm;
+};
+
+__device__ __forceinline__ void plastic_forward(const m3& Ft, int model, float mu, float lam, float q_max, float M,
+                                                float q_c, PlasticFwd& f) {
+  svd3(Ft, f.U, f.sig, f.V, true);
+  float e[3];
+#pragma unroll
+  for (int i = 0; i < 3; i++) e[i] = eps_of(f.sig[i]);
+  return_map(e, model, mu, lam, q_max, M, q_c, f.rm);
+#pragma unroll
+  for (int i = 0; i < 3; i++) f.ss[i] = f.rm.recompose ? expf(f.rm.es[i]) : f.sig[i];
+}
+
 // Adjoint of  F' = return_map(F_tmp), tau = hencky(F').
 // gFp = dL/dF', gT = dL/dtau (full symmetric matrix).  Returns dL/dF_tmp and
 // adds dL/d(mu, lambda, M, q_c) to g4.
-__device__ inline m3 plastic_hencky_adjoint(const m3& Ft, int model, float mu, float lam, float q_max, float M, float q_c,
-                                            const m3& gFp, const m3& gT, float g4[4]) {
-  m3 U, V;
-  float sig[3];
-  svd3(Ft, U, sig, V, true);
-  float e[3];
+__device__ inline m3 plastic_hencky_adjoint(const PlasticFwd& f, float mu, float lam, const m3& gFp, const m3& gT,
+                                            float g4[4]) {
+  const m3& U = f.U;
+  const m3& V = f.V;
+  const float* sig = f.sig;
+  const float* ss = f.ss;
+  const ReturnMap& rm = f.rm;
+  float ep[3];
 #pragma unroll
-  for (int i = 0; i < 3; i++) e[i] = eps_of(sig[i]);
-  ReturnMap rm;
-  return_map(e, model, mu, lam, q_max, M, q_c, rm);
-  float ss[3], ep[3];
-#pragma unroll
-  for (int i = 0; i < 3; i++) {
-    ss[i] = rm.recompose ? expf(rm.es[i]) : sig[i];
-    ep[i] = eps_of(ss[i]);
-  }
+  for (int i = 0; i < 3; i++) ep[i] = eps_of(ss[i]);
   // Hencky: tau = U diag(k) U^T, k_i = lam tr(ep) + 2 mu ep_i
   const m3 Gt = m3_mul(m3_tmul(U, gT), U);  // U^T gT U
   const float trG = Gt.m[0][0] + Gt.m[1][1] + Gt.m[2][2];
