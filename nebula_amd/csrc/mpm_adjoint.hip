@@ -4,7 +4,9 @@
 //
 // Per substep s (state x, v, F, B, tau -> x', v', F', B', tau'), given the
 // adjoint of the state after the substep, the kernels run in this order on
-// the recomputed forward (bins, P2G partials and grid velocities of s):
+// the bins of s (rebuilt from its kept keys) and the grid the recorded
+// forward kept for s (velocities after the update; momentum / mass and mass
+// before it):
 //   k_g2p_bwd      per particle: the G2P chain backwards (advection, clamps,
 //                  push-out, J reset, return map + Hencky adjoint, F update);
 //                  writes d/dF, the position part from G2P's weights, and the
@@ -70,6 +72,223 @@ __device__ __forceinline__ void node_w(const AxisD& X, const AxisD& Y, const Axi
   n.d[0] = X.dx[ix];
   n.d[1] = Y.dx[iy];
   n.d[2] = Z.dx[iz];
+}
+
+// ---- separable gathers ----
+// For a field f on the 27 stencil nodes, the backward kernels need the sums
+// of f times 22 products of per-axis factors (w, g = w', hh = w'', wd = w d,
+// gd = g d with d = x_node - x_p on that axis):
+//   kW               w
+//   kWD + a          w d_a
+//   kG + a           dw/dx_a
+//   gd_idx(b, c)     dw/dx_b d_c     (kGD + a on the diagonal)
+//   h_idx(a, b)      d2w/dx_a dx_b   (kHD + a on the diagonal)
+// The first seven are returned as sums; each of the last fifteen goes to the
+// caller's sink(k, z, q) as a term z * q[3] of its sum, so a caller that
+// only needs them contracted with per-particle coefficients keeps no sums.
+constexpr int kW = 0, kWD = 1, kG = 4, kGD = 7, kHD = 16, kNSum = 7;
+__host__ __device__ constexpr int gd_idx(int b, int c) { return b == c ? kGD + b : 10 + 2 * b + (c < b ? c : c - 1); }
+__host__ __device__ constexpr int h_idx(int a, int b) { return a == b ? kHD + a : 18 + a + b; }
+// axes (b, c) of gd_idx and (a, b), a <= b, of h_idx
+__host__ __device__ constexpr int sep_a(int k) {
+  return k < 10 ? k - kGD : (k < kHD ? (k - 10) / 2 : (k < 19 ? k - kHD : (k == 21 ? 1 : 0)));
+}
+__host__ __device__ constexpr int sep_b(int k) {
+  return k < 10 ? k - kGD
+                : (k < kHD ? ((k - 10) % 2 < (k - 10) / 2 ? (k - 10) % 2 : (k - 10) % 2 + 1)
+                           : (k < 19 ? k - kHD : (k == 19 ? 1 : 2)));
+}
+static_assert(gd_idx(2, 1) == 15 && h_idx(0, 1) == 19 && h_idx(2, 1) == 21, "separable product table");
+static_assert(sep_a(gd_idx(1, 2)) == 1 && sep_b(gd_idx(1, 2)) == 2 && sep_a(gd_idx(2, 0)) == 2 &&
+                  sep_b(gd_idx(2, 0)) == 0 && sep_a(h_idx(0, 2)) == 0 && sep_b(h_idx(0, 2)) == 2 &&
+                  sep_a(h_idx(1, 2)) == 1 && sep_b(h_idx(2, 1)) == 2 && sep_a(h_idx(1, 1)) == 1,
+              "separable axis table");
+
+struct AxisF {
+  float w[3], g[3], hh[3], wd[3], gd[3], dx[3];
+};
+__device__ __forceinline__ void axis_factors(const AxisD& a, AxisF& f) {
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    f.w[i] = a.w[i];
+    f.g[i] = a.g[i];
+    f.hh[i] = a.hh[i];
+    f.wd[i] = a.w[i] * a.dx[i];
+    f.gd[i] = a.g[i] * a.dx[i];
+    f.dx[i] = a.dx[i];
+  }
+}
+
+// Axis-by-axis contraction: x (5 factor kinds per row), then y (the 12
+// (x, y) pairs the products use), then z.  load(ix, iy, iz) returns the
+// node's field in xyz and, when WITH_M, a scalar field in w whose sums with
+// dw/dx_a go to Sm.  No weight cut: the caller takes this path only when no
+// node weight lies in (0, 1e-12).  The z loop stays rolled so that one slab
+// of 9 node loads is live at a time; its z factors sit in position 0 and are
+// rotated at the end of each iteration.
+template <bool WITH_M, class Load, class Sink>
+__device__ __forceinline__ void sep_gather(const AxisF& X, const AxisF& Y, const AxisF& Z, Load load,
+                                           float S[kNSum][3], float Sm[3], Sink sink) {
+#pragma unroll
+  for (int k = 0; k < kNSum; k++) S[k][0] = S[k][1] = S[k][2] = 0.f;
+  Sm[0] = Sm[1] = Sm[2] = 0.f;
+  float zw_[3] = {Z.w[0], Z.w[1], Z.w[2]}, zg_[3] = {Z.g[0], Z.g[1], Z.g[2]};
+  float zh_[3] = {Z.hh[0], Z.hh[1], Z.hh[2]}, zwd_[3] = {Z.wd[0], Z.wd[1], Z.wd[2]};
+  float zgd_[3] = {Z.gd[0], Z.gd[1], Z.gd[2]};
+#pragma unroll 1
+  for (int iz = 0; iz < 3; iz++) {
+    float Q[12][3], Qm[3];
+#pragma unroll
+    for (int k = 0; k < 12; k++) Q[k][0] = Q[k][1] = Q[k][2] = 0.f;
+    Qm[0] = Qm[1] = Qm[2] = 0.f;
+#pragma unroll
+    for (int iy = 0; iy < 3; iy++) {
+      float R[5][3], Rm[2];  // x kinds: w, g, hh, wd, gd
+#pragma unroll
+      for (int k = 0; k < 5; k++) R[k][0] = R[k][1] = R[k][2] = 0.f;
+      Rm[0] = Rm[1] = 0.f;
+#pragma unroll
+      for (int ix = 0; ix < 3; ix++) {
+        const float4 f4 = load(ix, iy, iz);
+        const float f[3] = {f4.x, f4.y, f4.z};
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+          R[0][r] = fmaf(X.w[ix], f[r], R[0][r]);
+          R[1][r] = fmaf(X.g[ix], f[r], R[1][r]);
+          R[2][r] = fmaf(X.hh[ix], f[r], R[2][r]);
+          R[3][r] = fmaf(X.wd[ix], f[r], R[3][r]);
+          R[4][r] = fmaf(X.gd[ix], f[r], R[4][r]);
+        }
+        if (WITH_M) {
+          Rm[0] = fmaf(X.w[ix], f4.w, Rm[0]);
+          Rm[1] = fmaf(X.g[ix], f4.w, Rm[1]);
+        }
+      }
+      const float yw = Y.w[iy], yg = Y.g[iy], yh = Y.hh[iy], ywd = Y.wd[iy], ygd = Y.gd[iy];
+#pragma unroll
+      for (int r = 0; r < 3; r++) {
+        Q[0][r] = fmaf(yw, R[0][r], Q[0][r]);    // (w, w)
+        Q[1][r] = fmaf(yw, R[3][r], Q[1][r]);    // (wd, w)
+        Q[2][r] = fmaf(ywd, R[0][r], Q[2][r]);   // (w, wd)
+        Q[3][r] = fmaf(yw, R[1][r], Q[3][r]);    // (g, w)
+        Q[4][r] = fmaf(yg, R[0][r], Q[4][r]);    // (w, g)
+        Q[5][r] = fmaf(yw, R[4][r], Q[5][r]);    // (gd, w)
+        Q[6][r] = fmaf(ygd, R[0][r], Q[6][r]);   // (w, gd)
+        Q[7][r] = fmaf(ywd, R[1][r], Q[7][r]);   // (g, wd)
+        Q[8][r] = fmaf(yg, R[3][r], Q[8][r]);    // (wd, g)
+        Q[9][r] = fmaf(yw, R[2][r], Q[9][r]);    // (hh, w)
+        Q[10][r] = fmaf(yh, R[0][r], Q[10][r]);  // (w, hh)
+        Q[11][r] = fmaf(yg, R[1][r], Q[11][r]);  // (g, g)
+      }
+      if (WITH_M) {
+        Qm[0] = fmaf(yw, Rm[1], Qm[0]);  // (g, w)
+        Qm[1] = fmaf(yg, Rm[0], Qm[1]);  // (w, g)
+        Qm[2] = fmaf(yw, Rm[0], Qm[2]);  // (w, w)
+      }
+    }
+    const float zw = zw_[0], zg = zg_[0], zh = zh_[0], zwd = zwd_[0], zgd = zgd_[0];
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+      S[kW][r] = fmaf(zw, Q[0][r], S[kW][r]);
+      S[kWD + 0][r] = fmaf(zw, Q[1][r], S[kWD + 0][r]);
+      S[kWD + 1][r] = fmaf(zw, Q[2][r], S[kWD + 1][r]);
+      S[kWD + 2][r] = fmaf(zwd, Q[0][r], S[kWD + 2][r]);
+      S[kG + 0][r] = fmaf(zw, Q[3][r], S[kG + 0][r]);
+      S[kG + 1][r] = fmaf(zw, Q[4][r], S[kG + 1][r]);
+      S[kG + 2][r] = fmaf(zg, Q[0][r], S[kG + 2][r]);
+    }
+    sink(gd_idx(0, 0), zw, Q[5]);
+    sink(gd_idx(1, 1), zw, Q[6]);
+    sink(gd_idx(2, 2), zgd, Q[0]);
+    sink(gd_idx(0, 1), zw, Q[7]);
+    sink(gd_idx(0, 2), zwd, Q[3]);
+    sink(gd_idx(1, 0), zw, Q[8]);
+    sink(gd_idx(1, 2), zwd, Q[4]);
+    sink(gd_idx(2, 0), zg, Q[1]);
+    sink(gd_idx(2, 1), zg, Q[2]);
+    sink(h_idx(0, 0), zw, Q[9]);
+    sink(h_idx(1, 1), zw, Q[10]);
+    sink(h_idx(2, 2), zh, Q[0]);
+    sink(h_idx(0, 1), zw, Q[11]);
+    sink(h_idx(0, 2), zg, Q[3]);
+    sink(h_idx(1, 2), zg, Q[4]);
+    if (WITH_M) {
+      Sm[0] = fmaf(zw, Qm[0], Sm[0]);
+      Sm[1] = fmaf(zw, Qm[1], Sm[1]);
+      Sm[2] = fmaf(zg, Qm[2], Sm[2]);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+      zw_[i] = zw_[i + 1];
+      zg_[i] = zg_[i + 1];
+      zh_[i] = zh_[i + 1];
+      zwd_[i] = zwd_[i + 1];
+      zgd_[i] = zgd_[i + 1];
+    }
+  }
+}
+
+// The same sums node by node with the transfers' weight cut (nodes with
+// w < 1e-12 are skipped), for waves where a weight lies in (0, 1e-12).
+template <bool WITH_M, class Load, class Sink>
+__device__ __forceinline__ void exact_gather(const AxisF& X, const AxisF& Y, const AxisF& Z, Load load,
+                                             float S[kNSum][3], float Sm[3], Sink sink) {
+#pragma unroll
+  for (int k = 0; k < kNSum; k++) S[k][0] = S[k][1] = S[k][2] = 0.f;
+  Sm[0] = Sm[1] = Sm[2] = 0.f;
+  float zw_[3] = {Z.w[0], Z.w[1], Z.w[2]}, zg_[3] = {Z.g[0], Z.g[1], Z.g[2]};
+  float zh_[3] = {Z.hh[0], Z.hh[1], Z.hh[2]}, zd_[3] = {Z.dx[0], Z.dx[1], Z.dx[2]};
+#pragma unroll 1
+  for (int iz = 0; iz < 3; iz++) {
+    const float wz = zw_[0], gz = zg_[0], hz = zh_[0], dz = zd_[0];
+#pragma unroll
+    for (int o = 0; o < 9; o++) {
+      const int ix = o % 3, iy = o / 3;
+      const float wx = X.w[ix], wy = Y.w[iy];
+      const float w = wx * wy * wz;
+      if (w < 1e-12f) continue;
+      const float gx = X.g[ix], gy = Y.g[iy];
+      const float dw[3] = {gx * wy * wz, wx * gy * wz, wx * wy * gz};
+      const float d[3] = {X.dx[ix], Y.dx[iy], dz};
+      const float4 f4 = load(ix, iy, iz);
+      const float f[3] = {f4.x, f4.y, f4.z};
+#pragma unroll
+      for (int r = 0; r < 3; r++) {
+        S[kW][r] = fmaf(w, f[r], S[kW][r]);
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+          S[kWD + a][r] = fmaf(w * d[a], f[r], S[kWD + a][r]);
+          S[kG + a][r] = fmaf(dw[a], f[r], S[kG + a][r]);
+        }
+      }
+#pragma unroll
+      for (int b = 0; b < 3; b++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) sink(gd_idx(b, c), dw[b] * d[c], f);
+      sink(h_idx(0, 0), X.hh[ix] * wy * wz, f);
+      sink(h_idx(1, 1), wx * Y.hh[iy] * wz, f);
+      sink(h_idx(2, 2), wx * wy * hz, f);
+      sink(h_idx(0, 1), gx * gy * wz, f);
+      sink(h_idx(0, 2), gx * wy * gz, f);
+      sink(h_idx(1, 2), wx * gy * gz, f);
+      if (WITH_M) {
+#pragma unroll
+        for (int a = 0; a < 3; a++) Sm[a] = fmaf(dw[a], f4.w, Sm[a]);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+      zw_[i] = zw_[i + 1];
+      zg_[i] = zg_[i + 1];
+      zh_[i] = zh_[i + 1];
+      zd_[i] = zd_[i + 1];
+    }
+  }
+}
+
+// a weight of the stencil lies in (0, 1e-12): the transfers' cut removes it
+__device__ __forceinline__ bool has_tiny(const AxisF& X, const AxisF& Y, const AxisF& Z) {
+  return (min_positive3(X.w) * min_positive3(Y.w)) * min_positive3(Z.w) < 1e-12f;
 }
 
 __device__ __forceinline__ int adj_model(uint32_t m) {
@@ -140,46 +359,61 @@ __global__ __launch_bounds__(256) void k_g2p_bwd(SubParams p, const float4* __re
     const nb_material mat = mats[mid];
     const int model = adj_model(mat.model);
     int base[3];
-    AxisD ax[3];
+    AxisF ax[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
       float f = (xs[a] - p.wmin[a]) / p.h;
       f = fminf(fmaxf(f, -4.f), float(p.grid_res[a]) + 4.f);
       base[a] = int(floorf(f)) - 1;
-      axis_stencil_d(xs[a], p.wmin[a], p.h, base[a], ax[a]);
+      AxisD d;
+      axis_stencil_d(xs[a], p.wmin[a], p.h, base[a], d);
+      axis_factors(d, ax[a]);
     }
-    auto node_at = [&](int ix, int iy, int iz, float4& g) -> bool {
+    // grid velocity of node (ix, iy, iz); out-of-grid and massless nodes
+    // (mass < 1e-10) read as zero (shaders/mpm_g2p.comp:93-101)
+    auto node_v = [&](int ix, int iy, int iz) -> float4 {
       const int nx = base[0] + ix, ny = base[1] + iy, nz = base[2] + iz;
-      if (nx < 0 || nx >= p.grid_res[0] || ny < 0 || ny >= p.grid_res[1] || nz < 0 || nz >= p.grid_res[2])
-        return false;
-      g = grid_node[morton3(nx, ny, nz)];
-      return g.w >= 1e-10f;
+      const bool in = nx >= 0 && nx < p.grid_res[0] && ny >= 0 && ny < p.grid_res[1] && nz >= 0 && nz < p.grid_res[2];
+      const uint32_t cx = uint32_t(min(max(nx, 0), p.grid_res[0] - 1));
+      const uint32_t cy = uint32_t(min(max(ny, 0), p.grid_res[1] - 1));
+      const uint32_t cz = uint32_t(min(max(nz, 0), p.grid_res[2] - 1));
+      const float4 g = grid_node[morton3(cx, cy, cz)];
+      return (in && g.w >= 1e-10f) ? make_float4(g.x, g.y, g.z, 0.f) : make_float4(0.f, 0.f, 0.f, 0.f);
     };
-    // forward recompute (shaders/mpm_g2p.comp:87-118)
-    float vp[3] = {0.f, 0.f, 0.f};
-    m3 vg = m3_zero(), Bn = m3_zero();
+    // The incoming adjoint of B is needed during the gather: the position
+    // adjoint's gradient-times-offset terms are contracted with it there.
+    const bool apic = p.flip_ratio < -0.5f;
+    const float4* ga = adj_in + size_t(q) * kRec;
+    const m3 MB = apic ? rec_B(ga) : m3_zero();
+    // one gather serves the forward recompute (shaders/mpm_g2p.comp:87-118)
+    // and the position adjoint of the weights
+    float S[kNSum][3], Sm_unused[3];
+    float gxB[3] = {0.f, 0.f, 0.f};  // sum_i (v_i^T M_B d_i) grad w_i
+    float SH[6][3];                  // sum_i H_i v_i (h_idx order)
 #pragma unroll
-    for (int iz = 0; iz < 3; iz++)
+    for (int k = 0; k < 6; k++) SH[k][0] = SH[k][1] = SH[k][2] = 0.f;
+    auto sink = [&](int k, float z, const float qv[3]) {
+      if (k < kHD) {
+        const int b = sep_a(k), c = sep_b(k);
+        gxB[b] = fmaf(z, MB.m[c][0] * qv[0] + MB.m[c][1] * qv[1] + MB.m[c][2] * qv[2], gxB[b]);
+      } else {
 #pragma unroll
-      for (int iy = 0; iy < 3; iy++)
+        for (int r = 0; r < 3; r++) SH[k - kHD][r] = fmaf(z, qv[r], SH[k - kHD][r]);
+      }
+    };
+    if (!__any(has_tiny(ax[0], ax[1], ax[2])))
+      sep_gather<false>(ax[0], ax[1], ax[2], node_v, S, Sm_unused, sink);
+    else
+      exact_gather<false>(ax[0], ax[1], ax[2], node_v, S, Sm_unused, sink);
+    const float vp[3] = {S[kW][0], S[kW][1], S[kW][2]};
+    m3 vg, Bn;
 #pragma unroll
-        for (int ix = 0; ix < 3; ix++) {
-          float4 g;
-          if (!node_at(ix, iy, iz, g)) continue;
-          NodeW n;
-          node_w(ax[0], ax[1], ax[2], ix, iy, iz, n);
-          if (n.w < 1e-12f) continue;
-          const float v[3] = {g.x, g.y, g.z};
+    for (int c = 0; c < 3; c++)
 #pragma unroll
-          for (int rr = 0; rr < 3; rr++) {
-            vp[rr] += n.w * v[rr];
-#pragma unroll
-            for (int c = 0; c < 3; c++) {
-              vg.m[c][rr] += v[rr] * n.dw[c];
-              Bn.m[c][rr] += n.w * v[rr] * n.d[c];
-            }
-          }
-        }
+      for (int rr = 0; rr < 3; rr++) {
+        vg.m[c][rr] = S[kG + c][rr];
+        Bn.m[c][rr] = S[kWD + c][rr];
+      }
     const m3 F = rec_F(r);
     m3 L;
 #pragma unroll
@@ -192,7 +426,6 @@ __global__ __launch_bounds__(256) void k_g2p_bwd(SubParams p, const float4* __re
     // det F' = ss0 ss1 ss2 when recomposed from rotations, det F_tmp otherwise
     const float Jd = pf.rm.recompose ? pf.ss[0] * pf.ss[1] * pf.ss[2] : fabsf(m3_det(Ft));
     const bool jreset = Jd < 0.15f || Jd > 10.f;
-    const bool apic = p.flip_ratio < -0.5f;
     float v1[3] = {vp[0], vp[1], vp[2]};
     if (jreset) { v1[0] *= 0.1f; v1[1] *= 0.1f; v1[2] *= 0.1f; }
     float x2[3] = {xs[0] + p.dt * v1[0], xs[1] + p.dt * v1[1], xs[2] + p.dt * v1[2]};
@@ -246,10 +479,8 @@ __global__ __launch_bounds__(256) void k_g2p_bwd(SubParams p, const float4* __re
     }
 
     // ---- backward ----
-    const float4* ga = adj_in + size_t(q) * kRec;
     const float4 gP = ga[R_P], gV = ga[R_V];
     float gx2[3] = {gP.x, gP.y, gP.z}, gv2[3] = {gV.x, gV.y, gV.z};
-    m3 gB2 = rec_B(ga);
     m3 gT = adj_tau_full(ga);
     m3 gF2 = rec_F(ga);
 #pragma unroll
@@ -283,7 +514,6 @@ __global__ __launch_bounds__(256) void k_g2p_bwd(SubParams p, const float4* __re
 #pragma unroll
       for (int a = 0; a < 3; a++) A[a] = gv2[a];
     }
-    const m3 MB = apic ? gB2 : m3_zero();
     m3 gFt = m3_zero();
     if (!jreset && !pushed) gFt = plastic_hencky_adjoint(pf, mat.mu, mat.lambda, gF2, gT, g4);
     // F_tmp = L F,  L = I + dt grad v
@@ -293,32 +523,20 @@ __global__ __launch_bounds__(256) void k_g2p_bwd(SubParams p, const float4* __re
 #pragma unroll
       for (int rr = 0; rr < 3; rr++) MG.m[c][rr] *= p.dt;
     const m3 gF = m3_tmul(L, gFt);
-    // position through G2P's weights
+    // position through G2P's weights: with v_i the node velocities,
+    //   sum_i (v_i.A + v_i^T M_B d_i) grad w_i - w_i M_B^T v_i + H_i M_G^T v_i
 #pragma unroll
-    for (int iz = 0; iz < 3; iz++)
+    for (int b = 0; b < 3; b++) {
+      float acc = gxB[b];
 #pragma unroll
-      for (int iy = 0; iy < 3; iy++)
+      for (int a = 0; a < 3; a++) {
+        acc = fmaf(A[a], S[kG + b][a], acc);
+        acc = fmaf(-MB.m[b][a], S[kW][a], acc);
 #pragma unroll
-        for (int ix = 0; ix < 3; ix++) {
-          float4 g;
-          if (!node_at(ix, iy, iz, g)) continue;
-          NodeW n;
-          node_w(ax[0], ax[1], ax[2], ix, iy, iz, n);
-          if (n.w < 1e-12f) continue;
-          const float v[3] = {g.x, g.y, g.z};
-          const float vA = v[0] * A[0] + v[1] * A[1] + v[2] * A[2];
-          float vMBd = 0.f, MBtv[3], MGtv[3];
-#pragma unroll
-          for (int c = 0; c < 3; c++) {
-            MBtv[c] = v[0] * MB.m[c][0] + v[1] * MB.m[c][1] + v[2] * MB.m[c][2];
-            MGtv[c] = v[0] * MG.m[c][0] + v[1] * MG.m[c][1] + v[2] * MG.m[c][2];
-            vMBd += MBtv[c] * n.d[c];
-          }
-#pragma unroll
-          for (int b = 0; b < 3; b++)
-            gx[b] += (vA + vMBd) * n.dw[b] - n.w * MBtv[b] + MGtv[0] * n.H[0][b] + MGtv[1] * n.H[1][b] +
-                     MGtv[2] * n.H[2][b];
-        }
+        for (int c = 0; c < 3; c++) acc = fmaf(MG.m[c][a], SH[h_idx(c, b) - kHD][a], acc);
+      }
+      gx[b] += acc;
+    }
     float4* go = adj_out + size_t(q) * kRec;
     go[R_P] = make_float4(gx[0], gx[1], gx[2], 0.f);
     go[R_T] = make_float4(0.f, 0.f, 0.f, gF.m[0][0]);
@@ -472,7 +690,7 @@ __global__ __launch_bounds__(kAdjThreads) void k_adj_scatter(SubParams p, const 
 // ---------------------------------------------------------------------------
 // grid update backward
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(512) void k_grid_bwd(SubParams p, const float4* __restrict__ partial,
+__global__ __launch_bounds__(512) void k_grid_bwd(SubParams p, const float4* __restrict__ grid_pre,
                                                   const float4* __restrict__ partial_adj,
                                                   const int32_t* __restrict__ tile_act,
                                                   const uint32_t* __restrict__ tiles, const uint32_t* __restrict__ count,
@@ -488,7 +706,7 @@ __global__ __launch_bounds__(512) void k_grid_bwd(SubParams p, const float4* __r
     tile_coords(N, tx, ty, tz);
     const int nx = tx * 8 + lx, ny = ty * 8 + ly, nz = tz * 8 + lz;
     const size_t nid = (size_t(N) << kTileBits) | threadIdx.x;
-    float mx = 0.f, my = 0.f, mz = 0.f, mass = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
+    float gx = 0.f, gy = 0.f, gz = 0.f;
     bool covered = false;
     for (int dz = -1; dz <= 1; dz++) {
       if ((dz < 0 && lz != 0) || (dz > 0 && lz != 7)) continue;
@@ -506,8 +724,6 @@ __global__ __launch_bounds__(512) void k_grid_bwd(SubParams p, const float4* __r
           if (a < 0) continue;
           const int bx = lx - 8 * dx + 1, by = ly - 8 * dy + 1, bz = lz - 8 * dz + 1;
           const size_t e = size_t(a) * kBoxNodes + (bz * kBox + by) * kBox + bx;
-          const float4 q = partial[e];
-          mx += q.x; my += q.y; mz += q.z; mass += q.w;
           const float4 g = partial_adj[e];
           gx += g.x; gy += g.y; gz += g.z;
           covered = true;
@@ -515,12 +731,15 @@ __global__ __launch_bounds__(512) void k_grid_bwd(SubParams p, const float4* __r
       }
     }
     if (!covered) continue;
+    // the recorded forward's node state before the update: (momentum / mass, mass)
+    const float4 pre = grid_pre[nid];
+    const float mass = pre.w;
     if (mass < 1e-10f) {
       grid_adj[nid] = make_float4(0.f, 0.f, 0.f, 0.f);
       continue;
     }
     // forward recompute (shaders/mpm_grid_update.comp:17-101, mpm_nanovdb_bc.comp:23-54)
-    const float u0x = mx / mass, u0y = my / mass, u0z = mz / mass;
+    const float u0x = pre.x, u0y = pre.y, u0z = pre.z;
     const float sp = len3(u0x, u0y, u0z);
     const bool capped = sp > 15.f;
     float vx = u0x, vy = u0y, vz = u0z;
@@ -603,57 +822,64 @@ __global__ __launch_bounds__(256) void k_p2g_bwd(SubParams p, const float4* __re
   const float st = p.dt * pp.w;
   const float xs[3] = {pp.x, pp.y, pp.z}, v[3] = {vv.x, vv.y, vv.z};
   int cc[3];
-  AxisD ax[3];
+  AxisF ax[3];
 #pragma unroll
   for (int a = 0; a < 3; a++) {
     // the bin's cell (shaders/mpm_common.glsl:190-195); P2G's stencil is the cell's 27 nodes
     const float f = fminf(fmaxf((xs[a] - p.wmin[a]) / p.h, 0.f), float(p.grid_res[a]) - 1.f);
     cc[a] = int(f);
-    axis_stencil_d(xs[a], p.wmin[a], p.h, cc[a] - 1, ax[a]);
+    AxisD d;
+    axis_stencil_d(xs[a], p.wmin[a], p.h, cc[a] - 1, d);
+    axis_factors(d, ax[a]);
   }
-  const int cube = int(p.cube_res);
-  float gv[3] = {0.f, 0.f, 0.f}, gx[3] = {0.f, 0.f, 0.f};
-  m3 gC = m3_zero(), gT = m3_zero();
+  const uint32_t cube = p.cube_res;
+  // grid adjoint (dL/dp, dL/dm) of node (ix, iy, iz); nodes outside the cube read as zero
+  auto node_g = [&](int ix, int iy, int iz) -> float4 {
+    const uint32_t nx = uint32_t(cc[0] - 1 + ix), ny = uint32_t(cc[1] - 1 + iy), nz = uint32_t(cc[2] - 1 + iz);
+    const bool in = nx < cube && ny < cube && nz < cube;  // negative wraps to large
+    const float4 g = grid_adj[morton3(min(nx, cube - 1u), min(ny, cube - 1u), min(nz, cube - 1u))];
+    return in ? g : make_float4(0.f, 0.f, 0.f, 0.f);
+  };
+  // the gradient-times-offset and Hessian sums enter only the position
+  // adjoint: contracted with sc B and -dt Vp tau as they arrive
+  float S[kNSum][3], Sm[3];
+  float gxo[3] = {0.f, 0.f, 0.f};
+  auto sink = [&](int k, float z, const float qv[3]) {
+    const int a = sep_a(k), b = sep_b(k);
+    if (k < kHD) {  // (b, c) = (a, b): sc sum_r B[c][r] S[gd(b, c)][r] into d/dx_b
+      gxo[a] = fmaf(z, sc * (B.m[b][0] * qv[0] + B.m[b][1] * qv[1] + B.m[b][2] * qv[2]), gxo[a]);
+    } else {        // -st sum_r tau[c][r] S[h(c, b)][r] into d/dx_b, both orders of (a, b)
+      gxo[b] = fmaf(z, -st * (T.m[a][0] * qv[0] + T.m[a][1] * qv[1] + T.m[a][2] * qv[2]), gxo[b]);
+      if (a != b) gxo[a] = fmaf(z, -st * (T.m[b][0] * qv[0] + T.m[b][1] * qv[1] + T.m[b][2] * qv[2]), gxo[a]);
+    }
+  };
+  if (!__any(has_tiny(ax[0], ax[1], ax[2])))
+    sep_gather<true>(ax[0], ax[1], ax[2], node_g, S, Sm, sink);
+  else
+    exact_gather<true>(ax[0], ax[1], ax[2], node_g, S, Sm, sink);  // shaders/mpm_p2g.comp:61
+  // P2G adds  w (m v + C d) - dt Vp tau grad w  to the node momentum and  w m  to its mass
+  float gv[3], gx[3];
+  m3 gC, gT;
 #pragma unroll
-  for (int iz = 0; iz < 3; iz++)
+  for (int rr = 0; rr < 3; rr++) {
+    gv[rr] = mp * S[kW][rr];
 #pragma unroll
-    for (int iy = 0; iy < 3; iy++)
+    for (int c = 0; c < 3; c++) {
+      gC.m[c][rr] = S[kWD + c][rr];
+      gT.m[c][rr] = -st * S[kG + c][rr];
+    }
+  }
+  //   d/dx_b = m dL/dm . grad w + (dL/dp . (m v + C d)) grad w - w C^T dL/dp - dt Vp H tau dL/dp
 #pragma unroll
-      for (int ix = 0; ix < 3; ix++) {
-        const int nx = cc[0] - 1 + ix, ny = cc[1] - 1 + iy, nz = cc[2] - 1 + iz;
-        if (nx < 0 || ny < 0 || nz < 0 || nx >= cube || ny >= cube || nz >= cube) continue;
-        NodeW n;
-        node_w(ax[0], ax[1], ax[2], ix, iy, iz, n);
-        if (n.w < 1e-12f) continue;  // shaders/mpm_p2g.comp:61
-        const float4 g4 = grid_adj[morton3(nx, ny, nz)];
-        const float gp[3] = {g4.x, g4.y, g4.z};
-        const float gm = g4.w;
-        float mom[3];  // m v + C d
+  for (int b = 0; b < 3; b++) {
+    float acc = fmaf(mp, Sm[b], gxo[b]);
 #pragma unroll
-        for (int rr = 0; rr < 3; rr++)
-          mom[rr] = mp * v[rr] + sc * (B.m[0][rr] * n.d[0] + B.m[1][rr] * n.d[1] + B.m[2][rr] * n.d[2]);
-        const float gpm = gp[0] * mom[0] + gp[1] * mom[1] + gp[2] * mom[2];
-        float Ctg[3], TgH[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-        for (int c = 0; c < 3; c++) Ctg[c] = sc * (B.m[c][0] * gp[0] + B.m[c][1] * gp[1] + B.m[c][2] * gp[2]);
-        // (tau^T gp)_c
-        float Ttg[3];
-#pragma unroll
-        for (int c = 0; c < 3; c++) Ttg[c] = T.m[c][0] * gp[0] + T.m[c][1] * gp[1] + T.m[c][2] * gp[2];
-#pragma unroll
-        for (int b = 0; b < 3; b++) TgH[b] = Ttg[0] * n.H[0][b] + Ttg[1] * n.H[1][b] + Ttg[2] * n.H[2][b];
-#pragma unroll
-        for (int rr = 0; rr < 3; rr++) {
-          gv[rr] += mp * n.w * gp[rr];
-#pragma unroll
-          for (int c = 0; c < 3; c++) {
-            gC.m[c][rr] += n.w * gp[rr] * n.d[c];
-            gT.m[c][rr] += -st * gp[rr] * n.dw[c];
-          }
-        }
-#pragma unroll
-        for (int b = 0; b < 3; b++) gx[b] += (mp * gm + gpm) * n.dw[b] - n.w * Ctg[b] - st * TgH[b];
-      }
+    for (int rr = 0; rr < 3; rr++) {
+      acc = fmaf(mp * v[rr], S[kG + b][rr], acc);
+      acc = fmaf(-sc * B.m[b][rr], S[kW][rr], acc);
+    }
+    gx[b] = acc;
+  }
   float4* go = adj_out + size_t(q) * kRec;
   const float4 g0 = go[R_P];
   const float4 gt = go[R_T];
@@ -702,7 +928,7 @@ void launch_adjoint_substep(const DevState& d, const AdjState& a, const SubParam
                                   d.sdf);
   k_adj_scatter<<<tile_grid_a(n_tiles), kAdjThreads, 0, s>>>(p, d.rec, a.coef, d.cell_count, d.cell_end, d.sorted,
                                                              d.act_tiles, d.counters, a.partial_adj);
-  k_grid_bwd<<<tile_grid_a(n_tiles), 512, 0, s>>>(p, d.partial, a.partial_adj, d.tile_act, d.node_tiles,
+  k_grid_bwd<<<tile_grid_a(n_tiles), 512, 0, s>>>(p, d.grid_pre, a.partial_adj, d.tile_act, d.node_tiles,
                                                    d.counters + CNT_NODE_TILES, a.grid_adj, d.sdf, d.forces);
   k_p2g_bwd<<<nblk, 256, 0, s>>>(p, d.rec, adj_out, d.materials, d.sorted, a.grid_adj);
   const uint32_t n_chunks = cdiv_a(nblk, kPgChunk);

@@ -87,6 +87,9 @@ struct nb_mpm {
     float4* tape = nullptr;   // (seg - 1) * cap * kRec
     uint32_t* ckpt_keys = nullptr;  // bin keys of the same states: n_ckpt * cap
     uint32_t* tape_keys = nullptr;  // (seg - 1) * cap
+    // grids of substep j of the tape's segment: node velocities after the
+    // update at grid_tape[2j], (momentum / mass, mass) before it at [2j + 1]
+    float4* grid_tape = nullptr;    // seg * 2 * NT * 512
     int tape_seg = -1;        // the segment whose records the tape holds
     float4* adj_a = nullptr;  // cap * kRec
     float4* adj_b = nullptr;
@@ -194,22 +197,40 @@ int sync_out(nb_mpm* h) {
 
 // one substep on the records (the caller brackets a run of them with
 // sync_in / sync_out)
-// One substep.  rec_in / rec_out and keys_in / keys_out (default: the
-// engine's records and bin keys, in place) let the recorded run move the
-// state along its tape without copies.
-int enqueue_substep(nb_mpm* h, float dt, Prof* pf, float4* rec_in = nullptr, float4* rec_out = nullptr,
-                    uint32_t* keys_in = nullptr, uint32_t* keys_out = nullptr) {
+// Where a recorded substep reads and writes: records and bin keys before
+// and after, and the grids it keeps.  Null members mean the engine's own
+// buffers (records and keys updated in place); with g2p = false the substep
+// stops after the grid update.
+struct TapeIO {
+  float4* rec_in = nullptr;
+  float4* rec_out = nullptr;
+  uint32_t* keys_in = nullptr;
+  uint32_t* keys_out = nullptr;
+  float4* grid_node = nullptr;
+  float4* grid_pre = nullptr;
+  bool g2p = true;
+};
+
+int enqueue_substep(nb_mpm* h, float dt, Prof* pf, const TapeIO& io = TapeIO{}) {
   if (h->n_live == 0) return NB_OK;
   const bool flip = h->knobs.flip_ratio > 0.01f;
   SubParams p = h->params(dt, flip);
   DevState dv = h->dev();
-  if (rec_in) dv.rec = rec_in;
-  if (keys_in) dv.keys = keys_in;
+  if (io.rec_in) dv.rec = io.rec_in;
+  if (io.keys_in) dv.keys = io.keys_in;
+  if (io.grid_node) dv.grid_node = io.grid_node;
+  dv.grid_pre = io.grid_pre;
   launch_bin(dv, p, h->NT, h->keys_valid, h->stream, pf);
   launch_p2g(dv, p, h->NT, h->stream, pf);
   launch_grid(dv, p, h->NT, true, true, h->d.node_tiles, h->d.counters + CNT_NODE_TILES, h->stream, pf);
-  dv.keys = keys_out ? keys_out : h->d.keys;  // G2P writes the next substep's keys
-  launch_g2p(dv, p, true, h->stream, pf, rec_out);
+  if (!io.g2p) {
+    NB_HIP(hipGetLastError());
+    h->bins_valid = true;
+    h->keys_valid = true;
+    return NB_OK;
+  }
+  dv.keys = io.keys_out ? io.keys_out : h->d.keys;  // G2P writes the next substep's keys
+  launch_g2p(dv, p, true, h->stream, pf, io.rec_out);
   NB_HIP(hipGetLastError());
   h->bins_valid = true;
   h->keys_valid = true;
@@ -852,6 +873,8 @@ int nb_mpm_active_tile_count(nb_mpm* h, uint32_t* tiles) {
   return NB_OK;
 }
 
+}  // extern "C"
+
 // ---- reverse mode ----
 namespace {
 FieldPtrs adj_fields(nb_mpm* h) {
@@ -894,7 +917,24 @@ uint32_t* key_slot(nb_mpm* h, uint32_t s) {
   return j == 0 ? A.ckpt_keys + size_t(sg) * h->cap : A.tape_keys + size_t(j - 1) * h->cap;
 }
 uint32_t* key_next(nb_mpm* h, uint32_t s) { return s + 1 == h->adj.n ? h->d.keys : key_slot(h, s + 1); }
+// the grids kept for substep s
+float4* grid_slot(nb_mpm* h, uint32_t s, int which) {
+  return h->adj.grid_tape + (size_t(s % h->adj.seg) * 2 + which) * h->NT * kTileCells;
+}
+// substep s of the recorded run: slot to slot, keeping its grids
+TapeIO tape_io(nb_mpm* h, uint32_t s, float4* rec_out, uint32_t* keys_out) {
+  TapeIO io;
+  io.rec_in = adj_slot(h, s);
+  io.rec_out = rec_out;
+  io.keys_in = key_slot(h, s);
+  io.keys_out = keys_out;
+  io.grid_node = grid_slot(h, s, 0);
+  io.grid_pre = grid_slot(h, s, 1);
+  return io;
+}
 }  // namespace
+
+extern "C" {
 
 int nb_mpm_adjoint_begin(nb_mpm* h, uint32_t n_substeps, uint32_t segment) {
   if (!h) return fail(NB_ERR_ARG, "null handle");
@@ -923,6 +963,7 @@ int nb_mpm_adjoint_begin(nb_mpm* h, uint32_t n_substeps, uint32_t segment) {
   float4* kt = nullptr;
   if (rc == NB_OK) rc = adj_alloc(h, &kc, (size_t(h->cap) * A.n_ckpt + 3) / 4);
   if (rc == NB_OK) rc = adj_alloc(h, &kt, (size_t(h->cap) * (A.seg - 1) + 3) / 4);
+  if (rc == NB_OK) rc = adj_alloc(h, &A.grid_tape, size_t(A.seg) * 2 * h->NT * kTileCells);
   if (rc == NB_OK) rc = adj_alloc(h, &A.adj_a, rec);
   if (rc == NB_OK) rc = adj_alloc(h, &A.adj_b, rec);
   for (int f = 0; f < NB_NUM_FIELDS && rc == NB_OK; f++) rc = adj_alloc(h, &A.field[f], h->cap);
@@ -963,7 +1004,8 @@ int nb_mpm_forward_recorded(nb_mpm* h, float dt_sub) {
   if (rc == NB_OK) rc = copy_rec(h, A.ckpt, h->d.rec);
   h->keys_valid = false;  // the first substep writes its keys into the first key slot
   for (uint32_t s = 0; s < A.n && rc == NB_OK; s++)
-    rc = enqueue_substep(h, dt_sub, nullptr, adj_slot(h, s), adj_next(h, s), key_slot(h, s), key_next(h, s));
+    rc = enqueue_substep(h, dt_sub, nullptr, tape_io(h, s, adj_next(h, s), key_next(h, s)));
+  h->grid_mode = GridMode::NONE;  // the grids went to the tape
   if (rc == NB_OK) rc = sync_out(h);
   if (rc != NB_OK) return rc;
   NB_HIP(hipStreamSynchronize(h->stream));
@@ -1010,21 +1052,26 @@ int nb_mpm_backward(nb_mpm* h, float* d_params, uint32_t params_cap) {
     const uint32_t s0 = uint32_t(sg) * A.seg;
     const uint32_t len = std::min(A.seg, A.n - s0);
     if (A.tape_seg != sg) {
-      // recompute the segment's tape from its checkpoint
+      // recompute the segment's records and grids from its checkpoint (its
+      // last substep only up to the grid update: its result is the next checkpoint)
       h->keys_valid = true;
-      for (uint32_t j = 0; j + 1 < len && rc == NB_OK; j++)
-        rc = enqueue_substep(h, A.dt, nullptr, adj_slot(h, s0 + j), adj_slot(h, s0 + j + 1), key_slot(h, s0 + j),
-                             key_slot(h, s0 + j + 1));
+      for (uint32_t j = 0; j < len && rc == NB_OK; j++) {
+        const bool last = j + 1 == len;
+        TapeIO io = tape_io(h, s0 + j, last ? nullptr : adj_slot(h, s0 + j + 1), last ? nullptr : key_slot(h, s0 + j + 1));
+        io.g2p = !last;
+        rc = enqueue_substep(h, A.dt, nullptr, io);
+      }
       A.tape_seg = sg;
     }
     for (int j = int(len) - 1; j >= 0 && rc == NB_OK; j--) {
+      const uint32_t s = s0 + uint32_t(j);
       SubParams p = h->params(A.dt, false);
       DevState dv = h->dev();
-      dv.rec = adj_slot(h, s0 + uint32_t(j));
-      dv.keys = key_slot(h, s0 + uint32_t(j));
+      dv.rec = adj_slot(h, s);
+      dv.keys = key_slot(h, s);
+      dv.grid_node = grid_slot(h, s, 0);
+      dv.grid_pre = grid_slot(h, s, 1);
       launch_bin(dv, p, h->NT, true, h->stream, nullptr);
-      launch_p2g(dv, p, h->NT, h->stream, nullptr);
-      launch_grid(dv, p, h->NT, true, true, h->d.node_tiles, h->d.counters + CNT_NODE_TILES, h->stream, nullptr);
       launch_adjoint_substep(dv, A.a, p, h->NT, in, out, h->stream);
       NB_HIP(hipGetLastError());
       std::swap(in, out);

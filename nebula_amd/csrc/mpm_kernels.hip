@@ -614,8 +614,9 @@ template <bool FROM_PARTIALS, bool UPDATE>
 __global__ __launch_bounds__(512) void k_grid(SubParams p, const float4* __restrict__ partial,
                                               const int32_t* __restrict__ tile_act, const uint32_t* __restrict__ tiles,
                                               const uint32_t* __restrict__ count, float4* __restrict__ grid_node,
-                                              float4* __restrict__ grid_old, const float* __restrict__ sdf,
-                                              const nb_collider* __restrict__ cols, const nb_force* __restrict__ forces) {
+                                              float4* __restrict__ grid_old, float4* __restrict__ grid_pre,
+                                              const float* __restrict__ sdf, const nb_collider* __restrict__ cols,
+                                              const nb_force* __restrict__ forces) {
   const uint32_t nb_tiles = *count;
   // thread j holds the tile's j-th node in Morton order (contiguous in HBM)
   const int lx = int(morton_compact(threadIdx.x)), ly = int(morton_compact(threadIdx.x >> 1)),
@@ -660,6 +661,8 @@ __global__ __launch_bounds__(512) void k_grid(SubParams p, const float4* __restr
       const float4 g = grid_node[nid];
       mx = g.x; my = g.y; mz = g.z; mass = g.w;
     }
+    if (grid_pre)
+      grid_pre[nid] = mass < 1e-10f ? make_float4(0.f, 0.f, 0.f, mass) : make_float4(mx / mass, my / mass, mz / mass, mass);
     if (!UPDATE || mass < 1e-10f) {
       // massless nodes keep their momentum (shaders/mpm_grid_update.comp:109-110)
       grid_node[nid] = make_float4(mx, my, mz, mass);
@@ -1176,11 +1179,14 @@ void launch_grid(const DevState& d, const SubParams& p, uint32_t n_tiles, bool f
   ProfScope ps(pf, 9, s);
   const uint32_t g = tile_grid(n_tiles);
   if (from_partials && update)
-    k_grid<true, true><<<g, 512, 0, s>>>(p, d.partial, d.tile_act, tiles, count, d.grid_node, d.grid_old, d.sdf, d.colliders, d.forces);
+    k_grid<true, true><<<g, 512, 0, s>>>(p, d.partial, d.tile_act, tiles, count, d.grid_node, d.grid_old, d.grid_pre,
+                                         d.sdf, d.colliders, d.forces);
   else if (from_partials)
-    k_grid<true, false><<<g, 512, 0, s>>>(p, d.partial, d.tile_act, tiles, count, d.grid_node, d.grid_old, d.sdf, d.colliders, d.forces);
+    k_grid<true, false><<<g, 512, 0, s>>>(p, d.partial, d.tile_act, tiles, count, d.grid_node, d.grid_old, d.grid_pre,
+                                          d.sdf, d.colliders, d.forces);
   else
-    k_grid<false, true><<<g, 512, 0, s>>>(p, d.partial, d.tile_act, tiles, count, d.grid_node, d.grid_old, d.sdf, d.colliders, d.forces);
+    k_grid<false, true><<<g, 512, 0, s>>>(p, d.partial, d.tile_act, tiles, count, d.grid_node, d.grid_old, d.grid_pre,
+                                          d.sdf, d.colliders, d.forces);
 }
 
 void launch_g2p(const DevState& d, const SubParams& p, bool use_sorted, hipStream_t s, Prof* pf, float4* rec_out) {

@@ -88,6 +88,7 @@ struct DevState {
   float4* partial;        // cap_tiles * 1000
   float4* grid_node;      // NT*512
   float4* grid_old;       // NT*512
+  float4* grid_pre;       // NT*512 or null: (momentum / mass, mass) before the update, kept for the adjoint
   float* sdf;             // NC (null when no SDF collider)
   nb_material* materials;
   nb_collider* colliders;
