@@ -550,6 +550,12 @@ class MPMEngine:
     def setMaterials(self, mats: Sequence[MaterialParams]):
         arr = (MaterialParams * max(len(mats), 1))(*mats)
         _check(lib().nb_mpm_set_materials(self._h, C.cast(arr, C.c_void_p), len(mats)))
+        if mats:
+            self._materials = [MaterialParams.from_buffer_copy(bytes(m)) for m in mats]
+
+    def materials(self) -> List[MaterialParams]:
+        """Copies of the material table last given to setMaterials (empty if none)."""
+        return [MaterialParams.from_buffer_copy(bytes(m)) for m in getattr(self, "_materials", [])]
 
     def setParticleMaterialIds(self, ids):
         a = np.ascontiguousarray(ids, np.uint32)
@@ -625,6 +631,19 @@ class MPMEngine:
     def getState(self):
         """(8, n, 4) float32: P, v, F0, F1, F2, B0, B1, B2."""
         return np.stack([self.getField(f) for f in range(8)])
+
+    def setStateFrom(self, ptr: int, n: int):
+        """Set the live count to n and copy the state from a host or device
+        address holding (8, n, 4) float32 (the eight arrays back to back)."""
+        _check(lib().nb_mpm_set_live_particle_count(self._h, n))
+        for i in range(8):
+            _check(lib().nb_mpm_set_field(self._h, i, C.cast(C.c_void_p(ptr + i * n * 16), C.POINTER(C.c_float)), 0, n))
+
+    def getStateTo(self, ptr: int):
+        """Copy the (8, n, 4) state to a host or device address."""
+        n = self.liveParticleCount()
+        for i in range(8):
+            _check(lib().nb_mpm_get_field(self._h, i, C.cast(C.c_void_p(ptr + i * n * 16), C.POINTER(C.c_float)), 0, n))
 
     def setState(self, fields):
         f = np.ascontiguousarray(fields, np.float32)
