@@ -16,8 +16,7 @@
 //   k_grid_bwd     per node: the grid update backwards -> dL/dp, dL/dm
 //   k_p2g_bwd      per particle: P2G backwards as a 27-node gather
 //                  -> d/dv, d/dB, d/dtau and the position part from P2G
-//   k_pgrad_chunks / k_pgrad_total  per-block material-parameter partials
-//                  summed in a fixed order
+//   k_pgrad_reduce per-block material-parameter partials summed in a fixed order
 // Positions enter every weight, so both transfers contribute the weight
 // gradients and Hessians.  Everything is deterministic.
 #include <hip/hip_runtime.h>
@@ -557,7 +556,8 @@ __global__ __launch_bounds__(256) void k_g2p_bwd(SubParams p, const float4* __re
     for (int j = 0; j < kCoef; j++) co[j] = make_float4(cf[4 * j], cf[4 * j + 1], cf[4 * j + 2], cf[4 * j + 3]);
   }
   // material-parameter partials: slot (material, parameter), 64 per wave,
-  // summed over the block's waves in a fixed order, one row per block
+  // summed over the block's waves in a fixed order; stored slot-major
+  // (pgrad_w[slot * blocks + block]) so that each slot's sum reads contiguously
   static_assert(NB_MAX_MATERIALS * 4 == 64, "one slot per lane");
   __shared__ float s_pg[4][64];
   const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
@@ -583,7 +583,7 @@ __global__ __launch_bounds__(256) void k_g2p_bwd(SubParams p, const float4* __re
   }
   __syncthreads();
   if (threadIdx.x < 64)
-    pgrad_w[size_t(blockIdx.x) * 64 + lane] = ((s_pg[0][lane] + s_pg[1][lane]) + s_pg[2][lane]) + s_pg[3][lane];
+    pgrad_w[size_t(lane) * gridDim.x + blockIdx.x] = ((s_pg[0][lane] + s_pg[1][lane]) + s_pg[2][lane]) + s_pg[3][lane];
 }
 
 // ---------------------------------------------------------------------------
@@ -892,27 +892,23 @@ __global__ __launch_bounds__(256) void k_p2g_bwd(SubParams p, const float4* __re
   go[R_T] = make_float4(gT.m[0][0], gT.m[1][1], gT.m[2][2], gt.w);
 }
 
-// Fixed-order sum of the per-block parameter rows (64 floats each): every
-// block of 256 threads sums kPgChunk consecutive rows, lane = slot, the four
-// waves taking interleaved rows; then one block adds the chunk sums, in
-// order, to the running total.
-constexpr uint32_t kPgChunk = 256;
-__global__ __launch_bounds__(256) void k_pgrad_chunks(const float* __restrict__ rows, uint32_t n_rows,
-                                                      float* __restrict__ chunks) {
-  __shared__ float s[4][64];
-  const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
-  const uint32_t r0 = blockIdx.x * kPgChunk, r1 = min(r0 + kPgChunk, n_rows);
+// Fixed-order sum of the per-block parameter partials, one block per slot:
+// thread t sums the partials t, t + 256, ... (coalesced across the block),
+// then a fixed LDS tree adds the 256 sums to the running total.
+__global__ __launch_bounds__(256) void k_pgrad_reduce(const float* __restrict__ part, uint32_t n_part,
+                                                      float* __restrict__ pgrad) {
+  __shared__ float s[256];
+  const float* p = part + size_t(blockIdx.x) * n_part;
   float acc = 0.f;
-  for (uint32_t r = r0 + wv; r < r1; r += 4) acc += rows[size_t(r) * 64 + lane];
-  s[wv][lane] = acc;
+#pragma unroll 8
+  for (uint32_t i = threadIdx.x; i < n_part; i += 256) acc += p[i];
+  s[threadIdx.x] = acc;
   __syncthreads();
-  if (threadIdx.x < 64) chunks[size_t(blockIdx.x) * 64 + lane] = ((s[0][lane] + s[1][lane]) + s[2][lane]) + s[3][lane];
-}
-__global__ __launch_bounds__(64) void k_pgrad_total(const float* __restrict__ chunks, uint32_t n_chunks,
-                                                    float* __restrict__ pgrad) {
-  float acc = 0.f;
-  for (uint32_t c = 0; c < n_chunks; c++) acc += chunks[size_t(c) * 64 + threadIdx.x];
-  pgrad[threadIdx.x] += acc;
+  for (int o = 128; o > 0; o >>= 1) {
+    if (int(threadIdx.x) < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) pgrad[blockIdx.x] += s[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -931,10 +927,7 @@ void launch_adjoint_substep(const DevState& d, const AdjState& a, const SubParam
   k_grid_bwd<<<tile_grid_a(n_tiles), 512, 0, s>>>(p, d.grid_pre, a.partial_adj, d.tile_act, d.node_tiles,
                                                    d.counters + CNT_NODE_TILES, a.grid_adj, d.sdf, d.forces);
   k_p2g_bwd<<<nblk, 256, 0, s>>>(p, d.rec, adj_out, d.materials, d.sorted, a.grid_adj);
-  const uint32_t n_chunks = cdiv_a(nblk, kPgChunk);
-  float* chunks = a.pgrad_w + size_t(nblk) * 64;
-  k_pgrad_chunks<<<n_chunks, 256, 0, s>>>(a.pgrad_w, nblk, chunks);
-  k_pgrad_total<<<1, 64, 0, s>>>(chunks, n_chunks, a.pgrad);
+  k_pgrad_reduce<<<NB_MAX_MATERIALS * 4, 256, 0, s>>>(a.pgrad_w, nblk, a.pgrad);
 }
 
 }  // namespace nb

@@ -955,7 +955,6 @@ int nb_mpm_adjoint_begin(nb_mpm* h, uint32_t n_substeps, uint32_t segment) {
   A.n_ckpt = (n_substeps + A.seg - 1) / A.seg;
   const size_t rec = size_t(h->cap) * kRec;
   const uint32_t n_blocks = (h->cap + 255u) / 256u;  // blocks of the G2P backward
-  const uint32_t n_chunks = (n_blocks + 255u) / 256u;
   int rc = NB_OK;
   if (rc == NB_OK) rc = adj_alloc(h, &A.ckpt, rec * A.n_ckpt);
   if (rc == NB_OK) rc = adj_alloc(h, &A.tape, rec * (A.seg - 1));
@@ -972,7 +971,7 @@ int nb_mpm_adjoint_begin(nb_mpm* h, uint32_t n_substeps, uint32_t segment) {
   if (rc == NB_OK) rc = adj_alloc(h, &A.a.grid_adj, size_t(h->NT) * kTileCells);
   float4* pw = nullptr;
   float4* pg = nullptr;
-  if (rc == NB_OK) rc = adj_alloc(h, &pw, size_t(n_blocks + n_chunks) * NB_MAX_MATERIALS);
+  if (rc == NB_OK) rc = adj_alloc(h, &pw, size_t(n_blocks) * NB_MAX_MATERIALS);
   if (rc == NB_OK) rc = adj_alloc(h, &pg, NB_MAX_MATERIALS);
   if (rc != NB_OK) {
     std::string keep = g_err;

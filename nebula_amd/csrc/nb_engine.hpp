@@ -141,7 +141,7 @@ struct AdjState {
   float4* coef;         // capacity * 6: per-particle grid-adjoint coefficients
   float4* partial_adj;  // cap_tiles * kBoxNodes: per-tile grid-adjoint boxes
   float4* grid_adj;     // NT * 512: (dL/dp, dL/dm) per node
-  float* pgrad_w;       // per block of the G2P backward, then per 256 blocks: NB_MAX_MATERIALS x 4 partials
+  float* pgrad_w;       // NB_MAX_MATERIALS x 4 partials per block of the G2P backward, slot-major
   float* pgrad;         // NB_MAX_MATERIALS x 4: running d/d(mu, lambda, M, q_c)
 };
 // One substep backwards on the recomputed forward of state d.rec (bins, P2G
