@@ -170,6 +170,38 @@ int nb_mpm_create(const nb_mpm_config* cfg, void* hip_stream, nb_mpm** out);
 /* MPMEngine::cleanup (src/engine/MPMEngine.cpp:160-163). */
 int nb_mpm_destroy(nb_mpm* h);
 
+/* ---- Batches of scenes ----
+ * No counterpart in the reference, whose MPMEngine holds one scene.  For
+ * callers that advance many independent scenes (parameter sweeps, training
+ * data, differentiable fits): one handle holds them all and every substep
+ * advances every scene in the same eleven launches.
+ *
+ * Each scene has its own domain [0, domain_size] and cell size (its grid is
+ * ceil(domain_size / cell_size) cells, Domain.h's rule) and owns
+ * particle_count consecutive particle rows, scenes in table order.  Particle
+ * positions are in the scene's own coordinates.  All the particle, material,
+ * force, knob, step and reverse-mode entry points work on a batch over the
+ * concatenated rows; material ids index one shared table of up to
+ * NB_MAX_BATCH_MATERIALS materials; forces, knobs and analytic colliders apply
+ * to every scene (colliders in each scene's coordinates).  Scenes never
+ * interact.  A batch starts with every particle at the origin, at rest,
+ * F = I and material 0: set the state with nb_mpm_set_field.  Not available
+ * on a batch (NB_ERR_UNSUPPORTED): the dense SDF, the pass-level entry points
+ * and their exports, and changing the particle count. */
+#define NB_MAX_BATCH_MATERIALS 1024
+typedef struct nb_mpm_scene {
+  float domain_size[3];
+  float cell_size;
+  uint32_t particle_count;
+  uint32_t reserved[3];
+} nb_mpm_scene;
+int nb_mpm_create_batch(const nb_mpm_scene* scenes, uint32_t n_scenes, void* hip_stream, nb_mpm** out);
+/* Scenes of a handle (0 for a plain engine). */
+int nb_mpm_scene_count(const nb_mpm* h, uint32_t* n);
+/* Grid resolution, first particle row and particle count of one scene. */
+int nb_mpm_scene_info(const nb_mpm* h, uint32_t scene, uint32_t res[3], uint32_t* first_particle,
+                      uint32_t* particle_count);
+
 /* MPMEngine::step (src/engine/MPMEngine.cpp:404-474): forces, then
  * knobs.num_substeps substeps of dt/num_substeps.  Asynchronous on the stream. */
 int nb_mpm_step(nb_mpm* h, float dt);
@@ -191,7 +223,7 @@ int nb_mpm_total_cells(const nb_mpm* h, uint32_t* total_cells);
 int nb_mpm_live_particle_count(const nb_mpm* h, uint32_t* n);
 
 /* MPMEngine::setMaterials (src/engine/MPMEngine.cpp:239-243): n == 0 is a no-op,
- * n > NB_MAX_MATERIALS is NB_ERR_RANGE. */
+ * n > NB_MAX_MATERIALS is NB_ERR_RANGE (NB_MAX_BATCH_MATERIALS on a batch). */
 int nb_mpm_set_materials(nb_mpm* h, const nb_material* mats, uint32_t n);
 /* MPMEngine::setParticleMaterialIds (src/engine/MPMEngine.cpp:245-256): overwrites
  * v[0..n) with (0, 0, 0, bits(id)). */

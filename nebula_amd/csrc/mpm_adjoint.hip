@@ -121,8 +121,9 @@ __device__ __forceinline__ void axis_factors(const AxisD& a, AxisF& f) {
 // Axis-by-axis contraction: x (5 factor kinds per row), then y (the 12
 // (x, y) pairs the products use), then z.  load(ix, iy, iz) returns the
 // node's field in xyz and, when WITH_M, a scalar field in w whose sums with
-// dw/dx_a go to Sm.  No weight cut: the caller takes this path only when no
-// node weight lies in (0, 1e-12).  The z loop stays rolled so that one slab
+// dw/dx_a go to Sm.  No weight cut: the sums stand only when no node weight
+// lies in (0, 1e-12); the caller redoes such a particle with exact_gather.
+// The z loop stays rolled so that one slab
 // of 9 node loads is live at a time; its z factors sit in position 0 and are
 // rotated at the end of each iteration.
 template <bool WITH_M, class Load, class Sink>
@@ -228,7 +229,7 @@ __device__ __forceinline__ void sep_gather(const AxisF& X, const AxisF& Y, const
 }
 
 // The same sums node by node with the transfers' weight cut (nodes with
-// w < 1e-12 are skipped), for waves where a weight lies in (0, 1e-12).
+// w < 1e-12 are skipped), for particles with a weight in (0, 1e-12).
 template <bool WITH_M, class Load, class Sink>
 __device__ __forceinline__ void exact_gather(const AxisF& X, const AxisF& Y, const AxisF& Z, Load load,
                                              float S[kNSum][3], float Sm[3], Sink sink) {
@@ -338,9 +339,11 @@ constexpr int kCoef = 6;  // float4 per particle: A (3), M_B (9), M_G (9)
 // ---------------------------------------------------------------------------
 // G2P backward
 // ---------------------------------------------------------------------------
+template <bool BATCH>
 __global__ __launch_bounds__(256) void k_g2p_bwd(SubParams p, const float4* __restrict__ rec,
                                                  const float4* __restrict__ adj_in, float4* __restrict__ adj_out,
-                                                 float4* __restrict__ coef, float* __restrict__ pgrad_w,
+                                                 float4* __restrict__ coef, uint32_t* __restrict__ pg_mid,
+                                                 float4* __restrict__ pg_val, uint32_t* __restrict__ pg_cnt,
                                                  const nb_material* __restrict__ mats,
                                                  const uint32_t* __restrict__ sorted,
                                                  const float4* __restrict__ grid_node, const float* __restrict__ sdf) {
@@ -348,6 +351,7 @@ __global__ __launch_bounds__(256) void k_g2p_bwd(SubParams p, const float4* __re
   const bool valid = k < p.n;
   const uint32_t q = valid ? sorted[k] : 0u;
   const float4* r = rec + size_t(q) * kRec;
+  const SceneGeo sg = particle_geo<BATCH>(p, q);
   float g4[4] = {0.f, 0.f, 0.f, 0.f};
   uint32_t mid = 0;
   if (valid) {
@@ -361,22 +365,22 @@ __global__ __launch_bounds__(256) void k_g2p_bwd(SubParams p, const float4* __re
     AxisF ax[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-      float f = (xs[a] - p.wmin[a]) / p.h;
-      f = fminf(fmaxf(f, -4.f), float(p.grid_res[a]) + 4.f);
+      float f = (xs[a] - sg.wmin[a]) / sg.h;
+      f = fminf(fmaxf(f, -4.f), float(sg.grid_res[a]) + 4.f);
       base[a] = int(floorf(f)) - 1;
       AxisD d;
-      axis_stencil_d(xs[a], p.wmin[a], p.h, base[a], d);
+      axis_stencil_d(xs[a], sg.wmin[a], sg.h, base[a], d);
       axis_factors(d, ax[a]);
     }
     // grid velocity of node (ix, iy, iz); out-of-grid and massless nodes
     // (mass < 1e-10) read as zero (shaders/mpm_g2p.comp:93-101)
     auto node_v = [&](int ix, int iy, int iz) -> float4 {
       const int nx = base[0] + ix, ny = base[1] + iy, nz = base[2] + iz;
-      const bool in = nx >= 0 && nx < p.grid_res[0] && ny >= 0 && ny < p.grid_res[1] && nz >= 0 && nz < p.grid_res[2];
-      const uint32_t cx = uint32_t(min(max(nx, 0), p.grid_res[0] - 1));
-      const uint32_t cy = uint32_t(min(max(ny, 0), p.grid_res[1] - 1));
-      const uint32_t cz = uint32_t(min(max(nz, 0), p.grid_res[2] - 1));
-      const float4 g = grid_node[morton3(cx, cy, cz)];
+      const bool in = nx >= 0 && nx < sg.grid_res[0] && ny >= 0 && ny < sg.grid_res[1] && nz >= 0 && nz < sg.grid_res[2];
+      const uint32_t cx = uint32_t(min(max(nx, 0), sg.grid_res[0] - 1));
+      const uint32_t cy = uint32_t(min(max(ny, 0), sg.grid_res[1] - 1));
+      const uint32_t cz = uint32_t(min(max(nz, 0), sg.grid_res[2] - 1));
+      const float4 g = grid_node[node_id(sg, cx, cy, cz)];
       return (in && g.w >= 1e-10f) ? make_float4(g.x, g.y, g.z, 0.f) : make_float4(0.f, 0.f, 0.f, 0.f);
     };
     // The incoming adjoint of B is needed during the gather: the position
@@ -400,10 +404,15 @@ __global__ __launch_bounds__(256) void k_g2p_bwd(SubParams p, const float4* __re
         for (int r = 0; r < 3; r++) SH[k - kHD][r] = fmaf(z, qv[r], SH[k - kHD][r]);
       }
     };
-    if (!__any(has_tiny(ax[0], ax[1], ax[2])))
-      sep_gather<false>(ax[0], ax[1], ax[2], node_v, S, Sm_unused, sink);
-    else
+    // every lane sums axis by axis; a lane with a weight under the cut redoes
+    // its sums node by node (a particle's result depends on its own data only)
+    sep_gather<false>(ax[0], ax[1], ax[2], node_v, S, Sm_unused, sink);
+    if (has_tiny(ax[0], ax[1], ax[2])) {
+      gxB[0] = gxB[1] = gxB[2] = 0.f;
+#pragma unroll
+      for (int k = 0; k < 6; k++) SH[k][0] = SH[k][1] = SH[k][2] = 0.f;
       exact_gather<false>(ax[0], ax[1], ax[2], node_v, S, Sm_unused, sink);
+    }
     const float vp[3] = {S[kW][0], S[kW][1], S[kW][2]};
     m3 vg, Bn;
 #pragma unroll
@@ -432,11 +441,11 @@ __global__ __launch_bounds__(256) void k_g2p_bwd(SubParams p, const float4* __re
     bool pushed = false, push_proj = false, push_stuck = false;
     float u[3] = {0.f, 0.f, 0.f};
     if (sdf) {
-      const int gmx[3] = {p.grid_res[0] - 1, p.grid_res[1] - 1, p.grid_res[2] - 1};
+      const int gmx[3] = {sg.grid_res[0] - 1, sg.grid_res[1] - 1, sg.grid_res[2] - 1};
       int ic[3];
 #pragma unroll
       for (int a = 0; a < 3; a++) {
-        const float f = fminf(fmaxf((x2[a] - p.wmin[a]) / p.h, -4.f), float(p.grid_res[a]) + 4.f);
+        const float f = fminf(fmaxf((x2[a] - sg.wmin[a]) / sg.h, -4.f), float(sg.grid_res[a]) + 4.f);
         ic[a] = min(max(int(floorf(f)), 0), gmx[a]);
       }
       const float psdf = sdf[morton3(ic[0], ic[1], ic[2])];
@@ -466,10 +475,10 @@ __global__ __launch_bounds__(256) void k_g2p_bwd(SubParams p, const float4* __re
     // clamp flags (shaders/mpm_g2p.comp:255-263)
     bool xcl[3], vcl[3];
     {
-      const float mg = p.h * 1.5f;
+      const float mg = sg.h * 1.5f;
 #pragma unroll
       for (int a = 0; a < 3; a++) {
-        const float lo = p.wmin[a] + mg, hi = p.wmax[a] - mg;
+        const float lo = sg.wmin[a] + mg, hi = sg.wmax[a] - mg;
         xcl[a] = false;
         vcl[a] = false;
         if (x2[a] < lo) { x2[a] = lo; xcl[a] = true; if (v2[a] < 0.f) { vcl[a] = true; v2[a] = 0.f; } }
@@ -555,13 +564,14 @@ __global__ __launch_bounds__(256) void k_g2p_bwd(SubParams p, const float4* __re
 #pragma unroll
     for (int j = 0; j < kCoef; j++) co[j] = make_float4(cf[4 * j], cf[4 * j + 1], cf[4 * j + 2], cf[4 * j + 3]);
   }
-  // material-parameter partials: slot (material, parameter), 64 per wave,
-  // summed over the block's waves in a fixed order; stored slot-major
-  // (pgrad_w[slot * blocks + block]) so that each slot's sum reads contiguously
-  static_assert(NB_MAX_MATERIALS * 4 == 64, "one slot per lane");
-  __shared__ float s_pg[4][64];
+  // Material-parameter partials: each wave sums its lanes per material (the
+  // groups in order of their first lane), and the block writes the groups of
+  // its waves, wave by wave, as its entry list.
+  __shared__ uint32_t s_gmid[4][64];
+  __shared__ float4 s_gval[4][64];
+  __shared__ uint32_t s_gcnt[4];
   const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
-  s_pg[wv][lane] = 0.f;
+  uint32_t n_grp = 0;
   uint64_t pending = __ballot(valid);
   while (pending) {
     const int leader = __ffsll((unsigned long long)pending) - 1;
@@ -576,14 +586,23 @@ __global__ __launch_bounds__(256) void k_g2p_bwd(SubParams p, const float4* __re
       s4[j] = x;
     }
     if (lane == 0) {
-#pragma unroll
-      for (int j = 0; j < 4; j++) s_pg[wv][lmid * 4 + j] = s4[j];
+      s_gmid[wv][n_grp] = lmid;
+      s_gval[wv][n_grp] = make_float4(s4[0], s4[1], s4[2], s4[3]);
     }
+    n_grp++;
     pending &= ~__ballot(mine);
   }
+  if (lane == 0) s_gcnt[wv] = n_grp;
   __syncthreads();
-  if (threadIdx.x < 64)
-    pgrad_w[size_t(lane) * gridDim.x + blockIdx.x] = ((s_pg[0][lane] + s_pg[1][lane]) + s_pg[2][lane]) + s_pg[3][lane];
+  const uint32_t c0 = s_gcnt[0], c1 = s_gcnt[1], c2 = s_gcnt[2], c3 = s_gcnt[3];
+  const size_t eb = size_t(blockIdx.x) * kPgEntries;
+  for (uint32_t e = threadIdx.x; e < c0 + c1 + c2 + c3; e += blockDim.x) {
+    const uint32_t w = e < c0 ? 0u : (e < c0 + c1 ? 1u : (e < c0 + c1 + c2 ? 2u : 3u));
+    const uint32_t i = e - (w > 0 ? c0 : 0u) - (w > 1 ? c1 : 0u) - (w > 2 ? c2 : 0u);
+    pg_mid[eb + e] = s_gmid[w][i];
+    pg_val[eb + e] = s_gval[w][i];
+  }
+  if (threadIdx.x == 0) pg_cnt[blockIdx.x] = c0 + c1 + c2 + c3;
 }
 
 // ---------------------------------------------------------------------------
@@ -591,6 +610,7 @@ __global__ __launch_bounds__(256) void k_g2p_bwd(SubParams p, const float4* __re
 // ---------------------------------------------------------------------------
 constexpr int kAdjThreads = 256;
 
+template <bool BATCH>
 __global__ __launch_bounds__(kAdjThreads) void k_adj_scatter(SubParams p, const float4* __restrict__ rec,
                                                              const float4* __restrict__ coef,
                                                              const uint32_t* __restrict__ cell_count,
@@ -605,8 +625,9 @@ __global__ __launch_bounds__(kAdjThreads) void k_adj_scatter(SubParams p, const 
   const uint32_t na = counters[CNT_ACTIVE];
   for (uint32_t a = blockIdx.x; a < na; a += gridDim.x) {
     const uint32_t T = act_tiles[a];
+    const SceneGeo sg = tile_geo<BATCH>(p, T);
     int tx, ty, tz;
-    tile_coords(T, tx, ty, tz);
+    tile_coords(sg, T, tx, ty, tz);
     const size_t tbase = size_t(T) << kTileBits;
     for (int e = threadIdx.x; e < kBoxNodes; e += kAdjThreads) { s_x[e] = 0.f; s_y[e] = 0.f; s_z[e] = 0.f; }
     const uint32_t ca = 2u * threadIdx.x, cb = ca + 1u;
@@ -641,9 +662,9 @@ __global__ __launch_bounds__(kAdjThreads) void k_adj_scatter(SubParams p, const 
           cf[4 * jj] = v4.x; cf[4 * jj + 1] = v4.y; cf[4 * jj + 2] = v4.z; cf[4 * jj + 3] = v4.w;
         }
         AxisD X, Y, Z;
-        axis_stencil_d(pp.x, p.wmin[0], p.h, cx - 1, X);
-        axis_stencil_d(pp.y, p.wmin[1], p.h, cy - 1, Y);
-        axis_stencil_d(pp.z, p.wmin[2], p.h, cz - 1, Z);
+        axis_stencil_d(pp.x, sg.wmin[0], sg.h, cx - 1, X);
+        axis_stencil_d(pp.y, sg.wmin[1], sg.h, cy - 1, Y);
+        axis_stencil_d(pp.z, sg.wmin[2], sg.h, cz - 1, Z);
 #pragma unroll
         for (int kz = 0; kz < 3; kz++)
 #pragma unroll
@@ -690,6 +711,7 @@ __global__ __launch_bounds__(kAdjThreads) void k_adj_scatter(SubParams p, const 
 // ---------------------------------------------------------------------------
 // grid update backward
 // ---------------------------------------------------------------------------
+template <bool BATCH>
 __global__ __launch_bounds__(512) void k_grid_bwd(SubParams p, const float4* __restrict__ grid_pre,
                                                   const float4* __restrict__ partial_adj,
                                                   const int32_t* __restrict__ tile_act,
@@ -699,11 +721,12 @@ __global__ __launch_bounds__(512) void k_grid_bwd(SubParams p, const float4* __r
   const uint32_t nb_tiles = *count;
   const int lx = int(morton_compact(threadIdx.x)), ly = int(morton_compact(threadIdx.x >> 1)),
             lz = int(morton_compact(threadIdx.x >> 2));
-  const int td = int(p.tile_dim);
   for (uint32_t b = blockIdx.x; b < nb_tiles; b += gridDim.x) {
     const uint32_t N = tiles[b];
+    const SceneGeo sg = tile_geo<BATCH>(p, N);
+    const int td = int(sg.tile_dim);
     int tx, ty, tz;
-    tile_coords(N, tx, ty, tz);
+    tile_coords(sg, N, tx, ty, tz);
     const int nx = tx * 8 + lx, ny = ty * 8 + ly, nz = tz * 8 + lz;
     const size_t nid = (size_t(N) << kTileBits) | threadIdx.x;
     float gx = 0.f, gy = 0.f, gz = 0.f;
@@ -720,7 +743,7 @@ __global__ __launch_bounds__(512) void k_grid_bwd(SubParams p, const float4* __r
           if ((dx < 0 && lx != 0) || (dx > 0 && lx != 7)) continue;
           const int sx = tx + dx;
           if (sx < 0 || sx >= td) continue;
-          const int32_t a = tile_act[morton3(sx, sy, sz)];
+          const int32_t a = tile_act[tile_id(sg, sx, sy, sz)];
           if (a < 0) continue;
           const int bx = lx - 8 * dx + 1, by = ly - 8 * dy + 1, bz = lz - 8 * dz + 1;
           const size_t e = size_t(a) * kBoxNodes + (bz * kBox + by) * kBox + bx;
@@ -755,7 +778,7 @@ __global__ __launch_bounds__(512) void k_grid_bwd(SubParams p, const float4* __r
         vz += (f.data[2] * s) * p.dt;
       }
     }
-    const int Nx = p.grid_res[0] - 1, Ny = p.grid_res[1] - 1, Nz = p.grid_res[2] - 1;
+    const int Nx = sg.grid_res[0] - 1, Ny = sg.grid_res[1] - 1, Nz = sg.grid_res[2] - 1;
     bool wx = false, wy = false, wz = false;
     if (nx <= 1 && vx < 0.f) { vx = 0.f; wx = true; }
     if (nx >= Nx - 1 && vx > 0.f) { vx = 0.f; wx = true; }
@@ -766,13 +789,13 @@ __global__ __launch_bounds__(512) void k_grid_bwd(SubParams p, const float4* __r
     bool sdf_zero = false, sdf_proj = false;
     float ux = 0.f, uy = 0.f, uz = 0.f;
     if (sdf) {
-      const float sd = sample_sdf(sdf, p, nx, ny, nz);
+      const float sd = sample_sdf(sdf, sg, nx, ny, nz);
       if (sd < 0.f) {
         sdf_zero = true;
-      } else if (sd < p.h * 1.5f) {
-        const float ggx = sample_sdf(sdf, p, nx + 1, ny, nz) - sample_sdf(sdf, p, nx - 1, ny, nz);
-        const float ggy = sample_sdf(sdf, p, nx, ny + 1, nz) - sample_sdf(sdf, p, nx, ny - 1, nz);
-        const float ggz = sample_sdf(sdf, p, nx, ny, nz + 1) - sample_sdf(sdf, p, nx, ny, nz - 1);
+      } else if (sd < sg.h * 1.5f) {
+        const float ggx = sample_sdf(sdf, sg, nx + 1, ny, nz) - sample_sdf(sdf, sg, nx - 1, ny, nz);
+        const float ggy = sample_sdf(sdf, sg, nx, ny + 1, nz) - sample_sdf(sdf, sg, nx, ny - 1, nz);
+        const float ggz = sample_sdf(sdf, sg, nx, ny, nz + 1) - sample_sdf(sdf, sg, nx, ny, nz - 1);
         const float gl = len3(ggx, ggy, ggz);
         if (gl > 1e-6f) {
           ux = ggx / gl; uy = ggy / gl; uz = ggz / gl;
@@ -803,6 +826,7 @@ __global__ __launch_bounds__(512) void k_grid_bwd(SubParams p, const float4* __r
 // ---------------------------------------------------------------------------
 // P2G backward
 // ---------------------------------------------------------------------------
+template <bool BATCH>
 __global__ __launch_bounds__(256) void k_p2g_bwd(SubParams p, const float4* __restrict__ rec,
                                                  float4* __restrict__ adj_out, const nb_material* __restrict__ mats,
                                                  const uint32_t* __restrict__ sorted,
@@ -811,12 +835,13 @@ __global__ __launch_bounds__(256) void k_p2g_bwd(SubParams p, const float4* __re
   if (k >= p.n) return;
   const uint32_t q = sorted[k];
   const float4* r = rec + size_t(q) * kRec;
+  const SceneGeo sg = particle_geo<BATCH>(p, q);
   const float4 pp = r[R_P], vv = r[R_V];
   uint32_t mid = __float_as_uint(vv.w);
   mid = mid > p.material_count - 1u ? p.material_count - 1u : mid;
   const float mp = mats[mid].rho0 * pp.w;
   const bool apic = p.flip_ratio < -0.5f;
-  const float sc = apic ? mp * (4.f / (p.h * p.h)) : 0.f;
+  const float sc = apic ? mp * (4.f / (sg.h * sg.h)) : 0.f;
   const m3 B = rec_B(r);
   const m3 T = rec_tau(r);
   const float st = p.dt * pp.w;
@@ -826,18 +851,18 @@ __global__ __launch_bounds__(256) void k_p2g_bwd(SubParams p, const float4* __re
 #pragma unroll
   for (int a = 0; a < 3; a++) {
     // the bin's cell (shaders/mpm_common.glsl:190-195); P2G's stencil is the cell's 27 nodes
-    const float f = fminf(fmaxf((xs[a] - p.wmin[a]) / p.h, 0.f), float(p.grid_res[a]) - 1.f);
+    const float f = fminf(fmaxf((xs[a] - sg.wmin[a]) / sg.h, 0.f), float(sg.grid_res[a]) - 1.f);
     cc[a] = int(f);
     AxisD d;
-    axis_stencil_d(xs[a], p.wmin[a], p.h, cc[a] - 1, d);
+    axis_stencil_d(xs[a], sg.wmin[a], sg.h, cc[a] - 1, d);
     axis_factors(d, ax[a]);
   }
-  const uint32_t cube = p.cube_res;
+  const uint32_t cube = sg.cube_res;
   // grid adjoint (dL/dp, dL/dm) of node (ix, iy, iz); nodes outside the cube read as zero
   auto node_g = [&](int ix, int iy, int iz) -> float4 {
     const uint32_t nx = uint32_t(cc[0] - 1 + ix), ny = uint32_t(cc[1] - 1 + iy), nz = uint32_t(cc[2] - 1 + iz);
     const bool in = nx < cube && ny < cube && nz < cube;  // negative wraps to large
-    const float4 g = grid_adj[morton3(min(nx, cube - 1u), min(ny, cube - 1u), min(nz, cube - 1u))];
+    const float4 g = grid_adj[node_id(sg, min(nx, cube - 1u), min(ny, cube - 1u), min(nz, cube - 1u))];
     return in ? g : make_float4(0.f, 0.f, 0.f, 0.f);
   };
   // the gradient-times-offset and Hessian sums enter only the position
@@ -853,10 +878,11 @@ __global__ __launch_bounds__(256) void k_p2g_bwd(SubParams p, const float4* __re
       if (a != b) gxo[a] = fmaf(z, -st * (T.m[b][0] * qv[0] + T.m[b][1] * qv[1] + T.m[b][2] * qv[2]), gxo[a]);
     }
   };
-  if (!__any(has_tiny(ax[0], ax[1], ax[2])))
-    sep_gather<true>(ax[0], ax[1], ax[2], node_g, S, Sm, sink);
-  else
-    exact_gather<true>(ax[0], ax[1], ax[2], node_g, S, Sm, sink);  // shaders/mpm_p2g.comp:61
+  sep_gather<true>(ax[0], ax[1], ax[2], node_g, S, Sm, sink);
+  if (has_tiny(ax[0], ax[1], ax[2])) {  // the weight cut of shaders/mpm_p2g.comp:61, per particle
+    gxo[0] = gxo[1] = gxo[2] = 0.f;
+    exact_gather<true>(ax[0], ax[1], ax[2], node_g, S, Sm, sink);
+  }
   // P2G adds  w (m v + C d) - dt Vp tau grad w  to the node momentum and  w m  to its mass
   float gv[3], gx[3];
   m3 gC, gT;
@@ -892,23 +918,42 @@ __global__ __launch_bounds__(256) void k_p2g_bwd(SubParams p, const float4* __re
   go[R_T] = make_float4(gT.m[0][0], gT.m[1][1], gT.m[2][2], gt.w);
 }
 
-// Fixed-order sum of the per-block parameter partials, one block per slot:
-// thread t sums the partials t, t + 256, ... (coalesced across the block),
-// then a fixed LDS tree adds the 256 sums to the running total.
-__global__ __launch_bounds__(256) void k_pgrad_reduce(const float* __restrict__ part, uint32_t n_part,
+// Fixed-order sum of the per-block entry lists, one block per material:
+// thread t walks the lists of blocks t, t + 256, ... in order, adding the
+// entries of its material; a fixed LDS tree adds the 256 sums to the
+// running total.
+__global__ __launch_bounds__(256) void k_pgrad_reduce(const uint32_t* __restrict__ pg_mid,
+                                                      const float4* __restrict__ pg_val,
+                                                      const uint32_t* __restrict__ pg_cnt, uint32_t n_blocks,
                                                       float* __restrict__ pgrad) {
-  __shared__ float s[256];
-  const float* p = part + size_t(blockIdx.x) * n_part;
-  float acc = 0.f;
-#pragma unroll 8
-  for (uint32_t i = threadIdx.x; i < n_part; i += 256) acc += p[i];
+  __shared__ float4 s[256];
+  const uint32_t m = blockIdx.x;
+  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (uint32_t b = threadIdx.x; b < n_blocks; b += 256) {
+    const uint32_t c = pg_cnt[b];
+    for (uint32_t j = 0; j < c; j++) {
+      const size_t e = size_t(b) * kPgEntries + j;
+      if (pg_mid[e] == m) {
+        const float4 v = pg_val[e];
+        acc.x += v.x; acc.y += v.y; acc.z += v.z; acc.w += v.w;
+      }
+    }
+  }
   s[threadIdx.x] = acc;
   __syncthreads();
   for (int o = 128; o > 0; o >>= 1) {
-    if (int(threadIdx.x) < o) s[threadIdx.x] += s[threadIdx.x + o];
+    if (int(threadIdx.x) < o) {
+      const float4 v = s[threadIdx.x + o];
+      s[threadIdx.x].x += v.x; s[threadIdx.x].y += v.y; s[threadIdx.x].z += v.z; s[threadIdx.x].w += v.w;
+    }
     __syncthreads();
   }
-  if (threadIdx.x == 0) pgrad[blockIdx.x] += s[0];
+  if (threadIdx.x == 0) {
+    pgrad[4 * m + 0] += s[0].x;
+    pgrad[4 * m + 1] += s[0].y;
+    pgrad[4 * m + 2] += s[0].z;
+    pgrad[4 * m + 3] += s[0].w;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -917,17 +962,27 @@ __global__ __launch_bounds__(256) void k_pgrad_reduce(const float* __restrict__ 
 static inline uint32_t cdiv_a(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 static inline uint32_t tile_grid_a(uint32_t n) { return n < 2048u ? (n ? n : 1u) : 2048u; }
 
+template <bool BATCH>
+static void adjoint_kernels(const DevState& d, const AdjState& a, const SubParams& p, uint32_t n_tiles,
+                            const float4* adj_in, float4* adj_out, hipStream_t s) {
+  const uint32_t nblk = cdiv_a(p.n, 256);
+  k_g2p_bwd<BATCH><<<nblk, 256, 0, s>>>(p, d.rec, adj_in, adj_out, a.coef, a.pg_mid, a.pg_val, a.pg_cnt,
+                                         d.materials, d.sorted, d.grid_node, d.sdf);
+  k_adj_scatter<BATCH><<<tile_grid_a(n_tiles), kAdjThreads, 0, s>>>(p, d.rec, a.coef, d.cell_count, d.cell_end,
+                                                                    d.sorted, d.act_tiles, d.counters, a.partial_adj);
+  k_grid_bwd<BATCH><<<tile_grid_a(n_tiles), 512, 0, s>>>(p, d.grid_pre, a.partial_adj, d.tile_act, d.node_tiles,
+                                                          d.counters + CNT_NODE_TILES, a.grid_adj, d.sdf, d.forces);
+  k_p2g_bwd<BATCH><<<nblk, 256, 0, s>>>(p, d.rec, adj_out, d.materials, d.sorted, a.grid_adj);
+}
+
 void launch_adjoint_substep(const DevState& d, const AdjState& a, const SubParams& p, uint32_t n_tiles,
                             const float4* adj_in, float4* adj_out, hipStream_t s) {
   const uint32_t nblk = cdiv_a(p.n, 256);
-  k_g2p_bwd<<<nblk, 256, 0, s>>>(p, d.rec, adj_in, adj_out, a.coef, a.pgrad_w, d.materials, d.sorted, d.grid_node,
-                                  d.sdf);
-  k_adj_scatter<<<tile_grid_a(n_tiles), kAdjThreads, 0, s>>>(p, d.rec, a.coef, d.cell_count, d.cell_end, d.sorted,
-                                                             d.act_tiles, d.counters, a.partial_adj);
-  k_grid_bwd<<<tile_grid_a(n_tiles), 512, 0, s>>>(p, d.grid_pre, a.partial_adj, d.tile_act, d.node_tiles,
-                                                   d.counters + CNT_NODE_TILES, a.grid_adj, d.sdf, d.forces);
-  k_p2g_bwd<<<nblk, 256, 0, s>>>(p, d.rec, adj_out, d.materials, d.sorted, a.grid_adj);
-  k_pgrad_reduce<<<NB_MAX_MATERIALS * 4, 256, 0, s>>>(a.pgrad_w, nblk, a.pgrad);
+  if (p.n_scenes > 0)
+    adjoint_kernels<true>(d, a, p, n_tiles, adj_in, adj_out, s);
+  else
+    adjoint_kernels<false>(d, a, p, n_tiles, adj_in, adj_out, s);
+  k_pgrad_reduce<<<p.material_count, 256, 0, s>>>(a.pg_mid, a.pg_val, a.pg_cnt, nblk, a.pgrad);
 }
 
 }  // namespace nb

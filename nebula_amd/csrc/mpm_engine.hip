@@ -72,6 +72,17 @@ struct nb_mpm {
   bool old_valid = false;
   std::vector<void*> allocs;
 
+  // batch of scenes (n_scenes > 0): the scene table, the scene of every tile
+  // and of every particle row; the single-scene grid fields above are unused
+  uint32_t n_scenes = 0;
+  std::vector<SceneGeo> scenes_host;
+  std::vector<uint32_t> scene_first;  // first particle row of each scene
+  SceneGeo* d_scenes = nullptr;
+  uint32_t* d_tile_scene = nullptr;
+  uint32_t* d_pscene = nullptr;
+  bool batch() const { return n_scenes > 0; }
+  uint32_t max_materials() const { return batch() ? kBatchMaxMaterials : NB_MAX_MATERIALS; }
+
   // reverse mode: checkpoints of a recorded run of n substeps every `seg`,
   // the substep states of one segment, the adjoint in both layouts
   struct Adjoint {
@@ -133,6 +144,12 @@ struct nb_mpm {
     p.force_count = force_count;
     p.write_old = write_old ? 1u : 0u;
     p.corrected_svd = knobs.corrected_svd ? 1u : 0u;
+    if (batch()) {
+      p.n_scenes = n_scenes;
+      p.scenes = d_scenes;
+      p.tile_scene = d_tile_scene;
+      p.pscene = d_pscene;
+    }
     return p;
   }
 };
@@ -239,6 +256,8 @@ int enqueue_substep(nb_mpm* h, float dt, Prof* pf, const TapeIO& io = TapeIO{}) 
   return NB_OK;
 }
 
+int engine_alloc(nb_mpm* h, void* hip_stream);
+
 int run_substeps(nb_mpm* h, float dt, uint32_t n, Prof* pf) {
   if (n == 0 || h->n_live == 0) return NB_OK;
   int rc = sync_in(h);
@@ -318,62 +337,9 @@ int nb_mpm_create(const nb_mpm_config* cfg, void* hip_stream, nb_mpm** out) {
     return rc;
   };
 
-  if (hip_stream) {
-    h->stream = reinterpret_cast<hipStream_t>(hip_stream);
-  } else {
-    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
-      return cleanup_fail(fail(NB_ERR_HIP, "hipStreamCreate failed"));
-    h->own_stream = true;
-  }
-
-  DevState& d = h->d;
-  const size_t cells = size_t(h->NT) * kTileCells;
-  int rc = NB_OK;
-  for (int f = 0; f < NB_NUM_FIELDS && rc == NB_OK; f++) rc = h->alloc(&d.field[f], h->cap);
-  if (rc == NB_OK) rc = h->alloc(&d.rec, size_t(h->cap) * kRec);
-  if (rc == NB_OK) rc = h->alloc(&d.keys, h->cap);
-  if (rc == NB_OK) rc = h->alloc(&d.rank, h->cap);
-  if (rc == NB_OK) rc = h->alloc(&d.sorted, h->cap);
-  if (rc == NB_OK) rc = h->alloc(&d.cell_count, cells);
-  if (rc == NB_OK) rc = h->alloc(&d.cell_end, cells);
-  if (rc == NB_OK) rc = h->alloc(&d.tile_flag, h->NT);
-  if (rc == NB_OK) rc = h->alloc(&d.tile_mark, h->NT);
-  if (rc == NB_OK) rc = h->alloc(&d.blk_cnt, (h->NT + 255) / 256);
-  if (rc == NB_OK) rc = h->alloc(&d.tile_act, h->NT);
-  if (rc == NB_OK) rc = h->alloc(&d.act_tiles, h->NT);
-  if (rc == NB_OK) rc = h->alloc(&d.node_tiles, h->NT);
-  if (rc == NB_OK) rc = h->alloc(&d.all_tiles, h->NT);
-  if (rc == NB_OK) rc = h->alloc(&d.tile_total, h->NT);
-  if (rc == NB_OK) rc = h->alloc(&d.tile_start, h->NT);
-  if (rc == NB_OK) rc = h->alloc(&d.counters, CNT_SLOTS);
-  if (rc == NB_OK) rc = h->alloc(&d.partial, size_t(h->cap_tiles) * kBoxNodes);
-  if (rc == NB_OK) rc = h->alloc(&d.grid_node, cells);
-  if (rc == NB_OK) rc = h->alloc(&d.grid_old, cells);
-  if (rc == NB_OK) rc = h->alloc(&d.materials, NB_MAX_MATERIALS);
-  if (rc == NB_OK) rc = h->alloc(&d.colliders, NB_MAX_COLLIDERS);
-  if (rc == NB_OK) rc = h->alloc(&d.forces, NB_MAX_FORCES);
+  int rc = engine_alloc(h, hip_stream);
   if (rc != NB_OK) return cleanup_fail(rc);
-  d.sdf = nullptr;
-
-  auto zero = [&](void* p, size_t bytes) { return hipMemsetAsync(p, 0, bytes, h->stream); };
-  if (zero(d.cell_count, cells * 4) != hipSuccess || zero(d.cell_end, cells * 4) != hipSuccess ||
-      zero(d.tile_flag, h->NT) != hipSuccess || zero(d.counters, CNT_SLOTS * 4) != hipSuccess ||
-      zero(d.grid_node, cells * 16) != hipSuccess || zero(d.grid_old, cells * 16) != hipSuccess ||
-      zero(d.materials, NB_MAX_MATERIALS * 64) != hipSuccess || zero(d.colliders, NB_MAX_COLLIDERS * 64) != hipSuccess ||
-      zero(d.forces, NB_MAX_FORCES * 32) != hipSuccess)
-    return cleanup_fail(fail(NB_ERR_HIP, "hipMemsetAsync failed"));
-  for (int f = 0; f < NB_NUM_FIELDS; f++)
-    if (zero(d.field[f], size_t(std::max(h->cap, 1u)) * 16) != hipSuccess)
-      return cleanup_fail(fail(NB_ERR_HIP, "hipMemsetAsync failed"));
-  if (hipMemsetAsync(d.tile_act, 0xFF, size_t(h->NT) * 4, h->stream) != hipSuccess)
-    return cleanup_fail(fail(NB_ERR_HIP, "hipMemsetAsync failed"));
-  launch_fill_identity(d.all_tiles, h->NT, h->stream);
-  {
-    uint32_t all = h->NT;
-    if (hipMemcpyAsync(d.counters + CNT_ALL_TILES, &all, 4, hipMemcpyHostToDevice, h->stream) != hipSuccess ||
-        hipStreamSynchronize(h->stream) != hipSuccess)
-      return cleanup_fail(fail(NB_ERR_HIP, "counter upload failed"));
-  }
+  DevState& d = h->d;
 
   // default material (src/engine/MPMEngine.cpp:67-78)
   {
@@ -414,6 +380,196 @@ int nb_mpm_create(const nb_mpm_config* cfg, void* hip_stream, nb_mpm** out) {
   *out = h;
   return NB_OK;
 }
+
+int nb_mpm_create_batch(const nb_mpm_scene* scenes, uint32_t n_scenes, void* hip_stream, nb_mpm** out) {
+  if (!scenes || !out) return fail(NB_ERR_ARG, "nb_mpm_create_batch: null argument");
+  *out = nullptr;
+  if (n_scenes == 0) return fail(NB_ERR_ARG, "a batch needs at least one scene");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(NB_ERR_HIP, "no HIP device available");
+  // scene table: each scene's Morton cube at the next multiple of 512 keys
+  std::vector<SceneGeo> geo(n_scenes);
+  std::vector<uint32_t> first(n_scenes);
+  uint64_t tiles = 0, particles = 0;
+  for (uint32_t s = 0; s < n_scenes; s++) {
+    const nb_mpm_scene& sc = scenes[s];
+    if (!(sc.cell_size > 0.f)) return fail(NB_ERR_ARG, "scene %u: cell_size must be > 0", s);
+    nb_mpm_config c{};
+    for (int a = 0; a < 3; a++) {
+      if (!(sc.domain_size[a] >= 0.f)) return fail(NB_ERR_ARG, "scene %u: domain_size must be >= 0", s);
+      c.domain_size[a] = sc.domain_size[a];
+    }
+    c.cell_size = sc.cell_size;
+    uint32_t res[3], cube = 1;
+    domain_dims(c, res, cube);
+    if (cube > 1024u) return fail(NB_ERR_RANGE, "scene %u: %u cells per axis exceed the 10-bit Morton range", s, cube);
+    SceneGeo& g = geo[s];
+    g = SceneGeo{};
+    g.h = sc.cell_size;
+    g.inv_h = 1.0f / sc.cell_size;
+    for (int a = 0; a < 3; a++) {
+      g.wmin[a] = 0.f;
+      g.wmax[a] = sc.domain_size[a];
+      g.grid_res[a] = int32_t(res[a]);
+    }
+    g.cube_res = cube;
+    g.tile_dim = std::max(cube, 8u) / 8u;
+    g.key_base = uint32_t(tiles * kTileCells);
+    tiles += uint64_t(g.tile_dim) * g.tile_dim * g.tile_dim;
+    if (tiles * kTileCells > 0xFFFFFFFFull) return fail(NB_ERR_RANGE, "the scenes' cells exceed the 32-bit key space");
+    first[s] = uint32_t(particles);
+    particles += sc.particle_count;
+    if (particles > 0xFFFFFFF0ull) return fail(NB_ERR_RANGE, "too many particles in the batch");
+  }
+
+  nb_mpm* h = new nb_mpm();
+  auto cleanup_fail = [&](int rc) {
+    std::string keep = g_err;
+    nb_mpm_destroy(h);
+    g_err = keep;
+    return rc;
+  };
+  nb_mpm_config_default(&h->cfg);
+  h->cfg.cell_size = geo[0].h;
+  for (int a = 0; a < 3; a++) {
+    h->cfg.domain_size[a] = geo[0].wmax[a];
+    h->grid_res[a] = uint32_t(geo[0].grid_res[a]);
+  }
+  h->cfg.nx = h->cfg.ny = h->cfg.nz = 0;
+  h->cfg.max_particles = uint32_t(particles);
+  nb_mpm_knobs_default(&h->knobs);
+  h->cube_res = geo[0].cube_res;
+  h->tile_dim = geo[0].tile_dim;
+  h->NT = uint32_t(tiles);
+  h->NC = h->NT * kTileCells;
+  h->cap = uint32_t(particles);
+  h->cap_tiles = std::max<uint32_t>(1u, std::min<uint32_t>(h->NT, std::max<uint32_t>(h->cap, 1u)));
+  h->n_scenes = n_scenes;
+  h->scenes_host = geo;
+  h->scene_first = first;
+  int rc = engine_alloc(h, hip_stream);
+  if (rc == NB_OK) rc = h->alloc(&h->d_scenes, n_scenes);
+  if (rc == NB_OK) rc = h->alloc(&h->d_tile_scene, h->NT);
+  if (rc == NB_OK) rc = h->alloc(&h->d_pscene, h->cap);
+  if (rc != NB_OK) return cleanup_fail(rc);
+  {
+    std::vector<uint32_t> ts(h->NT), ps(std::max(h->cap, 1u));
+    for (uint32_t s = 0; s < n_scenes; s++) {
+      const uint32_t t0 = geo[s].key_base / kTileCells, nt = geo[s].tile_dim * geo[s].tile_dim * geo[s].tile_dim;
+      std::fill(ts.begin() + t0, ts.begin() + t0 + nt, s);
+      std::fill(ps.begin() + first[s], ps.begin() + first[s] + scenes[s].particle_count, s);
+    }
+    if ((rc = upload(h, h->d_scenes, geo.data(), geo.size() * sizeof(SceneGeo))) != NB_OK) return cleanup_fail(rc);
+    if ((rc = upload(h, h->d_tile_scene, ts.data(), size_t(h->NT) * 4)) != NB_OK) return cleanup_fail(rc);
+    if ((rc = upload(h, h->d_pscene, ps.data(), size_t(h->cap) * 4)) != NB_OK) return cleanup_fail(rc);
+  }
+  // default material as nb_mpm_create's, from nb_mpm_config_default's E, nu, rho0
+  {
+    nb_material m = preset_jelly(h->cfg.E, h->cfg.nu, h->cfg.rho0);
+    m.model = NB_MAT_ELASTIC;
+    m.M_friction = h->knobs.M_friction;
+    m.q_cohesion = h->knobs.q_cohesion;
+    m.q_max = h->knobs.q_max;
+    if ((rc = upload(h, h->d.materials, &m, sizeof(m))) != NB_OK) return cleanup_fail(rc);
+    h->material_count = 1;
+    h->mats_host.assign(1, m);
+  }
+  // every particle starts at rest with F = I, B = 0, tau = 0 and material 0;
+  // positions are the caller's (nb_mpm_set_field)
+  h->n_live = h->cap;
+  if (h->n_live) launch_init_particles(h->d, 0, h->n_live, h->stream);
+  if (hipStreamSynchronize(h->stream) != hipSuccess) return cleanup_fail(fail(NB_ERR_HIP, "init failed"));
+  *out = h;
+  return NB_OK;
+}
+
+int nb_mpm_scene_count(const nb_mpm* h, uint32_t* n) {
+  if (!h || !n) return fail(NB_ERR_ARG, "null argument");
+  *n = h->n_scenes;
+  return NB_OK;
+}
+
+int nb_mpm_scene_info(const nb_mpm* h, uint32_t scene, uint32_t res[3], uint32_t* first_particle,
+                      uint32_t* particle_count) {
+  if (!h) return fail(NB_ERR_ARG, "null handle");
+  if (!h->batch()) return fail(NB_ERR_STATE, "not a batch");
+  if (scene >= h->n_scenes) return fail(NB_ERR_RANGE, "scene %u of %u", scene, h->n_scenes);
+  const SceneGeo& g = h->scenes_host[scene];
+  if (res)
+    for (int a = 0; a < 3; a++) res[a] = uint32_t(g.grid_res[a]);
+  const uint32_t f = h->scene_first[scene];
+  const uint32_t e = scene + 1 < h->n_scenes ? h->scene_first[scene + 1] : h->cap;
+  if (first_particle) *first_particle = f;
+  if (particle_count) *particle_count = e - f;
+  return NB_OK;
+}
+
+}  // extern "C"
+
+namespace {
+// Device buffers of an engine whose sizes (cap, NT, cap_tiles) are set.
+int engine_alloc(nb_mpm* h, void* hip_stream) {
+  if (hip_stream) {
+    h->stream = reinterpret_cast<hipStream_t>(hip_stream);
+  } else {
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
+      return fail(NB_ERR_HIP, "hipStreamCreate failed");
+    h->own_stream = true;
+  }
+
+  DevState& d = h->d;
+  const size_t cells = size_t(h->NT) * kTileCells;
+  int rc = NB_OK;
+  for (int f = 0; f < NB_NUM_FIELDS && rc == NB_OK; f++) rc = h->alloc(&d.field[f], h->cap);
+  if (rc == NB_OK) rc = h->alloc(&d.rec, size_t(h->cap) * kRec);
+  if (rc == NB_OK) rc = h->alloc(&d.keys, h->cap);
+  if (rc == NB_OK) rc = h->alloc(&d.rank, h->cap);
+  if (rc == NB_OK) rc = h->alloc(&d.sorted, h->cap);
+  if (rc == NB_OK) rc = h->alloc(&d.cell_count, cells);
+  if (rc == NB_OK) rc = h->alloc(&d.cell_end, cells);
+  if (rc == NB_OK) rc = h->alloc(&d.tile_flag, h->NT);
+  if (rc == NB_OK) rc = h->alloc(&d.tile_mark, h->NT);
+  if (rc == NB_OK) rc = h->alloc(&d.blk_cnt, (h->NT + 255) / 256);
+  if (rc == NB_OK) rc = h->alloc(&d.tile_act, h->NT);
+  if (rc == NB_OK) rc = h->alloc(&d.act_tiles, h->NT);
+  if (rc == NB_OK) rc = h->alloc(&d.node_tiles, h->NT);
+  if (rc == NB_OK) rc = h->alloc(&d.all_tiles, h->NT);
+  if (rc == NB_OK) rc = h->alloc(&d.tile_total, h->NT);
+  if (rc == NB_OK) rc = h->alloc(&d.tile_start, h->NT);
+  if (rc == NB_OK) rc = h->alloc(&d.counters, CNT_SLOTS);
+  if (rc == NB_OK) rc = h->alloc(&d.partial, size_t(h->cap_tiles) * kBoxNodes);
+  if (rc == NB_OK) rc = h->alloc(&d.grid_node, cells);
+  if (rc == NB_OK) rc = h->alloc(&d.grid_old, cells);
+  if (rc == NB_OK) rc = h->alloc(&d.materials, h->max_materials());
+  if (rc == NB_OK) rc = h->alloc(&d.colliders, NB_MAX_COLLIDERS);
+  if (rc == NB_OK) rc = h->alloc(&d.forces, NB_MAX_FORCES);
+  if (rc != NB_OK) return rc;
+  d.sdf = nullptr;
+
+  auto zero = [&](void* p, size_t bytes) { return hipMemsetAsync(p, 0, bytes, h->stream); };
+  if (zero(d.cell_count, cells * 4) != hipSuccess || zero(d.cell_end, cells * 4) != hipSuccess ||
+      zero(d.tile_flag, h->NT) != hipSuccess || zero(d.counters, CNT_SLOTS * 4) != hipSuccess ||
+      zero(d.grid_node, cells * 16) != hipSuccess || zero(d.grid_old, cells * 16) != hipSuccess ||
+      zero(d.materials, size_t(h->max_materials()) * 64) != hipSuccess ||
+      zero(d.colliders, NB_MAX_COLLIDERS * 64) != hipSuccess || zero(d.forces, NB_MAX_FORCES * 32) != hipSuccess)
+    return fail(NB_ERR_HIP, "hipMemsetAsync failed");
+  for (int f = 0; f < NB_NUM_FIELDS; f++)
+    if (zero(d.field[f], size_t(std::max(h->cap, 1u)) * 16) != hipSuccess)
+      return fail(NB_ERR_HIP, "hipMemsetAsync failed");
+  if (hipMemsetAsync(d.tile_act, 0xFF, size_t(h->NT) * 4, h->stream) != hipSuccess)
+    return fail(NB_ERR_HIP, "hipMemsetAsync failed");
+  launch_fill_identity(d.all_tiles, h->NT, h->stream);
+  {
+    uint32_t all = h->NT;
+    if (hipMemcpyAsync(d.counters + CNT_ALL_TILES, &all, 4, hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+        hipStreamSynchronize(h->stream) != hipSuccess)
+      return fail(NB_ERR_HIP, "counter upload failed");
+  }
+  return NB_OK;
+}
+}  // namespace
+
+extern "C" {
 
 int nb_mpm_destroy(nb_mpm* h) {
   if (!h) return NB_OK;
@@ -484,7 +640,7 @@ int nb_mpm_set_materials(nb_mpm* h, const nb_material* mats, uint32_t n) {
   if (!h) return fail(NB_ERR_ARG, "null handle");
   if (n == 0) return NB_OK;  // src/engine/MPMEngine.cpp:240
   if (!mats) return fail(NB_ERR_ARG, "null materials");
-  if (n > NB_MAX_MATERIALS) return fail(NB_ERR_RANGE, "%u materials exceed the %u slots", n, NB_MAX_MATERIALS);
+  if (n > h->max_materials()) return fail(NB_ERR_RANGE, "%u materials exceed the %u slots", n, h->max_materials());
   int rc = upload(h, h->d.materials, mats, sizeof(nb_material) * n);
   if (rc == NB_OK) {
     h->material_count = n;
@@ -526,6 +682,7 @@ static int ensure_sdf(nb_mpm* h) {
 
 int nb_mpm_set_collider_sdf(nb_mpm* h, const float* sdf, uint64_t n) {
   if (!h) return fail(NB_ERR_ARG, "null handle");
+  if (h->batch()) return fail(NB_ERR_UNSUPPORTED, "%s: not available on a batch of scenes", __func__);
   if (!sdf) return fail(NB_ERR_ARG, "null sdf");
   if (n != h->NC) return fail(NB_ERR_RANGE, "sdf has %llu values, the Morton cube has %u cells", (unsigned long long)n, h->NC);
   int rc = ensure_sdf(h);
@@ -537,6 +694,7 @@ int nb_mpm_set_collider_sdf(nb_mpm* h, const float* sdf, uint64_t n) {
 
 int nb_mpm_set_collider_sphere(nb_mpm* h, float r, float cx, float cy, float cz) {
   if (!h) return fail(NB_ERR_ARG, "null handle");
+  if (h->batch()) return fail(NB_ERR_UNSUPPORTED, "%s: not available on a batch of scenes", __func__);
   int rc = ensure_sdf(h);
   if (rc != NB_OK) return rc;
   launch_sdf_sphere(h->d.sdf, h->params(0.f, false), r, cx, cy, cz, h->stream);
@@ -569,6 +727,7 @@ int nb_mpm_set_forces(nb_mpm* h, const nb_force* forces, uint32_t n) {
 
 int nb_mpm_append_particles(nb_mpm* h, const float* pos4, const float* vel4, uint32_t n, uint32_t* appended) {
   if (!h) return fail(NB_ERR_ARG, "null handle");
+  if (h->batch()) return fail(NB_ERR_UNSUPPORTED, "%s: not available on a batch of scenes", __func__);
   if (appended) *appended = 0;
   const uint32_t nn = std::min(n, h->cap - h->n_live);
   if (nn == 0) return NB_OK;
@@ -588,6 +747,7 @@ int nb_mpm_append_particles(nb_mpm* h, const float* pos4, const float* vel4, uin
 
 int nb_mpm_reset_particles(nb_mpm* h) {
   if (!h) return fail(NB_ERR_ARG, "null handle");
+  if (h->batch()) return fail(NB_ERR_UNSUPPORTED, "%s: not available on a batch of scenes", __func__);
   NB_HIP(hipStreamSynchronize(h->stream));
   h->n_live = 0;
   h->bins_valid = false;
@@ -620,6 +780,7 @@ int nb_mpm_field_device_ptr(nb_mpm* h, int field, void** dptr) {
 
 int nb_mpm_set_live_particle_count(nb_mpm* h, uint32_t n) {
   if (!h) return fail(NB_ERR_ARG, "null handle");
+  if (h->batch() && n != h->cap) return fail(NB_ERR_UNSUPPORTED, "a batch's particle count is fixed by its scenes");
   if (n > h->cap) return fail(NB_ERR_RANGE, "%u exceeds capacity %u", n, h->cap);
   h->n_live = n;
   h->bins_valid = false;
@@ -630,6 +791,7 @@ int nb_mpm_set_live_particle_count(nb_mpm* h, uint32_t n) {
 // ---- pass-level access ----
 int nb_mpm_run_bin(nb_mpm* h) {
   if (!h) return fail(NB_ERR_ARG, "null handle");
+  if (h->batch()) return fail(NB_ERR_UNSUPPORTED, "%s: not available on a batch of scenes", __func__);
   SubParams p = h->params(0.f, false);
   int rc = sync_in(h);
   if (rc != NB_OK) return rc;
@@ -642,6 +804,7 @@ int nb_mpm_run_bin(nb_mpm* h) {
 
 int nb_mpm_get_bins(nb_mpm* h, uint32_t* keys, uint32_t* sorted, uint32_t* cell_count, uint32_t* cell_end) {
   if (!h) return fail(NB_ERR_ARG, "null handle");
+  if (h->batch()) return fail(NB_ERR_UNSUPPORTED, "%s: not available on a batch of scenes", __func__);
   if (!h->bins_valid) return fail(NB_ERR_STATE, "no bins: call nb_mpm_run_bin first");
   const uint32_t n = h->n_live;
   int rc = NB_OK;
@@ -670,6 +833,7 @@ int nb_mpm_get_bins(nb_mpm* h, uint32_t* keys, uint32_t* sorted, uint32_t* cell_
 
 int nb_mpm_run_p2g(nb_mpm* h, float dt_sub) {
   if (!h) return fail(NB_ERR_ARG, "null handle");
+  if (h->batch()) return fail(NB_ERR_UNSUPPORTED, "%s: not available on a batch of scenes", __func__);
   if (!h->bins_valid) return fail(NB_ERR_STATE, "no bins: call nb_mpm_run_bin first");
   SubParams p = h->params(dt_sub, false);
   int rc = sync_in(h);
@@ -687,6 +851,7 @@ int nb_mpm_run_p2g(nb_mpm* h, float dt_sub) {
 
 int nb_mpm_run_grid_update(nb_mpm* h, float dt_sub) {
   if (!h) return fail(NB_ERR_ARG, "null handle");
+  if (h->batch()) return fail(NB_ERR_UNSUPPORTED, "%s: not available on a batch of scenes", __func__);
   if (h->grid_mode == GridMode::NONE) return NB_OK;
   SubParams p = h->params(dt_sub, true);
   if (h->grid_mode == GridMode::DENSE)
@@ -701,6 +866,7 @@ int nb_mpm_run_grid_update(nb_mpm* h, float dt_sub) {
 
 int nb_mpm_run_g2p(nb_mpm* h, float dt_sub) {
   if (!h) return fail(NB_ERR_ARG, "null handle");
+  if (h->batch()) return fail(NB_ERR_UNSUPPORTED, "%s: not available on a batch of scenes", __func__);
   SubParams p = h->params(dt_sub, false);
   int rc = sync_in(h);
   if (rc != NB_OK) return rc;
@@ -715,6 +881,7 @@ int nb_mpm_run_g2p(nb_mpm* h, float dt_sub) {
 
 int nb_mpm_get_grid(nb_mpm* h, float* mom4, float* mass) {
   if (!h) return fail(NB_ERR_ARG, "null handle");
+  if (h->batch()) return fail(NB_ERR_UNSUPPORTED, "%s: not available on a batch of scenes", __func__);
   const size_t NC = h->NC;
   if (mom4) std::memset(mom4, 0, NC * 2 * 16);
   if (mass) std::memset(mass, 0, NC * 4);
@@ -781,6 +948,7 @@ int nb_mpm_get_grid(nb_mpm* h, float* mom4, float* mass) {
 
 int nb_mpm_set_grid(nb_mpm* h, const float* mom4, const float* mass, const float* old4) {
   if (!h || !mom4 || !mass) return fail(NB_ERR_ARG, "null argument");
+  if (h->batch()) return fail(NB_ERR_UNSUPPORTED, "%s: not available on a batch of scenes", __func__);
   const size_t cells = size_t(h->NT) * kTileCells;
   std::vector<float> node(cells * 4, 0.f);
   for (size_t c = 0; c < h->NC; c++) {
@@ -969,17 +1137,21 @@ int nb_mpm_adjoint_begin(nb_mpm* h, uint32_t n_substeps, uint32_t segment) {
   if (rc == NB_OK) rc = adj_alloc(h, &A.a.coef, size_t(h->cap) * 6);
   if (rc == NB_OK) rc = adj_alloc(h, &A.a.partial_adj, size_t(h->cap_tiles) * kBoxNodes);
   if (rc == NB_OK) rc = adj_alloc(h, &A.a.grid_adj, size_t(h->NT) * kTileCells);
-  float4* pw = nullptr;
+  float4* pm = nullptr;
+  float4* pc = nullptr;
   float4* pg = nullptr;
-  if (rc == NB_OK) rc = adj_alloc(h, &pw, size_t(n_blocks) * NB_MAX_MATERIALS);
-  if (rc == NB_OK) rc = adj_alloc(h, &pg, NB_MAX_MATERIALS);
+  if (rc == NB_OK) rc = adj_alloc(h, &A.a.pg_val, size_t(n_blocks) * kPgEntries);
+  if (rc == NB_OK) rc = adj_alloc(h, &pm, size_t(n_blocks) * kPgEntries / 4);
+  if (rc == NB_OK) rc = adj_alloc(h, &pc, (n_blocks + 3) / 4);
+  if (rc == NB_OK) rc = adj_alloc(h, &pg, h->max_materials());
   if (rc != NB_OK) {
     std::string keep = g_err;
     adj_free(h);
     g_err = keep;
     return rc;
   }
-  A.a.pgrad_w = reinterpret_cast<float*>(pw);
+  A.a.pg_mid = reinterpret_cast<uint32_t*>(pm);
+  A.a.pg_cnt = reinterpret_cast<uint32_t*>(pc);
   A.ckpt_keys = reinterpret_cast<uint32_t*>(kc);
   A.tape_keys = reinterpret_cast<uint32_t*>(kt);
   A.a.pgrad = reinterpret_cast<float*>(pg);
@@ -1042,7 +1214,7 @@ int nb_mpm_backward(nb_mpm* h, float* d_params, uint32_t params_cap) {
   if (!h) return fail(NB_ERR_ARG, "null handle");
   auto& A = h->adj;
   if (!A.on || !A.recorded) return fail(NB_ERR_STATE, "call nb_mpm_forward_recorded first");
-  NB_HIP(hipMemsetAsync(A.a.pgrad, 0, NB_MAX_MATERIALS * 4 * sizeof(float), h->stream));
+  NB_HIP(hipMemsetAsync(A.a.pgrad, 0, size_t(h->max_materials()) * 4 * sizeof(float), h->stream));
   launch_soa_to_aos_ptrs(adj_fields(h), A.adj_a, h->n_live, h->stream);
   float4* in = A.adj_a;
   float4* out = A.adj_b;
@@ -1084,10 +1256,10 @@ int nb_mpm_backward(nb_mpm* h, float* d_params, uint32_t params_cap) {
   h->bins_valid = false;
   h->grid_mode = GridMode::NONE;
   if (d_params) {
-    float pg[NB_MAX_MATERIALS * 4];
-    if ((rc = download(h, pg, A.a.pgrad, sizeof(pg))) != NB_OK) return rc;
+    std::vector<float> pg(size_t(h->material_count) * 4);
+    if ((rc = download(h, pg.data(), A.a.pgrad, pg.size() * sizeof(float))) != NB_OK) return rc;
     const uint32_t n = std::min(params_cap, h->material_count * 4u);
-    std::memcpy(d_params, pg, n * sizeof(float));
+    std::memcpy(d_params, pg.data(), n * sizeof(float));
   }
   NB_HIP(hipStreamSynchronize(h->stream));
   return NB_OK;

@@ -58,7 +58,7 @@ __global__ __launch_bounds__(512) void k_bin_clear(uint32_t* __restrict__ cell_c
 // Key of every particle (computed from the record, or read when the previous
 // G2P wrote it), its rank among equal keys (wave-aggregated atomics: one
 // atomic per distinct key per wave), and the occupied-tile flags.
-template <bool FROM_REC>
+template <bool FROM_REC, bool BATCH>
 __global__ __launch_bounds__(256) void k_bin_keys(SubParams p, const float4* __restrict__ rec,
                                                   uint32_t* __restrict__ keys, uint32_t* __restrict__ rank,
                                                   uint32_t* __restrict__ cell_count, uint8_t* __restrict__ tile_flag) {
@@ -68,7 +68,7 @@ __global__ __launch_bounds__(256) void k_bin_keys(SubParams p, const float4* __r
   if (valid) {
     if (FROM_REC) {
       const float4 x = rec[size_t(i) * kRec + R_P];
-      key = cell_key(p, x.x, x.y, x.z);
+      key = cell_key(particle_geo<BATCH>(p, i), x.x, x.y, x.z);
       keys[i] = key;
     } else {
       key = keys[i];
@@ -104,23 +104,25 @@ __global__ __launch_bounds__(256) void k_bin_keys(SubParams p, const float4* __r
 // Per tile: bit0 = occupied, bit1 = some tile in its 3x3x3 neighbourhood is
 // occupied (its nodes can receive mass).  Each block of 256 tiles also stores
 // its counts of both kinds (occupied in the low, node tiles in the high half).
-__global__ __launch_bounds__(256) void k_bin_mark(const uint8_t* __restrict__ tile_flag,
+template <bool BATCH>
+__global__ __launch_bounds__(256) void k_bin_mark(SubParams p, const uint8_t* __restrict__ tile_flag,
                                                   uint8_t* __restrict__ tile_mark, uint64_t* __restrict__ blk_cnt,
-                                                  uint32_t tile_dim, uint32_t nt) {
+                                                  uint32_t nt) {
   __shared__ uint64_t s_scan[256 / 64 + 1];
   const uint32_t T = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t m = 0;
   if (T < nt) {
+    const SceneGeo g = tile_geo<BATCH>(p, T);  // neighbours stay inside the tile's scene
     int tx, ty, tz;
-    tile_coords(T, tx, ty, tz);
-    const int td = int(tile_dim);
+    tile_coords(g, T, tx, ty, tz);
+    const int td = int(g.tile_dim);
     uint8_t node = 0;
     for (int dz = -1; dz <= 1; dz++)
       for (int dy = -1; dy <= 1; dy++)
         for (int dx = -1; dx <= 1; dx++) {
           int x = tx + dx, y = ty + dy, z = tz + dz;
           if (x < 0 || y < 0 || z < 0 || x >= td || y >= td || z >= td) continue;
-          node |= tile_flag[morton3(x, y, z)];
+          node |= tile_flag[tile_id(g, x, y, z)];
         }
     m = (tile_flag[T] ? 1u : 0u) | (node ? 2u : 0u);
     tile_mark[T] = uint8_t(m);
@@ -342,6 +344,7 @@ __device__ __forceinline__ uint32_t p2g_group(uint32_t count, uint32_t chunks) {
   return per >= 12u ? 0u : (per >= 10u ? 1u : (per >= 8u ? 2u : (per >= 6u ? 3u : 4u)));
 }
 
+template <bool BATCH>
 __global__ __launch_bounds__(kP2GThreads, 2) void k_p2g(SubParams p, const float4* __restrict__ rec,
                                                      const nb_material* __restrict__ mats,
                                                      const uint32_t* __restrict__ cell_count,
@@ -357,18 +360,19 @@ __global__ __launch_bounds__(kP2GThreads, 2) void k_p2g(SubParams p, const float
   __shared__ uint32_t s_layers;
   __shared__ uint32_t s_idx[kIdxStage];
   __shared__ uint64_t s_scan64[kP2GThreads / 64 + 1];
-  __shared__ float s_rho0[NB_MAX_MATERIALS];
-  if (threadIdx.x < p.material_count) s_rho0[threadIdx.x] = mats[threadIdx.x].rho0;
+  __shared__ float s_rho0[BATCH ? kBatchMaxMaterials : NB_MAX_MATERIALS];
+  for (uint32_t i = threadIdx.x; i < p.material_count; i += kP2GThreads) s_rho0[i] = mats[i].rho0;
   const uint32_t na = counters[CNT_ACTIVE];
   const bool apic = p.flip_ratio < -0.5f;
-  const float D_inv = 4.f / (p.h * p.h);
-  const float inv_h = 1.f / p.h;
   const uint32_t mlast = p.material_count - 1u;
 
   for (uint32_t a = blockIdx.x; a < na; a += gridDim.x) {
     const uint32_t T = act_tiles[a];
+    const SceneGeo g = tile_geo<BATCH>(p, T);
+    const float D_inv = 4.f / (g.h * g.h);
+    const float inv_h = g.inv_h;
     int tx, ty, tz;
-    tile_coords(T, tx, ty, tz);
+    tile_coords(g, T, tx, ty, tz);
     const size_t tbase = size_t(T) << kTileBits;
     for (int e = threadIdx.x; e < kBoxNodes; e += kP2GThreads) {
       s_m[e] = 0.f; s_x[e] = 0.f; s_y[e] = 0.f; s_z[e] = 0.f;
@@ -448,9 +452,9 @@ __global__ __launch_bounds__(kP2GThreads, 2) void k_p2g(SubParams p, const float
       // (the stencil's quotient (x - wmin)/h is taken as a product with 1/h:
       // the cell is the bin's, only the weights use it)
       auto stencil = [&](const float4& pp, Src& q) {
-        axis_stencil_f((pp.x - p.wmin[0]) * inv_h, pp.x, p.wmin[0], p.h, inv_h, cx - 1, q.sx);
-        axis_stencil_f((pp.y - p.wmin[1]) * inv_h, pp.y, p.wmin[1], p.h, inv_h, cy - 1, q.sy);
-        axis_stencil_f((pp.z - p.wmin[2]) * inv_h, pp.z, p.wmin[2], p.h, inv_h, cz - 1, q.sz);
+        axis_stencil_f((pp.x - g.wmin[0]) * inv_h, pp.x, g.wmin[0], g.h, inv_h, cx - 1, q.sx);
+        axis_stencil_f((pp.y - g.wmin[1]) * inv_h, pp.y, g.wmin[1], g.h, inv_h, cy - 1, q.sy);
+        axis_stencil_f((pp.z - g.wmin[2]) * inv_h, pp.z, g.wmin[2], g.h, inv_h, cz - 1, q.sz);
       };
       // keep = 0 scales the particle's contribution to exactly zero
       auto setup = [&](const float4& pp, const float4& vv, const float4& b0, const float4& b1, const float4& b2,
@@ -610,7 +614,7 @@ __global__ __launch_bounds__(kP2GThreads, 2) void k_p2g(SubParams p, const float
 // ---------------------------------------------------------------------------
 // GRID UPDATE
 // ---------------------------------------------------------------------------
-template <bool FROM_PARTIALS, bool UPDATE>
+template <bool FROM_PARTIALS, bool UPDATE, bool BATCH>
 __global__ __launch_bounds__(512) void k_grid(SubParams p, const float4* __restrict__ partial,
                                               const int32_t* __restrict__ tile_act, const uint32_t* __restrict__ tiles,
                                               const uint32_t* __restrict__ count, float4* __restrict__ grid_node,
@@ -621,11 +625,12 @@ __global__ __launch_bounds__(512) void k_grid(SubParams p, const float4* __restr
   // thread j holds the tile's j-th node in Morton order (contiguous in HBM)
   const int lx = int(morton_compact(threadIdx.x)), ly = int(morton_compact(threadIdx.x >> 1)),
             lz = int(morton_compact(threadIdx.x >> 2));
-  const int td = int(p.tile_dim);
   for (uint32_t b = blockIdx.x; b < nb_tiles; b += gridDim.x) {
     const uint32_t N = tiles[b];
+    const SceneGeo g = tile_geo<BATCH>(p, N);
+    const int td = int(g.tile_dim);
     int tx, ty, tz;
-    tile_coords(N, tx, ty, tz);
+    tile_coords(g, N, tx, ty, tz);
     const int nx = tx * 8 + lx, ny = ty * 8 + ly, nz = tz * 8 + lz;
     const size_t nid = (size_t(N) << kTileBits) | threadIdx.x;
     float mx, my, mz, mass;
@@ -645,7 +650,7 @@ __global__ __launch_bounds__(512) void k_grid(SubParams p, const float4* __restr
             if ((dx < 0 && lx != 0) || (dx > 0 && lx != 7)) continue;
             const int sx = tx + dx;
             if (sx < 0 || sx >= td) continue;
-            const int32_t a = tile_act[morton3(sx, sy, sz)];
+            const int32_t a = tile_act[tile_id(g, sx, sy, sz)];
             if (a < 0) continue;
             const int bx = lx - 8 * dx + 1, by = ly - 8 * dy + 1, bz = lz - 8 * dz + 1;
             const float4 q = partial[size_t(a) * kBoxNodes + (bz * kBox + by) * kBox + bx];
@@ -678,9 +683,9 @@ __global__ __launch_bounds__(512) void k_grid(SubParams p, const float4* __restr
       }
     }
     if (p.write_old) grid_old[nid] = make_float4(vx, vy, vz, 0.f);
-    const float px = (float(nx) + 0.5f) * p.h + p.wmin[0];
-    const float py = (float(ny) + 0.5f) * p.h + p.wmin[1];
-    const float pz = (float(nz) + 0.5f) * p.h + p.wmin[2];
+    const float px = (float(nx) + 0.5f) * g.h + g.wmin[0];
+    const float py = (float(ny) + 0.5f) * g.h + g.wmin[1];
+    const float pz = (float(nz) + 0.5f) * g.h + g.wmin[2];
     // force block generated by src/core/ForceShaderCompiler.cpp:72-93 (typeFlag 0)
     for (uint32_t fi = 0; fi < p.force_count; fi++) {
       const nb_force f = forces[fi];
@@ -694,7 +699,7 @@ __global__ __launch_bounds__(512) void k_grid(SubParams p, const float4* __restr
       }
     }
     // separating walls on the two outer node layers (shaders/mpm_grid_update.comp:54-60)
-    const int Nx = p.grid_res[0] - 1, Ny = p.grid_res[1] - 1, Nz = p.grid_res[2] - 1;
+    const int Nx = g.grid_res[0] - 1, Ny = g.grid_res[1] - 1, Nz = g.grid_res[2] - 1;
     if (nx <= 1 && vx < 0.f) vx = 0.f;
     if (nx >= Nx - 1 && vx > 0.f) vx = 0.f;
     if (ny <= 1 && vy < 0.f) vy = 0.f;
@@ -709,13 +714,13 @@ __global__ __launch_bounds__(512) void k_grid(SubParams p, const float4* __restr
     }
     // dense-SDF BC (shaders/mpm_nanovdb_bc.comp:23-54)
     if (sdf) {
-      const float sd = sample_sdf(sdf, p, nx, ny, nz);
+      const float sd = sample_sdf(sdf, g, nx, ny, nz);
       if (sd < 0.f) {
         vx = vy = vz = 0.f;
-      } else if (sd < p.h * 1.5f) {
-        const float gx = sample_sdf(sdf, p, nx + 1, ny, nz) - sample_sdf(sdf, p, nx - 1, ny, nz);
-        const float gy = sample_sdf(sdf, p, nx, ny + 1, nz) - sample_sdf(sdf, p, nx, ny - 1, nz);
-        const float gz = sample_sdf(sdf, p, nx, ny, nz + 1) - sample_sdf(sdf, p, nx, ny, nz - 1);
+      } else if (sd < g.h * 1.5f) {
+        const float gx = sample_sdf(sdf, g, nx + 1, ny, nz) - sample_sdf(sdf, g, nx - 1, ny, nz);
+        const float gy = sample_sdf(sdf, g, nx, ny + 1, nz) - sample_sdf(sdf, g, nx, ny - 1, nz);
+        const float gz = sample_sdf(sdf, g, nx, ny, nz + 1) - sample_sdf(sdf, g, nx, ny, nz - 1);
         const float gl = len3(gx, gy, gz);
         if (gl > 1e-6f) {
           const float ux = gx / gl, uy = gy / gl, uz = gz / gl;
@@ -735,15 +740,20 @@ __global__ __launch_bounds__(512) void k_grid(SubParams p, const float4* __restr
 // ---------------------------------------------------------------------------
 // rec_out may equal rec_in: every record slot written is data-dependent on the
 // load of the same slot (or not read at all), so updating in place is safe.
-template <bool FLIP>
-__global__ __launch_bounds__(256) void k_g2p(SubParams p, const float4* rec_in, float4* rec_out,
+// 4 waves per SIMD (<= 128 VGPRs): the gather's latency needs the occupancy
+template <bool FLIP, bool BATCH>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_g2p(SubParams p,
+                                                                                   const float4* rec_in,
+                                                                                   float4* rec_out,
                                              uint32_t* __restrict__ keys,
                                              const nb_material* __restrict__ mats,
                                              const uint32_t* __restrict__ sorted,
                                              const float4* __restrict__ grid_node,
                                              const float4* __restrict__ grid_old, const float* __restrict__ sdf) {
-  __shared__ nb_material s_mats[NB_MAX_MATERIALS];
-  {
+  // a plain engine's material table is staged in LDS; a batch's (up to
+  // kBatchMaxMaterials) is read through the caches
+  __shared__ nb_material s_mats[BATCH ? 1 : NB_MAX_MATERIALS];
+  if constexpr (!BATCH) {
     float* dst = reinterpret_cast<float*>(s_mats);
     const float* src = reinterpret_cast<const float*>(mats);
     const uint32_t nf = p.material_count * uint32_t(sizeof(nb_material) / 4);
@@ -753,6 +763,7 @@ __global__ __launch_bounds__(256) void k_g2p(SubParams p, const float4* rec_in, 
   const uint32_t k = xcd_chunk(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (k >= p.n) return;
   const uint32_t q = sorted ? sorted[k] : k;
+  const SceneGeo g = particle_geo<BATCH>(p, q);
   const float4* r = rec_in + size_t(q) * kRec;
   float4* ro = rec_out + size_t(q) * kRec;
   const float4 pp = r[R_P];
@@ -764,32 +775,32 @@ __global__ __launch_bounds__(256) void k_g2p(SubParams p, const float4* rec_in, 
   int base[3];
   Axis3 ax3[3];
   const float xs[3] = {xp, yp, zp};
-  const float inv_h = 1.f / p.h;
+  const float inv_h = g.inv_h;
 #pragma unroll
   for (int a = 0; a < 3; a++) {
-    float f = (xs[a] - p.wmin[a]) / p.h;
-    f = fminf(fmaxf(f, -4.f), float(p.grid_res[a]) + 4.f);
+    float f = (xs[a] - g.wmin[a]) / g.h;
+    f = fminf(fmaxf(f, -4.f), float(g.grid_res[a]) + 4.f);
     base[a] = int(floorf(f)) - 1;
-    axis_stencil(xs[a], p.wmin[a], p.h, inv_h, base[a], ax3[a]);
+    axis_stencil(xs[a], g.wmin[a], g.h, inv_h, base[a], ax3[a]);
   }
   // Node (i,j,k) takes part when it lies in the grid and carries mass >= 1e-10
   // and w = (wx*wy)*wz >= 1e-12 (shaders/mpm_g2p.comp:93-101).  Out-of-grid and
-  // massless nodes are folded in as zero velocity.  If no node of the wave has
-  // 0 < w < 1e-12 (the smallest positive per-axis weights bound every w from
-  // below), the weight cut removes nothing and the sums are contracted axis by
-  // axis; otherwise every node is visited with the cut applied.
+  // massless nodes are folded in as zero velocity.  If none of the particle's
+  // nodes has 0 < w < 1e-12 (the smallest positive per-axis weights bound every
+  // w from below), the weight cut removes nothing and the axis-by-axis sums
+  // stand; otherwise the particle's nodes are visited again with the cut.
   const bool tiny = (min_positive3(ax3[0].w) * min_positive3(ax3[1].w)) * min_positive3(ax3[2].w) < 1e-12f;
   // out-of-grid nodes read a clamped (valid) address; the caller drops them
   // together with the massless ones
   auto node_in = [&](int ix, int iy, int iz) -> bool {
     const int nx = base[0] + ix, ny = base[1] + iy, nz = base[2] + iz;
-    return nx >= 0 && nx < p.grid_res[0] && ny >= 0 && ny < p.grid_res[1] && nz >= 0 && nz < p.grid_res[2];
+    return nx >= 0 && nx < g.grid_res[0] && ny >= 0 && ny < g.grid_res[1] && nz >= 0 && nz < g.grid_res[2];
   };
   auto node_vel = [&](const float4* __restrict__ grid, int ix, int iy, int iz) -> float4 {
-    const uint32_t cx = uint32_t(min(max(base[0] + ix, 0), p.grid_res[0] - 1));
-    const uint32_t cy = uint32_t(min(max(base[1] + iy, 0), p.grid_res[1] - 1));
-    const uint32_t cz = uint32_t(min(max(base[2] + iz, 0), p.grid_res[2] - 1));
-    return grid[morton3(cx, cy, cz)];
+    const uint32_t cx = uint32_t(min(max(base[0] + ix, 0), g.grid_res[0] - 1));
+    const uint32_t cy = uint32_t(min(max(base[1] + iy, 0), g.grid_res[1] - 1));
+    const uint32_t cz = uint32_t(min(max(base[2] + iz, 0), g.grid_res[2] - 1));
+    return grid[node_id(g, cx, cy, cz)];
   };
 
   float vpx = 0.f, vpy = 0.f, vpz = 0.f, vox = 0.f, voy = 0.f, voz = 0.f;
@@ -801,8 +812,10 @@ __global__ __launch_bounds__(256) void k_g2p(SubParams p, const float4* rec_in, 
   float zw[3] = {ax3[2].w[0], ax3[2].w[1], ax3[2].w[2]};
   float zg[3] = {ax3[2].g[0], ax3[2].g[1], ax3[2].g[2]};
   float zd[3] = {ax3[2].dx[0], ax3[2].dx[1], ax3[2].dx[2]};
-  if (!__any(tiny)) {
-    // axis-by-axis contraction: x, then y, then z
+  {
+    // axis-by-axis contraction: x, then y, then z (every lane; a lane with a
+    // weight under the cut redoes its sums node by node below, so that each
+    // particle's result depends on its own data only)
     float wdx[3], wdy[3];
 #pragma unroll
     for (int i = 0; i < 3; i++) {
@@ -877,7 +890,17 @@ __global__ __launch_bounds__(256) void k_g2p(SubParams p, const float4* rec_in, 
       zg[0] = zg[1]; zg[1] = zg[2];
       zd[0] = zd[1]; zd[1] = zd[2];
     }
-  } else {
+  }
+  if (tiny) {
+    vpx = vpy = vpz = vox = voy = voz = 0.f;
+    vg = m3_zero();
+    Bn = m3_zero();
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      zw[i] = ax3[2].w[i];
+      zg[i] = ax3[2].g[i];
+      zd[i] = ax3[2].dx[i];
+    }
 #pragma unroll 1
     for (int iz = 0; iz < 3; iz++) {
       const float wz = zw[0], gz = zg[0], dxz = zd[0];
@@ -921,7 +944,7 @@ __global__ __launch_bounds__(256) void k_g2p(SubParams p, const float4* rec_in, 
   const float4 vel = r[R_V];
   uint32_t mid = __float_as_uint(vel.w);
   mid = mid > p.material_count - 1u ? p.material_count - 1u : mid;
-  const nb_material mat = s_mats[mid];
+  const nb_material mat = BATCH ? mats[mid] : s_mats[mid];
 
   const float4 rt = r[R_T], fa = r[R_FA], fb = r[R_FB];
   m3 F;
@@ -996,12 +1019,15 @@ __global__ __launch_bounds__(256) void k_g2p(SubParams p, const float4* rec_in, 
   if (jreset) { vnx *= 0.1f; vny *= 0.1f; vnz *= 0.1f; }
   float xn = xp + p.dt * vnx, yn = yp + p.dt * vny, zn = zp + p.dt * vnz;
 
+  // The geometry is read again for the tail rather than kept live through
+  // the SVD (a plain engine re-reads its kernel arguments).
+  const SceneGeo ge = particle_geo<BATCH>(p, q);
   // particle-level SDF push-out (shaders/mpm_g2p.comp:220-252)
   if (sdf) {
-    const int gx = p.grid_res[0] - 1, gy = p.grid_res[1] - 1, gz = p.grid_res[2] - 1;
-    float fx = fminf(fmaxf((xn - p.wmin[0]) / p.h, -4.f), float(p.grid_res[0]) + 4.f);
-    float fy = fminf(fmaxf((yn - p.wmin[1]) / p.h, -4.f), float(p.grid_res[1]) + 4.f);
-    float fz = fminf(fmaxf((zn - p.wmin[2]) / p.h, -4.f), float(p.grid_res[2]) + 4.f);
+    const int gx = ge.grid_res[0] - 1, gy = ge.grid_res[1] - 1, gz = ge.grid_res[2] - 1;
+    float fx = fminf(fmaxf((xn - ge.wmin[0]) / ge.h, -4.f), float(ge.grid_res[0]) + 4.f);
+    float fy = fminf(fmaxf((yn - ge.wmin[1]) / ge.h, -4.f), float(ge.grid_res[1]) + 4.f);
+    float fz = fminf(fmaxf((zn - ge.wmin[2]) / ge.h, -4.f), float(ge.grid_res[2]) + 4.f);
     const int ix = min(max(int(floorf(fx)), 0), gx);
     const int iy = min(max(int(floorf(fy)), 0), gy);
     const int iz = min(max(int(floorf(fz)), 0), gz);
@@ -1026,9 +1052,9 @@ __global__ __launch_bounds__(256) void k_g2p(SubParams p, const float4* rec_in, 
   }
   // world clamp at a 1.5h margin (shaders/mpm_g2p.comp:255-263)
   {
-    const float mg = p.h * 1.5f;
-    const float lox = p.wmin[0] + mg, loy = p.wmin[1] + mg, loz = p.wmin[2] + mg;
-    const float hix = p.wmax[0] - mg, hiy = p.wmax[1] - mg, hiz = p.wmax[2] - mg;
+    const float mg = ge.h * 1.5f;
+    const float lox = ge.wmin[0] + mg, loy = ge.wmin[1] + mg, loz = ge.wmin[2] + mg;
+    const float hix = ge.wmax[0] - mg, hiy = ge.wmax[1] - mg, hiz = ge.wmax[2] - mg;
     if (xn < lox) { xn = lox; vnx = fmaxf(vnx, 0.f); }
     if (xn > hix) { xn = hix; vnx = fminf(vnx, 0.f); }
     if (yn < loy) { yn = loy; vny = fmaxf(vny, 0.f); }
@@ -1044,7 +1070,7 @@ __global__ __launch_bounds__(256) void k_g2p(SubParams p, const float4* rec_in, 
   ro[R_T] = make_float4(tau.m[0][0], tau.m[1][1], tau.m[2][2], Fn.m[0][0]);
   ro[R_FA] = make_float4(Fn.m[0][1], Fn.m[0][2], Fn.m[1][0], Fn.m[1][1]);
   ro[R_FB] = make_float4(Fn.m[1][2], Fn.m[2][0], Fn.m[2][1], Fn.m[2][2]);
-  keys[q] = cell_key(p, xn, yn, zn);  // the next substep's bin key
+  keys[q] = cell_key(ge, xn, yn, zn);  // the next substep's bin key
 }
 
 // ---------------------------------------------------------------------------
@@ -1144,17 +1170,24 @@ struct ProfScope {
 
 void launch_bin(const DevState& d, const SubParams& p, uint32_t n_tiles, bool keys_valid, hipStream_t s, Prof* pf) {
   const uint32_t tg = tile_grid(n_tiles);
+  const bool batch = p.n_scenes > 0;
   { ProfScope ps(pf, 0, s); k_bin_clear<<<tg, 512, 0, s>>>(d.cell_count, d.act_tiles, d.counters); }
   {
     ProfScope ps(pf, 1, s);
+    const uint32_t nb = cdiv(p.n, 256);
     if (keys_valid)
-      k_bin_keys<false><<<cdiv(p.n, 256), 256, 0, s>>>(p, d.rec, d.keys, d.rank, d.cell_count, d.tile_flag);
+      k_bin_keys<false, false><<<nb, 256, 0, s>>>(p, d.rec, d.keys, d.rank, d.cell_count, d.tile_flag);
+    else if (batch)
+      k_bin_keys<true, true><<<nb, 256, 0, s>>>(p, d.rec, d.keys, d.rank, d.cell_count, d.tile_flag);
     else
-      k_bin_keys<true><<<cdiv(p.n, 256), 256, 0, s>>>(p, d.rec, d.keys, d.rank, d.cell_count, d.tile_flag);
+      k_bin_keys<true, false><<<nb, 256, 0, s>>>(p, d.rec, d.keys, d.rank, d.cell_count, d.tile_flag);
   }
   {
     ProfScope ps(pf, 2, s);
-    k_bin_mark<<<cdiv(n_tiles, 256), 256, 0, s>>>(d.tile_flag, d.tile_mark, d.blk_cnt, p.tile_dim, n_tiles);
+    if (batch)
+      k_bin_mark<true><<<cdiv(n_tiles, 256), 256, 0, s>>>(p, d.tile_flag, d.tile_mark, d.blk_cnt, n_tiles);
+    else
+      k_bin_mark<false><<<cdiv(n_tiles, 256), 256, 0, s>>>(p, d.tile_flag, d.tile_mark, d.blk_cnt, n_tiles);
   }
   {
     ProfScope ps(pf, 3, s);
@@ -1169,34 +1202,50 @@ void launch_bin(const DevState& d, const SubParams& p, uint32_t n_tiles, bool ke
 
 void launch_p2g(const DevState& d, const SubParams& p, uint32_t n_tiles, hipStream_t s, Prof* pf) {
   ProfScope ps(pf, 8, s);
-  k_p2g<<<tile_grid(n_tiles), kP2GThreads, 0, s>>>(p, d.rec, d.materials, d.cell_count, d.cell_end, d.sorted,
-                                                   d.act_tiles, d.tile_start, d.tile_total, d.counters,
-                                                   d.partial);
+  if (p.n_scenes > 0)
+    k_p2g<true><<<tile_grid(n_tiles), kP2GThreads, 0, s>>>(p, d.rec, d.materials, d.cell_count, d.cell_end, d.sorted,
+                                                           d.act_tiles, d.tile_start, d.tile_total, d.counters,
+                                                           d.partial);
+  else
+    k_p2g<false><<<tile_grid(n_tiles), kP2GThreads, 0, s>>>(p, d.rec, d.materials, d.cell_count, d.cell_end,
+                                                            d.sorted, d.act_tiles, d.tile_start, d.tile_total,
+                                                            d.counters, d.partial);
 }
 
 void launch_grid(const DevState& d, const SubParams& p, uint32_t n_tiles, bool from_partials, bool update,
                  const uint32_t* tiles, const uint32_t* count, hipStream_t s, Prof* pf) {
   ProfScope ps(pf, 9, s);
   const uint32_t g = tile_grid(n_tiles);
-  if (from_partials && update)
-    k_grid<true, true><<<g, 512, 0, s>>>(p, d.partial, d.tile_act, tiles, count, d.grid_node, d.grid_old, d.grid_pre,
-                                         d.sdf, d.colliders, d.forces);
-  else if (from_partials)
-    k_grid<true, false><<<g, 512, 0, s>>>(p, d.partial, d.tile_act, tiles, count, d.grid_node, d.grid_old, d.grid_pre,
-                                          d.sdf, d.colliders, d.forces);
-  else
-    k_grid<false, true><<<g, 512, 0, s>>>(p, d.partial, d.tile_act, tiles, count, d.grid_node, d.grid_old, d.grid_pre,
-                                          d.sdf, d.colliders, d.forces);
+#define NB_GRID_LAUNCH(FP, UP, BT)                                                                                  \
+  k_grid<FP, UP, BT><<<g, 512, 0, s>>>(p, d.partial, d.tile_act, tiles, count, d.grid_node, d.grid_old, d.grid_pre, \
+                                       d.sdf, d.colliders, d.forces)
+  if (p.n_scenes > 0) {
+    if (from_partials && update) NB_GRID_LAUNCH(true, true, true);
+    else if (from_partials) NB_GRID_LAUNCH(true, false, true);
+    else NB_GRID_LAUNCH(false, true, true);
+  } else {
+    if (from_partials && update) NB_GRID_LAUNCH(true, true, false);
+    else if (from_partials) NB_GRID_LAUNCH(true, false, false);
+    else NB_GRID_LAUNCH(false, true, false);
+  }
+#undef NB_GRID_LAUNCH
 }
 
 void launch_g2p(const DevState& d, const SubParams& p, bool use_sorted, hipStream_t s, Prof* pf, float4* rec_out) {
   ProfScope ps(pf, 10, s);
   const uint32_t* srt = use_sorted ? d.sorted : nullptr;
   float4* out = rec_out ? rec_out : d.rec;
-  if (p.flip_ratio > 0.01f)
-    k_g2p<true><<<cdiv(p.n, 256), 256, 0, s>>>(p, d.rec, out, d.keys, d.materials, srt, d.grid_node, d.grid_old, d.sdf);
-  else
-    k_g2p<false><<<cdiv(p.n, 256), 256, 0, s>>>(p, d.rec, out, d.keys, d.materials, srt, d.grid_node, d.grid_old, d.sdf);
+#define NB_G2P_LAUNCH(FL, BT) \
+  k_g2p<FL, BT><<<cdiv(p.n, 256), 256, 0, s>>>(p, d.rec, out, d.keys, d.materials, srt, d.grid_node, d.grid_old, d.sdf)
+  const bool flip = p.flip_ratio > 0.01f;
+  if (p.n_scenes > 0) {
+    if (flip) NB_G2P_LAUNCH(true, true);
+    else NB_G2P_LAUNCH(false, true);
+  } else {
+    if (flip) NB_G2P_LAUNCH(true, false);
+    else NB_G2P_LAUNCH(false, false);
+  }
+#undef NB_G2P_LAUNCH
 }
 
 void launch_sdf_sphere(float* sdf, const SubParams& p, float r, float cx, float cy, float cz, hipStream_t s) {

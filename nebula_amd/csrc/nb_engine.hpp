@@ -48,6 +48,23 @@ struct FieldPtrs {
 // counters[] slots (device)
 enum : int { CNT_ACTIVE = 0, CNT_NODE_TILES = 1, CNT_ALL_TILES = 2, CNT_SLOTS = 4 };
 
+// Geometry of one scene's grid.  A plain engine holds one scene, described by
+// SubParams' own fields (key_base 0).  A batch holds a table of scenes, each
+// with its own Morton cube placed at key_base (a multiple of 512) in the
+// shared key space, so that each scene's tiles are consecutive tile ids;
+// positions stay in the scene's own coordinates.
+struct SceneGeo {
+  float h, inv_h;      // cell size and its reciprocal
+  float wmin[3];
+  float wmax[3];
+  int32_t grid_res[3];
+  uint32_t cube_res;
+  uint32_t tile_dim;   // tiles per axis of the scene's cube
+  uint32_t key_base;   // first cell key of the scene
+  uint32_t pad[2];
+};
+static_assert(sizeof(SceneGeo) == 64, "one scene record per 64 bytes");
+
 // Substep constants, passed by value to every kernel (the role MPMSimPC plays
 // for the reference's shaders, src/core/MPMSimPC.h:14-73).
 struct SubParams {
@@ -65,7 +82,15 @@ struct SubParams {
   uint32_t force_count;
   uint32_t write_old;  // write the pre-force velocity (FLIP, or pass-level export)
   uint32_t corrected_svd;  // 1: singular vectors as columns (see nb_mpm_knobs)
+  // batch (n_scenes > 0): the scene table, the scene of every tile and of
+  // every particle; the single-scene fields above are then unused
+  uint32_t n_scenes;
+  const SceneGeo* scenes;
+  const uint32_t* tile_scene;
+  const uint32_t* pscene;
 };
+// materials a batch's table may hold (a plain engine: NB_MAX_MATERIALS)
+constexpr uint32_t kBatchMaxMaterials = NB_MAX_BATCH_MATERIALS;
 
 struct DevState {
   float4* field[NB_NUM_FIELDS];  // reference layout, what the ABI exposes
@@ -141,12 +166,17 @@ struct AdjState {
   float4* coef;         // capacity * 6: per-particle grid-adjoint coefficients
   float4* partial_adj;  // cap_tiles * kBoxNodes: per-tile grid-adjoint boxes
   float4* grid_adj;     // NT * 512: (dL/dp, dL/dm) per node
-  float* pgrad_w;       // NB_MAX_MATERIALS x 4 partials per block of the G2P backward, slot-major
-  float* pgrad;         // NB_MAX_MATERIALS x 4: running d/d(mu, lambda, M, q_c)
+  // material-parameter partials of the G2P backward: per block, a list of
+  // up to kPgEntries (material, d/d(mu, lambda, M, q_c)) entries and its length
+  uint32_t* pg_mid;     // blocks * kPgEntries
+  float4* pg_val;       // blocks * kPgEntries
+  uint32_t* pg_cnt;     // blocks
+  float* pgrad;         // materials x 4: running d/d(mu, lambda, M, q_c)
 };
 // One substep backwards on the recomputed forward of state d.rec (bins, P2G
 // partials and grid velocities already in d): adj_in is the adjoint of the
 // state after the substep, adj_out receives the adjoint of d.rec.
+constexpr uint32_t kPgEntries = 256;  // 4 waves x at most 64 material groups each
 void launch_adjoint_substep(const DevState& d, const AdjState& a, const SubParams& p, uint32_t n_tiles,
                             const float4* adj_in, float4* adj_out, hipStream_t s);
 

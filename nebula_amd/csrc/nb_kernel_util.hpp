@@ -65,14 +65,55 @@ __device__ __forceinline__ uint32_t xcd_chunk(uint32_t b, uint32_t nb) {
   return x < r ? x * (q + 1u) + i : r * (q + 1u) + (x - r) * q + i;
 }
 
+// ---- scene geometry ----
+// the plain engine's one scene, from the kernel arguments (stays in scalar registers)
+__device__ __forceinline__ SceneGeo geo_of(const SubParams& p) {
+  SceneGeo g;
+  g.h = p.h;
+  g.inv_h = 1.f / p.h;
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    g.wmin[a] = p.wmin[a];
+    g.wmax[a] = p.wmax[a];
+    g.grid_res[a] = p.grid_res[a];
+  }
+  g.cube_res = p.cube_res;
+  g.tile_dim = p.tile_dim;
+  g.key_base = 0u;
+  g.pad[0] = g.pad[1] = 0u;
+  return g;
+}
+// the scene of tile T (uniform across a block) and of particle q
+template <bool BATCH>
+__device__ __forceinline__ SceneGeo tile_geo(const SubParams& p, uint32_t T) {
+  if constexpr (BATCH) return p.scenes[p.tile_scene[T]];
+  else return geo_of(p);
+}
+template <bool BATCH>
+__device__ __forceinline__ SceneGeo particle_geo(const SubParams& p, uint32_t q) {
+  if constexpr (BATCH) return p.scenes[p.pscene[q]];
+  else return geo_of(p);
+}
+// tile coordinates within the scene's cube, and the tile id of coordinates
+__device__ __forceinline__ void tile_coords(const SceneGeo& g, uint32_t T, int& tx, int& ty, int& tz) {
+  tile_coords(T - (g.key_base >> kTileBits), tx, ty, tz);
+}
+__device__ __forceinline__ uint32_t tile_id(const SceneGeo& g, int tx, int ty, int tz) {
+  return (g.key_base >> kTileBits) + morton3(uint32_t(tx), uint32_t(ty), uint32_t(tz));
+}
+// index of node (x, y, z) of the scene in the Morton-ordered node arrays
+__device__ __forceinline__ uint32_t node_id(const SceneGeo& g, uint32_t x, uint32_t y, uint32_t z) {
+  return g.key_base + morton3(x, y, z);
+}
+
 // cell key of a position: shaders/mpm_common.glsl:190-195 (IEEE quotient,
 // clamp to [0, gridRes-1], truncation; NaN maps to 0 as fmaxf returns the
-// non-NaN operand)
-__device__ __forceinline__ uint32_t cell_key(const SubParams& p, float x, float y, float z) {
-  float lx = fminf(fmaxf((x - p.wmin[0]) / p.h, 0.f), float(p.grid_res[0]) - 1.f);
-  float ly = fminf(fmaxf((y - p.wmin[1]) / p.h, 0.f), float(p.grid_res[1]) - 1.f);
-  float lz = fminf(fmaxf((z - p.wmin[2]) / p.h, 0.f), float(p.grid_res[2]) - 1.f);
-  return morton3(uint32_t(lx), uint32_t(ly), uint32_t(lz));
+// non-NaN operand), offset by the scene's key base
+__device__ __forceinline__ uint32_t cell_key(const SceneGeo& g, float x, float y, float z) {
+  float lx = fminf(fmaxf((x - g.wmin[0]) / g.h, 0.f), float(g.grid_res[0]) - 1.f);
+  float ly = fminf(fmaxf((y - g.wmin[1]) / g.h, 0.f), float(g.grid_res[1]) - 1.f);
+  float lz = fminf(fmaxf((z - g.wmin[2]) / g.h, 0.f), float(g.grid_res[2]) - 1.f);
+  return g.key_base + morton3(uint32_t(lx), uint32_t(ly), uint32_t(lz));
 }
 
 
@@ -109,8 +150,9 @@ __device__ __forceinline__ void axis_stencil(float xp, float wmin, float h, floa
   axis_stencil_f((xp - wmin) / h, xp, wmin, h, inv_h, first_node, a);
 }
 
-__device__ __forceinline__ float sample_sdf(const float* sdf, const SubParams& p, int x, int y, int z) {
-  if (x < 0 || y < 0 || z < 0 || x >= p.grid_res[0] || y >= p.grid_res[1] || z >= p.grid_res[2]) return 1e6f;
+// the dense SDF (plain engine only: Morton order over its cube)
+__device__ __forceinline__ float sample_sdf(const float* sdf, const SceneGeo& g, int x, int y, int z) {
+  if (x < 0 || y < 0 || z < 0 || x >= g.grid_res[0] || y >= g.grid_res[1] || z >= g.grid_res[2]) return 1e6f;
   return sdf[morton3(x, y, z)];
 }
 

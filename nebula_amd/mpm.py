@@ -375,7 +375,23 @@ EXPORTED_SYMBOLS = (
     "nb_mpm_profile_substeps", "nb_mpm_calibration_copy", "nb_mpm_active_node_count", "nb_mpm_active_tile_count",
     "nb_mpm_adjoint_begin", "nb_mpm_forward_recorded", "nb_mpm_set_adjoint", "nb_mpm_get_adjoint",
     "nb_mpm_adjoint_device_ptr", "nb_mpm_backward", "nb_mpm_adjoint_end",
+    "nb_mpm_create_batch", "nb_mpm_scene_count", "nb_mpm_scene_info",
 )
+
+MAX_BATCH_MATERIALS = 1024
+
+
+class BatchScene(C.Structure):
+    """One scene of a batch (nb_mpm_scene): its domain [0, domain_size],
+    cell size and number of particle rows."""
+    _fields_ = [("domain_size", C.c_float * 3), ("cell_size", C.c_float), ("particle_count", C.c_uint32),
+                ("reserved", C.c_uint32 * 3)]
+
+    def __init__(self, domainSize=(1.0, 1.0, 1.0), cellSize=1.0 / 64, particleCount=0):
+        super().__init__()
+        self.domain_size[:] = [float(x) for x in domainSize]
+        self.cell_size = float(cellSize)
+        self.particle_count = int(particleCount)
 
 
 class NebulaError(RuntimeError):
@@ -392,6 +408,9 @@ def lib():
         vp, fp, up = C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_uint32)
         L.nb_last_error.restype = C.c_char_p
         L.nb_mpm_create.argtypes = [C.POINTER(MPMConfig), vp, C.POINTER(vp)]
+        L.nb_mpm_create_batch.argtypes = [C.POINTER(BatchScene), C.c_uint32, vp, C.POINTER(vp)]
+        L.nb_mpm_scene_count.argtypes = [vp, up]
+        L.nb_mpm_scene_info.argtypes = [vp, C.c_uint32, up, up, up]
         L.nb_mpm_config_default.argtypes = [C.POINTER(MPMConfig)]
         L.nb_mpm_config_default.restype = None
         L.nb_mpm_knobs_default.argtypes = [C.POINTER(_Knobs)]
@@ -472,6 +491,32 @@ class MPMEngine:
         self._h = h
         self._cfg = cfg
         self._n_cells = self.totalCells()
+
+    @classmethod
+    def batch(cls, scenes: Sequence[BatchScene], stream: Optional[int] = None) -> "MPMEngine":
+        """An engine over a batch of independent scenes (nb_mpm_create_batch):
+        scene s owns particle rows [first, first + particle_count) in table
+        order, positions in its own coordinates."""
+        e = cls()
+        arr = (BatchScene * len(scenes))(*scenes)
+        h = C.c_void_p()
+        _check(lib().nb_mpm_create_batch(arr, len(scenes), C.c_void_p(stream) if stream else None, C.byref(h)))
+        e._h = h
+        e._cfg = e.config()
+        e._n_cells = e.totalCells()
+        return e
+
+    def sceneCount(self) -> int:
+        n = C.c_uint32()
+        _check(lib().nb_mpm_scene_count(self._h, C.byref(n)))
+        return n.value
+
+    def sceneInfo(self, scene: int):
+        """(grid resolution, first particle row, particle count) of one scene."""
+        res = (C.c_uint32 * 3)()
+        f, c = C.c_uint32(), C.c_uint32()
+        _check(lib().nb_mpm_scene_info(self._h, scene, res, C.byref(f), C.byref(c)))
+        return tuple(res), f.value, c.value
 
     def cleanup(self):
         if self._h is not None:
@@ -763,10 +808,10 @@ class MPMEngine:
     def backward(self):
         """Adjoint of the recorded run: returns dL/d(mu, lambda, M_friction,
         q_cohesion) per material, (material_count, 4)."""
-        cap = 4 * 16
-        buf = np.zeros(cap, np.float32)
-        _check(lib().nb_mpm_backward(self._h, _f(buf), cap))
-        return buf.reshape(16, 4)
+        n = max(len(getattr(self, "_materials", [])), 1)
+        buf = np.zeros(4 * n, np.float32)
+        _check(lib().nb_mpm_backward(self._h, _f(buf), buf.size))
+        return buf.reshape(n, 4)
 
     def adjointEnd(self):
         _check(lib().nb_mpm_adjoint_end(self._h))

@@ -239,3 +239,82 @@ def make_engine(scene: Scene, stream=None):
     eng.corrected_svd = scene.corrected_svd
     eng.numSubsteps = 1
     return eng
+
+
+def make_batch_engine(scenes: List[Scene], stream=None):
+    """One engine over a batch of the scenes (nb_mpm_create_batch): each
+    scene's particles, in table order, with its materials appended to the
+    shared table (material ids offset accordingly).  The scenes must agree on
+    dt, transfer mode, forces and SVD convention; each keeps its own domain
+    and cell size.  No SDF or colliders."""
+    from .mpm import BatchScene, MPMEngine
+
+    s0 = scenes[0]
+    for sc in scenes:
+        if sc.sdf is not None or sc.colliders.count():
+            raise ValueError(f"scene {sc.name}: a batch takes no SDF or colliders")
+        if sc.flip_ratio != s0.flip_ratio or sc.corrected_svd != s0.corrected_svd:
+            raise ValueError("the scenes of a batch share the transfer mode and SVD convention")
+    specs = [BatchScene(tuple(sc.cfg.domainSize), sc.cfg.cellSize, sc.n) for sc in scenes]
+    eng = MPMEngine.batch(specs, stream)
+    mats, fields = [], []
+    for sc in scenes:
+        f = sc.fields.copy()
+        ids = f[1, :, 3].view(np.uint32) + np.uint32(len(mats))
+        f[1, :, 3] = ids.view(np.float32)
+        fields.append(f)
+        mats.extend(sc.materials)
+    eng.setMaterials(mats)
+    eng.setForces(s0.forces)
+    eng.setState(np.concatenate(fields, axis=1))
+    eng.flip_ratio = s0.flip_ratio
+    eng.corrected_svd = s0.corrected_svd
+    eng.numSubsteps = 1
+    return eng
+
+
+def falling_box(name: str, grid: int, n: int, E: float, nu: float, model: int, rng: np.random.Generator,
+                h: float = 1.0 / 64, dt: float = 1e-4, flip_ratio: float = -1.0, rho0: float = 1000.0,
+                corrected_svd: bool = False) -> Scene:
+    """A box of n particles (8 per cell, jittered) at a seeded place and
+    velocity in the upper part of a grid^3-cell domain of cell size h,
+    falling under gravity; one material of the given model (Von Mises
+    q_max = 0.02 E, Drucker-Prager friction 30 deg, no cohesion)."""
+    cells_n = -(-n // 8)
+    e = max(1, round(cells_n ** (1.0 / 3.0)))
+    dims = [e, e, -(-cells_n // (e * e))]
+    lo = [int(rng.integers(3, max(4, grid - d - 3))) for d in dims]
+    lo[1] = max(lo[1], grid // 3) if grid - dims[1] - 3 > grid // 3 else lo[1]
+    g = [np.arange(lo[a], lo[a] + dims[a]) for a in range(3)]
+    cx, cy, cz = np.meshgrid(*g, indexing="ij")
+    cells = np.stack([cx.ravel(), cy.ravel(), cz.ravel()], axis=1)[:cells_n]
+    pos = _seed_cells(cells, h, 8, rng)[:n]
+    v0 = rng.uniform(-1.0, 1.0, 3).astype(np.float32)
+    vel = np.broadcast_to(v0, (n, 3)).astype(np.float32)
+    f = _fields(pos, vel, h ** 3 / 8)
+    cfg = MPMConfig(nx=0, ny=0, nz=0, domainSize=(grid * h,) * 3, cellSize=h, maxParticles=n, E=E, nu=nu,
+                    rho0=rho0)
+    M = math.tan(math.radians(30)) if model == MaterialModel.DRUCKER_PRAGER else 0.0
+    mats = [_material(E, nu, rho0, model, M=M, qc=0.0, qmax=0.02 * E)]
+    return Scene(name, cfg, mats, f, dt=dt, flip_ratio=flip_ratio, substeps=200,
+                 forces=[GravityForce((0.0, 1.0, 0.0), -9.8)], corrected_svd=corrected_svd)
+
+
+def batch_mix_256(seed: int = 4, n_scenes: int = 256) -> List[Scene]:
+    """The 256-scene mix: particle counts lognormal(ln 16000, 0.8) clipped to
+    [2000, 64000], grid edges 32..96 cells (cell size 1/64), materials drawn
+    uniformly from elastic / Von Mises / Drucker-Prager with E in [1e3, 1e5]
+    and nu in [0.1, 0.4], each scene a falling box; dt = 1e-4, APIC."""
+    rng = np.random.default_rng(seed)
+    models = (MaterialModel.ELASTIC, MaterialModel.VON_MISES, MaterialModel.DRUCKER_PRAGER)
+    out = []
+    for s in range(n_scenes):
+        n = int(np.clip(rng.lognormal(math.log(16000.0), 0.8), 2000, 64000))
+        cells = -(-n // 8)
+        edge = int(math.ceil(cells ** (1.0 / 3.0)))
+        grid = int(rng.integers(max(32, edge + 10), 97))
+        model = models[int(rng.integers(0, 3))]
+        E = float(rng.uniform(1e3, 1e5))
+        nu = float(rng.uniform(0.1, 0.4))
+        out.append(falling_box(f"mix{s}", grid, n, E, nu, model, rng))
+    return out
