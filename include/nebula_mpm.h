@@ -196,6 +196,37 @@ typedef struct nb_mpm_scene {
   uint32_t reserved[3];
 } nb_mpm_scene;
 int nb_mpm_create_batch(const nb_mpm_scene* scenes, uint32_t n_scenes, void* hip_stream, nb_mpm** out);
+/* ---- Launch shapes, the autotuner and graph replay ----
+ * No counterpart in the reference, whose dispatch sizes are fixed in its
+ * shaders.  tile_blocks: blocks of the per-tile kernels (bin, P2G, grid; a
+ * block walks tiles in strides); g2p_block: threads per G2P block (64, 128
+ * or 256); p2g_item: particles one P2G thread takes from a cell before the
+ * cell is split into more work items (4..32).  tile_blocks and g2p_block
+ * never change results; p2g_item changes the order of P2G's sums, so results
+ * move within rounding.  ms_per_substep: what nb_mpm_autotune measured for
+ * the choice (0 when set by hand). */
+typedef struct nb_mpm_tuning {
+  uint32_t tile_blocks;
+  uint32_t g2p_block;
+  uint32_t p2g_item;
+  float ms_per_substep;
+} nb_mpm_tuning;
+int nb_mpm_get_tuning(const nb_mpm* h, nb_mpm_tuning* t);
+int nb_mpm_set_tuning(nb_mpm* h, const nb_mpm_tuning* t);
+/* Time candidate shapes on the engine's current state (trial_substeps
+ * substeps of dt_sub each, the particle state restored after every trial)
+ * and keep the fastest: a coordinate search over tile_blocks, g2p_block
+ * and, with NB_TUNE_ROUNDING in flags, p2g_item.  The state is left as it
+ * was; the choice is returned in *chosen (may be NULL). */
+#define NB_TUNE_ROUNDING 1u
+int nb_mpm_autotune(nb_mpm* h, float dt_sub, uint32_t trial_substeps, uint32_t flags, nb_mpm_tuning* chosen);
+/* n substeps as replays of one captured hipGraph of the substep's launches
+ * (captured on first use, again whenever dt_sub, the particle count, the
+ * material/force/collider counts, the knobs or the tuning change).  Same
+ * results as nb_mpm_substeps, bit for bit; for short substeps where launch
+ * overhead matters (batches of small scenes). */
+int nb_mpm_graph_substeps(nb_mpm* h, float dt_sub, uint32_t n);
+
 /* Scenes of a handle (0 for a plain engine). */
 int nb_mpm_scene_count(const nb_mpm* h, uint32_t* n);
 /* Grid resolution, first particle row and particle count of one scene. */

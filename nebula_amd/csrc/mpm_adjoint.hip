@@ -960,7 +960,10 @@ __global__ __launch_bounds__(256) void k_pgrad_reduce(const uint32_t* __restrict
 // launchers
 // ---------------------------------------------------------------------------
 static inline uint32_t cdiv_a(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
-static inline uint32_t tile_grid_a(uint32_t n) { return n < 2048u ? (n ? n : 1u) : 2048u; }
+static inline uint32_t tile_grid_a(uint32_t n, uint32_t cap) {
+  const uint32_t c = cap ? cap : 2048u;
+  return n < c ? (n ? n : 1u) : c;
+}
 
 template <bool BATCH>
 static void adjoint_kernels(const DevState& d, const AdjState& a, const SubParams& p, uint32_t n_tiles,
@@ -968,9 +971,9 @@ static void adjoint_kernels(const DevState& d, const AdjState& a, const SubParam
   const uint32_t nblk = cdiv_a(p.n, 256);
   k_g2p_bwd<BATCH><<<nblk, 256, 0, s>>>(p, d.rec, adj_in, adj_out, a.coef, a.pg_mid, a.pg_val, a.pg_cnt,
                                          d.materials, d.sorted, d.grid_node, d.sdf);
-  k_adj_scatter<BATCH><<<tile_grid_a(n_tiles), kAdjThreads, 0, s>>>(p, d.rec, a.coef, d.cell_count, d.cell_end,
+  k_adj_scatter<BATCH><<<tile_grid_a(n_tiles, p.tile_blocks), kAdjThreads, 0, s>>>(p, d.rec, a.coef, d.cell_count, d.cell_end,
                                                                     d.sorted, d.act_tiles, d.counters, a.partial_adj);
-  k_grid_bwd<BATCH><<<tile_grid_a(n_tiles), 512, 0, s>>>(p, d.grid_pre, a.partial_adj, d.tile_act, d.node_tiles,
+  k_grid_bwd<BATCH><<<tile_grid_a(n_tiles, p.tile_blocks), 512, 0, s>>>(p, d.grid_pre, a.partial_adj, d.tile_act, d.node_tiles,
                                                           d.counters + CNT_NODE_TILES, a.grid_adj, d.sdf, d.forces);
   k_p2g_bwd<BATCH><<<nblk, 256, 0, s>>>(p, d.rec, adj_out, d.materials, d.sorted, a.grid_adj);
 }

@@ -81,6 +81,12 @@ struct nb_mpm {
   uint32_t* d_tile_scene = nullptr;
   uint32_t* d_pscene = nullptr;
   bool batch() const { return n_scenes > 0; }
+
+  // launch shapes and the captured substep (nb_mpm_graph_substeps)
+  nb_mpm_tuning tuning{2048u, 256u, 12u, 0.f};
+  hipGraphExec_t graph_exec = nullptr;
+  SubParams graph_p{};
+  DevState graph_d{};
   uint32_t max_materials() const { return batch() ? kBatchMaxMaterials : NB_MAX_MATERIALS; }
 
   // reverse mode: checkpoints of a recorded run of n substeps every `seg`,
@@ -127,7 +133,8 @@ struct nb_mpm {
   }
 
   SubParams params(float dt, bool write_old) const {
-    SubParams p{};
+    SubParams p;
+    std::memset(&p, 0, sizeof(p));  // padding too: captured substeps are matched bytewise
     p.dt = dt;
     p.h = cfg.cell_size;
     for (int a = 0; a < 3; a++) {
@@ -144,6 +151,9 @@ struct nb_mpm {
     p.force_count = force_count;
     p.write_old = write_old ? 1u : 0u;
     p.corrected_svd = knobs.corrected_svd ? 1u : 0u;
+    p.tile_blocks = tuning.tile_blocks;
+    p.g2p_block = tuning.g2p_block;
+    p.p2g_item = tuning.p2g_item;
     if (batch()) {
       p.n_scenes = n_scenes;
       p.scenes = d_scenes;
@@ -574,6 +584,7 @@ extern "C" {
 int nb_mpm_destroy(nb_mpm* h) {
   if (!h) return NB_OK;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
+  if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
   for (void* p : h->allocs) (void)hipFree(p);
   for (void* p : h->adj.allocs) (void)hipFree(p);
   if (h->d.sdf) (void)hipFree(h->d.sdf);
@@ -602,6 +613,150 @@ int nb_mpm_sync(nb_mpm* h) {
   if (!h) return fail(NB_ERR_ARG, "null handle");
   NB_HIP(hipStreamSynchronize(h->stream));
   NB_HIP(hipGetLastError());
+  return NB_OK;
+}
+
+int nb_mpm_get_tuning(const nb_mpm* h, nb_mpm_tuning* t) {
+  if (!h || !t) return fail(NB_ERR_ARG, "null argument");
+  *t = h->tuning;
+  return NB_OK;
+}
+
+int nb_mpm_set_tuning(nb_mpm* h, const nb_mpm_tuning* t) {
+  if (!h || !t) return fail(NB_ERR_ARG, "null argument");
+  if (t->tile_blocks < 1u || t->tile_blocks > 65536u) return fail(NB_ERR_RANGE, "tile_blocks %u not in [1, 65536]", t->tile_blocks);
+  if (t->g2p_block != 64u && t->g2p_block != 128u && t->g2p_block != 256u)
+    return fail(NB_ERR_RANGE, "g2p_block %u: 64, 128 or 256", t->g2p_block);
+  if (t->p2g_item < 4u || t->p2g_item > 32u) return fail(NB_ERR_RANGE, "p2g_item %u not in [4, 32]", t->p2g_item);
+  h->tuning = *t;
+  return NB_OK;
+}
+
+int nb_mpm_graph_substeps(nb_mpm* h, float dt_sub, uint32_t n) {
+  if (!h) return fail(NB_ERR_ARG, "null handle");
+  if (n == 0 || h->n_live == 0) return NB_OK;
+  int rc = sync_in(h);
+  // the captured substep reads the keys the previous G2P wrote
+  if (rc == NB_OK && !h->keys_valid) {
+    rc = enqueue_substep(h, dt_sub, nullptr);
+    n--;
+  }
+  if (rc == NB_OK && n) {
+    const SubParams p = h->params(dt_sub, h->knobs.flip_ratio > 0.01f);
+    const DevState dv = h->dev();
+    if (!h->graph_exec || std::memcmp(&p, &h->graph_p, sizeof(p)) != 0 ||
+        std::memcmp(&dv, &h->graph_d, sizeof(dv)) != 0) {
+      if (h->graph_exec) {
+        (void)hipGraphExecDestroy(h->graph_exec);
+        h->graph_exec = nullptr;
+      }
+      NB_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+      rc = enqueue_substep(h, dt_sub, nullptr);
+      hipGraph_t g = nullptr;
+      const hipError_t e = hipStreamEndCapture(h->stream, &g);
+      if (rc != NB_OK || e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc != NB_OK ? rc : fail(NB_ERR_HIP, "hipStreamEndCapture failed: %s", hipGetErrorString(e));
+      }
+      const hipError_t ei = hipGraphInstantiate(&h->graph_exec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ei != hipSuccess) {
+        h->graph_exec = nullptr;
+        return fail(NB_ERR_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(ei));
+      }
+      h->graph_p = p;
+      h->graph_d = dv;
+    }
+    for (uint32_t i = 0; i < n; i++) NB_HIP(hipGraphLaunch(h->graph_exec, h->stream));
+  }
+  if (rc == NB_OK) rc = sync_out(h);
+  return rc;
+}
+
+int nb_mpm_autotune(nb_mpm* h, float dt_sub, uint32_t trial_substeps, uint32_t flags, nb_mpm_tuning* chosen) {
+  if (!h) return fail(NB_ERR_ARG, "null handle");
+  if (h->n_live == 0) return fail(NB_ERR_STATE, "no particles to time");
+  if (h->adj.on) return fail(NB_ERR_STATE, "not during a recorded run (nb_mpm_adjoint_end first)");
+  const uint32_t reps = trial_substeps ? trial_substeps : 3u;
+  int rc = sync_in(h);
+  if (rc != NB_OK) return rc;
+  const size_t rec_bytes = size_t(h->n_live) * kRec * sizeof(float4), key_bytes = size_t(h->n_live) * 4;
+  void* save_rec = nullptr;
+  void* save_keys = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  auto release = [&]() {
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    if (save_rec) (void)hipFree(save_rec);
+    if (save_keys) (void)hipFree(save_keys);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+  };
+  if (hipMalloc(&save_rec, rec_bytes) != hipSuccess || hipMalloc(&save_keys, key_bytes) != hipSuccess ||
+      hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    release();
+    return fail(NB_ERR_HIP, "autotune: allocation failed");
+  }
+  const bool keys0 = h->keys_valid;
+  const nb_mpm_tuning start = h->tuning;
+  auto copy = [&](void* dst, const void* src, size_t b) {
+    return hipMemcpyAsync(dst, src, b, hipMemcpyDeviceToDevice, h->stream) == hipSuccess;
+  };
+  bool ok = copy(save_rec, h->d.rec, rec_bytes) && copy(save_keys, h->d.keys, key_bytes);
+  auto restore = [&]() {
+    ok = ok && copy(h->d.rec, save_rec, rec_bytes) && copy(h->d.keys, save_keys, key_bytes);
+    h->keys_valid = keys0;
+  };
+  // one trial: from the saved state, one warm substep, then reps timed
+  auto trial = [&](const nb_mpm_tuning& t) -> float {
+    h->tuning = t;
+    restore();
+    int r = enqueue_substep(h, dt_sub, nullptr);
+    ok = ok && r == NB_OK && hipEventRecord(e0, h->stream) == hipSuccess;
+    for (uint32_t i = 0; i < reps && ok; i++) ok = enqueue_substep(h, dt_sub, nullptr) == NB_OK;
+    ok = ok && hipEventRecord(e1, h->stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
+    float ms = 0.f;
+    ok = ok && hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+    return ok ? ms / float(reps) : 1e30f;
+  };
+  nb_mpm_tuning best = start;
+  float best_ms = trial(best);
+  auto consider = [&](nb_mpm_tuning t) {
+    const float ms = trial(t);
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = t;
+    }
+  };
+  for (uint32_t tb : {256u, 512u, 1024u, 2048u, 4096u, 8192u}) {
+    nb_mpm_tuning t = best;
+    t.tile_blocks = tb;
+    if (tb != best.tile_blocks) consider(t);
+  }
+  for (uint32_t gb : {64u, 128u, 256u}) {
+    nb_mpm_tuning t = best;
+    t.g2p_block = gb;
+    if (gb != best.g2p_block) consider(t);
+  }
+  if (flags & NB_TUNE_ROUNDING) {
+    for (uint32_t it : {6u, 8u, 12u, 16u, 24u}) {
+      nb_mpm_tuning t = best;
+      t.p2g_item = it;
+      if (it != best.p2g_item) consider(t);
+    }
+  }
+  restore();
+  release();
+  if (!ok) {
+    h->tuning = start;
+    return fail(NB_ERR_HIP, "autotune: a trial failed");
+  }
+  best.ms_per_substep = best_ms;
+  h->tuning = best;
+  // the records and keys are back; bins and grid are those of the last trial
+  h->bins_valid = false;
+  h->grid_mode = GridMode::NONE;
+  h->old_valid = false;
+  if (chosen) *chosen = best;
   return NB_OK;
 }
 

@@ -319,7 +319,7 @@ __global__ __launch_bounds__(512) void k_bin_stabilize(uint32_t n, const uint32_
 // ---------------------------------------------------------------------------
 // P2G: one block per occupied tile.  The tile's occupied cells are compacted
 // (in Morton order) onto the block's threads as work items: a cell of up to
-// kChunk particles is one item, a more crowded cell is split into up to
+// p.p2g_item particles (tuned, 12 by default) is one item, a more crowded cell is split into up to
 // kMaxChunks items of consecutive particles (so one crowded cell does not hold
 // its wave for the whole walk).  A thread walks its item's particles in bin
 // order, reading the first 96 bytes of each record, and sums their
@@ -330,11 +330,11 @@ __global__ __launch_bounds__(512) void k_bin_stabilize(uint32_t n, const uint32_
 // is exact and the order of the sums is fixed.
 constexpr int kP2GThreads = 256;
 constexpr uint32_t kIdxStage = 4096;  // particle indices staged in LDS per pass
-constexpr uint32_t kChunk = 12;
+
 constexpr uint32_t kMaxChunks = 4;
 
-__device__ __forceinline__ uint32_t p2g_chunks(uint32_t count) {
-  return count == 0u ? 0u : min((count + kChunk - 1u) / kChunk, kMaxChunks);
+__device__ __forceinline__ uint32_t p2g_chunks(uint32_t count, uint32_t item) {
+  return count == 0u ? 0u : min((count + item - 1u) / item, kMaxChunks);
 }
 // item length groups, longest first: >= 12, 10-11, 8-9, 6-7, <= 5 particles
 constexpr uint32_t kGroups = 5;
@@ -385,7 +385,7 @@ __global__ __launch_bounds__(kP2GThreads, 2) void k_p2g(SubParams p, const float
     if (threadIdx.x == 0) s_layers = 0u;
     const uint32_t ca = 2u * threadIdx.x, cb = ca + 1u;
     const uint32_t na_ = cell_count[tbase + ca], nb_ = cell_count[tbase + cb];
-    const uint32_t qa = p2g_chunks(na_), qb = p2g_chunks(nb_);
+    const uint32_t qa = p2g_chunks(na_, p.p2g_item), qb = p2g_chunks(nb_, p.p2g_item);
     const uint32_t ga = p2g_group(na_, qa), gb = p2g_group(nb_, qb);
     const uint64_t mine = (uint64_t(qa) << (12 * ga)) + (uint64_t(qb) << (12 * gb));
     uint64_t tot64;
@@ -432,7 +432,7 @@ __global__ __launch_bounds__(kP2GThreads, 2) void k_p2g(SubParams p, const float
       uint32_t kbeg = 0, kend = 0;
       if (have) {
         const uint32_t cnt = cell_count[tbase + c], end = cell_end[tbase + c];
-        const uint32_t per = (cnt + p2g_chunks(cnt) - 1u) / p2g_chunks(cnt);
+        const uint32_t per = (cnt + p2g_chunks(cnt, p.p2g_item) - 1u) / p2g_chunks(cnt, p.p2g_item);
         kbeg = end - cnt + chunk * per;
         kend = min(kbeg + per, end);
       }
@@ -1151,7 +1151,11 @@ __global__ __launch_bounds__(256) void k_copy(float4* __restrict__ dst, const fl
 // ---------------------------------------------------------------------------
 static inline uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 // blocks for a tile-stride kernel: enough to fill the chip, never more than tiles
-static inline uint32_t tile_grid(uint32_t n_tiles) { return n_tiles < 2048u ? (n_tiles ? n_tiles : 1u) : 2048u; }
+// blocks for a tile-stride kernel: never more than tiles, at most the tuned cap
+static inline uint32_t tile_grid(uint32_t n_tiles, uint32_t cap) {
+  const uint32_t c = cap ? cap : 2048u;
+  return n_tiles < c ? (n_tiles ? n_tiles : 1u) : c;
+}
 
 struct ProfScope {
   Prof* pf;
@@ -1169,7 +1173,7 @@ struct ProfScope {
 };
 
 void launch_bin(const DevState& d, const SubParams& p, uint32_t n_tiles, bool keys_valid, hipStream_t s, Prof* pf) {
-  const uint32_t tg = tile_grid(n_tiles);
+  const uint32_t tg = tile_grid(n_tiles, p.tile_blocks);
   const bool batch = p.n_scenes > 0;
   { ProfScope ps(pf, 0, s); k_bin_clear<<<tg, 512, 0, s>>>(d.cell_count, d.act_tiles, d.counters); }
   {
@@ -1203,11 +1207,11 @@ void launch_bin(const DevState& d, const SubParams& p, uint32_t n_tiles, bool ke
 void launch_p2g(const DevState& d, const SubParams& p, uint32_t n_tiles, hipStream_t s, Prof* pf) {
   ProfScope ps(pf, 8, s);
   if (p.n_scenes > 0)
-    k_p2g<true><<<tile_grid(n_tiles), kP2GThreads, 0, s>>>(p, d.rec, d.materials, d.cell_count, d.cell_end, d.sorted,
+    k_p2g<true><<<tile_grid(n_tiles, p.tile_blocks), kP2GThreads, 0, s>>>(p, d.rec, d.materials, d.cell_count, d.cell_end, d.sorted,
                                                            d.act_tiles, d.tile_start, d.tile_total, d.counters,
                                                            d.partial);
   else
-    k_p2g<false><<<tile_grid(n_tiles), kP2GThreads, 0, s>>>(p, d.rec, d.materials, d.cell_count, d.cell_end,
+    k_p2g<false><<<tile_grid(n_tiles, p.tile_blocks), kP2GThreads, 0, s>>>(p, d.rec, d.materials, d.cell_count, d.cell_end,
                                                             d.sorted, d.act_tiles, d.tile_start, d.tile_total,
                                                             d.counters, d.partial);
 }
@@ -1215,7 +1219,7 @@ void launch_p2g(const DevState& d, const SubParams& p, uint32_t n_tiles, hipStre
 void launch_grid(const DevState& d, const SubParams& p, uint32_t n_tiles, bool from_partials, bool update,
                  const uint32_t* tiles, const uint32_t* count, hipStream_t s, Prof* pf) {
   ProfScope ps(pf, 9, s);
-  const uint32_t g = tile_grid(n_tiles);
+  const uint32_t g = tile_grid(n_tiles, p.tile_blocks);
 #define NB_GRID_LAUNCH(FP, UP, BT)                                                                                  \
   k_grid<FP, UP, BT><<<g, 512, 0, s>>>(p, d.partial, d.tile_act, tiles, count, d.grid_node, d.grid_old, d.grid_pre, \
                                        d.sdf, d.colliders, d.forces)
@@ -1236,8 +1240,9 @@ void launch_g2p(const DevState& d, const SubParams& p, bool use_sorted, hipStrea
   const uint32_t* srt = use_sorted ? d.sorted : nullptr;
   float4* out = rec_out ? rec_out : d.rec;
 #define NB_G2P_LAUNCH(FL, BT) \
-  k_g2p<FL, BT><<<cdiv(p.n, 256), 256, 0, s>>>(p, d.rec, out, d.keys, d.materials, srt, d.grid_node, d.grid_old, d.sdf)
+  k_g2p<FL, BT><<<cdiv(p.n, bs), bs, 0, s>>>(p, d.rec, out, d.keys, d.materials, srt, d.grid_node, d.grid_old, d.sdf)
   const bool flip = p.flip_ratio > 0.01f;
+  const uint32_t bs = (p.g2p_block == 64u || p.g2p_block == 128u) ? p.g2p_block : 256u;  // launch bound 256
   if (p.n_scenes > 0) {
     if (flip) NB_G2P_LAUNCH(true, true);
     else NB_G2P_LAUNCH(false, true);

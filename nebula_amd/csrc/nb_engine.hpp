@@ -84,6 +84,11 @@ struct SubParams {
   uint32_t corrected_svd;  // 1: singular vectors as columns (see nb_mpm_knobs)
   // batch (n_scenes > 0): the scene table, the scene of every tile and of
   // every particle; the single-scene fields above are then unused
+  // launch shape (nb_mpm_tuning): blocks of the per-tile kernels, threads
+  // per G2P block, particles per P2G work item
+  uint32_t tile_blocks;
+  uint32_t g2p_block;
+  uint32_t p2g_item;
   uint32_t n_scenes;
   const SceneGeo* scenes;
   const uint32_t* tile_scene;

@@ -376,7 +376,21 @@ EXPORTED_SYMBOLS = (
     "nb_mpm_adjoint_begin", "nb_mpm_forward_recorded", "nb_mpm_set_adjoint", "nb_mpm_get_adjoint",
     "nb_mpm_adjoint_device_ptr", "nb_mpm_backward", "nb_mpm_adjoint_end",
     "nb_mpm_create_batch", "nb_mpm_scene_count", "nb_mpm_scene_info",
+    "nb_mpm_get_tuning", "nb_mpm_set_tuning", "nb_mpm_autotune", "nb_mpm_graph_substeps",
 )
+
+TUNE_ROUNDING = 1
+
+
+class Tuning(C.Structure):
+    """Launch shape (nb_mpm_tuning): blocks of the per-tile kernels, threads
+    per G2P block, particles per P2G work item, and the autotuner's timing."""
+    _fields_ = [("tile_blocks", C.c_uint32), ("g2p_block", C.c_uint32), ("p2g_item", C.c_uint32),
+                ("ms_per_substep", C.c_float)]
+
+    def __repr__(self):
+        return (f"Tuning(tile_blocks={self.tile_blocks}, g2p_block={self.g2p_block}, p2g_item={self.p2g_item}, "
+                f"ms_per_substep={self.ms_per_substep:.4f})")
 
 MAX_BATCH_MATERIALS = 1024
 
@@ -411,6 +425,10 @@ def lib():
         L.nb_mpm_create_batch.argtypes = [C.POINTER(BatchScene), C.c_uint32, vp, C.POINTER(vp)]
         L.nb_mpm_scene_count.argtypes = [vp, up]
         L.nb_mpm_scene_info.argtypes = [vp, C.c_uint32, up, up, up]
+        L.nb_mpm_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
+        L.nb_mpm_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
+        L.nb_mpm_autotune.argtypes = [vp, C.c_float, C.c_uint32, C.c_uint32, C.POINTER(Tuning)]
+        L.nb_mpm_graph_substeps.argtypes = [vp, C.c_float, C.c_uint32]
         L.nb_mpm_config_default.argtypes = [C.POINTER(MPMConfig)]
         L.nb_mpm_config_default.restype = None
         L.nb_mpm_knobs_default.argtypes = [C.POINTER(_Knobs)]
@@ -505,6 +523,27 @@ class MPMEngine:
         e._cfg = e.config()
         e._n_cells = e.totalCells()
         return e
+
+    # launch shapes and graph replay ---------------------------------------
+    def tuning(self) -> "Tuning":
+        t = Tuning()
+        _check(lib().nb_mpm_get_tuning(self._h, C.byref(t)))
+        return t
+
+    def setTuning(self, t: "Tuning"):
+        _check(lib().nb_mpm_set_tuning(self._h, C.byref(t)))
+
+    def autotune(self, dt_sub: float, trial_substeps: int = 3, rounding: bool = False) -> "Tuning":
+        """Time candidate launch shapes on the current state (left as it
+        was) and keep the fastest (nb_mpm_autotune)."""
+        t = Tuning()
+        _check(lib().nb_mpm_autotune(self._h, dt_sub, trial_substeps, TUNE_ROUNDING if rounding else 0,
+                                     C.byref(t)))
+        return t
+
+    def graphSubstep(self, dt_sub: float, n: int = 1):
+        """n substeps as replays of the captured substep graph (nb_mpm_graph_substeps)."""
+        _check(lib().nb_mpm_graph_substeps(self._h, dt_sub, n))
 
     def sceneCount(self) -> int:
         n = C.c_uint32()
