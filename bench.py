@@ -14,12 +14,21 @@ Also printed in the JSON line:
                 8 TB/s HBM peak (bytes per particle / per active node as in the
                 substep model B = 288 N + 68 G_act: bin 20 N + 8 G, P2G 92 N +
                 16 G, grid 28 G, G2P 176 N + 16 G)
-  adjoint       the reverse-mode leg of the metric on the same scene with the
-                proper polar SVD (the mode the adjoint is defined for):
-                forward-only time of K substeps against a recorded forward of
-                K substeps plus the backward (dL/d initial state and
-                dL/d material parameters), keeping all K states and with
-                sqrt(K)-substep checkpoint segments
+  adjoint       the reverse-mode leg of the metric on the differentiable
+                batch (64 scenes of 32,768 particles on 64^3 grids, an elastic
+                layer on a Drucker-Prager layer, hidden E, nu, friction angle
+                per scene, proper polar SVD): forward-only time of 400
+                substeps against a recorded forward of 400 substeps with a
+                checkpoint every 20 plus the backward (dL/d initial state,
+                dL/d material parameters)
+  adjoint_single_scene  the same ratio on this scene (proper polar SVD),
+                K substeps, all states kept and sqrt(K)-substep segments
+  batch         the 256-scene mix (particle counts lognormal(ln 16000, 0.8)
+                clipped to [2000, 64000], grids 32..96 cells, elastic / Von
+                Mises / Drucker-Prager falling boxes, seed 4): the plain
+                per-scene loop (one engine per scene, each stepped in turn)
+                against one batch engine stepping all scenes together, eager,
+                after the autotuner, and as hipGraph replays
   cpu_baseline  the FP32 CPU oracle (a restatement of the reference's shaders,
                 OpenMP over the host cores) timed on a bounded sample: a few
                 substeps of the same scene
@@ -87,6 +96,98 @@ def adjoint_leg(sc, k):
     return res
 
 
+def diff_batch_leg(k, seg):
+    """Forward+adjoint over forward time on the 64-scene differentiable batch."""
+    import numpy as np
+
+    from nebula_amd.scenes import batch_layers_64, make_batch_engine
+
+    scenes = batch_layers_64()
+    dt = scenes[0].dt
+    n = sum(s.n for s in scenes)
+    log(f"[bench] differentiable batch: {len(scenes)} scenes, {n} particles, {k} substeps, checkpoint every {seg}")
+    eng = make_batch_engine(scenes)
+    init = eng.getState()
+    eng.substep(dt, 2)
+    eng.sync()
+    eng.setState(init)
+    t0 = time.perf_counter()
+    eng.substep(dt, k)
+    eng.sync()
+    t_fwd = time.perf_counter() - t0
+    eng.setState(init)
+    eng.adjointBegin(k, seg)
+    t0 = time.perf_counter()
+    eng.forwardRecorded(dt)
+    t_rec = time.perf_counter() - t0
+    final = eng.getState()
+    target = final.copy()
+    target[0, :, :3] += np.float32(1e-3)  # loss = MSE of final positions against shifted targets
+    g = np.zeros_like(final)
+    g[0, :, :3] = 2.0 * (final[0, :, :3] - target[0, :, :3]) / n
+    eng.setAdjoint(g)
+    t0 = time.perf_counter()
+    dpar = eng.backward()
+    t_bwd = time.perf_counter() - t0
+    eng.adjointEnd()
+    ratio = (t_rec + t_bwd) / t_fwd
+    log(f"[bench] differentiable batch: fwd {t_fwd / k * 1e3:.3f} ms, recorded fwd {t_rec / k * 1e3:.3f} ms, "
+        f"bwd {t_bwd / k * 1e3:.3f} ms per substep; ratio {ratio:.3f}")
+    return {"workload": "64 scenes x 32,768 particles on 64^3 grids, elastic layer on Drucker-Prager layer, "
+                        "hidden (E, nu, friction angle) per scene (seed 5), proper polar SVD, APIC, dt 1e-4",
+            "particles": n, "substeps": k, "segment": seg, "fwd_ms_per_substep": round(t_fwd / k * 1e3, 4),
+            "recorded_fwd_ms_per_substep": round(t_rec / k * 1e3, 4), "bwd_ms_per_substep": round(t_bwd / k * 1e3, 4),
+            "ratio": round(ratio, 3), "finite_gradients": bool(np.isfinite(dpar).all())}
+
+
+def batch_leg(k):
+    """The 256-scene mix: plain per-scene loop against one batch engine."""
+    from nebula_amd.scenes import batch_mix_256, make_batch_engine, make_engine
+
+    scenes = batch_mix_256()
+    dt = scenes[0].dt
+    n = sum(s.n for s in scenes)
+    log(f"[bench] 256-scene mix: {len(scenes)} scenes, {n} particles; plain loop, {k} substeps")
+    engines = [make_engine(s) for s in scenes]
+    for e in engines:
+        e.substep(dt, 2)
+    for e in engines:
+        e.sync()
+    t0 = time.perf_counter()
+    for e in engines:
+        e.substep(dt, k)
+    for e in engines:
+        e.sync()
+    t_loop = (time.perf_counter() - t0) / k
+    del engines
+    b = make_batch_engine(scenes)
+    b.substep(dt, 2)
+    b.sync()
+
+    def timed(step):
+        t0 = time.perf_counter()
+        step()
+        b.sync()
+        return (time.perf_counter() - t0) / k
+
+    t_eager = timed(lambda: b.substep(dt, k))
+    tune = b.autotune(dt, trial_substeps=4)
+    t_tuned = timed(lambda: b.substep(dt, k))
+    b.graphSubstep(dt, 2)
+    b.sync()
+    t_graph = timed(lambda: b.graphSubstep(dt, k))
+    best = min(t_tuned, t_graph)
+    log(f"[bench] 256-scene mix per substep: loop {t_loop * 1e3:.3f} ms, batch {t_eager * 1e3:.3f} ms, "
+        f"tuned {t_tuned * 1e3:.3f} ms, graph {t_graph * 1e3:.3f} ms; speedup {t_loop / best:.2f}x ({tune})")
+    return {"workload": "256 falling-box scenes, particle counts lognormal(ln 16000, 0.8) clipped to [2000, 64000], "
+                        "grids 32..96 cells at h = 1/64, elastic / Von Mises / Drucker-Prager, seed 4, APIC, dt 1e-4",
+            "scenes": len(scenes), "particles": n, "substeps": k,
+            "loop_ms_per_substep": round(t_loop * 1e3, 4), "batch_ms_per_substep": round(t_eager * 1e3, 4),
+            "tuned_ms_per_substep": round(t_tuned * 1e3, 4), "graph_ms_per_substep": round(t_graph * 1e3, 4),
+            "particle_substeps_per_s": round(n / best, 1), "speedup_vs_loop": round(t_loop / best, 3),
+            "tuning": {"tile_blocks": tune.tile_blocks, "g2p_block": tune.g2p_block, "p2g_item": tune.p2g_item}}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,7 +195,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--profile-steps", type=int, default=20, help="substeps timed per kernel with HIP events")
     ap.add_argument("--cpu-steps", type=int, default=5, help="substeps of the CPU oracle sample (0 = skip)")
-    ap.add_argument("--adjoint-substeps", type=int, default=16, help="substeps of the reverse-mode leg (0 = skip)")
+    ap.add_argument("--adjoint-substeps", type=int, default=16,
+                    help="substeps of the single-scene reverse-mode leg (0 = skip)")
+    ap.add_argument("--diff-substeps", type=int, default=400,
+                    help="substeps of the differentiable-batch reverse-mode leg (0 = skip)")
+    ap.add_argument("--diff-segment", type=int, default=20, help="checkpoint spacing of that leg")
+    ap.add_argument("--batch-substeps", type=int, default=20, help="substeps of the 256-scene leg (0 = skip)")
     ap.add_argument("--calibration-copy-mb", type=int, default=0,
                     help="also run a plain copy kernel of this many MiB (HBM byte-counter calibration)")
     args = ap.parse_args()
@@ -139,7 +245,10 @@ def main():
     if args.calibration_copy_mb > 0:
         eng.calibrationCopy(args.calibration_copy_mb << 20, 3)
 
-    adjoint = adjoint_leg(sc, args.adjoint_substeps) if args.adjoint_substeps > 0 else None
+    adjoint1 = adjoint_leg(sc, args.adjoint_substeps) if args.adjoint_substeps > 0 else None
+    del eng
+    adjoint = diff_batch_leg(args.diff_substeps, args.diff_segment) if args.diff_substeps > 0 else None
+    batch = batch_leg(args.batch_substeps) if args.batch_substeps > 0 else None
 
     cpu = None
     if args.cpu_steps > 0:
@@ -174,6 +283,8 @@ def main():
                              "unit": "GB/s", "frac": round(substep_rate / HBM_PEAK_GBS, 4)},
         "kernels_avg_ms": {k: round(v, 5) for k, v in avg_ms.items()},
         "adjoint": adjoint,
+        "adjoint_single_scene": adjoint1,
+        "batch": batch,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

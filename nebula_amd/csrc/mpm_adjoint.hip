@@ -121,9 +121,10 @@ __device__ __forceinline__ void axis_factors(const AxisD& a, AxisF& f) {
 // Axis-by-axis contraction: x (5 factor kinds per row), then y (the 12
 // (x, y) pairs the products use), then z.  load(ix, iy, iz) returns the
 // node's field in xyz and, when WITH_M, a scalar field in w whose sums with
-// dw/dx_a go to Sm.  No weight cut: the sums stand only when no node weight
-// lies in (0, 1e-12); the caller redoes such a particle with exact_gather.
-// The z loop stays rolled so that one slab
+// dw/dx_a go to Sm.  The transfers' cut of node weights below 1e-12 is not
+// applied: it changes the forward map by less than 1e-12 relative, so its
+// derivative terms are below any gradient tolerance, and every particle then
+// takes the same path.  The z loop stays rolled so that one slab
 // of 9 node loads is live at a time; its z factors sit in position 0 and are
 // rotated at the end of each iteration.
 template <bool WITH_M, class Load, class Sink>
@@ -226,69 +227,6 @@ __device__ __forceinline__ void sep_gather(const AxisF& X, const AxisF& Y, const
       zgd_[i] = zgd_[i + 1];
     }
   }
-}
-
-// The same sums node by node with the transfers' weight cut (nodes with
-// w < 1e-12 are skipped), for particles with a weight in (0, 1e-12).
-template <bool WITH_M, class Load, class Sink>
-__device__ __forceinline__ void exact_gather(const AxisF& X, const AxisF& Y, const AxisF& Z, Load load,
-                                             float S[kNSum][3], float Sm[3], Sink sink) {
-#pragma unroll
-  for (int k = 0; k < kNSum; k++) S[k][0] = S[k][1] = S[k][2] = 0.f;
-  Sm[0] = Sm[1] = Sm[2] = 0.f;
-  float zw_[3] = {Z.w[0], Z.w[1], Z.w[2]}, zg_[3] = {Z.g[0], Z.g[1], Z.g[2]};
-  float zh_[3] = {Z.hh[0], Z.hh[1], Z.hh[2]}, zd_[3] = {Z.dx[0], Z.dx[1], Z.dx[2]};
-#pragma unroll 1
-  for (int iz = 0; iz < 3; iz++) {
-    const float wz = zw_[0], gz = zg_[0], hz = zh_[0], dz = zd_[0];
-#pragma unroll
-    for (int o = 0; o < 9; o++) {
-      const int ix = o % 3, iy = o / 3;
-      const float wx = X.w[ix], wy = Y.w[iy];
-      const float w = wx * wy * wz;
-      if (w < 1e-12f) continue;
-      const float gx = X.g[ix], gy = Y.g[iy];
-      const float dw[3] = {gx * wy * wz, wx * gy * wz, wx * wy * gz};
-      const float d[3] = {X.dx[ix], Y.dx[iy], dz};
-      const float4 f4 = load(ix, iy, iz);
-      const float f[3] = {f4.x, f4.y, f4.z};
-#pragma unroll
-      for (int r = 0; r < 3; r++) {
-        S[kW][r] = fmaf(w, f[r], S[kW][r]);
-#pragma unroll
-        for (int a = 0; a < 3; a++) {
-          S[kWD + a][r] = fmaf(w * d[a], f[r], S[kWD + a][r]);
-          S[kG + a][r] = fmaf(dw[a], f[r], S[kG + a][r]);
-        }
-      }
-#pragma unroll
-      for (int b = 0; b < 3; b++)
-#pragma unroll
-        for (int c = 0; c < 3; c++) sink(gd_idx(b, c), dw[b] * d[c], f);
-      sink(h_idx(0, 0), X.hh[ix] * wy * wz, f);
-      sink(h_idx(1, 1), wx * Y.hh[iy] * wz, f);
-      sink(h_idx(2, 2), wx * wy * hz, f);
-      sink(h_idx(0, 1), gx * gy * wz, f);
-      sink(h_idx(0, 2), gx * wy * gz, f);
-      sink(h_idx(1, 2), wx * gy * gz, f);
-      if (WITH_M) {
-#pragma unroll
-        for (int a = 0; a < 3; a++) Sm[a] = fmaf(dw[a], f4.w, Sm[a]);
-      }
-    }
-#pragma unroll
-    for (int i = 0; i < 2; i++) {
-      zw_[i] = zw_[i + 1];
-      zg_[i] = zg_[i + 1];
-      zh_[i] = zh_[i + 1];
-      zd_[i] = zd_[i + 1];
-    }
-  }
-}
-
-// a weight of the stencil lies in (0, 1e-12): the transfers' cut removes it
-__device__ __forceinline__ bool has_tiny(const AxisF& X, const AxisF& Y, const AxisF& Z) {
-  return (min_positive3(X.w) * min_positive3(Y.w)) * min_positive3(Z.w) < 1e-12f;
 }
 
 __device__ __forceinline__ int adj_model(uint32_t m) {
@@ -404,15 +342,7 @@ __global__ __launch_bounds__(256) void k_g2p_bwd(SubParams p, const float4* __re
         for (int r = 0; r < 3; r++) SH[k - kHD][r] = fmaf(z, qv[r], SH[k - kHD][r]);
       }
     };
-    // every lane sums axis by axis; a lane with a weight under the cut redoes
-    // its sums node by node (a particle's result depends on its own data only)
     sep_gather<false>(ax[0], ax[1], ax[2], node_v, S, Sm_unused, sink);
-    if (has_tiny(ax[0], ax[1], ax[2])) {
-      gxB[0] = gxB[1] = gxB[2] = 0.f;
-#pragma unroll
-      for (int k = 0; k < 6; k++) SH[k][0] = SH[k][1] = SH[k][2] = 0.f;
-      exact_gather<false>(ax[0], ax[1], ax[2], node_v, S, Sm_unused, sink);
-    }
     const float vp[3] = {S[kW][0], S[kW][1], S[kW][2]};
     m3 vg, Bn;
 #pragma unroll
@@ -879,10 +809,6 @@ __global__ __launch_bounds__(256) void k_p2g_bwd(SubParams p, const float4* __re
     }
   };
   sep_gather<true>(ax[0], ax[1], ax[2], node_g, S, Sm, sink);
-  if (has_tiny(ax[0], ax[1], ax[2])) {  // the weight cut of shaders/mpm_p2g.comp:61, per particle
-    gxo[0] = gxo[1] = gxo[2] = 0.f;
-    exact_gather<true>(ax[0], ax[1], ax[2], node_g, S, Sm, sink);
-  }
   // P2G adds  w (m v + C d) - dt Vp tau grad w  to the node momentum and  w m  to its mass
   float gv[3], gx[3];
   m3 gC, gT;

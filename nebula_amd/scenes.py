@@ -318,3 +318,36 @@ def batch_mix_256(seed: int = 4, n_scenes: int = 256) -> List[Scene]:
         nu = float(rng.uniform(0.1, 0.4))
         out.append(falling_box(f"mix{s}", grid, n, E, nu, model, rng))
     return out
+
+
+def layered_pair(name: str, E: float, nu: float, phi_deg: float, rng: np.random.Generator, h: float = 1.0 / 64,
+                 dt: float = 1e-4) -> Scene:
+    """One scene of the differentiable batch: on a 64^3 grid, an elastic
+    layer resting on a Drucker-Prager layer (each 16 x 8 x 16 cells at 8
+    jittered particles per cell: 32,768 particles), both falling at
+    (0, -1, 0) plus seeded noise; E, nu shared by the layers, friction angle
+    phi for the sand; proper polar SVD."""
+    layers = []
+    for y0 in (4, 12):
+        g = [np.arange(24, 40), np.arange(y0, y0 + 8), np.arange(24, 40)]
+        cx, cy, cz = np.meshgrid(*g, indexing="ij")
+        layers.append(np.stack([cx.ravel(), cy.ravel(), cz.ravel()], axis=1))
+    pos = np.concatenate([_seed_cells(c, h, 8, rng) for c in layers])
+    n = pos.shape[0]
+    vel = (np.array([0.0, -1.0, 0.0]) + 0.05 * rng.standard_normal((n, 3))).astype(np.float32)
+    mid = np.concatenate([np.ones(n // 2, np.uint32), np.zeros(n - n // 2, np.uint32)])  # sand below, elastic above
+    f = _fields(pos, vel, h ** 3 / 8, mid)
+    cfg = MPMConfig(nx=0, ny=0, nz=0, domainSize=(64 * h,) * 3, cellSize=h, maxParticles=n, E=E, nu=nu, rho0=1000.0)
+    mats = [_material(E, nu, 1000.0, MaterialModel.ELASTIC),
+            _material(E, nu, 1000.0, MaterialModel.DRUCKER_PRAGER, M=math.tan(math.radians(phi_deg)), qc=0.0)]
+    return Scene(name, cfg, mats, f, dt=dt, flip_ratio=-1.0, substeps=400,
+                 forces=[GravityForce((0.0, 1.0, 0.0), -9.8)], corrected_svd=True)
+
+
+def batch_layers_64(seed: int = 5, n_scenes: int = 64) -> List[Scene]:
+    """The differentiable batch: 64 layered scenes (layered_pair) with hidden
+    E in [1e4, 5e4], nu in [0.2, 0.35] and friction angle in [25, 40] deg
+    drawn per scene; 2,097,152 particles in all."""
+    rng = np.random.default_rng(seed)
+    return [layered_pair(f"layers{s}", float(rng.uniform(1e4, 5e4)), float(rng.uniform(0.2, 0.35)),
+                         float(rng.uniform(25.0, 40.0)), rng) for s in range(n_scenes)]
