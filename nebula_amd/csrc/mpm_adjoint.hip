@@ -595,29 +595,35 @@ __global__ __launch_bounds__(kAdjThreads) void k_adj_scatter(SubParams p, const 
         axis_stencil_d(pp.x, sg.wmin[0], sg.h, cx - 1, X);
         axis_stencil_d(pp.y, sg.wmin[1], sg.h, cy - 1, Y);
         axis_stencil_d(pp.z, sg.wmin[2], sg.h, cz - 1, Z);
+        // dL/dv_node = w (A + M_B d) + M_G grad w, factored per (y, z) row:
+        //   = wx P + (wx dx) Q + gx R  with
+        //   P = wy wz (A + dy M_B[1] + dz M_B[2]) + gy wz M_G[1] + wy gz M_G[2],
+        //   Q = wy wz M_B[0],  R = wy wz M_G[0]
+        // (no weight cut, as in the G2P backward's gather)
+        float wdx[3];
+#pragma unroll
+        for (int ix = 0; ix < 3; ix++) wdx[ix] = X.w[ix] * X.dx[ix];
 #pragma unroll
         for (int kz = 0; kz < 3; kz++)
 #pragma unroll
-          for (int jy = 0; jy < 3; jy++)
+          for (int jy = 0; jy < 3; jy++) {
+            const float wyz = Y.w[jy] * Z.w[kz], gyz = Y.g[jy] * Z.w[kz], wgz = Y.w[jy] * Z.g[kz];
+            float P[3], Q[3], R[3];
+#pragma unroll
+            for (int rr = 0; rr < 3; rr++) {
+              const float t = fmaf(Z.dx[kz], cf[9 + rr], fmaf(Y.dx[jy], cf[6 + rr], cf[rr]));
+              P[rr] = fmaf(wgz, cf[18 + rr], fmaf(gyz, cf[15 + rr], wyz * t));
+              Q[rr] = wyz * cf[3 + rr];
+              R[rr] = wyz * cf[12 + rr];
+            }
 #pragma unroll
             for (int ix = 0; ix < 3; ix++) {
               const int o = (kz * 3 + jy) * 3 + ix;
-              const float w = X.w[ix] * Y.w[jy] * Z.w[kz];
-              if (w < 1e-12f) continue;  // G2P's weight cut
-              const float dw[3] = {X.g[ix] * Y.w[jy] * Z.w[kz], X.w[ix] * Y.g[jy] * Z.w[kz], X.w[ix] * Y.w[jy] * Z.g[kz]};
-              const float d[3] = {X.dx[ix], Y.dx[jy], Z.dx[kz]};
-              float out[3];
-#pragma unroll
-              for (int rr = 0; rr < 3; rr++) {
-                float s = w * cf[rr];
-#pragma unroll
-                for (int cc = 0; cc < 3; cc++) s += w * cf[3 + 3 * cc + rr] * d[cc] + cf[12 + 3 * cc + rr] * dw[cc];
-                out[rr] = s;
-              }
-              ax[o] += out[0];
-              ay[o] += out[1];
-              az[o] += out[2];
+              ax[o] = fmaf(X.g[ix], R[0], fmaf(wdx[ix], Q[0], fmaf(X.w[ix], P[0], ax[o])));
+              ay[o] = fmaf(X.g[ix], R[1], fmaf(wdx[ix], Q[1], fmaf(X.w[ix], P[1], ay[o])));
+              az[o] = fmaf(X.g[ix], R[2], fmaf(wdx[ix], Q[2], fmaf(X.w[ix], P[2], az[o])));
             }
+          }
       }
 #pragma unroll
       for (int kz = 0; kz < 3; kz++)
