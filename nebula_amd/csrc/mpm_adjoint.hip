@@ -16,7 +16,8 @@
 //   k_grid_bwd     per node: the grid update backwards -> dL/dp, dL/dm
 //   k_p2g_bwd      per particle: P2G backwards as a 27-node gather
 //                  -> d/dv, d/dB, d/dtau and the position part from P2G
-//   k_pgrad_reduce per-block material-parameter partials summed in a fixed order
+//   k_pgrad_chunks / k_pgrad_total  per-block material-parameter partials
+//                  summed in a fixed order
 // Positions enter every weight, so both transfers contribute the weight
 // gradients and Hessians.  Everything is deterministic.
 #include <hip/hip_runtime.h>
@@ -850,18 +851,23 @@ __global__ __launch_bounds__(256) void k_p2g_bwd(SubParams p, const float4* __re
   go[R_T] = make_float4(gT.m[0][0], gT.m[1][1], gT.m[2][2], gt.w);
 }
 
-// Fixed-order sum of the per-block entry lists, one block per material:
-// thread t walks the lists of blocks t, t + 256, ... in order, adding the
-// entries of its material; a fixed LDS tree adds the 256 sums to the
-// running total.
-__global__ __launch_bounds__(256) void k_pgrad_reduce(const uint32_t* __restrict__ pg_mid,
+// Fixed-order sum of the per-block entry lists in two levels.  Level 1: one
+// wave per (chunk of 64 source blocks, material); lane l takes block
+// chunk * 64 + l and adds its entries of that material; a fixed shuffle tree
+// sums the lanes into chunk_part[material][chunk].  Level 2: one block per
+// material adds its chunk sums (thread t: chunks t, t + 256, ..., then a
+// fixed LDS tree) to the running total.
+constexpr uint32_t kPgChunk = 64;
+__global__ __launch_bounds__(256) void k_pgrad_chunks(const uint32_t* __restrict__ pg_mid,
                                                       const float4* __restrict__ pg_val,
                                                       const uint32_t* __restrict__ pg_cnt, uint32_t n_blocks,
-                                                      float* __restrict__ pgrad) {
-  __shared__ float4 s[256];
-  const uint32_t m = blockIdx.x;
+                                                      uint32_t n_materials, uint32_t n_chunks,
+                                                      float4* __restrict__ chunk_part) {
+  const uint32_t chunk = blockIdx.x, m = blockIdx.y * 4u + (threadIdx.x >> 6), lane = threadIdx.x & 63u;
+  if (m >= n_materials) return;  // whole waves
+  const uint32_t b = chunk * kPgChunk + lane;
   float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-  for (uint32_t b = threadIdx.x; b < n_blocks; b += 256) {
+  if (b < n_blocks) {
     const uint32_t c = pg_cnt[b];
     for (uint32_t j = 0; j < c; j++) {
       const size_t e = size_t(b) * kPgEntries + j;
@@ -870,6 +876,24 @@ __global__ __launch_bounds__(256) void k_pgrad_reduce(const uint32_t* __restrict
         acc.x += v.x; acc.y += v.y; acc.z += v.z; acc.w += v.w;
       }
     }
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    acc.x += __shfl_xor(acc.x, o);
+    acc.y += __shfl_xor(acc.y, o);
+    acc.z += __shfl_xor(acc.z, o);
+    acc.w += __shfl_xor(acc.w, o);
+  }
+  if (lane == 0) chunk_part[size_t(m) * n_chunks + chunk] = acc;
+}
+__global__ __launch_bounds__(256) void k_pgrad_total(const float4* __restrict__ chunk_part, uint32_t n_chunks,
+                                                     float* __restrict__ pgrad) {
+  __shared__ float4 s[256];
+  const uint32_t m = blockIdx.x;
+  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (uint32_t c = threadIdx.x; c < n_chunks; c += 256) {
+    const float4 v = chunk_part[size_t(m) * n_chunks + c];
+    acc.x += v.x; acc.y += v.y; acc.z += v.z; acc.w += v.w;
   }
   s[threadIdx.x] = acc;
   __syncthreads();
@@ -917,7 +941,10 @@ void launch_adjoint_substep(const DevState& d, const AdjState& a, const SubParam
     adjoint_kernels<true>(d, a, p, n_tiles, adj_in, adj_out, s);
   else
     adjoint_kernels<false>(d, a, p, n_tiles, adj_in, adj_out, s);
-  k_pgrad_reduce<<<p.material_count, 256, 0, s>>>(a.pg_mid, a.pg_val, a.pg_cnt, nblk, a.pgrad);
+  const uint32_t n_chunks = cdiv_a(nblk, kPgChunk);
+  k_pgrad_chunks<<<dim3(n_chunks, cdiv_a(p.material_count, 4u)), 256, 0, s>>>(
+      a.pg_mid, a.pg_val, a.pg_cnt, nblk, p.material_count, n_chunks, a.pg_chunk);
+  k_pgrad_total<<<p.material_count, 256, 0, s>>>(a.pg_chunk, n_chunks, a.pgrad);
 }
 
 }  // namespace nb

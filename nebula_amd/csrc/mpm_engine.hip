@@ -1298,6 +1298,7 @@ int nb_mpm_adjoint_begin(nb_mpm* h, uint32_t n_substeps, uint32_t segment) {
   if (rc == NB_OK) rc = adj_alloc(h, &A.a.pg_val, size_t(n_blocks) * kPgEntries);
   if (rc == NB_OK) rc = adj_alloc(h, &pm, size_t(n_blocks) * kPgEntries / 4);
   if (rc == NB_OK) rc = adj_alloc(h, &pc, (n_blocks + 3) / 4);
+  if (rc == NB_OK) rc = adj_alloc(h, &A.a.pg_chunk, size_t(h->max_materials()) * ((n_blocks + 63) / 64));
   if (rc == NB_OK) rc = adj_alloc(h, &pg, h->max_materials());
   if (rc != NB_OK) {
     std::string keep = g_err;

@@ -176,6 +176,7 @@ struct AdjState {
   uint32_t* pg_mid;     // blocks * kPgEntries
   float4* pg_val;       // blocks * kPgEntries
   uint32_t* pg_cnt;     // blocks
+  float4* pg_chunk;     // materials x ceil(blocks / 64): per-chunk sums
   float* pgrad;         // materials x 4: running d/d(mu, lambda, M, q_c)
 };
 // One substep backwards on the recomputed forward of state d.rec (bins, P2G
