@@ -269,8 +269,11 @@ int nb_mpm_set_collider_sdf(nb_mpm* h, const float* morton_sdf, uint64_t n);
 int nb_mpm_set_collider_sphere(nb_mpm* h, float radius, float cx, float cy, float cz);
 int nb_mpm_clear_collider(nb_mpm* h);
 /* EngineBase::setForces / clearForces (src/engine/EngineBase.cpp:22-40).
- * Gravity, constant wind and z-clamp (a no-op on a grid, as in the reference)
- * are supported; turbulence and noise return NB_ERR_UNSUPPORTED. */
+ * Gravity, constant wind, turbulence (curl of fBm noise), noise (fBm along a
+ * direction) and z-clamp (a no-op on a grid, as in the reference); the force
+ * records are read by a compiled device switch (the reference regenerates
+ * its grid shader instead, src/core/ForceShaderCompiler.cpp:139-166).  The
+ * caller advances the records' seeds between steps (Force::advanceTime). */
 int nb_mpm_set_forces(nb_mpm* h, const nb_force* forces, uint32_t n);
 
 /* MPMEngine::appendParticles (src/engine/MPMEngine.cpp:211-235): host arrays of

@@ -704,17 +704,10 @@ __global__ __launch_bounds__(512) void k_grid_bwd(SubParams p, const float4* __r
     const bool capped = sp > 15.f;
     float vx = u0x, vy = u0y, vz = u0z;
     if (capped) { const float kk = 15.f / sp; vx *= kk; vy *= kk; vz *= kk; }
-    for (uint32_t fi = 0; fi < p.force_count; fi++) {
-      const nb_force f = forces[fi];
-      const uint32_t mask = f.affect_mask;
-      if (mask != 0u && (mask & 1u) == 0u) continue;
-      if (f.type == NB_FORCE_GRAVITY || f.type == NB_FORCE_CONSTANT_WIND) {
-        const float s = f.data[3];
-        vx += (f.data[0] * s) * p.dt;
-        vy += (f.data[1] * s) * p.dt;
-        vz += (f.data[2] * s) * p.dt;
-      }
-    }
+    // forces add constants with respect to the state: they pass the adjoint
+    // through, but the flags below depend on the velocity they produce
+    apply_forces(vx, vy, vz, (float(nx) + 0.5f) * sg.h + sg.wmin[0], (float(ny) + 0.5f) * sg.h + sg.wmin[1],
+                 (float(nz) + 0.5f) * sg.h + sg.wmin[2], forces, p.force_count, p.dt);
     const int Nx = sg.grid_res[0] - 1, Ny = sg.grid_res[1] - 1, Nz = sg.grid_res[2] - 1;
     bool wx = false, wy = false, wz = false;
     if (nx <= 1 && vx < 0.f) { vx = 0.f; wx = true; }

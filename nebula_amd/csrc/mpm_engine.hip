@@ -871,8 +871,6 @@ int nb_mpm_set_forces(nb_mpm* h, const nb_force* forces, uint32_t n) {
   if (n && !forces) return fail(NB_ERR_ARG, "null forces");
   for (uint32_t i = 0; i < n; i++) {
     uint32_t t = forces[i].type;
-    if (t == NB_FORCE_TURBULENCE || t == NB_FORCE_NOISE)
-      return fail(NB_ERR_UNSUPPORTED, "force type %u (curl/fBm noise) is not implemented", t);
     if (t > NB_FORCE_Z_CLAMP) return fail(NB_ERR_ARG, "unknown force type %u", t);
   }
   int rc = upload(h, h->d.forces, forces, sizeof(nb_force) * n);

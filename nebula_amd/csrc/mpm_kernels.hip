@@ -686,18 +686,7 @@ __global__ __launch_bounds__(512) void k_grid(SubParams p, const float4* __restr
     const float px = (float(nx) + 0.5f) * g.h + g.wmin[0];
     const float py = (float(ny) + 0.5f) * g.h + g.wmin[1];
     const float pz = (float(nz) + 0.5f) * g.h + g.wmin[2];
-    // force block generated by src/core/ForceShaderCompiler.cpp:72-93 (typeFlag 0)
-    for (uint32_t fi = 0; fi < p.force_count; fi++) {
-      const nb_force f = forces[fi];
-      const uint32_t mask = f.affect_mask;
-      if (mask != 0u && (mask & 1u) == 0u) continue;
-      if (f.type == NB_FORCE_GRAVITY || f.type == NB_FORCE_CONSTANT_WIND) {
-        const float s = f.data[3];
-        vx += (f.data[0] * s) * p.dt;
-        vy += (f.data[1] * s) * p.dt;
-        vz += (f.data[2] * s) * p.dt;
-      }
-    }
+    apply_forces(vx, vy, vz, px, py, pz, forces, p.force_count, p.dt);
     // separating walls on the two outer node layers (shaders/mpm_grid_update.comp:54-60)
     const int Nx = g.grid_res[0] - 1, Ny = g.grid_res[1] - 1, Nz = g.grid_res[2] - 1;
     if (nx <= 1 && vx < 0.f) vx = 0.f;

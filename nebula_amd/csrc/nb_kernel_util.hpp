@@ -158,6 +158,98 @@ __device__ __forceinline__ float sample_sdf(const float* sdf, const SceneGeo& g,
 
 __device__ __forceinline__ float len3(float x, float y, float z) { return sqrtf(x * x + y * y + z * z); }
 
+// ---- procedural noise of the turbulence and noise forces ----
+// shaders/force_noise.glsl:7-84 with GLSL's fract(x) = x - floor(x),
+// mix(a, b, t) = a + (b - a) t and dot summed left to right.  The hash
+// fract(sin(a) 43758.5453) turns one rounding step of a into a different
+// value, so these functions are compiled without FMA contraction, every
+// product and sum rounded as written.
+__device__ __forceinline__ float gl_fract(float x) { return x - floorf(x); }
+__device__ __forceinline__ float gl_mix(float a, float b, float t) { return a + (b - a) * t; }
+__device__ __forceinline__ float3 noise_hash33(float x, float y, float z) {  // force_noise.glsl:7-12
+#pragma clang fp contract(off)
+  const float a = x * 127.1f + y * 311.7f + z * 74.7f;
+  const float b = x * 269.5f + y * 183.3f + z * 246.1f;
+  const float c = x * 113.5f + y * 271.9f + z * 124.6f;
+  return make_float3(-1.f + 2.f * gl_fract(sinf(a) * 43758.5453123f), -1.f + 2.f * gl_fract(sinf(b) * 43758.5453123f),
+                     -1.f + 2.f * gl_fract(sinf(c) * 43758.5453123f));
+}
+__device__ inline float noise_value3(float px, float py, float pz) {  // force_noise.glsl:14-37
+#pragma clang fp contract(off)
+  const float ix = floorf(px), iy = floorf(py), iz = floorf(pz);
+  const float fx = gl_fract(px), fy = gl_fract(py), fz = gl_fract(pz);
+  const float ux = fx * fx * (3.f - 2.f * fx), uy = fy * fy * (3.f - 2.f * fy), uz = fz * fz * (3.f - 2.f * fz);
+  float n[8];
+#pragma unroll
+  for (int c = 0; c < 8; c++) {
+    const float ox = float(c & 1), oy = float((c >> 1) & 1), oz = float((c >> 2) & 1);
+    const float3 g = noise_hash33(ix + ox, iy + oy, iz + oz);
+    n[c] = g.x * (fx - ox) + g.y * (fy - oy) + g.z * (fz - oz);
+  }
+  const float nx00 = gl_mix(n[0], n[1], ux), nx10 = gl_mix(n[2], n[3], ux);
+  const float nx01 = gl_mix(n[4], n[5], ux), nx11 = gl_mix(n[6], n[7], ux);
+  return gl_mix(gl_mix(nx00, nx10, uy), gl_mix(nx01, nx11, uy), uz);
+}
+__device__ inline float noise_fbm3(float px, float py, float pz, uint32_t octaves) {  // force_noise.glsl:40-50
+#pragma clang fp contract(off)
+  float sum = 0.f, amp = 0.5f, freq = 1.f;
+  const uint32_t n = octaves > 1u ? octaves : 1u;
+  for (uint32_t i = 0; i < n; i++) {
+    sum += amp * noise_value3(px * freq, py * freq, pz * freq);
+    freq *= 2.f;
+    amp *= 0.5f;
+  }
+  return sum;
+}
+__device__ inline float3 noise_curl3(float px, float py, float pz, uint32_t oct) {  // force_noise.glsl:54-82
+#pragma clang fp contract(off)
+  const float e = 0.1f;
+  const float px1 = noise_fbm3(px + 19.1f, py + e, pz, oct), px2 = noise_fbm3(px + 19.1f, py - e, pz, oct);
+  const float py3 = noise_fbm3(px + e, py + 47.3f, pz, oct), py4 = noise_fbm3(px - e, py + 47.3f, pz, oct);
+  const float py1 = noise_fbm3(px, py + 47.3f, pz + e, oct), py2 = noise_fbm3(px, py + 47.3f, pz - e, oct);
+  const float pz3 = noise_fbm3(px, py + e, pz + 71.7f, oct), pz4 = noise_fbm3(px, py - e, pz + 71.7f, oct);
+  const float px3 = noise_fbm3(px + 19.1f, py, pz + e, oct), px4 = noise_fbm3(px + 19.1f, py, pz - e, oct);
+  const float pz1 = noise_fbm3(px + e, py, pz + 71.7f, oct), pz2 = noise_fbm3(px - e, py, pz + 71.7f, oct);
+  const float s = 2.f * e;
+  return make_float3((pz3 - pz4) / s - (py1 - py2) / s, (px3 - px4) / s - (pz1 - pz2) / s,
+                     (py3 - py4) / s - (px1 - px2) / s);
+}
+
+// The grid's force block (src/core/ForceShaderCompiler.cpp:72-93 for the MPM
+// grid, typeFlag 0; Force.h:136-256): gravity and constant wind add
+// dir strength dt, turbulence its curl noise, noise its fBm along dir; the
+// z clamp constrains positions only and does nothing here.  (x, y, z) is
+// the node's world position.
+__device__ inline void apply_forces(float& vx, float& vy, float& vz, float x, float y, float z,
+                                    const nb_force* forces, uint32_t count, float dt) {
+  for (uint32_t fi = 0; fi < count; fi++) {
+    const nb_force f = forces[fi];
+    const uint32_t mask = f.affect_mask;
+    if (mask != 0u && (mask & 1u) == 0u) continue;
+    if (f.type == NB_FORCE_GRAVITY || f.type == NB_FORCE_CONSTANT_WIND) {
+      const float s = f.data[3];
+      vx += (f.data[0] * s) * dt;
+      vy += (f.data[1] * s) * dt;
+      vz += (f.data[2] * s) * dt;
+    } else if (f.type == NB_FORCE_TURBULENCE) {  // data: strength, frequency, octaves bits, seed
+#pragma clang fp contract(off)
+      const float fr = f.data[1], sd = f.data[3];
+      const float3 c = noise_curl3(x * fr + sd, y * fr + sd, z * fr + sd, __float_as_uint(f.data[2]));
+      vx += (c.x * f.data[0]) * dt;
+      vy += (c.y * f.data[0]) * dt;
+      vz += (c.z * f.data[0]) * dt;
+    } else if (f.type == NB_FORCE_NOISE) {  // data: dir xyz, strength, frequency, octaves bits, seed
+#pragma clang fp contract(off)
+      const float fr = f.data[4], sd = f.data[6];
+      const float n = noise_fbm3(x * fr + sd, y * fr + sd, z * fr + sd, __float_as_uint(f.data[5]));
+      const float k = (n * f.data[3]) * dt;
+      vx += f.data[0] * k;
+      vy += f.data[1] * k;
+      vz += f.data[2] * k;
+    }
+  }
+}
+
 // applyColliderBC — shaders/collider_common.glsl:62-73
 __device__ __forceinline__ void collider_bc(float& vx, float& vy, float& vz, float nx, float ny, float nz,
                                             const nb_collider& c) {

@@ -291,7 +291,7 @@ class ColliderSet:
 
 
 # ---------------------------------------------------------------------------
-# forces — src/core/Force.h:48-181 (gravity and constant wind act on the grid)
+# forces — src/core/Force.h:48-256 (gravity, wind, turbulence and noise act on the grid)
 # ---------------------------------------------------------------------------
 class Force:
     type_id = -1
@@ -337,6 +337,46 @@ class ConstantWindForce(GravityForce):
     @staticmethod
     def FromDirection(direction, magnitude):
         return ConstantWindForce(direction, magnitude)
+
+
+class TurbulenceForce(Force):
+    """Curl-noise wind (src/core/Force.h:180-215): data = strength, frequency,
+    octaves (uint bits), seed; advanceTime moves the seed by speed * dt."""
+    type_id = ForceType.TURBULENCE
+
+    def __init__(self, strength=1.0, frequency=0.5, octaves=3, seed=0.0, speed=0.3):
+        self.strength, self.frequency, self.octaves, self.seed, self.speed = strength, frequency, octaves, seed, speed
+        self.affectMask = 0
+
+    def pack_data(self, data):
+        data[0] = self.strength
+        data[1] = self.frequency
+        data[2] = np.array([self.octaves], np.uint32).view(np.float32)[0]
+        data[3] = self.seed
+
+    def advanceTime(self, dt):
+        self.seed = float(np.float32(self.seed) + np.float32(self.speed) * np.float32(dt))
+
+
+class NoiseForce(Force):
+    """fBm noise along a direction (src/core/Force.h:217-256): data = dir,
+    strength, frequency, octaves (uint bits), seed."""
+    type_id = ForceType.NOISE
+
+    def __init__(self, direction=(0.0, 1.0, 0.0), strength=1.0, frequency=1.0, octaves=1, seed=0.0, speed=0.0):
+        self.direction = tuple(direction)
+        self.strength, self.frequency, self.octaves, self.seed, self.speed = strength, frequency, octaves, seed, speed
+        self.affectMask = 0
+
+    def pack_data(self, data):
+        data[0], data[1], data[2] = self.direction
+        data[3] = self.strength
+        data[4] = self.frequency
+        data[5] = np.array([self.octaves], np.uint32).view(np.float32)[0]
+        data[6] = self.seed
+
+    def advanceTime(self, dt):
+        self.seed = float(np.float32(self.seed) + np.float32(self.speed) * np.float32(dt))
 
 
 class ZClampForce(Force):

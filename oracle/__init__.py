@@ -92,6 +92,12 @@ def lib():
                       ("orc_bspline2_f64", C.c_double), ("orc_bspline2g_f64", C.c_double)):
             getattr(L, nm).restype = t
             getattr(L, nm).argtypes = [t]
+        L.orc_noise_value.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int]
+        L.orc_noise_value.restype = C.c_double
+        L.orc_noise_fbm.argtypes = [C.c_double, C.c_double, C.c_double, C.c_uint32, C.c_int]
+        L.orc_noise_fbm.restype = C.c_double
+        L.orc_noise_curl.argtypes = [C.c_double, C.c_double, C.c_double, C.c_uint32, C.c_int, dp]
+        L.orc_noise_hash.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int, dp]
         L.orc_svd3_f32.argtypes = [fp, fp, fp, fp]
         L.orc_svd3_f64.argtypes = [dp, dp, dp, dp]
         L.orc_hencky_f32.argtypes = [fp, C.c_float, C.c_float, fp]
@@ -335,3 +341,24 @@ def project_drucker_prager(F, mu, lam, M, q_c, precision=32):
     else:
         lib().orc_project_dp_f64(_dptr(f), mu, lam, M, q_c, _dptr(o))
     return _unm(o)
+
+
+# procedural noise of the turbulence / noise forces (shaders/force_noise.glsl)
+def noise_value(p, precision=32):
+    return lib().orc_noise_value(float(p[0]), float(p[1]), float(p[2]), precision)
+
+
+def noise_fbm(p, octaves, precision=32):
+    return lib().orc_noise_fbm(float(p[0]), float(p[1]), float(p[2]), octaves, precision)
+
+
+def noise_curl(p, octaves, precision=32):
+    out = np.zeros(3, np.float64)
+    lib().orc_noise_curl(float(p[0]), float(p[1]), float(p[2]), octaves, precision, _dptr(out))
+    return out
+
+
+def noise_hash(p, precision=32):
+    out = np.zeros(3, np.float64)
+    lib().orc_noise_hash(float(p[0]), float(p[1]), float(p[2]), precision, _dptr(out))
+    return out

@@ -571,8 +571,58 @@ template <class R> inline void apply_collider_bc(V3<R>& v, V3<R> n, V3<R> vc, R 
 }
 
 // ---------------------------------------------------------------------------
+// Procedural noise of the turbulence and noise forces — shaders/force_noise.glsl
+// (hash 7-12, value noise 14-37, fBm 40-50, curl 54-82), in precision R, with
+// GLSL's fract(x) = x - floor(x) and mix(a, b, t) = a + (b - a) t.
+template <class R> R gl_fract(R x) { return x - std::floor(x); }
+template <class R> R gl_mix(R a, R b, R t) { return a + (b - a) * t; }
+template <class R> V3<R> noise_hash33(const V3<R>& p) {
+  const R a = p.x * R(127.1f) + p.y * R(311.7f) + p.z * R(74.7f);
+  const R b = p.x * R(269.5f) + p.y * R(183.3f) + p.z * R(246.1f);
+  const R c = p.x * R(113.5f) + p.y * R(271.9f) + p.z * R(124.6f);
+  const R k = R(43758.5453123f);
+  return V3<R>(R(-1) + R(2) * gl_fract(std::sin(a) * k), R(-1) + R(2) * gl_fract(std::sin(b) * k),
+               R(-1) + R(2) * gl_fract(std::sin(c) * k));
+}
+template <class R> R noise_value3(const V3<R>& p) {
+  const V3<R> i(std::floor(p.x), std::floor(p.y), std::floor(p.z));
+  const V3<R> f(gl_fract(p.x), gl_fract(p.y), gl_fract(p.z));
+  const V3<R> u(f.x * f.x * (R(3) - R(2) * f.x), f.y * f.y * (R(3) - R(2) * f.y), f.z * f.z * (R(3) - R(2) * f.z));
+  R n[8];
+  for (int c = 0; c < 8; c++) {
+    const V3<R> o(R(c & 1), R((c >> 1) & 1), R((c >> 2) & 1));
+    const V3<R> g = noise_hash33(V3<R>(i.x + o.x, i.y + o.y, i.z + o.z));
+    n[c] = g.x * (f.x - o.x) + g.y * (f.y - o.y) + g.z * (f.z - o.z);
+  }
+  const R x00 = gl_mix(n[0], n[1], u.x), x10 = gl_mix(n[2], n[3], u.x);
+  const R x01 = gl_mix(n[4], n[5], u.x), x11 = gl_mix(n[6], n[7], u.x);
+  return gl_mix(gl_mix(x00, x10, u.y), gl_mix(x01, x11, u.y), u.z);
+}
+template <class R> R noise_fbm3(const V3<R>& p, uint32_t octaves) {
+  R sum = 0, amp = R(0.5), freq = 1;
+  const uint32_t n = octaves > 1u ? octaves : 1u;
+  for (uint32_t i = 0; i < n; i++) {
+    sum += amp * noise_value3(V3<R>(p.x * freq, p.y * freq, p.z * freq));
+    freq *= R(2);
+    amp *= R(0.5);
+  }
+  return sum;
+}
+template <class R> V3<R> noise_curl3(const V3<R>& p, uint32_t oct) {
+  const R e = R(0.1f), ox = R(19.1f), oy = R(47.3f), oz = R(71.7f);
+  auto F = [&](R x, R y, R z) { return noise_fbm3(V3<R>(x, y, z), oct); };
+  const R px1 = F(p.x + ox, p.y + e, p.z), px2 = F(p.x + ox, p.y - e, p.z);
+  const R py3 = F(p.x + e, p.y + oy, p.z), py4 = F(p.x - e, p.y + oy, p.z);
+  const R py1 = F(p.x, p.y + oy, p.z + e), py2 = F(p.x, p.y + oy, p.z - e);
+  const R pz3 = F(p.x, p.y + e, p.z + oz), pz4 = F(p.x, p.y - e, p.z + oz);
+  const R px3 = F(p.x + ox, p.y, p.z + e), px4 = F(p.x + ox, p.y, p.z - e);
+  const R pz1 = F(p.x + e, p.y, p.z + oz), pz2 = F(p.x - e, p.y, p.z + oz);
+  const R s = R(2) * e;
+  return V3<R>((pz3 - pz4) / s - (py1 - py2) / s, (px3 - px4) / s - (pz1 - pz2) / s, (py3 - py4) / s - (px1 - px2) / s);
+}
+
 // Grid update — shaders/mpm_grid_update.comp:17-101, with the force block the
-// reference generates (src/core/ForceShaderCompiler.cpp:72-93, Force.h:136-142).
+// reference generates (src/core/ForceShaderCompiler.cpp:72-93, Force.h:136-256).
 // ---------------------------------------------------------------------------
 template <class R> void grid_update(const Params& p, State<R>& s) {
   const uint32_t NC = total_cells(p);
@@ -608,6 +658,19 @@ template <class R> void grid_update(const Params& p, State<R>& s) {
         if (f.type == 0u || f.type == 1u) {  // gravity, constant wind: v += dir*strength*dt
           V3<R> dir(R(f.data[0]), R(f.data[1]), R(f.data[2]));
           fv = fv + (dir * R(f.data[3])) * dt;
+        } else if (f.type == 2u) {  // turbulence: v += curl(fBm)(pos freq + seed) strength dt
+          uint32_t oct;
+          std::memcpy(&oct, &f.data[2], 4);
+          const R fr = R(f.data[1]), sd = R(f.data[3]);
+          const V3<R> c = noise_curl3(V3<R>(fp.x * fr + sd, fp.y * fr + sd, fp.z * fr + sd), oct);
+          fv = fv + (c * R(f.data[0])) * dt;
+        } else if (f.type == 3u) {  // noise: v += dir (fBm(pos freq + seed) strength dt)
+          uint32_t oct;
+          std::memcpy(&oct, &f.data[5], 4);
+          const R fr = R(f.data[4]), sd = R(f.data[6]);
+          const R n = noise_fbm3(V3<R>(fp.x * fr + sd, fp.y * fr + sd, fp.z * fr + sd), oct);
+          V3<R> dir(R(f.data[0]), R(f.data[1]), R(f.data[2]));
+          fv = fv + dir * ((n * R(f.data[3])) * dt);
         }
       }
       v = fv;

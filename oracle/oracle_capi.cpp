@@ -283,4 +283,33 @@ void orc_project_dp_f64(const double* F, double mu, double lam, double M, double
   store_m3(project_drucker_prager(load_m3(F), mu, lam, M, qc), out);
 }
 
+
+// procedural noise of the turbulence / noise forces (shaders/force_noise.glsl)
+double orc_noise_value(double x, double y, double z, int precision) {
+  if (precision == 64) return oracle::noise_value3(oracle::V3<double>(x, y, z));
+  return double(oracle::noise_value3(oracle::V3<float>(float(x), float(y), float(z))));
+}
+double orc_noise_fbm(double x, double y, double z, uint32_t octaves, int precision) {
+  if (precision == 64) return oracle::noise_fbm3(oracle::V3<double>(x, y, z), octaves);
+  return double(oracle::noise_fbm3(oracle::V3<float>(float(x), float(y), float(z)), octaves));
+}
+void orc_noise_curl(double x, double y, double z, uint32_t octaves, int precision, double out[3]) {
+  if (precision == 64) {
+    const oracle::V3<double> c = oracle::noise_curl3(oracle::V3<double>(x, y, z), octaves);
+    out[0] = c.x; out[1] = c.y; out[2] = c.z;
+  } else {
+    const oracle::V3<float> c = oracle::noise_curl3(oracle::V3<float>(float(x), float(y), float(z)), octaves);
+    out[0] = c.x; out[1] = c.y; out[2] = c.z;
+  }
+}
+void orc_noise_hash(double x, double y, double z, int precision, double out[3]) {
+  if (precision == 64) {
+    const oracle::V3<double> c = oracle::noise_hash33(oracle::V3<double>(x, y, z));
+    out[0] = c.x; out[1] = c.y; out[2] = c.z;
+  } else {
+    const oracle::V3<float> c = oracle::noise_hash33(oracle::V3<float>(float(x), float(y), float(z)));
+    out[0] = c.x; out[1] = c.y; out[2] = c.z;
+  }
+}
+
 }  // extern "C"

@@ -501,3 +501,37 @@ def test_fp32_oracle_tracks_fp64_twin():
     h = 1.0 / 16
     assert np.abs(a[0, :, :3] - b[0, :, :3]).max() < 1e-4 * h
     assert np.abs(a[1, :, :3] - b[1, :, :3]).max() < 1e-4
+
+
+# ---- procedural noise of the turbulence / noise forces (shaders/force_noise.glsl) ----
+def test_noise_vanishes_on_the_lattice_and_hash_is_bounded():
+    """Value noise is sum over corners of dot(hash, f - corner) blended by the
+    smoothstep of f: at an integer point f = 0 and the blend picks corner 0,
+    whose dot product is 0."""
+    rng = np.random.default_rng(0)
+    for prec in (32, 64):
+        for p in rng.integers(-50, 50, (20, 3)):
+            assert oracle.noise_value(p, prec) == 0.0
+        for p in rng.uniform(-100, 100, (200, 3)):
+            h = oracle.noise_hash(p, prec)
+            assert np.all(np.abs(h) <= 1.0)
+
+
+def test_noise_is_continuous_across_cell_faces():
+    rng = np.random.default_rng(1)
+    for p in rng.uniform(-20, 20, (50, 3)):
+        a = np.floor(p) + np.array([1.0 - 1e-9, 0.3, 0.6])
+        b = np.floor(p) + np.array([1.0 + 1e-9, 0.3, 0.6])
+        assert abs(oracle.noise_value(a, 64) - oracle.noise_value(b, 64)) < 1e-6
+
+
+def test_fbm_octaves_add_halved_amplitude_at_doubled_frequency():
+    rng = np.random.default_rng(2)
+    for prec in (32, 64):
+        for p in rng.uniform(-10, 10, (30, 3)):
+            one = oracle.noise_fbm(p, 1, prec)
+            two = oracle.noise_fbm(p, 2, prec)
+            assert one == pytest.approx(0.5 * oracle.noise_value(p, prec), abs=1e-6)
+            assert two == pytest.approx(one + 0.25 * oracle.noise_value(2.0 * np.asarray(p), prec), abs=1e-6)
+            assert oracle.noise_fbm(p, 0, prec) == one  # octaves clamp to >= 1
+
