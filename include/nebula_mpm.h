@@ -280,8 +280,44 @@ int nb_mpm_set_forces(nb_mpm* h, const nb_force* forces, uint32_t n);
  * n float4 (pos = (x, Vp), vel = (v, bits(id))); F = I, B = 0, tau = 0; silently
  * clamped to capacity.  *appended (may be NULL) receives the count written. */
 int nb_mpm_append_particles(nb_mpm* h, const float* pos4, const float* vel4, uint32_t n, uint32_t* appended);
-/* MPMEngine::resetParticles (src/engine/MPMEngine.cpp:281-287). */
+/* MPMEngine::resetParticles (src/engine/MPMEngine.cpp:281-287): no live
+ * particles; every emitter's step count and the emitter RNG (mt19937, seed
+ * 12345) start again. */
 int nb_mpm_reset_particles(nb_mpm* h);
+
+/* ---- Emitters ----
+ * MPMEngine::addEmitter / clearEmitters / emitFromEmitters
+ * (src/engine/MPMEngine.cpp:271-352) with the shapes of src/core/Emitter.h:
+ * 39-224 in one record.  nb_mpm_step emits first, as the reference's step
+ * does; nb_mpm_emit runs that part alone.  Particles are sampled on the host
+ * with std::mt19937 (seed 12345, MPMEngine.h:165) and
+ * std::uniform_real_distribution<float>, shape by shape as Emitter::sample,
+ * then velocities as Emitter::sample_velocity; where the reference draws the
+ * three components inside one constructor call (an order C++ leaves to the
+ * compiler) they are drawn x, y, z.  Volume per particle:
+ * MPMConfig::particleVolume (domain / (nx, ny, nz)). */
+#define NB_EMITTER_AABB 0u
+#define NB_EMITTER_SPHERE 1u
+#define NB_EMITTER_ELLIPSE 2u
+typedef struct nb_emitter {
+  uint32_t shape;              /* NB_EMITTER_* */
+  float center[3];
+  float size[3];               /* AABB: edge lengths; SPHERE: radius in size[0];
+                                  ELLIPSE: semi-axes along x and y in size[0], size[1] (z = center z) */
+  float vel[3];                /* initial velocity (its length along the normal with emit_along_normal) */
+  float center_vel[3];         /* the emitter moves by center_vel dt per emitting step */
+  int32_t particles_per_step;
+  uint32_t particle_type;      /* material id */
+  uint32_t emit_along_normal;  /* 1: along the shape's outward normal */
+  float velocity_randomness;   /* > 0: add a uniform [-r, r]^3 perturbation */
+  int32_t step_count;          /* 0: every step, -1: the first step only, n > 0: n steps */
+  uint32_t reserved[2];
+} nb_emitter;
+int nb_mpm_add_emitter(nb_mpm* h, const nb_emitter* e);
+int nb_mpm_clear_emitters(nb_mpm* h);
+int nb_mpm_emit(nb_mpm* h, float dt);
+/* Emitter i as it stands (its center moves); *count the number of emitters. */
+int nb_mpm_get_emitter(const nb_mpm* h, uint32_t i, nb_emitter* e, uint32_t* count);
 
 /* Bulk state access in the reference's float4 layout (host pointers,
  * synchronous).  Rows [first, first+count) of field NB_FIELD_*. */

@@ -160,6 +160,108 @@ struct GravityForce : Force {
 struct ConstantWindForce : GravityForce {
   uint32_t type() const override { return NB_FORCE_CONSTANT_WIND; }
 };
+// src/core/Force.h:180-256: curl-noise turbulence and fBm noise, their seeds
+// advanced by speed * dt on every step
+struct TurbulenceForce : Force {
+  float strength = 1.0f, frequency = 0.5f;
+  uint32_t octaves = 3u;
+  float seed = 0.0f, speed = 0.3f;
+  uint32_t type() const override { return NB_FORCE_TURBULENCE; }
+  void packData(float* out) const override {
+    out[0] = strength; out[1] = frequency;
+    std::memcpy(&out[2], &octaves, 4);
+    out[3] = seed;
+  }
+  void advanceTime(float dt) override { seed += speed * dt; }
+};
+struct NoiseForce : Force {
+  vec4 direction{0.0f, 1.0f, 0.0f, 0.0f};
+  float strength = 1.0f, frequency = 1.0f;
+  uint32_t octaves = 1u;
+  float seed = 0.0f, speed = 0.0f;
+  uint32_t type() const override { return NB_FORCE_NOISE; }
+  void packData(float* out) const override {
+    out[0] = direction[0]; out[1] = direction[1]; out[2] = direction[2];
+    out[3] = strength; out[4] = frequency;
+    std::memcpy(&out[5], &octaves, 4);
+    out[6] = seed;
+  }
+  void advanceTime(float dt) override { seed += speed * dt; }
+};
+
+// src/core/Emitter.h:39-224 (the MPM-side fields): shapes as nb_emitter records
+struct Emitter {
+  virtual ~Emitter() = default;
+  vec4 center{0.f, 0.f, 0.f, 0.f};
+  vec4 vel{0.f, 0.f, 0.f, 0.f};
+  vec4 center_vel{0.f, 0.f, 0.f, 0.f};
+  int particles_per_step = 1024;
+  uint32_t particleType = 1u;
+  bool emitAlongNormal = false;
+  float velocityRandomness = 0.0f;
+  int step_count = -1;
+  virtual nb_emitter pack() const = 0;
+
+ protected:
+  nb_emitter packCommon(uint32_t shape) const {
+    nb_emitter e{};
+    e.shape = shape;
+    for (int a = 0; a < 3; a++) {
+      e.center[a] = center[a];
+      e.vel[a] = vel[a];
+      e.center_vel[a] = center_vel[a];
+    }
+    e.particles_per_step = particles_per_step;
+    e.particle_type = particleType;
+    e.emit_along_normal = emitAlongNormal ? 1u : 0u;
+    e.velocity_randomness = velocityRandomness;
+    e.step_count = step_count;
+    return e;
+  }
+};
+struct AABBEmitter : Emitter {
+  vec4 size{1.f, 1.f, 1.f, 0.f};  // edge lengths
+  nb_emitter pack() const override {
+    nb_emitter e = packCommon(NB_EMITTER_AABB);
+    for (int a = 0; a < 3; a++) e.size[a] = size[a];
+    return e;
+  }
+  static std::shared_ptr<AABBEmitter> FromAABB(vec4 mn, vec4 mx, vec4 vel, int particles_per_step) {
+    auto e = std::make_shared<AABBEmitter>();
+    for (int a = 0; a < 3; a++) {
+      e->center[a] = (mn[a] + mx[a]) * 0.5f;
+      e->size[a] = mx[a] - mn[a];
+    }
+    e->vel = vel;
+    e->particles_per_step = particles_per_step;
+    return e;
+  }
+};
+struct SphereEmitter : Emitter {
+  float radius = 1.0f;
+  nb_emitter pack() const override {
+    nb_emitter e = packCommon(NB_EMITTER_SPHERE);
+    e.size[0] = e.size[1] = e.size[2] = radius;
+    return e;
+  }
+  static std::shared_ptr<SphereEmitter> FromSphere(vec4 center, float radius, vec4 vel, int particles_per_step) {
+    auto e = std::make_shared<SphereEmitter>();
+    e->center = center;
+    e->radius = radius;
+    e->vel = vel;
+    e->particles_per_step = particles_per_step;
+    return e;
+  }
+};
+struct EllipseEmitter : Emitter {
+  float semiA = 1.0f, semiB = 1.0f;  // x and y semi-axes (z fixed at the center)
+  nb_emitter pack() const override {
+    nb_emitter e = packCommon(NB_EMITTER_ELLIPSE);
+    e.size[0] = semiA;
+    e.size[1] = semiB;
+    return e;
+  }
+};
 
 // MPMConfig (src/engine/MPMEngine.h:22-55)
 struct MPMConfig : nb_mpm_config {
@@ -190,6 +292,22 @@ class MPMEngine {
     pushKnobs();
     if (!forces_.empty()) uploadForces(dt);
     check(nb_mpm_step(h_, dt));
+    syncEmitters();
+  }
+
+  // emitters (src/engine/MPMEngine.cpp:271-352); their centers follow the engine's
+  void addEmitter(std::shared_ptr<Emitter> e) {
+    const nb_emitter rec = e->pack();
+    check(nb_mpm_add_emitter(h_, &rec));
+    emitters_.push_back(std::move(e));
+  }
+  void clearEmitters() {
+    check(nb_mpm_clear_emitters(h_));
+    emitters_.clear();
+  }
+  void emitFromEmitters(float dt) {
+    check(nb_mpm_emit(h_, dt));
+    syncEmitters();
   }
   void sync() { check(nb_mpm_sync(h_)); }
 
@@ -307,6 +425,14 @@ class MPMEngine {
   MPMConfig cfg_;
   nb_mpm_knobs k_{};
   std::vector<std::shared_ptr<Force>> forces_;
+  std::vector<std::shared_ptr<Emitter>> emitters_;
+  void syncEmitters() {
+    for (uint32_t i = 0; i < emitters_.size(); i++) {
+      nb_emitter rec{};
+      check(nb_mpm_get_emitter(h_, i, &rec, nullptr));
+      for (int a = 0; a < 3; a++) emitters_[i]->center[a] = rec.center[a];
+    }
+  }
 };
 
 }  // namespace nebula

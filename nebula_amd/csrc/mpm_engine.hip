@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <random>
 #include <vector>
 
 #include "../../include/nebula_mpm.h"
@@ -81,6 +82,11 @@ struct nb_mpm {
   uint32_t* d_tile_scene = nullptr;
   uint32_t* d_pscene = nullptr;
   bool batch() const { return n_scenes > 0; }
+
+  // emitters (MPMEngine.h:162-166)
+  std::vector<nb_emitter> emitters;
+  std::vector<int> emit_done;
+  std::mt19937 emit_rng{12345u};
 
   // launch shapes and the captured substep (nb_mpm_graph_substeps)
   nb_mpm_tuning tuning{2048u, 256u, 12u, 0.f};
@@ -267,6 +273,73 @@ int enqueue_substep(nb_mpm* h, float dt, Prof* pf, const TapeIO& io = TapeIO{}) 
 }
 
 int engine_alloc(nb_mpm* h, void* hip_stream);
+
+// ---- emitter sampling (src/core/Emitter.h) ----
+// A point of the emitter's shape: AABB uniform in the box (Emitter.h:100-107),
+// sphere r = R cbrt(u), theta, phi uniform (:147-155), ellipse by rejection in
+// the x-y bounding box with z at the center (:198-208).
+void emitter_sample(const nb_emitter& e, std::mt19937& rng, float& x, float& y, float& z) {
+  const float cx = e.center[0], cy = e.center[1], cz = e.center[2];
+  if (e.shape == NB_EMITTER_AABB) {
+    const float hx = e.size[0] * 0.5f, hy = e.size[1] * 0.5f, hz = e.size[2] * 0.5f;
+    std::uniform_real_distribution<float> dx(-hx, hx), dy(-hy, hy), dz(-hz, hz);
+    const float ox = dx(rng);
+    const float oy = dy(rng);
+    const float oz = dz(rng);
+    x = cx + ox; y = cy + oy; z = cz + oz;
+  } else if (e.shape == NB_EMITTER_SPHERE) {
+    std::uniform_real_distribution<float> dr(0.0f, 1.0f), dt(0.0f, 3.14159265f), dp(0.0f, 6.28318530f);
+    const float r = e.size[0] * std::cbrt(dr(rng));
+    const float th = dt(rng);
+    const float ph = dp(rng);
+    x = cx + r * std::sin(th) * std::cos(ph);
+    y = cy + r * std::sin(th) * std::sin(ph);
+    z = cz + r * std::cos(th);
+  } else {
+    const float a = e.size[0], b = e.size[1];
+    std::uniform_real_distribution<float> dx(-a, a), dy(-b, b);
+    float ex, ey;
+    do {
+      ex = dx(rng);
+      ey = dy(rng);
+    } while ((ex * ex) / (a * a) + (ey * ey) / (b * b) > 1.0f);
+    x = cx + ex; y = cy + ey; z = cz;
+  }
+}
+
+// Emitter::sample_velocity (Emitter.h:66-74) with the outward normals of
+// Emitter::surface_normal (:59-63; the box's own at :109-117).
+void emitter_velocity(const nb_emitter& e, float px, float py, float pz, std::mt19937& rng, float& vx, float& vy,
+                      float& vz) {
+  vx = e.vel[0]; vy = e.vel[1]; vz = e.vel[2];
+  if (e.emit_along_normal) {
+    float nx, ny, nz;
+    const float dx = px - e.center[0], dy = py - e.center[1], dz = pz - e.center[2];
+    if (e.shape == NB_EMITTER_AABB) {
+      const float hx = std::max(e.size[0] * 0.5f, 1e-6f), hy = std::max(e.size[1] * 0.5f, 1e-6f),
+                  hz = std::max(e.size[2] * 0.5f, 1e-6f);
+      const float ux = dx / hx, uy = dy / hy, uz = dz / hz;
+      const float ax = std::fabs(ux), ay = std::fabs(uy), az = std::fabs(uz);
+      nx = ny = nz = 0.0f;
+      if (ax >= ay && ax >= az) nx = ux >= 0.0f ? 1.0f : -1.0f;
+      else if (ay >= az) ny = uy >= 0.0f ? 1.0f : -1.0f;
+      else nz = uz >= 0.0f ? 1.0f : -1.0f;
+    } else {
+      const float len = std::sqrt(dx * dx + dy * dy + dz * dz);
+      if (len > 1e-6f) { nx = dx / len; ny = dy / len; nz = dz / len; }
+      else { nx = 0.0f; ny = 0.0f; nz = 1.0f; }
+    }
+    const float speed = std::sqrt(e.vel[0] * e.vel[0] + e.vel[1] * e.vel[1] + e.vel[2] * e.vel[2]);
+    vx = nx * speed; vy = ny * speed; vz = nz * speed;
+  }
+  if (e.velocity_randomness > 0.0f) {
+    std::uniform_real_distribution<float> d(-e.velocity_randomness, e.velocity_randomness);
+    const float rx = d(rng);
+    const float ry = d(rng);
+    const float rz = d(rng);
+    vx += rx; vy += ry; vz += rz;
+  }
+}
 
 int run_substeps(nb_mpm* h, float dt, uint32_t n, Prof* pf) {
   if (n == 0 || h->n_live == 0) return NB_OK;
@@ -595,6 +668,10 @@ int nb_mpm_destroy(nb_mpm* h) {
 
 int nb_mpm_step(nb_mpm* h, float dt) {
   if (!h) return fail(NB_ERR_ARG, "null handle");
+  if (!h->emitters.empty()) {  // src/engine/MPMEngine.cpp:405-406
+    const int rc = nb_mpm_emit(h, dt);
+    if (rc != NB_OK) return rc;
+  }
   const int ns = std::max(1, h->knobs.num_substeps);
   return run_substeps(h, dt / float(ns), uint32_t(ns), nullptr);
 }
@@ -904,6 +981,73 @@ int nb_mpm_reset_particles(nb_mpm* h) {
   NB_HIP(hipStreamSynchronize(h->stream));
   h->n_live = 0;
   h->bins_valid = false;
+  std::fill(h->emit_done.begin(), h->emit_done.end(), 0);
+  h->emit_rng.seed(12345u);
+  return NB_OK;
+}
+
+int nb_mpm_add_emitter(nb_mpm* h, const nb_emitter* e) {
+  if (!h || !e) return fail(NB_ERR_ARG, "null argument");
+  if (h->batch()) return fail(NB_ERR_UNSUPPORTED, "%s: not available on a batch of scenes", __func__);
+  if (e->shape > NB_EMITTER_ELLIPSE) return fail(NB_ERR_ARG, "unknown emitter shape %u", e->shape);
+  if (e->shape == NB_EMITTER_ELLIPSE && !(e->size[0] > 0.f && e->size[1] > 0.f))
+    return fail(NB_ERR_ARG, "an ellipse needs positive semi-axes");
+  h->emitters.push_back(*e);
+  h->emit_done.push_back(0);
+  return NB_OK;
+}
+
+int nb_mpm_clear_emitters(nb_mpm* h) {
+  if (!h) return fail(NB_ERR_ARG, "null handle");
+  h->emitters.clear();
+  h->emit_done.clear();
+  return NB_OK;
+}
+
+int nb_mpm_get_emitter(const nb_mpm* h, uint32_t i, nb_emitter* e, uint32_t* count) {
+  if (!h) return fail(NB_ERR_ARG, "null handle");
+  if (count) *count = uint32_t(h->emitters.size());
+  if (!e) return NB_OK;
+  if (i >= h->emitters.size()) return fail(NB_ERR_RANGE, "emitter %u of %zu", i, h->emitters.size());
+  *e = h->emitters[i];
+  return NB_OK;
+}
+
+int nb_mpm_emit(nb_mpm* h, float dt) {
+  if (!h) return fail(NB_ERR_ARG, "null handle");
+  if (h->batch()) return fail(NB_ERR_UNSUPPORTED, "%s: not available on a batch of scenes", __func__);
+  const float Vp = (h->cfg.domain_size[0] / float(h->cfg.nx)) * (h->cfg.domain_size[1] / float(h->cfg.ny)) *
+                   (h->cfg.domain_size[2] / float(h->cfg.nz));  // MPMConfig::particleVolume
+  for (size_t si = 0; si < h->emitters.size(); si++) {
+    nb_emitter& e = h->emitters[si];
+    int& done = h->emit_done[si];
+    const bool go = e.step_count == -1 ? done == 0 : (e.step_count == 0 ? true : done < e.step_count);
+    if (!go) continue;
+    const int avail = int(h->cap) - int(h->n_live);
+    const int n = std::min(e.particles_per_step, avail);
+    if (n <= 0) {
+      done++;
+      continue;
+    }
+    std::vector<float> pos(size_t(n) * 4), vel(size_t(n) * 4);
+    float id_bits;
+    std::memcpy(&id_bits, &e.particle_type, 4);
+    for (int j = 0; j < n; j++) {  // all positions, then all velocities (MPMEngine.cpp:324-336)
+      float x, y, z;
+      emitter_sample(e, h->emit_rng, x, y, z);
+      pos[4 * j + 0] = x; pos[4 * j + 1] = y; pos[4 * j + 2] = z; pos[4 * j + 3] = Vp;
+    }
+    for (int j = 0; j < n; j++) {
+      float vx, vy, vz;
+      emitter_velocity(e, pos[4 * j + 0], pos[4 * j + 1], pos[4 * j + 2], h->emit_rng, vx, vy, vz);
+      vel[4 * j + 0] = vx; vel[4 * j + 1] = vy; vel[4 * j + 2] = vz; vel[4 * j + 3] = id_bits;
+    }
+    uint32_t got = 0;
+    const int rc = nb_mpm_append_particles(h, pos.data(), vel.data(), uint32_t(n), &got);
+    if (rc != NB_OK) return rc;
+    done++;
+    for (int a = 0; a < 3; a++) e.center[a] += e.center_vel[a] * dt;
+  }
   return NB_OK;
 }
 

@@ -417,7 +417,99 @@ EXPORTED_SYMBOLS = (
     "nb_mpm_adjoint_device_ptr", "nb_mpm_backward", "nb_mpm_adjoint_end",
     "nb_mpm_create_batch", "nb_mpm_scene_count", "nb_mpm_scene_info",
     "nb_mpm_get_tuning", "nb_mpm_set_tuning", "nb_mpm_autotune", "nb_mpm_graph_substeps",
+    "nb_mpm_add_emitter", "nb_mpm_clear_emitters", "nb_mpm_emit", "nb_mpm_get_emitter",
 )
+
+
+class EmitterGPU(C.Structure):
+    """nb_emitter: one emitter record (shape, center, size, velocities, rate, ...)."""
+    _fields_ = [("shape", C.c_uint32), ("center", C.c_float * 3), ("size", C.c_float * 3), ("vel", C.c_float * 3),
+                ("center_vel", C.c_float * 3), ("particles_per_step", C.c_int32), ("particle_type", C.c_uint32),
+                ("emit_along_normal", C.c_uint32), ("velocity_randomness", C.c_float), ("step_count", C.c_int32),
+                ("reserved", C.c_uint32 * 2)]
+
+
+class Emitter:
+    """src/core/Emitter.h:39-96 (the MPM-side fields)."""
+    shape = 0
+
+    def __init__(self):
+        self.center = (0.0, 0.0, 0.0)
+        self.vel = (0.0, 0.0, 0.0)
+        self.center_vel = (0.0, 0.0, 0.0)
+        self.particles_per_step = 1024
+        self.particleType = 1
+        self.emitAlongNormal = False
+        self.velocityRandomness = 0.0
+        self.step_count = -1
+
+    def _size(self):
+        return (0.0, 0.0, 0.0)
+
+    def pack(self) -> EmitterGPU:
+        g = EmitterGPU()
+        g.shape = self.shape
+        g.center[:] = [float(x) for x in self.center]
+        g.size[:] = [float(x) for x in self._size()]
+        g.vel[:] = [float(x) for x in self.vel]
+        g.center_vel[:] = [float(x) for x in self.center_vel]
+        g.particles_per_step = int(self.particles_per_step)
+        g.particle_type = int(self.particleType)
+        g.emit_along_normal = 1 if self.emitAlongNormal else 0
+        g.velocity_randomness = float(self.velocityRandomness)
+        g.step_count = int(self.step_count)
+        return g
+
+
+class AABBEmitter(Emitter):
+    """src/core/Emitter.h:97-138: uniform in a box of edge lengths ``size``."""
+    shape = 0
+
+    def __init__(self):
+        super().__init__()
+        self.size = (1.0, 1.0, 1.0)
+
+    def _size(self):
+        return self.size
+
+    @staticmethod
+    def FromAABB(mn, mx, vel, particles_per_step):
+        e = AABBEmitter()
+        e.center = tuple((a + b) * 0.5 for a, b in zip(mn, mx))
+        e.size = tuple(b - a for a, b in zip(mn, mx))
+        e.vel = tuple(vel)
+        e.particles_per_step = particles_per_step
+        return e
+
+
+class SphereEmitter(Emitter):
+    """src/core/Emitter.h:141-174: uniform in a ball."""
+    shape = 1
+
+    def __init__(self):
+        super().__init__()
+        self.radius = 1.0
+
+    def _size(self):
+        return (self.radius, self.radius, self.radius)
+
+    @staticmethod
+    def FromSphere(center, radius, vel, particles_per_step):
+        e = SphereEmitter()
+        e.center, e.radius, e.vel, e.particles_per_step = tuple(center), radius, tuple(vel), particles_per_step
+        return e
+
+
+class EllipseEmitter(Emitter):
+    """src/core/Emitter.h:176-224 (sample(): semiA along x, semiB along y, z at the center)."""
+    shape = 2
+
+    def __init__(self):
+        super().__init__()
+        self.semiA, self.semiB = 1.0, 1.0
+
+    def _size(self):
+        return (self.semiA, self.semiB, 0.0)
 
 TUNE_ROUNDING = 1
 
@@ -469,6 +561,10 @@ def lib():
         L.nb_mpm_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
         L.nb_mpm_autotune.argtypes = [vp, C.c_float, C.c_uint32, C.c_uint32, C.POINTER(Tuning)]
         L.nb_mpm_graph_substeps.argtypes = [vp, C.c_float, C.c_uint32]
+        L.nb_mpm_add_emitter.argtypes = [vp, C.POINTER(EmitterGPU)]
+        L.nb_mpm_clear_emitters.argtypes = [vp]
+        L.nb_mpm_emit.argtypes = [vp, C.c_float]
+        L.nb_mpm_get_emitter.argtypes = [vp, C.c_uint32, C.POINTER(EmitterGPU), up]
         L.nb_mpm_config_default.argtypes = [C.POINTER(MPMConfig)]
         L.nb_mpm_config_default.restype = None
         L.nb_mpm_knobs_default.argtypes = [C.POINTER(_Knobs)]
@@ -564,6 +660,26 @@ class MPMEngine:
         e._n_cells = e.totalCells()
         return e
 
+    # emitters (src/engine/MPMEngine.cpp:271-352) ------------------------------
+    def addEmitter(self, e: "Emitter"):
+        g = e.pack()
+        _check(lib().nb_mpm_add_emitter(self._h, C.byref(g)))
+        self.__dict__.setdefault("_emitters", []).append(e)
+
+    def clearEmitters(self):
+        _check(lib().nb_mpm_clear_emitters(self._h))
+        self._emitters = []
+
+    def emitFromEmitters(self, dt: float):
+        _check(lib().nb_mpm_emit(self._h, dt))
+        self._sync_emitters()
+
+    def _sync_emitters(self):
+        for i, e in enumerate(getattr(self, "_emitters", [])):
+            g = EmitterGPU()
+            _check(lib().nb_mpm_get_emitter(self._h, i, C.byref(g), None))
+            e.center = tuple(g.center)
+
     # launch shapes and graph replay ---------------------------------------
     def tuning(self) -> "Tuning":
         t = Tuning()
@@ -619,6 +735,7 @@ class MPMEngine:
         if self._forces:
             self._upload_forces()
         _check(lib().nb_mpm_step(self._h, dt))
+        self._sync_emitters()
 
     def substep(self, dt_sub: float, n: int = 1):
         _check(lib().nb_mpm_substeps(self._h, dt_sub, n))

@@ -92,6 +92,8 @@ def lib():
                       ("orc_bspline2_f64", C.c_double), ("orc_bspline2g_f64", C.c_double)):
             getattr(L, nm).restype = t
             getattr(L, nm).argtypes = [t]
+        L.orc_emit.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_float, C.c_uint32, C.c_uint32, C.c_float, fp, fp]
+        L.orc_emit.restype = C.c_uint32
         L.orc_noise_value.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int]
         L.orc_noise_value.restype = C.c_double
         L.orc_noise_fbm.argtypes = [C.c_double, C.c_double, C.c_double, C.c_uint32, C.c_int]
@@ -362,3 +364,14 @@ def noise_hash(p, precision=32):
     out = np.zeros(3, np.float64)
     lib().orc_noise_hash(float(p[0]), float(p[1]), float(p[2]), precision, _dptr(out))
     return out
+
+
+def emit(records: bytes, n_emitters: int, n_calls: int, dt: float, capacity: int, live: int, Vp: float):
+    """The particles MPMEngine::emitFromEmitters would append over n_calls
+    calls: (pos4, vel4) float32 arrays."""
+    room = max(capacity - live, 0)
+    pos = np.zeros((room, 4), np.float32)
+    vel = np.zeros((room, 4), np.float32)
+    buf = C.create_string_buffer(bytes(records), max(len(records), 1))
+    n = lib().orc_emit(buf, n_emitters, n_calls, dt, capacity, live, Vp, _fptr(pos), _fptr(vel))
+    return pos[:n], vel[:n]

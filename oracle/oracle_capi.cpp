@@ -5,6 +5,7 @@
 #include <chrono>
 #include <cstdint>
 #include <cstring>
+#include <random>
 #include <vector>
 
 #include "mpm_oracle.hpp"
@@ -310,6 +311,115 @@ void orc_noise_hash(double x, double y, double z, int precision, double out[3]) 
     const oracle::V3<float> c = oracle::noise_hash33(oracle::V3<float>(float(x), float(y), float(z)));
     out[0] = c.x; out[1] = c.y; out[2] = c.z;
   }
+}
+
+
+// MPMEngine::emitFromEmitters (src/engine/MPMEngine.cpp:289-352) over n_calls
+// calls with dt: the particles it would append, in order (positions (x, Vp),
+// velocities (v, bits(type))).  Records are the library's 80-byte emitter
+// records; rng: std::mt19937 seeded 12345 (MPMEngine.h:165); the components
+// of one glm::vec3 constructed from three draws are drawn x, y, z.
+struct OrcEmitter {
+  uint32_t shape;
+  float center[3], size[3], vel[3], center_vel[3];
+  int32_t particles_per_step;
+  uint32_t particle_type, emit_along_normal;
+  float velocity_randomness;
+  int32_t step_count;
+  uint32_t reserved[2];
+};
+static_assert(sizeof(OrcEmitter) == 80, "emitter record");
+
+uint32_t orc_emit(const void* records, uint32_t n_emitters, uint32_t n_calls, float dt, uint32_t capacity,
+                  uint32_t live, float Vp, float* pos4, float* vel4) {
+  std::vector<OrcEmitter> em(n_emitters);
+  std::memcpy(em.data(), records, sizeof(OrcEmitter) * n_emitters);
+  std::vector<int> done(n_emitters, 0);
+  std::mt19937 rng(12345u);
+  uint32_t out = 0;
+  for (uint32_t call = 0; call < n_calls; call++) {
+    for (uint32_t si = 0; si < n_emitters; si++) {
+      OrcEmitter& e = em[si];
+      bool emit;
+      if (e.step_count == -1) emit = done[si] == 0;
+      else if (e.step_count == 0) emit = true;
+      else emit = done[si] < e.step_count;
+      if (!emit) continue;
+      const int n = std::min(e.particles_per_step, int(capacity) - int(live));
+      if (n <= 0) {
+        done[si]++;
+        continue;
+      }
+      float idf;
+      std::memcpy(&idf, &e.particle_type, 4);
+      std::vector<float> px(n), py(n), pz(n);
+      for (int j = 0; j < n; j++) {  // Emitter::sample, Emitter.h:100-107, 147-155, 198-208
+        if (e.shape == 0u) {
+          std::uniform_real_distribution<float> ux(-e.size[0] * 0.5f, e.size[0] * 0.5f);
+          std::uniform_real_distribution<float> uy(-e.size[1] * 0.5f, e.size[1] * 0.5f);
+          std::uniform_real_distribution<float> uz(-e.size[2] * 0.5f, e.size[2] * 0.5f);
+          const float a = ux(rng);
+          const float b = uy(rng);
+          const float c = uz(rng);
+          px[j] = e.center[0] + a; py[j] = e.center[1] + b; pz[j] = e.center[2] + c;
+        } else if (e.shape == 1u) {
+          std::uniform_real_distribution<float> u01(0.0f, 1.0f), uth(0.0f, 3.14159265f), uph(0.0f, 6.28318530f);
+          const float r = e.size[0] * std::cbrt(u01(rng));
+          const float theta = uth(rng);
+          const float phi = uph(rng);
+          px[j] = e.center[0] + r * std::sin(theta) * std::cos(phi);
+          py[j] = e.center[1] + r * std::sin(theta) * std::sin(phi);
+          pz[j] = e.center[2] + r * std::cos(theta);
+        } else {
+          const float A = e.size[0], B = e.size[1];
+          std::uniform_real_distribution<float> ux(-A, A), uy(-B, B);
+          float a, b;
+          do {
+            a = ux(rng);
+            b = uy(rng);
+          } while ((a * a) / (A * A) + (b * b) / (B * B) > 1.0f);
+          px[j] = e.center[0] + a; py[j] = e.center[1] + b; pz[j] = e.center[2];
+        }
+      }
+      for (int j = 0; j < n; j++) {  // Emitter::sample_velocity, Emitter.h:66-74
+        float v[3] = {e.vel[0], e.vel[1], e.vel[2]};
+        if (e.emit_along_normal) {
+          const float d[3] = {px[j] - e.center[0], py[j] - e.center[1], pz[j] - e.center[2]};
+          float nrm[3] = {0.0f, 0.0f, 0.0f};
+          if (e.shape == 0u) {  // AABBEmitter::surface_normal, Emitter.h:109-117
+            float q[3], aq[3];
+            for (int k = 0; k < 3; k++) {
+              q[k] = d[k] / std::max(e.size[k] * 0.5f, 1e-6f);
+              aq[k] = std::fabs(q[k]);
+            }
+            if (aq[0] >= aq[1] && aq[0] >= aq[2]) nrm[0] = q[0] >= 0.0f ? 1.0f : -1.0f;
+            else if (aq[1] >= aq[2]) nrm[1] = q[1] >= 0.0f ? 1.0f : -1.0f;
+            else nrm[2] = q[2] >= 0.0f ? 1.0f : -1.0f;
+          } else {  // Emitter::surface_normal, Emitter.h:59-63
+            const float len = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+            if (len > 1e-6f) { nrm[0] = d[0] / len; nrm[1] = d[1] / len; nrm[2] = d[2] / len; }
+            else nrm[2] = 1.0f;
+          }
+          const float sp = std::sqrt(e.vel[0] * e.vel[0] + e.vel[1] * e.vel[1] + e.vel[2] * e.vel[2]);
+          for (int k = 0; k < 3; k++) v[k] = nrm[k] * sp;
+        }
+        if (e.velocity_randomness > 0.0f) {
+          std::uniform_real_distribution<float> ur(-e.velocity_randomness, e.velocity_randomness);
+          const float a = ur(rng);
+          const float b = ur(rng);
+          const float c = ur(rng);
+          v[0] += a; v[1] += b; v[2] += c;
+        }
+        pos4[4 * out + 0] = px[j]; pos4[4 * out + 1] = py[j]; pos4[4 * out + 2] = pz[j]; pos4[4 * out + 3] = Vp;
+        vel4[4 * out + 0] = v[0]; vel4[4 * out + 1] = v[1]; vel4[4 * out + 2] = v[2]; vel4[4 * out + 3] = idf;
+        out++;
+      }
+      live += uint32_t(n);
+      done[si]++;
+      for (int k = 0; k < 3; k++) e.center[k] += e.center_vel[k] * dt;
+    }
+  }
+  return out;
 }
 
 }  // extern "C"
