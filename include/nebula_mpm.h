@@ -265,6 +265,21 @@ int nb_mpm_clear_analytic_colliders(nb_mpm* h);
 /* MPMEngine::setColliderSDF (src/engine/MPMEngine.cpp:204-209): a dense float
  * SDF indexed by Morton cell id; n must equal the total cell count. */
 int nb_mpm_set_collider_sdf(nb_mpm* h, const float* morton_sdf, uint64_t n);
+/* buildMeshSDF (src/core/MeshSDF.h:85-126) computed on the GPU and installed
+ * as the collider SDF, as the reference's examples do with setColliderSDF
+ * (examples/pyro_basic.cpp:123-134): tris holds n_tris triangles of 12 floats
+ * (normal, v0, v1, v2; loadBinarySTL's MeshTriangle order), in the engine's
+ * world coordinates.  Cells are signed by the normal of their closest
+ * triangle; cube cells beyond the grid hold 1e9.
+ * exact = 0 reproduces closestPtOnTriangle (MeshSDF.h:48-74), which has no
+ * Voronoi region for the edge v1-v2: a cell nearest that edge gets its
+ * distance to the triangle's plane, and near a mesh's convex edges that can
+ * be too small and of the wrong sign.  exact = 1 adds the region (Ericson,
+ * Real-Time Collision Detection, 5.1.5) and gives the true distance.  No
+ * counterpart in the reference. */
+int nb_mpm_set_collider_mesh(nb_mpm* h, const float* tris, uint32_t n_tris, int exact);
+/* The collider SDF in force (totalCells floats, Morton order). */
+int nb_mpm_get_collider_sdf(nb_mpm* h, float* out);
 /* MPMEngine::setColliderSphere / clearCollider (src/engine/MPMEngine.cpp:180-202). */
 int nb_mpm_set_collider_sphere(nb_mpm* h, float radius, float cx, float cy, float cz);
 int nb_mpm_clear_collider(nb_mpm* h);

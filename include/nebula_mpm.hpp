@@ -363,6 +363,18 @@ class MPMEngine {
   void setColliderSDF(const std::vector<float>& mortonSDF) {
     check(nb_mpm_set_collider_sdf(h_, mortonSDF.data(), mortonSDF.size()));
   }
+  // buildMeshSDF (src/core/MeshSDF.h:85-126) + setColliderSDF in one call, on
+  // the GPU: tris holds 12 floats per triangle (normal, v0, v1, v2).
+  void setColliderMesh(const std::vector<float>& tris, bool exact = false) {
+    check(nb_mpm_set_collider_mesh(h_, tris.data(), uint32_t(tris.size() / 12), exact ? 1 : 0));
+  }
+  std::vector<float> getColliderSDF() {
+    uint32_t cells = 0;
+    check(nb_mpm_total_cells(h_, &cells));
+    std::vector<float> out(cells);
+    check(nb_mpm_get_collider_sdf(h_, out.data()));
+    return out;
+  }
 
   // EngineBase force list (src/engine/EngineBase.cpp:22-40)
   void addForce(std::shared_ptr<Force> f) {

@@ -660,7 +660,6 @@ int nb_mpm_destroy(nb_mpm* h) {
   if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
   for (void* p : h->allocs) (void)hipFree(p);
   for (void* p : h->adj.allocs) (void)hipFree(p);
-  if (h->d.sdf) (void)hipFree(h->d.sdf);
   if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return NB_OK;
@@ -908,8 +907,7 @@ int nb_mpm_clear_analytic_colliders(nb_mpm* h) {
 
 static int ensure_sdf(nb_mpm* h) {
   if (h->d.sdf) return NB_OK;
-  NB_HIP(hipMalloc(&h->d.sdf, size_t(h->NC) * 4));
-  return NB_OK;
+  return h->alloc(&h->d.sdf, h->NC);
 }
 
 int nb_mpm_set_collider_sdf(nb_mpm* h, const float* sdf, uint64_t n) {
@@ -934,6 +932,34 @@ int nb_mpm_set_collider_sphere(nb_mpm* h, float r, float cx, float cy, float cz)
   NB_HIP(hipStreamSynchronize(h->stream));
   h->has_sdf = true;
   return NB_OK;
+}
+
+int nb_mpm_set_collider_mesh(nb_mpm* h, const float* tris, uint32_t n_tris, int exact) {
+  if (!h) return fail(NB_ERR_ARG, "null handle");
+  if (h->batch()) return fail(NB_ERR_UNSUPPORTED, "%s: not available on a batch of scenes", __func__);
+  if (n_tris && !tris) return fail(NB_ERR_ARG, "null triangles");
+  int rc = ensure_sdf(h);
+  if (rc != NB_OK) return rc;
+  float* d_tris = nullptr;
+  const size_t bytes = std::max<size_t>(size_t(n_tris) * 12 * 4, 4);
+  NB_HIP(hipMalloc(&d_tris, bytes));
+  rc = n_tris ? upload(h, d_tris, tris, size_t(n_tris) * 12 * 4) : NB_OK;
+  if (rc == NB_OK) {
+    const int32_t res[3] = {int32_t(h->grid_res[0]), int32_t(h->grid_res[1]), int32_t(h->grid_res[2])};
+    launch_mesh_sdf(d_tris, n_tris, res, h->cube_res, h->cfg.cell_size, exact != 0, h->d.sdf, h->stream);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) rc = fail(NB_ERR_HIP, "mesh SDF build failed: %s", hipGetErrorString(e));
+  }
+  (void)hipFree(d_tris);
+  if (rc == NB_OK) h->has_sdf = true;
+  return rc;
+}
+
+int nb_mpm_get_collider_sdf(nb_mpm* h, float* out) {
+  if (!h || !out) return fail(NB_ERR_ARG, "null argument");
+  if (!h->has_sdf) return fail(NB_ERR_STATE, "no collider SDF is set");
+  return download(h, out, h->d.sdf, size_t(h->NC) * 4);
 }
 
 int nb_mpm_clear_collider(nb_mpm* h) {

@@ -1242,6 +1242,100 @@ void launch_g2p(const DevState& d, const SubParams& p, bool use_sorted, hipStrea
 #undef NB_G2P_LAUNCH
 }
 
+// ---------------------------------------------------------------------------
+// mesh SDF (src/core/MeshSDF.h:85-126): per cell of the grid, the distance to
+// the closest point of any triangle (Ericson's closest-point regions,
+// MeshSDF.h:42-75), signed by that triangle's normal (the first triangle at
+// the minimum distance); cube cells outside the grid hold 1e9.  One thread
+// per cell, triangles staged through LDS; every product and sum is rounded
+// as written (no FMA contraction), as on the host.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kMeshChunk = 256;
+template <bool EXACT>
+__global__ __launch_bounds__(256) void k_mesh_sdf(const float* __restrict__ tris, uint32_t n_tris, int rx, int ry,
+                                                  int rz, float cs, float* __restrict__ sdf) {
+#pragma clang fp contract(off)
+  __shared__ float s_t[kMeshChunk * 12];
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = c < uint32_t(rx) * uint32_t(ry) * uint32_t(rz);
+  const int ix = int(c % uint32_t(rx)), iy = int((c / uint32_t(rx)) % uint32_t(ry)), iz = int(c / (uint32_t(rx) * uint32_t(ry)));
+  const float px = (float(ix) + 0.5f) * cs, py = (float(iy) + 0.5f) * cs, pz = (float(iz) + 0.5f) * cs;
+  float best = 1e9f;
+  bool outer = true;
+  auto dot3 = [](float ax, float ay, float az, float bx, float by, float bz) { return (ax * bx + ay * by) + az * bz; };
+  for (uint32_t t0 = 0; t0 < n_tris; t0 += kMeshChunk) {
+    const uint32_t nt = min(kMeshChunk, n_tris - t0);
+    __syncthreads();
+    for (uint32_t e = threadIdx.x; e < nt * 12u; e += blockDim.x) s_t[e] = tris[size_t(t0) * 12 + e];
+    __syncthreads();
+    if (!valid) continue;
+    for (uint32_t j = 0; j < nt; j++) {
+      const float* t = s_t + 12 * j;  // normal, v0, v1, v2
+      const float ax = t[3], ay = t[4], az = t[5];
+      const float abx = t[6] - ax, aby = t[7] - ay, abz = t[8] - az;
+      const float acx = t[9] - ax, acy = t[10] - ay, acz = t[11] - az;
+      const float apx = px - ax, apy = py - ay, apz = pz - az;
+      float qx, qy, qz;  // closest point
+      const float d1 = dot3(abx, aby, abz, apx, apy, apz), d2 = dot3(acx, acy, acz, apx, apy, apz);
+      if (d1 <= 0.f && d2 <= 0.f) {
+        qx = ax; qy = ay; qz = az;
+      } else {
+        const float bpx = px - t[6], bpy = py - t[7], bpz = pz - t[8];
+        const float d3 = dot3(abx, aby, abz, bpx, bpy, bpz), d4 = dot3(acx, acy, acz, bpx, bpy, bpz);
+        const float vc = d1 * d4 - d3 * d2;
+        if (d3 >= 0.f && d4 <= d3) {
+          qx = t[6]; qy = t[7]; qz = t[8];
+        } else if (vc <= 0.f && d1 >= 0.f && d3 <= 0.f) {
+          const float v = d1 / (d1 - d3);
+          qx = ax + v * abx; qy = ay + v * aby; qz = az + v * abz;
+        } else {
+          const float cpx = px - t[9], cpy = py - t[10], cpz = pz - t[11];
+          const float d5 = dot3(abx, aby, abz, cpx, cpy, cpz), d6 = dot3(acx, acy, acz, cpx, cpy, cpz);
+          const float vb = d5 * d2 - d1 * d6;
+          if (d6 >= 0.f && d5 <= d6) {
+            qx = t[9]; qy = t[10]; qz = t[11];
+          } else if (vb <= 0.f && d2 >= 0.f && d6 <= 0.f) {
+            const float w = d2 / (d2 - d6);
+            qx = ax + w * acx; qy = ay + w * acy; qz = az + w * acz;
+          } else {
+            const float va = d3 * d6 - d5 * d4;
+            if (EXACT && va <= 0.f && d4 - d3 >= 0.f && d5 - d6 >= 0.f) {  // edge v1-v2
+              const float w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+              qx = t[6] + w * (t[9] - t[6]); qy = t[7] + w * (t[10] - t[7]); qz = t[8] + w * (t[11] - t[8]);
+            } else {
+              const float sum = va + vb + vc;
+              const float v = vb / sum, w = vc / sum;
+              qx = ax + v * abx + w * acx; qy = ay + v * aby + w * acy; qz = az + v * abz + w * acz;
+            }
+          }
+        }
+      }
+      const float dx = px - qx, dy = py - qy, dz = pz - qz;
+      const float d = sqrtf(dot3(dx, dy, dz, dx, dy, dz));
+      if (d < best) {
+        best = d;
+        outer = dot3(dx, dy, dz, t[0], t[1], t[2]) >= 0.f;
+      }
+    }
+  }
+  if (valid) sdf[morton3(uint32_t(ix), uint32_t(iy), uint32_t(iz))] = outer ? best : -best;
+}
+__global__ void k_fill(float* __restrict__ a, uint32_t n, float v) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+void launch_mesh_sdf(const float* tris, uint32_t n_tris, const int32_t res[3], uint32_t cube, float cs, bool exact,
+                     float* sdf, hipStream_t s) {
+  const uint32_t nc = cube * cube * cube, cells = uint32_t(res[0]) * uint32_t(res[1]) * uint32_t(res[2]);
+  k_fill<<<cdiv(nc, 256), 256, 0, s>>>(sdf, nc, 1e9f);
+  if (!cells) return;
+  if (exact)
+    k_mesh_sdf<true><<<cdiv(cells, 256), 256, 0, s>>>(tris, n_tris, res[0], res[1], res[2], cs, sdf);
+  else
+    k_mesh_sdf<false><<<cdiv(cells, 256), 256, 0, s>>>(tris, n_tris, res[0], res[1], res[2], cs, sdf);
+}
+
 void launch_sdf_sphere(float* sdf, const SubParams& p, float r, float cx, float cy, float cz, hipStream_t s) {
   const uint32_t nc = p.cube_res * p.cube_res * p.cube_res;
   k_sdf_sphere<<<cdiv(nc, 256), 256, 0, s>>>(sdf, p, r, cx, cy, cz);

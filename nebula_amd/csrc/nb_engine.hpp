@@ -158,6 +158,9 @@ void launch_grid(const DevState& d, const SubParams& p, uint32_t n_tiles, bool f
 void launch_g2p(const DevState& d, const SubParams& p, bool use_sorted, hipStream_t s, Prof* pf,
                 float4* rec_out = nullptr);
 void launch_sdf_sphere(float* sdf, const SubParams& p, float r, float cx, float cy, float cz, hipStream_t s);
+// the Morton SDF of a triangle mesh (n x 12 floats: normal, v0, v1, v2) on a grid of res cells of size cs
+void launch_mesh_sdf(const float* tris, uint32_t n_tris, const int32_t res[3], uint32_t cube, float cs, bool exact,
+                     float* sdf, hipStream_t s);
 void launch_fill_identity(uint32_t* a, uint32_t n, hipStream_t s);
 void launch_init_particles(const DevState& d, uint32_t first, uint32_t count, hipStream_t s);
 void launch_copy(float4* dst, const float4* src, uint64_t n4, hipStream_t s);

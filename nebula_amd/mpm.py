@@ -418,6 +418,7 @@ EXPORTED_SYMBOLS = (
     "nb_mpm_create_batch", "nb_mpm_scene_count", "nb_mpm_scene_info",
     "nb_mpm_get_tuning", "nb_mpm_set_tuning", "nb_mpm_autotune", "nb_mpm_graph_substeps",
     "nb_mpm_add_emitter", "nb_mpm_clear_emitters", "nb_mpm_emit", "nb_mpm_get_emitter",
+    "nb_mpm_set_collider_mesh", "nb_mpm_get_collider_sdf",
 )
 
 
@@ -562,6 +563,8 @@ def lib():
         L.nb_mpm_autotune.argtypes = [vp, C.c_float, C.c_uint32, C.c_uint32, C.POINTER(Tuning)]
         L.nb_mpm_graph_substeps.argtypes = [vp, C.c_float, C.c_uint32]
         L.nb_mpm_add_emitter.argtypes = [vp, C.POINTER(EmitterGPU)]
+        L.nb_mpm_set_collider_mesh.argtypes = [vp, fp, C.c_uint32, C.c_int]
+        L.nb_mpm_get_collider_sdf.argtypes = [vp, fp]
         L.nb_mpm_clear_emitters.argtypes = [vp]
         L.nb_mpm_emit.argtypes = [vp, C.c_float]
         L.nb_mpm_get_emitter.argtypes = [vp, C.c_uint32, C.POINTER(EmitterGPU), up]
@@ -816,6 +819,19 @@ class MPMEngine:
 
     def setColliderSphere(self, radius, cx, cy, cz):
         _check(lib().nb_mpm_set_collider_sphere(self._h, radius, cx, cy, cz))
+
+    def setColliderMesh(self, tris, exact=False):
+        """buildMeshSDF on the GPU, installed as the collider SDF
+        (nb_mpm_set_collider_mesh); tris (n, 12): normal, v0, v1, v2.
+        exact=False keeps closestPtOnTriangle's missing edge-v1-v2 region
+        (MeshSDF.h:48-74); exact=True measures the true distance."""
+        t = np.ascontiguousarray(tris, np.float32).reshape(-1, 12)
+        _check(lib().nb_mpm_set_collider_mesh(self._h, _f(t), t.shape[0], int(bool(exact))))
+
+    def getColliderSDF(self):
+        out = np.empty(self._n_cells, np.float32)
+        _check(lib().nb_mpm_get_collider_sdf(self._h, _f(out)))
+        return out
 
     def clearCollider(self):
         _check(lib().nb_mpm_clear_collider(self._h))

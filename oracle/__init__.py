@@ -92,6 +92,7 @@ def lib():
                       ("orc_bspline2_f64", C.c_double), ("orc_bspline2g_f64", C.c_double)):
             getattr(L, nm).restype = t
             getattr(L, nm).argtypes = [t]
+        L.orc_mesh_sdf.argtypes = [fp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_int, fp]
         L.orc_emit.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_float, C.c_uint32, C.c_uint32, C.c_float, fp, fp]
         L.orc_emit.restype = C.c_uint32
         L.orc_noise_value.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int]
@@ -375,3 +376,14 @@ def emit(records: bytes, n_emitters: int, n_calls: int, dt: float, capacity: int
     buf = C.create_string_buffer(bytes(records), max(len(records), 1))
     n = lib().orc_emit(buf, n_emitters, n_calls, dt, capacity, live, Vp, _fptr(pos), _fptr(vel))
     return pos[:n], vel[:n]
+
+
+def mesh_sdf(tris, grid_res, cell_size, exact=False):
+    """buildMeshSDF: the Morton SDF (cube^3 floats) of (n, 12) triangles;
+    exact adds the closest-point region of edge v1-v2 the reference lacks."""
+    t = np.ascontiguousarray(tris, np.float32).reshape(-1, 12)
+    cube = cube_res_of(grid_res)
+    out = np.empty(cube ** 3, np.float32)
+    lib().orc_mesh_sdf(_fptr(t), t.shape[0], int(grid_res[0]), int(grid_res[1]), int(grid_res[2]), cube,
+                       float(cell_size), int(bool(exact)), _fptr(out))
+    return out

@@ -422,4 +422,76 @@ uint32_t orc_emit(const void* records, uint32_t n_emitters, uint32_t n_calls, fl
   return out;
 }
 
+
+// buildMeshSDF (src/core/MeshSDF.h:85-126) with closestPtOnTriangle
+// (MeshSDF.h:42-75): cells (i + 0.5) cs of the res grid, Morton order, cube
+// cells beyond the grid 1e9; sign from the normal of the first closest
+// triangle.  tris: n x 12 floats (normal, v0, v1, v2).  exact = 0 keeps
+// the reference's missing region for edge b-c (a cell there gets the plane
+// projection); exact = 1 adds it (Ericson, Real-Time Collision Detection,
+// 5.1.5).
+static void orc_closest(const float* p, const float* a, const float* b, const float* c, bool exact, float* q) {
+  auto sub = [](const float* x, const float* y, float* o) { o[0] = x[0] - y[0]; o[1] = x[1] - y[1]; o[2] = x[2] - y[2]; };
+  auto dot = [](const float* x, const float* y) { return (x[0] * y[0] + x[1] * y[1]) + x[2] * y[2]; };
+  float ab[3], ac[3], ap[3];
+  sub(b, a, ab); sub(c, a, ac); sub(p, a, ap);
+  const float d1 = dot(ab, ap), d2 = dot(ac, ap);
+  if (d1 <= 0.0f && d2 <= 0.0f) { q[0] = a[0]; q[1] = a[1]; q[2] = a[2]; return; }
+  float bp[3];
+  sub(p, b, bp);
+  const float d3 = dot(ab, bp), d4 = dot(ac, bp);
+  if (d3 >= 0.0f && d4 <= d3) { q[0] = b[0]; q[1] = b[1]; q[2] = b[2]; return; }
+  const float vc = d1 * d4 - d3 * d2;
+  if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) {
+    const float v = d1 / (d1 - d3);
+    for (int k = 0; k < 3; k++) q[k] = a[k] + v * ab[k];
+    return;
+  }
+  float cp[3];
+  sub(p, c, cp);
+  const float d5 = dot(ab, cp), d6 = dot(ac, cp);
+  if (d6 >= 0.0f && d5 <= d6) { q[0] = c[0]; q[1] = c[1]; q[2] = c[2]; return; }
+  const float vb = d5 * d2 - d1 * d6;
+  if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) {
+    const float w = d2 / (d2 - d6);
+    for (int k = 0; k < 3; k++) q[k] = a[k] + w * ac[k];
+    return;
+  }
+  const float va = d3 * d6 - d5 * d4;
+  if (exact && va <= 0.0f && d4 - d3 >= 0.0f && d5 - d6 >= 0.0f) {  // edge b-c, absent from the reference
+    const float w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+    for (int k = 0; k < 3; k++) q[k] = b[k] + w * (c[k] - b[k]);
+    return;
+  }
+  const float sum = va + vb + vc;
+  const float v = vb / sum, w = vc / sum;
+  for (int k = 0; k < 3; k++) q[k] = a[k] + v * ab[k] + w * ac[k];
+}
+
+void orc_mesh_sdf(const float* tris, uint32_t n, uint32_t rx, uint32_t ry, uint32_t rz, uint32_t cube, float cs,
+                  int exact, float* sdf) {
+  const size_t nc = size_t(cube) * cube * cube;
+  for (size_t i = 0; i < nc; i++) sdf[i] = 1e9f;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int64_t iz = 0; iz < int64_t(rz); iz++)
+    for (uint32_t iy = 0; iy < ry; iy++)
+      for (uint32_t ix = 0; ix < rx; ix++) {
+        const float p[3] = {(float(ix) + 0.5f) * cs, (float(iy) + 0.5f) * cs, (float(iz) + 0.5f) * cs};
+        float best = 1e9f;
+        bool outer = true;
+        for (uint32_t t = 0; t < n; t++) {
+          const float* tr = tris + 12 * size_t(t);
+          float q[3];
+          orc_closest(p, tr + 3, tr + 6, tr + 9, exact != 0, q);
+          const float d3[3] = {p[0] - q[0], p[1] - q[1], p[2] - q[2]};
+          const float d = std::sqrt((d3[0] * d3[0] + d3[1] * d3[1]) + d3[2] * d3[2]);
+          if (d < best) {
+            best = d;
+            outer = (d3[0] * tr[0] + d3[1] * tr[1]) + d3[2] * tr[2] >= 0.0f;
+          }
+        }
+        sdf[oracle::morton_encode(int(ix), int(iy), int(iz))] = outer ? best : -best;
+      }
+}
+
 }  // extern "C"
