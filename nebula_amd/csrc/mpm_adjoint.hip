@@ -648,7 +648,7 @@ __global__ __launch_bounds__(kAdjThreads) void k_adj_scatter(SubParams p, const 
 // ---------------------------------------------------------------------------
 // grid update backward
 // ---------------------------------------------------------------------------
-template <bool BATCH>
+template <bool BATCH, bool NOISE>
 __global__ __launch_bounds__(512) void k_grid_bwd(SubParams p, const float4* __restrict__ grid_pre,
                                                   const float4* __restrict__ partial_adj,
                                                   const int32_t* __restrict__ tile_act,
@@ -706,8 +706,8 @@ __global__ __launch_bounds__(512) void k_grid_bwd(SubParams p, const float4* __r
     if (capped) { const float kk = 15.f / sp; vx *= kk; vy *= kk; vz *= kk; }
     // forces add constants with respect to the state: they pass the adjoint
     // through, but the flags below depend on the velocity they produce
-    apply_forces(vx, vy, vz, (float(nx) + 0.5f) * sg.h + sg.wmin[0], (float(ny) + 0.5f) * sg.h + sg.wmin[1],
-                 (float(nz) + 0.5f) * sg.h + sg.wmin[2], forces, p.force_count, p.dt);
+    apply_forces<NOISE>(vx, vy, vz, (float(nx) + 0.5f) * sg.h + sg.wmin[0], (float(ny) + 0.5f) * sg.h + sg.wmin[1],
+                        (float(nz) + 0.5f) * sg.h + sg.wmin[2], forces, p.force_count, p.dt);
     const int Nx = sg.grid_res[0] - 1, Ny = sg.grid_res[1] - 1, Nz = sg.grid_res[2] - 1;
     bool wx = false, wy = false, wz = false;
     if (nx <= 1 && vx < 0.f) { vx = 0.f; wx = true; }
@@ -922,8 +922,12 @@ static void adjoint_kernels(const DevState& d, const AdjState& a, const SubParam
                                          d.materials, d.sorted, d.grid_node, d.sdf);
   k_adj_scatter<BATCH><<<tile_grid_a(n_tiles, p.tile_blocks), kAdjThreads, 0, s>>>(p, d.rec, a.coef, d.cell_count, d.cell_end,
                                                                     d.sorted, d.act_tiles, d.counters, a.partial_adj);
-  k_grid_bwd<BATCH><<<tile_grid_a(n_tiles, p.tile_blocks), 512, 0, s>>>(p, d.grid_pre, a.partial_adj, d.tile_act, d.node_tiles,
-                                                          d.counters + CNT_NODE_TILES, a.grid_adj, d.sdf, d.forces);
+  if (p.noise_forces)
+    k_grid_bwd<BATCH, true><<<tile_grid_a(n_tiles, p.tile_blocks), 512, 0, s>>>(
+        p, d.grid_pre, a.partial_adj, d.tile_act, d.node_tiles, d.counters + CNT_NODE_TILES, a.grid_adj, d.sdf, d.forces);
+  else
+    k_grid_bwd<BATCH, false><<<tile_grid_a(n_tiles, p.tile_blocks), 512, 0, s>>>(
+        p, d.grid_pre, a.partial_adj, d.tile_act, d.node_tiles, d.counters + CNT_NODE_TILES, a.grid_adj, d.sdf, d.forces);
   k_p2g_bwd<BATCH><<<nblk, 256, 0, s>>>(p, d.rec, adj_out, d.materials, d.sorted, a.grid_adj);
 }
 

@@ -57,7 +57,7 @@ struct nb_mpm {
   hipStream_t stream = nullptr;
   bool own_stream = false;
   DevState d{};
-  uint32_t material_count = 1, collider_count = 0, force_count = 0;
+  uint32_t material_count = 1, collider_count = 0, force_count = 0, noise_force_count = 0;
   std::vector<nb_material> mats_host;  // host copy of the material table
   bool has_sdf = false;
   bool bins_valid = false;
@@ -155,6 +155,7 @@ struct nb_mpm {
     p.material_count = material_count;
     p.collider_count = collider_count;
     p.force_count = force_count;
+    p.noise_forces = noise_force_count;
     p.write_old = write_old ? 1u : 0u;
     p.corrected_svd = knobs.corrected_svd ? 1u : 0u;
     p.tile_blocks = tuning.tile_blocks;
@@ -972,12 +973,17 @@ int nb_mpm_set_forces(nb_mpm* h, const nb_force* forces, uint32_t n) {
   if (!h) return fail(NB_ERR_ARG, "null handle");
   if (n > NB_MAX_FORCES) return fail(NB_ERR_RANGE, "%u forces exceed the %u slots", n, NB_MAX_FORCES);
   if (n && !forces) return fail(NB_ERR_ARG, "null forces");
+  uint32_t noise = 0;
   for (uint32_t i = 0; i < n; i++) {
     uint32_t t = forces[i].type;
     if (t > NB_FORCE_Z_CLAMP) return fail(NB_ERR_ARG, "unknown force type %u", t);
+    if (t == NB_FORCE_TURBULENCE || t == NB_FORCE_NOISE) noise++;
   }
   int rc = upload(h, h->d.forces, forces, sizeof(nb_force) * n);
-  if (rc == NB_OK) h->force_count = n;
+  if (rc == NB_OK) {
+    h->force_count = n;
+    h->noise_force_count = noise;
+  }
   return rc;
 }
 

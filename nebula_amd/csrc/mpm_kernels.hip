@@ -614,7 +614,7 @@ __global__ __launch_bounds__(kP2GThreads, 2) void k_p2g(SubParams p, const float
 // ---------------------------------------------------------------------------
 // GRID UPDATE
 // ---------------------------------------------------------------------------
-template <bool FROM_PARTIALS, bool UPDATE, bool BATCH>
+template <bool FROM_PARTIALS, bool UPDATE, bool BATCH, bool NOISE>
 __global__ __launch_bounds__(512) void k_grid(SubParams p, const float4* __restrict__ partial,
                                               const int32_t* __restrict__ tile_act, const uint32_t* __restrict__ tiles,
                                               const uint32_t* __restrict__ count, float4* __restrict__ grid_node,
@@ -686,7 +686,7 @@ __global__ __launch_bounds__(512) void k_grid(SubParams p, const float4* __restr
     const float px = (float(nx) + 0.5f) * g.h + g.wmin[0];
     const float py = (float(ny) + 0.5f) * g.h + g.wmin[1];
     const float pz = (float(nz) + 0.5f) * g.h + g.wmin[2];
-    apply_forces(vx, vy, vz, px, py, pz, forces, p.force_count, p.dt);
+    apply_forces<NOISE>(vx, vy, vz, px, py, pz, forces, p.force_count, p.dt);
     // separating walls on the two outer node layers (shaders/mpm_grid_update.comp:54-60)
     const int Nx = g.grid_res[0] - 1, Ny = g.grid_res[1] - 1, Nz = g.grid_res[2] - 1;
     if (nx <= 1 && vx < 0.f) vx = 0.f;
@@ -1209,17 +1209,26 @@ void launch_grid(const DevState& d, const SubParams& p, uint32_t n_tiles, bool f
                  const uint32_t* tiles, const uint32_t* count, hipStream_t s, Prof* pf) {
   ProfScope ps(pf, 9, s);
   const uint32_t g = tile_grid(n_tiles, p.tile_blocks);
-#define NB_GRID_LAUNCH(FP, UP, BT)                                                                                  \
-  k_grid<FP, UP, BT><<<g, 512, 0, s>>>(p, d.partial, d.tile_act, tiles, count, d.grid_node, d.grid_old, d.grid_pre, \
-                                       d.sdf, d.colliders, d.forces)
+#define NB_GRID_LAUNCH(FP, UP, BT, NZ)                                                                       \
+  k_grid<FP, UP, BT, NZ><<<g, 512, 0, s>>>(p, d.partial, d.tile_act, tiles, count, d.grid_node, d.grid_old, \
+                                           d.grid_pre, d.sdf, d.colliders, d.forces)
+  const bool nz = p.noise_forces != 0;
   if (p.n_scenes > 0) {
-    if (from_partials && update) NB_GRID_LAUNCH(true, true, true);
-    else if (from_partials) NB_GRID_LAUNCH(true, false, true);
-    else NB_GRID_LAUNCH(false, true, true);
+    if (from_partials && update) {
+      if (nz) NB_GRID_LAUNCH(true, true, true, true); else NB_GRID_LAUNCH(true, true, true, false);
+    } else if (from_partials) {
+      NB_GRID_LAUNCH(true, false, true, false);
+    } else {
+      if (nz) NB_GRID_LAUNCH(false, true, true, true); else NB_GRID_LAUNCH(false, true, true, false);
+    }
   } else {
-    if (from_partials && update) NB_GRID_LAUNCH(true, true, false);
-    else if (from_partials) NB_GRID_LAUNCH(true, false, false);
-    else NB_GRID_LAUNCH(false, true, false);
+    if (from_partials && update) {
+      if (nz) NB_GRID_LAUNCH(true, true, false, true); else NB_GRID_LAUNCH(true, true, false, false);
+    } else if (from_partials) {
+      NB_GRID_LAUNCH(true, false, false, false);
+    } else {
+      if (nz) NB_GRID_LAUNCH(false, true, false, true); else NB_GRID_LAUNCH(false, true, false, false);
+    }
   }
 #undef NB_GRID_LAUNCH
 }

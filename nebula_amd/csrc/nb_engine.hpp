@@ -82,6 +82,7 @@ struct SubParams {
   uint32_t force_count;
   uint32_t write_old;  // write the pre-force velocity (FLIP, or pass-level export)
   uint32_t corrected_svd;  // 1: singular vectors as columns (see nb_mpm_knobs)
+  uint32_t noise_forces;   // turbulence or noise records among the forces
   // batch (n_scenes > 0): the scene table, the scene of every tile and of
   // every particle; the single-scene fields above are then unused
   // launch shape (nb_mpm_tuning): blocks of the per-tile kernels, threads

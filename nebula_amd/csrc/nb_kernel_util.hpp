@@ -219,7 +219,11 @@ __device__ inline float3 noise_curl3(float px, float py, float pz, uint32_t oct)
 // grid, typeFlag 0; Force.h:136-256): gravity and constant wind add
 // dir strength dt, turbulence its curl noise, noise its fBm along dir; the
 // z clamp constrains positions only and does nothing here.  (x, y, z) is
-// the node's world position.
+// the node's world position.  NOISE = false compiles the turbulence and
+// noise branches out: the caller launches that form when the force list has
+// neither (the noise code's registers would otherwise cost every launch
+// occupancy).
+template <bool NOISE>
 __device__ inline void apply_forces(float& vx, float& vy, float& vz, float x, float y, float z,
                                     const nb_force* forces, uint32_t count, float dt) {
   for (uint32_t fi = 0; fi < count; fi++) {
@@ -231,14 +235,14 @@ __device__ inline void apply_forces(float& vx, float& vy, float& vz, float x, fl
       vx += (f.data[0] * s) * dt;
       vy += (f.data[1] * s) * dt;
       vz += (f.data[2] * s) * dt;
-    } else if (f.type == NB_FORCE_TURBULENCE) {  // data: strength, frequency, octaves bits, seed
+    } else if (NOISE && f.type == NB_FORCE_TURBULENCE) {  // data: strength, frequency, octaves bits, seed
 #pragma clang fp contract(off)
       const float fr = f.data[1], sd = f.data[3];
       const float3 c = noise_curl3(x * fr + sd, y * fr + sd, z * fr + sd, __float_as_uint(f.data[2]));
       vx += (c.x * f.data[0]) * dt;
       vy += (c.y * f.data[0]) * dt;
       vz += (c.z * f.data[0]) * dt;
-    } else if (f.type == NB_FORCE_NOISE) {  // data: dir xyz, strength, frequency, octaves bits, seed
+    } else if (NOISE && f.type == NB_FORCE_NOISE) {  // data: dir xyz, strength, frequency, octaves bits, seed
 #pragma clang fp contract(off)
       const float fr = f.data[4], sd = f.data[6];
       const float n = noise_fbm3(x * fr + sd, y * fr + sd, z * fr + sd, __float_as_uint(f.data[5]));
