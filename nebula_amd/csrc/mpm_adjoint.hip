@@ -582,16 +582,26 @@ __global__ __launch_bounds__(kAdjThreads) void k_adj_scatter(SubParams p, const 
       float ax[27], ay[27], az[27];
 #pragma unroll
       for (int o = 0; o < 27; o++) { ax[o] = 0.f; ay[o] = 0.f; az[o] = 0.f; }
-      for (uint32_t kk = kbeg; kk < kend; kk++) {
+      // the next particle's position and coefficients are loaded while this
+      // one is scattered
+      float4 npp, nco[kCoef];
+      auto load = [&](uint32_t kk) {
         const uint32_t q = sorted[kk];
-        const float4 pp = rec[size_t(q) * kRec + R_P];
-        float cf[kCoef * 4];
+        npp = rec[size_t(q) * kRec + R_P];
         const float4* co = coef + size_t(q) * kCoef;
 #pragma unroll
+        for (int jj = 0; jj < kCoef; jj++) nco[jj] = co[jj];
+      };
+      if (kbeg < kend) load(kbeg);
+      for (uint32_t kk = kbeg; kk < kend; kk++) {
+        const float4 pp = npp;
+        float cf[kCoef * 4];
+#pragma unroll
         for (int jj = 0; jj < kCoef; jj++) {
-          const float4 v4 = co[jj];
+          const float4 v4 = nco[jj];
           cf[4 * jj] = v4.x; cf[4 * jj + 1] = v4.y; cf[4 * jj + 2] = v4.z; cf[4 * jj + 3] = v4.w;
         }
+        if (kk + 1 < kend) load(kk + 1);
         AxisD X, Y, Z;
         axis_stencil_d(pp.x, sg.wmin[0], sg.h, cx - 1, X);
         axis_stencil_d(pp.y, sg.wmin[1], sg.h, cy - 1, Y);
