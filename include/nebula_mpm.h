@@ -409,9 +409,10 @@ int nb_mpm_active_tile_count(nb_mpm* h, uint32_t* tiles);
  * d_params.  The engine is left in the state the recorded run ended in (its
  * grid is not kept: nb_mpm_get_grid needs another substep first).
  * Memory: per particle of capacity (ceil(n/segment) + segment - 1) x 132 B
- * (records and bin keys of the kept states) + 480 B; per grid node
- * segment x 32 B (each substep's grid, kept so that the backward does not
- * redo P2G and the grid update) + 32 B. */
+ * (records and bin keys of the kept states) + segment x 8 B (each substep's
+ * sorted order) + 480 B; per grid node segment x 40 B (each substep's grid
+ * and cell counts, kept so that the backward does not bin again or redo P2G
+ * and the grid update) + 32 B. */
 int nb_mpm_adjoint_begin(nb_mpm* h, uint32_t n_substeps, uint32_t segment);
 int nb_mpm_forward_recorded(nb_mpm* h, float dt_sub);
 int nb_mpm_set_adjoint(nb_mpm* h, int field, const float* in4, uint32_t first, uint32_t count);

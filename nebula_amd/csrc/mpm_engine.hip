@@ -114,6 +114,10 @@ struct nb_mpm {
     // update at grid_tape[2j], (momentum / mass, mass) before it at [2j + 1]
     float4* grid_tape = nullptr;    // seg * 2 * NT * 512
     int tape_seg = -1;        // the segment whose records the tape holds
+    // the bins substep j of the tape's segment built (sorted order, cell
+    // counts and ends, tile lists): the backward reads them instead of
+    // binning again
+    std::vector<DevState> slot_bins;  // seg
     float4* adj_a = nullptr;  // cap * kRec
     float4* adj_b = nullptr;
     float4* field[NB_NUM_FIELDS] = {};  // adjoint in the reference layout
@@ -235,6 +239,23 @@ int sync_out(nb_mpm* h) {
 // and after, and the grids it keeps.  Null members mean the engine's own
 // buffers (records and keys updated in place); with g2p = false the substep
 // stops after the grid update.
+// the bin buffers of dv replaced by those of b
+void use_bins(DevState& dv, const DevState& b) {
+  dv.rank = b.rank;
+  dv.sorted = b.sorted;
+  dv.cell_count = b.cell_count;
+  dv.cell_end = b.cell_end;
+  dv.tile_flag = b.tile_flag;
+  dv.tile_mark = b.tile_mark;
+  dv.blk_cnt = b.blk_cnt;
+  dv.tile_act = b.tile_act;
+  dv.act_tiles = b.act_tiles;
+  dv.node_tiles = b.node_tiles;
+  dv.tile_total = b.tile_total;
+  dv.tile_start = b.tile_start;
+  dv.counters = b.counters;
+}
+
 struct TapeIO {
   float4* rec_in = nullptr;
   float4* rec_out = nullptr;
@@ -243,6 +264,7 @@ struct TapeIO {
   float4* grid_node = nullptr;
   float4* grid_pre = nullptr;
   bool g2p = true;
+  const DevState* bins = nullptr;  // bin buffers to use instead of the engine's
 };
 
 int enqueue_substep(nb_mpm* h, float dt, Prof* pf, const TapeIO& io = TapeIO{}) {
@@ -250,13 +272,14 @@ int enqueue_substep(nb_mpm* h, float dt, Prof* pf, const TapeIO& io = TapeIO{}) 
   const bool flip = h->knobs.flip_ratio > 0.01f;
   SubParams p = h->params(dt, flip);
   DevState dv = h->dev();
+  if (io.bins) use_bins(dv, *io.bins);
   if (io.rec_in) dv.rec = io.rec_in;
   if (io.keys_in) dv.keys = io.keys_in;
   if (io.grid_node) dv.grid_node = io.grid_node;
   dv.grid_pre = io.grid_pre;
   launch_bin(dv, p, h->NT, h->keys_valid, h->stream, pf);
   launch_p2g(dv, p, h->NT, h->stream, pf);
-  launch_grid(dv, p, h->NT, true, true, h->d.node_tiles, h->d.counters + CNT_NODE_TILES, h->stream, pf);
+  launch_grid(dv, p, h->NT, true, true, dv.node_tiles, dv.counters + CNT_NODE_TILES, h->stream, pf);
   if (!io.g2p) {
     NB_HIP(hipGetLastError());
     h->bins_valid = true;
@@ -1379,14 +1402,41 @@ FieldPtrs adj_fields(nb_mpm* h) {
   for (int i = 0; i < NB_NUM_FIELDS; i++) f.f[i] = h->adj.field[i];
   return f;
 }
-int adj_alloc(nb_mpm* h, float4** p, size_t count) {
+template <class T>
+int adj_alloc(nb_mpm* h, T** p, size_t count) {
   void* q = nullptr;
-  const size_t bytes = std::max<size_t>(count, 1) * sizeof(float4);
+  const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
   hipError_t e = hipMalloc(&q, bytes);
   if (e != hipSuccess) return fail(NB_ERR_HIP, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
   h->adj.allocs.push_back(q);
-  *p = reinterpret_cast<float4*>(q);
+  *p = reinterpret_cast<T*>(q);
   return NB_OK;
+}
+// one set of bin buffers, in the state a fresh engine's bins start in
+int adj_alloc_bins(nb_mpm* h, DevState& b) {
+  const size_t cells = size_t(h->NT) * kTileCells;
+  int rc = NB_OK;
+  if (rc == NB_OK) rc = adj_alloc(h, &b.rank, h->cap);
+  if (rc == NB_OK) rc = adj_alloc(h, &b.sorted, h->cap);
+  if (rc == NB_OK) rc = adj_alloc(h, &b.cell_count, cells);
+  if (rc == NB_OK) rc = adj_alloc(h, &b.cell_end, cells);
+  if (rc == NB_OK) rc = adj_alloc(h, &b.tile_flag, h->NT);
+  if (rc == NB_OK) rc = adj_alloc(h, &b.tile_mark, h->NT);
+  if (rc == NB_OK) rc = adj_alloc(h, &b.blk_cnt, (h->NT + 255) / 256);
+  if (rc == NB_OK) rc = adj_alloc(h, &b.tile_act, h->NT);
+  if (rc == NB_OK) rc = adj_alloc(h, &b.act_tiles, h->NT);
+  if (rc == NB_OK) rc = adj_alloc(h, &b.node_tiles, h->NT);
+  if (rc == NB_OK) rc = adj_alloc(h, &b.tile_total, h->NT);
+  if (rc == NB_OK) rc = adj_alloc(h, &b.tile_start, h->NT);
+  if (rc == NB_OK) rc = adj_alloc(h, &b.counters, CNT_SLOTS);
+  if (rc == NB_OK &&
+      (hipMemsetAsync(b.cell_count, 0, cells * 4, h->stream) != hipSuccess ||
+       hipMemsetAsync(b.cell_end, 0, cells * 4, h->stream) != hipSuccess ||
+       hipMemsetAsync(b.tile_flag, 0, h->NT, h->stream) != hipSuccess ||
+       hipMemsetAsync(b.counters, 0, CNT_SLOTS * 4, h->stream) != hipSuccess ||
+       hipMemsetAsync(b.tile_act, 0xFF, size_t(h->NT) * 4, h->stream) != hipSuccess))
+    rc = fail(NB_ERR_HIP, "hipMemsetAsync failed");
+  return rc;
 }
 void adj_free(nb_mpm* h) {
   if (h->stream) (void)hipStreamSynchronize(h->stream);
@@ -1427,6 +1477,7 @@ TapeIO tape_io(nb_mpm* h, uint32_t s, float4* rec_out, uint32_t* keys_out) {
   io.keys_out = keys_out;
   io.grid_node = grid_slot(h, s, 0);
   io.grid_pre = grid_slot(h, s, 1);
+  io.bins = &h->adj.slot_bins[s % h->adj.seg];
   return io;
 }
 }  // namespace
@@ -1460,6 +1511,8 @@ int nb_mpm_adjoint_begin(nb_mpm* h, uint32_t n_substeps, uint32_t segment) {
   if (rc == NB_OK) rc = adj_alloc(h, &kc, (size_t(h->cap) * A.n_ckpt + 3) / 4);
   if (rc == NB_OK) rc = adj_alloc(h, &kt, (size_t(h->cap) * (A.seg - 1) + 3) / 4);
   if (rc == NB_OK) rc = adj_alloc(h, &A.grid_tape, size_t(A.seg) * 2 * h->NT * kTileCells);
+  A.slot_bins.assign(A.seg, DevState{});
+  for (uint32_t j = 0; j < A.seg && rc == NB_OK; j++) rc = adj_alloc_bins(h, A.slot_bins[j]);
   if (rc == NB_OK) rc = adj_alloc(h, &A.adj_a, rec);
   if (rc == NB_OK) rc = adj_alloc(h, &A.adj_b, rec);
   for (int f = 0; f < NB_NUM_FIELDS && rc == NB_OK; f++) rc = adj_alloc(h, &A.field[f], h->cap);
@@ -1507,6 +1560,7 @@ int nb_mpm_forward_recorded(nb_mpm* h, float dt_sub) {
   for (uint32_t s = 0; s < A.n && rc == NB_OK; s++)
     rc = enqueue_substep(h, dt_sub, nullptr, tape_io(h, s, adj_next(h, s), key_next(h, s)));
   h->grid_mode = GridMode::NONE;  // the grids went to the tape
+  h->bins_valid = false;          // and the bins to the tape's bin buffers
   if (rc == NB_OK) rc = sync_out(h);
   if (rc != NB_OK) return rc;
   NB_HIP(hipStreamSynchronize(h->stream));
@@ -1568,11 +1622,11 @@ int nb_mpm_backward(nb_mpm* h, float* d_params, uint32_t params_cap) {
       const uint32_t s = s0 + uint32_t(j);
       SubParams p = h->params(A.dt, false);
       DevState dv = h->dev();
+      use_bins(dv, A.slot_bins[s % A.seg]);  // as the forward substep binned them
       dv.rec = adj_slot(h, s);
       dv.keys = key_slot(h, s);
       dv.grid_node = grid_slot(h, s, 0);
       dv.grid_pre = grid_slot(h, s, 1);
-      launch_bin(dv, p, h->NT, true, h->stream, nullptr);
       launch_adjoint_substep(dv, A.a, p, h->NT, in, out, h->stream);
       NB_HIP(hipGetLastError());
       std::swap(in, out);
