@@ -44,6 +44,13 @@ sys.path.insert(0, ROOT)
 
 METRIC = "particle-substeps/s at 4M DP particles (1 GPU); fwd+adjoint / fwd time"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the substep kernels on this workload, measured with
+# two rocprofv3 counter passes (FETCH_SIZE, then WRITE_SIZE) and scaled by what
+# the same counters read for a 256 MiB copy kernel in the same process (fetch
+# x 2.0, write x 1.0).  A counter pass cannot run inside the timed bench; the
+# commands and the per-kernel table are in the file named below.
+TRAFFIC_SOURCE = "profiles/hbm_traffic_snow_terrain_256.md"
+MEASURED_TRAFFIC = {"g2p": 1337.9e6, "p2g": 856.3e6, "grid_update": 116.9e6}
 
 
 def log(*a):
@@ -238,7 +245,8 @@ def main():
     achieved = dbytes / (avg_ms[dom] * 1e-3) / 1e9
     b_alg = 288 * n + 68 * g_act
     roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": MEASURED_TRAFFIC.get(dom),
+                "traffic_source": TRAFFIC_SOURCE if dom in MEASURED_TRAFFIC else None,
                 "bytes_per_launch": dbytes, "avg_launch_ms": round(avg_ms[dom], 5)}
     substep_rate = b_alg / (ms_per_step * 1e-3) / 1e9
 
