@@ -96,3 +96,29 @@ def test_create_without_device_fails_loudly(lib):
     assert b"device" in lib.nb_last_error()
     with pytest.raises(mpm.NebulaError):
         mpm.MPMEngine(c)
+
+
+def _c_layout(struct, fields):
+    """sizeof and offsetof of a header struct, from the C compiler itself."""
+    import tempfile
+    src = ['#include <stdio.h>', '#include <stddef.h>', '#include "nebula_mpm.h"', 'int main(void) {',
+           f'  printf("%zu", sizeof({struct}));']
+    src += [f'  printf(" %zu", offsetof({struct}, {f}));' for f in fields]
+    src += ['  printf("\\n");', '  return 0;', '}']
+    with tempfile.TemporaryDirectory() as d:
+        c, exe = os.path.join(d, "l.c"), os.path.join(d, "l")
+        with open(c, "w") as fh:
+            fh.write("\n".join(src))
+        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
+        out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()
+    return [int(v) for v in out]
+
+
+@pytest.mark.parametrize("struct,cls", [("nb_emitter", "EmitterGPU"), ("nb_mpm_scene", "BatchScene"),
+                                        ("nb_mpm_tuning", "Tuning")])
+def test_extension_records_match_the_header(struct, cls):
+    py = getattr(mpm, cls)
+    names = [f[0] for f in py._fields_]
+    got = _c_layout(struct, names)
+    assert got[0] == C.sizeof(py), f"{struct}: sizeof"
+    assert got[1:] == [getattr(py, n).offset for n in names], f"{struct}: offsets"
