@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "nebula_mpm.h"
+#include "nebula_emitter.hpp"
 
 namespace nebula {
 
@@ -201,6 +202,20 @@ struct Emitter {
   float velocityRandomness = 0.0f;
   int step_count = -1;
   virtual nb_emitter pack() const = 0;
+  // Emitter::sample / sample_velocity (src/core/Emitter.h:66-74, 100-208):
+  // the draws the engine makes for each emitted particle
+  vec4 sample(std::mt19937& rng) const {
+    const nb_emitter e = pack();
+    float x, y, z;
+    emitter_sample(e, rng, x, y, z);
+    return {x, y, z, 0.f};
+  }
+  vec4 sample_velocity(const vec4& p, std::mt19937& rng) const {
+    const nb_emitter e = pack();
+    float vx, vy, vz;
+    emitter_velocity(e, p[0], p[1], p[2], rng, vx, vy, vz);
+    return {vx, vy, vz, 0.f};
+  }
 
  protected:
   nb_emitter packCommon(uint32_t shape) const {

@@ -32,3 +32,17 @@ def test_facade_runs_reference_style_tests(gpu_lib):
     r = subprocess.run([OUT], capture_output=True, text=True, timeout=120)
     print(r.stdout, r.stderr)
     assert r.returncode == 0
+
+
+def test_facade_emitters_run_reference_emitter_tests():
+    """The reference's emitter tests (tests/test_mpm.cpp:175-226) through the
+    facade's Emitter::sample, which draws exactly what the engine draws; a
+    header-only program, no GPU."""
+    src = os.path.join(ROOT, "tests", "cpp", "emitter_cpp_test.cpp")
+    out = os.path.join(ROOT, "_build", "emitter_cpp_test")
+    os.makedirs(os.path.dirname(out), exist_ok=True)
+    subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-I", os.path.join(ROOT, "include"), src, "-o", out],
+                   check=True)
+    r = subprocess.run([out], capture_output=True, text=True, timeout=60)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
