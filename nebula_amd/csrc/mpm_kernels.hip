@@ -1259,7 +1259,8 @@ void launch_g2p(const DevState& d, const SubParams& p, bool use_sorted, hipStrea
 // per cell, triangles staged through LDS; every product and sum is rounded
 // as written (no FMA contraction), as on the host.
 // ---------------------------------------------------------------------------
-constexpr uint32_t kMeshChunk = 256;
+constexpr uint32_t kMeshChunk = 256;         // triangles staged in LDS at a time
+constexpr uint32_t kMeshLaunchTris = 2048;   // triangles per launch
 template <bool EXACT>
 __global__ __launch_bounds__(256) void k_mesh_sdf(const float* __restrict__ tris, uint32_t n_tris, int rx, int ry,
                                                   int rz, float cs, float* __restrict__ sdf) {
@@ -1269,8 +1270,12 @@ __global__ __launch_bounds__(256) void k_mesh_sdf(const float* __restrict__ tris
   const bool valid = c < uint32_t(rx) * uint32_t(ry) * uint32_t(rz);
   const int ix = int(c % uint32_t(rx)), iy = int((c / uint32_t(rx)) % uint32_t(ry)), iz = int(c / (uint32_t(rx) * uint32_t(ry)));
   const float px = (float(ix) + 0.5f) * cs, py = (float(iy) + 0.5f) * cs, pz = (float(iz) + 0.5f) * cs;
-  float best = 1e9f;
-  bool outer = true;
+  const uint32_t m = valid ? morton3(uint32_t(ix), uint32_t(iy), uint32_t(iz)) : 0u;
+  // the minimum over the triangles of earlier launches: its magnitude and
+  // whether it lies outside (1e9 and outside before the first)
+  const float prev = valid ? sdf[m] : 1e9f;
+  float best = fabsf(prev);
+  bool outer = !signbit(prev);
   auto dot3 = [](float ax, float ay, float az, float bx, float by, float bz) { return (ax * bx + ay * by) + az * bz; };
   for (uint32_t t0 = 0; t0 < n_tris; t0 += kMeshChunk) {
     const uint32_t nt = min(kMeshChunk, n_tris - t0);
@@ -1327,7 +1332,7 @@ __global__ __launch_bounds__(256) void k_mesh_sdf(const float* __restrict__ tris
       }
     }
   }
-  if (valid) sdf[morton3(uint32_t(ix), uint32_t(iy), uint32_t(iz))] = outer ? best : -best;
+  if (valid) sdf[m] = outer ? best : -best;
 }
 __global__ void k_fill(float* __restrict__ a, uint32_t n, float v) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1339,10 +1344,16 @@ void launch_mesh_sdf(const float* tris, uint32_t n_tris, const int32_t res[3], u
   const uint32_t nc = cube * cube * cube, cells = uint32_t(res[0]) * uint32_t(res[1]) * uint32_t(res[2]);
   k_fill<<<cdiv(nc, 256), 256, 0, s>>>(sdf, nc, 1e9f);
   if (!cells) return;
-  if (exact)
-    k_mesh_sdf<true><<<cdiv(cells, 256), 256, 0, s>>>(tris, n_tris, res[0], res[1], res[2], cs, sdf);
-  else
-    k_mesh_sdf<false><<<cdiv(cells, 256), 256, 0, s>>>(tris, n_tris, res[0], res[1], res[2], cs, sdf);
+  // one launch per chunk of triangles, so that no launch runs long on a large
+  // mesh; each continues from the minimum the earlier ones left in sdf
+  for (uint32_t t0 = 0; t0 < n_tris; t0 += kMeshLaunchTris) {
+    const uint32_t nt = std::min(kMeshLaunchTris, n_tris - t0);
+    const float* tr = tris + size_t(t0) * 12;
+    if (exact)
+      k_mesh_sdf<true><<<cdiv(cells, 256), 256, 0, s>>>(tr, nt, res[0], res[1], res[2], cs, sdf);
+    else
+      k_mesh_sdf<false><<<cdiv(cells, 256), 256, 0, s>>>(tr, nt, res[0], res[1], res[2], cs, sdf);
+  }
 }
 
 void launch_sdf_sphere(float* sdf, const SubParams& p, float r, float cx, float cy, float cz, hipStream_t s) {

@@ -4,8 +4,8 @@ oracle's buildMeshSDF (src/core/MeshSDF.h:85-126).
 Both round every operation as written (no FMA contraction) and divide and
 take square roots correctly rounded, so the SDFs must agree bit for bit, in
 the reference's variant and in the exact one, for a box, a rotated box read
-back from an STL file, a triangle soup that spans several LDS chunks, and no
-triangles at all.  A run with the mesh installed must equal a run with the
+back from an STL file, triangle soups that span several LDS chunks and
+several launches, and no triangles at all.  A run with the mesh installed must equal a run with the
 same SDF handed over as a dense Morton array (setColliderSDF)."""
 import numpy as np
 import pytest
@@ -33,8 +33,14 @@ def _meshes(tmp_path):
     nrm = np.cross(v[:, 1] - v[:, 0], v[:, 2] - v[:, 0])
     nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
     soup = np.concatenate([nrm, v.reshape(600, 9)], axis=1).astype(np.float32)
+    # small triangles spread over the domain: more than one launch's worth (2048)
+    c = rng.uniform(0.05, 0.6, (4500, 1, 3))
+    v2 = (c + rng.uniform(-0.02, 0.02, (4500, 3, 3))).astype(np.float32)
+    n2 = np.cross(v2[:, 1] - v2[:, 0], v2[:, 2] - v2[:, 0])
+    n2 /= np.linalg.norm(n2, axis=1, keepdims=True)
+    soup2 = np.concatenate([n2, v2.reshape(4500, 9)], axis=1).astype(np.float32)
     return {"box": box, "rotated_stl": mesh.load_binary_stl(tmp_path / "rot.stl"), "soup_600": soup,
-            "empty": np.zeros((0, 12), np.float32)}
+            "soup_4500": soup2, "empty": np.zeros((0, 12), np.float32)}
 
 
 @pytest.mark.parametrize("exact", [False, True], ids=["reference", "exact"])
