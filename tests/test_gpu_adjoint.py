@@ -27,7 +27,7 @@ pytestmark = pytest.mark.gpu
 N_SUB, SEG = 4, 2
 
 
-def _scene(flip_ratio=-1.0, seed=11):
+def _scene(flip_ratio=-1.0, seed=11, forces=None):
     rng = np.random.default_rng(seed)
     res, h = 16, 1.0 / 16
     g = np.arange(6, 10)
@@ -49,7 +49,7 @@ def _scene(flip_ratio=-1.0, seed=11):
     mats = [_material(2e3, 0.3, 1e3, MaterialModel.ELASTIC),
             _material(2e3, 0.3, 1e3, MaterialModel.DRUCKER_PRAGER, M=math.tan(math.radians(30)), qc=2.0)]
     return Scene("adjoint_block", cfg, mats, f, dt=2e-3, flip_ratio=flip_ratio, substeps=N_SUB,
-                 forces=[GravityForce((0.0, 1.0, 0.0), -9.8)], corrected_svd=True)
+                 forces=forces if forces is not None else [GravityForce((0.0, 1.0, 0.0), -9.8)], corrected_svd=True)
 
 
 # lanes of the eight float4 arrays that carry each input group
@@ -139,6 +139,18 @@ def test_adjoint_apic_matches_fp64_finite_differences(gpu_lib):
 
 def test_adjoint_pic_matches_fp64_finite_differences(gpu_lib):
     _check_scene(_scene(flip_ratio=0.0))
+
+
+def test_adjoint_with_noise_forces_matches_fp64_finite_differences(gpu_lib):
+    """With turbulence and noise forces the grid update's backward recomputes
+    its branch flags through the forces' noise (its NOISE form).  The forces
+    add constants that depend on node positions only; kept weak, the FP32
+    and FP64 noise (whose hash differs in the last bits) moves the FP64
+    trajectory too little to matter for the derivatives."""
+    forces = [GravityForce((0.0, 1.0, 0.0), -9.8),
+              mpm.TurbulenceForce(strength=0.05, frequency=3.0, octaves=2, seed=0.3),
+              mpm.NoiseForce((1.0, 0.0, 0.0), strength=0.05, frequency=2.0, octaves=1, seed=0.1)]
+    _check_scene(_scene(flip_ratio=-1.0, forces=forces))
 
 
 def test_adjoint_rejects_unsupported_configurations(gpu_lib):
