@@ -60,6 +60,33 @@ def test_batch_equals_scenes_run_alone(gpu_lib, flip_ratio):
     _compare_states(b.getState(), singles, scenes)
 
 
+def test_batch_with_noise_forces_equals_scenes_run_alone(gpu_lib):
+    """Turbulence and noise forces evaluate their noise at each node's world
+    position in its scene's own coordinates, so the batch's noise form of the
+    grid update (BATCH and NOISE) keeps each scene bit for bit."""
+    forces = [mpm.GravityForce((0.0, 1.0, 0.0), -9.8),
+              mpm.TurbulenceForce(strength=2.0, frequency=3.0, octaves=2, seed=0.3),
+              mpm.NoiseForce((1.0, 0.0, 0.0), strength=1.5, frequency=2.0, octaves=2, seed=0.1)]
+    scenes = _scenes()
+    for sc in scenes:
+        sc.forces = forces
+    singles = []
+    for sc in scenes:
+        e = make_engine(sc)
+        e.substep(sc.dt, N_SUB)
+        e.sync()
+        singles.append(e.getState())
+    b = make_batch_engine(scenes)
+    b.substep(scenes[0].dt, N_SUB)
+    b.sync()
+    _compare_states(b.getState(), singles, scenes)
+    # the forces acted: the same run without them ends elsewhere
+    plain = make_engine(_scenes()[0])
+    plain.substep(scenes[0].dt, N_SUB)
+    plain.sync()
+    assert not np.array_equal(plain.getState()[0], singles[0][0])
+
+
 def test_batch_reverse_mode_equals_scenes_run_alone(gpu_lib):
     scenes = _scenes(corrected=True)
     rng = np.random.default_rng(3)
