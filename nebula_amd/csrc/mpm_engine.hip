@@ -1012,8 +1012,8 @@ int nb_mpm_emit(nb_mpm* h, float dt) {
     int& done = h->emit_done[si];
     const bool go = e.step_count == -1 ? done == 0 : (e.step_count == 0 ? true : done < e.step_count);
     if (!go) continue;
-    const int avail = int(h->cap) - int(h->n_live);
-    const int n = std::min(e.particles_per_step, avail);
+    const int64_t avail = int64_t(h->cap) - int64_t(h->n_live);
+    const int64_t n = std::min<int64_t>(e.particles_per_step, avail);
     if (n <= 0) {
       done++;
       continue;
@@ -1021,12 +1021,12 @@ int nb_mpm_emit(nb_mpm* h, float dt) {
     std::vector<float> pos(size_t(n) * 4), vel(size_t(n) * 4);
     float id_bits;
     std::memcpy(&id_bits, &e.particle_type, 4);
-    for (int j = 0; j < n; j++) {  // all positions, then all velocities (MPMEngine.cpp:324-336)
+    for (int64_t j = 0; j < n; j++) {  // all positions, then all velocities (MPMEngine.cpp:324-336)
       float x, y, z;
       nebula::emitter_sample(e, h->emit_rng, x, y, z);
       pos[4 * j + 0] = x; pos[4 * j + 1] = y; pos[4 * j + 2] = z; pos[4 * j + 3] = Vp;
     }
-    for (int j = 0; j < n; j++) {
+    for (int64_t j = 0; j < n; j++) {
       float vx, vy, vz;
       nebula::emitter_velocity(e, pos[4 * j + 0], pos[4 * j + 1], pos[4 * j + 2], h->emit_rng, vx, vy, vz);
       vel[4 * j + 0] = vx; vel[4 * j + 1] = vy; vel[4 * j + 2] = vz; vel[4 * j + 3] = id_bits;
