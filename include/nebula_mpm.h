@@ -242,6 +242,12 @@ int nb_mpm_substep(nb_mpm* h, float dt_sub);
 int nb_mpm_substeps(nb_mpm* h, float dt_sub, uint32_t n);
 /* Wait for all work enqueued on the engine's stream. */
 int nb_mpm_sync(nb_mpm* h);
+/* Enqueue the engine's later work on hip_stream (a hipStream_t), or on an
+ * engine-owned stream of default priority when NULL: the counterpart of the
+ * command buffer MPMEngine::step records into (src/engine/MPMEngine.cpp:404).
+ * Waits first for what the previous stream holds.  The caller keeps its
+ * stream alive while the engine uses it. */
+int nb_mpm_set_stream(nb_mpm* h, void* hip_stream);
 
 int nb_mpm_set_knobs(nb_mpm* h, const nb_mpm_knobs* knobs);
 int nb_mpm_get_knobs(const nb_mpm* h, nb_mpm_knobs* knobs);

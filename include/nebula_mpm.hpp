@@ -309,6 +309,13 @@ class MPMEngine {
     check(nb_mpm_step(h_, dt));
     syncEmitters();
   }
+  // step(VkCommandBuffer, dt) records into the caller's buffer; this enqueues
+  // on the caller's hipStream_t (passed as void*), and later calls stay there
+  void step(void* hip_stream, float dt) {
+    setStream(hip_stream);
+    step(dt);
+  }
+  void setStream(void* hip_stream) { check(nb_mpm_set_stream(h_, hip_stream)); }
 
   // emitters (src/engine/MPMEngine.cpp:271-352); their centers follow the engine's
   void addEmitter(std::shared_ptr<Emitter> e) {

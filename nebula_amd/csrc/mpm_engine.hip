@@ -55,8 +55,8 @@ struct nb_mpm {
   uint32_t grid_res[3] = {1, 1, 1};
   uint32_t cube_res = 1, NC = 1, tile_dim = 1, NT = 1, cap = 0, cap_tiles = 1;
   uint32_t n_live = 0;
-  hipStream_t stream = nullptr;
-  bool own_stream = false;
+  hipStream_t stream = nullptr;  // where work is enqueued
+  hipStream_t owned = nullptr;   // the stream the engine created, if any
   DevState d{};
   uint32_t material_count = 1, collider_count = 0, force_count = 0, noise_force_count = 0;
   std::vector<nb_material> mats_host;  // host copy of the material table
@@ -553,9 +553,9 @@ int engine_alloc(nb_mpm* h, void* hip_stream) {
   if (hip_stream) {
     h->stream = reinterpret_cast<hipStream_t>(hip_stream);
   } else {
-    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
+    if (hipStreamCreateWithFlags(&h->owned, hipStreamNonBlocking) != hipSuccess)
       return fail(NB_ERR_HIP, "hipStreamCreate failed");
-    h->own_stream = true;
+    h->stream = h->owned;
   }
 
   DevState& d = h->d;
@@ -618,7 +618,7 @@ int nb_mpm_destroy(nb_mpm* h) {
   if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
   for (void* p : h->allocs) (void)hipFree(p);
   for (void* p : h->adj.allocs) (void)hipFree(p);
-  if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
+  if (h->owned) (void)hipStreamDestroy(h->owned);
   delete h;
   return NB_OK;
 }
@@ -641,6 +641,17 @@ int nb_mpm_substep(nb_mpm* h, float dt_sub) {
 int nb_mpm_substeps(nb_mpm* h, float dt_sub, uint32_t n) {
   if (!h) return fail(NB_ERR_ARG, "null handle");
   return run_substeps(h, dt_sub, n, nullptr);
+}
+
+int nb_mpm_set_stream(nb_mpm* h, void* hip_stream) {
+  if (!h) return fail(NB_ERR_ARG, "null handle");
+  if (!hip_stream && !h->owned && hipStreamCreateWithFlags(&h->owned, hipStreamNonBlocking) != hipSuccess)
+    return fail(NB_ERR_HIP, "hipStreamCreate failed");
+  hipStream_t next = hip_stream ? reinterpret_cast<hipStream_t>(hip_stream) : h->owned;
+  if (next == h->stream) return NB_OK;
+  NB_HIP(hipStreamSynchronize(h->stream));  // what the old stream holds comes first
+  h->stream = next;
+  return NB_OK;
 }
 
 int nb_mpm_sync(nb_mpm* h) {

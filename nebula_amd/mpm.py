@@ -404,7 +404,8 @@ _lib = None
 
 EXPORTED_SYMBOLS = (
     "nb_last_error", "nb_mpm_abi_version", "nb_mpm_config_default", "nb_mpm_knobs_default", "nb_mpm_create",
-    "nb_mpm_destroy", "nb_mpm_step", "nb_mpm_substep", "nb_mpm_substeps", "nb_mpm_sync", "nb_mpm_set_knobs",
+    "nb_mpm_destroy", "nb_mpm_step", "nb_mpm_substep", "nb_mpm_substeps", "nb_mpm_sync", "nb_mpm_set_stream",
+    "nb_mpm_set_knobs",
     "nb_mpm_get_knobs", "nb_mpm_get_config", "nb_mpm_grid_res", "nb_mpm_total_cells",
     "nb_mpm_live_particle_count", "nb_mpm_set_materials", "nb_mpm_set_particle_material_ids",
     "nb_mpm_set_colliders", "nb_mpm_clear_analytic_colliders", "nb_mpm_set_collider_sdf",
@@ -578,6 +579,7 @@ def lib():
         for nm in ("nb_mpm_step", "nb_mpm_substep", "nb_mpm_run_p2g", "nb_mpm_run_grid_update", "nb_mpm_run_g2p"):
             getattr(L, nm).argtypes = [vp, C.c_float]
         L.nb_mpm_substeps.argtypes = [vp, C.c_float, C.c_uint32]
+        L.nb_mpm_set_stream.argtypes = [vp, vp]
         L.nb_mpm_set_knobs.argtypes = [vp, C.POINTER(_Knobs)]
         L.nb_mpm_get_knobs.argtypes = [vp, C.POINTER(_Knobs)]
         L.nb_mpm_get_config.argtypes = [vp, C.POINTER(MPMConfig)]
@@ -745,6 +747,12 @@ class MPMEngine:
 
     def sync(self):
         _check(lib().nb_mpm_sync(self._h))
+
+    def setStream(self, stream: Optional[int] = None):
+        """Enqueue later work on the hipStream_t `stream` (an integer handle,
+        e.g. torch.cuda.Stream().cuda_stream), or on the engine's own stream
+        when None (nb_mpm_set_stream)."""
+        _check(lib().nb_mpm_set_stream(self._h, C.c_void_p(stream) if stream else None))
 
     # knobs ---------------------------------------------------------------
     def _knobs(self) -> _Knobs:
