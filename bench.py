@@ -103,6 +103,30 @@ def adjoint_leg(sc, k):
     return res
 
 
+def vm_leg(sc, k, warmup):
+    """The scene's Von Mises variant (q_max = 500 in place of Drucker-Prager,
+    same particles, SDF and dt): a secondary line of the headline workload."""
+    import dataclasses
+
+    from nebula_amd.mpm import MaterialModel
+    from nebula_amd.scenes import _material, make_engine
+
+    vm = dataclasses.replace(sc, name="snow_terrain_256_vm",
+                             materials=[_material(1.4e5, 0.2, 400.0, MaterialModel.VON_MISES, qmax=500.0)])
+    log(f"[bench] Von Mises variant: {k} substeps after {warmup}")
+    eng = make_engine(vm)
+    eng.substep(vm.dt, warmup)
+    eng.sync()
+    t0 = time.perf_counter()
+    eng.substep(vm.dt, k)
+    eng.sync()
+    el = time.perf_counter() - t0
+    log(f"[bench] Von Mises variant: {el / k * 1e3:.4f} ms/substep")
+    return {"workload": "snow_terrain_256 with Von Mises snow (q_max 500): 4,194,304 particles, 256^3 grid, APIC, "
+                        "dt 5e-5", "substeps": k, "ms_per_substep": round(el / k * 1e3, 5),
+            "particle_substeps_per_s": round(vm.n * k / el, 1)}
+
+
 def diff_batch_leg(k, seg):
     """Forward+adjoint over forward time on the 64-scene differentiable batch."""
     import numpy as np
@@ -208,6 +232,7 @@ def main():
                     help="substeps of the differentiable-batch reverse-mode leg (0 = skip)")
     ap.add_argument("--diff-segment", type=int, default=20, help="checkpoint spacing of that leg")
     ap.add_argument("--batch-substeps", type=int, default=20, help="substeps of the 256-scene leg (0 = skip)")
+    ap.add_argument("--vm-steps", type=int, default=100, help="substeps of the Von Mises variant (0 = skip)")
     ap.add_argument("--calibration-copy-mb", type=int, default=0,
                     help="also run a plain copy kernel of this many MiB (HBM byte-counter calibration)")
     args = ap.parse_args()
@@ -255,6 +280,7 @@ def main():
 
     adjoint1 = adjoint_leg(sc, args.adjoint_substeps) if args.adjoint_substeps > 0 else None
     del eng
+    vm = vm_leg(sc, args.vm_steps, args.warmup) if args.vm_steps > 0 else None
     adjoint = diff_batch_leg(args.diff_substeps, args.diff_segment) if args.diff_substeps > 0 else None
     batch = batch_leg(args.batch_substeps) if args.batch_substeps > 0 else None
 
@@ -290,6 +316,7 @@ def main():
         "substep_roofline": {"bytes_per_step": b_alg, "achieved": round(substep_rate, 2), "peak": HBM_PEAK_GBS,
                              "unit": "GB/s", "frac": round(substep_rate / HBM_PEAK_GBS, 4)},
         "kernels_avg_ms": {k: round(v, 5) for k, v in avg_ms.items()},
+        "von_mises_variant": vm,
         "adjoint": adjoint,
         "adjoint_single_scene": adjoint1,
         "batch": batch,
