@@ -46,3 +46,18 @@ def test_facade_emitters_run_reference_emitter_tests():
     r = subprocess.run([out], capture_output=True, text=True, timeout=60)
     print(r.stdout, r.stderr)
     assert r.returncode == 0
+
+
+def test_facade_host_code_is_clean_under_sanitizers():
+    """The facade's host-side code (emitter sampling, record packing) built
+    with AddressSanitizer and UndefinedBehaviorSanitizer: the emitter test
+    program runs without a report.  Host code only, no GPU."""
+    src = os.path.join(ROOT, "tests", "cpp", "emitter_cpp_test.cpp")
+    out = os.path.join(ROOT, "_build", "emitter_cpp_test_san")
+    os.makedirs(os.path.dirname(out), exist_ok=True)
+    subprocess.run(["g++", "-std=c++17", "-O1", "-g", "-fsanitize=address,undefined", "-fno-sanitize-recover=all",
+                    "-I", os.path.join(ROOT, "include"), src, "-o", out], check=True)
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=0", UBSAN_OPTIONS="print_stacktrace=1")
+    r = subprocess.run([out], capture_output=True, text=True, timeout=120, env=env)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0 and "runtime error" not in r.stderr and "AddressSanitizer" not in r.stderr
