@@ -190,3 +190,4 @@ def test_adjoint_segmentations_agree_bitwise(gpu_lib):
                 ref = (g, dp)
             assert np.array_equal(g, ref[0]), f"segment {seg}: state gradient differs"
             assert np.array_equal(dp, ref[1]), f"segment {seg}: parameter gradient differs"
+
