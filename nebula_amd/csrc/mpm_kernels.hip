@@ -960,15 +960,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     // powder and viscoplastic mud): return map, then Hencky stress of the
     // projected F.  In corrected mode the return map's SVD is the true SVD of
     // the projected F, so its factors give the stress without a second SVD.
+    // In either mode a Von Mises particle inside the yield surface keeps F
+    // unchanged, so the second SVD would repeat the first on the same input.
     const bool dp = mat.model == NB_MAT_DRUCKER_PRAGER || mat.model == NB_MAT_GRANULAR_POWDER ||
                     mat.model == NB_MAT_VISCOPLASTIC_MUD;
     const bool plastic = dp || mat.model == NB_MAT_VON_MISES;
+    bool have_tau = false;
     if (plastic) {
       const PlasticOut po = project_plastic(Fn, dp, mat.mu, mat.lambda, mat.q_max, mat.M_friction, mat.q_cohesion, cs);
       Fn = po.F;
-      if (cs) tau = hencky_from_log(po.U, po.eps[0], po.eps[1], po.eps[2], mat.mu, mat.lambda);
+      if (cs || !po.recomposed) {
+        tau = hencky_from_log(po.U, po.eps[0], po.eps[1], po.eps[2], mat.mu, mat.lambda);
+        have_tau = true;
+      }
     }
-    if (!(plastic && cs)) tau = hencky_stress(Fn, mat.mu, mat.lambda, cs);
+    if (!have_tau) tau = hencky_stress(Fn, mat.mu, mat.lambda, cs);
     if (mat.model == NB_MAT_VISCOPLASTIC_MUD) {
       m3 D;
 #pragma unroll

@@ -244,6 +244,7 @@ struct PlasticOut {
   m3 F;
   m3 U;
   float eps[3];
+  bool recomposed;  // false: F was returned as given, and U, eps are its own SVD's
 };
 __device__ __forceinline__ PlasticOut project_plastic(const m3& F, bool dp, float mu, float lam, float q_max, float M,
                                                       float q_c, bool corrected) {
@@ -280,6 +281,7 @@ __device__ __forceinline__ PlasticOut project_plastic(const m3& F, bool dp, floa
   }
   PlasticOut o;
   o.U = U;
+  o.recomposed = recompose;
   if (recompose) {
     const float x0 = hw_exp(e0), x1 = hw_exp(e1), x2 = hw_exp(e2);
     o.F = m3_udvt(U, x0, x1, x2, V);
