@@ -106,6 +106,48 @@ def adjoint_leg(sc, k):
     return res
 
 
+def host_transfer_leg(eng, sc, frame_substeps=20, reps=3):
+    """Copies across the boundary for callers that hand over host arrays:
+    the whole state (eight float4 arrays, 128 B per particle) each way and
+    the positions alone back, through nb_mpm_set_field / nb_mpm_get_field
+    from pageable numpy arrays; then whole frames of frame_substeps substeps
+    (the reference's default numSubsteps) that upload the state first and
+    read the positions back after, timed end to end (median of reps)."""
+    n = sc.n
+    state = eng.getState()
+    eng.setState(state)
+    eng.sync()
+
+    def timed(fn):
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            eng.sync()
+            ts.append(time.perf_counter() - t0)
+        return sorted(ts)[len(ts) // 2]
+
+    t_down = timed(eng.getState)
+    t_up = timed(lambda: eng.setState(state))
+    t_pos = timed(lambda: eng.getField(0))
+
+    def frame():
+        eng.setState(state)
+        eng.substep(sc.dt, frame_substeps)
+        eng.getField(0)
+
+    t_frame = timed(frame)
+    gb = n * 128 / 1e9
+    log(f"[bench] host copies: state up {t_up * 1e3:.1f} ms, down {t_down * 1e3:.1f} ms, positions down "
+        f"{t_pos * 1e3:.1f} ms; frame of {frame_substeps} substeps with upload and readback {t_frame * 1e3:.1f} ms")
+    return {"state_bytes": n * 128, "state_upload_ms": round(t_up * 1e3, 3),
+            "state_upload_GBps": round(gb / t_up, 2), "state_download_ms": round(t_down * 1e3, 3),
+            "state_download_GBps": round(gb / t_down, 2), "positions_download_ms": round(t_pos * 1e3, 3),
+            "frame_substeps": frame_substeps, "frame_ms": round(t_frame * 1e3, 3),
+            "frame_particle_substeps_per_s": round(n * frame_substeps / t_frame, 1),
+            "note": "host arrays are pageable numpy memory; value is measured with the state resident in HBM"}
+
+
 def vm_leg(sc, k, warmup):
     """The scene's Von Mises variant (q_max = 500 in place of Drucker-Prager,
     same particles, SDF and dt): a secondary line of the headline workload."""
@@ -242,6 +284,7 @@ def main():
     ap.add_argument("--diff-segment", type=int, default=20, help="checkpoint spacing of that leg")
     ap.add_argument("--batch-substeps", type=int, default=20, help="substeps of the 256-scene leg (0 = skip)")
     ap.add_argument("--vm-steps", type=int, default=100, help="substeps of the Von Mises variant (0 = skip)")
+    ap.add_argument("--host-frames", type=int, default=1, help="time host copies and whole frames with them (0 = skip)")
     ap.add_argument("--calibration-copy-mb", type=int, default=0,
                     help="also run a plain copy kernel of this many MiB (HBM byte-counter calibration)")
     args = ap.parse_args()
@@ -287,6 +330,7 @@ def main():
     if args.calibration_copy_mb > 0:
         eng.calibrationCopy(args.calibration_copy_mb << 20, 3)
 
+    host = host_transfer_leg(eng, sc) if args.host_frames else None
     adjoint1 = adjoint_leg(sc, args.adjoint_substeps) if args.adjoint_substeps > 0 else None
     del eng
     vm = vm_leg(sc, args.vm_steps, args.warmup) if args.vm_steps > 0 else None
@@ -325,6 +369,7 @@ def main():
         "substep_roofline": {"bytes_per_step": b_alg, "achieved": round(substep_rate, 2), "peak": HBM_PEAK_GBS,
                              "unit": "GB/s", "frac": round(substep_rate / HBM_PEAK_GBS, 4)},
         "kernels_avg_ms": {k: round(v, 5) for k, v in avg_ms.items()},
+        "host_transfer": host,
         "von_mises_variant": vm,
         "adjoint": adjoint,
         "adjoint_single_scene": adjoint1,
