@@ -12,7 +12,8 @@ import numpy as np
 import pytest
 
 import oracle
-from nbtest import F_of, harness_scene, oracle_sim, rel_err, stress_of
+from nbtest import (F_of, check_jacobian_reset, degenerate_hencky, degenerate_svd_cases, harness_scene,
+                    jacobian_reset_state, oracle_sim, rel_err, stress_of)
 from nebula_amd import mpm
 from nebula_amd.mpm import ColliderSet, MaterialModel, MaterialParams
 from nebula_amd.scenes import (elastic_block_64, make_engine, multi_material_128, snow_terrain_256)
@@ -441,6 +442,58 @@ def test_g2p_matches_oracle(gpu_lib, model, mode, corrected):
             rep.append(f"{k} full-match {frac:.4f}")
             assert frac >= 0.99
     say(f"  g2p {model} {sc.flip_ratio:+.2f} {'corr' if corrected else 'ref'}: " + " ".join(rep))
+
+
+@pytest.mark.parametrize("name", ["stall", "zero_sigma"])
+def test_g2p_degenerate_svd_branches(gpu_lib, name):
+    """G2P's stress on the two inputs of degenerate_svd_cases, which reach the
+    Jacobi SVD's degenerate branches: the sign(0) stall
+    (shaders/mpm_common.glsl:226) and the unit vector in place of a zero
+    singular value's (shaders/mpm_common.glsl:274), with the reference SVD
+    convention.  On a zero grid the velocity gradient is exactly zero, so F
+    passes through bit for bit and every particle's tau must equal the
+    hand-worked Hencky stress (tolerance 1e-5 of its largest entry; taking the
+    non-degenerate branch instead changes tau by more than 10 % of it)."""
+    sc = elastic_block_64()
+    sc.corrected_svd = False
+    m = sc.materials[0]
+    F = degenerate_svd_cases()[name][0].astype(np.float32)
+    st = sc.fields.copy()
+    st[1, :, :3] = 0.0
+    st[5:8, :, :3] = 0.0
+    for c in range(3):
+        st[2 + c, :, :3] = F[:, c]
+    e = make_engine(sc)
+    e.setState(st)
+    nc = e.totalCells()
+    e.setGrid(np.zeros((nc, 4), np.float32), np.zeros(nc, np.float32))
+    e.runBin()
+    e.runG2P(sc.dt)
+    g = e.getState()
+    assert np.array_equal(g[2:5, :, :3], st[2:5, :, :3])
+    ref = degenerate_hencky(name, m.mu, m.lambda_)
+    want = np.array([ref[0, 0], ref[1, 1], ref[2, 2], ref[0, 1], ref[0, 2], ref[1, 2]])
+    got = g[2:8, :, 3].astype(np.float64)  # txx, tyy, tzz, txy, txz, tyz per particle
+    err = float(np.abs(got - want[:, None]).max() / np.abs(want).max())
+    say(f"  g2p degenerate SVD {name}: tau {want.round(3)}, max error {err:.1e} of max |tau|")
+    assert err <= 1e-5
+
+
+def test_g2p_jacobian_reset(gpu_lib):
+    """G2P's Jacobian reset (shaders/mpm_g2p.comp:186-193) on a zero grid:
+    F = s I with det F on both sides of 0.15 and of 10; reset particles get
+    F = I and tau = 0 exactly, the others keep F exactly and get the closed-form
+    Hencky stress of s I (within 2e-6 of the largest)."""
+    sc = elastic_block_64()
+    m = sc.materials[0]
+    st, s = jacobian_reset_state(sc.fields)
+    e = make_engine(sc)
+    e.setState(st)
+    nc = e.totalCells()
+    e.setGrid(np.zeros((nc, 4), np.float32), np.zeros(nc, np.float32))
+    e.runBin()
+    e.runG2P(sc.dt)
+    check_jacobian_reset(e.getState(), s, m.mu, m.lambda_)
 
 
 # ---------------------------------------------------------------------------

@@ -11,7 +11,8 @@ import numpy as np
 import pytest
 
 import oracle
-from nbtest import F_of, harness_scene, oracle_sim, stress_of
+from nbtest import (F_of, check_jacobian_reset, degenerate_hencky, degenerate_svd_cases, harness_scene,
+                    jacobian_reset_state, oracle_sim, stress_of)
 from nebula_amd import mpm
 from nebula_amd.scenes import elastic_block_64
 
@@ -174,6 +175,36 @@ def test_svd_fp32_close_to_fp64():
         _, s32, _ = oracle.svd3(F, 32)
         _, s64, _ = oracle.svd3(F.astype(np.float64), 64)
         assert np.allclose(s32, s64, rtol=2e-5, atol=2e-6)
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+@pytest.mark.parametrize("name", ["stall", "zero_sigma"])
+def test_svd_degenerate_branches_closed_form(name, precision):
+    F, s_ref, U_ref, V_ref = degenerate_svd_cases()[name]
+    U, s, V = oracle.svd3(F, precision)
+    tol = 1e-6 if precision == 32 else 1e-14
+    assert np.allclose(s, s_ref, rtol=tol, atol=0)
+    assert np.allclose(U, U_ref, rtol=0, atol=tol)
+    assert np.allclose(V, V_ref, rtol=0, atol=tol)
+    tau = oracle.hencky_stress(F, MU, LAM, precision)
+    ref = degenerate_hencky(name, MU, LAM)
+    assert np.allclose(tau, ref, rtol=0, atol=(3e-6 if precision == 32 else 1e-12) * np.abs(ref).max())
+
+
+def test_g2p_jacobian_reset():  # shaders/mpm_g2p.comp:186-193
+    """On a zero grid F passes through G2P unchanged unless |det F| < 0.15 or
+    > 10, where F becomes I and tau 0; elsewhere tau is the Hencky stress of
+    the diagonal F, diag(lam tr(eps) + 2 mu eps) with eps = log s."""
+    sc = elastic_block_64()
+    m = sc.materials[0]
+    st, s = jacobian_reset_state(sc.fields)
+    sim = oracle_sim(sc, 32)
+    sim.set_state(st)
+    sim.bin()
+    sim.set_grid(np.zeros((sim.NC, 4), np.float32), np.zeros(sim.NC, np.float32))
+    sim.g2p()
+    out = sim.get_state()
+    check_jacobian_reset(out, s, m.mu, m.lambda_)
 
 
 def test_hencky_identity_is_stress_free():  # tests/test_mpm.cpp:137-144
