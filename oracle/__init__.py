@@ -12,9 +12,11 @@ on the arithmetic type: ``precision=32`` is the reference-semantics FP32 oracle,
 
 Parity status: every function is pinned by a test in ``tests/test_oracle*.py``
 (known-answer tests restated from the reference's tests/test_mpm.cpp and
-tests/test_domain.cpp, closed forms, numpy LAPACK, brute force), except the
+tests/test_domain.cpp, closed forms, numpy LAPACK, brute force).  The
 degenerate SVD branches (the ``sign(0)`` Jacobi stall and the ``sigma <= 1e-8``
-identity columns), which follow the GLSL text only: parity unpinned there.
+unit vectors) and G2P's Jacobian reset are pinned by results worked by hand
+from the GLSL text on inputs that reach them (``degenerate_svd_cases`` and
+``jacobian_reset_state`` in tests/nbtest.py).
 """
 from __future__ import annotations
 

@@ -17,8 +17,9 @@
 //   grid update / walls / forces  KATs 3-1, 3-2; collider algebra
 //   G2P ......................... KATs 4-1, 6-2, 7-1, 7-2, 7-3, 7-4, 8-1
 //   SVD / Hencky / return maps .. KATs 5-1, 5-2; numpy LAPACK SVD; yield invariants
-//   Degenerate SVD branches (sign(0) stall, sigma <= 1e-8 columns) are restated
-//   from the GLSL text only: parity unpinned beyond that text.
+//   Degenerate SVD branches (sign(0) stall, sigma <= 1e-8 unit vectors) and the
+//   Jacobian reset: results worked by hand from the GLSL text on inputs that
+//   reach them (test_oracle_kats, and through G2P on the GPU)
 #pragma once
 
 #include <algorithm>
