@@ -1,11 +1,13 @@
 """The C ABI boundary (CPU-only): the built library exports every entry point
 include/nebula_mpm.h declares, the POD records keep the reference's byte
 layouts, the host-only defaults equal the reference's, and without a device
-creation fails with a status and a message instead of falling back."""
+creation fails with a status and a message instead of falling back, and so
+does any engine call when the library itself is missing."""
 import ctypes as C
 import os
 import re
 import subprocess
+import sys
 
 import pytest
 
@@ -96,6 +98,21 @@ def test_create_without_device_fails_loudly(lib):
     assert b"device" in lib.nb_last_error()
     with pytest.raises(mpm.NebulaError):
         mpm.MPMEngine(c)
+
+
+def test_missing_library_fails_loudly():
+    """Without the HIP library every engine call raises; there is no CPU path."""
+    code = ("import nebula_amd.mpm as m\n"
+            "m.LIB_PATH = '/nonexistent/libnebula_mpm.so'\n"
+            "try:\n"
+            "    m.MPMEngine(m.MPMConfig())\n"
+            "except m.NebulaError as e:\n"
+            "    print('raised:', e)\n"
+            "else:\n"
+            "    print('no error')\n")
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, cwd=ROOT, timeout=120)
+    assert out.returncode == 0, out.stderr
+    assert out.stdout.startswith("raised: HIP library missing"), out.stdout
 
 
 def _c_layout(struct, fields):
