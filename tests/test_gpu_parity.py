@@ -615,3 +615,41 @@ def test_snow_terrain_p2g_and_properties(gpu_lib):
     assert np.isfinite(st).all()
     h = sc.cfg.cellSize
     assert (st[0, :, :3] >= 1.5 * h - 1e-6).all() and (st[0, :, :3] <= 1 - 1.5 * h + 1e-6).all()
+
+
+def test_snow_terrain_long_run_properties(gpu_lib):
+    """The 4,194,304-particle DP snow scene over 2,000 substeps (0.1 s): two
+    engines from the same start end bit-identical; the state is finite,
+    confined to the domain's interior, every |det F| lies in the Jacobian
+    reset's [0.15, 10], material ids are untouched, and P2G of the final state
+    conserves mass (the grid's total equals the particles' total rho0 Vp to
+    1e-5)."""
+    sc = snow_terrain_256()
+    n_sub = 2000
+    a = make_engine(sc)
+    b = make_engine(sc)
+    t0 = time.perf_counter()
+    a.substep(sc.dt, n_sub)
+    b.substep(sc.dt, n_sub)
+    a.sync()
+    b.sync()
+    say(f"  snow long run: 2 x {n_sub} substeps in {time.perf_counter() - t0:.2f} s")
+    st = a.getState()
+    assert np.array_equal(st.view(np.uint32), b.getState().view(np.uint32))
+    del b
+    assert np.isfinite(st).all()
+    assert np.array_equal(st[1, :, 3].view(np.uint32), sc.fields[1, :, 3].view(np.uint32))
+    h = sc.cfg.cellSize
+    assert (st[0, :, :3] >= 1.5 * h - 1e-6).all() and (st[0, :, :3] <= 1 - 1.5 * h + 1e-6).all()
+    F = np.stack([st[2, :, :3], st[3, :, :3], st[4, :, :3]], axis=2).astype(np.float64)
+    J = np.abs(np.linalg.det(F))
+    say(f"  snow long run: |det F| in [{J.min():.4f}, {J.max():.4f}], mean {J.mean():.4f}")
+    assert J.min() >= 0.15 * (1 - 1e-4) and J.max() <= 10.0 * (1 + 1e-4)
+    rho = np.array([m.rho0 for m in sc.materials], np.float64)[st[1, :, 3].view(np.uint32)]
+    want = float((rho * st[0, :, 3].astype(np.float64)).sum())
+    a.runBin()
+    a.runP2G(sc.dt)
+    _, gms = a.getGrid()
+    got = float(gms.astype(np.float64).sum())
+    say(f"  snow long run: grid mass {got:.8e}, particles {want:.8e}")
+    assert abs(got - want) <= 1e-5 * want
