@@ -172,6 +172,39 @@ def test_kat_engine_default_init_and_rect_domain(gpu_lib):  # 8-1 and MPMEngine:
     assert e.appendParticles(np.zeros((10, 4)), np.zeros((10, 4))) == 3
 
 
+def test_empty_engine(gpu_lib):
+    """No live particles: stepping, graph replay, every pass and the exports
+    are no-ops that succeed (empty bins, an all-zero grid); particles appended
+    afterwards step as usual, and resetParticles empties the engine again."""
+    cfg = mpm.MPMConfig(nx=0, ny=0, nz=0, domainSize=(1.0, 1.0, 1.0), cellSize=1.0 / 16, maxParticles=64)
+    e = mpm.MPMEngine(cfg)
+    e.addForce(mpm.GravityForce((0.0, 1.0, 0.0), -9.8))
+    assert e.liveParticleCount() == 0
+    e.step(1.0 / 60.0)
+    e.substep(1e-4, 5)
+    e.graphSubstep(1e-4, 3)
+    e.sync()
+    e.runBin()
+    keys, srt, cnt, end = e.getBins()
+    assert keys.size == 0 and srt.size == 0 and not cnt.any() and not end.any()
+    e.runP2G(1e-4)
+    e.runGridUpdate(1e-4)
+    e.runG2P(1e-4)
+    mom, mass = e.getGrid()
+    assert not mom.any() and not mass.any()
+    assert e.getState().shape == (8, 0, 4)
+    h = 1.0 / 16
+    n = e.appendParticles(np.array([[0.5, 0.5, 0.5, h ** 3]]), np.array([[0.0, 0.0, 0.0, 0.0]]))
+    assert n == 1
+    e.substep(1e-4, 10)
+    p = e.getField(mpm.FIELD_P)[0]
+    assert np.isfinite(p).all() and p[1] < 0.5  # falls under gravity
+    e.resetParticles()
+    assert e.liveParticleCount() == 0
+    e.substep(1e-4, 2)
+    e.sync()
+
+
 # ---------------------------------------------------------------------------
 # binning: bit-exact keys, stable order, counts, END offsets
 # ---------------------------------------------------------------------------
