@@ -148,6 +148,35 @@ def host_transfer_leg(eng, sc, frame_substeps=20, reps=3):
             "note": "host arrays are pageable numpy memory; value is measured with the state resident in HBM"}
 
 
+SWEEP_SIZES = [(128, 8), (256, 4), (256, 8), (256, 16), (512, 8), (512, 16)]
+
+
+def size_sweep_leg(k, warmup):
+    """The headline scene at other sizes on the one GPU: grid resolution res
+    (h = 1/res) and snow layer thickness (cells), 8 particles per cell, from
+    1.05 M to 33.5 M particles; dt scales with h (5e-5 at res 256)."""
+    from nebula_amd.scenes import make_engine, snow_terrain_256
+
+    out = []
+    for res, layer in SWEEP_SIZES:
+        sc = snow_terrain_256(res=res, layer=layer)
+        dt = sc.dt * 256.0 / res
+        eng = make_engine(sc)
+        eng.substep(dt, warmup)
+        eng.sync()
+        t0 = time.perf_counter()
+        eng.substep(dt, k)
+        eng.sync()
+        el = time.perf_counter() - t0
+        del eng
+        row = {"grid": res, "layer_cells": layer, "particles": sc.n, "dt": dt, "substeps": k,
+               "ms_per_substep": round(el / k * 1e3, 4), "particle_substeps_per_s": round(sc.n * k / el, 1)}
+        log(f"[bench] size sweep: {res}^3 grid, {sc.n} particles: {row['ms_per_substep']:.3f} ms/substep, "
+            f"{row['particle_substeps_per_s']:.4e} particle-substeps/s")
+        out.append(row)
+    return out
+
+
 def vm_leg(sc, k, warmup):
     """The scene's Von Mises variant (q_max = 500 in place of Drucker-Prager,
     same particles, SDF and dt): a secondary line of the headline workload."""
@@ -285,6 +314,8 @@ def main():
     ap.add_argument("--batch-substeps", type=int, default=20, help="substeps of the 256-scene leg (0 = skip)")
     ap.add_argument("--vm-steps", type=int, default=100, help="substeps of the Von Mises variant (0 = skip)")
     ap.add_argument("--host-frames", type=int, default=1, help="time host copies and whole frames with them (0 = skip)")
+    ap.add_argument("--sweep-substeps", type=int, default=0,
+                    help="substeps per size of the scene-size sweep, 1.05 M to 33.5 M particles (0 = skip)")
     ap.add_argument("--calibration-copy-mb", type=int, default=0,
                     help="also run a plain copy kernel of this many MiB (HBM byte-counter calibration)")
     args = ap.parse_args()
@@ -336,6 +367,7 @@ def main():
     vm = vm_leg(sc, args.vm_steps, args.warmup) if args.vm_steps > 0 else None
     adjoint = diff_batch_leg(args.diff_substeps, args.diff_segment) if args.diff_substeps > 0 else None
     batch = batch_leg(args.batch_substeps) if args.batch_substeps > 0 else None
+    sweep = size_sweep_leg(args.sweep_substeps, args.warmup) if args.sweep_substeps > 0 else None
 
     cpu = None
     if args.cpu_steps > 0:
@@ -374,6 +406,7 @@ def main():
         "adjoint": adjoint,
         "adjoint_single_scene": adjoint1,
         "batch": batch,
+        "size_sweep": sweep,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
