@@ -1145,7 +1145,6 @@ __global__ __launch_bounds__(256) void k_copy(float4* __restrict__ dst, const fl
 // launchers
 // ---------------------------------------------------------------------------
 static inline uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
-// blocks for a tile-stride kernel: enough to fill the chip, never more than tiles
 // blocks for a tile-stride kernel: never more than tiles, at most the tuned cap
 static inline uint32_t tile_grid(uint32_t n_tiles, uint32_t cap) {
   const uint32_t c = cap ? cap : 2048u;
