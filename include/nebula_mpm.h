@@ -204,7 +204,8 @@ int nb_mpm_create_batch(const nb_mpm_scene* scenes, uint32_t n_scenes, void* hip
  * cell is split into more work items (4..32).  tile_blocks and g2p_block
  * never change results; p2g_item changes the order of P2G's sums, so results
  * move within rounding.  ms_per_substep: what nb_mpm_autotune measured for
- * the choice (0 when set by hand). */
+ * the choice (0 when set by hand).  A new handle starts at tile_blocks 2048,
+ * g2p_block 128, p2g_item 12. */
 typedef struct nb_mpm_tuning {
   uint32_t tile_blocks;
   uint32_t g2p_block;
