@@ -50,7 +50,7 @@ HBM_PEAK_GBS = 8000.0
 # x 2.0, write x 1.0).  A counter pass cannot run inside the timed bench; the
 # commands and the per-kernel table are in the file named below.
 TRAFFIC_SOURCE = "profiles/hbm_traffic_snow_terrain_256.md"
-MEASURED_TRAFFIC = {"g2p": 1337.9e6, "p2g": 856.3e6, "grid_update": 116.9e6}
+MEASURED_TRAFFIC = {"g2p": 1469.9e6, "p2g": 856.5e6, "grid_update": 116.9e6}
 
 
 def log(*a):
