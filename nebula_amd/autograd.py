@@ -21,12 +21,20 @@ APIC; elastic, Von Mises or Drucker-Prager materials; no analytic
 colliders).  One recorded run is held per engine: the backward of a rollout
 must come before the engine records another one.  The engine keeps the
 recording's memory until the backward (or ``engine.adjointEnd()``).
+
+Import this module (or torch) before the engine library loads: the library
+then binds to torch's HIP runtime, and tensors' device pointers are valid for
+it.  Importing it after the library loaded without torch raises, because the
+process then holds two HIP runtimes (``mpm.check_single_hip_runtime``).
 """
 import itertools
 
 import torch
 
 from . import mpm
+
+if mpm._lib is not None:
+    mpm.check_single_hip_runtime()
 
 _record_ids = itertools.count(1)
 

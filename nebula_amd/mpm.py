@@ -546,8 +546,35 @@ class NebulaError(RuntimeError):
     pass
 
 
+def hip_runtimes_mapped():
+    """Distinct HIP runtime files mapped into this process (from
+    /proc/self/maps; empty where that file does not exist)."""
+    try:
+        with open("/proc/self/maps") as fh:
+            return sorted({os.path.realpath(ln.split()[-1]) for ln in fh if "libamdhip64" in ln and "/" in ln})
+    except OSError:
+        return []
+
+
+def check_single_hip_runtime():
+    """Raise when two HIP runtimes are mapped into this process.
+
+    torch's ROCm wheel links its own copy of the runtime under the name
+    libamdhip64.so, and this library asks the loader for libamdhip64.so.7.
+    The dynamic loader takes the two names for different libraries: when
+    this library is loaded before torch, importing torch maps a second HIP
+    (and HSA) runtime, and device pointers, streams and events of one are
+    meaningless to the other.  Importing torch first makes this library bind
+    to torch's runtime, which is what nebula_amd.autograd does."""
+    rts = hip_runtimes_mapped()
+    if len(rts) > 1:
+        raise NebulaError("two HIP runtimes are mapped into this process (" + ", ".join(rts) + "); torch was "
+                          "imported after the nebula_amd library loaded: import torch first")
+
+
 def lib():
-    """Load libnebula_mpm.so (raises if it was not built)."""
+    """Load libnebula_mpm.so (raises if it was not built, or if it would run
+    beside a second HIP runtime)."""
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
@@ -612,6 +639,7 @@ def lib():
         L.nb_mpm_backward.argtypes = [vp, fp, C.c_uint32]
         L.nb_mpm_adjoint_end.argtypes = [vp]
         _lib = L
+        check_single_hip_runtime()
     return _lib
 
 
@@ -646,7 +674,9 @@ class MPMEngine:
     # lifecycle -----------------------------------------------------------
     def init(self, cfg: MPMConfig, stream: Optional[int] = None):
         h = C.c_void_p()
-        _check(lib().nb_mpm_create(C.byref(cfg), C.c_void_p(stream) if stream else None, C.byref(h)))
+        L = lib()
+        check_single_hip_runtime()
+        _check(L.nb_mpm_create(C.byref(cfg), C.c_void_p(stream) if stream else None, C.byref(h)))
         self._h = h
         self._cfg = cfg
         self._n_cells = self.totalCells()
@@ -659,7 +689,9 @@ class MPMEngine:
         e = cls()
         arr = (BatchScene * len(scenes))(*scenes)
         h = C.c_void_p()
-        _check(lib().nb_mpm_create_batch(arr, len(scenes), C.c_void_p(stream) if stream else None, C.byref(h)))
+        L = lib()
+        check_single_hip_runtime()
+        _check(L.nb_mpm_create_batch(arr, len(scenes), C.c_void_p(stream) if stream else None, C.byref(h)))
         e._h = h
         e._cfg = e.config()
         e._n_cells = e.totalCells()

@@ -14,6 +14,9 @@ KATs and closed forms in test_oracle_kats.py) computes for it:
   mesh_box_sdf.npz           the mesh SDF of a box, both closest-point variants
   noise.npz                  value noise, fBm and curl at sample points
   emit.npz                   three emission calls of three emitters
+  forces_grid.npz            the grid update with gravity, turbulence and
+                             noise forces on a P2G grid (occupied nodes only),
+                             and the same update with gravity alone
 
 Run from the repository root:  python tests/golden/make_golden.py
 Every array is saved with numpy (no pickled objects)."""
@@ -145,12 +148,50 @@ def emit_case():
                 dt=np.asarray(0.01), capacity=np.asarray(1000), Vp=np.asarray(Vp), pos=pos, vel=vel)
 
 
+def forces_grid_case():
+    """A P2G grid of a moving block, then the grid update with gravity,
+    turbulence (curl of fBm) and noise (fBm along a direction), and with
+    gravity alone (so a test can see that the other two forces acted).  Only
+    nodes with mass >= 1e-10 are stored: their Morton ids, the input
+    (momentum, mass) and the updated velocities."""
+    res = 24
+    pos = _block(res, (7, 8, 7), (17, 14, 17), 4, 6)
+    n = pos.shape[0]
+    rng = np.random.default_rng(6)
+    f = _fields(pos, 0.5 * rng.standard_normal((n, 3)), (1.0 / res) ** 3 / 4)
+    f[5:8, :, :3] = 0.05 * rng.standard_normal((3, n, 3))
+    f[2:8, :, 3] = 3.0 * rng.standard_normal((6, n))
+    mats = mpm.materials_bytes([_material(1e4, 0.2, 1.0, mpm.MaterialModel.ELASTIC)])
+    sc = oracle.make_scene(2e-3, 1.0 / res, (res, res, res), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), -1.0)
+    gravity = mpm.GravityForce((0.0, 1.0, 0.0), -9.8)
+    forces = [gravity, mpm.TurbulenceForce(strength=3.0, frequency=1.5, octaves=3, seed=0.37),
+              mpm.NoiseForce((0.3, 0.8, 0.1), strength=4.0, frequency=2.0, octaves=2, seed=1.1)]
+    sim = oracle.Sim(sc, mats)
+    sim.set_state(f)
+    sim.bin()
+    sim.p2g()
+    mom, mass = sim.get_grid()
+    ids = np.nonzero(mass >= 1e-10)[0].astype(np.uint32)
+    out = {}
+    for key, fl in (("vel_forces", forces), ("vel_gravity", [gravity])):
+        s2 = oracle.Sim(sc, mats, b"", mpm.forces_bytes(fl))
+        s2.set_state(f)
+        s2.set_grid(mom[: s2.NC], mass)
+        s2.grid_update()
+        m2, _ = s2.get_grid()
+        out[key] = m2[ids]
+    recs = np.frombuffer(mpm.forces_bytes(forces), np.uint8)
+    return dict(res=np.asarray(res), dt=np.asarray(2e-3), forces=recs, node_ids=ids, mom_in=mom[ids],
+                mass_in=mass[ids], **out)
+
+
 def build_all():
     out = dict(substep_cases())
     out["bins_edge"] = bins_case()
     out["mesh_box_sdf"] = mesh_case()
     out["noise"] = noise_case()
     out["emit"] = emit_case()
+    out["forces_grid"] = forces_grid_case()
     return out
 
 
