@@ -16,7 +16,11 @@ tests/test_domain.cpp, closed forms, numpy LAPACK, brute force).  The
 degenerate SVD branches (the ``sign(0)`` Jacobi stall and the ``sigma <= 1e-8``
 unit vectors) and G2P's Jacobian reset are pinned by results worked by hand
 from the GLSL text on inputs that reach them (``degenerate_svd_cases`` and
-``jacobian_reset_state`` in tests/nbtest.py).
+``jacobian_reset_state`` in tests/nbtest.py).  The analytic collider SDFs,
+applyColliderBC and its use in the grid update, the dense-SDF node BC, G2P's
+SDF push-out, the FLIP blend, the fluid and the viscoplastic-mud stress are
+pinned by closed forms computed in numpy from the GLSL definitions
+(tests/test_oracle_bc_kats.py).
 """
 from __future__ import annotations
 
@@ -85,6 +89,10 @@ def lib():
         L.orc_sim_get_grid.argtypes = [vp, fp, fp]
         L.orc_sim_get_grid_f64.argtypes = [vp, dp, dp]
         L.orc_sim_set_grid.argtypes = [vp, fp, fp]
+        L.orc_sim_set_grid_old.argtypes = [vp, fp]
+        L.orc_collider_sdf.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_int, dp]
+        L.orc_collider_sdf.restype = C.c_double
+        L.orc_collider_bc.argtypes = [dp, dp, dp, C.c_double, C.c_double, C.c_int]
         L.orc_morton_encode.restype = C.c_uint32
         L.orc_morton_encode.argtypes = [C.c_int, C.c_int, C.c_int]
         L.orc_morton_decode.argtypes = [C.c_uint32, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
@@ -268,6 +276,12 @@ class Sim:
         ms = np.ascontiguousarray(mass, np.float32)
         lib().orc_sim_set_grid(self.h, _fptr(np.ascontiguousarray(m)), _fptr(ms))
 
+    def set_grid_old(self, old4):
+        """The saved pre-force velocities (the [NC, 2NC) half FLIP reads)."""
+        o = np.ascontiguousarray(np.asarray(old4, np.float32).reshape(-1, 4)[: self.NC])
+        assert o.shape[0] == self.NC
+        lib().orc_sim_set_grid_old(self.h, _fptr(o))
+
 
 # ---- scalar helpers (property tests) ----
 
@@ -346,6 +360,23 @@ def project_drucker_prager(F, mu, lam, M, q_c, precision=32):
     else:
         lib().orc_project_dp_f64(_dptr(f), mu, lam, M, q_c, _dptr(o))
     return _unm(o)
+
+
+def collider_sdf(record: bytes, x, precision=32):
+    """(sdf, outward normal) of one 64-byte collider record at point x."""
+    buf = C.create_string_buffer(bytes(record), 64)
+    n = np.zeros(3, np.float64)
+    d = lib().orc_collider_sdf(buf, float(x[0]), float(x[1]), float(x[2]), precision, _dptr(n))
+    return d, n
+
+
+def collider_bc(v, n, vc, restitution, friction, precision=32):
+    """applyColliderBC on one velocity; returns the new velocity."""
+    vv = np.array(v, np.float64)
+    nn = np.array(n, np.float64)
+    cc = np.array(vc, np.float64)
+    lib().orc_collider_bc(_dptr(vv), _dptr(nn), _dptr(cc), float(restitution), float(friction), precision)
+    return vv
 
 
 # procedural noise of the turbulence / noise forces (shaders/force_noise.glsl)

@@ -221,6 +221,16 @@ void orc_sim_set_grid(orc_sim* s, const float* mom4, const float* mass) {
     std::memcpy(s->s32.grid_mass.data(), mass, size_t(NC) * 4);
   }
 }
+// The saved pre-force velocities (the grid's [NC, 2NC) half, what FLIP's
+// vpic_old gathers): old4 = NC float4.
+void orc_sim_set_grid_old(orc_sim* s, const float* old4) {
+  uint32_t NC = s->sc.cube_res * s->sc.cube_res * s->sc.cube_res;
+  if (s->precision == 64) {
+    for (size_t i = 0; i < size_t(NC) * 4; i++) s->s64.grid_mom[size_t(NC) * 4 + i] = old4[i];
+  } else {
+    std::memcpy(s->s32.grid_mom.data() + size_t(NC) * 4, old4, size_t(NC) * 16);
+  }
+}
 // Set the bins directly (tests inject host-built bins, as MPMHarness::buildHashGridCPU does).
 void orc_sim_set_bins(orc_sim* s, const uint32_t* sorted, const uint32_t* cell_count, const uint32_t* cell_end) {
   uint32_t NC = s->sc.cube_res * s->sc.cube_res * s->sc.cube_res;
@@ -284,6 +294,47 @@ void orc_project_dp_f64(const double* F, double mu, double lam, double M, double
   store_m3(project_drucker_prager(load_m3(F), mu, lam, M, qc), out);
 }
 
+
+// One analytic collider (a 64-byte ColliderPrimitive record) at point x, as
+// the grid update evaluates it (shaders/collider_common.glsl:22-68,
+// mpm_grid_update.comp:84-93): returns the SDF, writes the outward normal.
+}  // extern "C"
+template <class R> static R collider_sdf_at(const Collider& c, const V3<R>& x, V3<R>& on) {
+  V3<R> cpos(c.px, c.py, c.pz), cn(c.nx, c.ny, c.nz);
+  if (c.type == 1u) return sphere_sdf(x, cpos, R(c.radius), on);
+  if (c.type == 2u) return box_sdf(x, cpos, cn, on);
+  if (c.type == 3u) return capsule_sdf(x, cpos, cn, R(c.radius), on);
+  return plane_sdf(x, cpos, cn, on);
+}
+extern "C" {
+double orc_collider_sdf(const void* rec, double x, double y, double z, int precision, double n_out[3]) {
+  Collider c;
+  std::memcpy(&c, rec, sizeof(Collider));
+  if (precision == 64) {
+    V3<double> on;
+    const double d = collider_sdf_at<double>(c, V3<double>(x, y, z), on);
+    n_out[0] = on.x; n_out[1] = on.y; n_out[2] = on.z;
+    return d;
+  }
+  V3<float> on;
+  const float d = collider_sdf_at<float>(c, V3<float>(float(x), float(y), float(z)), on);
+  n_out[0] = on.x; n_out[1] = on.y; n_out[2] = on.z;
+  return double(d);
+}
+// applyColliderBC (shaders/collider_common.glsl:73-85) on one velocity v (in/out).
+void orc_collider_bc(double v[3], const double n[3], const double vc[3], double rest, double fric, int precision) {
+  if (precision == 64) {
+    V3<double> vv(v[0], v[1], v[2]);
+    apply_collider_bc(vv, V3<double>(n[0], n[1], n[2]), V3<double>(vc[0], vc[1], vc[2]), rest, fric);
+    v[0] = vv.x; v[1] = vv.y; v[2] = vv.z;
+  } else {
+    V3<float> vv(static_cast<float>(v[0]), static_cast<float>(v[1]), static_cast<float>(v[2]));
+    apply_collider_bc(vv, V3<float>(static_cast<float>(n[0]), static_cast<float>(n[1]), static_cast<float>(n[2])),
+                      V3<float>(static_cast<float>(vc[0]), static_cast<float>(vc[1]), static_cast<float>(vc[2])),
+                      static_cast<float>(rest), static_cast<float>(fric));
+    v[0] = vv.x; v[1] = vv.y; v[2] = vv.z;
+  }
+}
 
 // procedural noise of the turbulence / noise forces (shaders/force_noise.glsl)
 double orc_noise_value(double x, double y, double z, int precision) {
