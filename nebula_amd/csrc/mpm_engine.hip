@@ -49,6 +49,19 @@ enum class GridMode { NONE, SPARSE, DENSE };
 
 }  // namespace
 
+namespace nb {
+// the thread-local message nb_last_error() returns, for the other entry-point files
+int set_error(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+}  // namespace nb
+
 struct nb_mpm {
   nb_mpm_config cfg{};
   nb_mpm_knobs knobs{};

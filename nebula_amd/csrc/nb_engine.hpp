@@ -190,4 +190,7 @@ constexpr uint32_t kPgEntries = 256;  // 4 waves x at most 64 material groups ea
 void launch_adjoint_substep(const DevState& d, const AdjState& a, const SubParams& p, uint32_t n_tiles,
                             const float4* adj_in, float4* adj_out, hipStream_t s);
 
+// Sets the thread-local message nb_last_error() returns; returns code.
+int set_error(int code, const char* fmt, ...);
+
 }  // namespace nb

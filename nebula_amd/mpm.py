@@ -25,6 +25,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libnebula_mpm.so")
 
 NB_OK = 0
+# status codes (include/nebula_mpm.h)
+NB_ERR_ARG, NB_ERR_RANGE, NB_ERR_HIP, NB_ERR_STATE, NB_ERR_UNSUPPORTED = -1, -2, -3, -4, -5
 FIELD_P, FIELD_V, FIELD_F0, FIELD_F1, FIELD_F2, FIELD_B0, FIELD_B1, FIELD_B2 = range(8)
 FIELD_NAMES = ("P", "v", "F0", "F1", "F2", "B0", "B1", "B2")
 MAX_MATERIALS, MAX_COLLIDERS, MAX_FORCES = 16, 64, 32

@@ -93,6 +93,8 @@ def lib():
         L.orc_collider_sdf.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_int, dp]
         L.orc_collider_sdf.restype = C.c_double
         L.orc_collider_bc.argtypes = [dp, dp, dp, C.c_double, C.c_double, C.c_int]
+        L.orc_hash_build.argtypes = [fp, up, C.c_uint32, fp, C.c_float, up, C.c_uint32, up, up, up]
+        L.orc_hash_build.restype = C.c_uint32
         L.orc_morton_encode.restype = C.c_uint32
         L.orc_morton_encode.argtypes = [C.c_int, C.c_int, C.c_int]
         L.orc_morton_decode.argtypes = [C.c_uint32, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
@@ -377,6 +379,25 @@ def collider_bc(v, n, vc, restitution, friction, precision=32):
     cc = np.array(vc, np.float64)
     lib().orc_collider_bc(_dptr(vv), _dptr(nn), _dptr(cc), float(restitution), float(friction), precision)
     return vv
+
+
+def hash_build(pred4, type_flag, world_min, cell_size, grid_res):
+    """The PBF / cloth spatial hash of (n, 4) float32 positions: (cell_count,
+    cell_end, sorted[:m]) with cell_end the END offsets (hash_cells = the
+    Morton cube of grid_res); particles with type_flag 6 are skipped."""
+    p = np.ascontiguousarray(pred4, np.float32).reshape(-1, 4)
+    n = p.shape[0]
+    tf = None if type_flag is None else np.ascontiguousarray(type_flag, np.uint32)
+    cube = cube_res_of(grid_res)
+    nc = cube ** 3
+    cnt = np.empty(nc, np.uint32)
+    end = np.empty(nc, np.uint32)
+    srt = np.zeros(max(n, 1), np.uint32)
+    wm = np.ascontiguousarray(world_min, np.float32)
+    gr = np.ascontiguousarray(grid_res, np.uint32)
+    m = lib().orc_hash_build(_fptr(p), _uptr(tf), n, _fptr(wm), float(cell_size), _uptr(gr), nc, _uptr(cnt),
+                             _uptr(end), _uptr(srt))
+    return cnt, end, srt[:m]
 
 
 # procedural noise of the turbulence / noise forces (shaders/force_noise.glsl)

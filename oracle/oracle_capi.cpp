@@ -336,6 +336,43 @@ void orc_collider_bc(double v[3], const double n[3], const double vc[3], double 
   }
 }
 
+// The spatial hash of the PBF and cloth engines (zero_cells, hash_count,
+// hash_scan_*, hash_add_base, hash_sort; dispatched at
+// src/engine/FluidEngine.cpp:444-474): the stable counting sort of the
+// particles whose typeFlag is not 6 (shaders/hash_count.comp:15,
+// hash_sort.comp:16) by cellId (shaders/common.glsl:100-105), END offsets,
+// ascending index within a cell.  type_flag may be null.  Returns the number
+// of particles sorted.
+uint32_t orc_hash_build(const float* pred4, const uint32_t* type_flag, uint32_t n, const float* world_min,
+                        float cell_size, const uint32_t* grid_res, uint32_t hash_cells, uint32_t* cell_count,
+                        uint32_t* cell_end, uint32_t* sorted) {
+  Params p{};
+  p.cell_size = cell_size;
+  for (int a = 0; a < 3; a++) {
+    p.grid_res[a] = grid_res[a];
+    p.world_min[a] = world_min[a];
+  }
+  std::vector<uint32_t> key(n);
+  std::fill(cell_count, cell_count + hash_cells, 0u);
+  for (uint32_t i = 0; i < n; i++) {
+    if (type_flag && type_flag[i] == 6u) continue;
+    key[i] = cell_id_from_pos<float>(p, pred4[4 * i + 0], pred4[4 * i + 1], pred4[4 * i + 2]);
+    cell_count[key[i]]++;
+  }
+  uint32_t acc = 0;
+  for (uint32_t c = 0; c < hash_cells; c++) {
+    acc += cell_count[c];
+    cell_end[c] = acc;
+  }
+  std::vector<uint32_t> wp(hash_cells);
+  for (uint32_t c = 0; c < hash_cells; c++) wp[c] = cell_end[c] - cell_count[c];
+  for (uint32_t i = 0; i < n; i++) {
+    if (type_flag && type_flag[i] == 6u) continue;
+    sorted[wp[key[i]]++] = i;
+  }
+  return acc;
+}
+
 // procedural noise of the turbulence / noise forces (shaders/force_noise.glsl)
 double orc_noise_value(double x, double y, double z, int precision) {
   if (precision == 64) return oracle::noise_value3(oracle::V3<double>(x, y, z));

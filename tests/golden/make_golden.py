@@ -14,6 +14,9 @@ KATs and closed forms in test_oracle_kats.py) computes for it:
   mesh_box_sdf.npz           the mesh SDF of a box, both closest-point variants
   noise.npz                  value noise, fBm and curl at sample points
   emit.npz                   three emission calls of three emitters
+  hash_pbf.npz               the PBF / cloth spatial hash (counts, END offsets,
+                             sorted indices) with unused slots, particles
+                             outside the domain and a crowded cell
   forces_grid.npz            the grid update with gravity, turbulence and
                              noise forces on a P2G grid (occupied nodes only),
                              and the same update with gravity alone
@@ -185,6 +188,26 @@ def forces_grid_case():
                 mass_in=mass[ids], **out)
 
 
+def hash_case():
+    """Unused slots (typeFlag 6) at the origin, as the reference's PBF engine
+    leaves them, a clump of 300 particles in one cell, and particles spread
+    over and beyond a non-cubic domain."""
+    rng = np.random.default_rng(8)
+    n = 3000
+    p = np.zeros((n, 4), np.float32)
+    tf = rng.integers(0, 6, n).astype(np.uint32)
+    tf[:500] = 6
+    p[500:800, :3] = rng.uniform(0.41, 0.43, (300, 3))
+    p[800:, :3] = rng.uniform(-0.1, 1.1, (n - 800, 3)) * np.array([1.0, 0.5, 1.0], np.float32)
+    p[:, 3] = rng.uniform(0.5, 2.0, n)
+    perm = rng.permutation(n)
+    p, tf = p[perm], tf[perm]
+    wmin, h, gr = np.array([0.0, 0.0, 0.0], np.float32), np.float32(1.0 / 24), np.array([24, 12, 24], np.uint32)
+    cnt, end, srt = oracle.hash_build(p, tf, wmin, float(h), tuple(int(x) for x in gr))
+    return dict(pred=p, type_flag=tf, world_min=wmin, cell_size=np.asarray(h), grid_res=gr, cell_count=cnt,
+                cell_end=end, sorted=srt)
+
+
 def build_all():
     out = dict(substep_cases())
     out["bins_edge"] = bins_case()
@@ -192,6 +215,7 @@ def build_all():
     out["noise"] = noise_case()
     out["emit"] = emit_case()
     out["forces_grid"] = forces_grid_case()
+    out["hash_pbf"] = hash_case()
     return out
 
 

@@ -10,7 +10,7 @@ import pytest
 from golden import make_golden
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-NAMES = ["substep_elastic_apic", "substep_dp_pic", "bins_edge", "mesh_box_sdf", "noise", "emit", "forces_grid"]
+NAMES = ["substep_elastic_apic", "substep_dp_pic", "bins_edge", "mesh_box_sdf", "noise", "emit", "forces_grid", "hash_pbf"]
 
 
 def load(name):
