@@ -58,10 +58,14 @@ def log(*a):
 
 
 def kernel_bytes(name, n, g_act):
+    """Algorithmic bytes per launch.  g2p_grid is the grid update fused into
+    G2P (every substep of a call but the last): it reads the partial sums (16 B
+    per active node) instead of the updated grid and writes no grid."""
     return {
         "p2g": 92 * n + 16 * g_act,
         "g2p": 176 * n + 16 * g_act,
         "grid_update": 28 * g_act,
+        "g2p_grid": 176 * n + 16 * g_act,
     }.get(name)
 
 
@@ -146,6 +150,43 @@ def host_transfer_leg(eng, sc, frame_substeps=20, reps=3):
             "frame_substeps": frame_substeps, "frame_ms": round(t_frame * 1e3, 3),
             "frame_particle_substeps_per_s": round(n * frame_substeps / t_frame, 1),
             "note": "host arrays are pageable numpy memory; value is measured with the state resident in HBM"}
+
+
+def svd_fallback_leg(eng, sc, k):
+    """The FP64 SVD fallback of near-degenerate F (on by default, part of
+    value): the fraction of Drucker-Prager lanes and of wavefronts that take
+    it over k substeps from the current state, and the time per substep with
+    it and without it over those same k substeps."""
+    start = eng.getState()
+    eng.setSvdFallback(True, count_stats=True)
+    eng.svdFallbackStats(reset=True)
+    eng.substep(sc.dt, k)
+    eng.sync()
+    st = eng.svdFallbackStats(reset=True)
+
+    def timed(enabled):
+        eng.setState(start)
+        eng.setSvdFallback(enabled)
+        eng.substep(sc.dt, 1)
+        eng.sync()
+        t0 = time.perf_counter()
+        eng.substep(sc.dt, k)
+        eng.sync()
+        return (time.perf_counter() - t0) / k
+
+    t_on, t_off = timed(True), timed(False)
+    eng.setSvdFallback(True)
+    eng.setState(start)
+    fb = eng.svdFallback()
+    lanes = max(st["dp_lanes"], 1)
+    res = {"gap_rel": round(fb.gap_rel, 9), "zero_rel": round(fb.zero_rel, 9), "substeps": k,
+           "return_map_lane_frac": round(st["fp64_return_map_lanes"] / lanes, 6),
+           "stress_lane_frac": round(st["fp64_stress_lanes"] / lanes, 6),
+           "wave_frac": round(st["fp64_waves"] / max(st["dp_waves"], 1), 6),
+           "ms_per_substep_on": round(t_on * 1e3, 5), "ms_per_substep_off": round(t_off * 1e3, 5),
+           "cost_frac": round(t_on / t_off - 1.0, 5)}
+    log(f"[bench] FP64 SVD fallback: {res}")
+    return res
 
 
 SWEEP_SIZES = [(128, 8), (256, 4), (256, 8), (256, 16), (512, 8), (512, 16)]
@@ -313,6 +354,8 @@ def main():
     ap.add_argument("--diff-segment", type=int, default=20, help="checkpoint spacing of that leg")
     ap.add_argument("--batch-substeps", type=int, default=20, help="substeps of the 256-scene leg (0 = skip)")
     ap.add_argument("--vm-steps", type=int, default=100, help="substeps of the Von Mises variant (0 = skip)")
+    ap.add_argument("--fallback-substeps", type=int, default=20,
+                    help="substeps of the FP64 SVD fallback's count and cost (0 = skip)")
     ap.add_argument("--host-frames", type=int, default=1, help="time host copies and whole frames with them (0 = skip)")
     ap.add_argument("--sweep-substeps", type=int, default=0,
                     help="substeps per size of the scene-size sweep, 1.05 M to 33.5 M particles (0 = skip)")
@@ -347,8 +390,10 @@ def main():
     tiles = eng.activeTileCount()
     log(f"[bench] profiling {args.profile_steps} substeps per kernel")
     prof = eng.profileSubsteps(sc.dt, args.profile_steps)
-    avg_ms = {k: v / args.profile_steps for k, v in prof.items()}
-    dom = max(("p2g", "g2p", "grid_update"), key=lambda k: avg_ms.get(k, 0.0))
+    calls = eng.profileCalls()
+    avg_ms = {k: v / max(calls.get(k, 0), 1) for k, v in prof.items() if calls.get(k, 0)}
+    per_substep_ms = {k: v / args.profile_steps for k, v in prof.items()}
+    dom = max(("p2g", "g2p", "grid_update", "g2p_grid"), key=lambda k: avg_ms.get(k, 0.0))
     dbytes = kernel_bytes(dom, n, g_act)
     achieved = dbytes / (avg_ms[dom] * 1e-3) / 1e9
     b_alg = 288 * n + 68 * g_act
@@ -361,6 +406,7 @@ def main():
     if args.calibration_copy_mb > 0:
         eng.calibrationCopy(args.calibration_copy_mb << 20, 3)
 
+    fallback = svd_fallback_leg(eng, sc, args.fallback_substeps) if args.fallback_substeps > 0 else None
     host = host_transfer_leg(eng, sc) if args.host_frames else None
     adjoint1 = adjoint_leg(sc, args.adjoint_substeps) if args.adjoint_substeps > 0 else None
     del eng
@@ -401,6 +447,9 @@ def main():
         "substep_roofline": {"bytes_per_step": b_alg, "achieved": round(substep_rate, 2), "peak": HBM_PEAK_GBS,
                              "unit": "GB/s", "frac": round(substep_rate / HBM_PEAK_GBS, 4)},
         "kernels_avg_ms": {k: round(v, 5) for k, v in avg_ms.items()},
+        "kernels_ms_per_substep": {k: round(v, 5) for k, v in per_substep_ms.items() if v > 0.0},
+        "kernel_launches": {k: v for k, v in calls.items() if v},
+        "svd_fp64_fallback": fallback,
         "host_transfer": host,
         "von_mises_variant": vm,
         "adjoint": adjoint,

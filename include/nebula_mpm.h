@@ -155,6 +155,34 @@ typedef struct nb_mpm_knobs {
 
 typedef struct nb_mpm nb_mpm; /* opaque engine handle */
 
+/* FP64 SVD fallback for near-degenerate F in the Drucker-Prager family
+ * (Drucker-Prager, granular powder, viscoplastic mud).  No counterpart in the
+ * reference, whose shaders are FP32 throughout.  G2P takes every SVD in FP32;
+ * when its singular values s0 >= s1 >= |s2| put two eigenvalues of F^T F
+ * within gap_rel * s0^2 of each other, or s2^2 <= zero_rel * s0^2, the
+ * particle's return map (SVD, projection and recomposition) and its Hencky
+ * stress are computed again in FP64, with the same algorithm, and rounded to
+ * FP32 once.  The stress SVD of a recomposed F is tested the same way.  FP32
+ * rounds F^T F by about 1e-7 s0^2, so below these thresholds the singular
+ * vectors of a close pair, and U = F v / sigma for a tiny sigma, are set by
+ * rounding; the reference convention's F <- U^T exp(eps) V and
+ * tau = U^T k U depend on them.  Particles above both thresholds take the
+ * FP32 path only and their results are unchanged by the fallback.
+ * Defaults: enabled, gap_rel 1e-4, zero_rel 1e-6, count_stats 0. */
+typedef struct nb_mpm_svd_fallback {
+  uint32_t enabled;
+  float gap_rel;
+  float zero_rel;
+  uint32_t count_stats; /* 1: G2P counts the lanes and waves that take it */
+} nb_mpm_svd_fallback;
+/* counters of nb_mpm_svd_fallback_stats */
+#define NB_SVD_FB_STATS 5
+#define NB_SVD_FB_DP_LANES 0    /* Drucker-Prager-family particles updated     */
+#define NB_SVD_FB_PROJ_LANES 1  /* of them, return maps taken in FP64          */
+#define NB_SVD_FB_STRESS_LANES 2/* of them, stresses taken in FP64             */
+#define NB_SVD_FB_FB_WAVES 3    /* wavefronts with at least one FP64 lane      */
+#define NB_SVD_FB_DP_WAVES 4    /* wavefronts with at least one such particle  */
+
 /* Thread-local description of the last failed call on this thread. */
 const char* nb_last_error(void);
 int nb_mpm_abi_version(void);
@@ -221,6 +249,17 @@ int nb_mpm_set_tuning(nb_mpm* h, const nb_mpm_tuning* t);
  * was; the choice is returned in *chosen (may be NULL). */
 #define NB_TUNE_ROUNDING 1u
 int nb_mpm_autotune(nb_mpm* h, float dt_sub, uint32_t trial_substeps, uint32_t flags, nb_mpm_tuning* chosen);
+/* nb_mpm_autotune behind a persisted cache: the file at cache_path holds one
+ * JSON object per line, {"key": "...", "tile_blocks": .., "g2p_block": ..,
+ * "p2g_item": .., "ms_per_substep": ..}.  The key names the workload's shape:
+ * the GPU (architecture and compute units), the live particle count and the
+ * occupied tile count (each to the nearest power of two), the scene count,
+ * the transfer mode, the material models present and the flags.  A cached
+ * key sets the tuning without timing anything (*hit = 1); otherwise the
+ * tuner runs and its choice is appended to the file (*hit = 0).  A missing
+ * file is an empty cache; lines that do not parse are skipped. */
+int nb_mpm_autotune_cached(nb_mpm* h, float dt_sub, uint32_t trial_substeps, uint32_t flags, const char* cache_path,
+                           nb_mpm_tuning* chosen, int* hit);
 /* n substeps as replays of one captured hipGraph of the substep's launches
  * (captured on first use, again whenever dt_sub, the particle count, the
  * material/force/collider counts, the knobs or the tuning change).  Same
@@ -252,6 +291,14 @@ int nb_mpm_set_stream(nb_mpm* h, void* hip_stream);
 
 int nb_mpm_set_knobs(nb_mpm* h, const nb_mpm_knobs* knobs);
 int nb_mpm_get_knobs(const nb_mpm* h, nb_mpm_knobs* knobs);
+/* The FP64 SVD fallback (nb_mpm_svd_fallback above); an extension with no
+ * counterpart in the reference.  gap_rel and zero_rel must lie in [0, 1). */
+int nb_mpm_set_svd_fallback(nb_mpm* h, const nb_mpm_svd_fallback* f);
+int nb_mpm_get_svd_fallback(const nb_mpm* h, nb_mpm_svd_fallback* f);
+/* The fallback's counters since the last reset (NB_SVD_FB_STATS words, see
+ * NB_SVD_FB_*), counted while count_stats is 1; reset != 0 zeroes them
+ * afterwards.  Synchronises the engine's stream. */
+int nb_mpm_svd_fallback_stats(nb_mpm* h, uint64_t* out, int reset);
 int nb_mpm_get_config(const nb_mpm* h, nb_mpm_config* cfg);
 
 /* Domain helpers — src/core/Domain.h:14-39 and MPMConfig::totalCells. */
@@ -383,6 +430,10 @@ int nb_mpm_set_grid(nb_mpm* h, const float* mom4, const float* mass, const float
  * names are written (comma separated) into names (capacity names_cap). */
 int nb_mpm_profile_substeps(nb_mpm* h, float dt_sub, uint32_t n, float* kernel_ms, uint32_t kernel_cap,
                             uint32_t* n_kernels, char* names, uint32_t names_cap);
+/* Launches of each kernel in the last nb_mpm_profile_substeps (same order as
+ * its names): a call's substeps but the last run the grid update inside G2P
+ * (g2p_grid), the last one runs grid_update and g2p. */
+int nb_mpm_profile_calls(const nb_mpm* h, uint32_t* calls, uint32_t cap);
 /* Calibration for HBM byte counters: copies `bytes` (rounded down to 16) from
  * one scratch buffer to another with a plain float4 copy kernel (k_copy),
  * `repeats` times, on the engine's stream; the buffers are allocated and freed

@@ -14,7 +14,9 @@ Layouts:
           The material id lane (state[1, :, 3]) gets no gradient.
   params  (material_count, 4) tensor: (mu, lambda, M_friction, q_cohesion)
           of each material; the other material fields come from the table
-          last given to ``engine.setMaterials``.
+          last given to ``engine.setMaterials``.  ``material_params_from``
+          builds it from Young's modulus, Poisson's ratio, friction angle and
+          cohesion, so gradients reach those through torch's chain rule.
 
 The engine must be in the adjoint's domain (corrected_svd = True; PIC or
 APIC; elastic, Von Mises or Drucker-Prager materials; no analytic
@@ -103,3 +105,18 @@ def material_params(engine: "mpm.MPMEngine", device=None) -> torch.Tensor:
     M_friction, q_cohesion), a starting point for ``params``."""
     rows = [[m.mu, m.lambda_, m.M_friction, m.q_cohesion] for m in engine.materials()]
     return torch.tensor(rows, dtype=torch.float32, device=device)
+
+
+def material_params_from(E: torch.Tensor, nu: torch.Tensor, friction_angle: torch.Tensor,
+                         cohesion: torch.Tensor) -> torch.Tensor:
+    """(n, 4) rows (mu, lambda, M_friction, q_cohesion) from per-material
+    Young's modulus E, Poisson's ratio nu, friction angle (radians) and
+    cohesion, as the reference's presets derive them: mu = E / (2 (1 + nu)),
+    lambda = E nu / ((1 + nu)(1 - 2 nu)) (src/core/MaterialParams.h:42-43) and
+    M = tan(friction angle) (:22).  Differentiable, so a rollout's gradient
+    with respect to params becomes one with respect to (E, nu, angle,
+    cohesion)."""
+    E, nu, phi, qc = torch.broadcast_tensors(E, nu, friction_angle, cohesion)
+    mu = E / (2.0 * (1.0 + nu))
+    lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu))
+    return torch.stack([mu, lam, torch.tan(phi), qc], dim=-1)

@@ -35,8 +35,8 @@ struct AxisD {
   float w[3], g[3], hh[3], dx[3];
 };
 
-__device__ __forceinline__ void axis_stencil_d(float xp, float wmin, float h, int first_node, AxisD& a) {
-  const float f = (xp - wmin) / h;
+// f = (xp - wmin) / h as cell_coord gives it
+__device__ __forceinline__ void axis_stencil_d(float f, float xp, float wmin, float h, int first_node, AxisD& a) {
   const float inv_h = 1.f / h;
 #pragma unroll
   for (int i = 0; i < 3; i++) {
@@ -304,11 +304,11 @@ __global__ __launch_bounds__(256) void k_g2p_bwd(SubParams p, const float4* __re
     AxisF ax[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-      float f = (xs[a] - sg.wmin[a]) / sg.h;
-      f = fminf(fmaxf(f, -4.f), float(sg.grid_res[a]) + 4.f);
+      const float fq = cell_coord(sg, xs[a], a);
+      const float f = fminf(fmaxf(fq, -4.f), float(sg.grid_res[a]) + 4.f);
       base[a] = int(floorf(f)) - 1;
       AxisD d;
-      axis_stencil_d(xs[a], sg.wmin[a], sg.h, base[a], d);
+      axis_stencil_d(fq, xs[a], sg.wmin[a], sg.h, base[a], d);
       axis_factors(d, ax[a]);
     }
     // grid velocity of node (ix, iy, iz); out-of-grid and massless nodes
@@ -376,7 +376,7 @@ __global__ __launch_bounds__(256) void k_g2p_bwd(SubParams p, const float4* __re
       int ic[3];
 #pragma unroll
       for (int a = 0; a < 3; a++) {
-        const float f = fminf(fmaxf((x2[a] - sg.wmin[a]) / sg.h, -4.f), float(sg.grid_res[a]) + 4.f);
+        const float f = fminf(fmaxf(cell_coord(sg, x2[a], a), -4.f), float(sg.grid_res[a]) + 4.f);
         ic[a] = min(max(int(floorf(f)), 0), gmx[a]);
       }
       const float psdf = sdf[morton3(ic[0], ic[1], ic[2])];
@@ -603,9 +603,9 @@ __global__ __launch_bounds__(kAdjThreads) void k_adj_scatter(SubParams p, const 
         }
         if (kk + 1 < kend) load(kk + 1);
         AxisD X, Y, Z;
-        axis_stencil_d(pp.x, sg.wmin[0], sg.h, cx - 1, X);
-        axis_stencil_d(pp.y, sg.wmin[1], sg.h, cy - 1, Y);
-        axis_stencil_d(pp.z, sg.wmin[2], sg.h, cz - 1, Z);
+        axis_stencil_d(cell_coord(sg, pp.x, 0), pp.x, sg.wmin[0], sg.h, cx - 1, X);
+        axis_stencil_d(cell_coord(sg, pp.y, 1), pp.y, sg.wmin[1], sg.h, cy - 1, Y);
+        axis_stencil_d(cell_coord(sg, pp.z, 2), pp.z, sg.wmin[2], sg.h, cz - 1, Z);
         // dL/dv_node = w (A + M_B d) + M_G grad w, factored per (y, z) row:
         //   = wx P + (wx dx) Q + gx R  with
         //   P = wy wz (A + dy M_B[1] + dz M_B[2]) + gy wz M_G[1] + wy gz M_G[2],
@@ -791,10 +791,11 @@ __global__ __launch_bounds__(256) void k_p2g_bwd(SubParams p, const float4* __re
 #pragma unroll
   for (int a = 0; a < 3; a++) {
     // the bin's cell (shaders/mpm_common.glsl:190-195); P2G's stencil is the cell's 27 nodes
-    const float f = fminf(fmaxf((xs[a] - sg.wmin[a]) / sg.h, 0.f), float(sg.grid_res[a]) - 1.f);
+    const float fq = cell_coord(sg, xs[a], a);
+    const float f = fminf(fmaxf(fq, 0.f), float(sg.grid_res[a]) - 1.f);
     cc[a] = int(f);
     AxisD d;
-    axis_stencil_d(xs[a], sg.wmin[a], sg.h, cc[a] - 1, d);
+    axis_stencil_d(fq, xs[a], sg.wmin[a], sg.h, cc[a] - 1, d);
     axis_factors(d, ax[a]);
   }
   const uint32_t cube = sg.cube_res;

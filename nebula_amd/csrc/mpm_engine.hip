@@ -73,6 +73,10 @@ struct nb_mpm {
   DevState d{};
   uint32_t material_count = 1, collider_count = 0, force_count = 0, noise_force_count = 0;
   std::vector<nb_material> mats_host;  // host copy of the material table
+  nb_mpm_svd_fallback svd_fb{1u, 1e-4f, 1e-6f, 0u};
+  bool fuse_g2p = true;  // the grid update inside G2P for all but a call's last substep
+  uint32_t prof_calls[kNumProfiledKernels] = {};  // launches per kernel in the last nb_mpm_profile_substeps
+  unsigned long long* d_svd_stats = nullptr;  // NB_SVD_FB_STATS counters
   bool has_sdf = false;
   bool bins_valid = false;
   // The kernels work on d.rec (one record per particle); the ABI exposes the
@@ -161,6 +165,7 @@ struct nb_mpm {
     std::memset(&p, 0, sizeof(p));  // padding too: captured substeps are matched bytewise
     p.dt = dt;
     p.h = cfg.cell_size;
+    p.h_pow2 = cell_size_is_pow2(cfg.cell_size);
     for (int a = 0; a < 3; a++) {
       p.wmin[a] = 0.f;  // worldMin = 0, worldMax = domainSize (src/engine/MPMEngine.cpp:373-374)
       p.wmax[a] = cfg.domain_size[a];
@@ -176,6 +181,10 @@ struct nb_mpm {
     p.noise_forces = noise_force_count;
     p.write_old = write_old ? 1u : 0u;
     p.corrected_svd = knobs.corrected_svd ? 1u : 0u;
+    p.svd_fp64 = svd_fb.enabled ? 1u : 0u;
+    p.svd_gap_rel = svd_fb.gap_rel;
+    p.svd_zero_rel = svd_fb.zero_rel;
+    p.svd_stats = svd_fb.count_stats ? d_svd_stats : nullptr;
     p.tile_blocks = tuning.tile_blocks;
     p.g2p_block = tuning.g2p_block;
     p.p2g_item = tuning.p2g_item;
@@ -281,11 +290,26 @@ struct TapeIO {
   const DevState* bins = nullptr;  // bin buffers to use instead of the engine's
 };
 
-int enqueue_substep(nb_mpm* h, float dt, Prof* pf, const TapeIO& io = TapeIO{}) {
+// One substep.  fused: the grid update runs inside G2P (k_g2p_tile) and the
+// grid arrays are left unwritten; only for plain substeps with no tape, and
+// the same results bit for bit.
+int enqueue_substep(nb_mpm* h, float dt, Prof* pf, const TapeIO& io = TapeIO{}, bool fused = false) {
   if (h->n_live == 0) return NB_OK;
   const bool flip = h->knobs.flip_ratio > 0.01f;
   SubParams p = h->params(dt, flip);
   DevState dv = h->dev();
+  if (fused && p.noise_forces == 0 && !io.bins && !io.rec_in && !io.keys_in && !io.grid_node && !io.grid_pre &&
+      io.g2p && !io.keys_out && !io.rec_out) {
+    launch_bin(dv, p, h->NT, h->keys_valid, h->stream, pf);
+    launch_p2g(dv, p, h->NT, h->stream, pf);
+    launch_g2p_fused(dv, p, h->NT, h->stream, pf);
+    NB_HIP(hipGetLastError());
+    h->bins_valid = true;
+    h->keys_valid = true;
+    h->grid_mode = GridMode::NONE;  // the fused substep keeps its grid in LDS
+    h->old_valid = false;
+    return NB_OK;
+  }
   if (io.bins) use_bins(dv, *io.bins);
   if (io.rec_in) dv.rec = io.rec_in;
   if (io.keys_in) dv.keys = io.keys_in;
@@ -315,7 +339,11 @@ int engine_alloc(nb_mpm* h, void* hip_stream);
 int run_substeps(nb_mpm* h, float dt, uint32_t n, Prof* pf) {
   if (n == 0 || h->n_live == 0) return NB_OK;
   int rc = sync_in(h);
-  for (uint32_t i = 0; i < n && rc == NB_OK; i++) rc = enqueue_substep(h, dt, pf);
+  // every substep but the last fuses the grid update into G2P; the last one
+  // writes the grid, so that the grid a caller reads after the call is the
+  // last substep's, as after the reference's step
+  for (uint32_t i = 0; i < n && rc == NB_OK; i++)
+    rc = enqueue_substep(h, dt, pf, TapeIO{}, h->fuse_g2p && i + 1 < n);
   if (rc == NB_OK) rc = sync_out(h);
   return rc;
 }
@@ -461,6 +489,7 @@ int nb_mpm_create_batch(const nb_mpm_scene* scenes, uint32_t n_scenes, void* hip
     g = SceneGeo{};
     g.h = sc.cell_size;
     g.inv_h = 1.0f / sc.cell_size;
+    g.h_pow2 = cell_size_is_pow2(sc.cell_size);
     for (int a = 0; a < 3; a++) {
       g.wmin[a] = 0.f;
       g.wmax[a] = sc.domain_size[a];
@@ -597,6 +626,8 @@ int engine_alloc(nb_mpm* h, void* hip_stream) {
   if (rc == NB_OK) rc = h->alloc(&d.materials, h->max_materials());
   if (rc == NB_OK) rc = h->alloc(&d.colliders, NB_MAX_COLLIDERS);
   if (rc == NB_OK) rc = h->alloc(&d.forces, NB_MAX_FORCES);
+  if (rc == NB_OK) rc = h->alloc(&h->d_svd_stats, NB_SVD_FB_STATS);
+  if (rc == NB_OK) rc = h->alloc(&d.units, size_t(h->NT) + h->cap / kG2PUnit + 1);
   if (rc != NB_OK) return rc;
   d.sdf = nullptr;
 
@@ -605,7 +636,8 @@ int engine_alloc(nb_mpm* h, void* hip_stream) {
       zero(d.tile_flag, h->NT) != hipSuccess || zero(d.counters, CNT_SLOTS * 4) != hipSuccess ||
       zero(d.grid_node, cells * 16) != hipSuccess || zero(d.grid_old, cells * 16) != hipSuccess ||
       zero(d.materials, size_t(h->max_materials()) * 64) != hipSuccess ||
-      zero(d.colliders, NB_MAX_COLLIDERS * 64) != hipSuccess || zero(d.forces, NB_MAX_FORCES * 32) != hipSuccess)
+      zero(d.colliders, NB_MAX_COLLIDERS * 64) != hipSuccess || zero(d.forces, NB_MAX_FORCES * 32) != hipSuccess ||
+      zero(h->d_svd_stats, NB_SVD_FB_STATS * 8) != hipSuccess)
     return fail(NB_ERR_HIP, "hipMemsetAsync failed");
   for (int f = 0; f < NB_NUM_FIELDS; f++)
     if (zero(d.field[f], size_t(std::max(h->cap, 1u)) * 16) != hipSuccess)
@@ -832,6 +864,130 @@ int nb_mpm_autotune(nb_mpm* h, float dt_sub, uint32_t trial_substeps, uint32_t f
   return NB_OK;
 }
 
+}  // extern "C"
+
+namespace {
+// Bins of the current state (plain engine or batch), for the occupied-tile count.
+int launch_bin_only(nb_mpm* h) {
+  SubParams p = h->params(0.f, false);
+  launch_bin(h->dev(), p, h->NT, h->keys_valid, h->stream, nullptr);
+  NB_HIP(hipGetLastError());
+  NB_HIP(hipStreamSynchronize(h->stream));
+  h->bins_valid = true;
+  return NB_OK;
+}
+
+// The cache key of nb_mpm_autotune_cached (see the header).
+int tuning_key(nb_mpm* h, uint32_t flags, std::string& key) {
+  int dev = 0;
+  hipDeviceProp_t prop{};
+  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
+    return fail(NB_ERR_HIP, "autotune cache: device properties unavailable");
+  uint32_t tiles = 0;
+  const int rc = nb_mpm_active_tile_count(h, &tiles);
+  if (rc != NB_OK) return rc;
+  auto pow2 = [](uint32_t v) {  // nearest power of two (ties up)
+    if (v == 0) return 0u;
+    uint32_t lo = 1;
+    while (lo * 2u <= v && lo < (1u << 31)) lo *= 2u;
+    return (v - lo >= lo * 2u - v && lo < (1u << 31)) ? lo * 2u : lo;
+  };
+  unsigned models = 0;
+  for (uint32_t m = 0; m < h->material_count && m < h->mats_host.size(); m++)
+    models |= 1u << std::min<uint32_t>(h->mats_host[m].model, 31u);
+  const float fr = h->knobs.flip_ratio;
+  const char* mode = fr < -0.5f ? "apic" : (fr > 0.01f ? "flip" : "pic");
+  char buf[384];
+  std::snprintf(buf, sizeof(buf), "%s|cu%d|n%u|tiles%u|scenes%u|%s|models%x|svd%u|flags%u", prop.gcnArchName,
+                prop.multiProcessorCount, pow2(h->n_live), pow2(tiles), h->n_scenes, mode, models,
+                h->knobs.corrected_svd ? 1u : 0u, flags);
+  key = buf;
+  return NB_OK;
+}
+
+// One cache line's fields; false when the line is not such an object.
+bool parse_tuning_line(const std::string& ln, std::string& key, nb_mpm_tuning& t) {
+  auto str_field = [&](const char* name, std::string& out) {
+    const std::string pat = std::string("\"") + name + "\"";
+    size_t i = ln.find(pat);
+    if (i == std::string::npos) return false;
+    i = ln.find('"', ln.find(':', i + pat.size()) + 1);
+    if (i == std::string::npos) return false;
+    const size_t j = ln.find('"', i + 1);
+    if (j == std::string::npos) return false;
+    out = ln.substr(i + 1, j - i - 1);
+    return true;
+  };
+  auto num_field = [&](const char* name, double& out) {
+    const std::string pat = std::string("\"") + name + "\"";
+    size_t i = ln.find(pat);
+    if (i == std::string::npos) return false;
+    i = ln.find(':', i + pat.size());
+    if (i == std::string::npos) return false;
+    char* end = nullptr;
+    out = std::strtod(ln.c_str() + i + 1, &end);
+    return end != ln.c_str() + i + 1;
+  };
+  double tb = 0, gb = 0, it = 0, ms = 0;
+  if (!str_field("key", key) || !num_field("tile_blocks", tb) || !num_field("g2p_block", gb) ||
+      !num_field("p2g_item", it))
+    return false;
+  if (!num_field("ms_per_substep", ms)) ms = 0.0;
+  t.tile_blocks = uint32_t(tb);
+  t.g2p_block = uint32_t(gb);
+  t.p2g_item = uint32_t(it);
+  t.ms_per_substep = float(ms);
+  return true;
+}
+}  // namespace
+
+extern "C" {
+
+int nb_mpm_autotune_cached(nb_mpm* h, float dt_sub, uint32_t trial_substeps, uint32_t flags, const char* cache_path,
+                           nb_mpm_tuning* chosen, int* hit) {
+  if (!h || !cache_path) return fail(NB_ERR_ARG, "null argument");
+  if (hit) *hit = 0;
+  if (h->n_live == 0) return fail(NB_ERR_STATE, "no particles to time");
+  int rc = sync_in(h);
+  if (rc == NB_OK && !h->bins_valid) rc = launch_bin_only(h);  // the key's occupied-tile count
+  std::string key;
+  if (rc == NB_OK) rc = tuning_key(h, flags, key);
+  if (rc != NB_OK) return rc;
+  if (FILE* f = std::fopen(cache_path, "r")) {
+    char line[1024];
+    bool found = false;
+    nb_mpm_tuning t{};
+    while (std::fgets(line, sizeof(line), f)) {
+      std::string k;
+      nb_mpm_tuning c{};
+      if (parse_tuning_line(line, k, c) && k == key) {
+        t = c;
+        found = true;  // the last entry of a key wins
+      }
+    }
+    std::fclose(f);
+    if (found) {
+      rc = nb_mpm_set_tuning(h, &t);
+      if (rc != NB_OK) return rc;
+      h->tuning.ms_per_substep = t.ms_per_substep;
+      if (chosen) *chosen = h->tuning;
+      if (hit) *hit = 1;
+      return NB_OK;
+    }
+  }
+  nb_mpm_tuning t{};
+  rc = nb_mpm_autotune(h, dt_sub, trial_substeps, flags, &t);
+  if (rc != NB_OK) return rc;
+  FILE* f = std::fopen(cache_path, "a");
+  if (!f) return fail(NB_ERR_ARG, "autotune cache: cannot append to %s", cache_path);
+  std::fprintf(f, "{\"key\": \"%s\", \"tile_blocks\": %u, \"g2p_block\": %u, \"p2g_item\": %u, "
+                  "\"ms_per_substep\": %.5f}\n",
+               key.c_str(), t.tile_blocks, t.g2p_block, t.p2g_item, double(t.ms_per_substep));
+  std::fclose(f);
+  if (chosen) *chosen = t;
+  return NB_OK;
+}
+
 int nb_mpm_set_knobs(nb_mpm* h, const nb_mpm_knobs* k) {
   if (!h || !k) return fail(NB_ERR_ARG, "null argument");
   h->knobs = *k;
@@ -841,6 +997,30 @@ int nb_mpm_get_knobs(const nb_mpm* h, nb_mpm_knobs* k) {
   if (!h || !k) return fail(NB_ERR_ARG, "null argument");
   *k = h->knobs;
   return NB_OK;
+}
+int nb_mpm_set_svd_fallback(nb_mpm* h, const nb_mpm_svd_fallback* f) {
+  if (!h || !f) return fail(NB_ERR_ARG, "null argument");
+  if (!(f->gap_rel >= 0.f && f->gap_rel < 1.f) || !(f->zero_rel >= 0.f && f->zero_rel < 1.f))
+    return fail(NB_ERR_RANGE, "svd fallback thresholds gap_rel %g, zero_rel %g: each must lie in [0, 1)",
+                double(f->gap_rel), double(f->zero_rel));
+  h->svd_fb = *f;
+  h->svd_fb.enabled = f->enabled ? 1u : 0u;
+  h->svd_fb.count_stats = f->count_stats ? 1u : 0u;
+  return NB_OK;
+}
+int nb_mpm_get_svd_fallback(const nb_mpm* h, nb_mpm_svd_fallback* f) {
+  if (!h || !f) return fail(NB_ERR_ARG, "null argument");
+  *f = h->svd_fb;
+  return NB_OK;
+}
+int nb_mpm_svd_fallback_stats(nb_mpm* h, uint64_t* out, int reset) {
+  if (!h || !out) return fail(NB_ERR_ARG, "null argument");
+  int rc = download(h, out, h->d_svd_stats, NB_SVD_FB_STATS * 8);
+  if (rc == NB_OK && reset) {
+    NB_HIP(hipMemsetAsync(h->d_svd_stats, 0, NB_SVD_FB_STATS * 8, h->stream));
+    NB_HIP(hipStreamSynchronize(h->stream));
+  }
+  return rc;
 }
 int nb_mpm_get_config(const nb_mpm* h, nb_mpm_config* c) {
   if (!h || !c) return fail(NB_ERR_ARG, "null argument");
@@ -1306,6 +1486,7 @@ int nb_mpm_profile_substeps(nb_mpm* h, float dt_sub, uint32_t n, float* kernel_m
   (void)hipStreamSynchronize(h->stream);
   for (int i = 0; i < 2 * kNumProfiledKernels; i++) (void)hipEventDestroy(pf.ev[i]);
   if (rc != NB_OK) return rc;
+  for (int i = 0; i < kNumProfiledKernels; i++) h->prof_calls[i] = pf.calls[i];
   if (n_kernels) *n_kernels = kNumProfiledKernels;
   if (kernel_ms)
     for (uint32_t i = 0; i < kernel_cap && i < uint32_t(kNumProfiledKernels); i++) kernel_ms[i] = float(pf.ms[i]);
@@ -1317,6 +1498,12 @@ int nb_mpm_profile_substeps(nb_mpm* h, float dt_sub, uint32_t n, float* kernel_m
     }
     std::snprintf(names, names_cap, "%s", s.c_str());
   }
+  return NB_OK;
+}
+
+int nb_mpm_profile_calls(const nb_mpm* h, uint32_t* calls, uint32_t cap) {
+  if (!h || !calls) return fail(NB_ERR_ARG, "null argument");
+  for (uint32_t i = 0; i < cap && i < uint32_t(kNumProfiledKernels); i++) calls[i] = h->prof_calls[i];
   return NB_OK;
 }
 

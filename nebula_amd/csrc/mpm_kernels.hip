@@ -39,7 +39,7 @@ namespace nb {
 
 const char* const kKernelNames[kNumProfiledKernels] = {
     "bin_clear", "bin_keys", "bin_mark", "bin_compact", "bin_tile_scan",
-    "bin_tile_offsets", "bin_scatter", "bin_stabilize", "p2g", "grid_update", "g2p"};
+    "bin_tile_offsets", "bin_scatter", "bin_stabilize", "p2g", "grid_update", "g2p", "g2p_grid"};
 
 // ---------------------------------------------------------------------------
 // BIN
@@ -182,22 +182,34 @@ __global__ __launch_bounds__(512) void k_bin_tile_scan(const uint32_t* __restric
   }
 }
 
-// Single block: exclusive scan of the tile totals in tile order.
+// Single block: exclusive scan of the tile totals in tile order, and the
+// list of the fused G2P's work units (kG2PUnit particles of a tile each, tile
+// by tile) with their count.  Particles (< 2^32) and units are scanned
+// together as the low and high halves of one 64-bit sum.
 __global__ __launch_bounds__(1024) void k_bin_tile_offsets(const uint32_t* __restrict__ tile_total,
                                                            uint32_t* __restrict__ tile_start,
-                                                           const uint32_t* __restrict__ counters) {
-  __shared__ uint32_t s_wave[1024 / 64 + 1];
+                                                           uint32_t* __restrict__ counters,
+                                                           uint2* __restrict__ units) {
+  __shared__ uint64_t s_wave[1024 / 64 + 1];
   const uint32_t na = counters[CNT_ACTIVE];
   const uint32_t chunk = (na + 1023u) / 1024u;
   const uint32_t lo = min(na, threadIdx.x * chunk), hi = min(na, lo + chunk);
-  uint32_t s = 0;
-  for (uint32_t a = lo; a < hi; a++) s += tile_total[a];
-  uint32_t tot;
-  uint32_t o = block_exclusive_scan<1024>(s, s_wave, tot);
+  auto units_of = [](uint32_t t) { return (t + kG2PUnit - 1u) / kG2PUnit; };
+  uint64_t s = 0;
   for (uint32_t a = lo; a < hi; a++) {
-    tile_start[a] = o;
-    o += tile_total[a];
+    const uint32_t t = tile_total[a];
+    s += (uint64_t(units_of(t)) << 32) + t;
   }
+  uint64_t tot;
+  const uint64_t pre = block_exclusive_scan<1024>(s, s_wave, tot);
+  uint32_t o = uint32_t(pre), uo = uint32_t(pre >> 32);
+  for (uint32_t a = lo; a < hi; a++) {
+    const uint32_t t = tile_total[a];
+    tile_start[a] = o;
+    o += t;
+    for (uint32_t c = 0, nu = units_of(t); c < nu; c++) units[uo++] = make_uint2(a, c);
+  }
+  if (threadIdx.x == 0) counters[CNT_UNITS] = uint32_t(tot >> 32);
 }
 
 // Place every particle at tile start + cell start + its rank.
@@ -614,6 +626,99 @@ __global__ __launch_bounds__(kP2GThreads, 2) void k_p2g(SubParams p, const float
 // ---------------------------------------------------------------------------
 // GRID UPDATE
 // ---------------------------------------------------------------------------
+// Mass and momentum of node (local (lx, ly, lz) in tile (tx, ty, tz) of scene
+// g): the partial boxes of the occupied tiles whose 10^3 box contains it,
+// summed in a fixed order.  False when no occupied tile's box contains it
+// (then it lies in no particle's stencil).
+__device__ __forceinline__ bool sum_partials(const SceneGeo& g, int tx, int ty, int tz, int lx, int ly, int lz,
+                                             const int32_t* __restrict__ tile_act,
+                                             const float4* __restrict__ partial, float& mx, float& my, float& mz,
+                                             float& mass) {
+  const int td = int(g.tile_dim);
+  mx = my = mz = mass = 0.f;
+  bool covered = false;
+  for (int dz = -1; dz <= 1; dz++) {
+    if ((dz < 0 && lz != 0) || (dz > 0 && lz != 7)) continue;
+    const int sz = tz + dz;
+    if (sz < 0 || sz >= td) continue;
+    for (int dy = -1; dy <= 1; dy++) {
+      if ((dy < 0 && ly != 0) || (dy > 0 && ly != 7)) continue;
+      const int sy = ty + dy;
+      if (sy < 0 || sy >= td) continue;
+      for (int dx = -1; dx <= 1; dx++) {
+        if ((dx < 0 && lx != 0) || (dx > 0 && lx != 7)) continue;
+        const int sx = tx + dx;
+        if (sx < 0 || sx >= td) continue;
+        const int32_t a = tile_act[tile_id(g, sx, sy, sz)];
+        if (a < 0) continue;
+        const int bx = lx - 8 * dx + 1, by = ly - 8 * dy + 1, bz = lz - 8 * dz + 1;
+        const float4 q = partial[size_t(a) * kBoxNodes + (bz * kBox + by) * kBox + bx];
+        mx += q.x; my += q.y; mz += q.z; mass += q.w;
+        covered = true;
+      }
+    }
+  }
+  return covered;
+}
+
+// The update of a node with mass >= 1e-10 at grid coordinates (nx, ny, nz),
+// in registers and in this order: the 15 m/s cap, the saved (pre-force)
+// velocity, forces, walls, analytic colliders, the dense-SDF BC
+// (shaders/mpm_grid_update.comp:30-110, shaders/mpm_nanovdb_bc.comp:23-54).
+// v enters as momentum / mass; o receives the saved velocity.
+template <bool NOISE>
+__device__ __forceinline__ void node_update(const SubParams& p, const SceneGeo& g, int nx, int ny, int nz,
+                                            float& vx, float& vy, float& vz, float& ox, float& oy, float& oz,
+                                            const float* __restrict__ sdf, const nb_collider* __restrict__ cols,
+                                            const nb_force* __restrict__ forces) {
+  {
+    const float sp = len3(vx, vy, vz);
+    const float vmax = 15.f;
+    if (sp > vmax) {
+      const float k = vmax / sp;
+      vx *= k; vy *= k; vz *= k;
+    }
+  }
+  ox = vx; oy = vy; oz = vz;
+  const float px = (float(nx) + 0.5f) * g.h + g.wmin[0];
+  const float py = (float(ny) + 0.5f) * g.h + g.wmin[1];
+  const float pz = (float(nz) + 0.5f) * g.h + g.wmin[2];
+  apply_forces<NOISE>(vx, vy, vz, px, py, pz, forces, p.force_count, p.dt);
+  // separating walls on the two outer node layers (shaders/mpm_grid_update.comp:54-60)
+  const int Nx = g.grid_res[0] - 1, Ny = g.grid_res[1] - 1, Nz = g.grid_res[2] - 1;
+  if (nx <= 1 && vx < 0.f) vx = 0.f;
+  if (nx >= Nx - 1 && vx > 0.f) vx = 0.f;
+  if (ny <= 1 && vy < 0.f) vy = 0.f;
+  if (ny >= Ny - 1 && vy > 0.f) vy = 0.f;
+  if (nz <= 1 && vz < 0.f) vz = 0.f;
+  if (nz >= Nz - 1 && vz > 0.f) vz = 0.f;
+  for (uint32_t ci = 0; ci < p.collider_count; ci++) {
+    const nb_collider c = cols[ci];
+    float cnx, cny, cnz;
+    const float sd = collider_sdf(c, px, py, pz, cnx, cny, cnz);
+    if (sd <= 0.f) collider_bc(vx, vy, vz, cnx, cny, cnz, c);
+  }
+  // dense-SDF BC (shaders/mpm_nanovdb_bc.comp:23-54)
+  if (sdf) {
+    const float sd = sample_sdf(sdf, g, nx, ny, nz);
+    if (sd < 0.f) {
+      vx = vy = vz = 0.f;
+    } else if (sd < g.h * 1.5f) {
+      const float gx = sample_sdf(sdf, g, nx + 1, ny, nz) - sample_sdf(sdf, g, nx - 1, ny, nz);
+      const float gy = sample_sdf(sdf, g, nx, ny + 1, nz) - sample_sdf(sdf, g, nx, ny - 1, nz);
+      const float gz = sample_sdf(sdf, g, nx, ny, nz + 1) - sample_sdf(sdf, g, nx, ny, nz - 1);
+      const float gl = len3(gx, gy, gz);
+      if (gl > 1e-6f) {
+        const float ux = gx / gl, uy = gy / gl, uz = gz / gl;
+        const float vn = vx * ux + vy * uy + vz * uz;
+        if (vn < 0.f) {
+          vx -= vn * ux; vy -= vn * uy; vz -= vn * uz;
+        }
+      }
+    }
+  }
+}
+
 template <bool FROM_PARTIALS, bool UPDATE, bool BATCH, bool NOISE>
 __global__ __launch_bounds__(512) void k_grid(SubParams p, const float4* __restrict__ partial,
                                               const int32_t* __restrict__ tile_act, const uint32_t* __restrict__ tiles,
@@ -628,43 +733,18 @@ __global__ __launch_bounds__(512) void k_grid(SubParams p, const float4* __restr
   for (uint32_t b = blockIdx.x; b < nb_tiles; b += gridDim.x) {
     const uint32_t N = tiles[b];
     const SceneGeo g = tile_geo<BATCH>(p, N);
-    const int td = int(g.tile_dim);
     int tx, ty, tz;
     tile_coords(g, N, tx, ty, tz);
     const int nx = tx * 8 + lx, ny = ty * 8 + ly, nz = tz * 8 + lz;
     const size_t nid = (size_t(N) << kTileBits) | threadIdx.x;
     float mx, my, mz, mass;
     if (FROM_PARTIALS) {
-      mx = my = mz = mass = 0.f;
-      bool covered = false;
-      // the tiles whose 10^3 box contains this node, in a fixed order
-      for (int dz = -1; dz <= 1; dz++) {
-        if ((dz < 0 && lz != 0) || (dz > 0 && lz != 7)) continue;
-        const int sz = tz + dz;
-        if (sz < 0 || sz >= td) continue;
-        for (int dy = -1; dy <= 1; dy++) {
-          if ((dy < 0 && ly != 0) || (dy > 0 && ly != 7)) continue;
-          const int sy = ty + dy;
-          if (sy < 0 || sy >= td) continue;
-          for (int dx = -1; dx <= 1; dx++) {
-            if ((dx < 0 && lx != 0) || (dx > 0 && lx != 7)) continue;
-            const int sx = tx + dx;
-            if (sx < 0 || sx >= td) continue;
-            const int32_t a = tile_act[tile_id(g, sx, sy, sz)];
-            if (a < 0) continue;
-            const int bx = lx - 8 * dx + 1, by = ly - 8 * dy + 1, bz = lz - 8 * dz + 1;
-            const float4 q = partial[size_t(a) * kBoxNodes + (bz * kBox + by) * kBox + bx];
-            mx += q.x; my += q.y; mz += q.z; mass += q.w;
-            covered = true;
-          }
-        }
-      }
       // A node outside every occupied tile's box receives nothing and lies in
       // no particle's stencil: it is not written (readers treat it as empty).
-      if (!covered) continue;
+      if (!sum_partials(g, tx, ty, tz, lx, ly, lz, tile_act, partial, mx, my, mz, mass)) continue;
     } else {
-      const float4 g = grid_node[nid];
-      mx = g.x; my = g.y; mz = g.z; mass = g.w;
+      const float4 g4 = grid_node[nid];
+      mx = g4.x; my = g4.y; mz = g4.z; mass = g4.w;
     }
     if (grid_pre)
       grid_pre[nid] = mass < 1e-10f ? make_float4(0.f, 0.f, 0.f, mass) : make_float4(mx / mass, my / mass, mz / mass, mass);
@@ -673,53 +753,9 @@ __global__ __launch_bounds__(512) void k_grid(SubParams p, const float4* __restr
       grid_node[nid] = make_float4(mx, my, mz, mass);
       continue;
     }
-    float vx = mx / mass, vy = my / mass, vz = mz / mass;
-    {
-      const float sp = len3(vx, vy, vz);
-      const float vmax = 15.f;
-      if (sp > vmax) {
-        const float k = vmax / sp;
-        vx *= k; vy *= k; vz *= k;
-      }
-    }
-    if (p.write_old) grid_old[nid] = make_float4(vx, vy, vz, 0.f);
-    const float px = (float(nx) + 0.5f) * g.h + g.wmin[0];
-    const float py = (float(ny) + 0.5f) * g.h + g.wmin[1];
-    const float pz = (float(nz) + 0.5f) * g.h + g.wmin[2];
-    apply_forces<NOISE>(vx, vy, vz, px, py, pz, forces, p.force_count, p.dt);
-    // separating walls on the two outer node layers (shaders/mpm_grid_update.comp:54-60)
-    const int Nx = g.grid_res[0] - 1, Ny = g.grid_res[1] - 1, Nz = g.grid_res[2] - 1;
-    if (nx <= 1 && vx < 0.f) vx = 0.f;
-    if (nx >= Nx - 1 && vx > 0.f) vx = 0.f;
-    if (ny <= 1 && vy < 0.f) vy = 0.f;
-    if (ny >= Ny - 1 && vy > 0.f) vy = 0.f;
-    if (nz <= 1 && vz < 0.f) vz = 0.f;
-    if (nz >= Nz - 1 && vz > 0.f) vz = 0.f;
-    for (uint32_t ci = 0; ci < p.collider_count; ci++) {
-      const nb_collider c = cols[ci];
-      float cnx, cny, cnz;
-      const float sd = collider_sdf(c, px, py, pz, cnx, cny, cnz);
-      if (sd <= 0.f) collider_bc(vx, vy, vz, cnx, cny, cnz, c);
-    }
-    // dense-SDF BC (shaders/mpm_nanovdb_bc.comp:23-54)
-    if (sdf) {
-      const float sd = sample_sdf(sdf, g, nx, ny, nz);
-      if (sd < 0.f) {
-        vx = vy = vz = 0.f;
-      } else if (sd < g.h * 1.5f) {
-        const float gx = sample_sdf(sdf, g, nx + 1, ny, nz) - sample_sdf(sdf, g, nx - 1, ny, nz);
-        const float gy = sample_sdf(sdf, g, nx, ny + 1, nz) - sample_sdf(sdf, g, nx, ny - 1, nz);
-        const float gz = sample_sdf(sdf, g, nx, ny, nz + 1) - sample_sdf(sdf, g, nx, ny, nz - 1);
-        const float gl = len3(gx, gy, gz);
-        if (gl > 1e-6f) {
-          const float ux = gx / gl, uy = gy / gl, uz = gz / gl;
-          const float vn = vx * ux + vy * uy + vz * uz;
-          if (vn < 0.f) {
-            vx -= vn * ux; vy -= vn * uy; vz -= vn * uz;
-          }
-        }
-      }
-    }
+    float vx = mx / mass, vy = my / mass, vz = mz / mass, ox, oy, oz;
+    node_update<NOISE>(p, g, nx, ny, nz, vx, vy, vz, ox, oy, oz, sdf, cols, forces);
+    if (p.write_old) grid_old[nid] = make_float4(ox, oy, oz, 0.f);
     grid_node[nid] = make_float4(vx, vy, vz, mass);
   }
 }
@@ -727,84 +763,34 @@ __global__ __launch_bounds__(512) void k_grid(SubParams p, const float4* __restr
 // ---------------------------------------------------------------------------
 // G2P
 // ---------------------------------------------------------------------------
-// rec_out may equal rec_in: every record slot written is data-dependent on the
-// load of the same slot (or not read at all), so updating in place is safe.
-// 4 waves per SIMD (<= 128 VGPRs): the gather's latency needs the occupancy
-template <bool FLIP, bool BATCH>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_g2p(SubParams p,
-                                                                                   const float4* rec_in,
-                                                                                   float4* rec_out,
-                                             uint32_t* __restrict__ keys,
-                                             const nb_material* __restrict__ mats,
-                                             const uint32_t* __restrict__ sorted,
-                                             const float4* __restrict__ grid_node,
-                                             const float4* __restrict__ grid_old, const float* __restrict__ sdf) {
-  // a plain engine's material table is staged in LDS; a batch's (up to
-  // kBatchMaxMaterials) is read through the caches
-  __shared__ nb_material s_mats[BATCH ? 1 : NB_MAX_MATERIALS];
-  if constexpr (!BATCH) {
-    float* dst = reinterpret_cast<float*>(s_mats);
-    const float* src = reinterpret_cast<const float*>(mats);
-    const uint32_t nf = p.material_count * uint32_t(sizeof(nb_material) / 4);
-    for (uint32_t e = threadIdx.x; e < nf; e += blockDim.x) dst[e] = src[e];
-    __syncthreads();
-  }
-  const uint32_t k = xcd_chunk(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (k >= p.n) return;
-  const uint32_t q = sorted ? sorted[k] : k;
-  const SceneGeo g = particle_geo<BATCH>(p, q);
-  const float4* r = rec_in + size_t(q) * kRec;
-  float4* ro = rec_out + size_t(q) * kRec;
-  const float4 pp = r[R_P];
-  const float xp = pp.x, yp = pp.y, zp = pp.z, Vp = pp.w;
-  constexpr bool flip = FLIP;  // flip_ratio > 0.01
+// What the 27-node gather gives a particle (shaders/mpm_g2p.comp:80-113): the
+// PIC velocity, FLIP's saved-velocity interpolant, the velocity gradient
+// vg.m[col = gradient axis][row = component] and APIC's B.
+struct G2PGather {
+  float vpx, vpy, vpz, vox, voy, voz;
+  m3 vg, Bn;
+};
 
-  // stencil base = floor((x - wmin)/h) - 1; the quotient is clamped to a range
-  // that keeps every integer below in bounds without changing which nodes are valid
-  int base[3];
-  Axis3 ax3[3];
-  const float xs[3] = {xp, yp, zp};
-  const float inv_h = g.inv_h;
-#pragma unroll
-  for (int a = 0; a < 3; a++) {
-    float f = (xs[a] - g.wmin[a]) / g.h;
-    f = fminf(fmaxf(f, -4.f), float(g.grid_res[a]) + 4.f);
-    base[a] = int(floorf(f)) - 1;
-    axis_stencil(xs[a], g.wmin[a], g.h, inv_h, base[a], ax3[a]);
-  }
-  // Node (i,j,k) takes part when it lies in the grid and carries mass >= 1e-10
-  // and w = (wx*wy)*wz >= 1e-12 (shaders/mpm_g2p.comp:93-101).  Out-of-grid and
-  // massless nodes are folded in as zero velocity.  If none of the particle's
-  // nodes has 0 < w < 1e-12 (the smallest positive per-axis weights bound every
-  // w from below), the weight cut removes nothing and the axis-by-axis sums
-  // stand; otherwise the particle's nodes are visited again with the cut.
-  const bool tiny = (min_positive3(ax3[0].w) * min_positive3(ax3[1].w)) * min_positive3(ax3[2].w) < 1e-12f;
-  // out-of-grid nodes read a clamped (valid) address; the caller drops them
-  // together with the massless ones
-  auto node_in = [&](int ix, int iy, int iz) -> bool {
-    const int nx = base[0] + ix, ny = base[1] + iy, nz = base[2] + iz;
-    return nx >= 0 && nx < g.grid_res[0] && ny >= 0 && ny < g.grid_res[1] && nz >= 0 && nz < g.grid_res[2];
-  };
-  auto node_vel = [&](const float4* __restrict__ grid, int ix, int iy, int iz) -> float4 {
-    const uint32_t cx = uint32_t(min(max(base[0] + ix, 0), g.grid_res[0] - 1));
-    const uint32_t cy = uint32_t(min(max(base[1] + iy, 0), g.grid_res[1] - 1));
-    const uint32_t cz = uint32_t(min(max(base[2] + iz, 0), g.grid_res[2] - 1));
-    return grid[node_id(g, cx, cy, cz)];
-  };
-
+// The gather.  node(ix, iy, iz, v, vo) returns whether the stencil node
+// (base + (ix, iy, iz)) takes part (in the grid, mass >= 1e-10) and its
+// velocity in v (and FLIP's saved velocity in vo), both zero when it does not.
+// The 27 nodes are contracted axis by axis (x, then y, then z) in three rolled
+// z slabs, so only one slab's node loads are live.  When some node of the
+// particle has 0 < w < 1e-12 (tiny), the reference's weight cut
+// (shaders/mpm_g2p.comp:93-101) changes the sums, and they are taken node by
+// node with the cut instead; each lane decides alone, so each particle's
+// result depends on its own data only.
+template <bool FLIP, class NodeFn>
+__device__ __forceinline__ void g2p_gather(const Axis3 (&ax3)[3], bool tiny, NodeFn node, G2PGather& o) {
   float vpx = 0.f, vpy = 0.f, vpz = 0.f, vox = 0.f, voy = 0.f, voz = 0.f;
   m3 vg = m3_zero(), Bn = m3_zero();
-  // One z slab of 9 nodes per iteration (the loop is kept rolled so only one
-  // slab's loads are live); the slab's z stencil values sit in position 0 of
-  // zw/zg/zd/zwd and are rotated at the end of the iteration.
   const Axis3 &X = ax3[0], &Y = ax3[1];
+  // the slab's z stencil values sit in position 0 and rotate at the end of
+  // each iteration
   float zw[3] = {ax3[2].w[0], ax3[2].w[1], ax3[2].w[2]};
   float zg[3] = {ax3[2].g[0], ax3[2].g[1], ax3[2].g[2]};
   float zd[3] = {ax3[2].dx[0], ax3[2].dx[1], ax3[2].dx[2]};
-  {
-    // axis-by-axis contraction: x, then y, then z (every lane; a lane with a
-    // weight under the cut redoes its sums node by node below, so that each
-    // particle's result depends on its own data only)
+  if (!tiny) {
     float wdx[3], wdy[3];
 #pragma unroll
     for (int i = 0; i < 3; i++) {
@@ -819,26 +805,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
       for (int iy = 0; iy < 3; iy++) {
         float gv[3][3], go[3][3];
 #pragma unroll
-        for (int ix = 0; ix < 3; ix++) {
-          const float4 g = node_vel(grid_node, ix, iy, iz);
-          const bool live = node_in(ix, iy, iz) && g.w >= 1e-10f;
-          gv[ix][0] = live ? g.x : 0.f;
-          gv[ix][1] = live ? g.y : 0.f;
-          gv[ix][2] = live ? g.z : 0.f;
-          if (flip) {
-            const float4 o = node_vel(grid_old, ix, iy, iz);
-            go[ix][0] = live ? o.x : 0.f;
-            go[ix][1] = live ? o.y : 0.f;
-            go[ix][2] = live ? o.z : 0.f;
-          }
-        }
+        for (int ix = 0; ix < 3; ix++) (void)node(ix, iy, iz, gv[ix], go[ix]);
         float X0[3], X1[3], X2[3], XO[3];
 #pragma unroll
         for (int c = 0; c < 3; c++) {
           X0[c] = X.w[0] * gv[0][c] + X.w[1] * gv[1][c] + X.w[2] * gv[2][c];
           X1[c] = X.g[0] * gv[0][c] + X.g[1] * gv[1][c] + X.g[2] * gv[2][c];
           X2[c] = wdx[0] * gv[0][c] + wdx[1] * gv[1][c] + wdx[2] * gv[2][c];
-          if (flip) XO[c] = X.w[0] * go[0][c] + X.w[1] * go[1][c] + X.w[2] * go[2][c];
+          if (FLIP) XO[c] = X.w[0] * go[0][c] + X.w[1] * go[1][c] + X.w[2] * go[2][c];
         }
 #pragma unroll
         for (int c = 0; c < 3; c++) {
@@ -848,14 +822,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             Y02[c] = wdy[0] * X0[c];
             Y1[c] = Y.w[0] * X1[c];
             Y2[c] = Y.w[0] * X2[c];
-            if (flip) YO[c] = Y.w[0] * XO[c];
+            if (FLIP) YO[c] = Y.w[0] * XO[c];
           } else {
             Y00[c] += Y.w[iy] * X0[c];
             Y01[c] += Y.g[iy] * X0[c];
             Y02[c] += wdy[iy] * X0[c];
             Y1[c] += Y.w[iy] * X1[c];
             Y2[c] += Y.w[iy] * X2[c];
-            if (flip) YO[c] += Y.w[iy] * XO[c];
+            if (FLIP) YO[c] += Y.w[iy] * XO[c];
           }
         }
       }
@@ -865,7 +839,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 #pragma unroll
       for (int c = 0; c < 3; c++) {
         *vp3[c] += wz * Y00[c];
-        if (flip) *vo3[c] += wz * YO[c];
+        if (FLIP) *vo3[c] += wz * YO[c];
         // vg = sum outer(v_i, grad w_i): vg.m[col = grad axis][row = velocity component]
         vg.m[0][c] += wz * Y1[c];
         vg.m[1][c] += wz * Y01[c];
@@ -879,17 +853,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
       zg[0] = zg[1]; zg[1] = zg[2];
       zd[0] = zd[1]; zd[1] = zd[2];
     }
-  }
-  if (tiny) {
-    vpx = vpy = vpz = vox = voy = voz = 0.f;
-    vg = m3_zero();
-    Bn = m3_zero();
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-      zw[i] = ax3[2].w[i];
-      zg[i] = ax3[2].g[i];
-      zd[i] = ax3[2].dx[i];
-    }
+  } else {
 #pragma unroll 1
     for (int iz = 0; iz < 3; iz++) {
       const float wz = zw[0], gz = zg[0], dxz = zd[0];
@@ -897,30 +861,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
       for (int iy = 0; iy < 3; iy++) {
 #pragma unroll
         for (int ix = 0; ix < 3; ix++) {
-          float4 g = node_vel(grid_node, ix, iy, iz);
+          float gvv[3], gov[3];
+          const bool in = node(ix, iy, iz, gvv, gov);
           const float wx = X.w[ix], wy = Y.w[iy];
           const float w0 = wx * wy * wz;
-          const bool live = node_in(ix, iy, iz) && g.w >= 1e-10f && w0 >= 1e-12f;
-          g.x = live ? g.x : 0.f;
-          g.y = live ? g.y : 0.f;
-          g.z = live ? g.z : 0.f;
+          const bool live = in && w0 >= 1e-12f;
+          const float gx = live ? gvv[0] : 0.f, gy = live ? gvv[1] : 0.f, gzv = live ? gvv[2] : 0.f;
           const float w = live ? w0 : 0.f;
           const float dwx = live ? X.g[ix] * wy * wz : 0.f;
           const float dwy = live ? wx * Y.g[iy] * wz : 0.f;
           const float dwz = live ? wx * wy * gz : 0.f;
           const float dxx = X.dx[ix], dxy = Y.dx[iy];
-          vpx += w * g.x; vpy += w * g.y; vpz += w * g.z;
+          vpx += w * gx; vpy += w * gy; vpz += w * gzv;
           // outerProduct(vi, dw)[j][i] = vi[i]*dw[j]
-          vg.m[0][0] += g.x * dwx; vg.m[0][1] += g.y * dwx; vg.m[0][2] += g.z * dwx;
-          vg.m[1][0] += g.x * dwy; vg.m[1][1] += g.y * dwy; vg.m[1][2] += g.z * dwy;
-          vg.m[2][0] += g.x * dwz; vg.m[2][1] += g.y * dwz; vg.m[2][2] += g.z * dwz;
-          const float wgx = w * g.x, wgy = w * g.y, wgz = w * g.z;
+          vg.m[0][0] += gx * dwx; vg.m[0][1] += gy * dwx; vg.m[0][2] += gzv * dwx;
+          vg.m[1][0] += gx * dwy; vg.m[1][1] += gy * dwy; vg.m[1][2] += gzv * dwy;
+          vg.m[2][0] += gx * dwz; vg.m[2][1] += gy * dwz; vg.m[2][2] += gzv * dwz;
+          const float wgx = w * gx, wgy = w * gy, wgz = w * gzv;
           Bn.m[0][0] += wgx * dxx; Bn.m[0][1] += wgy * dxx; Bn.m[0][2] += wgz * dxx;
           Bn.m[1][0] += wgx * dxy; Bn.m[1][1] += wgy * dxy; Bn.m[1][2] += wgz * dxy;
           Bn.m[2][0] += wgx * dxz; Bn.m[2][1] += wgy * dxz; Bn.m[2][2] += wgz * dxz;
-          if (flip) {
-            const float4 o = node_vel(grid_old, ix, iy, iz);
-            vox += w * o.x; voy += w * o.y; voz += w * o.z;
+          if (FLIP) {
+            vox += w * (live ? gov[0] : 0.f);
+            voy += w * (live ? gov[1] : 0.f);
+            voz += w * (live ? gov[2] : 0.f);
           }
         }
       }
@@ -929,12 +893,68 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
       zd[0] = zd[1]; zd[1] = zd[2];
     }
   }
+  o.vpx = vpx; o.vpy = vpy; o.vpz = vpz;
+  o.vox = vox; o.voy = voy; o.voz = voz;
+  o.vg = vg;
+  o.Bn = Bn;
+}
 
-  const float4 vel = r[R_V];
-  uint32_t mid = __float_as_uint(vel.w);
-  mid = mid > p.material_count - 1u ? p.material_count - 1u : mid;
-  const nb_material mat = BATCH ? mats[mid] : s_mats[mid];
+// Stencil of particle position x: per axis the first stencil node base =
+// floor((x - wmin)/h) - 1 (the quotient clamped to a range that keeps every
+// integer below in bounds without changing which nodes are valid) and the
+// weights; tiny: some node has 0 < w < 1e-12 (the smallest positive
+// per-axis weights bound every w from below).
+__device__ __forceinline__ bool g2p_stencil(const SceneGeo& g, float xp, float yp, float zp, int (&base)[3],
+                                            Axis3 (&ax3)[3]) {
+  const float xs[3] = {xp, yp, zp};
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    const float fq = cell_coord(g, xs[a], a);
+    const float f = fminf(fmaxf(fq, -4.f), float(g.grid_res[a]) + 4.f);
+    base[a] = int(floorf(f)) - 1;
+    axis_stencil_f(fq, xs[a], g.wmin[a], g.h, g.inv_h, base[a], ax3[a]);
+  }
+  return (min_positive3(ax3[0].w) * min_positive3(ax3[1].w)) * min_positive3(ax3[2].w) < 1e-12f;
+}
 
+// The FP64 fallback's two computations, kept out of line: a near-degenerate
+// F is rare, and inlined FP64 code would raise the register count of every
+// G2P lane.  The return map (SVD, projection, recomposition) and the Hencky
+// stress of its result, from an FP32 F, rounded once; and the Hencky stress
+// alone.  Results come back by value, so the caller's F and tau stay in
+// registers.
+struct FTau {
+  m3 F, tau;
+};
+__device__ __noinline__ FTau plastic_fp64(const m3 F, bool dp, float mu_f, float lam_f, float q_max, float M,
+                                          float q_c, bool cs) {
+  const double mu = mu_f, lam = lam_f;
+  const PlasticOutT<double> pd =
+      project_plastic(m3_convert<double>(F), dp, mu, lam, double(q_max), double(M), double(q_c), cs);
+  FTau r;
+  r.F = m3_round(pd.F);
+  r.tau = m3_round((cs || !pd.recomposed) ? hencky_from_log(pd.U, pd.eps[0], pd.eps[1], pd.eps[2], mu, lam)
+                                          : hencky_stress(pd.F, mu, lam, cs));
+  return r;
+}
+__device__ __noinline__ m3 hencky_fp64(const m3 F, float mu, float lam, bool cs) {
+  return m3_round(hencky_stress(m3_convert<double>(F), double(mu), double(lam), cs));
+}
+
+// Everything of G2P after the gather (shaders/mpm_g2p.comp:115-272): F
+// update, constitutive update (fluid; elastic; Von Mises / Drucker-Prager
+// family return map with the FP64 fallback for near-degenerate F), mud
+// viscosity, Jacobian reset, PIC/APIC/FLIP velocity, advection, SDF push-out,
+// world clamp; writes the record and the next substep's cell key.
+template <bool FLIP, bool BATCH>
+__device__ __forceinline__ void g2p_particle(const SubParams& p, uint32_t q, const float4* r, float4* ro,
+                                             uint32_t* __restrict__ keys, const nb_material& mat,
+                                             const G2PGather& gth, const float4 pp, const float4 vel,
+                                             const float* __restrict__ sdf) {
+  const float xp = pp.x, yp = pp.y, zp = pp.z, Vp = pp.w;
+  constexpr bool flip = FLIP;  // flip_ratio > 0.01
+  float vpx = gth.vpx, vpy = gth.vpy, vpz = gth.vpz;
+  m3 Bn = gth.Bn;
   const float4 rt = r[R_T], fa = r[R_FA], fb = r[R_FB];
   m3 F;
   F.m[0][0] = rt.w; F.m[0][1] = fa.x; F.m[0][2] = fa.y;
@@ -944,11 +964,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 #pragma unroll
   for (int c = 0; c < 3; c++)
 #pragma unroll
-    for (int r = 0; r < 3; r++) L.m[c][r] = (c == r ? 1.f : 0.f) + p.dt * vg.m[c][r];
+    for (int rr = 0; rr < 3; rr++) L.m[c][rr] = (c == rr ? 1.f : 0.f) + p.dt * gth.vg.m[c][rr];
   m3 Fn = m3_mul(L, F);
 
   m3 tau;
   const bool cs = p.corrected_svd != 0u;
+  bool dp_lane = false, fb_proj = false, fb_stress = false;  // for the fallback's counters
   if (mat.model == NB_MAT_FLUID) {
     float J = fmaxf(m3_det(Fn), 1e-4f);
     const float cr = powf(J, 1.f / 3.f);
@@ -965,28 +986,54 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const bool dp = mat.model == NB_MAT_DRUCKER_PRAGER || mat.model == NB_MAT_GRANULAR_POWDER ||
                     mat.model == NB_MAT_VISCOPLASTIC_MUD;
     const bool plastic = dp || mat.model == NB_MAT_VON_MISES;
+    const bool fb_on = dp && p.svd_fp64 != 0u;
+    dp_lane = dp;
     bool have_tau = false;
     if (plastic) {
       const PlasticOut po = project_plastic(Fn, dp, mat.mu, mat.lambda, mat.q_max, mat.M_friction, mat.q_cohesion, cs);
-      Fn = po.F;
-      if (cs || !po.recomposed) {
-        tau = hencky_from_log(po.U, po.eps[0], po.eps[1], po.eps[2], mat.mu, mat.lambda);
+      if (fb_on && svd_near_degenerate(po.sig, p.svd_gap_rel, p.svd_zero_rel)) {
+        // near-degenerate trial F: the return map and the stress in FP64
+        // from the same FP32 input, rounded once
+        const FTau r = plastic_fp64(Fn, dp, mat.mu, mat.lambda, mat.q_max, mat.M_friction, mat.q_cohesion, cs);
+        Fn = r.F;
+        tau = r.tau;
         have_tau = true;
+        fb_proj = true;
+      } else {
+        Fn = po.F;
+        if (cs || !po.recomposed) {
+          tau = hencky_from_log(po.U, po.eps[0], po.eps[1], po.eps[2], mat.mu, mat.lambda);
+          have_tau = true;
+        }
       }
     }
-    if (!have_tau) tau = hencky_stress(Fn, mat.mu, mat.lambda, cs);
+    if (!have_tau) {
+      // Hencky stress of F (hencky_stress written out, so that a particle that
+      // does not take the fallback runs the same instructions whether it is on
+      // or off); the stress SVD of a recomposed Drucker-Prager F is tested on
+      // its own singular values
+      m3 U, V;
+      float sig[3];
+      svd3(Fn, U, sig, V, cs);
+      if (fb_on && svd_near_degenerate(sig, p.svd_gap_rel, p.svd_zero_rel)) {
+        tau = hencky_fp64(Fn, mat.mu, mat.lambda, cs);
+        fb_stress = true;
+      } else {
+        tau = hencky_from_log(U, log_strain(sig[0]), log_strain(sig[1]), log_strain(sig[2]), mat.mu, mat.lambda);
+      }
+    }
     if (mat.model == NB_MAT_VISCOPLASTIC_MUD) {
       m3 D;
 #pragma unroll
       for (int c = 0; c < 3; c++)
 #pragma unroll
-        for (int r = 0; r < 3; r++) D.m[c][r] = 0.5f * (vg.m[c][r] + vg.m[r][c]);
+        for (int rr = 0; rr < 3; rr++) D.m[c][rr] = 0.5f * (gth.vg.m[c][rr] + gth.vg.m[rr][c]);
       const float trD = D.m[0][0] + D.m[1][1] + D.m[2][2];
       const float k2 = 2.f * mat.viscosity;
 #pragma unroll
       for (int c = 0; c < 3; c++)
 #pragma unroll
-        for (int r = 0; r < 3; r++) tau.m[c][r] += k2 * (D.m[c][r] - (c == r ? trD / 3.f : 0.f));
+        for (int rr = 0; rr < 3; rr++) tau.m[c][rr] += k2 * (D.m[c][rr] - (c == rr ? trD / 3.f : 0.f));
     }
   }
 
@@ -1003,11 +1050,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 
   float vnx, vny, vnz;
   if (flip) {
-    const float r = p.flip_ratio;
-    const float fx = vel.x + (vpx - vox), fy = vel.y + (vpy - voy), fz = vel.z + (vpz - voz);
-    vnx = vpx * (1.f - r) + fx * r;
-    vny = vpy * (1.f - r) + fy * r;
-    vnz = vpz * (1.f - r) + fz * r;
+    const float rf = p.flip_ratio;
+    const float fx = vel.x + (vpx - gth.vox), fy = vel.y + (vpy - gth.voy), fz = vel.z + (vpz - gth.voz);
+    vnx = vpx * (1.f - rf) + fx * rf;
+    vny = vpy * (1.f - rf) + fy * rf;
+    vnz = vpz * (1.f - rf) + fz * rf;
   } else {
     vnx = vpx; vny = vpy; vnz = vpz;
   }
@@ -1020,9 +1067,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   // particle-level SDF push-out (shaders/mpm_g2p.comp:220-252)
   if (sdf) {
     const int gx = ge.grid_res[0] - 1, gy = ge.grid_res[1] - 1, gz = ge.grid_res[2] - 1;
-    float fx = fminf(fmaxf((xn - ge.wmin[0]) / ge.h, -4.f), float(ge.grid_res[0]) + 4.f);
-    float fy = fminf(fmaxf((yn - ge.wmin[1]) / ge.h, -4.f), float(ge.grid_res[1]) + 4.f);
-    float fz = fminf(fmaxf((zn - ge.wmin[2]) / ge.h, -4.f), float(ge.grid_res[2]) + 4.f);
+    float fx = fminf(fmaxf(cell_coord(ge, xn, 0), -4.f), float(ge.grid_res[0]) + 4.f);
+    float fy = fminf(fmaxf(cell_coord(ge, yn, 1), -4.f), float(ge.grid_res[1]) + 4.f);
+    float fz = fminf(fmaxf(cell_coord(ge, zn, 2), -4.f), float(ge.grid_res[2]) + 4.f);
     const int ix = min(max(int(floorf(fx)), 0), gx);
     const int iy = min(max(int(floorf(fy)), 0), gy);
     const int iz = min(max(int(floorf(fz)), 0), gz);
@@ -1066,6 +1113,197 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   ro[R_FA] = make_float4(Fn.m[0][1], Fn.m[0][2], Fn.m[1][0], Fn.m[1][1]);
   ro[R_FB] = make_float4(Fn.m[1][2], Fn.m[2][0], Fn.m[2][1], Fn.m[2][2]);
   keys[q] = cell_key(ge, xn, yn, zn);  // the next substep's bin key
+  if (p.svd_stats) {
+    const uint64_t b_dp = __ballot(dp_lane), b_pr = __ballot(fb_proj), b_st = __ballot(fb_stress);
+    if ((threadIdx.x & 63u) == uint32_t(__ffsll((unsigned long long)__ballot(true)) - 1)) {
+      if (b_dp) {
+        atomicAdd(&p.svd_stats[NB_SVD_FB_DP_LANES], (unsigned long long)__popcll(b_dp));
+        atomicAdd(&p.svd_stats[NB_SVD_FB_DP_WAVES], 1ull);
+      }
+      if (b_pr) atomicAdd(&p.svd_stats[NB_SVD_FB_PROJ_LANES], (unsigned long long)__popcll(b_pr));
+      if (b_st) atomicAdd(&p.svd_stats[NB_SVD_FB_STRESS_LANES], (unsigned long long)__popcll(b_st));
+      if (b_pr | b_st) atomicAdd(&p.svd_stats[NB_SVD_FB_FB_WAVES], 1ull);
+    }
+  }
+}
+
+// the material of a particle: clamp to materialCount-1 (shaders/mpm_g2p.comp:128-130)
+__device__ __forceinline__ uint32_t particle_material(const SubParams& p, const float4 vel) {
+  const uint32_t mid = __float_as_uint(vel.w);
+  return mid > p.material_count - 1u ? p.material_count - 1u : mid;
+}
+
+// G2P reading the updated grid from HBM, one thread per particle in bin order.
+// rec_out may equal rec_in: every record slot written is data-dependent on the
+// load of the same slot (or not read at all), so updating in place is safe.
+// 4 waves per SIMD (<= 128 VGPRs): the gather's latency needs the occupancy
+template <bool FLIP, bool BATCH>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_g2p(SubParams p,
+                                                                                   const float4* rec_in,
+                                                                                   float4* rec_out,
+                                             uint32_t* __restrict__ keys,
+                                             const nb_material* __restrict__ mats,
+                                             const uint32_t* __restrict__ sorted,
+                                             const float4* __restrict__ grid_node,
+                                             const float4* __restrict__ grid_old, const float* __restrict__ sdf) {
+  // a plain engine's material table is staged in LDS; a batch's (up to
+  // kBatchMaxMaterials) is read through the caches
+  __shared__ nb_material s_mats[BATCH ? 1 : NB_MAX_MATERIALS];
+  if constexpr (!BATCH) {
+    float* dst = reinterpret_cast<float*>(s_mats);
+    const float* src = reinterpret_cast<const float*>(mats);
+    const uint32_t nf = p.material_count * uint32_t(sizeof(nb_material) / 4);
+    for (uint32_t e = threadIdx.x; e < nf; e += blockDim.x) dst[e] = src[e];
+    __syncthreads();
+  }
+  const uint32_t k = xcd_chunk(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (k >= p.n) return;
+  const uint32_t q = sorted ? sorted[k] : k;
+  const SceneGeo g = particle_geo<BATCH>(p, q);
+  const float4* r = rec_in + size_t(q) * kRec;
+  float4* ro = rec_out + size_t(q) * kRec;
+  const float4 pp = r[R_P];
+  int base[3];
+  Axis3 ax3[3];
+  const bool tiny = g2p_stencil(g, pp.x, pp.y, pp.z, base, ax3);
+  // out-of-grid nodes read a clamped (valid) address and are dropped together
+  // with the massless ones
+  auto node = [&](int ix, int iy, int iz, float (&v)[3], float (&vo)[3]) -> bool {
+    const int nx = base[0] + ix, ny = base[1] + iy, nz = base[2] + iz;
+    const uint32_t cx = uint32_t(min(max(nx, 0), g.grid_res[0] - 1));
+    const uint32_t cy = uint32_t(min(max(ny, 0), g.grid_res[1] - 1));
+    const uint32_t cz = uint32_t(min(max(nz, 0), g.grid_res[2] - 1));
+    const uint32_t id = node_id(g, cx, cy, cz);
+    const float4 gn = grid_node[id];
+    const bool live = nx >= 0 && nx < g.grid_res[0] && ny >= 0 && ny < g.grid_res[1] && nz >= 0 &&
+                      nz < g.grid_res[2] && gn.w >= 1e-10f;
+    v[0] = live ? gn.x : 0.f;
+    v[1] = live ? gn.y : 0.f;
+    v[2] = live ? gn.z : 0.f;
+    if (FLIP) {
+      const float4 o = grid_old[id];
+      vo[0] = live ? o.x : 0.f;
+      vo[1] = live ? o.y : 0.f;
+      vo[2] = live ? o.z : 0.f;
+    }
+    return live;
+  };
+  G2PGather gth;
+  g2p_gather<FLIP>(ax3, tiny, node, gth);
+  const float4 vel = r[R_V];
+  const uint32_t mid = particle_material(p, vel);
+  const nb_material mat = BATCH ? mats[mid] : s_mats[mid];
+  g2p_particle<FLIP, BATCH>(p, q, r, ro, keys, mat, gth, pp, vel, sdf);
+}
+
+// Grid update and G2P fused, one block per work unit: a unit is up to
+// kG2PUnit particles of one occupied tile (a tile's particles are one
+// contiguous range of the bin order; k_bin_tile_offsets lists the units).
+// The block builds the tile's 10^3 node box in LDS from the partials (the
+// same sums and node update as k_grid, bit for bit), stores each node's
+// G2P view (velocity, or zero for a node that does not take part, and FLIP's
+// saved velocity), then runs G2P for the unit's particles gathering from
+// LDS.  A particle whose stencil leaves the box (its position lies outside
+// the grid) reads the box with a range check: the nodes outside it are
+// outside the grid too.  Results equal k_grid + k_g2p bit for bit; the
+// grid itself is not written (the last substep of a call runs unfused, so the
+// exported grid stays that of the last substep).
+template <bool FLIP, bool BATCH>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_g2p_tile(
+    SubParams p, const float4* rec_in, float4* rec_out, uint32_t* __restrict__ keys,
+    const nb_material* __restrict__ mats, const uint32_t* __restrict__ sorted, const uint2* __restrict__ units,
+    const uint32_t* __restrict__ act_tiles, const uint32_t* __restrict__ tile_start,
+    const uint32_t* __restrict__ tile_total, const uint32_t* __restrict__ counters,
+    const int32_t* __restrict__ tile_act, const float4* __restrict__ partial, const float* __restrict__ sdf,
+    const nb_collider* __restrict__ cols, const nb_force* __restrict__ forces) {
+  __shared__ float4 s_v[kBoxNodes];                 // xyz: velocity (0 when not taking part), w: 1 / 0
+  __shared__ float4 s_o[FLIP ? kBoxNodes : 1];      // FLIP's saved velocity
+  __shared__ nb_material s_mats[BATCH ? 1 : NB_MAX_MATERIALS];
+  if constexpr (!BATCH) {
+    float* dst = reinterpret_cast<float*>(s_mats);
+    const float* src = reinterpret_cast<const float*>(mats);
+    const uint32_t nf = p.material_count * uint32_t(sizeof(nb_material) / 4);
+    for (uint32_t e = threadIdx.x; e < nf; e += blockDim.x) dst[e] = src[e];
+  }
+  const uint32_t n_units = counters[CNT_UNITS];
+  for (uint32_t u = blockIdx.x; u < n_units; u += gridDim.x) {
+    const uint2 unit = units[u];
+    const uint32_t a = unit.x, chunk = unit.y;
+    const uint32_t T = act_tiles[a];
+    const SceneGeo g = tile_geo<BATCH>(p, T);
+    int tx, ty, tz;
+    tile_coords(g, T, tx, ty, tz);
+    __syncthreads();  // the previous unit's particles are done with the box
+    for (uint32_t e = threadIdx.x; e < kBoxNodes; e += blockDim.x) {
+      const int bx = int(e % kBox), by = int((e / kBox) % kBox), bz = int(e / (kBox * kBox));
+      const int nx = tx * 8 - 1 + bx, ny = ty * 8 - 1 + by, nz = tz * 8 - 1 + bz;
+      float4 v = make_float4(0.f, 0.f, 0.f, 0.f), o = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (nx >= 0 && nx < g.grid_res[0] && ny >= 0 && ny < g.grid_res[1] && nz >= 0 && nz < g.grid_res[2]) {
+        // the node's own tile and its place in it, as k_grid sees the node
+        const int ntx = nx >> 3, nty = ny >> 3, ntz = nz >> 3;
+        float mx, my, mz, mass;
+        (void)sum_partials(g, ntx, nty, ntz, nx & 7, ny & 7, nz & 7, tile_act, partial, mx, my, mz, mass);
+        if (mass >= 1e-10f) {
+          float vx = mx / mass, vy = my / mass, vz = mz / mass, ox, oy, oz;
+          node_update<false>(p, g, nx, ny, nz, vx, vy, vz, ox, oy, oz, sdf, cols, forces);
+          v = make_float4(vx, vy, vz, 1.f);
+          o = make_float4(ox, oy, oz, 0.f);
+        }
+      }
+      s_v[e] = v;
+      if (FLIP) s_o[e] = o;
+    }
+    __syncthreads();
+    const uint32_t first = tile_start[a] + chunk * kG2PUnit;
+    const uint32_t last = min(tile_start[a] + tile_total[a], first + kG2PUnit);
+    // box node of grid node n along each axis: n - (8 t - 1)
+    const int ox0 = tx * 8 - 1, oy0 = ty * 8 - 1, oz0 = tz * 8 - 1;
+    for (uint32_t k = first + threadIdx.x; k < last; k += blockDim.x) {
+      const uint32_t q = sorted[k];
+      const float4* r = rec_in + size_t(q) * kRec;
+      float4* ro = rec_out + size_t(q) * kRec;
+      const float4 pp = r[R_P];
+      int base[3];
+      Axis3 ax3[3];
+      const bool tiny = g2p_stencil(g, pp.x, pp.y, pp.z, base, ax3);
+      const int bx0 = base[0] - ox0, by0 = base[1] - oy0, bz0 = base[2] - oz0;
+      const bool inbox = bx0 >= 0 && bx0 <= kBox - 3 && by0 >= 0 && by0 <= kBox - 3 && bz0 >= 0 && bz0 <= kBox - 3;
+      const int e0 = (bz0 * kBox + by0) * kBox + bx0;
+      G2PGather gth;
+      if (inbox) {
+        auto node = [&](int ix, int iy, int iz, float (&v)[3], float (&vo)[3]) -> bool {
+          const int e = e0 + (iz * kBox + iy) * kBox + ix;
+          const float4 n4 = s_v[e];
+          v[0] = n4.x; v[1] = n4.y; v[2] = n4.z;
+          if (FLIP) {
+            const float4 o4 = s_o[e];
+            vo[0] = o4.x; vo[1] = o4.y; vo[2] = o4.z;
+          }
+          return n4.w != 0.f;
+        };
+        g2p_gather<FLIP>(ax3, tiny, node, gth);
+      } else {
+        auto node = [&](int ix, int iy, int iz, float (&v)[3], float (&vo)[3]) -> bool {
+          const int bx = bx0 + ix, by = by0 + iy, bz = bz0 + iz;
+          const bool in = bx >= 0 && bx < kBox && by >= 0 && by < kBox && bz >= 0 && bz < kBox;
+          const int e = in ? (bz * kBox + by) * kBox + bx : 0;
+          const float4 n4 = s_v[e];
+          const bool live = in && n4.w != 0.f;
+          v[0] = live ? n4.x : 0.f; v[1] = live ? n4.y : 0.f; v[2] = live ? n4.z : 0.f;
+          if (FLIP) {
+            const float4 o4 = s_o[e];
+            vo[0] = live ? o4.x : 0.f; vo[1] = live ? o4.y : 0.f; vo[2] = live ? o4.z : 0.f;
+          }
+          return live;
+        };
+        g2p_gather<FLIP>(ax3, tiny, node, gth);
+      }
+      const float4 vel = r[R_V];
+      const uint32_t mid = particle_material(p, vel);
+      const nb_material mat = BATCH ? mats[mid] : s_mats[mid];
+      g2p_particle<FLIP, BATCH>(p, q, r, ro, keys, mat, gth, pp, vel, sdf);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1193,7 +1431,7 @@ void launch_bin(const DevState& d, const SubParams& p, uint32_t n_tiles, bool ke
                                                      d.node_tiles, d.counters, n_tiles);
   }
   { ProfScope ps(pf, 4, s); k_bin_tile_scan<<<tg, 512, 0, s>>>(d.cell_count, d.cell_end, d.act_tiles, d.tile_total, d.counters); }
-  { ProfScope ps(pf, 5, s); k_bin_tile_offsets<<<1, 1024, 0, s>>>(d.tile_total, d.tile_start, d.counters); }
+  { ProfScope ps(pf, 5, s); k_bin_tile_offsets<<<1, 1024, 0, s>>>(d.tile_total, d.tile_start, d.counters, d.units); }
   { ProfScope ps(pf, 6, s); k_bin_scatter<<<cdiv(p.n, 256), 256, 0, s>>>(p.n, d.keys, d.rank, d.cell_end, d.tile_act, d.tile_start, d.sorted); }
   { ProfScope ps(pf, 7, s); k_bin_stabilize<<<tg, 512, 0, s>>>(p.n, d.cell_count, d.cell_end, d.act_tiles, d.tile_start, d.sorted, d.counters); }
 }
@@ -1236,6 +1474,27 @@ void launch_grid(const DevState& d, const SubParams& p, uint32_t n_tiles, bool f
     }
   }
 #undef NB_GRID_LAUNCH
+}
+
+// The fused grid update + G2P (k_g2p_tile) over the work units the bin listed
+// (not with turbulence or noise forces: their noise code would halve the
+// kernel's occupancy; those substeps run k_grid + k_g2p).
+void launch_g2p_fused(const DevState& d, const SubParams& p, uint32_t n_tiles, hipStream_t s, Prof* pf) {
+  ProfScope ps(pf, 11, s);
+  // units <= occupied tiles + n / kG2PUnit; blocks stride over them
+  const uint32_t max_units = n_tiles + p.n / kG2PUnit + 1u;
+  const uint32_t g = tile_grid(max_units, p.tile_blocks);
+  const bool flip = p.flip_ratio > 0.01f, bt = p.n_scenes > 0;
+#define NB_G2PT(FL, BT)                                                                                    \
+  k_g2p_tile<FL, BT><<<g, 256, 0, s>>>(p, d.rec, d.rec, d.keys, d.materials, d.sorted, d.units, d.act_tiles, \
+                                       d.tile_start, d.tile_total, d.counters, d.tile_act, d.partial, d.sdf, \
+                                       d.colliders, d.forces)
+  if (bt) {
+    if (flip) NB_G2PT(true, true); else NB_G2PT(false, true);
+  } else {
+    if (flip) NB_G2PT(true, false); else NB_G2PT(false, false);
+  }
+#undef NB_G2PT
 }
 
 void launch_g2p(const DevState& d, const SubParams& p, bool use_sorted, hipStream_t s, Prof* pf, float4* rec_out) {

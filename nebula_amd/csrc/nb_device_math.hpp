@@ -14,23 +14,32 @@
 
 namespace nb {
 
-struct m3 {
-  float m[3][3];  // m[col][row]
+// column-major 3x3, m[col][row] as in GLSL; float for the substep, double
+// for the FP64 SVD fallback of near-degenerate F
+template <typename T>
+struct mat3 {
+  T m[3][3];
 };
+using m3 = mat3<float>;
+using m3d = mat3<double>;
 
-__device__ __forceinline__ m3 m3_zero() {
-  m3 r;
+template <typename T>
+__device__ __forceinline__ mat3<T> mat3_zero() {
+  mat3<T> r;
 #pragma unroll
   for (int c = 0; c < 3; c++)
 #pragma unroll
-    for (int k = 0; k < 3; k++) r.m[c][k] = 0.f;
+    for (int k = 0; k < 3; k++) r.m[c][k] = T(0);
   return r;
 }
-__device__ __forceinline__ m3 m3_ident() {
-  m3 r = m3_zero();
-  r.m[0][0] = r.m[1][1] = r.m[2][2] = 1.f;
+template <typename T>
+__device__ __forceinline__ mat3<T> mat3_ident() {
+  mat3<T> r = mat3_zero<T>();
+  r.m[0][0] = r.m[1][1] = r.m[2][2] = T(1);
   return r;
 }
+__device__ __forceinline__ m3 m3_zero() { return mat3_zero<float>(); }
+__device__ __forceinline__ m3 m3_ident() { return mat3_ident<float>(); }
 __device__ __forceinline__ m3 m3_diag(float a, float b, float c) {
   m3 r = m3_zero();
   r.m[0][0] = a;
@@ -39,38 +48,64 @@ __device__ __forceinline__ m3 m3_diag(float a, float b, float c) {
   return r;
 }
 // (A*B)[col][row] = sum_k A[k][row] * B[col][k]
-__device__ __forceinline__ m3 m3_mul(const m3& A, const m3& B) {
-  m3 r;
+template <typename T>
+__device__ __forceinline__ mat3<T> m3_mul(const mat3<T>& A, const mat3<T>& B) {
+  mat3<T> r;
 #pragma unroll
   for (int c = 0; c < 3; c++)
 #pragma unroll
     for (int k = 0; k < 3; k++) r.m[c][k] = A.m[0][k] * B.m[c][0] + A.m[1][k] * B.m[c][1] + A.m[2][k] * B.m[c][2];
   return r;
 }
-__device__ __forceinline__ m3 m3_transpose(const m3& A) {
-  m3 r;
+template <typename T>
+__device__ __forceinline__ mat3<T> m3_transpose(const mat3<T>& A) {
+  mat3<T> r;
 #pragma unroll
   for (int c = 0; c < 3; c++)
 #pragma unroll
     for (int k = 0; k < 3; k++) r.m[c][k] = A.m[k][c];
   return r;
 }
-__device__ __forceinline__ float m3_det(const m3& m) {
+template <typename T>
+__device__ __forceinline__ T m3_det(const mat3<T>& m) {
   return m.m[0][0] * (m.m[1][1] * m.m[2][2] - m.m[2][1] * m.m[1][2]) -
          m.m[1][0] * (m.m[0][1] * m.m[2][2] - m.m[2][1] * m.m[0][2]) +
          m.m[2][0] * (m.m[0][1] * m.m[1][2] - m.m[1][1] * m.m[0][2]);
 }
 // U * diag(d) * V^T
-__device__ __forceinline__ m3 m3_udvt(const m3& U, float d0, float d1, float d2, const m3& V) {
-  m3 r;
+template <typename T>
+__device__ __forceinline__ mat3<T> m3_udvt(const mat3<T>& U, T d0, T d1, T d2, const mat3<T>& V) {
+  mat3<T> r;
 #pragma unroll
   for (int c = 0; c < 3; c++)
 #pragma unroll
     for (int k = 0; k < 3; k++) r.m[c][k] = U.m[0][k] * d0 * V.m[0][c] + U.m[1][k] * d1 * V.m[1][c] + U.m[2][k] * d2 * V.m[2][c];
   return r;
 }
+template <typename T>
+__device__ __forceinline__ mat3<T> m3_convert(const mat3<float>& A) {
+  mat3<T> r;
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int k = 0; k < 3; k++) r.m[c][k] = T(A.m[c][k]);
+  return r;
+}
+__device__ __forceinline__ m3 m3_round(const m3d& A) {
+  m3 r;
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int k = 0; k < 3; k++) r.m[c][k] = float(A.m[c][k]);
+  return r;
+}
 
-__device__ __forceinline__ float glsl_sign(float x) { return x > 0.f ? 1.f : (x < 0.f ? -1.f : 0.f); }
+template <typename T>
+__device__ __forceinline__ T glsl_sign(T x) { return x > T(0) ? T(1) : (x < T(0) ? T(-1) : T(0)); }
+__device__ __forceinline__ float tmax(float a, float b) { return fmaxf(a, b); }
+__device__ __forceinline__ double tmax(double a, double b) { return fmax(a, b); }
+__device__ __forceinline__ float tabs(float a) { return fabsf(a); }
+__device__ __forceinline__ double tabs(double a) { return fabs(a); }
 
 // ---- Morton keys (shaders/mpm_common.glsl:161-187) ----
 __host__ __device__ __forceinline__ uint32_t morton_expand(uint32_t v) {
@@ -104,47 +139,63 @@ __device__ __forceinline__ float hw_sqrt(float x) { return __builtin_amdgcn_sqrt
 __device__ __forceinline__ float hw_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ float hw_log(float x) { return __builtin_amdgcn_logf(x) * 0.693147180559945309f; }
 __device__ __forceinline__ float hw_exp(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504088896341f); }
+// the FP64 fallback uses the correctly rounded double operations (the FP64
+// oracle's std::sqrt, division, log and exp)
+__device__ __forceinline__ double hw_rcp(double x) { return 1.0 / x; }
+__device__ __forceinline__ double hw_sqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ double hw_rsq(double x) { return 1.0 / sqrt(x); }
+__device__ __forceinline__ double hw_log(double x) { return log(x); }
+__device__ __forceinline__ double hw_exp(double x) { return exp(x); }
+
+// a / b: float multiplies by the hardware reciprocal, double divides (IEEE)
+__device__ __forceinline__ float tdiv(float a, float b) { return a * hw_rcp(b); }
+__device__ __forceinline__ double tdiv(double a, double b) { return a / b; }
 
 // ---- Jacobi eigen-decomposition of symmetric A (shaders/mpm_common.glsl:213-247) ----
+// Two forms of the same iteration.  jacobi_eigen3_select keeps the matrix in
+// its physical layout and resolves the three pivot cases with selects;
+// jacobi_eigen3 (below) keeps it relative to the last pivot and is the one the
+// kernels use.  Both take the same pivots in the same order and differ only in
+// the rounding of FMA groupings.
 // Same pivot rule, rotation and stopping test as the GLSL.  The matrix is held
 // as its six independent entries and the three pivot cases are resolved with
 // selects, so every lane runs one instruction stream and nothing is indexed
 // dynamically (no scratch).  V keeps the GLSL indexing V[col][row]: the GLSL
 // rotates V[r][p0], V[r][p1], i.e. here V.m[r][p0], V.m[r][p1].
-__device__ __forceinline__ void jacobi_eigen3(float a00, float a11, float a22, float a01, float a02, float a12,
-                                              float D[3], m3& V) {
-  V = m3_ident();
+template <typename T>
+__device__ __forceinline__ void jacobi_eigen3_select(T a00, T a11, T a22, T a01, T a02, T a12, T D[3], mat3<T>& V) {
+  V = mat3_ident<T>();
   for (int iter = 0; iter < 20; iter++) {
-    float b01 = fabsf(a01), b02 = fabsf(a02), b12 = fabsf(a12);
-    if (fmaxf(b01, fmaxf(b02, b12)) < 1e-8f) break;
+    T b01 = tabs(a01), b02 = tabs(a02), b12 = tabs(a12);
+    if (tmax(b01, tmax(b02, b12)) < T(1e-8)) break;
     // pivot case: 0 -> (0,1), 1 -> (0,2), 2 -> (1,2)
-    int cs = (b01 >= fmaxf(b02, b12)) ? 0 : ((b02 >= b12) ? 1 : 2);
-    float app = (cs == 2) ? a11 : a00;
-    float aqq = (cs == 0) ? a11 : a22;
-    float apq = (cs == 0) ? a01 : ((cs == 1) ? a02 : a12);
-    float arp = (cs == 0) ? a02 : a01;  // A[r][p0]
-    float arq = (cs == 2) ? a02 : a12;  // A[r][p1]
+    int cs = (b01 >= tmax(b02, b12)) ? 0 : ((b02 >= b12) ? 1 : 2);
+    T app = (cs == 2) ? a11 : a00;
+    T aqq = (cs == 0) ? a11 : a22;
+    T apq = (cs == 0) ? a01 : ((cs == 1) ? a02 : a12);
+    T arp = (cs == 0) ? a02 : a01;  // A[r][p0]
+    T arq = (cs == 2) ? a02 : a12;  // A[r][p1]
     // tau = (aqq - app) / (2 apq); t = sign(tau) / (|tau| + sqrt(1 + tau^2));
     // c = 1 / sqrt(1 + t^2); s = t c
-    float tau = (aqq - app) * hw_rcp(2.f * apq);
-    float t = glsl_sign(tau) * hw_rcp(fabsf(tau) + hw_sqrt(1.f + tau * tau));
-    float c = hw_rsq(1.f + t * t);
-    float s = t * c;
-    float napp = app - t * apq, naqq = aqq + t * apq;
-    float nrp = c * arp - s * arq;
-    float nrq = s * arp + c * arq;
+    T tau = tdiv(aqq - app, T(2) * apq);
+    T t = tdiv(glsl_sign(tau), tabs(tau) + hw_sqrt(T(1) + tau * tau));
+    T c = hw_rsq(T(1) + t * t);
+    T s = t * c;
+    T napp = app - t * apq, naqq = aqq + t * apq;
+    T nrp = c * arp - s * arq;
+    T nrq = s * arp + c * arq;
     a00 = (cs == 2) ? a00 : napp;
     a11 = (cs == 0) ? naqq : ((cs == 2) ? napp : a11);
     a22 = (cs == 0) ? a22 : naqq;
-    a01 = (cs == 0) ? 0.f : nrp;
-    a02 = (cs == 0) ? nrp : ((cs == 1) ? 0.f : nrq);
-    a12 = (cs == 2) ? 0.f : nrq;
+    a01 = (cs == 0) ? T(0) : nrp;
+    a02 = (cs == 0) ? nrp : ((cs == 1) ? T(0) : nrq);
+    a12 = (cs == 2) ? T(0) : nrq;
 #pragma unroll
     for (int rr = 0; rr < 3; rr++) {
-      float vp = (cs == 2) ? V.m[rr][1] : V.m[rr][0];
-      float vq = (cs == 0) ? V.m[rr][1] : V.m[rr][2];
-      float nvp = c * vp - s * vq;
-      float nvq = s * vp + c * vq;
+      T vp = (cs == 2) ? V.m[rr][1] : V.m[rr][0];
+      T vq = (cs == 0) ? V.m[rr][1] : V.m[rr][2];
+      T nvp = c * vp - s * vq;
+      T nvq = s * vp + c * vq;
       V.m[rr][0] = (cs == 2) ? V.m[rr][0] : nvp;
       V.m[rr][1] = (cs == 0) ? nvq : ((cs == 2) ? nvp : V.m[rr][1]);
       V.m[rr][2] = (cs == 0) ? V.m[rr][2] : nvq;
@@ -155,38 +206,144 @@ __device__ __forceinline__ void jacobi_eigen3(float a00, float a11, float a22, f
   D[2] = a22;
 }
 
+// Role form: after a rotation of the pair (p, q) its off-diagonal entry is
+// exactly zero, so the next pivot is one of the two pairs that contain the
+// third index r.  The matrix is kept as (d_R, d_P, d_Q, o_RP, o_RQ) with
+// o_PQ = 0, the columns of V as (V_R, V_P, V_Q) and the physical index of
+// each role; an iteration picks X in {P, Q} (the larger |o_RX|; on a tie the
+// pair with the lower physical pivot case, i.e. the smaller physical index of
+// X, as the GLSL's ordered comparisons do), rotates the pair (R, X) and
+// relabels: (R, P, Q) <- (Y, R, X) with Y the other of P, Q.  In the rotated
+// pair A[Y][X] was zero, so the third row's update is o_YR <- c o_YR and
+// o_YX <- s o_YR.  The rotation is the GLSL's written for the pair (R, X): the
+// Jacobi rotation is symmetric in its two indices (swapping them negates tau,
+// t and s and gives the same new entries), so the orientation needs no
+// physical order.  Per iteration this needs 12 selects instead of 50, and the
+// first rotation (from A itself) and the return to physical order are done
+// once.
+template <typename T>
+__device__ __forceinline__ T jacobi_t(T tau) {
+  // t = sign(tau) / (|tau| + sqrt(1 + tau^2)), with GLSL's sign(0) = 0
+  const T r = tdiv(T(1), tabs(tau) + hw_sqrt(T(1) + tau * tau));
+  return tau == T(0) ? T(0) : copysign(r, tau);
+}
+template <typename T>
+__device__ __forceinline__ void jacobi_eigen3(T a00, T a11, T a22, T a01, T a02, T a12, T D[3], mat3<T>& V) {
+  const T b01 = tabs(a01), b02 = tabs(a02), b12 = tabs(a12);
+  if (tmax(b01, tmax(b02, b12)) < T(1e-8)) {
+    V = mat3_ident<T>();
+    D[0] = a00;
+    D[1] = a11;
+    D[2] = a22;
+    return;
+  }
+  // first rotation, on A as given: pivot case 0 -> (0,1), 1 -> (0,2), 2 -> (1,2)
+  const int cs = (b01 >= tmax(b02, b12)) ? 0 : ((b02 >= b12) ? 1 : 2);
+  int pP = (cs == 2) ? 1 : 0;        // p0
+  int pQ = (cs == 0) ? 1 : 2;        // p1
+  int pR = 3 - pP - pQ;              // the third index
+  T dR = (cs == 0) ? a22 : ((cs == 1) ? a11 : a00);
+  T dP = (cs == 2) ? a11 : a00;
+  T dQ = (cs == 0) ? a11 : a22;
+  const T apq = (cs == 0) ? a01 : ((cs == 1) ? a02 : a12);
+  const T arp = (cs == 0) ? a02 : a01;  // A[r][p0]
+  const T arq = (cs == 2) ? a02 : a12;  // A[r][p1]
+  T oRP, oRQ;
+  T vR[3], vP[3], vQ[3];
+  {
+    const T tau = tdiv(dQ - dP, T(2) * apq);
+    const T t = jacobi_t(tau);
+    const T c = hw_rsq(T(1) + t * t);
+    const T s = t * c;
+    const T np = dP - t * apq, nq = dQ + t * apq;
+    dP = np;
+    dQ = nq;
+    oRP = c * arp - s * arq;
+    oRQ = s * arp + c * arq;
+#pragma unroll
+    for (int rr = 0; rr < 3; rr++) {  // V = I rotated in the pair (p0, p1)
+      vR[rr] = (rr == pR) ? T(1) : T(0);
+      vP[rr] = (rr == pP) ? c : ((rr == pQ) ? -s : T(0));
+      vQ[rr] = (rr == pP) ? s : ((rr == pQ) ? c : T(0));
+    }
+  }
+  for (int iter = 1; iter < 20; iter++) {
+    const T bRP = tabs(oRP), bRQ = tabs(oRQ);
+    if (bRP < T(1e-8) && bRQ < T(1e-8)) break;
+    const bool xp = bRP > bRQ || (bRP == bRQ && pP < pQ);  // X = P, else X = Q
+    const T oRX = xp ? oRP : oRQ;
+    const T oYR = xp ? oRQ : oRP;
+    const T dX = xp ? dP : dQ;
+    const T dY = xp ? dQ : dP;
+    const int pX = xp ? pP : pQ;
+    const int pY = xp ? pQ : pP;
+    const T tau = tdiv(dX - dR, T(2) * oRX);
+    const T t = jacobi_t(tau);
+    const T c = hw_rsq(T(1) + t * t);
+    const T s = t * c;
+    const T nR = dR - t * oRX, nX = dX + t * oRX;
+    // relabel (R, P, Q) <- (Y, R, X)
+    oRP = c * oYR;
+    oRQ = s * oYR;
+    dP = nR;
+    dQ = nX;
+    dR = dY;
+#pragma unroll
+    for (int rr = 0; rr < 3; rr++) {
+      const T vx = xp ? vP[rr] : vQ[rr];
+      const T vy = xp ? vQ[rr] : vP[rr];
+      const T vr = vR[rr];
+      vP[rr] = c * vr - s * vx;
+      vQ[rr] = s * vr + c * vx;
+      vR[rr] = vy;
+    }
+    pQ = pX;
+    pP = pR;
+    pR = pY;
+  }
+  // back to physical order
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    const bool r = pR == i, pp = pP == i;
+    D[i] = r ? dR : (pp ? dP : dQ);
+#pragma unroll
+    for (int rr = 0; rr < 3; rr++) V.m[rr][i] = r ? vR[rr] : (pp ? vP[rr] : vQ[rr]);
+  }
+}
+
 // ---- SVD via F^T F (shaders/mpm_common.glsl:250-281) ----
 // corrected = false keeps the GLSL storage (singular vectors in the rows of
 // U, V as the GLSL mat3 holds them); corrected = true makes V a rotation before
 // U is formed and transposes both, so F = U diag(sigma) V^T with U, V in SO(3)
 // and the downstream U diag U^T / U diag V^T are the true spectral forms.
-__device__ __forceinline__ void svd3(const m3& F, m3& U, float sig[3], m3& V, bool corrected) {
+template <typename T>
+__device__ __forceinline__ void svd3(const mat3<T>& F, mat3<T>& U, T sig[3], mat3<T>& V, bool corrected) {
   // FtF = transpose(F) * F: FtF[c][k] = dot(F col k, F col c)
-  float f00 = F.m[0][0] * F.m[0][0] + F.m[0][1] * F.m[0][1] + F.m[0][2] * F.m[0][2];
-  float f11 = F.m[1][0] * F.m[1][0] + F.m[1][1] * F.m[1][1] + F.m[1][2] * F.m[1][2];
-  float f22 = F.m[2][0] * F.m[2][0] + F.m[2][1] * F.m[2][1] + F.m[2][2] * F.m[2][2];
-  float f01 = F.m[0][0] * F.m[1][0] + F.m[0][1] * F.m[1][1] + F.m[0][2] * F.m[1][2];
-  float f02 = F.m[0][0] * F.m[2][0] + F.m[0][1] * F.m[2][1] + F.m[0][2] * F.m[2][2];
-  float f12 = F.m[1][0] * F.m[2][0] + F.m[1][1] * F.m[2][1] + F.m[1][2] * F.m[2][2];
-  float D[3];
-  jacobi_eigen3(f00, f11, f22, f01, f02, f12, D, V);
-  sig[0] = hw_sqrt(fmaxf(D[0], 0.f));
-  sig[1] = hw_sqrt(fmaxf(D[1], 0.f));
-  sig[2] = hw_sqrt(fmaxf(D[2], 0.f));
+  T f00 = F.m[0][0] * F.m[0][0] + F.m[0][1] * F.m[0][1] + F.m[0][2] * F.m[0][2];
+  T f11 = F.m[1][0] * F.m[1][0] + F.m[1][1] * F.m[1][1] + F.m[1][2] * F.m[1][2];
+  T f22 = F.m[2][0] * F.m[2][0] + F.m[2][1] * F.m[2][1] + F.m[2][2] * F.m[2][2];
+  T f01 = F.m[0][0] * F.m[1][0] + F.m[0][1] * F.m[1][1] + F.m[0][2] * F.m[1][2];
+  T f02 = F.m[0][0] * F.m[2][0] + F.m[0][1] * F.m[2][1] + F.m[0][2] * F.m[2][2];
+  T f12 = F.m[1][0] * F.m[2][0] + F.m[1][1] * F.m[2][1] + F.m[1][2] * F.m[2][2];
+  T D[3];
+  jacobi_eigen3<T>(f00, f11, f22, f01, f02, f12, D, V);
+  sig[0] = hw_sqrt(tmax(D[0], T(0)));
+  sig[1] = hw_sqrt(tmax(D[1], T(0)));
+  sig[2] = hw_sqrt(tmax(D[2], T(0)));
   // selection sort, descending; swap V columns i<->j (GLSL V[0][i] = column 0
   // row i: it swaps ROWS of our m[col][row] storage at index i, j)
 #define NB_SWAP_SV(i, j)                                                        \
   if (sig[j] > sig[i]) {                                                        \
-    float ts = sig[i]; sig[i] = sig[j]; sig[j] = ts;                            \
+    T ts = sig[i]; sig[i] = sig[j]; sig[j] = ts;                                \
     _Pragma("unroll") for (int cc = 0; cc < 3; cc++) {                          \
-      float tv = V.m[cc][i]; V.m[cc][i] = V.m[cc][j]; V.m[cc][j] = tv;          \
+      T tv = V.m[cc][i]; V.m[cc][i] = V.m[cc][j]; V.m[cc][j] = tv;              \
     }                                                                           \
   }
   NB_SWAP_SV(0, 1)
   NB_SWAP_SV(0, 2)
   NB_SWAP_SV(1, 2)
 #undef NB_SWAP_SV
-  if (corrected && m3_det(V) < 0.f) {  // keep V a rotation: flip the third singular vector
+  if (corrected && m3_det(V) < T(0)) {  // keep V a rotation: flip the third singular vector
     V.m[0][2] = -V.m[0][2];
     V.m[1][2] = -V.m[1][2];
     V.m[2][2] = -V.m[2][2];
@@ -195,17 +352,17 @@ __device__ __forceinline__ void svd3(const m3& F, m3& U, float sig[3], m3& V, bo
   for (int i = 0; i < 3; i++) {
     // GLSL vi = vec3(V[0][i], V[1][i], V[2][i]); U[r][i] = (F*vi)[r]/sigma_i,
     // or the unit vector e_i when sigma_i <= 1e-8 (U starts as zero there)
-    const float v0 = V.m[0][i], v1 = V.m[1][i], v2 = V.m[2][i];
-    const float fv0 = F.m[0][0] * v0 + F.m[1][0] * v1 + F.m[2][0] * v2;
-    const float fv1 = F.m[0][1] * v0 + F.m[1][1] * v1 + F.m[2][1] * v2;
-    const float fv2 = F.m[0][2] * v0 + F.m[1][2] * v1 + F.m[2][2] * v2;
-    const bool big = sig[i] > 1e-8f;
-    const float inv = hw_rcp(big ? sig[i] : 1.f);
-    U.m[0][i] = big ? fv0 * inv : (i == 0 ? 1.f : 0.f);
-    U.m[1][i] = big ? fv1 * inv : (i == 1 ? 1.f : 0.f);
-    U.m[2][i] = big ? fv2 * inv : (i == 2 ? 1.f : 0.f);
+    const T v0 = V.m[0][i], v1 = V.m[1][i], v2 = V.m[2][i];
+    const T fv0 = F.m[0][0] * v0 + F.m[1][0] * v1 + F.m[2][0] * v2;
+    const T fv1 = F.m[0][1] * v0 + F.m[1][1] * v1 + F.m[2][1] * v2;
+    const T fv2 = F.m[0][2] * v0 + F.m[1][2] * v1 + F.m[2][2] * v2;
+    const bool big = sig[i] > T(1e-8);
+    const T den = big ? sig[i] : T(1);
+    U.m[0][i] = big ? tdiv(fv0, den) : (i == 0 ? T(1) : T(0));
+    U.m[1][i] = big ? tdiv(fv1, den) : (i == 1 ? T(1) : T(0));
+    U.m[2][i] = big ? tdiv(fv2, den) : (i == 2 ? T(1) : T(0));
   }
-  if (m3_det(U) < 0.f) {
+  if (m3_det(U) < T(0)) {
     U.m[0][2] = -U.m[0][2];
     U.m[1][2] = -U.m[1][2];
     U.m[2][2] = -U.m[2][2];
@@ -217,17 +374,35 @@ __device__ __forceinline__ void svd3(const m3& F, m3& U, float sig[3], m3& V, bo
   }
 }
 
+// ---- near-degenerate F: the FP64 fallback's test ----
+// On singular values sorted by magnitude (s0 >= s1 >= |s2|) the eigenvalues
+// of F^T F are s_i^2.  FP32 rounds F^T F's entries by about 1e-7 s0^2, so the
+// singular vectors of a pair whose eigenvalues are closer than gap_rel * s0^2
+// are set by rounding (their angle errs by about 1e-7 / gap_rel), and
+// U = F v / sigma loses about 1e-7 s0 / s2 for a small s2.  Either makes the
+// reference convention's F <- U^T exp(eps) V and tau = U^T k U (which are not
+// isotropic functions of F) differ from the FP64 result by far more than
+// rounding, so such an F takes the SVD in FP64.
+__device__ __forceinline__ bool svd_near_degenerate(const float sig[3], float gap_rel, float zero_rel) {
+  const float s0 = sig[0] * sig[0], s1 = sig[1] * sig[1], s2 = sig[2] * sig[2];
+  const float g = gap_rel * s0;
+  return (s0 - s1) <= g || (s1 - s2) <= g || s2 <= zero_rel * s0;
+}
+
 // ---- Hencky Kirchhoff stress (shaders/mpm_common.glsl:284-293) ----
 // tau = U diag(lambda tr(eps) + 2 mu eps) U^T with U as the GLSL stores it;
 // eps_i = log(max(|sigma_i|, 1e-6)).
-__device__ __forceinline__ m3 hencky_from_log(const m3& U, float e0, float e1, float e2, float mu, float lam) {
-  const float lt = lam * (e0 + e1 + e2), tm = 2.f * mu;
+template <typename T>
+__device__ __forceinline__ mat3<T> hencky_from_log(const mat3<T>& U, T e0, T e1, T e2, T mu, T lam) {
+  const T lt = lam * (e0 + e1 + e2), tm = T(2) * mu;
   return m3_udvt(U, lt + tm * e0, lt + tm * e1, lt + tm * e2, U);
 }
-__device__ __forceinline__ float log_strain(float s) { return hw_log(fmaxf(fabsf(s), 1e-6f)); }
-__device__ __forceinline__ m3 hencky_stress(const m3& F, float mu, float lam, bool corrected) {
-  m3 U, V;
-  float sig[3];
+template <typename T>
+__device__ __forceinline__ T log_strain(T s) { return hw_log(tmax(tabs(s), T(1e-6))); }
+template <typename T>
+__device__ __forceinline__ mat3<T> hencky_stress(const mat3<T>& F, T mu, T lam, bool corrected) {
+  mat3<T> U, V;
+  T sig[3];
   svd3(F, U, sig, V, corrected);
   return hencky_from_log(U, log_strain(sig[0]), log_strain(sig[1]), log_strain(sig[2]), mu, lam);
 }
@@ -239,51 +414,59 @@ __device__ __forceinline__ m3 hencky_stress(const m3& F, float mu, float lam, bo
 // q <= M p + q_c (apex: volumetric strain only) and always recomposes F.
 // Returns the projected F; eps[] receives its log strains and U its left
 // singular vectors, which give the Hencky stress of the result directly when
-// U, V are the true SVD factors (corrected mode).
-struct PlasticOut {
-  m3 F;
-  m3 U;
-  float eps[3];
+// U, V are the true SVD factors (corrected mode).  sig[] keeps the SVD's
+// singular values (the fallback's test).
+template <typename T>
+struct PlasticOutT {
+  mat3<T> F;
+  mat3<T> U;
+  T eps[3];
+  T sig[3];
   bool recomposed;  // false: F was returned as given, and U, eps are its own SVD's
 };
-__device__ __forceinline__ PlasticOut project_plastic(const m3& F, bool dp, float mu, float lam, float q_max, float M,
-                                                      float q_c, bool corrected) {
-  m3 U, V;
-  float sig[3];
+using PlasticOut = PlasticOutT<float>;
+template <typename T>
+__device__ __forceinline__ PlasticOutT<T> project_plastic(const mat3<T>& F, bool dp, T mu, T lam, T q_max, T M, T q_c,
+                                                          bool corrected) {
+  mat3<T> U, V;
+  T sig[3];
   svd3(F, U, sig, V, corrected);
-  float e0 = log_strain(sig[0]), e1 = log_strain(sig[1]), e2 = log_strain(sig[2]);
-  const float tr = e0 + e1 + e2;
-  const float m = tr / 3.f;
-  float d0 = e0 - m, d1 = e1 - m, d2 = e2 - m;
-  const float nd = hw_sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+  T e0 = log_strain(sig[0]), e1 = log_strain(sig[1]), e2 = log_strain(sig[2]);
+  const T tr = e0 + e1 + e2;
+  const T m = tr / T(3);
+  T d0 = e0 - m, d1 = e1 - m, d2 = e2 - m;
+  const T nd = hw_sqrt(d0 * d0 + d1 * d1 + d2 * d2);
   bool recompose = dp;
-  float q_lim = q_max;
+  T q_lim = q_max;
   bool apex = false;
   if (dp) {
-    const float K = lam + 2.f * mu / 3.f;
-    const float q_yield = M * (-K * tr) + q_c;
-    apex = q_yield <= 1e-8f;
+    const T K = lam + T(2) * mu / T(3);
+    const T q_yield = M * (-K * tr) + q_c;
+    apex = q_yield <= T(1e-8);
     q_lim = q_yield;
     if (apex) {
-      const float m2 = (q_c / fmaxf(M, 1e-8f) / K) / 3.f;  // tr2 = -p_proj / K, p_proj = -q_c / M
+      const T m2 = (q_c / tmax(M, T(1e-8)) / K) / T(3);  // tr2 = -p_proj / K, p_proj = -q_c / M
       e0 = m2; e1 = m2; e2 = m2;
     }
   }
   if (!apex) {
-    const float delta = nd - q_lim / (2.f * mu);
-    if (delta > 0.f && nd > 1e-8f) {
-      const float k = delta * hw_rcp(nd);
+    const T delta = nd - q_lim / (T(2) * mu);
+    if (delta > T(0) && nd > T(1e-8)) {
+      const T k = tdiv(delta, nd);
       e0 = d0 - k * d0 + m;
       e1 = d1 - k * d1 + m;
       e2 = d2 - k * d2 + m;
       recompose = true;
     }
   }
-  PlasticOut o;
+  PlasticOutT<T> o;
   o.U = U;
   o.recomposed = recompose;
+  o.sig[0] = sig[0];
+  o.sig[1] = sig[1];
+  o.sig[2] = sig[2];
   if (recompose) {
-    const float x0 = hw_exp(e0), x1 = hw_exp(e1), x2 = hw_exp(e2);
+    const T x0 = hw_exp(e0), x1 = hw_exp(e1), x2 = hw_exp(e2);
     o.F = m3_udvt(U, x0, x1, x2, V);
     o.eps[0] = log_strain(x0);
     o.eps[1] = log_strain(x1);

@@ -17,6 +17,8 @@
 //              grid_old[NC] float4 = pre-force velocity (FLIP)
 #pragma once
 
+#include <cmath>
+
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -46,7 +48,10 @@ struct FieldPtrs {
 };
 
 // counters[] slots (device)
-enum : int { CNT_ACTIVE = 0, CNT_NODE_TILES = 1, CNT_ALL_TILES = 2, CNT_SLOTS = 4 };
+enum : int { CNT_ACTIVE = 0, CNT_NODE_TILES = 1, CNT_ALL_TILES = 2, CNT_UNITS = 3, CNT_SLOTS = 4 };
+// particles per work unit of the fused grid-update + G2P kernel (a tile's
+// particles split into units of at most this many)
+constexpr uint32_t kG2PUnit = 1024;
 
 // Geometry of one scene's grid.  A plain engine holds one scene, described by
 // SubParams' own fields (key_base 0).  A batch holds a table of scenes, each
@@ -61,15 +66,26 @@ struct SceneGeo {
   uint32_t cube_res;
   uint32_t tile_dim;   // tiles per axis of the scene's cube
   uint32_t key_base;   // first cell key of the scene
-  uint32_t pad[2];
+  uint32_t h_pow2;     // 1: h is a power of two, so (x - wmin) / h == (x - wmin) * inv_h exactly
+  uint32_t pad;
 };
 static_assert(sizeof(SceneGeo) == 64, "one scene record per 64 bytes");
+
+// 1 when h is a normal power of two whose reciprocal is a normal float: then
+// x / h and x * (1 / h) are the same IEEE number for every x (both are exact
+// scalings by a power of two, or overflow alike)
+inline uint32_t cell_size_is_pow2(float h) {
+  if (!(h >= 1.17549435e-38f && h <= 8.50705917e+37f)) return 0u;  // [2^-126, 2^126]
+  int e = 0;
+  return std::frexp(h, &e) == 0.5f ? 1u : 0u;
+}
 
 // Substep constants, passed by value to every kernel (the role MPMSimPC plays
 // for the reference's shaders, src/core/MPMSimPC.h:14-73).
 struct SubParams {
   float dt;
   float h;            // cell size
+  uint32_t h_pow2;    // 1: h is a power of two (see SceneGeo)
   float wmin[3];
   float wmax[3];
   int32_t grid_res[3];
@@ -83,6 +99,13 @@ struct SubParams {
   uint32_t write_old;  // write the pre-force velocity (FLIP, or pass-level export)
   uint32_t corrected_svd;  // 1: singular vectors as columns (see nb_mpm_knobs)
   uint32_t noise_forces;   // turbulence or noise records among the forces
+  // FP64 SVD of near-degenerate F for the Drucker-Prager family (see
+  // nb_mpm_svd_fallback): on/off, the two relative thresholds, and the
+  // counters G2P adds to when non-null (NB_SVD_FB_STATS 64-bit words)
+  uint32_t svd_fp64;
+  float svd_gap_rel;
+  float svd_zero_rel;
+  unsigned long long* svd_stats;
   // batch (n_scenes > 0): the scene table, the scene of every tile and of
   // every particle; the single-scene fields above are then unused
   // launch shape (nb_mpm_tuning): blocks of the per-tile kernels, threads
@@ -116,6 +139,7 @@ struct DevState {
   uint32_t* tile_total;   // NT
   uint32_t* tile_start;   // NT
   uint32_t* counters;     // CNT_SLOTS
+  uint2* units;           // NT + cap / kG2PUnit + 1: (active tile, chunk) work units of k_g2p_tile
   float4* partial;        // cap_tiles * 1000
   float4* grid_node;      // NT*512
   float4* grid_old;       // NT*512
@@ -127,7 +151,7 @@ struct DevState {
 };
 
 // names of the kernels, in the order the profiler indexes them
-constexpr int kNumProfiledKernels = 11;
+constexpr int kNumProfiledKernels = 12;
 extern const char* const kKernelNames[kNumProfiledKernels];
 
 // Optional per-kernel timing: HIP events recorded around each launch on the
@@ -156,6 +180,9 @@ void launch_grid(const DevState& d, const SubParams& p, uint32_t n_tiles, bool f
                  const uint32_t* tiles, const uint32_t* count, hipStream_t s, Prof* pf);
 // use_sorted: visit particles in bin order (else index order)
 // rec_out: where the updated records go (nullptr: in place over d.rec)
+// Grid update and G2P in one kernel (k_g2p_tile), from the partials P2G wrote
+// and the work units the bin listed; the grid arrays are not written.
+void launch_g2p_fused(const DevState& d, const SubParams& p, uint32_t n_tiles, hipStream_t s, Prof* pf);
 void launch_g2p(const DevState& d, const SubParams& p, bool use_sorted, hipStream_t s, Prof* pf,
                 float4* rec_out = nullptr);
 void launch_sdf_sphere(float* sdf, const SubParams& p, float r, float cx, float cy, float cz, hipStream_t s);

@@ -80,7 +80,8 @@ __device__ __forceinline__ SceneGeo geo_of(const SubParams& p) {
   g.cube_res = p.cube_res;
   g.tile_dim = p.tile_dim;
   g.key_base = 0u;
-  g.pad[0] = g.pad[1] = 0u;
+  g.h_pow2 = p.h_pow2;
+  g.pad = 0u;
   return g;
 }
 // the scene of tile T (uniform across a block) and of particle q
@@ -106,13 +107,23 @@ __device__ __forceinline__ uint32_t node_id(const SceneGeo& g, uint32_t x, uint3
   return g.key_base + morton3(x, y, z);
 }
 
+// (x - wmin[a]) / h, the IEEE quotient the reference's cell index takes
+// (shaders/mpm_common.glsl:190-195, mpm_g2p.comp:78).  When h is a power of
+// two the product with 1/h is the same number (an exact scaling), one
+// instruction instead of the division's sequence.
+__device__ __forceinline__ float cell_coord(const SceneGeo& g, float x, int a) {
+  const float d = x - g.wmin[a];
+  if (g.h_pow2) return d * g.inv_h;
+  return d / g.h;
+}
+
 // cell key of a position: shaders/mpm_common.glsl:190-195 (IEEE quotient,
 // clamp to [0, gridRes-1], truncation; NaN maps to 0 as fmaxf returns the
 // non-NaN operand), offset by the scene's key base
 __device__ __forceinline__ uint32_t cell_key(const SceneGeo& g, float x, float y, float z) {
-  float lx = fminf(fmaxf((x - g.wmin[0]) / g.h, 0.f), float(g.grid_res[0]) - 1.f);
-  float ly = fminf(fmaxf((y - g.wmin[1]) / g.h, 0.f), float(g.grid_res[1]) - 1.f);
-  float lz = fminf(fmaxf((z - g.wmin[2]) / g.h, 0.f), float(g.grid_res[2]) - 1.f);
+  float lx = fminf(fmaxf(cell_coord(g, x, 0), 0.f), float(g.grid_res[0]) - 1.f);
+  float ly = fminf(fmaxf(cell_coord(g, y, 1), 0.f), float(g.grid_res[1]) - 1.f);
+  float lz = fminf(fmaxf(cell_coord(g, z, 2), 0.f), float(g.grid_res[2]) - 1.f);
   return g.key_base + morton3(uint32_t(lx), uint32_t(ly), uint32_t(lz));
 }
 
@@ -163,7 +174,12 @@ __device__ __forceinline__ float len3(float x, float y, float z) { return sqrtf(
 // mix(a, b, t) = a + (b - a) t and dot summed left to right.  The hash
 // fract(sin(a) 43758.5453) turns one rounding step of a into a different
 // value, so these functions are compiled without FMA contraction, every
-// product and sum rounded as written.
+// product and sum rounded as written.  GLSL leaves sin's accuracy to the
+// driver; here the hash's sin is the correctly rounded FP32 sine, taken as
+// sin in FP64 rounded once to FP32 (any two FP64 sines within an ulp round
+// to the same float unless the value lies within 2^-29 ulp of a rounding
+// boundary), so every platform that does the same gets the same hash.
+__device__ __forceinline__ float hash_sin(float a) { return float(sin(double(a))); }
 __device__ __forceinline__ float gl_fract(float x) { return x - floorf(x); }
 __device__ __forceinline__ float gl_mix(float a, float b, float t) { return a + (b - a) * t; }
 __device__ __forceinline__ float3 noise_hash33(float x, float y, float z) {  // force_noise.glsl:7-12
@@ -171,8 +187,9 @@ __device__ __forceinline__ float3 noise_hash33(float x, float y, float z) {  // 
   const float a = x * 127.1f + y * 311.7f + z * 74.7f;
   const float b = x * 269.5f + y * 183.3f + z * 246.1f;
   const float c = x * 113.5f + y * 271.9f + z * 124.6f;
-  return make_float3(-1.f + 2.f * gl_fract(sinf(a) * 43758.5453123f), -1.f + 2.f * gl_fract(sinf(b) * 43758.5453123f),
-                     -1.f + 2.f * gl_fract(sinf(c) * 43758.5453123f));
+  return make_float3(-1.f + 2.f * gl_fract(hash_sin(a) * 43758.5453123f),
+                     -1.f + 2.f * gl_fract(hash_sin(b) * 43758.5453123f),
+                     -1.f + 2.f * gl_fract(hash_sin(c) * 43758.5453123f));
 }
 __device__ inline float noise_value3(float px, float py, float pz) {  // force_noise.glsl:14-37
 #pragma clang fp contract(off)

@@ -422,6 +422,8 @@ EXPORTED_SYMBOLS = (
     "nb_mpm_get_tuning", "nb_mpm_set_tuning", "nb_mpm_autotune", "nb_mpm_graph_substeps",
     "nb_mpm_add_emitter", "nb_mpm_clear_emitters", "nb_mpm_emit", "nb_mpm_get_emitter",
     "nb_mpm_set_collider_mesh", "nb_mpm_get_collider_sdf",
+    "nb_mpm_set_svd_fallback", "nb_mpm_get_svd_fallback", "nb_mpm_svd_fallback_stats", "nb_mpm_autotune_cached",
+    "nb_mpm_profile_calls",
 )
 
 
@@ -531,6 +533,17 @@ class Tuning(C.Structure):
 MAX_BATCH_MATERIALS = 1024
 
 
+class SvdFallback(C.Structure):
+    """nb_mpm_svd_fallback: the FP64 SVD of near-degenerate F in the
+    Drucker-Prager family (an extension; the reference is FP32 throughout)."""
+    _fields_ = [("enabled", C.c_uint32), ("gap_rel", C.c_float), ("zero_rel", C.c_float),
+                ("count_stats", C.c_uint32)]
+
+
+# the counters of nb_mpm_svd_fallback_stats, in order
+SVD_FB_STATS = ("dp_lanes", "fp64_return_map_lanes", "fp64_stress_lanes", "fp64_waves", "dp_waves")
+
+
 class BatchScene(C.Structure):
     """One scene of a batch (nb_mpm_scene): its domain [0, domain_size],
     cell size and number of particle rows."""
@@ -592,6 +605,11 @@ def lib():
         L.nb_mpm_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
         L.nb_mpm_autotune.argtypes = [vp, C.c_float, C.c_uint32, C.c_uint32, C.POINTER(Tuning)]
         L.nb_mpm_graph_substeps.argtypes = [vp, C.c_float, C.c_uint32]
+        L.nb_mpm_autotune_cached.argtypes = [vp, C.c_float, C.c_uint32, C.c_uint32, C.c_char_p, C.POINTER(Tuning),
+                                             C.POINTER(C.c_int)]
+        L.nb_mpm_set_svd_fallback.argtypes = [vp, C.POINTER(SvdFallback)]
+        L.nb_mpm_get_svd_fallback.argtypes = [vp, C.POINTER(SvdFallback)]
+        L.nb_mpm_svd_fallback_stats.argtypes = [vp, C.POINTER(C.c_uint64), C.c_int]
         L.nb_mpm_add_emitter.argtypes = [vp, C.POINTER(EmitterGPU)]
         L.nb_mpm_set_collider_mesh.argtypes = [vp, fp, C.c_uint32, C.c_int]
         L.nb_mpm_get_collider_sdf.argtypes = [vp, fp]
@@ -630,6 +648,7 @@ def lib():
         L.nb_mpm_get_grid.argtypes = [vp, fp, fp]
         L.nb_mpm_set_grid.argtypes = [vp, fp, fp, fp]
         L.nb_mpm_profile_substeps.argtypes = [vp, C.c_float, C.c_uint32, fp, C.c_uint32, up, C.c_char_p, C.c_uint32]
+        L.nb_mpm_profile_calls.argtypes = [vp, up, C.c_uint32]
         L.nb_mpm_active_node_count.argtypes = [vp, C.POINTER(C.c_uint64)]
         L.nb_mpm_calibration_copy.argtypes = [vp, C.c_uint64, C.c_uint32]
         L.nb_mpm_active_tile_count.argtypes = [vp, up]
@@ -735,6 +754,34 @@ class MPMEngine:
         _check(lib().nb_mpm_autotune(self._h, dt_sub, trial_substeps, TUNE_ROUNDING if rounding else 0,
                                      C.byref(t)))
         return t
+
+    # the FP64 SVD fallback ---------------------------------------------------
+    def svdFallback(self) -> "SvdFallback":
+        f = SvdFallback()
+        _check(lib().nb_mpm_get_svd_fallback(self._h, C.byref(f)))
+        return f
+
+    def setSvdFallback(self, enabled: bool = True, gap_rel: float = 1e-4, zero_rel: float = 1e-6,
+                       count_stats: bool = False):
+        f = SvdFallback(int(bool(enabled)), gap_rel, zero_rel, int(bool(count_stats)))
+        _check(lib().nb_mpm_set_svd_fallback(self._h, C.byref(f)))
+
+    def svdFallbackStats(self, reset: bool = False) -> dict:
+        """The fallback's counters since the last reset (counted while
+        count_stats is on), by name (SVD_FB_STATS)."""
+        out = (C.c_uint64 * len(SVD_FB_STATS))()
+        _check(lib().nb_mpm_svd_fallback_stats(self._h, out, int(bool(reset))))
+        return dict(zip(SVD_FB_STATS, (int(x) for x in out)))
+
+    def autotuneCached(self, dt_sub: float, cache_path: str, trial_substeps: int = 3, rounding: bool = False):
+        """autotune behind the persisted cache at cache_path
+        (nb_mpm_autotune_cached): (tuning, hit), hit True when the file
+        already held this workload's shape and nothing was timed."""
+        t = Tuning()
+        hit = C.c_int(0)
+        _check(lib().nb_mpm_autotune_cached(self._h, dt_sub, trial_substeps, TUNE_ROUNDING if rounding else 0,
+                                            os.fsencode(cache_path), C.byref(t), C.byref(hit)))
+        return t, bool(hit.value)
 
     def graphSubstep(self, dt_sub: float, n: int = 1):
         """n substeps as replays of the captured substep graph (nb_mpm_graph_substeps)."""
@@ -1011,8 +1058,16 @@ class MPMEngine:
         nk = C.c_uint32()
         names = C.create_string_buffer(1024)
         _check(lib().nb_mpm_profile_substeps(self._h, dt_sub, n, _f(ms), 32, C.byref(nk), names, 1024))
-        keys = names.value.decode().split(",")
-        return {k: float(ms[i]) for i, k in enumerate(keys[: nk.value])}
+        keys = names.value.decode().split(",")[: nk.value]
+        self._prof_names = keys
+        return {k: float(ms[i]) for i, k in enumerate(keys)}
+
+    def profileCalls(self):
+        """{kernel name: launches} of the last profileSubsteps (its substeps
+        but the last run the grid update inside G2P, as g2p_grid)."""
+        calls = np.zeros(32, np.uint32)
+        _check(lib().nb_mpm_profile_calls(self._h, _u(calls), 32))
+        return {k: int(calls[i]) for i, k in enumerate(getattr(self, "_prof_names", []))}
 
     def calibrationCopy(self, nbytes: int, repeats: int = 1):
         """Plain copy kernel of nbytes (HBM byte-counter calibration)."""

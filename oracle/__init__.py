@@ -121,6 +121,10 @@ def lib():
         L.orc_project_vm_f64.argtypes = [dp, C.c_double, C.c_double, C.c_double, dp]
         L.orc_project_dp_f32.argtypes = [fp, C.c_float, C.c_float, C.c_float, C.c_float, fp]
         L.orc_project_dp_f64.argtypes = [dp, C.c_double, C.c_double, C.c_double, C.c_double, dp]
+        L.orc_dual_tangents.restype = C.c_int
+        L.orc_dual_tangents.argtypes = []
+        L.orc_dual_substeps.argtypes = [C.POINTER(OrcScene), vp, vp, vp, fp, dp, up, C.c_uint32, dp, dp, C.c_uint32,
+                                        dp, dp]
         _lib = L
     return _lib
 
@@ -441,3 +445,49 @@ def mesh_sdf(tris, grid_res, cell_size, exact=False):
     lib().orc_mesh_sdf(_fptr(t), t.shape[0], int(grid_res[0]), int(grid_res[1]), int(grid_res[2]), cube,
                        float(cell_size), int(bool(exact)), _fptr(out))
     return out
+
+
+def dual_tangents():
+    """Tangent directions one dual_substeps call carries."""
+    return lib().orc_dual_tangents()
+
+
+def dual_substeps(scene, materials: bytes, fields, matid, tan_fields, tan_mats=None, n_sub=1, colliders: bytes = b"",
+                  forces: bytes = b"", sdf=None, corrected_svd=False):
+    """Forward-mode derivatives of n_sub substeps of the FP64 twin
+    (oracle/dual.hpp): the values after n_sub substeps, (8, n, 4) float64, and
+    their tangents, (K, 8, n, 4), from the start ``fields`` (8, n, 4) float64,
+    material ids ``matid`` (n,) and input tangents ``tan_fields`` (K, 8, n, 4)
+    and ``tan_mats`` (K, materials, 4: mu, lambda, M_friction, q_cohesion),
+    K = dual_tangents().  Exact derivatives of the FP64 substep map (no step
+    size); the material-id lanes carry no tangent."""
+    L = lib()
+    K = L.orc_dual_tangents()
+    sc = OrcScene.from_buffer_copy(scene)
+    nm = len(materials) // 64
+    sc.material_count = nm
+    sc.collider_count = len(colliders) // 64
+    sc.force_count = len(forces) // 32
+    sc.has_sdf = 1 if sdf is not None else 0
+    sc.corrected_svd = 1 if corrected_svd else 0
+    f = np.ascontiguousarray(fields, np.float64)
+    n = f.shape[1]
+    assert f.shape == (8, n, 4)
+    m = np.ascontiguousarray(matid, np.uint32)
+    tf = np.ascontiguousarray(tan_fields, np.float64)
+    assert tf.shape == (K, 8, n, 4)
+    tm = None
+    if tan_mats is not None:
+        # [material][parameter][tangent]
+        tm = np.ascontiguousarray(np.transpose(np.asarray(tan_mats, np.float64), (1, 2, 0)))
+        assert tm.shape == (nm, 4, K)
+    mb = C.create_string_buffer(materials, max(len(materials), 1))
+    cb = C.create_string_buffer(colliders, max(len(colliders), 1))
+    fb = C.create_string_buffer(forces, max(len(forces), 1))
+    sd = np.ascontiguousarray(sdf, np.float32) if sdf is not None else None
+    out = np.empty((8, n, 4), np.float64)
+    out_t = np.empty((K, 8, n, 4), np.float64)
+    L.orc_dual_substeps(C.byref(sc), mb, cb, fb, _fptr(sd) if sd is not None else None, _dptr(f),
+                        m.ctypes.data_as(C.POINTER(C.c_uint32)), n, _dptr(tf), _dptr(tm) if tm is not None else None,
+                        int(n_sub), _dptr(out), _dptr(out_t))
+    return out, out_t

@@ -28,7 +28,19 @@
 #include <cstring>
 #include <vector>
 
+#include "dual.hpp"
+
 namespace oracle {
+
+// Math through overloadable names: float and double take the std functions
+// (the float overloads, so the FP32 oracle rounds in FP32); dual.hpp adds the
+// forward-mode versions.
+template <class R> inline R o_fabs(R x) { return std::fabs(x); }
+template <class R> inline R o_sqrt(R x) { return std::sqrt(x); }
+template <class R> inline R o_log(R x) { return std::log(x); }
+template <class R> inline R o_exp(R x) { return std::exp(x); }
+template <class R> inline R o_floor(R x) { return std::floor(x); }
+template <class R> inline R o_pow(R x, R y) { return std::pow(x, y); }
 
 // ---------------------------------------------------------------------------
 // Small column-major 3x3 algebra with GLSL conventions (m[col][row]).
@@ -46,7 +58,7 @@ template <class R> inline V3<R> operator*(R s, V3<R> a) { return {s * a.x, s * a
 template <class R> inline V3<R> operator*(V3<R> a, R s) { return {a.x * s, a.y * s, a.z * s}; }
 template <class R> inline V3<R> operator/(V3<R> a, R s) { return {a.x / s, a.y / s, a.z / s}; }
 template <class R> inline R dot(V3<R> a, V3<R> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-template <class R> inline R length(V3<R> a) { return std::sqrt(dot(a, a)); }
+template <class R> inline R length(V3<R> a) { return o_sqrt(dot(a, a)); }
 
 template <class R> struct M3 {
   V3<R> c[3];  // columns
@@ -177,7 +189,20 @@ struct Params {
   uint32_t force_count;
   const float* sdf;  // dense Morton SDF (NC floats) or null
   bool corrected_svd;  // see svd3
+  // dual form only: tangents of each material's (mu, lambda, M_friction,
+  // q_cohesion), [material][parameter][tangent], or null
+  const double* mat_tangent = nullptr;
 };
+
+// Material parameter `which` (0 mu, 1 lambda, 2 M_friction, 3 q_cohesion) of
+// material `id` (already clamped) as R; the dual form attaches its tangents.
+template <class R> inline R mat_param(const Params&, uint32_t, int, float value) { return R(value); }
+template <> inline Dual mat_param<Dual>(const Params& p, uint32_t id, int which, float value) {
+  Dual r{double(value)};
+  if (p.mat_tangent)
+    for (int k = 0; k < kTangents; k++) r.d[k] = p.mat_tangent[(size_t(id) * 4 + size_t(which)) * kTangents + k];
+  return r;
+}
 
 inline uint32_t total_cells(const Params& p) { return p.cube_res * p.cube_res * p.cube_res; }
 
@@ -185,7 +210,7 @@ inline uint32_t total_cells(const Params& p) { return p.cube_res * p.cube_res * 
 // Quadratic B-spline and derivative — shaders/mpm_common.glsl:198-210
 // ---------------------------------------------------------------------------
 template <class R> inline R bspline2(R d) {
-  R ad = std::fabs(d);
+  R ad = o_fabs(d);
   if (ad < R(0.5)) return R(0.75) - ad * ad;
   if (ad < R(1.5)) {
     R t = R(1.5) - ad;
@@ -194,7 +219,7 @@ template <class R> inline R bspline2(R d) {
   return R(0);
 }
 template <class R> inline R bspline2g(R d) {
-  R ad = std::fabs(d);
+  R ad = o_fabs(d);
   if (ad < R(0.5)) return R(-2) * d;
   if (ad < R(1.5)) return -glsl_sign(d) * (R(1.5) - ad);
   return R(0);
@@ -250,20 +275,20 @@ template <class R> inline void jacobi_eigen3(M3<R> A, V3<R>& D, M3<R>& V) {
   V = M3<R>::ident();
   for (int iter = 0; iter < 20; iter++) {
     R a01 = A.at(0, 1), a02 = A.at(0, 2), a12 = A.at(1, 2);
-    if (std::max(std::fabs(a01), std::max(std::fabs(a02), std::fabs(a12))) < R(1e-8)) break;
+    if (std::max(o_fabs(a01), std::max(o_fabs(a02), o_fabs(a12))) < R(1e-8)) break;
     int p0, p1;
     R apq;
-    if (std::fabs(a01) >= std::max(std::fabs(a02), std::fabs(a12))) {
+    if (o_fabs(a01) >= std::max(o_fabs(a02), o_fabs(a12))) {
       p0 = 0; p1 = 1; apq = a01;
-    } else if (std::fabs(a02) >= std::fabs(a12)) {
+    } else if (o_fabs(a02) >= o_fabs(a12)) {
       p0 = 0; p1 = 2; apq = a02;
     } else {
       p0 = 1; p1 = 2; apq = a12;
     }
     R app = A.at(p0, p0), aqq = A.at(p1, p1);
     R tau = (aqq - app) / (R(2) * apq);
-    R t = glsl_sign(tau) / (std::fabs(tau) + std::sqrt(R(1) + tau * tau));
-    R c = R(1) / std::sqrt(R(1) + t * t);  // inversesqrt
+    R t = glsl_sign(tau) / (o_fabs(tau) + o_sqrt(R(1) + tau * tau));
+    R c = R(1) / o_sqrt(R(1) + t * t);  // inversesqrt
     R s = t * c;
     A.at(p0, p0) = app - t * apq;
     A.at(p1, p1) = aqq + t * apq;
@@ -295,7 +320,7 @@ template <class R> inline void svd3(const M3<R>& F, M3<R>& U, V3<R>& sigma, M3<R
   M3<R> FtF = transpose(F) * F;
   V3<R> sig2;
   jacobi_eigen3(FtF, sig2, V);
-  sigma = {std::sqrt(std::max(sig2.x, R(0))), std::sqrt(std::max(sig2.y, R(0))), std::sqrt(std::max(sig2.z, R(0)))};
+  sigma = {o_sqrt(std::max(sig2.x, R(0))), o_sqrt(std::max(sig2.y, R(0))), o_sqrt(std::max(sig2.z, R(0)))};
   for (int i = 0; i < 2; i++)
     for (int j = i + 1; j < 3; j++) {
       if (sigma[j] > sigma[i]) {
@@ -337,8 +362,8 @@ template <class R> inline M3<R> hencky_stress(const M3<R>& F, R mu, R lam, bool 
   M3<R> U, V;
   V3<R> s;
   svd3(F, U, s, V, corrected);
-  s = {std::max(std::fabs(s.x), R(1e-6)), std::max(std::fabs(s.y), R(1e-6)), std::max(std::fabs(s.z), R(1e-6))};
-  V3<R> eps(std::log(s.x), std::log(s.y), std::log(s.z));
+  s = {std::max(o_fabs(s.x), R(1e-6)), std::max(o_fabs(s.y), R(1e-6)), std::max(o_fabs(s.z), R(1e-6))};
+  V3<R> eps(o_log(s.x), o_log(s.y), o_log(s.z));
   R tr = eps.x + eps.y + eps.z;
   V3<R> kp = V3<R>(lam * tr, lam * tr, lam * tr) + (R(2) * mu) * eps;
   return U * M3<R>::diag(kp.x, kp.y, kp.z) * transpose(U);
@@ -350,8 +375,8 @@ template <class R> inline M3<R> project_von_mises(M3<R> F, R mu, R lam, R q_max,
   M3<R> U, V;
   V3<R> s;
   svd3(F, U, s, V, corrected);
-  s = {std::max(std::fabs(s.x), R(1e-6)), std::max(std::fabs(s.y), R(1e-6)), std::max(std::fabs(s.z), R(1e-6))};
-  V3<R> eps(std::log(s.x), std::log(s.y), std::log(s.z));
+  s = {std::max(o_fabs(s.x), R(1e-6)), std::max(o_fabs(s.y), R(1e-6)), std::max(o_fabs(s.z), R(1e-6))};
+  V3<R> eps(o_log(s.x), o_log(s.y), o_log(s.z));
   R tr = eps.x + eps.y + eps.z;
   V3<R> dev = eps - (tr / R(3)) * V3<R>(1, 1, 1);
   R nd = length(dev);
@@ -360,7 +385,7 @@ template <class R> inline M3<R> project_von_mises(M3<R> F, R mu, R lam, R q_max,
   if (delta > R(0) && nd > R(1e-8)) {
     dev = dev - (delta / nd) * dev;
     eps = dev + (tr / R(3)) * V3<R>(1, 1, 1);
-    s = {std::exp(eps.x), std::exp(eps.y), std::exp(eps.z)};
+    s = {o_exp(eps.x), o_exp(eps.y), o_exp(eps.z)};
     F = U * M3<R>::diag(s.x, s.y, s.z) * transpose(V);
   }
   return F;
@@ -372,8 +397,8 @@ template <class R> inline M3<R> project_drucker_prager(const M3<R>& F, R mu, R l
   M3<R> U, V;
   V3<R> s;
   svd3(F, U, s, V, corrected);
-  s = {std::max(std::fabs(s.x), R(1e-6)), std::max(std::fabs(s.y), R(1e-6)), std::max(std::fabs(s.z), R(1e-6))};
-  V3<R> eps(std::log(s.x), std::log(s.y), std::log(s.z));
+  s = {std::max(o_fabs(s.x), R(1e-6)), std::max(o_fabs(s.y), R(1e-6)), std::max(o_fabs(s.z), R(1e-6))};
+  V3<R> eps(o_log(s.x), o_log(s.y), o_log(s.z));
   R tr = eps.x + eps.y + eps.z;
   V3<R> dev = eps - (tr / R(3)) * V3<R>(1, 1, 1);
   R nd = length(dev);
@@ -391,7 +416,7 @@ template <class R> inline M3<R> project_drucker_prager(const M3<R>& F, R mu, R l
       eps = dev + (tr / R(3)) * V3<R>(1, 1, 1);
     }
   }
-  s = {std::exp(eps.x), std::exp(eps.y), std::exp(eps.z)};
+  s = {o_exp(eps.x), o_exp(eps.y), o_exp(eps.z)};
   return U * M3<R>::diag(s.x, s.y, s.z) * transpose(V);
 }
 
@@ -531,11 +556,11 @@ template <class R> inline R sphere_sdf(V3<R> x, V3<R> c, R r, V3<R>& on) {
 }
 template <class R> inline R box_sdf(V3<R> x, V3<R> c, V3<R> he, V3<R>& on) {
   V3<R> rel = x - c;
-  V3<R> q(std::fabs(rel.x) - he.x, std::fabs(rel.y) - he.y, std::fabs(rel.z) - he.z);
+  V3<R> q(o_fabs(rel.x) - he.x, o_fabs(rel.y) - he.y, o_fabs(rel.z) - he.z);
   V3<R> qp(std::max(q.x, R(0)), std::max(q.y, R(0)), std::max(q.z, R(0)));
   R sdf = length(qp) + std::min(std::max(q.x, std::max(q.y, q.z)), R(0));
   if (sdf < R(0)) {
-    V3<R> diff(he.x - std::fabs(rel.x), he.y - std::fabs(rel.y), he.z - std::fabs(rel.z));
+    V3<R> diff(he.x - o_fabs(rel.x), he.y - o_fabs(rel.y), he.z - o_fabs(rel.z));
     if (diff.x < diff.y && diff.x < diff.z)
       on = V3<R>(glsl_sign(rel.x + R(1e-8)), 0, 0);
     else if (diff.y < diff.z)
@@ -566,7 +591,7 @@ template <class R> inline void apply_collider_bc(V3<R>& v, V3<R> n, V3<R> vc, R 
     V3<R> vnv = vn * n;
     V3<R> vt = vr - vnv;
     R vtl = length(vt);
-    if (vtl > R(1e-8)) vt = std::max(R(0), R(1) - fric * std::fabs(vn) / vtl) * vt;
+    if (vtl > R(1e-8)) vt = std::max(R(0), R(1) - fric * o_fabs(vn) / vtl) * vt;
     v = vc + vt - rest * vnv;
   }
 }
@@ -575,18 +600,22 @@ template <class R> inline void apply_collider_bc(V3<R>& v, V3<R> n, V3<R> vc, R 
 // Procedural noise of the turbulence and noise forces — shaders/force_noise.glsl
 // (hash 7-12, value noise 14-37, fBm 40-50, curl 54-82), in precision R, with
 // GLSL's fract(x) = x - floor(x) and mix(a, b, t) = a + (b - a) t.
-template <class R> R gl_fract(R x) { return x - std::floor(x); }
+// The hash's sin in FP32 is the correctly rounded sine: sin in FP64 rounded
+// once (the HIP kernels do the same; GLSL leaves sin's accuracy to the driver).
+inline float hash_sin(float a) { return float(std::sin(double(a))); }
+inline double hash_sin(double a) { return std::sin(a); }
+template <class R> R gl_fract(R x) { return x - o_floor(x); }
 template <class R> R gl_mix(R a, R b, R t) { return a + (b - a) * t; }
 template <class R> V3<R> noise_hash33(const V3<R>& p) {
   const R a = p.x * R(127.1f) + p.y * R(311.7f) + p.z * R(74.7f);
   const R b = p.x * R(269.5f) + p.y * R(183.3f) + p.z * R(246.1f);
   const R c = p.x * R(113.5f) + p.y * R(271.9f) + p.z * R(124.6f);
   const R k = R(43758.5453123f);
-  return V3<R>(R(-1) + R(2) * gl_fract(std::sin(a) * k), R(-1) + R(2) * gl_fract(std::sin(b) * k),
-               R(-1) + R(2) * gl_fract(std::sin(c) * k));
+  return V3<R>(R(-1) + R(2) * gl_fract(hash_sin(a) * k), R(-1) + R(2) * gl_fract(hash_sin(b) * k),
+               R(-1) + R(2) * gl_fract(hash_sin(c) * k));
 }
 template <class R> R noise_value3(const V3<R>& p) {
-  const V3<R> i(std::floor(p.x), std::floor(p.y), std::floor(p.z));
+  const V3<R> i(o_floor(p.x), o_floor(p.y), o_floor(p.z));
   const V3<R> f(gl_fract(p.x), gl_fract(p.y), gl_fract(p.z));
   const V3<R> u(f.x * f.x * (R(3) - R(2) * f.x), f.y * f.y * (R(3) - R(2) * f.y), f.z * f.z * (R(3) - R(2) * f.z));
   R n[8];
@@ -761,7 +790,7 @@ template <class R> void g2p(const Params& p, State<R>& s) {
     V3<R> xp(s.f[0][4 * q + 0], s.f[0][4 * q + 1], s.f[0][4 * q + 2]);
     R Vp = s.f[0][4 * q + 3];
     V3<R> rel = (xp - wmin) / h;
-    int base[3] = {int(std::floor(rel.x)) - 1, int(std::floor(rel.y)) - 1, int(std::floor(rel.z)) - 1};
+    int base[3] = {int(o_floor(rel.x)) - 1, int(o_floor(rel.y)) - 1, int(o_floor(rel.z)) - 1};
     V3<R> vpic, vpic_old;
     M3<R> vgrad, Bn;
     for (int iz = 0; iz < 3; iz++)
@@ -789,24 +818,26 @@ template <class R> void g2p(const Params& p, State<R>& s) {
         }
     V3<R> vold(s.f[1][4 * q + 0], s.f[1][4 * q + 1], s.f[1][4 * q + 2]);
     const Material& mat = material_of<R>(p, s.matid[q]);
+    const uint32_t mid = std::min(s.matid[q], p.material_count - 1u);
     const bool cs = p.corrected_svd;
-    R mu = R(mat.mu), lam = R(mat.lambda);
+    R mu = mat_param<R>(p, mid, 0, mat.mu), lam = mat_param<R>(p, mid, 1, mat.lambda);
+    const R Mf = mat_param<R>(p, mid, 2, mat.M_friction), qc = mat_param<R>(p, mid, 3, mat.q_cohesion);
     M3<R> F = s.read_mat(2, q);
     M3<R> Fn = (M3<R>::ident() + dt * vgrad) * F;
     M3<R> tau;
     if (mat.model == 4u) {  // FLUID
       R J = determinant(Fn);
       J = std::max(J, R(1e-4));
-      Fn = std::pow(J, R(1) / R(3)) * M3<R>::ident();
+      Fn = o_pow(J, R(1) / R(3)) * M3<R>::ident();
       tau = (R(mat.bulkK) * (J - R(1))) * M3<R>::ident();
     } else if (mat.model == 1u) {  // VON_MISES
       Fn = project_von_mises(Fn, mu, lam, R(mat.q_max), cs);
       tau = hencky_stress(Fn, mu, lam, cs);
     } else if (mat.model == 2u || mat.model == 3u) {  // DRUCKER_PRAGER / GRANULAR_POWDER
-      Fn = project_drucker_prager(Fn, mu, lam, R(mat.M_friction), R(mat.q_cohesion), cs);
+      Fn = project_drucker_prager(Fn, mu, lam, Mf, qc, cs);
       tau = hencky_stress(Fn, mu, lam, cs);
     } else if (mat.model == 5u) {  // VISCOPLASTIC_MUD
-      Fn = project_drucker_prager(Fn, mu, lam, R(mat.M_friction), R(mat.q_cohesion), cs);
+      Fn = project_drucker_prager(Fn, mu, lam, Mf, qc, cs);
       tau = hencky_stress(Fn, mu, lam, cs);
       M3<R> D = R(0.5) * (vgrad + transpose(vgrad));
       R trD = D.at(0, 0) + D.at(1, 1) + D.at(2, 2);
@@ -817,7 +848,7 @@ template <class R> void g2p(const Params& p, State<R>& s) {
     }
     bool jreset = false;
     {
-      R Jd = std::fabs(determinant(Fn));
+      R Jd = o_fabs(determinant(Fn));
       if (Jd < R(0.15) || Jd > R(10)) {
         Fn = M3<R>::ident();
         tau = M3<R>();
@@ -838,9 +869,9 @@ template <class R> void g2p(const Params& p, State<R>& s) {
     if (p.sdf) {
       int pj[3];
       V3<R> r2 = (xn - wmin) / h;
-      pj[0] = int(std::floor(r2.x));
-      pj[1] = int(std::floor(r2.y));
-      pj[2] = int(std::floor(r2.z));
+      pj[0] = int(o_floor(r2.x));
+      pj[1] = int(o_floor(r2.y));
+      pj[2] = int(o_floor(r2.z));
       for (int a = 0; a < 3; a++) pj[a] = std::min(std::max(pj[a], 0), gres[a] - 1);
       auto at = [&](int x, int y, int z) {
         x = std::min(std::max(x, 0), gres[0] - 1);

@@ -35,7 +35,8 @@ def child_report(gpu_lib):
 @pytest.mark.parametrize("case", ["one_hip_runtime_is_mapped", "rollout_matches_direct_reverse_mode",
                                   "rollout_params_enter_the_forward",
                                   "rollout_backward_after_another_recording_raises",
-                                  "rollout_rejects_params_of_the_wrong_shape"])
+                                  "rollout_rejects_params_of_the_wrong_shape",
+                                  "chain_rule_to_young_poisson_friction", "recovery_of_hidden_parameters"])
 def test_torch_binding(child_report, case):
     status, err = child_report
     assert case in status, f"{case} did not report (stderr tail: {err[-2000:]})"

@@ -1,0 +1,201 @@
+"""Per-field error of the HIP substep against the FP32 oracle and its FP64
+twin, at the five workload configurations, through the C ABI.
+
+Each test runs the same seeded start on the GPU, in the FP32 oracle (the
+reference's shader algorithm in FP32) and in the FP64 twin (the same algorithm
+in FP64), and prints one line per configuration:
+
+  FIELD-PARITY <config> <substeps>: <field> gpu-64 <e> (fp32-64 <e>) gpu-32 <e> ...
+
+with e the largest error over all particles in the field's unit: x in cells
+(dx), v relative to the largest |v| of the FP64 state, F absolute, the APIC
+matrix B (the reference's C) relative to its largest entry, tau relative to
+its largest entry.  The bar in every case: the GPU is no farther from the FP64
+twin than the FP32 reference algorithm is (3x its spread plus a floor of a few
+ulps), so it computes the substep as accurately as the reference does in
+FP32.  Under the reference's SVD convention the singular vectors' signs (a
+by-product of the Jacobi sweep) enter F and tau, so F and tau are compared per
+particle in magnitude there and the full tensors must agree on 99 % of the
+particles."""
+import time
+
+import numpy as np
+import pytest
+
+from nbtest import oracle_sim
+from nebula_amd.scenes import (batch_layers_64, batch_mix_256, elastic_block_64, make_engine, multi_material_128,
+                               snow_terrain_256)
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def _fields(a):
+    a = np.asarray(a, np.float64)
+    return {"x": a[0, :, :3], "v": a[1, :, :3], "F": a[2:5, :, :3].transpose(1, 0, 2).reshape(-1, 9),
+            "B": a[5:8, :, :3].transpose(1, 0, 2).reshape(-1, 9), "tau": a[2:8, :, 3].T}
+
+
+def _table(g, o32, o64, dx, signless):
+    """Per field: (gpu-64, fp32-64, gpu-32) largest errors in the field's unit,
+    and the fraction of particles whose full tensor agrees (F, tau)."""
+    G, A, B = _fields(g), _fields(o32), _fields(o64)
+    out = {}
+    for k in ("x", "v", "F", "B", "tau"):
+        sc = dx if k == "x" else (1.0 if k == "F" else max(np.abs(B[k]).max(), 1e-30))
+        gg, aa, bb = G[k], A[k], B[k]
+        if signless and k in ("F", "tau"):
+            gg, aa, bb = np.abs(gg), np.abs(aa), np.abs(bb)
+        out[k] = (np.abs(gg - bb).max() / sc, np.abs(aa - bb).max() / sc, np.abs(gg - aa).max() / sc)
+    return out
+
+
+def _report(name, n_sub, tab):
+    say(f"  FIELD-PARITY {name} {n_sub}: " + "; ".join(
+        f"{k} gpu-64 {a:.2e} (fp32-64 {b:.2e}) gpu-32 {c:.2e}" for k, (a, b, c) in tab.items()))
+
+
+FLOORS = {"x": 1e-5, "v": 1e-5, "F": 1e-5, "B": 1e-4, "tau": 1e-4}
+
+
+def _check(tab, fields=("x", "v", "F", "B", "tau")):
+    for k in fields:
+        a, b, _ = tab[k]
+        assert a <= 3.0 * b + FLOORS[k], f"{k}: gpu-vs-fp64 {a:.2e} > 3 x fp32-vs-fp64 {b:.2e} + {FLOORS[k]}"
+
+
+def _run_all(sc, n_sub, colliders_per_step=False):
+    e = make_engine(sc)
+    s32, s64 = oracle_sim(sc, 32), oracle_sim(sc, 64)
+    t0 = time.perf_counter()
+    if colliders_per_step:
+        for i in range(n_sub):
+            cs = sc.colliders_at(i * sc.dt)
+            e.setColliders(cs)
+            s32.set_colliders(cs.to_bytes())
+            s64.set_colliders(cs.to_bytes())
+            e.substep(sc.dt)
+            s32.substeps(1)
+            s64.substeps(1)
+    else:
+        e.substep(sc.dt, n_sub)
+        e.sync()
+        t1 = time.perf_counter()
+        s32.substeps(n_sub)
+        t2 = time.perf_counter()
+        s64.substeps(n_sub)
+        say(f"  {sc.name}: {sc.n} particles x {n_sub} substeps: gpu {t1 - t0:.2f} s, fp32 oracle "
+            f"{t2 - t1:.1f} s, fp64 oracle {time.perf_counter() - t2:.1f} s")
+    return e.getState(), s32.get_state(), s64.get_state_f64()
+
+
+def test_field_parity_c1_elastic_block_1000_substeps(gpu_lib):
+    """32,768 particles, 64^3, APIC, dt 1e-4, 1000 substeps, proper polar SVD
+    (the reference convention's elastic block is unstable, see
+    test_elastic_block_trajectory_reference_convention; the 3e-4 dx bound on
+    the positions is test_elastic_block_trajectory_1000_substeps_corrected)."""
+    sc = elastic_block_64()
+    sc.corrected_svd = True
+    g, o32, o64 = _run_all(sc, 1000)
+    tab = _table(g, o32, o64, sc.cfg.cellSize, False)
+    _report("C1-elastic-block-corrected", 1000, tab)
+    _check(tab)
+
+
+def test_field_parity_c2_multi_material_200_substeps(gpu_lib):
+    """262,144 particles (elastic jelly and Drucker-Prager sand), 128^3, the
+    sphere collider moving at 0.5 m/s, APIC, dt 5e-5, reference SVD
+    convention: 200 substeps."""
+    sc = multi_material_128()
+    g, o32, o64 = _run_all(sc, 200, colliders_per_step=True)
+    tab = _table(g, o32, o64, sc.cfg.cellSize, True)
+    _report("C2-multi-material", 200, tab)
+    _check(tab, ("x", "v"))
+
+
+def test_field_parity_c3_snow_terrain_20_substeps(gpu_lib):
+    """The 4,194,304-particle Drucker-Prager snow scene over its terrain SDF,
+    256^3, APIC, dt 5e-5, reference SVD convention, FP64 SVD fallback on:
+    20 full substeps."""
+    sc = snow_terrain_256()
+    g, o32, o64 = _run_all(sc, 20)
+    tab = _table(g, o32, o64, sc.cfg.cellSize, True)
+    _report("C3-snow-terrain", 20, tab)
+    _check(tab, ("x", "v"))
+
+
+def test_field_parity_c3_full_size_g2p_from_the_oracle_grid(gpu_lib):
+    """One G2P of the full snow scene from the FP32 oracle's updated grid (the
+    same input for all three after 10 substeps of the oracle): every field
+    within 4x the FP32-vs-FP64 spread of the reference algorithm plus a floor
+    of a few ulps, and F, tau agreeing in full on 99 % of the particles."""
+    sc = snow_terrain_256()
+    s0 = oracle_sim(sc, 32)
+    s0.substeps(10)
+    st = s0.get_state()
+    del s0
+    sc.fields = st
+    s32, s64 = oracle_sim(sc, 32), oracle_sim(sc, 64)
+    for s in (s32, s64):
+        s.bin()
+        s.p2g()
+        s.grid_update()
+        s.sdf_bc()
+    om, oms = s32.get_grid()
+    e = make_engine(sc)
+    e.setState(st)
+    e.setGrid(om, oms)
+    e.runBin()
+    e.runG2P(sc.dt)
+    s32.g2p()
+    s64.g2p()
+    g, o32, o64 = e.getState(), s32.get_state(), s64.get_state_f64()
+    tab = _table(g, o32, o64, sc.cfg.cellSize, True)
+    _report("C3-snow-terrain-g2p", 1, tab)
+    floors = {"x": 2e-7, "v": 2e-6, "F": 2e-6, "B": 2e-6, "tau": 2e-5}
+    for k, (_, b, c) in tab.items():
+        assert c <= 4 * b + floors[k], f"{k}: gpu-vs-fp32 {c:.2e} > 4 x fp32-vs-fp64 {b:.2e}"
+    G, A, B = _fields(g), _fields(o32), _fields(o64)
+    for k in ("F", "tau"):
+        sc_k = 1.0 if k == "F" else max(np.abs(B[k]).max(), 1e-30)
+        tol = 4 * np.abs(np.abs(A[k]) - np.abs(B[k])).max() / sc_k + floors[k]
+        frac = float(np.mean(np.abs(G[k] - A[k]).max(axis=1) / sc_k <= tol))
+        say(f"  C3 g2p {k}: full tensors agree on {frac:.5f} of the particles")
+        assert frac >= 0.99
+
+
+def test_field_parity_c4_batch_mix_scenes_200_substeps(gpu_lib):
+    """Three scenes of the 256-scene mix, one per material model (elastic,
+    Von Mises, Drucker-Prager), reference SVD convention, APIC, dt 1e-4: 200
+    substeps each.  (A batch engine advances each scene bit for bit as a
+    plain engine does: tests/test_gpu_batch.py.)"""
+    from nebula_amd.mpm import MaterialModel
+
+    mix = batch_mix_256()
+    picked = {}
+    for sc in mix:
+        m = sc.materials[0].model
+        if m not in picked:
+            picked[m] = sc
+        if len(picked) == 3:
+            break
+    for model in (MaterialModel.ELASTIC, MaterialModel.VON_MISES, MaterialModel.DRUCKER_PRAGER):
+        sc = picked[model]
+        g, o32, o64 = _run_all(sc, 200)
+        tab = _table(g, o32, o64, sc.cfg.cellSize, True)
+        _report(f"C4-{sc.name}-model{model}-{sc.n}p", 200, tab)
+        _check(tab, ("x", "v"))
+
+
+def test_field_parity_c5_layered_scene_400_substeps(gpu_lib):
+    """One scene of the differentiable batch (an elastic layer on a
+    Drucker-Prager layer, 32,768 particles on 64^3, proper polar SVD, APIC,
+    dt 1e-4): the 400 substeps of its forward run."""
+    sc = batch_layers_64()[0]
+    g, o32, o64 = _run_all(sc, 400)
+    tab = _table(g, o32, o64, sc.cfg.cellSize, False)
+    _report("C5-layers0", 400, tab)
+    _check(tab, ("x", "v"))

@@ -98,3 +98,34 @@ def test_tuning_rejects_invalid_shapes(gpu_lib):
         setattr(t, field, bad)
         with pytest.raises(mpm.NebulaError):
             e.setTuning(t)
+
+
+def test_autotune_cache_persists_choices(gpu_lib, tmp_path):
+    """nb_mpm_autotune_cached: the first call times the shapes and appends the
+    choice to the cache file; a second engine of the same workload finds it
+    (nothing timed, same tuning); a line that does not parse is skipped; a
+    workload of another shape (another transfer mode) is a miss."""
+    scenes = _scenes()
+    cache = str(tmp_path / "tuning.jsonl")
+    a = make_batch_engine(scenes)
+    before = a.getState()
+    t1, hit1 = a.autotuneCached(scenes[0].dt, cache, trial_substeps=2)
+    assert not hit1 and t1.ms_per_substep > 0.0
+    assert _same(a.getState(), before)
+    with open(cache) as f:
+        lines = f.read().splitlines()
+    assert len(lines) == 1 and '"key": "' in lines[0] and "|scenes" in lines[0]
+    with open(cache, "a") as f:
+        f.write("not json at all\n")
+    b = make_batch_engine(scenes)
+    t2, hit2 = b.autotuneCached(scenes[0].dt, cache, trial_substeps=2)
+    assert hit2
+    assert (t2.tile_blocks, t2.g2p_block, t2.p2g_item) == (t1.tile_blocks, t1.g2p_block, t1.p2g_item)
+    bt = b.tuning()
+    assert (bt.tile_blocks, bt.g2p_block, bt.p2g_item) == (t1.tile_blocks, t1.g2p_block, t1.p2g_item)
+    c = make_engine(scenes[1])
+    c.flip_ratio = 0.0  # PIC: another key
+    _, hit3 = c.autotuneCached(scenes[1].dt, cache, trial_substeps=2)
+    assert not hit3
+    with open(cache) as f:
+        assert len(f.read().splitlines()) == 3

@@ -567,7 +567,9 @@ def test_elastic_block_trajectory_1000_substeps_corrected(gpu_lib):
     """The same scene with singular vectors used as columns (corrected_svd): a
     stable fall.  After 1000 substeps the per-field errors against the FP32
     oracle and the FP64 twin are printed; the GPU must be as close to FP64 as
-    the FP32 oracle is (2x + 1e-5 dx) and within 1e-3 dx of the FP32 oracle."""
+    the FP32 oracle is (2x + 1e-5 dx) and within 3e-4 dx of the FP32 oracle
+    (measured 2.0e-4 dx; the target is 1e-4 dx, below the 6.5e-4 dx that
+    separates the FP32 oracle from its own FP64 twin at this dt)."""
     sc = elastic_block_64()
     sc.corrected_svd = True
     dx = sc.cfg.cellSize
@@ -585,9 +587,10 @@ def test_elastic_block_trajectory_1000_substeps_corrected(gpu_lib):
     say("  corr-svd 1000 substeps: " + "; ".join(
         f"{k} gpu-32 {a:.2e} gpu-64 {b:.2e} 32-64 {c:.2e}" for k, (a, b, c) in er.items()))
     a, b, c = er["x"]
-    say(f"  positions: |gpu-fp32| {a / dx:.2e} dx, |gpu-fp64| {b / dx:.2e} dx, |fp32-fp64| {c / dx:.2e} dx")
+    say(f"  positions: |gpu-fp32| {a / dx:.2e} dx ({a / dx / 1e-4:.1f}x the 1e-4 dx target), "
+        f"|gpu-fp64| {b / dx:.2e} dx, |fp32-fp64| {c / dx:.2e} dx")
     assert b <= 2 * c + 1e-5 * dx
-    assert a <= 1e-3 * dx
+    assert a <= 3e-4 * dx
 
 
 def test_substeps_are_deterministic(gpu_lib):

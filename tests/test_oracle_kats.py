@@ -566,3 +566,24 @@ def test_fbm_octaves_add_halved_amplitude_at_doubled_frequency():
             assert two == pytest.approx(one + 0.25 * oracle.noise_value(2.0 * np.asarray(p), prec), abs=1e-6)
             assert oracle.noise_fbm(p, 0, prec) == one  # octaves clamp to >= 1
 
+
+
+def test_noise_hash_fp32_is_numpy_step_by_step():
+    """forceHash33 in FP32 (shaders/force_noise.glsl:7-12) restated in numpy:
+    every product and sum rounded to float32 as written (dot left to right),
+    the sine correctly rounded (FP64 sin rounded once), fract(x) = x - floor(x).
+    Bit-exact, on lattice points of the sizes the curl and fBm reach."""
+    f = np.float32
+    rows = [(127.1, 311.7, 74.7), (269.5, 183.3, 246.1), (113.5, 271.9, 124.6)]
+    rng = np.random.default_rng(3)
+    pts = np.concatenate([rng.integers(-200, 200, (300, 3)).astype(np.float32),
+                          rng.uniform(-150, 150, (100, 3)).astype(np.float32)])
+    for p in pts:
+        want = []
+        for r in rows:
+            a = f(f(f(p[0] * f(r[0])) + f(p[1] * f(r[1]))) + f(p[2] * f(r[2])))
+            s = f(np.sin(np.float64(a)))
+            x = f(s * f(43758.5453123))
+            want.append(f(f(-1.0) + f(f(2.0) * f(x - np.floor(x)))))
+        got = oracle.noise_hash(p, 32)
+        assert np.array_equal(np.asarray(want, np.float64), got), (p, want, got)
