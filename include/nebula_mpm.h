@@ -157,17 +157,25 @@ typedef struct nb_mpm nb_mpm; /* opaque engine handle */
 
 /* FP64 SVD fallback for near-degenerate F in the Drucker-Prager family
  * (Drucker-Prager, granular powder, viscoplastic mud).  No counterpart in the
- * reference, whose shaders are FP32 throughout.  G2P takes every SVD in FP32;
- * when its singular values s0 >= s1 >= |s2| put two eigenvalues of F^T F
- * within gap_rel * s0^2 of each other, or s2^2 <= zero_rel * s0^2, the
- * particle's return map (SVD, projection and recomposition) and its Hencky
- * stress are computed again in FP64, with the same algorithm, and rounded to
- * FP32 once.  The stress SVD of a recomposed F is tested the same way.  FP32
- * rounds F^T F by about 1e-7 s0^2, so below these thresholds the singular
- * vectors of a close pair, and U = F v / sigma for a tiny sigma, are set by
- * rounding; the reference convention's F <- U^T exp(eps) V and
- * tau = U^T k U depend on them.  Particles above both thresholds take the
- * FP32 path only and their results are unchanged by the fallback.
+ * reference, whose shaders are FP32 throughout.  G2P takes every SVD in FP32.
+ * FP32 rounds F^T F by about 2^-22 s0^2, so for singular values
+ * s0 >= s1 >= |s2| the vectors of a pair with s_i^2 - s_j^2 <= gap_rel * s0^2
+ * are fixed only up to a rotation in their plane (of angle about
+ * 2^-22 s0^2 / (s_i^2 - s_j^2), at most 1), and U = F v / sigma is set by
+ * rounding when s2^2 <= zero_rel * s0^2.  The particle's return map (SVD,
+ * projection and recomposition) and its Hencky stress are computed again in
+ * FP64, with the same algorithm, and rounded to FP32 once, when s2 is that
+ * small, or when such a rotation of a close pair would move the FP32 F' or
+ * tau by more than 1e-6 of its scale (max |F'|; max(max |tau|,
+ * 2 mu + 3 lambda)).  The reference convention's F <- U^T exp(eps) V and
+ * tau = U^T k U change under the rotation by its angle times their
+ * commutator with the rotation's generator; the corrected convention's by
+ * the angle times the difference of the pair's values.  An F within rounding
+ * of a multiple of the identity (material at rest or in free fall) has every
+ * pair close but a result that commutes with every rotation, and stays on
+ * the FP32 path.  The stress SVD of a recomposed F is tested the same way.
+ * Particles that do not take the FP64 path give the same results bit for bit
+ * with the fallback on or off.
  * Defaults: enabled, gap_rel 1e-4, zero_rel 1e-6, count_stats 0. */
 typedef struct nb_mpm_svd_fallback {
   uint32_t enabled;
@@ -299,6 +307,12 @@ int nb_mpm_get_svd_fallback(const nb_mpm* h, nb_mpm_svd_fallback* f);
  * NB_SVD_FB_*), counted while count_stats is 1; reset != 0 zeroes them
  * afterwards.  Synchronises the engine's stream. */
 int nb_mpm_svd_fallback_stats(nb_mpm* h, uint64_t* out, int reset);
+/* on != 0: nb_mpm_substeps runs the grid update inside G2P (one kernel per
+ * occupied tile's particles, the tile's nodes in LDS) for every substep of a
+ * call but the last; results are the same bit for bit.  Off by default: on
+ * MI355X the fused kernel measures slower than the separate grid update and
+ * G2P.  An extension with no counterpart in the reference. */
+int nb_mpm_set_fused_g2p(nb_mpm* h, int on);
 int nb_mpm_get_config(const nb_mpm* h, nb_mpm_config* cfg);
 
 /* Domain helpers — src/core/Domain.h:14-39 and MPMConfig::totalCells. */

@@ -315,12 +315,14 @@ __global__ __launch_bounds__(256) void k_g2p_bwd(SubParams p, const float4* __re
     // (mass < 1e-10) read as zero (shaders/mpm_g2p.comp:93-101)
     auto node_v = [&](int ix, int iy, int iz) -> float4 {
       const int nx = base[0] + ix, ny = base[1] + iy, nz = base[2] + iz;
-      const bool in = nx >= 0 && nx < sg.grid_res[0] && ny >= 0 && ny < sg.grid_res[1] && nz >= 0 && nz < sg.grid_res[2];
       const uint32_t cx = uint32_t(min(max(nx, 0), sg.grid_res[0] - 1));
       const uint32_t cy = uint32_t(min(max(ny, 0), sg.grid_res[1] - 1));
       const uint32_t cz = uint32_t(min(max(nz, 0), sg.grid_res[2] - 1));
       const float4 g = grid_node[node_id(sg, cx, cy, cz)];
-      return (in && g.w >= 1e-10f) ? make_float4(g.x, g.y, g.z, 0.f) : make_float4(0.f, 0.f, 0.f, 0.f);
+      // bitwise tests: the load stays unconditional (one 16-byte load)
+      const bool live = (uint32_t(nx) < uint32_t(sg.grid_res[0])) & (uint32_t(ny) < uint32_t(sg.grid_res[1])) &
+                        (uint32_t(nz) < uint32_t(sg.grid_res[2])) & (g.w >= 1e-10f);
+      return live ? make_float4(g.x, g.y, g.z, 0.f) : make_float4(0.f, 0.f, 0.f, 0.f);
     };
     // The incoming adjoint of B is needed during the gather: the position
     // adjoint's gradient-times-offset terms are contracted with it there.

@@ -74,7 +74,7 @@ struct nb_mpm {
   uint32_t material_count = 1, collider_count = 0, force_count = 0, noise_force_count = 0;
   std::vector<nb_material> mats_host;  // host copy of the material table
   nb_mpm_svd_fallback svd_fb{1u, 1e-4f, 1e-6f, 0u};
-  bool fuse_g2p = true;  // the grid update inside G2P for all but a call's last substep
+  bool fuse_g2p = false;  // the grid update inside G2P for all but a call's last substep (nb_mpm_set_fused_g2p)
   uint32_t prof_calls[kNumProfiledKernels] = {};  // launches per kernel in the last nb_mpm_profile_substeps
   unsigned long long* d_svd_stats = nullptr;  // NB_SVD_FB_STATS counters
   bool has_sdf = false;
@@ -185,6 +185,9 @@ struct nb_mpm {
     p.svd_gap_rel = svd_fb.gap_rel;
     p.svd_zero_rel = svd_fb.zero_rel;
     p.svd_stats = svd_fb.count_stats ? d_svd_stats : nullptr;
+    p.fb_queue = d.fb_queue;
+    p.fb_ctl = d.fb_ctl;
+    p.fb_region = (n_live + kFbSlots - 1) / kFbSlots;
     p.tile_blocks = tuning.tile_blocks;
     p.g2p_block = tuning.g2p_block;
     p.p2g_item = tuning.p2g_item;
@@ -628,6 +631,8 @@ int engine_alloc(nb_mpm* h, void* hip_stream) {
   if (rc == NB_OK) rc = h->alloc(&d.forces, NB_MAX_FORCES);
   if (rc == NB_OK) rc = h->alloc(&h->d_svd_stats, NB_SVD_FB_STATS);
   if (rc == NB_OK) rc = h->alloc(&d.units, size_t(h->NT) + h->cap / kG2PUnit + 1);
+  if (rc == NB_OK) rc = h->alloc(&d.fb_queue, (size_t(h->cap) + kFbSlots) * kFbEntry);
+  if (rc == NB_OK) rc = h->alloc(&d.fb_ctl, kFbSlots * kFbCtlStride + 1);
   if (rc != NB_OK) return rc;
   d.sdf = nullptr;
 
@@ -637,7 +642,7 @@ int engine_alloc(nb_mpm* h, void* hip_stream) {
       zero(d.grid_node, cells * 16) != hipSuccess || zero(d.grid_old, cells * 16) != hipSuccess ||
       zero(d.materials, size_t(h->max_materials()) * 64) != hipSuccess ||
       zero(d.colliders, NB_MAX_COLLIDERS * 64) != hipSuccess || zero(d.forces, NB_MAX_FORCES * 32) != hipSuccess ||
-      zero(h->d_svd_stats, NB_SVD_FB_STATS * 8) != hipSuccess)
+      zero(h->d_svd_stats, NB_SVD_FB_STATS * 8) != hipSuccess || zero(d.fb_ctl, (kFbSlots * kFbCtlStride + 1) * 4) != hipSuccess)
     return fail(NB_ERR_HIP, "hipMemsetAsync failed");
   for (int f = 0; f < NB_NUM_FIELDS; f++)
     if (zero(d.field[f], size_t(std::max(h->cap, 1u)) * 16) != hipSuccess)
@@ -1021,6 +1026,11 @@ int nb_mpm_svd_fallback_stats(nb_mpm* h, uint64_t* out, int reset) {
     NB_HIP(hipStreamSynchronize(h->stream));
   }
   return rc;
+}
+int nb_mpm_set_fused_g2p(nb_mpm* h, int on) {
+  if (!h) return fail(NB_ERR_ARG, "null handle");
+  h->fuse_g2p = on != 0;
+  return NB_OK;
 }
 int nb_mpm_get_config(const nb_mpm* h, nb_mpm_config* c) {
   if (!h || !c) return fail(NB_ERR_ARG, "null argument");

@@ -39,7 +39,7 @@ namespace nb {
 
 const char* const kKernelNames[kNumProfiledKernels] = {
     "bin_clear", "bin_keys", "bin_mark", "bin_compact", "bin_tile_scan",
-    "bin_tile_offsets", "bin_scatter", "bin_stabilize", "p2g", "grid_update", "g2p", "g2p_grid"};
+    "bin_tile_offsets", "bin_scatter", "bin_stabilize", "p2g", "grid_update", "g2p", "g2p_grid", "g2p_fb"};
 
 // ---------------------------------------------------------------------------
 // BIN
@@ -917,17 +917,32 @@ __device__ __forceinline__ bool g2p_stencil(const SceneGeo& g, float xp, float y
   return (min_positive3(ax3[0].w) * min_positive3(ax3[1].w)) * min_positive3(ax3[2].w) < 1e-12f;
 }
 
-// The FP64 fallback's two computations, kept out of line: a near-degenerate
-// F is rare, and inlined FP64 code would raise the register count of every
-// G2P lane.  The return map (SVD, projection, recomposition) and the Hencky
-// stress of its result, from an FP32 F, rounded once; and the Hencky stress
-// alone.  Results come back by value, so the caller's F and tau stay in
-// registers.
+// The fallback's counters (NB_SVD_FB_*): one set of atomics per wave, from
+// ballots over the lanes that reach the call together.
+__device__ __forceinline__ void fb_count_stats(unsigned long long* st, bool dp_lane, bool fb_proj, bool fb_stress) {
+  const uint64_t b_dp = __ballot(dp_lane), b_pr = __ballot(fb_proj), b_st = __ballot(fb_stress);
+  if ((threadIdx.x & 63u) == uint32_t(__ffsll((unsigned long long)__ballot(true)) - 1)) {
+    if (b_dp) {
+      atomicAdd(&st[NB_SVD_FB_DP_LANES], (unsigned long long)__popcll(b_dp));
+      atomicAdd(&st[NB_SVD_FB_DP_WAVES], 1ull);
+    }
+    if (b_pr) atomicAdd(&st[NB_SVD_FB_PROJ_LANES], (unsigned long long)__popcll(b_pr));
+    if (b_st) atomicAdd(&st[NB_SVD_FB_STRESS_LANES], (unsigned long long)__popcll(b_st));
+    if (b_pr | b_st) atomicAdd(&st[NB_SVD_FB_FB_WAVES], 1ull);
+  }
+}
+
+// The FP64 fallback's computation, from an FP32 trial F: the return map
+// (SVD, projection, recomposition) and the Hencky stress of its result, in
+// FP64 and rounded once.  All of it stays FP64: the stress SVD of a
+// recomposed F' with a close pair is as sensitive to F' as the first SVD is
+// to F, so an F' rounded to FP32 (or built with FP32 exp) would decide its
+// singular vectors again.
 struct FTau {
   m3 F, tau;
 };
-__device__ __noinline__ FTau plastic_fp64(const m3 F, bool dp, float mu_f, float lam_f, float q_max, float M,
-                                          float q_c, bool cs) {
+__device__ __forceinline__ FTau plastic_fp64(const m3& F, bool dp, float mu_f, float lam_f, float q_max, float M,
+                                             float q_c, bool cs) {
   const double mu = mu_f, lam = lam_f;
   const PlasticOutT<double> pd =
       project_plastic(m3_convert<double>(F), dp, mu, lam, double(q_max), double(M), double(q_c), cs);
@@ -937,8 +952,54 @@ __device__ __noinline__ FTau plastic_fp64(const m3 F, bool dp, float mu_f, float
                                           : hencky_stress(pd.F, mu, lam, cs));
   return r;
 }
-__device__ __noinline__ m3 hencky_fp64(const m3 F, float mu, float lam, bool cs) {
-  return m3_round(hencky_stress(m3_convert<double>(F), double(mu), double(lam), cs));
+
+// The FP64 fallback's queue: kFbSlots slots of p.fb_region = ceil(n /
+// kFbSlots) entries; a particle at place k of the G2P order goes to slot
+// k / fb_region, so a slot never overflows and its atomics come from a
+// fraction of the waves.  An entry is kFbEntry float4: q, mode, then F_old
+// and the velocity gradient, 9 floats each in m[0][0], m[0][1], ..., m[2][2]
+// order (m[col][row] storage).  mode 1: return map and stress from the trial
+// F = (I + dt grad v) F_old; 2: stress of the record's F; 0: dropped.  Slot
+// counters sit kFbCtlStride words apart, the k_g2p_fb block count after them.
+constexpr uint32_t kFbNone = 0xFFFFFFFFu;
+
+// queue the particle if it takes the fallback (all lanes of the wave call
+// this together); returns its entry or kFbNone
+__device__ __forceinline__ uint32_t queue_fb(const SubParams& p, uint32_t k, uint32_t q, const float4* r,
+                                             const m3& vg, bool fb_proj, bool fb_stress) {
+  const bool fb = fb_proj || fb_stress;
+  uint64_t pending = __ballot(fb);
+  if (!pending) return kFbNone;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t slot = k / p.fb_region;
+  uint32_t at = kFbNone;
+  // the wave's queued lanes grouped by slot (one group unless the wave
+  // straddles a slot boundary); one atomic per group
+  while (pending) {
+    const uint32_t leader = uint32_t(__ffsll((unsigned long long)pending) - 1);
+    const uint32_t ls = uint32_t(__shfl(int(slot), int(leader)));
+    const uint64_t peers = __ballot(fb && slot == ls);
+    uint32_t base = 0u;
+    if (lane == leader) base = atomicAdd(&p.fb_ctl[ls * kFbCtlStride], uint32_t(__popcll(peers)));
+    base = uint32_t(__shfl(int(base), int(leader)));
+    if (fb && slot == ls) at = ls * p.fb_region + base + uint32_t(__popcll(peers & ((1ull << lane) - 1ull)));
+    pending &= ~peers;
+  }
+  if (fb) {
+    float4* e = p.fb_queue + size_t(at) * kFbEntry;
+    if (fb_proj) {
+      // F_old is still the record's (the record is written after this)
+      const float4 rt = r[R_T], fa = r[R_FA], fb4 = r[R_FB];
+      e[0] = make_float4(__uint_as_float(q), __uint_as_float(1u), rt.w, fa.x);
+      e[1] = make_float4(fa.y, fa.z, fa.w, fb4.x);
+      e[2] = make_float4(fb4.y, fb4.z, fb4.w, vg.m[0][0]);
+      e[3] = make_float4(vg.m[0][1], vg.m[0][2], vg.m[1][0], vg.m[1][1]);
+      e[4] = make_float4(vg.m[1][2], vg.m[2][0], vg.m[2][1], vg.m[2][2]);
+    } else {
+      e[0] = make_float4(__uint_as_float(q), __uint_as_float(2u), 0.f, 0.f);
+    }
+  }
+  return at;
 }
 
 // Everything of G2P after the gather (shaders/mpm_g2p.comp:115-272): F
@@ -946,8 +1007,16 @@ __device__ __noinline__ m3 hencky_fp64(const m3 F, float mu, float lam, bool cs)
 // family return map with the FP64 fallback for near-degenerate F), mud
 // viscosity, Jacobian reset, PIC/APIC/FLIP velocity, advection, SDF push-out,
 // world clamp; writes the record and the next substep's cell key.
+// A particle whose update the FP64 fallback takes over finishes here in FP32
+// with its stress set to the mud viscosity term alone (zero for the other
+// models) and is queued for k_g2p_fb (queue_fb), which replaces F and tau
+// with the FP64 results; if the Jacobian reset or the SDF push-out then sets
+// F = I, tau = 0 (both decided on the FP32 F), its entry is cancelled.  The
+// FP64 code thus stays out of the kernels that run every particle (a call
+// site alone costs such a kernel registers and time even when no lane takes
+// it).  k: the particle's place in the G2P order, which picks its queue slot.
 template <bool FLIP, bool BATCH>
-__device__ __forceinline__ void g2p_particle(const SubParams& p, uint32_t q, const float4* r, float4* ro,
+__device__ __forceinline__ void g2p_particle(const SubParams& p, uint32_t k, uint32_t q, const float4* r, float4* ro,
                                              uint32_t* __restrict__ keys, const nb_material& mat,
                                              const G2PGather& gth, const float4 pp, const float4 vel,
                                              const float* __restrict__ sdf) {
@@ -991,20 +1060,18 @@ __device__ __forceinline__ void g2p_particle(const SubParams& p, uint32_t q, con
     bool have_tau = false;
     if (plastic) {
       const PlasticOut po = project_plastic(Fn, dp, mat.mu, mat.lambda, mat.q_max, mat.M_friction, mat.q_cohesion, cs);
-      if (fb_on && svd_near_degenerate(po.sig, p.svd_gap_rel, p.svd_zero_rel)) {
-        // near-degenerate trial F: the return map and the stress in FP64
-        // from the same FP32 input, rounded once
-        const FTau r = plastic_fp64(Fn, dp, mat.mu, mat.lambda, mat.q_max, mat.M_friction, mat.q_cohesion, cs);
-        Fn = r.F;
-        tau = r.tau;
+      Fn = po.F;
+      if (cs || !po.recomposed) {
+        tau = hencky_from_log(po.U, po.eps[0], po.eps[1], po.eps[2], mat.mu, mat.lambda);
+        have_tau = true;
+      }
+      if (fb_on && fb_rounding_decides(po.sig, p.svd_gap_rel, p.svd_zero_rel, cs, po.recomposed, Fn, po.eps,
+                                       have_tau, tau, po.eps, mat.mu, mat.lambda)) {
+        // rounding decides the trial F's singular vectors and the result
+        // depends on them: k_g2p_fb redoes the return map and the stress in
+        // FP64 from the same FP32 trial F
         have_tau = true;
         fb_proj = true;
-      } else {
-        Fn = po.F;
-        if (cs || !po.recomposed) {
-          tau = hencky_from_log(po.U, po.eps[0], po.eps[1], po.eps[2], mat.mu, mat.lambda);
-          have_tau = true;
-        }
       }
     }
     if (!have_tau) {
@@ -1015,13 +1082,13 @@ __device__ __forceinline__ void g2p_particle(const SubParams& p, uint32_t q, con
       m3 U, V;
       float sig[3];
       svd3(Fn, U, sig, V, cs);
-      if (fb_on && svd_near_degenerate(sig, p.svd_gap_rel, p.svd_zero_rel)) {
-        tau = hencky_fp64(Fn, mat.mu, mat.lambda, cs);
+      const float e[3] = {log_strain(sig[0]), log_strain(sig[1]), log_strain(sig[2])};
+      tau = hencky_from_log(U, e[0], e[1], e[2], mat.mu, mat.lambda);
+      if (fb_on && fb_rounding_decides(sig, p.svd_gap_rel, p.svd_zero_rel, cs, false, Fn, e, true, tau, e, mat.mu,
+                                       mat.lambda))
         fb_stress = true;
-      } else {
-        tau = hencky_from_log(U, log_strain(sig[0]), log_strain(sig[1]), log_strain(sig[2]), mat.mu, mat.lambda);
-      }
     }
+    if (fb_proj || fb_stress) tau = m3_zero();  // k_g2p_fb adds the FP64 stress
     if (mat.model == NB_MAT_VISCOPLASTIC_MUD) {
       m3 D;
 #pragma unroll
@@ -1036,6 +1103,9 @@ __device__ __forceinline__ void g2p_particle(const SubParams& p, uint32_t q, con
         for (int rr = 0; rr < 3; rr++) tau.m[c][rr] += k2 * (D.m[c][rr] - (c == rr ? trD / 3.f : 0.f));
     }
   }
+  // every lane of the wave is here
+  if (p.svd_stats) fb_count_stats(p.svd_stats, dp_lane, fb_proj, fb_stress);
+  const uint32_t fb_at = p.svd_fp64 ? queue_fb(p, k, q, r, gth.vg, fb_proj, fb_stress) : kFbNone;
 
   bool jreset = false;
   {
@@ -1065,6 +1135,7 @@ __device__ __forceinline__ void g2p_particle(const SubParams& p, uint32_t q, con
   // the SVD (a plain engine re-reads its kernel arguments).
   const SceneGeo ge = particle_geo<BATCH>(p, q);
   // particle-level SDF push-out (shaders/mpm_g2p.comp:220-252)
+  bool pushed = false;
   if (sdf) {
     const int gx = ge.grid_res[0] - 1, gy = ge.grid_res[1] - 1, gz = ge.grid_res[2] - 1;
     float fx = fminf(fmaxf(cell_coord(ge, xn, 0), -4.f), float(ge.grid_res[0]) + 4.f);
@@ -1090,6 +1161,7 @@ __device__ __forceinline__ void g2p_particle(const SubParams& p, uint32_t q, con
       }
       Fn = m3_ident();
       tau = m3_zero();
+      pushed = true;
     }
   }
   // world clamp at a 1.5h margin (shaders/mpm_g2p.comp:255-263)
@@ -1113,18 +1185,8 @@ __device__ __forceinline__ void g2p_particle(const SubParams& p, uint32_t q, con
   ro[R_FA] = make_float4(Fn.m[0][1], Fn.m[0][2], Fn.m[1][0], Fn.m[1][1]);
   ro[R_FB] = make_float4(Fn.m[1][2], Fn.m[2][0], Fn.m[2][1], Fn.m[2][2]);
   keys[q] = cell_key(ge, xn, yn, zn);  // the next substep's bin key
-  if (p.svd_stats) {
-    const uint64_t b_dp = __ballot(dp_lane), b_pr = __ballot(fb_proj), b_st = __ballot(fb_stress);
-    if ((threadIdx.x & 63u) == uint32_t(__ffsll((unsigned long long)__ballot(true)) - 1)) {
-      if (b_dp) {
-        atomicAdd(&p.svd_stats[NB_SVD_FB_DP_LANES], (unsigned long long)__popcll(b_dp));
-        atomicAdd(&p.svd_stats[NB_SVD_FB_DP_WAVES], 1ull);
-      }
-      if (b_pr) atomicAdd(&p.svd_stats[NB_SVD_FB_PROJ_LANES], (unsigned long long)__popcll(b_pr));
-      if (b_st) atomicAdd(&p.svd_stats[NB_SVD_FB_STRESS_LANES], (unsigned long long)__popcll(b_st));
-      if (b_pr | b_st) atomicAdd(&p.svd_stats[NB_SVD_FB_FB_WAVES], 1ull);
-    }
-  }
+  // F and tau were set to I and 0: the fallback's entry is dropped
+  if (fb_at != kFbNone && (jreset || pushed)) reinterpret_cast<uint32_t*>(p.fb_queue + size_t(fb_at) * kFbEntry)[1] = 0u;
 }
 
 // the material of a particle: clamp to materialCount-1 (shaders/mpm_g2p.comp:128-130)
@@ -1133,32 +1195,12 @@ __device__ __forceinline__ uint32_t particle_material(const SubParams& p, const 
   return mid > p.material_count - 1u ? p.material_count - 1u : mid;
 }
 
-// G2P reading the updated grid from HBM, one thread per particle in bin order.
-// rec_out may equal rec_in: every record slot written is data-dependent on the
-// load of the same slot (or not read at all), so updating in place is safe.
-// 4 waves per SIMD (<= 128 VGPRs): the gather's latency needs the occupancy
+// G2P of particle q (place k of the G2P order) reading the updated grid from HBM.
 template <bool FLIP, bool BATCH>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_g2p(SubParams p,
-                                                                                   const float4* rec_in,
-                                                                                   float4* rec_out,
-                                             uint32_t* __restrict__ keys,
-                                             const nb_material* __restrict__ mats,
-                                             const uint32_t* __restrict__ sorted,
-                                             const float4* __restrict__ grid_node,
-                                             const float4* __restrict__ grid_old, const float* __restrict__ sdf) {
-  // a plain engine's material table is staged in LDS; a batch's (up to
-  // kBatchMaxMaterials) is read through the caches
-  __shared__ nb_material s_mats[BATCH ? 1 : NB_MAX_MATERIALS];
-  if constexpr (!BATCH) {
-    float* dst = reinterpret_cast<float*>(s_mats);
-    const float* src = reinterpret_cast<const float*>(mats);
-    const uint32_t nf = p.material_count * uint32_t(sizeof(nb_material) / 4);
-    for (uint32_t e = threadIdx.x; e < nf; e += blockDim.x) dst[e] = src[e];
-    __syncthreads();
-  }
-  const uint32_t k = xcd_chunk(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (k >= p.n) return;
-  const uint32_t q = sorted ? sorted[k] : k;
+__device__ __forceinline__ void g2p_one(const SubParams& p, uint32_t k, uint32_t q, const float4* rec_in, float4* rec_out,
+                                        uint32_t* __restrict__ keys, const nb_material* __restrict__ mats,
+                                        const nb_material* s_mats, const float4* __restrict__ grid_node,
+                                        const float4* __restrict__ grid_old, const float* __restrict__ sdf) {
   const SceneGeo g = particle_geo<BATCH>(p, q);
   const float4* r = rec_in + size_t(q) * kRec;
   float4* ro = rec_out + size_t(q) * kRec;
@@ -1175,8 +1217,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const uint32_t cz = uint32_t(min(max(nz, 0), g.grid_res[2] - 1));
     const uint32_t id = node_id(g, cx, cy, cz);
     const float4 gn = grid_node[id];
-    const bool live = nx >= 0 && nx < g.grid_res[0] && ny >= 0 && ny < g.grid_res[1] && nz >= 0 &&
-                      nz < g.grid_res[2] && gn.w >= 1e-10f;
+    // bitwise, not short-circuit: the load then has a use on every path and
+    // stays one unconditional 16-byte load, issued with the slab's others
+    const bool live = (uint32_t(nx) < uint32_t(g.grid_res[0])) & (uint32_t(ny) < uint32_t(g.grid_res[1])) &
+                      (uint32_t(nz) < uint32_t(g.grid_res[2])) & (gn.w >= 1e-10f);
     v[0] = live ? gn.x : 0.f;
     v[1] = live ? gn.y : 0.f;
     v[2] = live ? gn.z : 0.f;
@@ -1193,7 +1237,113 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   const float4 vel = r[R_V];
   const uint32_t mid = particle_material(p, vel);
   const nb_material mat = BATCH ? mats[mid] : s_mats[mid];
-  g2p_particle<FLIP, BATCH>(p, q, r, ro, keys, mat, gth, pp, vel, sdf);
+  g2p_particle<FLIP, BATCH>(p, k, q, r, ro, keys, mat, gth, pp, vel, sdf);
+}
+
+// a plain engine's material table is staged in LDS; a batch's (up to
+// kBatchMaxMaterials) is read through the caches
+template <bool BATCH>
+__device__ __forceinline__ void stage_materials(const SubParams& p, const nb_material* __restrict__ mats,
+                                                nb_material* s_mats) {
+  if constexpr (!BATCH) {
+    float* dst = reinterpret_cast<float*>(s_mats);
+    const float* src = reinterpret_cast<const float*>(mats);
+    const uint32_t nf = p.material_count * uint32_t(sizeof(nb_material) / 4);
+    for (uint32_t e = threadIdx.x; e < nf; e += blockDim.x) dst[e] = src[e];
+    __syncthreads();
+  }
+}
+
+// G2P reading the updated grid from HBM, one thread per particle in bin order.
+// rec_out may equal rec_in: every record slot written is data-dependent on the
+// load of the same slot (or not read at all), so updating in place is safe.
+// 4 waves per SIMD (<= 128 VGPRs): the gather's latency needs the occupancy
+template <bool FLIP, bool BATCH>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_g2p(SubParams p,
+                                                                                   const float4* rec_in,
+                                                                                   float4* rec_out,
+                                             uint32_t* __restrict__ keys,
+                                             const nb_material* __restrict__ mats,
+                                             const uint32_t* __restrict__ sorted,
+                                             const float4* __restrict__ grid_node,
+                                             const float4* __restrict__ grid_old, const float* __restrict__ sdf) {
+  __shared__ nb_material s_mats[BATCH ? 1 : NB_MAX_MATERIALS];
+  stage_materials<BATCH>(p, mats, s_mats);
+  const uint32_t k = xcd_chunk(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (k >= p.n) return;
+  const uint32_t q = sorted ? sorted[k] : k;
+  g2p_one<FLIP, BATCH>(p, k, q, rec_in, rec_out, keys, mats, s_mats, grid_node, grid_old, sdf);
+}
+
+// The FP64 fallback's second half: for every entry G2P queued (queue_fb),
+// the return map and Hencky stress in FP64 from the trial F (mode 1), or the
+// Hencky stress of the record's F (mode 2), rounded once; F is replaced and
+// the stress added to the record's (the mud viscosity term, or zero).  The
+// blocks of slot s are s, s + kFbSlots, ...; each reads its slot's length,
+// strides over the slot, then counts itself done, and the last block zeroes
+// the lengths and the count for the next G2P (every block has read its
+// length by then).
+template <bool BATCH>
+__global__ __launch_bounds__(256) void k_g2p_fb(SubParams p, float4* rec, const nb_material* __restrict__ mats) {
+  __shared__ nb_material s_mats[BATCH ? 1 : NB_MAX_MATERIALS];
+  stage_materials<BATCH>(p, mats, s_mats);
+  const uint32_t slot = blockIdx.x % kFbSlots, part = blockIdx.x / kFbSlots, parts = gridDim.x / kFbSlots;
+  const uint32_t len = p.fb_ctl[slot * kFbCtlStride];
+  const bool cs = p.corrected_svd != 0u;
+  for (uint32_t i = part * blockDim.x + threadIdx.x; i < len; i += parts * blockDim.x) {
+    const float4* e = p.fb_queue + (size_t(slot) * p.fb_region + i) * kFbEntry;
+    const float4 e0 = e[0];
+    const uint32_t q = __float_as_uint(e0.x), mode = __float_as_uint(e0.y);
+    if (mode == 0u) continue;
+    float4* ro = rec + size_t(q) * kRec;
+    const uint32_t mid = particle_material(p, ro[R_V]);
+    const nb_material mat = BATCH ? mats[mid] : s_mats[mid];
+    const bool dp = true;  // only the Drucker-Prager family is queued
+    const float4 rt = ro[R_T];
+    float* b0 = reinterpret_cast<float*>(ro + R_B0);
+    float* b1 = reinterpret_cast<float*>(ro + R_B1);
+    float* b2 = reinterpret_cast<float*>(ro + R_B2);
+    m3 F, tau;
+    if (mode == 1u) {
+      const float4 e1 = e[1], e2 = e[2], e3 = e[3], e4 = e[4];
+      m3 Fo, L;
+      Fo.m[0][0] = e0.z; Fo.m[0][1] = e0.w; Fo.m[0][2] = e1.x;
+      Fo.m[1][0] = e1.y; Fo.m[1][1] = e1.z; Fo.m[1][2] = e1.w;
+      Fo.m[2][0] = e2.x; Fo.m[2][1] = e2.y; Fo.m[2][2] = e2.z;
+      const float vg[9] = {e2.w, e3.x, e3.y, e3.z, e3.w, e4.x, e4.y, e4.z, e4.w};
+#pragma unroll
+      for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int rr = 0; rr < 3; rr++) L.m[c][rr] = (c == rr ? 1.f : 0.f) + p.dt * vg[c * 3 + rr];
+      const FTau ft = plastic_fp64(m3_mul(L, Fo), dp, mat.mu, mat.lambda, mat.q_max, mat.M_friction,
+                                   mat.q_cohesion, cs);
+      F = ft.F;
+      tau = ft.tau;
+    } else {
+      const float4 fa = ro[R_FA], fb = ro[R_FB];
+      F.m[0][0] = rt.w; F.m[0][1] = fa.x; F.m[0][2] = fa.y;
+      F.m[1][0] = fa.z; F.m[1][1] = fa.w; F.m[1][2] = fb.x;
+      F.m[2][0] = fb.y; F.m[2][1] = fb.z; F.m[2][2] = fb.w;
+      tau = m3_round(hencky_stress(m3_convert<double>(F), double(mat.mu), double(mat.lambda), cs));
+    }
+    // the record's stress: the viscosity term G2P left there
+    ro[R_T] = make_float4(tau.m[0][0] + rt.x, tau.m[1][1] + rt.y, tau.m[2][2] + rt.z, F.m[0][0]);
+    b0[3] = tau.m[0][1] + b0[3];
+    b1[3] = tau.m[0][2] + b1[3];
+    b2[3] = tau.m[1][2] + b2[3];
+    if (mode == 1u) {
+      ro[R_FA] = make_float4(F.m[0][1], F.m[0][2], F.m[1][0], F.m[1][1]);
+      ro[R_FB] = make_float4(F.m[1][2], F.m[2][0], F.m[2][1], F.m[2][2]);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(&p.fb_ctl[kFbSlots * kFbCtlStride], 1u) == gridDim.x - 1u) {
+      for (uint32_t t = 0; t < kFbSlots; t++) atomicExch(&p.fb_ctl[t * kFbCtlStride], 0u);
+      atomicExch(&p.fb_ctl[kFbSlots * kFbCtlStride], 0u);
+    }
+  }
 }
 
 // Grid update and G2P fused, one block per work unit: a unit is up to
@@ -1301,7 +1451,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
       const float4 vel = r[R_V];
       const uint32_t mid = particle_material(p, vel);
       const nb_material mat = BATCH ? mats[mid] : s_mats[mid];
-      g2p_particle<FLIP, BATCH>(p, q, r, ro, keys, mat, gth, pp, vel, sdf);
+      g2p_particle<FLIP, BATCH>(p, k, q, r, ro, keys, mat, gth, pp, vel, sdf);
     }
   }
 }
@@ -1480,7 +1630,6 @@ void launch_grid(const DevState& d, const SubParams& p, uint32_t n_tiles, bool f
 // (not with turbulence or noise forces: their noise code would halve the
 // kernel's occupancy; those substeps run k_grid + k_g2p).
 void launch_g2p_fused(const DevState& d, const SubParams& p, uint32_t n_tiles, hipStream_t s, Prof* pf) {
-  ProfScope ps(pf, 11, s);
   // units <= occupied tiles + n / kG2PUnit; blocks stride over them
   const uint32_t max_units = n_tiles + p.n / kG2PUnit + 1u;
   const uint32_t g = tile_grid(max_units, p.tile_blocks);
@@ -1489,30 +1638,49 @@ void launch_g2p_fused(const DevState& d, const SubParams& p, uint32_t n_tiles, h
   k_g2p_tile<FL, BT><<<g, 256, 0, s>>>(p, d.rec, d.rec, d.keys, d.materials, d.sorted, d.units, d.act_tiles, \
                                        d.tile_start, d.tile_total, d.counters, d.tile_act, d.partial, d.sdf, \
                                        d.colliders, d.forces)
-  if (bt) {
-    if (flip) NB_G2PT(true, true); else NB_G2PT(false, true);
-  } else {
-    if (flip) NB_G2PT(true, false); else NB_G2PT(false, false);
+  {
+    ProfScope ps(pf, 11, s);
+    if (bt) {
+      if (flip) NB_G2PT(true, true); else NB_G2PT(false, true);
+    } else {
+      if (flip) NB_G2PT(true, false); else NB_G2PT(false, false);
+    }
   }
 #undef NB_G2PT
+  launch_g2p_fb(d, p, d.rec, s, pf);
+}
+
+// The FP64 fallback's fix-up after a G2P (nothing to do when it is off).
+void launch_g2p_fb(const DevState& d, const SubParams& p, float4* rec, hipStream_t s, Prof* pf) {
+  if (!p.svd_fp64) return;
+  ProfScope ps(pf, 12, s);
+  // kFbSlots groups of blocks, sized to the queue's capacity
+  const uint32_t parts = std::max(1u, std::min(cdiv(p.n, 256u), 1024u) / kFbSlots);
+  if (p.n_scenes > 0)
+    k_g2p_fb<true><<<kFbSlots * parts, 256, 0, s>>>(p, rec, d.materials);
+  else
+    k_g2p_fb<false><<<kFbSlots * parts, 256, 0, s>>>(p, rec, d.materials);
 }
 
 void launch_g2p(const DevState& d, const SubParams& p, bool use_sorted, hipStream_t s, Prof* pf, float4* rec_out) {
-  ProfScope ps(pf, 10, s);
   const uint32_t* srt = use_sorted ? d.sorted : nullptr;
   float4* out = rec_out ? rec_out : d.rec;
+  const bool flip = p.flip_ratio > 0.01f, bt = p.n_scenes > 0;
+  {
+    ProfScope ps(pf, 10, s);
 #define NB_G2P_LAUNCH(FL, BT) \
   k_g2p<FL, BT><<<cdiv(p.n, bs), bs, 0, s>>>(p, d.rec, out, d.keys, d.materials, srt, d.grid_node, d.grid_old, d.sdf)
-  const bool flip = p.flip_ratio > 0.01f;
-  const uint32_t bs = (p.g2p_block == 64u || p.g2p_block == 128u) ? p.g2p_block : 256u;  // launch bound 256
-  if (p.n_scenes > 0) {
-    if (flip) NB_G2P_LAUNCH(true, true);
-    else NB_G2P_LAUNCH(false, true);
-  } else {
-    if (flip) NB_G2P_LAUNCH(true, false);
-    else NB_G2P_LAUNCH(false, false);
-  }
+    const uint32_t bs = (p.g2p_block == 64u || p.g2p_block == 128u) ? p.g2p_block : 256u;  // launch bound 256
+    if (bt) {
+      if (flip) NB_G2P_LAUNCH(true, true);
+      else NB_G2P_LAUNCH(false, true);
+    } else {
+      if (flip) NB_G2P_LAUNCH(true, false);
+      else NB_G2P_LAUNCH(false, false);
+    }
 #undef NB_G2P_LAUNCH
+  }
+  launch_g2p_fb(d, p, out, s, pf);
 }
 
 // ---------------------------------------------------------------------------

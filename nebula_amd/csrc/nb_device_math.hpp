@@ -376,17 +376,89 @@ __device__ __forceinline__ void svd3(const mat3<T>& F, mat3<T>& U, T sig[3], mat
 
 // ---- near-degenerate F: the FP64 fallback's test ----
 // On singular values sorted by magnitude (s0 >= s1 >= |s2|) the eigenvalues
-// of F^T F are s_i^2.  FP32 rounds F^T F's entries by about 1e-7 s0^2, so the
-// singular vectors of a pair whose eigenvalues are closer than gap_rel * s0^2
-// are set by rounding (their angle errs by about 1e-7 / gap_rel), and
-// U = F v / sigma loses about 1e-7 s0 / s2 for a small s2.  Either makes the
-// reference convention's F <- U^T exp(eps) V and tau = U^T k U (which are not
-// isotropic functions of F) differ from the FP64 result by far more than
-// rounding, so such an F takes the SVD in FP64.
-__device__ __forceinline__ bool svd_near_degenerate(const float sig[3], float gap_rel, float zero_rel) {
+// of F^T F are s_i^2.  FP32 rounds F^T F's entries by about 2^-22 s0^2, so
+// the singular vectors of a close pair (i, j) are fixed only up to a rotation
+// in their plane of angle theta_ij ~ min(1, 2^-22 s0^2 / (s_i^2 - s_j^2)),
+// and U = F v / sigma loses about 1e-7 s0 / s2 for a small s2.  Rotating the
+// pair (U -> U G, V -> V G) changes
+//   the reference convention's X = U^T D V (F' = U^T exp(eps) V and
+//   tau = U^T k U, not isotropic functions of F) to G^T X G, by
+//   theta [X, K_ij] (K_ij = e_i e_j^T - e_j e_i^T), and
+//   the corrected convention's X = U D V^T by theta U [K_ij, D] V^T, whose
+//   entries are theta (d_i - d_j).
+// A lane takes the FP64 path when s2^2 <= zero_rel s0^2, or when a pair
+// with s_i^2 - s_j^2 <= gap_rel s0^2 moves F' or tau by more than
+// kFbRoundTol of its scale (max |F'|; max(max |tau|, 2 mu + 3 lambda), the
+// stress of a unit strain).  Elsewhere rounding does not decide the result:
+// an F within rounding of a multiple of the identity (a body at rest or in
+// free fall) has all three pairs close, but there F' and tau are within
+// rounding of multiples of the identity too, and commute with every K_ij.
+constexpr float kFbRound = 2.3841858e-7f;  // 2^-22
+constexpr float kFbRoundTol = 1e-6f;
+
+// rotation angles of the pairs (0, 1), (1, 2), (0, 2); 0 for a pair whose
+// gap exceeds gap_rel (an equal pair: 1)
+__device__ __forceinline__ void fb_pair_angles(const float sig[3], float gap_rel, float th[3]) {
   const float s0 = sig[0] * sig[0], s1 = sig[1] * sig[1], s2 = sig[2] * sig[2];
-  const float g = gap_rel * s0;
-  return (s0 - s1) <= g || (s1 - s2) <= g || s2 <= zero_rel * s0;
+  const float g[3] = {s0 - s1, s1 - s2, s0 - s2};
+#pragma unroll
+  for (int k = 0; k < 3; k++) th[k] = g[k] <= gap_rel * s0 ? fminf(1.f, kFbRound * s0 / fmaxf(g[k], 0.f)) : 0.f;
+}
+
+// max |(X K_ij - K_ij X)(a, b)| with X(a, b) = X.m[b][a]
+template <int I, int J>
+__device__ __forceinline__ float comm_pair(const m3& X) {
+  float c = 0.f;
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int b = 0; b < 3; b++) {
+      const float xk = (b == J ? X.m[I][a] : 0.f) - (b == I ? X.m[J][a] : 0.f);
+      const float kx = (a == I ? X.m[b][J] : 0.f) - (a == J ? X.m[b][I] : 0.f);
+      c = fmaxf(c, fabsf(xk - kx));
+    }
+  return c;
+}
+
+__device__ __forceinline__ float m3_maxabs(const m3& X) {
+  float c = 0.f;
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int b = 0; b < 3; b++) c = fmaxf(c, fabsf(X.m[a][b]));
+  return c;
+}
+
+// Whether rounding decides the FP32 result of an SVD-based update (see
+// above).  sig: that SVD's singular values; cs: corrected convention;
+// with_F: F' = Fp was recomposed from this SVD (projected log strains eps_F);
+// with_tau: tau was formed from this SVD's U (projected log strains eps_T).
+__device__ __forceinline__ bool fb_rounding_decides(const float sig[3], float gap_rel, float zero_rel, bool cs,
+                                                    bool with_F, const m3& Fp, const float eps_F[3], bool with_tau,
+                                                    const m3& tau, const float eps_T[3], float mu, float lam) {
+  const float s0 = sig[0] * sig[0];
+  if (sig[2] * sig[2] <= zero_rel * s0) return true;
+  float th[3];
+  fb_pair_angles(sig, gap_rel, th);
+  if (th[0] == 0.f && th[1] == 0.f && th[2] == 0.f) return false;
+  const float tF = kFbRoundTol * m3_maxabs(Fp);
+  const float tT = kFbRoundTol * fmaxf(m3_maxabs(tau), 2.f * mu + 3.f * lam);
+  constexpr int PI[3] = {0, 1, 0}, PJ[3] = {1, 2, 2};
+  bool hit = false;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    if (th[k] == 0.f) continue;
+    float dF, dT;
+    if (cs) {
+      dF = fabsf(expf(eps_F[PI[k]]) - expf(eps_F[PJ[k]]));
+      dT = 2.f * mu * fabsf(eps_T[PI[k]] - eps_T[PJ[k]]);
+    } else {
+      dF = k == 0 ? comm_pair<0, 1>(Fp) : k == 1 ? comm_pair<1, 2>(Fp) : comm_pair<0, 2>(Fp);
+      dT = k == 0 ? comm_pair<0, 1>(tau) : k == 1 ? comm_pair<1, 2>(tau) : comm_pair<0, 2>(tau);
+    }
+    hit = hit || (with_F && th[k] * dF > tF) || (with_tau && th[k] * dT > tT);
+  }
+  return hit;
 }
 
 // ---- Hencky Kirchhoff stress (shaders/mpm_common.glsl:284-293) ----
@@ -425,12 +497,11 @@ struct PlasticOutT {
   bool recomposed;  // false: F was returned as given, and U, eps are its own SVD's
 };
 using PlasticOut = PlasticOutT<float>;
+// The return map on a given SVD of F (U, sig, V as svd3 returns them).
 template <typename T>
-__device__ __forceinline__ PlasticOutT<T> project_plastic(const mat3<T>& F, bool dp, T mu, T lam, T q_max, T M, T q_c,
-                                                          bool corrected) {
-  mat3<T> U, V;
-  T sig[3];
-  svd3(F, U, sig, V, corrected);
+__device__ __forceinline__ PlasticOutT<T> project_plastic_svd(const mat3<T>& F, const mat3<T>& U, const T sig[3],
+                                                              const mat3<T>& V, bool dp, T mu, T lam, T q_max, T M,
+                                                              T q_c) {
   T e0 = log_strain(sig[0]), e1 = log_strain(sig[1]), e2 = log_strain(sig[2]);
   const T tr = e0 + e1 + e2;
   const T m = tr / T(3);
@@ -478,6 +549,14 @@ __device__ __forceinline__ PlasticOutT<T> project_plastic(const mat3<T>& F, bool
     o.eps[2] = e2;
   }
   return o;
+}
+template <typename T>
+__device__ __forceinline__ PlasticOutT<T> project_plastic(const mat3<T>& F, bool dp, T mu, T lam, T q_max, T M, T q_c,
+                                                          bool corrected) {
+  mat3<T> U, V;
+  T sig[3];
+  svd3(F, U, sig, V, corrected);
+  return project_plastic_svd(F, U, sig, V, dp, mu, lam, q_max, M, q_c);
 }
 
 }  // namespace nb

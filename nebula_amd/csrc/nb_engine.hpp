@@ -101,11 +101,16 @@ struct SubParams {
   uint32_t noise_forces;   // turbulence or noise records among the forces
   // FP64 SVD of near-degenerate F for the Drucker-Prager family (see
   // nb_mpm_svd_fallback): on/off, the two relative thresholds, and the
-  // counters G2P adds to when non-null (NB_SVD_FB_STATS 64-bit words)
+  // counters G2P adds to when non-null (NB_SVD_FB_STATS 64-bit words).
+  // G2P queues the particles the fallback takes over (fb_queue, slot
+  // lengths in fb_ctl); k_g2p_fb then replaces their F and tau.
   uint32_t svd_fp64;
   float svd_gap_rel;
   float svd_zero_rel;
   unsigned long long* svd_stats;
+  float4* fb_queue;
+  uint32_t* fb_ctl;
+  uint32_t fb_region;  // entries per queue slot: ceil(n / kFbSlots)
   // batch (n_scenes > 0): the scene table, the scene of every tile and of
   // every particle; the single-scene fields above are then unused
   // launch shape (nb_mpm_tuning): blocks of the per-tile kernels, threads
@@ -118,6 +123,10 @@ struct SubParams {
   const uint32_t* tile_scene;
   const uint32_t* pscene;
 };
+// the FP64 fallback's queue: slots, float4 per entry, words between slot counters
+constexpr uint32_t kFbSlots = 64;
+constexpr uint32_t kFbEntry = 5;
+constexpr uint32_t kFbCtlStride = 64;
 // materials a batch's table may hold (a plain engine: NB_MAX_MATERIALS)
 constexpr uint32_t kBatchMaxMaterials = NB_MAX_BATCH_MATERIALS;
 
@@ -140,6 +149,8 @@ struct DevState {
   uint32_t* tile_start;   // NT
   uint32_t* counters;     // CNT_SLOTS
   uint2* units;           // NT + cap / kG2PUnit + 1: (active tile, chunk) work units of k_g2p_tile
+  float4* fb_queue;       // (cap + kFbSlots) * kFbEntry: particles G2P leaves to k_g2p_fb (FP64 fallback)
+  uint32_t* fb_ctl;       // kFbSlots * kFbCtlStride + 1: slot lengths, k_g2p_fb blocks done (zero between substeps)
   float4* partial;        // cap_tiles * 1000
   float4* grid_node;      // NT*512
   float4* grid_old;       // NT*512
@@ -151,7 +162,7 @@ struct DevState {
 };
 
 // names of the kernels, in the order the profiler indexes them
-constexpr int kNumProfiledKernels = 12;
+constexpr int kNumProfiledKernels = 13;
 extern const char* const kKernelNames[kNumProfiledKernels];
 
 // Optional per-kernel timing: HIP events recorded around each launch on the
@@ -183,6 +194,7 @@ void launch_grid(const DevState& d, const SubParams& p, uint32_t n_tiles, bool f
 // Grid update and G2P in one kernel (k_g2p_tile), from the partials P2G wrote
 // and the work units the bin listed; the grid arrays are not written.
 void launch_g2p_fused(const DevState& d, const SubParams& p, uint32_t n_tiles, hipStream_t s, Prof* pf);
+void launch_g2p_fb(const DevState& d, const SubParams& p, float4* rec, hipStream_t s, Prof* pf);
 void launch_g2p(const DevState& d, const SubParams& p, bool use_sorted, hipStream_t s, Prof* pf,
                 float4* rec_out = nullptr);
 void launch_sdf_sphere(float* sdf, const SubParams& p, float r, float cx, float cy, float cz, hipStream_t s);

@@ -423,7 +423,7 @@ EXPORTED_SYMBOLS = (
     "nb_mpm_add_emitter", "nb_mpm_clear_emitters", "nb_mpm_emit", "nb_mpm_get_emitter",
     "nb_mpm_set_collider_mesh", "nb_mpm_get_collider_sdf",
     "nb_mpm_set_svd_fallback", "nb_mpm_get_svd_fallback", "nb_mpm_svd_fallback_stats", "nb_mpm_autotune_cached",
-    "nb_mpm_profile_calls",
+    "nb_mpm_profile_calls", "nb_mpm_set_fused_g2p",
 )
 
 
@@ -610,6 +610,7 @@ def lib():
         L.nb_mpm_set_svd_fallback.argtypes = [vp, C.POINTER(SvdFallback)]
         L.nb_mpm_get_svd_fallback.argtypes = [vp, C.POINTER(SvdFallback)]
         L.nb_mpm_svd_fallback_stats.argtypes = [vp, C.POINTER(C.c_uint64), C.c_int]
+        L.nb_mpm_set_fused_g2p.argtypes = [vp, C.c_int]
         L.nb_mpm_add_emitter.argtypes = [vp, C.POINTER(EmitterGPU)]
         L.nb_mpm_set_collider_mesh.argtypes = [vp, fp, C.c_uint32, C.c_int]
         L.nb_mpm_get_collider_sdf.argtypes = [vp, fp]
@@ -754,6 +755,11 @@ class MPMEngine:
         _check(lib().nb_mpm_autotune(self._h, dt_sub, trial_substeps, TUNE_ROUNDING if rounding else 0,
                                      C.byref(t)))
         return t
+
+    def setFusedG2P(self, on: bool):
+        """Run the grid update inside G2P for all but a call's last substep
+        (nb_mpm_set_fused_g2p; off by default)."""
+        _check(lib().nb_mpm_set_fused_g2p(self._h, int(bool(on))))
 
     # the FP64 SVD fallback ---------------------------------------------------
     def svdFallback(self) -> "SvdFallback":

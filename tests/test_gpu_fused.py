@@ -20,6 +20,7 @@ def _same(a, b):
 
 def _one_call_vs_single_calls(make, dt, n, prof_check=None):
     a, b = make(), make()
+    a.setFusedG2P(True)
     a.substep(dt, n)
     a.sync()
     for _ in range(n):
