@@ -67,6 +67,24 @@ typedef struct nb_hash_grid {
 int nb_hash_build(const nb_hash_grid* grid, const float* pred_pos4, const uint32_t* type_flag, uint32_t n,
                   uint32_t* cell_count, uint32_t* cell_offset, uint32_t* sorted, void* hip_stream);
 
+/* Bytes of the per-GPU scratch array nb_hash_build_noalloc uses: the chunk
+ * sums of the largest hash (NB_HASH_MAX_CELLS / 4096 words) plus the list of
+ * crowded cells (65,600 words), 1.25 MiB. */
+#define NB_HASH_NOALLOC_SCRATCH_BYTES 1310976u
+
+/* nb_hash_build that allocates nothing.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, results, limits and errors as nb_hash_build; the scratch is a
+ * file-scope device array of NB_HASH_NOALLOC_SCRATCH_BYTES that each GPU
+ * holds from the first call on it until the process ends, instead of a
+ * hipMallocAsync.  Every call on one GPU shares that GPU's array, so calls
+ * that overlap on one GPU (on streams not ordered with each other) race; calls
+ * on one stream, or on synchronised streams, do not.  A hash larger than the
+ * array serves (hash_cells > NB_HASH_MAX_CELLS) is refused with NB_ERR_ARG. */
+int nb_hash_build_noalloc(const nb_hash_grid* grid, const float* pred_pos4, const uint32_t* type_flag, uint32_t n,
+                          uint32_t* cell_count, uint32_t* cell_offset, uint32_t* sorted, void* hip_stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -411,3 +411,64 @@ extern "C" int nb_hash_build(const nb_hash_grid* grid, const float* pred_pos4, c
   if (e != hipSuccess) return nb::set_error(NB_ERR_HIP, "nb_hash_build: hipFreeAsync failed: %s", hipGetErrorString(e));
   return NB_OK;
 }
+
+// ---- nb_hash_build_noalloc: the same build on a file-scope scratch array ----
+// The array lives in every GPU's copy of this code object.  Host code never
+// passes its name to a kernel (on the host the name is a stand-in, not a
+// device address): the device address comes from hipGetSymbolAddress, once per
+// device, with that device current, and is kept per device.
+#include <mutex>
+
+namespace nbh_noalloc {
+
+static __device__ __align__(256) uint32_t scratch[nbh::kScratchWords];
+static_assert(sizeof(uint32_t) * nbh::kScratchWords == NB_HASH_NOALLOC_SCRATCH_BYTES,
+              "NB_HASH_NOALLOC_SCRATCH_BYTES must equal the scratch the build uses");
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_mu;
+static uint32_t* g_ptr[kMaxDevices] = {};
+
+// The current device's address of `scratch` into *out (looked up on the
+// device's first use); NB_OK or an error code with the message set.
+static int device_scratch(uint32_t** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess)
+    return nb::set_error(NB_ERR_HIP, "nb_hash_build_noalloc: hipGetDevice failed: %s", hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxDevices)
+    return nb::set_error(NB_ERR_RANGE, "nb_hash_build_noalloc: device %d is beyond the %d this library serves", dev,
+                         kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_mu);
+  if (!g_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return nb::set_error(NB_ERR_HIP, "nb_hash_build_noalloc: hipGetSymbolAddress of the scratch array failed: %s",
+                           hipGetErrorString(e));
+    g_ptr[dev] = static_cast<uint32_t*>(p);
+  }
+  *out = g_ptr[dev];
+  return NB_OK;
+}
+
+}  // namespace nbh_noalloc
+
+extern "C" int nb_hash_build_noalloc(const nb_hash_grid* grid, const float* pred_pos4, const uint32_t* type_flag,
+                                     uint32_t n, uint32_t* cell_count, uint32_t* cell_offset, uint32_t* sorted,
+                                     void* hip_stream) {
+  // validate_hash_build refuses any hash_cells other than the Morton cube of
+  // a grid of at most 1024 per axis, so hash_cells <= NB_HASH_MAX_CELLS, the
+  // largest hash the array's chunk sums cover; checked again here because the
+  // array's size depends on it.
+  const int v = nb::validate_hash_build(grid, pred_pos4, n, cell_count, cell_offset, sorted);
+  if (v != NB_OK) return v;
+  if (grid->hash_cells > NB_HASH_MAX_CELLS)
+    return nb::set_error(NB_ERR_ARG, "nb_hash_build_noalloc: hash_cells = %u exceeds the scratch array's %u",
+                         grid->hash_cells, NB_HASH_MAX_CELLS);
+  uint32_t* s = nullptr;
+  const int rc = nbh_noalloc::device_scratch(&s);
+  if (rc != NB_OK) return rc;
+  return nb::enqueue_hash_build(grid, pred_pos4, type_flag, n, cell_count, cell_offset, sorted, s,
+                                static_cast<hipStream_t>(hip_stream));
+}

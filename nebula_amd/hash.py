@@ -40,6 +40,7 @@ def _lib():
     if not _bound:
         vp, up = C.c_void_p, C.POINTER(C.c_uint32)
         L.nb_hash_build.argtypes = [C.POINTER(HashGrid), vp, vp, C.c_uint32, vp, vp, vp, vp]
+        L.nb_hash_build_noalloc.argtypes = [C.POINTER(HashGrid), vp, vp, C.c_uint32, vp, vp, vp, vp]
         _bound = True
     return L
 
@@ -54,13 +55,17 @@ def make_grid(world_min: Sequence[float], cell_size: float, grid_res: Sequence[i
 
 
 def build(grid: HashGrid, pred_ptr: int, type_flag_ptr: Optional[int], n: int, cell_count_ptr: int,
-          cell_offset_ptr: int, sorted_ptr: int, stream: Optional[int] = None):
-    """nb_hash_build on device addresses (ints); asynchronous on ``stream``."""
-    rc = _lib().nb_hash_build(C.byref(grid), C.c_void_p(pred_ptr), C.c_void_p(type_flag_ptr) if type_flag_ptr else None,
-                              int(n), C.c_void_p(cell_count_ptr), C.c_void_p(cell_offset_ptr),
-                              C.c_void_p(sorted_ptr) if sorted_ptr else None, C.c_void_p(stream) if stream else None)
+          cell_offset_ptr: int, sorted_ptr: int, stream: Optional[int] = None, noalloc: bool = False):
+    """nb_hash_build on device addresses (ints); asynchronous on ``stream``.
+    noalloc=True calls nb_hash_build_noalloc, which takes its scratch from a
+    per-GPU array instead of allocating it."""
+    L = _lib()
+    fn, name = (L.nb_hash_build_noalloc, "nb_hash_build_noalloc") if noalloc else (L.nb_hash_build, "nb_hash_build")
+    rc = fn(C.byref(grid), C.c_void_p(pred_ptr), C.c_void_p(type_flag_ptr) if type_flag_ptr else None,
+            int(n), C.c_void_p(cell_count_ptr), C.c_void_p(cell_offset_ptr),
+            C.c_void_p(sorted_ptr) if sorted_ptr else None, C.c_void_p(stream) if stream else None)
     if rc != mpm.NB_OK:
-        raise mpm.NebulaError(f"nb_hash_build error {rc}: {mpm.lib().nb_last_error().decode()}")
+        raise mpm.NebulaError(f"{name} error {rc}: {mpm.lib().nb_last_error().decode()}")
 
 
 class SpatialHash:
