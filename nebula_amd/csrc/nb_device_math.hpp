@@ -139,17 +139,37 @@ __device__ __forceinline__ float hw_sqrt(float x) { return __builtin_amdgcn_sqrt
 __device__ __forceinline__ float hw_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ float hw_log(float x) { return __builtin_amdgcn_logf(x) * 0.693147180559945309f; }
 __device__ __forceinline__ float hw_exp(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504088896341f); }
-// the FP64 fallback uses the correctly rounded double operations (the FP64
-// oracle's std::sqrt, division, log and exp)
-__device__ __forceinline__ double hw_rcp(double x) { return 1.0 / x; }
-__device__ __forceinline__ double hw_sqrt(double x) { return sqrt(x); }
-__device__ __forceinline__ double hw_rsq(double x) { return 1.0 / sqrt(x); }
+// The FP64 fallback: the hardware reciprocal and reciprocal square root
+// refined by two Newton steps each, so quotients and roots land within an
+// ulp or two of FP64 (the oracle's correctly rounded operations), far below
+// the single FP32 rounding the fallback's results take; log and exp are the
+// library's.  Arguments are finite, and the reciprocal's nonzero.
+__device__ __forceinline__ double hw_rcp(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(r, fma(-x, r, 1.0), r);
+  return fma(r, fma(-x, r, 1.0), r);
+}
+__device__ __forceinline__ double hw_rsq(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double h = 0.5 * x;
+  y = fma(y, fma(-h * y, y, 0.5), y);
+  return fma(y, fma(-h * y, y, 0.5), y);
+}
+__device__ __forceinline__ double hw_sqrt(double x) {
+  if (!(x > 0.0)) return x == 0.0 ? 0.0 : sqrt(x);
+  const double y = hw_rsq(x), s = x * y;
+  return fma(0.5 * y, fma(-s, s, x), s);
+}
 __device__ __forceinline__ double hw_log(double x) { return log(x); }
 __device__ __forceinline__ double hw_exp(double x) { return exp(x); }
 
-// a / b: float multiplies by the hardware reciprocal, double divides (IEEE)
+// a / b: float multiplies by the hardware reciprocal; double multiplies by
+// the refined reciprocal and corrects the quotient once
 __device__ __forceinline__ float tdiv(float a, float b) { return a * hw_rcp(b); }
-__device__ __forceinline__ double tdiv(double a, double b) { return a / b; }
+__device__ __forceinline__ double tdiv(double a, double b) {
+  const double r = hw_rcp(b), q = a * r;
+  return fma(r, fma(-b, q, a), q);
+}
 
 // ---- Jacobi eigen-decomposition of symmetric A (shaders/mpm_common.glsl:213-247) ----
 // Two forms of the same iteration.  jacobi_eigen3_select keeps the matrix in
@@ -471,6 +491,10 @@ __device__ __forceinline__ mat3<T> hencky_from_log(const mat3<T>& U, T e0, T e1,
 }
 template <typename T>
 __device__ __forceinline__ T log_strain(T s) { return hw_log(tmax(tabs(s), T(1e-6))); }
+// log_strain(x) of x = exp(e): FP32 takes the log as the reference's shader
+// does; FP64 returns e (or the clamp), which that log returns to within an ulp
+__device__ __forceinline__ float log_strain_of_exp(float e, float x) { return log_strain(x); }
+__device__ __forceinline__ double log_strain_of_exp(double e, double x) { return x > 1e-6 ? e : log(1e-6); }
 template <typename T>
 __device__ __forceinline__ mat3<T> hencky_stress(const mat3<T>& F, T mu, T lam, bool corrected) {
   mat3<T> U, V;
@@ -539,9 +563,9 @@ __device__ __forceinline__ PlasticOutT<T> project_plastic_svd(const mat3<T>& F, 
   if (recompose) {
     const T x0 = hw_exp(e0), x1 = hw_exp(e1), x2 = hw_exp(e2);
     o.F = m3_udvt(U, x0, x1, x2, V);
-    o.eps[0] = log_strain(x0);
-    o.eps[1] = log_strain(x1);
-    o.eps[2] = log_strain(x2);
+    o.eps[0] = log_strain_of_exp(e0, x0);
+    o.eps[1] = log_strain_of_exp(e1, x1);
+    o.eps[2] = log_strain_of_exp(e2, x2);
   } else {
     o.F = F;
     o.eps[0] = e0;
