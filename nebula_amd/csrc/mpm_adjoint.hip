@@ -23,6 +23,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "nb_adjoint_math.hpp"
 #include "nb_device_math.hpp"
 #include "nb_engine.hpp"
@@ -122,10 +124,11 @@ __device__ __forceinline__ void axis_factors(const AxisD& a, AxisF& f) {
 // Axis-by-axis contraction: x (5 factor kinds per row), then y (the 12
 // (x, y) pairs the products use), then z.  load(ix, iy, iz) returns the
 // node's field in xyz and, when WITH_M, a scalar field in w whose sums with
-// dw/dx_a go to Sm.  The transfers' cut of node weights below 1e-12 is not
-// applied: it changes the forward map by less than 1e-12 relative, so its
-// derivative terms are below any gradient tolerance, and every particle then
-// takes the same path.  The z loop stays rolled so that one slab
+// dw/dx_a go to Sm.  The transfers' cut of nodes with weight below 1e-12 is
+// applied by load returning zero for such a node (every product above then
+// drops it): the cut node's weight is negligible, but its second derivative
+// is not (w'' = 1/h^2 on the support's edge), so the derivative of the cut
+// map leaves it out.  The z loop stays rolled so that one slab
 // of 9 node loads is live at a time; its z factors sit in position 0 and are
 // rotated at the end of each iteration.
 template <bool WITH_M, class Load, class Sink>
@@ -311,8 +314,11 @@ __global__ __launch_bounds__(256) void k_g2p_bwd(SubParams p, const float4* __re
       axis_stencil_d(fq, xs[a], sg.wmin[a], sg.h, base[a], d);
       axis_factors(d, ax[a]);
     }
-    // grid velocity of node (ix, iy, iz); out-of-grid and massless nodes
-    // (mass < 1e-10) read as zero (shaders/mpm_g2p.comp:93-101)
+    // G2P's weight cut (shaders/mpm_g2p.comp:93-101) as its forward takes it:
+    // on a particle with a node of 0 < w < 1e-12, nodes with (wx wy) wz < 1e-12
+    const bool tiny = (min_positive3(ax[0].w) * min_positive3(ax[1].w)) * min_positive3(ax[2].w) < 1e-12f;
+    // grid velocity of node (ix, iy, iz); out-of-grid, massless (mass < 1e-10)
+    // and cut nodes read as zero (shaders/mpm_g2p.comp:93-101)
     auto node_v = [&](int ix, int iy, int iz) -> float4 {
       const int nx = base[0] + ix, ny = base[1] + iy, nz = base[2] + iz;
       const uint32_t cx = uint32_t(min(max(nx, 0), sg.grid_res[0] - 1));
@@ -320,8 +326,9 @@ __global__ __launch_bounds__(256) void k_g2p_bwd(SubParams p, const float4* __re
       const uint32_t cz = uint32_t(min(max(nz, 0), sg.grid_res[2] - 1));
       const float4 g = grid_node[node_id(sg, cx, cy, cz)];
       // bitwise tests: the load stays unconditional (one 16-byte load)
+      const bool cut = tiny & (ax[0].w[ix] * ax[1].w[iy] * ax[2].w[iz] < 1e-12f);
       const bool live = (uint32_t(nx) < uint32_t(sg.grid_res[0])) & (uint32_t(ny) < uint32_t(sg.grid_res[1])) &
-                        (uint32_t(nz) < uint32_t(sg.grid_res[2])) & (g.w >= 1e-10f);
+                        (uint32_t(nz) < uint32_t(sg.grid_res[2])) & (g.w >= 1e-10f) & !cut;
       return live ? make_float4(g.x, g.y, g.z, 0.f) : make_float4(0.f, 0.f, 0.f, 0.f);
     };
     // The incoming adjoint of B is needed during the gather: the position
@@ -612,31 +619,50 @@ __global__ __launch_bounds__(kAdjThreads) void k_adj_scatter(SubParams p, const 
         //   = wx P + (wx dx) Q + gx R  with
         //   P = wy wz (A + dy M_B[1] + dz M_B[2]) + gy wz M_G[1] + wy gz M_G[2],
         //   Q = wy wz M_B[0],  R = wy wz M_G[0]
-        // (no weight cut, as in the G2P backward's gather)
+        // G2P's weight cut as its forward takes it: on a particle with a node
+        // of 0 < w < 1e-12, the nodes with (wx wy) wz < 1e-12 get nothing
+        // (a wave with no such particle takes the unmasked loop).
+        const bool tiny = (min_positive3(X.w) * min_positive3(Y.w)) * min_positive3(Z.w) < 1e-12f;
         float wdx[3];
 #pragma unroll
         for (int ix = 0; ix < 3; ix++) wdx[ix] = X.w[ix] * X.dx[ix];
+        auto rows = [&](auto masked) {
 #pragma unroll
-        for (int kz = 0; kz < 3; kz++)
+          for (int kz = 0; kz < 3; kz++)
 #pragma unroll
-          for (int jy = 0; jy < 3; jy++) {
-            const float wyz = Y.w[jy] * Z.w[kz], gyz = Y.g[jy] * Z.w[kz], wgz = Y.w[jy] * Z.g[kz];
-            float P[3], Q[3], R[3];
+            for (int jy = 0; jy < 3; jy++) {
+              const float wyz = Y.w[jy] * Z.w[kz], gyz = Y.g[jy] * Z.w[kz], wgz = Y.w[jy] * Z.g[kz];
+              float P[3], Q[3], R[3];
 #pragma unroll
-            for (int rr = 0; rr < 3; rr++) {
-              const float t = fmaf(Z.dx[kz], cf[9 + rr], fmaf(Y.dx[jy], cf[6 + rr], cf[rr]));
-              P[rr] = fmaf(wgz, cf[18 + rr], fmaf(gyz, cf[15 + rr], wyz * t));
-              Q[rr] = wyz * cf[3 + rr];
-              R[rr] = wyz * cf[12 + rr];
+              for (int rr = 0; rr < 3; rr++) {
+                const float t = fmaf(Z.dx[kz], cf[9 + rr], fmaf(Y.dx[jy], cf[6 + rr], cf[rr]));
+                P[rr] = fmaf(wgz, cf[18 + rr], fmaf(gyz, cf[15 + rr], wyz * t));
+                Q[rr] = wyz * cf[3 + rr];
+                R[rr] = wyz * cf[12 + rr];
+              }
+#pragma unroll
+              for (int ix = 0; ix < 3; ix++) {
+                const int o = (kz * 3 + jy) * 3 + ix;
+                const float nx_ = fmaf(X.g[ix], R[0], fmaf(wdx[ix], Q[0], fmaf(X.w[ix], P[0], ax[o])));
+                const float ny_ = fmaf(X.g[ix], R[1], fmaf(wdx[ix], Q[1], fmaf(X.w[ix], P[1], ay[o])));
+                const float nz_ = fmaf(X.g[ix], R[2], fmaf(wdx[ix], Q[2], fmaf(X.w[ix], P[2], az[o])));
+                if constexpr (decltype(masked)::value) {
+                  const bool cut = tiny & (X.w[ix] * Y.w[jy] * Z.w[kz] < 1e-12f);
+                  ax[o] = cut ? ax[o] : nx_;
+                  ay[o] = cut ? ay[o] : ny_;
+                  az[o] = cut ? az[o] : nz_;
+                } else {
+                  ax[o] = nx_;
+                  ay[o] = ny_;
+                  az[o] = nz_;
+                }
+              }
             }
-#pragma unroll
-            for (int ix = 0; ix < 3; ix++) {
-              const int o = (kz * 3 + jy) * 3 + ix;
-              ax[o] = fmaf(X.g[ix], R[0], fmaf(wdx[ix], Q[0], fmaf(X.w[ix], P[0], ax[o])));
-              ay[o] = fmaf(X.g[ix], R[1], fmaf(wdx[ix], Q[1], fmaf(X.w[ix], P[1], ay[o])));
-              az[o] = fmaf(X.g[ix], R[2], fmaf(wdx[ix], Q[2], fmaf(X.w[ix], P[2], az[o])));
-            }
-          }
+        };
+        if (__any(tiny))
+          rows(std::true_type{});
+        else
+          rows(std::false_type{});
       }
 #pragma unroll
       for (int kz = 0; kz < 3; kz++)
@@ -801,10 +827,16 @@ __global__ __launch_bounds__(256) void k_p2g_bwd(SubParams p, const float4* __re
     axis_factors(d, ax[a]);
   }
   const uint32_t cube = sg.cube_res;
-  // grid adjoint (dL/dp, dL/dm) of node (ix, iy, iz); nodes outside the cube read as zero
+  // P2G's weight cut (shaders/mpm_p2g.comp:61) as its forward takes it: a
+  // particle whose smallest positive per-axis weights multiply below
+  // 1.0001e-12 skips the nodes with wx (wy wz) < 1e-12
+  const bool tiny = (min_positive3(ax[0].w) * min_positive3(ax[1].w)) * min_positive3(ax[2].w) < 1.0001e-12f;
+  // grid adjoint (dL/dp, dL/dm) of node (ix, iy, iz); nodes outside the cube
+  // and cut nodes read as zero
   auto node_g = [&](int ix, int iy, int iz) -> float4 {
     const uint32_t nx = uint32_t(cc[0] - 1 + ix), ny = uint32_t(cc[1] - 1 + iy), nz = uint32_t(cc[2] - 1 + iz);
-    const bool in = nx < cube && ny < cube && nz < cube;  // negative wraps to large
+    const bool cut = tiny & (ax[0].w[ix] * (ax[1].w[iy] * ax[2].w[iz]) < 1e-12f);
+    const bool in = (nx < cube) & (ny < cube) & (nz < cube) & !cut;  // negative wraps to large
     const float4 g = grid_adj[node_id(sg, min(nx, cube - 1u), min(ny, cube - 1u), min(nz, cube - 1u))];
     return in ? g : make_float4(0.f, 0.f, 0.f, 0.f);
   };

@@ -125,6 +125,8 @@ def lib():
         L.orc_dual_tangents.argtypes = []
         L.orc_dual_substeps.argtypes = [C.POINTER(OrcScene), vp, vp, vp, fp, dp, up, C.c_uint32, dp, dp, C.c_uint32,
                                         dp, dp]
+        L.orc_dual32_substeps.argtypes = [C.POINTER(OrcScene), vp, vp, vp, fp, dp, up, C.c_uint32, dp, dp, C.c_uint32,
+                                        dp, dp]
         _lib = L
     return _lib
 
@@ -453,14 +455,15 @@ def dual_tangents():
 
 
 def dual_substeps(scene, materials: bytes, fields, matid, tan_fields, tan_mats=None, n_sub=1, colliders: bytes = b"",
-                  forces: bytes = b"", sdf=None, corrected_svd=False):
+                  forces: bytes = b"", sdf=None, corrected_svd=False, precision=64):
     """Forward-mode derivatives of n_sub substeps of the FP64 twin
     (oracle/dual.hpp): the values after n_sub substeps, (8, n, 4) float64, and
     their tangents, (K, 8, n, 4), from the start ``fields`` (8, n, 4) float64,
     material ids ``matid`` (n,) and input tangents ``tan_fields`` (K, 8, n, 4)
     and ``tan_mats`` (K, materials, 4: mu, lambda, M_friction, q_cohesion),
     K = dual_tangents().  Exact derivatives of the FP64 substep map (no step
-    size); the material-id lanes carry no tangent."""
+    size); the material-id lanes carry no tangent.  precision=32: the FP32
+    oracle's map, values and tangents in FP32 (Dual32), returned as float64."""
     L = lib()
     K = L.orc_dual_tangents()
     sc = OrcScene.from_buffer_copy(scene)
@@ -487,7 +490,10 @@ def dual_substeps(scene, materials: bytes, fields, matid, tan_fields, tan_mats=N
     sd = np.ascontiguousarray(sdf, np.float32) if sdf is not None else None
     out = np.empty((8, n, 4), np.float64)
     out_t = np.empty((K, 8, n, 4), np.float64)
-    L.orc_dual_substeps(C.byref(sc), mb, cb, fb, _fptr(sd) if sd is not None else None, _dptr(f),
+    if precision not in (32, 64):
+        raise ValueError(f"precision {precision}: 32 or 64")
+    fn = L.orc_dual_substeps if precision == 64 else L.orc_dual32_substeps
+    fn(C.byref(sc), mb, cb, fb, _fptr(sd) if sd is not None else None, _dptr(f),
                         m.ctypes.data_as(C.POINTER(C.c_uint32)), n, _dptr(tf), _dptr(tm) if tm is not None else None,
                         int(n_sub), _dptr(out), _dptr(out_t))
     return out, out_t

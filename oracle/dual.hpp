@@ -9,6 +9,13 @@
 // step size.  It checks the reverse-mode kernels where finite differences are
 // ill-posed (a perturbation moves particles across the yield surface).
 //
+// DualT<float> (Dual32) is the same arithmetic on FP32 values and tangents:
+// its values follow the FP32 oracle's, rounding for rounding (the FP32
+// oracle also computes with every operation rounded as written), and its
+// tangents are the derivatives of that FP32 map taken in FP32.  How far
+// they stray from Dual's shows what differentiation in FP32 can reach on a
+// given trajectory.
+//
 // Rules: +, -, *, / and the functions below differentiate as usual; floor,
 // comparisons and the integer conversions act on the value (their derivative
 // is zero or undefined, and the oracle only uses them to pick cells and
@@ -24,81 +31,104 @@ namespace oracle {
 
 constexpr int kTangents = 4;
 
-struct Dual {
-  double v;
-  double d[kTangents];
-  Dual() : v(0.0) {
-    for (int k = 0; k < kTangents; k++) d[k] = 0.0;
+template <class V>
+struct DualT {
+  V v;
+  V d[kTangents];
+  DualT() : v(0) {
+    for (int k = 0; k < kTangents; k++) d[k] = V(0);
   }
-  Dual(double a) : v(a) {  // NOLINT: implicit, as R(0.5) and mixed arithmetic need
-    for (int k = 0; k < kTangents; k++) d[k] = 0.0;
+  DualT(double a) : v(V(a)) {  // NOLINT: implicit, as R(0.5) and mixed arithmetic need
+    for (int k = 0; k < kTangents; k++) d[k] = V(0);
   }
-  explicit operator double() const { return v; }
+  explicit operator double() const { return double(v); }
   explicit operator float() const { return float(v); }
   explicit operator int() const { return int(v); }
   explicit operator uint32_t() const { return uint32_t(v); }
   explicit operator int64_t() const { return int64_t(v); }
 };
+using Dual = DualT<double>;
+using Dual32 = DualT<float>;
 
-inline Dual operator+(const Dual& a, const Dual& b) {
-  Dual r(a.v + b.v);
+template <class V>
+inline DualT<V> operator+(const DualT<V>& a, const DualT<V>& b) {
+  DualT<V> r;
+  r.v = a.v + b.v;
   for (int k = 0; k < kTangents; k++) r.d[k] = a.d[k] + b.d[k];
   return r;
 }
-inline Dual operator-(const Dual& a, const Dual& b) {
-  Dual r(a.v - b.v);
+template <class V>
+inline DualT<V> operator-(const DualT<V>& a, const DualT<V>& b) {
+  DualT<V> r;
+  r.v = a.v - b.v;
   for (int k = 0; k < kTangents; k++) r.d[k] = a.d[k] - b.d[k];
   return r;
 }
-inline Dual operator-(const Dual& a) {
-  Dual r(-a.v);
+template <class V>
+inline DualT<V> operator-(const DualT<V>& a) {
+  DualT<V> r;
+  r.v = -a.v;
   for (int k = 0; k < kTangents; k++) r.d[k] = -a.d[k];
   return r;
 }
-inline Dual operator*(const Dual& a, const Dual& b) {
-  Dual r(a.v * b.v);
+template <class V>
+inline DualT<V> operator*(const DualT<V>& a, const DualT<V>& b) {
+  DualT<V> r;
+  r.v = a.v * b.v;
   for (int k = 0; k < kTangents; k++) r.d[k] = a.d[k] * b.v + a.v * b.d[k];
   return r;
 }
-inline Dual operator/(const Dual& a, const Dual& b) {
-  const double q = a.v / b.v;
-  Dual r(q);
+template <class V>
+inline DualT<V> operator/(const DualT<V>& a, const DualT<V>& b) {
+  const V q = a.v / b.v;
+  DualT<V> r;
+  r.v = q;
   for (int k = 0; k < kTangents; k++) r.d[k] = (a.d[k] - q * b.d[k]) / b.v;
   return r;
 }
-inline Dual& operator+=(Dual& a, const Dual& b) { return a = a + b; }
-inline Dual& operator-=(Dual& a, const Dual& b) { return a = a - b; }
-inline Dual& operator*=(Dual& a, const Dual& b) { return a = a * b; }
-inline Dual& operator/=(Dual& a, const Dual& b) { return a = a / b; }
-inline bool operator<(const Dual& a, const Dual& b) { return a.v < b.v; }
-inline bool operator>(const Dual& a, const Dual& b) { return a.v > b.v; }
-inline bool operator<=(const Dual& a, const Dual& b) { return a.v <= b.v; }
-inline bool operator>=(const Dual& a, const Dual& b) { return a.v >= b.v; }
-inline bool operator==(const Dual& a, const Dual& b) { return a.v == b.v; }
-inline bool operator!=(const Dual& a, const Dual& b) { return a.v != b.v; }
+template <class V> inline DualT<V>& operator+=(DualT<V>& a, const DualT<V>& b) { return a = a + b; }
+template <class V> inline DualT<V>& operator-=(DualT<V>& a, const DualT<V>& b) { return a = a - b; }
+template <class V> inline DualT<V>& operator*=(DualT<V>& a, const DualT<V>& b) { return a = a * b; }
+template <class V> inline DualT<V>& operator/=(DualT<V>& a, const DualT<V>& b) { return a = a / b; }
+template <class V> inline bool operator<(const DualT<V>& a, const DualT<V>& b) { return a.v < b.v; }
+template <class V> inline bool operator>(const DualT<V>& a, const DualT<V>& b) { return a.v > b.v; }
+template <class V> inline bool operator<=(const DualT<V>& a, const DualT<V>& b) { return a.v <= b.v; }
+template <class V> inline bool operator>=(const DualT<V>& a, const DualT<V>& b) { return a.v >= b.v; }
+template <class V> inline bool operator==(const DualT<V>& a, const DualT<V>& b) { return a.v == b.v; }
+template <class V> inline bool operator!=(const DualT<V>& a, const DualT<V>& b) { return a.v != b.v; }
 
 // f(a) with f'(a) = fp
-inline Dual chain(const Dual& a, double f, double fp) {
-  Dual r(f);
+template <class V>
+inline DualT<V> chain(const DualT<V>& a, V f, V fp) {
+  DualT<V> r;
+  r.v = f;
   for (int k = 0; k < kTangents; k++) r.d[k] = fp * a.d[k];
   return r;
 }
-inline Dual o_fabs(const Dual& a) { return a.v < 0.0 ? -a : a; }
-inline Dual o_sqrt(const Dual& a) {
-  const double s = std::sqrt(a.v);
-  return chain(a, s, s > 0.0 ? 0.5 / s : 0.0);
+template <class V> inline DualT<V> o_fabs(const DualT<V>& a) { return a.v < V(0) ? -a : a; }
+template <class V>
+inline DualT<V> o_sqrt(const DualT<V>& a) {
+  const V s = std::sqrt(a.v);
+  return chain(a, s, s > V(0) ? V(0.5) / s : V(0));
 }
-inline Dual o_log(const Dual& a) { return chain(a, std::log(a.v), 1.0 / a.v); }
-inline Dual o_exp(const Dual& a) {
-  const double e = std::exp(a.v);
+template <class V> inline DualT<V> o_log(const DualT<V>& a) { return chain(a, V(std::log(a.v)), V(1) / a.v); }
+template <class V>
+inline DualT<V> o_exp(const DualT<V>& a) {
+  const V e = std::exp(a.v);
   return chain(a, e, e);
 }
-inline Dual o_floor(const Dual& a) { return Dual(std::floor(a.v)); }
-inline Dual o_pow(const Dual& a, const Dual& b) {
+template <class V> inline DualT<V> o_floor(const DualT<V>& a) { return DualT<V>(double(std::floor(a.v))); }
+template <class V>
+inline DualT<V> o_pow(const DualT<V>& a, const DualT<V>& b) {
   // the oracle raises to constant exponents only (J^(1/3)); b's tangent is dropped
-  const double p = std::pow(a.v, b.v);
-  return chain(a, p, a.v != 0.0 ? b.v * p / a.v : 0.0);
+  const V pw = std::pow(a.v, b.v);
+  return chain(a, pw, a.v != V(0) ? b.v * pw / a.v : V(0));
 }
+// the noise hash's sine: the FP32 oracle's correctly rounded float sine for
+// Dual32, the FP64 sine for Dual
 inline Dual hash_sin(const Dual& a) { return chain(a, std::sin(a.v), std::cos(a.v)); }
+inline Dual32 hash_sin(const Dual32& a) {
+  return chain(a, float(std::sin(double(a.v))), float(std::cos(double(a.v))));
+}
 
 }  // namespace oracle

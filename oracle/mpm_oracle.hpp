@@ -197,11 +197,18 @@ struct Params {
 // Material parameter `which` (0 mu, 1 lambda, 2 M_friction, 3 q_cohesion) of
 // material `id` (already clamped) as R; the dual form attaches its tangents.
 template <class R> inline R mat_param(const Params&, uint32_t, int, float value) { return R(value); }
-template <> inline Dual mat_param<Dual>(const Params& p, uint32_t id, int which, float value) {
-  Dual r{double(value)};
+template <class V>
+inline DualT<V> mat_param_dual(const Params& p, uint32_t id, int which, float value) {
+  DualT<V> r{double(value)};
   if (p.mat_tangent)
-    for (int k = 0; k < kTangents; k++) r.d[k] = p.mat_tangent[(size_t(id) * 4 + size_t(which)) * kTangents + k];
+    for (int k = 0; k < kTangents; k++) r.d[k] = V(p.mat_tangent[(size_t(id) * 4 + size_t(which)) * kTangents + k]);
   return r;
+}
+template <> inline Dual mat_param<Dual>(const Params& p, uint32_t id, int which, float value) {
+  return mat_param_dual<double>(p, id, which, value);
+}
+template <> inline Dual32 mat_param<Dual32>(const Params& p, uint32_t id, int which, float value) {
+  return mat_param_dual<float>(p, id, which, value);
 }
 
 inline uint32_t total_cells(const Params& p) { return p.cube_res * p.cube_res * p.cube_res; }

@@ -583,6 +583,40 @@ void orc_mesh_sdf(const float* tris, uint32_t n, uint32_t rx, uint32_t ry, uint3
 }
 
 
+}  // extern "C"
+
+// Forward-mode substeps over a dual-number type D (Dual or Dual32).
+template <class D>
+static void dual_substeps_impl(const orc_scene* sc, const void* mats, const void* cols, const void* forces,
+                               const float* sdf, const double* fields, const uint32_t* matid, uint32_t n,
+                               const double* tan_fields, const double* tan_mats, uint32_t nsub, double* out_fields,
+                               double* out_tan) {
+  orc_sim* sim = orc_sim_create(sc, mats, cols, forces, sdf, 64);
+  Params p = sim->params();
+  p.mat_tangent = tan_mats;
+  const uint32_t NC = sc->cube_res * sc->cube_res * sc->cube_res;
+  State<D> st;
+  st.resize(n, NC);
+  const size_t per = size_t(n) * 4;
+  for (int f = 0; f < 8; f++)
+    for (size_t i = 0; i < per; i++) {
+      D& x = st.f[f][i];
+      x = D(fields[size_t(f) * per + i]);
+      for (int k = 0; k < kTangents; k++) x.d[k] = decltype(x.v)(tan_fields[(size_t(k) * 8 + f) * per + i]);
+    }
+  for (uint32_t i = 0; i < n; i++) st.matid[i] = matid[i];
+  for (uint32_t i = 0; i < nsub; i++) substep<D>(p, st);
+  for (int f = 0; f < 8; f++)
+    for (size_t i = 0; i < per; i++) {
+      const D& x = st.f[f][i];
+      out_fields[size_t(f) * per + i] = double(x.v);
+      for (int k = 0; k < kTangents; k++) out_tan[(size_t(k) * 8 + f) * per + i] = double(x.d[k]);
+    }
+  orc_sim_destroy(sim);
+}
+
+extern "C" {
+
 // Forward-mode derivatives of nsub substeps (the FP64 twin with kTangents
 // tangents; oracle/dual.hpp).  fields: 8 arrays of n double4 (the reference
 // layout, the material-id lanes ignored), matid: n ids; tan_fields:
@@ -593,28 +627,16 @@ int orc_dual_tangents(void) { return kTangents; }
 void orc_dual_substeps(const orc_scene* sc, const void* mats, const void* cols, const void* forces, const float* sdf,
                        const double* fields, const uint32_t* matid, uint32_t n, const double* tan_fields,
                        const double* tan_mats, uint32_t nsub, double* out_fields, double* out_tan) {
-  orc_sim* sim = orc_sim_create(sc, mats, cols, forces, sdf, 64);
-  Params p = sim->params();
-  p.mat_tangent = tan_mats;
-  const uint32_t NC = sc->cube_res * sc->cube_res * sc->cube_res;
-  State<Dual> st;
-  st.resize(n, NC);
-  const size_t per = size_t(n) * 4;
-  for (int f = 0; f < 8; f++)
-    for (size_t i = 0; i < per; i++) {
-      Dual& x = st.f[f][i];
-      x.v = fields[size_t(f) * per + i];
-      for (int k = 0; k < kTangents; k++) x.d[k] = tan_fields[(size_t(k) * 8 + f) * per + i];
-    }
-  for (uint32_t i = 0; i < n; i++) st.matid[i] = matid[i];
-  for (uint32_t i = 0; i < nsub; i++) substep<Dual>(p, st);
-  for (int f = 0; f < 8; f++)
-    for (size_t i = 0; i < per; i++) {
-      const Dual& x = st.f[f][i];
-      out_fields[size_t(f) * per + i] = x.v;
-      for (int k = 0; k < kTangents; k++) out_tan[(size_t(k) * 8 + f) * per + i] = x.d[k];
-    }
-  orc_sim_destroy(sim);
+  dual_substeps_impl<Dual>(sc, mats, cols, forces, sdf, fields, matid, n, tan_fields, tan_mats, nsub, out_fields,
+                           out_tan);
+}
+// The same with FP32 values and tangents (Dual32): the FP32 oracle's map
+// differentiated in FP32.
+void orc_dual32_substeps(const orc_scene* sc, const void* mats, const void* cols, const void* forces, const float* sdf,
+                         const double* fields, const uint32_t* matid, uint32_t n, const double* tan_fields,
+                         const double* tan_mats, uint32_t nsub, double* out_fields, double* out_tan) {
+  dual_substeps_impl<Dual32>(sc, mats, cols, forces, sdf, fields, matid, n, tan_fields, tan_mats, nsub, out_fields,
+                             out_tan);
 }
 
 }  // extern "C"

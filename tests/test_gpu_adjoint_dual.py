@@ -45,7 +45,12 @@ def _directions(sc, rng):
     return D, Dm
 
 
-def _compare(sc, n_sub, seg, tol, seed=7):
+def _compare(sc, n_sub, seg, tol, seed=7, fp32_factor=None):
+    """Directional derivatives of L = <W, state after n_sub substeps>: the GPU
+    adjoint against the FP64 dual twin, relative error <= tol; with
+    fp32_factor, <= max(tol, fp32_factor x the FP32 dual twin's own error)
+    (Dual32 differentiates the FP32 oracle's map in FP32: what FP32 can reach
+    on this trajectory)."""
     n = sc.n
     rng = np.random.default_rng(seed)
     W = rng.standard_normal((8, n, 4)) * _mask(n)
@@ -64,17 +69,25 @@ def _compare(sc, n_sub, seg, tol, seed=7):
     f0[1, :, 3] = 0.0
     scn = oracle.make_scene(sc.dt, sc.cfg.cellSize, sc.grid_res(), (0.0, 0.0, 0.0), tuple(sc.cfg.domainSize),
                             sc.flip_ratio)
-    _, tan = oracle.dual_substeps(scn, mpm.materials_bytes(sc.materials), f0, mid, D, Dm, n_sub,
-                                  forces=mpm.forces_bytes(sc.forces), sdf=sc.sdf, corrected_svd=True)
+    args = (scn, mpm.materials_bytes(sc.materials), f0, mid, D, Dm, n_sub)
+    kw = dict(forces=mpm.forces_bytes(sc.forces), sdf=sc.sdf, corrected_svd=True)
+    _, tan = oracle.dual_substeps(*args, **kw)
+    tan32 = oracle.dual_substeps(*args, **kw, precision=32)[1] if fp32_factor else None
     worst = 0.0
     for j, name in enumerate(("x", "v", "F/tau/B", "materials")):
         dual = float(np.sum(W * tan[j]))
         adj = float(np.sum(grad * D[j])) + float(np.sum(dpar * Dm[j]))
         err = abs(adj - dual) / max(abs(dual), 1e-30)
-        worst = max(worst, err)
+        bar = tol
+        extra = ""
+        if fp32_factor:
+            e32 = abs(float(np.sum(W * tan32[j])) - dual) / max(abs(dual), 1e-30)
+            bar = max(tol, fp32_factor * e32)
+            extra = f"  (FP32 dual twin {e32:.2e})"
+        worst = max(worst, err / bar)
         print(f"  {sc.name} {n_sub} substeps (segments of {seg}): d/d{name}: adjoint {adj:+.7e}  dual {dual:+.7e}  "
-              f"rel {err:.2e}", flush=True)
-    assert worst <= tol, f"worst relative difference {worst:.2e} > {tol}"
+              f"rel {err:.2e}{extra}", flush=True)
+        assert err <= bar, f"d/d{name}: relative difference {err:.2e} > {bar:.2e}"
     return worst
 
 
@@ -86,8 +99,86 @@ def test_adjoint_matches_dual_twin_over_20_substeps(gpu_lib, flip_ratio):
     _compare(_scene(flip_ratio=flip_ratio), 20, 5, 1e-3)
 
 
-def test_adjoint_matches_dual_twin_on_a_differentiable_batch_scene(gpu_lib):
+@pytest.mark.parametrize("start", ["as_built", "pre_strained"])
+def test_adjoint_matches_dual_twin_on_a_differentiable_batch_scene(gpu_lib, start):
     """One scene of the differentiable batch at full size (32,768 particles on
     64^3, an elastic layer on a Drucker-Prager layer, proper polar SVD), 20
-    substeps in segments of 5."""
-    _compare(batch_layers_64()[0], 20, 5, 1e-3)
+    substeps in segments of 5, as built (F = I) and with each particle's F
+    pre-strained by 3 % noise (seed 3).
+
+    Here FP32 itself cannot reach 1e-3: from F = I the strains are ~1e-4, so
+    log(sigma) ~ sigma - 1 keeps ~3 digits in FP32, and the FP32 dual twin
+    (the FP32 oracle's map differentiated in FP32) strays from the FP64 twin
+    by 2-40 % along these directions (1-5 % pre-strained, where plastic flow
+    over 20 substeps amplifies rounding).  The bar is the FP32 twin's own
+    error times 3 (the GPU's FP32 trajectory sums in another order, a
+    different draw of the same rounding), or 1e-3 where that is larger."""
+    sc = batch_layers_64()[0]
+    if start == "pre_strained":
+        rng = np.random.default_rng(3)
+        f = sc.fields.copy()
+        for c in range(3):
+            f[2 + c, :, :3] = np.eye(3, dtype=np.float32)[c] + 0.03 * rng.standard_normal((sc.n, 3)).astype(np.float32)
+        sc.fields = f
+    _compare(sc, 20, 5, 1e-3, fp32_factor=3.0)
+
+
+def test_adjoint_leaves_out_the_nodes_the_weight_cut_drops(gpu_lib):
+    """Particles within ~1e-4 cells of a cell face have a stencil node whose
+    weight w = wx wy wz falls below the transfers' 1e-12 cut (shaders/
+    mpm_p2g.comp:61, mpm_g2p.comp:93-101): the forward drops that node, so
+    its derivative must drop it too, although its weight is negligible its
+    second derivative is not (w'' = 1/h^2 at the support's edge).  On the
+    pre-strained differentiable-batch scene, one substep: dL/dx of every such
+    particle against the FP64 dual twin, entry by entry (before the cut was
+    applied in the adjoint these entries were off by up to 10 %)."""
+    sc = batch_layers_64()[0]
+    rng = np.random.default_rng(3)
+    f = sc.fields.copy()
+    for c in range(3):
+        f[2 + c, :, :3] = np.eye(3, dtype=np.float32)[c] + 0.03 * rng.standard_normal((sc.n, 3)).astype(np.float32)
+    sc.fields = f
+    n, h = sc.n, np.float32(sc.cfg.cellSize)
+    fq = (f[0, :, :3] / h).astype(np.float32)
+    base = np.floor(fq) - 1
+    minw = np.ones(n, np.float32)
+    for a in range(3):
+        ws = []
+        for i in range(3):
+            ad = np.abs(fq[:, a] - (base[:, a] + i + np.float32(0.5)))
+            w = np.where(ad < 0.5, 0.75 - ad * ad, np.where(ad < 1.5, 0.5 * (1.5 - ad) ** 2, 0.0)).astype(np.float32)
+            ws.append(np.where(w > 0, w, np.inf))
+        minw = (minw * np.min(ws, axis=0)).astype(np.float32)
+    tiny = np.where(minw < 1e-12)[0][:12]
+    assert len(tiny) >= 8, len(tiny)
+    W = np.random.default_rng(7).standard_normal((8, n, 4)) * _mask(n)
+    e = make_engine(sc)
+    e.adjointBegin(1, 1)
+    e.forwardRecorded(sc.dt)
+    e.setAdjoint(W.astype(np.float32))
+    e.backward()
+    grad = e.getAdjoint().astype(np.float64)
+    e.adjointEnd()
+    del e
+    K = oracle.dual_tangents()
+    f0 = sc.fields.astype(np.float64)
+    mid = sc.fields[1, :, 3].view(np.uint32)
+    f0[1, :, 3] = 0.0
+    scn = oracle.make_scene(sc.dt, sc.cfg.cellSize, sc.grid_res(), (0.0, 0.0, 0.0), tuple(sc.cfg.domainSize),
+                            sc.flip_ratio)
+    entries = [(int(p), a) for p in tiny for a in range(3)]
+    got, want = [], []
+    for i in range(0, len(entries), K):
+        chunk = entries[i:i + K]
+        tan = np.zeros((K, 8, n, 4))
+        for j, (p, a) in enumerate(chunk):
+            tan[j, 0, p, a] = 1.0
+        _, t = oracle.dual_substeps(scn, mpm.materials_bytes(sc.materials), f0, mid, tan, None, 1,
+                                    forces=mpm.forces_bytes(sc.forces), corrected_svd=True)
+        for j, (p, a) in enumerate(chunk):
+            got.append(grad[0, p, a])
+            want.append(float(np.sum(W * t[j])))
+    got, want = np.array(got), np.array(want)
+    rel = np.abs(got - want) / np.maximum(np.abs(want), 1e-3 * np.abs(want).max())
+    print(f"  weight-cut particles {len(tiny)}: dL/dx entries, worst relative difference {rel.max():.2e}", flush=True)
+    assert rel.max() < 2e-3, (entries[int(np.argmax(rel))], got[int(np.argmax(rel))], want[int(np.argmax(rel))])

@@ -192,3 +192,37 @@ def test_dual_tangents_through_the_drucker_prager_return_map():
             scale = max(np.abs(b).max(), 1e-12)
             per = np.abs(a - b).max(axis=(0, 2)) / scale
             assert np.mean(per <= 1e-5) >= 0.99, f"tangent {j} {name}: {np.mean(per <= 1e-5):.3f} within 1e-5"
+
+
+@pytest.mark.parametrize("corrected", [False, True], ids=["refsvd", "corrsvd"])
+def test_dual32_values_are_the_fp32_oracle_and_its_tangents_track_the_fp64_twin(corrected):
+    """precision=32 (Dual32): the values are the FP32 oracle's bit for bit
+    (the same float operations, each rounded as written), and on a
+    well-conditioned elastic block (F = I + 0.05 noise, 4 substeps) the FP32
+    tangents stay within 1e-3 of the FP64 twin's on every field."""
+    f = _block(1)
+    n = f.shape[1]
+    mid = np.zeros(n, np.uint32)
+    sc = _scene()
+    tan = np.random.default_rng(12).standard_normal((_k(), 8, n, 4))
+    tan[:, 1, :, 3] = 0.0
+    val, t32 = oracle.dual_substeps(sc, _mats(), f.astype(np.float64), mid, tan, None, 4, forces=GRAV,
+                                    corrected_svd=corrected, precision=32)
+    sim = oracle.Sim(sc, _mats(), b"", GRAV, None, 32)
+    sim.set_corrected_svd(corrected)
+    fs = f.copy()
+    fs[1, :, 3] = mid.view(np.float32)
+    sim.set_state(fs)
+    sim.substeps(4)
+    want = sim.get_state()
+    got = val.astype(np.float32)
+    assert np.array_equal(got[:, :, :3], want[:, :, :3])
+    assert np.array_equal(got[2:, :, 3], want[2:, :, 3]) and np.array_equal(got[0, :, 3], want[0, :, 3])
+    _, t64 = oracle.dual_substeps(sc, _mats(), f.astype(np.float64), mid, tan, None, 4, forces=GRAV,
+                                  corrected_svd=corrected)
+    for lo, hi, name in ((0, 1, "x"), (1, 2, "v"), (2, 5, "F"), (5, 8, "B")):
+        a, b = t32[:, lo:hi, :, :3], t64[:, lo:hi, :, :3]
+        err = np.abs(a - b).max() / max(np.abs(b).max(), 1e-30)
+        assert err < 1e-3, f"{name}: FP32 tangents {err:.2e} from FP64"
+    with pytest.raises(ValueError):
+        oracle.dual_substeps(sc, _mats(), f.astype(np.float64), mid, tan, None, 1, precision=16)
