@@ -5,15 +5,22 @@ Workload (the single-GPU configuration the metric is quoted on): 4,194,304
 Drucker-Prager snow particles (E=1.4e5, nu=0.2, rho0=400, M=tan 30deg,
 cohesion 500) in an 8-cell layer over a seeded six-mode sine terrain given as
 a dense SDF, 256^3 grid on the unit box, APIC, gravity -9.8, dt=5e-5.  A step
-is one full substep (bin, P2G, grid update with SDF BC, G2P) over all
-particles, inputs resident in HBM.  `value` = particles x steps / seconds.
+is one full substep (bin, P2G, grid update with SDF BC, G2P and the FP64
+fallback's fix-up) over all particles, inputs resident in HBM.  The timed
+window is substeps 20..500 of the scene's 500-substep run (--warmup 20
+--steps 480).  `value` = particles x steps / seconds.
 
 Also printed in the JSON line:
   roofline      the dominant kernel's algorithmic bytes per launch over its
-                average HIP-event duration on the engine's stream, against the
-                8 TB/s HBM peak (bytes per particle / per active node as in the
-                substep model B = 288 N + 68 G_act: bin 20 N + 8 G, P2G 92 N +
-                16 G, grid 28 G, G2P 176 N + 16 G)
+                average HIP-event duration on the engine's stream (the same
+                window replayed on a fresh engine), against the 8 TB/s HBM
+                peak (bytes per particle / per active node as in the substep
+                model B = 288 N + 68 G_act: bin 20 N + 8 G, P2G 92 N + 16 G,
+                grid 28 G, G2P 176 N + 16 G); valu_frac: its measured VALU
+                instructions per launch against the VALU issue peak; bound:
+                whichever fraction is larger
+  svd_fp64_fallback  the window's lanes taking the FP64 fallback and its time
+                with the fallback off
   adjoint       the reverse-mode leg of the metric on the differentiable
                 batch (64 scenes of 32,768 particles on 64^3 grids, an elastic
                 layer on a Drucker-Prager layer, hidden E, nu, friction angle
@@ -44,6 +51,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "particle-substeps/s at 4M DP particles (1 GPU); fwd+adjoint / fwd time"
 HBM_PEAK_GBS = 8000.0
+# VALU issue: 1024 SIMDs at 2.4 GHz, 4 cycles per wave64 instruction (16
+# lanes a cycle for FP32 without packed math, and for FP64)
+SIMDS, CLOCK_HZ, VALU_CYCLES = 1024, 2.4e9, 4
 # HBM bytes per launch of the substep kernels on this workload, measured with
 # two rocprofv3 counter passes (FETCH_SIZE, then WRITE_SIZE) and scaled by what
 # the same counters read for a 256 MiB copy kernel in the same process (fetch
@@ -51,6 +61,11 @@ HBM_PEAK_GBS = 8000.0
 # commands and the per-kernel table are in the file named below.
 TRAFFIC_SOURCE = "profiles/hbm_traffic_snow_terrain_256.md"
 MEASURED_TRAFFIC = {"g2p": 1469.9e6, "p2g": 856.5e6, "grid_update": 116.9e6}
+# VALU instructions (wave64) per launch over the default window, from a
+# rocprofv3 SQ_INSTS_VALU pass of this command's window (file below): with the
+# live average launch time they give the fraction of the VALU issue peak.
+VALU_SOURCE = "profiles/counters_snow_terrain_256.md"
+MEASURED_VALU = {}
 
 
 def log(*a):
@@ -152,39 +167,50 @@ def host_transfer_leg(eng, sc, frame_substeps=20, reps=3):
             "note": "host arrays are pageable numpy memory; value is measured with the state resident in HBM"}
 
 
-def svd_fallback_leg(eng, sc, k):
-    """The FP64 SVD fallback of near-degenerate F (on by default, part of
-    value): the fraction of Drucker-Prager lanes and of wavefronts that take
-    it over k substeps from the current state, and the time per substep with
-    it and without it over those same k substeps."""
-    start = eng.getState()
-    eng.setSvdFallback(True, count_stats=True)
-    eng.svdFallbackStats(reset=True)
-    eng.substep(sc.dt, k)
+def window_run(sc, warmup, steps, fallback=True, stats=False, profile=False):
+    """A fresh engine on the scene's initial state: warmup substeps, then the
+    window of `steps` substeps, timed on the host clock with the stream
+    synchronised on both sides, or with HIP events around every kernel
+    (profile).  Returns (ms per substep, {kernel: ms} or None, {kernel:
+    launches} or None, fallback counters or None)."""
+    from nebula_amd.scenes import make_engine
+
+    eng = make_engine(sc)
+    eng.setSvdFallback(fallback, count_stats=stats)
+    eng.substep(sc.dt, warmup)
     eng.sync()
-    st = eng.svdFallbackStats(reset=True)
-
-    def timed(enabled):
-        eng.setState(start)
-        eng.setSvdFallback(enabled)
-        eng.substep(sc.dt, 1)
-        eng.sync()
+    if stats:
+        eng.svdFallbackStats(reset=True)
+    prof = calls = None
+    if profile:
+        prof = eng.profileSubsteps(sc.dt, steps)
+        calls = eng.profileCalls()
+        ms = sum(prof.values()) / steps
+    else:
         t0 = time.perf_counter()
-        eng.substep(sc.dt, k)
+        eng.substep(sc.dt, steps)
         eng.sync()
-        return (time.perf_counter() - t0) / k
+        ms = (time.perf_counter() - t0) / steps * 1e3
+    st = eng.svdFallbackStats() if stats else None
+    del eng
+    return ms, prof, calls, st
 
-    t_on, t_off = timed(True), timed(False)
-    eng.setSvdFallback(True)
-    eng.setState(start)
-    fb = eng.svdFallback()
+
+def svd_fallback_leg(sc, warmup, steps, ms_on):
+    """The FP64 SVD fallback of near-degenerate F (on by default, part of
+    value) over the timed window: the fraction of Drucker-Prager lanes and of
+    wavefronts that take it, and the window's time without it against the
+    time with it (ms_on, the window's own)."""
+    _, _, _, st = window_run(sc, warmup, steps, fallback=True, stats=True)
+    ms_off = window_run(sc, warmup, steps, fallback=False)[0]
+    from nebula_amd.mpm import SvdFallback  # noqa: F401  (the defaults below are the library's)
     lanes = max(st["dp_lanes"], 1)
-    res = {"gap_rel": round(fb.gap_rel, 9), "zero_rel": round(fb.zero_rel, 9), "substeps": k,
+    res = {"gap_rel": 1e-4, "zero_rel": 1e-6, "rounding_tol": 1e-6, "substeps": steps, "after": warmup,
            "return_map_lane_frac": round(st["fp64_return_map_lanes"] / lanes, 6),
            "stress_lane_frac": round(st["fp64_stress_lanes"] / lanes, 6),
            "wave_frac": round(st["fp64_waves"] / max(st["dp_waves"], 1), 6),
-           "ms_per_substep_on": round(t_on * 1e3, 5), "ms_per_substep_off": round(t_off * 1e3, 5),
-           "cost_frac": round(t_on / t_off - 1.0, 5)}
+           "ms_per_substep_on": round(ms_on, 5), "ms_per_substep_off": round(ms_off, 5),
+           "cost_frac": round(ms_on / ms_off - 1.0, 5)}
     log(f"[bench] FP64 SVD fallback: {res}")
     return res
 
@@ -343,9 +369,8 @@ def batch_leg(k):
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=480)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--profile-steps", type=int, default=20, help="substeps timed per kernel with HIP events")
     ap.add_argument("--cpu-steps", type=int, default=5, help="substeps of the CPU oracle sample (0 = skip)")
     ap.add_argument("--adjoint-substeps", type=int, default=16,
                     help="substeps of the single-scene reverse-mode leg (0 = skip)")
@@ -354,8 +379,8 @@ def main():
     ap.add_argument("--diff-segment", type=int, default=20, help="checkpoint spacing of that leg")
     ap.add_argument("--batch-substeps", type=int, default=20, help="substeps of the 256-scene leg (0 = skip)")
     ap.add_argument("--vm-steps", type=int, default=100, help="substeps of the Von Mises variant (0 = skip)")
-    ap.add_argument("--fallback-substeps", type=int, default=20,
-                    help="substeps of the FP64 SVD fallback's count and cost (0 = skip)")
+    ap.add_argument("--fallback", type=int, default=1,
+                    help="count the FP64 SVD fallback's lanes and time the window without it (0 = skip)")
     ap.add_argument("--host-frames", type=int, default=1, help="time host copies and whole frames with them (0 = skip)")
     ap.add_argument("--sweep-substeps", type=int, default=0,
                     help="substeps per size of the scene-size sweep, 1.05 M to 33.5 M particles (0 = skip)")
@@ -388,25 +413,32 @@ def main():
 
     g_act = eng.activeNodeCount()
     tiles = eng.activeTileCount()
-    log(f"[bench] profiling {args.profile_steps} substeps per kernel")
-    prof = eng.profileSubsteps(sc.dt, args.profile_steps)
-    calls = eng.profileCalls()
+    # the same window again on a fresh engine, every kernel timed with HIP
+    # events on the engine's stream
+    log(f"[bench] profiling the same {args.steps} substeps per kernel")
+    _, prof, calls, _ = window_run(sc, args.warmup, args.steps, profile=True)
     avg_ms = {k: v / max(calls.get(k, 0), 1) for k, v in prof.items() if calls.get(k, 0)}
-    per_substep_ms = {k: v / args.profile_steps for k, v in prof.items()}
+    per_substep_ms = {k: v / args.steps for k, v in prof.items()}
     dom = max(("p2g", "g2p", "grid_update", "g2p_grid"), key=lambda k: avg_ms.get(k, 0.0))
     dbytes = kernel_bytes(dom, n, g_act)
     achieved = dbytes / (avg_ms[dom] * 1e-3) / 1e9
     b_alg = 288 * n + 68 * g_act
-    roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": MEASURED_TRAFFIC.get(dom),
+    valu = MEASURED_VALU.get(dom)
+    valu_frac = valu * VALU_CYCLES / (SIMDS * CLOCK_HZ * avg_ms[dom] * 1e-3) if valu else None
+    hbm_frac = achieved / HBM_PEAK_GBS
+    roofline = {"kernel": dom, "bound": "valu" if valu_frac and valu_frac > hbm_frac else "hbm",
+                "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(hbm_frac, 4), "traffic": MEASURED_TRAFFIC.get(dom),
                 "traffic_source": TRAFFIC_SOURCE if dom in MEASURED_TRAFFIC else None,
+                "valu_frac": round(valu_frac, 4) if valu_frac else None,
+                "valu_insts_per_launch": valu, "valu_source": VALU_SOURCE if valu else None,
                 "bytes_per_launch": dbytes, "avg_launch_ms": round(avg_ms[dom], 5)}
     substep_rate = b_alg / (ms_per_step * 1e-3) / 1e9
 
     if args.calibration_copy_mb > 0:
         eng.calibrationCopy(args.calibration_copy_mb << 20, 3)
 
-    fallback = svd_fallback_leg(eng, sc, args.fallback_substeps) if args.fallback_substeps > 0 else None
+    fallback = svd_fallback_leg(sc, args.warmup, args.steps, ms_per_step) if args.fallback else None
     host = host_transfer_leg(eng, sc) if args.host_frames else None
     adjoint1 = adjoint_leg(sc, args.adjoint_substeps) if args.adjoint_substeps > 0 else None
     del eng

@@ -1278,20 +1278,39 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 // The FP64 fallback's second half: for every entry G2P queued (queue_fb),
 // the return map and Hencky stress in FP64 from the trial F (mode 1), or the
 // Hencky stress of the record's F (mode 2), rounded once; F is replaced and
-// the stress added to the record's (the mud viscosity term, or zero).  The
-// blocks of slot s are s, s + kFbSlots, ...; each reads its slot's length,
-// strides over the slot, then counts itself done, and the last block zeroes
-// the lengths and the count for the next G2P (every block has read its
-// length by then).
+// the stress added to the record's (the mud viscosity term, or zero).
+// Every block reads the slots' lengths and the grid strides over their
+// concatenation; each block then counts itself done, and the last block
+// zeroes the lengths and the count for the next G2P (every block has read
+// the lengths by then).
 template <bool BATCH>
 __global__ __launch_bounds__(256) void k_g2p_fb(SubParams p, float4* rec, const nb_material* __restrict__ mats) {
   __shared__ nb_material s_mats[BATCH ? 1 : NB_MAX_MATERIALS];
   stage_materials<BATCH>(p, mats, s_mats);
-  const uint32_t slot = blockIdx.x % kFbSlots, part = blockIdx.x / kFbSlots, parts = gridDim.x / kFbSlots;
-  const uint32_t len = p.fb_ctl[slot * kFbCtlStride];
+  // the slots' lengths and their exclusive prefix (wave 0), so that the
+  // grid's threads share the queue's entries evenly whatever the slots hold
+  __shared__ uint32_t s_pre[kFbSlots + 1];
+  if (threadIdx.x < 64u) {
+    const uint32_t len = threadIdx.x < kFbSlots ? p.fb_ctl[threadIdx.x * kFbCtlStride] : 0u;
+    uint32_t inc = len;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t o = uint32_t(__shfl_up(int(inc), d));
+      if (threadIdx.x >= uint32_t(d)) inc += o;
+    }
+    if (threadIdx.x < kFbSlots) s_pre[threadIdx.x] = inc - len;
+    if (threadIdx.x == kFbSlots - 1u) s_pre[kFbSlots] = inc;
+  }
+  __syncthreads();
+  const uint32_t total = s_pre[kFbSlots];
   const bool cs = p.corrected_svd != 0u;
-  for (uint32_t i = part * blockDim.x + threadIdx.x; i < len; i += parts * blockDim.x) {
-    const float4* e = p.fb_queue + (size_t(slot) * p.fb_region + i) * kFbEntry;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+    // the slot holding entry i of the concatenated queue: the last s with s_pre[s] <= i
+    uint32_t slot = 0u;
+#pragma unroll
+    for (uint32_t step = kFbSlots / 2u; step > 0u; step >>= 1)
+      if (s_pre[slot + step] <= i) slot += step;
+    const float4* e = p.fb_queue + (size_t(slot) * p.fb_region + (i - s_pre[slot])) * kFbEntry;
     const float4 e0 = e[0];
     const uint32_t q = __float_as_uint(e0.x), mode = __float_as_uint(e0.y);
     if (mode == 0u) continue;
@@ -1654,12 +1673,12 @@ void launch_g2p_fused(const DevState& d, const SubParams& p, uint32_t n_tiles, h
 void launch_g2p_fb(const DevState& d, const SubParams& p, float4* rec, hipStream_t s, Prof* pf) {
   if (!p.svd_fp64) return;
   ProfScope ps(pf, 12, s);
-  // kFbSlots groups of blocks, sized to the queue's capacity
-  const uint32_t parts = std::max(1u, std::min(cdiv(p.n, 256u), 1024u) / kFbSlots);
+  // a fixed grid, at most one thread per particle, that strides over the queue
+  const uint32_t g = std::min(cdiv(p.n, 256u), 1024u);
   if (p.n_scenes > 0)
-    k_g2p_fb<true><<<kFbSlots * parts, 256, 0, s>>>(p, rec, d.materials);
+    k_g2p_fb<true><<<g, 256, 0, s>>>(p, rec, d.materials);
   else
-    k_g2p_fb<false><<<kFbSlots * parts, 256, 0, s>>>(p, rec, d.materials);
+    k_g2p_fb<false><<<g, 256, 0, s>>>(p, rec, d.materials);
 }
 
 void launch_g2p(const DevState& d, const SubParams& p, bool use_sorted, hipStream_t s, Prof* pf, float4* rec_out) {
