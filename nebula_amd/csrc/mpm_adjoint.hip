@@ -23,8 +23,6 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#include <type_traits>
-
 #include "nb_adjoint_math.hpp"
 #include "nb_device_math.hpp"
 #include "nb_engine.hpp"
@@ -74,6 +72,28 @@ __device__ __forceinline__ void node_w(const AxisD& X, const AxisD& Y, const Axi
   n.d[0] = X.dx[ix];
   n.d[1] = Y.dx[iy];
   n.d[2] = Z.dx[iz];
+}
+
+// The transfers' weight cut as a bit per stencil node (bit (z 3 + y) 3 + x
+// set: node dropped) for per-axis weights wx, wy, wz: on a particle with a
+// node of 0 < w < limit (the smallest positive per-axis weights bound every
+// w from below), the nodes whose weight, multiplied in the forward kernel's
+// order, is below 1e-12.  Zero for every other particle.
+template <bool XY_FIRST>
+__device__ __forceinline__ uint32_t cut_mask(const float wx[3], const float wy[3], const float wz[3], float limit) {
+  uint32_t m = 0u;
+  if ((min_positive3(wx) * min_positive3(wy)) * min_positive3(wz) < limit) {
+#pragma unroll
+    for (int kz = 0; kz < 3; kz++)
+#pragma unroll
+      for (int jy = 0; jy < 3; jy++)
+#pragma unroll
+        for (int ix = 0; ix < 3; ix++) {
+          const float w = XY_FIRST ? (wx[ix] * wy[jy]) * wz[kz] : wx[ix] * (wy[jy] * wz[kz]);
+          if (w < 1e-12f) m |= 1u << ((kz * 3 + jy) * 3 + ix);
+        }
+  }
+  return m;
 }
 
 // ---- separable gathers ----
@@ -316,7 +336,7 @@ __global__ __launch_bounds__(256) void k_g2p_bwd(SubParams p, const float4* __re
     }
     // G2P's weight cut (shaders/mpm_g2p.comp:93-101) as its forward takes it:
     // on a particle with a node of 0 < w < 1e-12, nodes with (wx wy) wz < 1e-12
-    const bool tiny = (min_positive3(ax[0].w) * min_positive3(ax[1].w)) * min_positive3(ax[2].w) < 1e-12f;
+    const uint32_t cutm = cut_mask<true>(ax[0].w, ax[1].w, ax[2].w, 1e-12f);
     // grid velocity of node (ix, iy, iz); out-of-grid, massless (mass < 1e-10)
     // and cut nodes read as zero (shaders/mpm_g2p.comp:93-101)
     auto node_v = [&](int ix, int iy, int iz) -> float4 {
@@ -326,7 +346,7 @@ __global__ __launch_bounds__(256) void k_g2p_bwd(SubParams p, const float4* __re
       const uint32_t cz = uint32_t(min(max(nz, 0), sg.grid_res[2] - 1));
       const float4 g = grid_node[node_id(sg, cx, cy, cz)];
       // bitwise tests: the load stays unconditional (one 16-byte load)
-      const bool cut = tiny & (ax[0].w[ix] * ax[1].w[iy] * ax[2].w[iz] < 1e-12f);
+      const bool cut = (cutm >> ((iz * 3 + iy) * 3 + ix)) & 1u;
       const bool live = (uint32_t(nx) < uint32_t(sg.grid_res[0])) & (uint32_t(ny) < uint32_t(sg.grid_res[1])) &
                         (uint32_t(nz) < uint32_t(sg.grid_res[2])) & (g.w >= 1e-10f) & !cut;
       return live ? make_float4(g.x, g.y, g.z, 0.f) : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -620,49 +640,33 @@ __global__ __launch_bounds__(kAdjThreads) void k_adj_scatter(SubParams p, const 
         //   P = wy wz (A + dy M_B[1] + dz M_B[2]) + gy wz M_G[1] + wy gz M_G[2],
         //   Q = wy wz M_B[0],  R = wy wz M_G[0]
         // G2P's weight cut as its forward takes it: on a particle with a node
-        // of 0 < w < 1e-12, the nodes with (wx wy) wz < 1e-12 get nothing
-        // (a wave with no such particle takes the unmasked loop).
-        const bool tiny = (min_positive3(X.w) * min_positive3(Y.w)) * min_positive3(Z.w) < 1e-12f;
+        // of 0 < w < 1e-12, the nodes with (wx wy) wz < 1e-12 get nothing.
+        const uint32_t cutm = cut_mask<true>(X.w, Y.w, Z.w, 1e-12f);
         float wdx[3];
 #pragma unroll
         for (int ix = 0; ix < 3; ix++) wdx[ix] = X.w[ix] * X.dx[ix];
-        auto rows = [&](auto masked) {
 #pragma unroll
-          for (int kz = 0; kz < 3; kz++)
+        for (int kz = 0; kz < 3; kz++)
 #pragma unroll
-            for (int jy = 0; jy < 3; jy++) {
-              const float wyz = Y.w[jy] * Z.w[kz], gyz = Y.g[jy] * Z.w[kz], wgz = Y.w[jy] * Z.g[kz];
-              float P[3], Q[3], R[3];
+          for (int jy = 0; jy < 3; jy++) {
+            const float wyz = Y.w[jy] * Z.w[kz], gyz = Y.g[jy] * Z.w[kz], wgz = Y.w[jy] * Z.g[kz];
+            float P[3], Q[3], R[3];
 #pragma unroll
-              for (int rr = 0; rr < 3; rr++) {
-                const float t = fmaf(Z.dx[kz], cf[9 + rr], fmaf(Y.dx[jy], cf[6 + rr], cf[rr]));
-                P[rr] = fmaf(wgz, cf[18 + rr], fmaf(gyz, cf[15 + rr], wyz * t));
-                Q[rr] = wyz * cf[3 + rr];
-                R[rr] = wyz * cf[12 + rr];
-              }
-#pragma unroll
-              for (int ix = 0; ix < 3; ix++) {
-                const int o = (kz * 3 + jy) * 3 + ix;
-                const float nx_ = fmaf(X.g[ix], R[0], fmaf(wdx[ix], Q[0], fmaf(X.w[ix], P[0], ax[o])));
-                const float ny_ = fmaf(X.g[ix], R[1], fmaf(wdx[ix], Q[1], fmaf(X.w[ix], P[1], ay[o])));
-                const float nz_ = fmaf(X.g[ix], R[2], fmaf(wdx[ix], Q[2], fmaf(X.w[ix], P[2], az[o])));
-                if constexpr (decltype(masked)::value) {
-                  const bool cut = tiny & (X.w[ix] * Y.w[jy] * Z.w[kz] < 1e-12f);
-                  ax[o] = cut ? ax[o] : nx_;
-                  ay[o] = cut ? ay[o] : ny_;
-                  az[o] = cut ? az[o] : nz_;
-                } else {
-                  ax[o] = nx_;
-                  ay[o] = ny_;
-                  az[o] = nz_;
-                }
-              }
+            for (int rr = 0; rr < 3; rr++) {
+              const float t = fmaf(Z.dx[kz], cf[9 + rr], fmaf(Y.dx[jy], cf[6 + rr], cf[rr]));
+              P[rr] = fmaf(wgz, cf[18 + rr], fmaf(gyz, cf[15 + rr], wyz * t));
+              Q[rr] = wyz * cf[3 + rr];
+              R[rr] = wyz * cf[12 + rr];
             }
-        };
-        if (__any(tiny))
-          rows(std::true_type{});
-        else
-          rows(std::false_type{});
+#pragma unroll
+            for (int ix = 0; ix < 3; ix++) {
+              const int o = (kz * 3 + jy) * 3 + ix;
+              const bool cut = (cutm >> o) & 1u;
+              ax[o] = cut ? ax[o] : fmaf(X.g[ix], R[0], fmaf(wdx[ix], Q[0], fmaf(X.w[ix], P[0], ax[o])));
+              ay[o] = cut ? ay[o] : fmaf(X.g[ix], R[1], fmaf(wdx[ix], Q[1], fmaf(X.w[ix], P[1], ay[o])));
+              az[o] = cut ? az[o] : fmaf(X.g[ix], R[2], fmaf(wdx[ix], Q[2], fmaf(X.w[ix], P[2], az[o])));
+            }
+          }
       }
 #pragma unroll
       for (int kz = 0; kz < 3; kz++)
@@ -830,12 +834,12 @@ __global__ __launch_bounds__(256) void k_p2g_bwd(SubParams p, const float4* __re
   // P2G's weight cut (shaders/mpm_p2g.comp:61) as its forward takes it: a
   // particle whose smallest positive per-axis weights multiply below
   // 1.0001e-12 skips the nodes with wx (wy wz) < 1e-12
-  const bool tiny = (min_positive3(ax[0].w) * min_positive3(ax[1].w)) * min_positive3(ax[2].w) < 1.0001e-12f;
+  const uint32_t cutm = cut_mask<false>(ax[0].w, ax[1].w, ax[2].w, 1.0001e-12f);
   // grid adjoint (dL/dp, dL/dm) of node (ix, iy, iz); nodes outside the cube
   // and cut nodes read as zero
   auto node_g = [&](int ix, int iy, int iz) -> float4 {
     const uint32_t nx = uint32_t(cc[0] - 1 + ix), ny = uint32_t(cc[1] - 1 + iy), nz = uint32_t(cc[2] - 1 + iz);
-    const bool cut = tiny & (ax[0].w[ix] * (ax[1].w[iy] * ax[2].w[iz]) < 1e-12f);
+    const bool cut = (cutm >> ((iz * 3 + iy) * 3 + ix)) & 1u;
     const bool in = (nx < cube) & (ny < cube) & (nz < cube) & !cut;  // negative wraps to large
     const float4 g = grid_adj[node_id(sg, min(nx, cube - 1u), min(ny, cube - 1u), min(nz, cube - 1u))];
     return in ? g : make_float4(0.f, 0.f, 0.f, 0.f);
