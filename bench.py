@@ -205,7 +205,7 @@ def svd_fallback_leg(sc, warmup, steps, ms_on):
     ms_off = window_run(sc, warmup, steps, fallback=False)[0]
     from nebula_amd.mpm import SvdFallback  # noqa: F401  (the defaults below are the library's)
     lanes = max(st["dp_lanes"], 1)
-    res = {"gap_rel": 1e-4, "zero_rel": 1e-6, "rounding_tol": 1e-6, "substeps": steps, "after": warmup,
+    res = {"gap_rel": 1e-4, "zero_rel": 1e-6, "rounding_tol": 1e-5, "substeps": steps, "after": warmup,
            "return_map_lane_frac": round(st["fp64_return_map_lanes"] / lanes, 6),
            "stress_lane_frac": round(st["fp64_stress_lanes"] / lanes, 6),
            "wave_frac": round(st["fp64_waves"] / max(st["dp_waves"], 1), 6),

@@ -166,8 +166,9 @@ typedef struct nb_mpm nb_mpm; /* opaque engine handle */
  * projection and recomposition) and its Hencky stress are computed again in
  * FP64, with the same algorithm, and rounded to FP32 once, when s2 is that
  * small, or when such a rotation of a close pair would move the FP32 F' or
- * tau by more than 1e-6 of its scale (max |F'|; max(max |tau|,
- * 2 mu + 3 lambda)).  The reference convention's F <- U^T exp(eps) V and
+ * tau by more than 1e-5 of its scale (max |F'|; max(max |tau|,
+ * 2 mu + 3 lambda)), the accuracy the FP32 path has anyway where the
+ * singular values are well separated.  The reference convention's F <- U^T exp(eps) V and
  * tau = U^T k U change under the rotation by its angle times their
  * commutator with the rotation's generator; the corrected convention's by
  * the angle times the difference of the pair's values.  An F within rounding

@@ -409,12 +409,15 @@ __device__ __forceinline__ void svd3(const mat3<T>& F, mat3<T>& U, T sig[3], mat
 // A lane takes the FP64 path when s2^2 <= zero_rel s0^2, or when a pair
 // with s_i^2 - s_j^2 <= gap_rel s0^2 moves F' or tau by more than
 // kFbRoundTol of its scale (max |F'|; max(max |tau|, 2 mu + 3 lambda), the
-// stress of a unit strain).  Elsewhere rounding does not decide the result:
+// stress of a unit strain).  1e-5 is the accuracy the FP32 path has anyway on
+// well-separated singular values (its F' and tau land about 2e-6 and 1e-5 of
+// their scale from the FP64 map's there): below it the rotation's error
+// disappears in that.  Elsewhere rounding does not decide the result:
 // an F within rounding of a multiple of the identity (a body at rest or in
 // free fall) has all three pairs close, but there F' and tau are within
 // rounding of multiples of the identity too, and commute with every K_ij.
 constexpr float kFbRound = 2.3841858e-7f;  // 2^-22
-constexpr float kFbRoundTol = 1e-6f;
+constexpr float kFbRoundTol = 1e-5f;
 
 // rotation angles of the pairs (0, 1), (1, 2), (0, 2); 0 for a pair whose
 // gap exceeds gap_rel (an equal pair: 1)
@@ -422,7 +425,8 @@ __device__ __forceinline__ void fb_pair_angles(const float sig[3], float gap_rel
   const float s0 = sig[0] * sig[0], s1 = sig[1] * sig[1], s2 = sig[2] * sig[2];
   const float g[3] = {s0 - s1, s1 - s2, s0 - s2};
 #pragma unroll
-  for (int k = 0; k < 3; k++) th[k] = g[k] <= gap_rel * s0 ? fminf(1.f, kFbRound * s0 / fmaxf(g[k], 0.f)) : 0.f;
+  for (int k = 0; k < 3; k++)
+    th[k] = g[k] <= gap_rel * s0 ? fminf(1.f, kFbRound * s0 * hw_rcp(fmaxf(g[k], 0.f))) : 0.f;
 }
 
 // max |(X K_ij - K_ij X)(a, b)| with X(a, b) = X.m[b][a]
