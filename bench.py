@@ -54,18 +54,19 @@ HBM_PEAK_GBS = 8000.0
 # VALU issue: 1024 SIMDs at 2.4 GHz, 4 cycles per wave64 instruction (16
 # lanes a cycle for FP32 without packed math, and for FP64)
 SIMDS, CLOCK_HZ, VALU_CYCLES = 1024, 2.4e9, 4
-# HBM bytes per launch of the substep kernels on this workload, measured with
-# two rocprofv3 counter passes (FETCH_SIZE, then WRITE_SIZE) and scaled by what
-# the same counters read for a 256 MiB copy kernel in the same process (fetch
-# x 2.0, write x 1.0).  A counter pass cannot run inside the timed bench; the
-# commands and the per-kernel table are in the file named below.
+# HBM bytes per launch of the substep kernels over the default window, measured
+# with rocprofv3 counter passes of this build (FETCH_SIZE, then WRITE_SIZE),
+# scaled by what the same counters read for a 256 MiB copy kernel in the same
+# process (fetch x 2.0, write x 1.0) and checked against the TCC request
+# counters.  A counter pass cannot run inside the timed bench; the commands and
+# the per-kernel table are in the file named below.
 TRAFFIC_SOURCE = "profiles/hbm_traffic_snow_terrain_256.md"
-MEASURED_TRAFFIC = {"g2p": 1469.9e6, "p2g": 856.5e6, "grid_update": 116.9e6}
+MEASURED_TRAFFIC = {"g2p": 1525.7e6, "p2g": 856.5e6, "grid_update": 120.9e6}
 # VALU instructions (wave64) per launch over the default window, from a
-# rocprofv3 SQ_INSTS_VALU pass of this command's window (file below): with the
-# live average launch time they give the fraction of the VALU issue peak.
+# rocprofv3 SQ_INSTS_VALU pass of this build (file below): with the live
+# average launch time they give the fraction of the VALU issue peak.
 VALU_SOURCE = "profiles/counters_snow_terrain_256.md"
-MEASURED_VALU = {}
+MEASURED_VALU = {"g2p": 1.869e8, "p2g": 7.094e7, "grid_update": 5.518e6}
 
 
 def log(*a):

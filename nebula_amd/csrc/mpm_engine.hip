@@ -181,7 +181,15 @@ struct nb_mpm {
     p.noise_forces = noise_force_count;
     p.write_old = write_old ? 1u : 0u;
     p.corrected_svd = knobs.corrected_svd ? 1u : 0u;
-    p.svd_fp64 = svd_fb.enabled ? 1u : 0u;
+    // the fallback serves the Drucker-Prager family only: with no such
+    // material G2P queues nothing and its fix-up kernel is not launched
+    bool dp_family = false;
+    for (uint32_t m = 0; m < material_count && m < mats_host.size(); m++) {
+      const uint32_t model = mats_host[m].model;
+      dp_family = dp_family || model == NB_MAT_DRUCKER_PRAGER || model == NB_MAT_GRANULAR_POWDER ||
+                  model == NB_MAT_VISCOPLASTIC_MUD;
+    }
+    p.svd_fp64 = (svd_fb.enabled && dp_family) ? 1u : 0u;
     p.svd_gap_rel = svd_fb.gap_rel;
     p.svd_zero_rel = svd_fb.zero_rel;
     p.svd_stats = svd_fb.count_stats ? d_svd_stats : nullptr;
