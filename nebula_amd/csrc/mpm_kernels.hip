@@ -497,9 +497,13 @@ __global__ __launch_bounds__(kP2GThreads, 2) void k_p2g(SubParams p, const float
       // Q = wx dx C e_x - gx tau e_x: 7 FMAs per node (explicit FMA chains;
       // strict evaluation of acc + (a b + c d) costs a multiply, an FMA and an
       // add).  A particle with a sub-cutoff node is scaled to zero here and
-      // added by the exact loop below.  The next particle's record is loaded
-      // while this one is scattered.
-      bool any_tiny = false;
+      // added by the exact loop below, which visits only those particles:
+      // the first two by their offsets from kbeg (16 bits each in tiny_at,
+      // 0xFFFF = none), all of the item's particles when there are more (or
+      // an offset does not fit).  The next particle's record is loaded while
+      // this one is scattered.
+      uint32_t tiny_at = 0xFFFFFFFFu;
+      bool tiny_all = false;
       float4 npp, nvv, nb0, nb1, nb2, ntt;
       auto load_rec = [&](uint32_t q) {
         const float4* r = rec + size_t(q) * kRec;
@@ -514,7 +518,16 @@ __global__ __launch_bounds__(kP2GThreads, 2) void k_p2g(SubParams p, const float
         Src q;
         stencil(pp, q);
         const bool tiny = has_tiny(q);
-        any_tiny |= tiny;
+        if (tiny) {
+          const uint32_t off = k - kbeg;
+          if (off >= 0xFFFFu || (tiny_at >> 16) != 0xFFFFu) {
+            tiny_all = true;
+          } else if ((tiny_at & 0xFFFFu) == 0xFFFFu) {
+            tiny_at = 0xFFFF0000u | off;
+          } else {
+            tiny_at = (tiny_at & 0xFFFFu) | (off << 16);
+          }
+        }
         setup(pp, vv, b0, b1, b2, tt, tiny ? 0.f : 1.f, q);
         const float mp = q.mp, mvx = q.mvx, mvy = q.mvy, mvz = q.mvz;
         const float c00 = q.c00, c01 = q.c01, c02 = q.c02, c10 = q.c10, c11 = q.c11, c12 = q.c12;
@@ -554,9 +567,14 @@ __global__ __launch_bounds__(kP2GThreads, 2) void k_p2g(SubParams p, const float
           }
         }
       }
-      // Exact loop: the particles skipped above, node by node with the cutoff.
-      if (__any(any_tiny)) {
-        for (uint32_t k = kbeg; k < kend; k++) {
+      // Exact loop: the particles skipped above, node by node with the cutoff,
+      // in bin order (the order of the adds is the one a walk over the whole
+      // item would give).
+      const uint32_t n_exact = tiny_all ? kend - kbeg
+                                        : ((tiny_at & 0xFFFFu) != 0xFFFFu ? 1u : 0u) + ((tiny_at >> 16) != 0xFFFFu ? 1u : 0u);
+      if (__any(n_exact != 0u)) {
+        for (uint32_t s = 0; s < n_exact; s++) {
+          const uint32_t k = kbeg + (tiny_all ? s : (s == 0u ? (tiny_at & 0xFFFFu) : (tiny_at >> 16)));
           const float4* r = rec + size_t(idx_of(k)) * kRec;
           Src q;
           const float4 pp = r[R_P];
