@@ -370,6 +370,7 @@ __global__ __launch_bounds__(kP2GThreads, 2) void k_p2g(SubParams p, const float
   __shared__ float s_m[kBoxNodes], s_x[kBoxNodes], s_y[kBoxNodes], s_z[kBoxNodes];
   __shared__ uint16_t s_item[kTileCells * kMaxChunks];  // cell | chunk << 9
   __shared__ uint32_t s_layers;
+  __shared__ uint32_t s_cnt[kTileCells], s_end[kTileCells];
   __shared__ uint32_t s_idx[kIdxStage];
   __shared__ uint64_t s_scan64[kP2GThreads / 64 + 1];
   __shared__ float s_rho0[BATCH ? kBatchMaxMaterials : NB_MAX_MATERIALS];
@@ -397,6 +398,10 @@ __global__ __launch_bounds__(kP2GThreads, 2) void k_p2g(SubParams p, const float
     if (threadIdx.x == 0) s_layers = 0u;
     const uint32_t ca = 2u * threadIdx.x, cb = ca + 1u;
     const uint32_t na_ = cell_count[tbase + ca], nb_ = cell_count[tbase + cb];
+    s_cnt[ca] = na_;
+    s_cnt[cb] = nb_;
+    s_end[ca] = cell_end[tbase + ca];
+    s_end[cb] = cell_end[tbase + cb];
     const uint32_t qa = p2g_chunks(na_, p.p2g_item), qb = p2g_chunks(nb_, p.p2g_item);
     const uint32_t ga = p2g_group(na_, qa), gb = p2g_group(nb_, qb);
     const uint64_t mine = (uint64_t(qa) << (12 * ga)) + (uint64_t(qb) << (12 * gb));
@@ -443,7 +448,7 @@ __global__ __launch_bounds__(kP2GThreads, 2) void k_p2g(SubParams p, const float
       const int cx = tx * 8 + lx, cy = ty * 8 + ly, cz = tz * 8 + lz;
       uint32_t kbeg = 0, kend = 0;
       if (have) {
-        const uint32_t cnt = cell_count[tbase + c], end = cell_end[tbase + c];
+        const uint32_t cnt = s_cnt[c], end = s_end[c];
         const uint32_t per = (cnt + p2g_chunks(cnt, p.p2g_item) - 1u) / p2g_chunks(cnt, p.p2g_item);
         kbeg = end - cnt + chunk * per;
         kend = min(kbeg + per, end);
@@ -501,9 +506,10 @@ __global__ __launch_bounds__(kP2GThreads, 2) void k_p2g(SubParams p, const float
       // the first two by their offsets from kbeg (16 bits each in tiny_at,
       // 0xFFFF = none), all of the item's particles when there are more (or
       // an offset does not fit).  The next particle's record is loaded while
-      // this one is scattered.
+      // this one is scattered; the walk's last iteration loads the first such
+      // particle's record instead (pref), so the exact loop rarely waits.
       uint32_t tiny_at = 0xFFFFFFFFu;
-      bool tiny_all = false;
+      bool tiny_all = false, pref = false;
       float4 npp, nvv, nb0, nb1, nb2, ntt;
       auto load_rec = [&](uint32_t q) {
         const float4* r = rec + size_t(q) * kRec;
@@ -514,7 +520,12 @@ __global__ __launch_bounds__(kP2GThreads, 2) void k_p2g(SubParams p, const float
       if (kbeg < kend) load_rec(idx_of(kbeg));
       for (uint32_t k = kbeg; k < kend; k++) {
         const float4 pp = npp, vv = nvv, b0 = nb0, b1 = nb1, b2 = nb2, tt = ntt;
-        if (k + 1 < kend) load_rec(idx_of(k + 1));
+        if (k + 1 < kend) {
+          load_rec(idx_of(k + 1));
+        } else if ((tiny_at & 0xFFFFu) != 0xFFFFu) {
+          load_rec(idx_of(kbeg + (tiny_at & 0xFFFFu)));
+          pref = true;
+        }
         Src q;
         stencil(pp, q);
         const bool tiny = has_tiny(q);
@@ -575,12 +586,12 @@ __global__ __launch_bounds__(kP2GThreads, 2) void k_p2g(SubParams p, const float
       if (__any(n_exact != 0u)) {
         for (uint32_t s = 0; s < n_exact; s++) {
           const uint32_t k = kbeg + (tiny_all ? s : (s == 0u ? (tiny_at & 0xFFFFu) : (tiny_at >> 16)));
-          const float4* r = rec + size_t(idx_of(k)) * kRec;
+          if (!(pref && !tiny_all && s == 0u)) load_rec(idx_of(k));
           Src q;
-          const float4 pp = r[R_P];
+          const float4 pp = npp;
           stencil(pp, q);
           if (!has_tiny(q)) continue;
-          setup(pp, r[R_V], r[R_B0], r[R_B1], r[R_B2], r[R_T], 1.f, q);
+          setup(pp, nvv, nb0, nb1, nb2, ntt, 1.f, q);
 #pragma unroll
           for (int kz = 0; kz < 3; kz++) {
 #pragma unroll
