@@ -356,6 +356,53 @@ __device__ __forceinline__ uint32_t p2g_group(uint32_t count, uint32_t chunks) {
   return per >= 12u ? 0u : (per >= 10u ? 1u : (per >= 8u ? 2u : (per >= 6u ? 3u : 4u)));
 }
 
+// One particle's node contributions, in the factored chains described at the
+// walk below.  MASK: wx and wyz are set to zero at each node whose weight
+// (wx wy) wz the reference skips (fma(a, 0, s) leaves s as it was); every
+// other node gets the same bits as without the mask.
+#define NB_P2G_CHAINS(MASK)                                                                          \
+  {                                                                                                   \
+    const float mp = q.mp, mvx = q.mvx, mvy = q.mvy, mvz = q.mvz;                                     \
+    const float c00 = q.c00, c01 = q.c01, c02 = q.c02, c10 = q.c10, c11 = q.c11, c12 = q.c12;         \
+    const float c20 = q.c20, c21 = q.c21, c22 = q.c22;                                                \
+    const float t00 = q.t00, t11 = q.t11, t22 = q.t22, t01 = q.t01, t02 = q.t02, t12 = q.t12;         \
+    const Axis3 &sx = q.sx, &sy = q.sy, &sz = q.sz;                                                   \
+    float mwx[3], Qx[3], Qy[3], Qz[3], Ayx[3], Ayy[3], Ayz[3];                                        \
+    _Pragma("unroll") for (int i = 0; i < 3; i++) {                                                   \
+      mwx[i] = mp * sx.w[i];                                                                          \
+      const float wdx = sx.w[i] * sx.dx[i];                                                           \
+      Qx[i] = fmaf(-sx.g[i], t00, wdx * c00);                                                         \
+      Qy[i] = fmaf(-sx.g[i], t01, wdx * c01);                                                         \
+      Qz[i] = fmaf(-sx.g[i], t02, wdx * c02);                                                         \
+      Ayx[i] = fmaf(c10, sy.dx[i], mvx);                                                              \
+      Ayy[i] = fmaf(c11, sy.dx[i], mvy);                                                              \
+      Ayz[i] = fmaf(c12, sy.dx[i], mvz);                                                              \
+    }                                                                                                 \
+    _Pragma("unroll") for (int kz = 0; kz < 3; kz++) {                                                \
+      _Pragma("unroll") for (int jy = 0; jy < 3; jy++) {                                              \
+        const float wyz = sy.w[jy] * sz.w[kz];                                                        \
+        const float gy_wz = sy.g[jy] * sz.w[kz];                                                      \
+        const float wy_gz = sy.w[jy] * sz.g[kz];                                                      \
+        const float Px = fmaf(-t02, wy_gz, fmaf(wyz, fmaf(c20, sz.dx[kz], Ayx[jy]), -(t01 * gy_wz))); \
+        const float Py = fmaf(-t12, wy_gz, fmaf(wyz, fmaf(c21, sz.dx[kz], Ayy[jy]), -(t11 * gy_wz))); \
+        const float Pz = fmaf(-t22, wy_gz, fmaf(wyz, fmaf(c22, sz.dx[kz], Ayz[jy]), -(t12 * gy_wz))); \
+        _Pragma("unroll") for (int ix = 0; ix < 3; ix++) {                                            \
+          const int o = (kz * 3 + jy) * 3 + ix;                                                       \
+          float wx = sx.w[ix], wn = wyz;                                                              \
+          if (MASK) {                                                                                 \
+            const bool live = (sx.w[ix] * sy.w[jy]) * sz.w[kz] >= 1e-12f;                             \
+            wx = live ? wx : 0.f;                                                                     \
+            wn = live ? wn : 0.f;                                                                     \
+          }                                                                                           \
+          am[o] = fmaf(mwx[ix], wn, am[o]);                                                           \
+          ax[o] = fmaf(wn, Qx[ix], fmaf(wx, Px, ax[o]));                                              \
+          ay[o] = fmaf(wn, Qy[ix], fmaf(wx, Py, ay[o]));                                              \
+          az[o] = fmaf(wn, Qz[ix], fmaf(wx, Pz, az[o]));                                              \
+        }                                                                                             \
+      }                                                                                               \
+    }                                                                                                 \
+  }
+
 template <bool BATCH>
 __global__ __launch_bounds__(kP2GThreads, 2) void k_p2g(SubParams p, const float4* __restrict__ rec,
                                                      const nb_material* __restrict__ mats,
@@ -540,47 +587,11 @@ __global__ __launch_bounds__(kP2GThreads, 2) void k_p2g(SubParams p, const float
           }
         }
         setup(pp, vv, b0, b1, b2, tt, tiny ? 0.f : 1.f, q);
-        const float mp = q.mp, mvx = q.mvx, mvy = q.mvy, mvz = q.mvz;
-        const float c00 = q.c00, c01 = q.c01, c02 = q.c02, c10 = q.c10, c11 = q.c11, c12 = q.c12;
-        const float c20 = q.c20, c21 = q.c21, c22 = q.c22;
-        const float t00 = q.t00, t11 = q.t11, t22 = q.t22, t01 = q.t01, t02 = q.t02, t12 = q.t12;
-        const Axis3 &sx = q.sx, &sy = q.sy, &sz = q.sz;
-        float mwx[3], Qx[3], Qy[3], Qz[3], Ayx[3], Ayy[3], Ayz[3];
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-          mwx[i] = mp * sx.w[i];
-          const float wdx = sx.w[i] * sx.dx[i];
-          Qx[i] = fmaf(-sx.g[i], t00, wdx * c00);
-          Qy[i] = fmaf(-sx.g[i], t01, wdx * c01);
-          Qz[i] = fmaf(-sx.g[i], t02, wdx * c02);
-          Ayx[i] = fmaf(c10, sy.dx[i], mvx);
-          Ayy[i] = fmaf(c11, sy.dx[i], mvy);
-          Ayz[i] = fmaf(c12, sy.dx[i], mvz);
-        }
-#pragma unroll
-        for (int kz = 0; kz < 3; kz++) {
-#pragma unroll
-          for (int jy = 0; jy < 3; jy++) {
-            const float wyz = sy.w[jy] * sz.w[kz];
-            const float gy_wz = sy.g[jy] * sz.w[kz];
-            const float wy_gz = sy.w[jy] * sz.g[kz];
-            const float Px = fmaf(-t02, wy_gz, fmaf(wyz, fmaf(c20, sz.dx[kz], Ayx[jy]), -(t01 * gy_wz)));
-            const float Py = fmaf(-t12, wy_gz, fmaf(wyz, fmaf(c21, sz.dx[kz], Ayy[jy]), -(t11 * gy_wz)));
-            const float Pz = fmaf(-t22, wy_gz, fmaf(wyz, fmaf(c22, sz.dx[kz], Ayz[jy]), -(t12 * gy_wz)));
-#pragma unroll
-            for (int ix = 0; ix < 3; ix++) {
-              const int o = (kz * 3 + jy) * 3 + ix;
-              am[o] = fmaf(mwx[ix], wyz, am[o]);
-              ax[o] = fmaf(wyz, Qx[ix], fmaf(sx.w[ix], Px, ax[o]));
-              ay[o] = fmaf(wyz, Qy[ix], fmaf(sx.w[ix], Py, ay[o]));
-              az[o] = fmaf(wyz, Qz[ix], fmaf(sx.w[ix], Pz, az[o]));
-            }
-          }
-        }
+        NB_P2G_CHAINS(false)
       }
-      // Exact loop: the particles skipped above, node by node with the cutoff,
-      // in bin order (the order of the adds is the one a walk over the whole
-      // item would give).
+      // Exact loop: the particles skipped above, the same chains with the
+      // cutoff's node mask, in bin order (the order of the adds is the one a
+      // walk over the whole item would give).
       const uint32_t n_exact = tiny_all ? kend - kbeg
                                         : ((tiny_at & 0xFFFFu) != 0xFFFFu ? 1u : 0u) + ((tiny_at >> 16) != 0xFFFFu ? 1u : 0u);
       if (__any(n_exact != 0u)) {
@@ -592,37 +603,7 @@ __global__ __launch_bounds__(kP2GThreads, 2) void k_p2g(SubParams p, const float
           stencil(pp, q);
           if (!has_tiny(q)) continue;
           setup(pp, nvv, nb0, nb1, nb2, ntt, 1.f, q);
-#pragma unroll
-          for (int kz = 0; kz < 3; kz++) {
-#pragma unroll
-            for (int jy = 0; jy < 3; jy++) {
-              const float wyz = q.sy.w[jy] * q.sz.w[kz];
-              const float gy_wz = q.sy.g[jy] * q.sz.w[kz];
-              const float wy_gz = q.sy.w[jy] * q.sz.g[kz];
-              // m v + C (0, dy, dz)
-              const float Ax = q.mvx + q.c10 * q.sy.dx[jy] + q.c20 * q.sz.dx[kz];
-              const float Ay = q.mvy + q.c11 * q.sy.dx[jy] + q.c21 * q.sz.dx[kz];
-              const float Az = q.mvz + q.c12 * q.sy.dx[jy] + q.c22 * q.sz.dx[kz];
-              // tau (0, gy wz, wy gz); the wx factor is applied per node
-              const float Tx = q.t01 * gy_wz + q.t02 * wy_gz;
-              const float Ty = q.t11 * gy_wz + q.t12 * wy_gz;
-              const float Tz = q.t12 * gy_wz + q.t22 * wy_gz;
-#pragma unroll
-              for (int ix = 0; ix < 3; ix++) {
-                const int o = (kz * 3 + jy) * 3 + ix;
-                float w = q.sx.w[ix] * wyz;
-                const bool live = w >= 1e-12f;
-                w = live ? w : 0.f;
-                const float wx = live ? q.sx.w[ix] : 0.f;
-                const float wd = w * q.sx.dx[ix];
-                const float gw = live ? q.sx.g[ix] * wyz : 0.f;
-                am[o] += w * q.mp;
-                ax[o] += w * Ax + wd * q.c00 - gw * q.t00 - wx * Tx;
-                ay[o] += w * Ay + wd * q.c01 - gw * q.t01 - wx * Ty;
-                az[o] += w * Az + wd * q.c02 - gw * q.t02 - wx * Tz;
-              }
-            }
-          }
+          NB_P2G_CHAINS(true)
         }
       }
       for (uint32_t layer = 0; layer < n_layers; layer++) {
@@ -651,6 +632,8 @@ __global__ __launch_bounds__(kP2GThreads, 2) void k_p2g(SubParams p, const float
     __syncthreads();
   }
 }
+
+#undef NB_P2G_CHAINS
 
 // ---------------------------------------------------------------------------
 // GRID UPDATE
@@ -1309,9 +1292,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 // Hencky stress of the record's F (mode 2), rounded once; F is replaced and
 // the stress added to the record's (the mud viscosity term, or zero).
 // Every block reads the slots' lengths and the grid strides over their
-// concatenation; each block then counts itself done, and the last block
-// zeroes the lengths and the count for the next G2P (every block has read
-// the lengths by then).
+// concatenation; k_g2p_fb_reset then zeroes the lengths for the next G2P.
+// (A count of finished blocks, the last one zeroing, cost tens of ns per
+// block: device-scope atomics on one address serialise across the XCDs.)
 template <bool BATCH>
 __global__ __launch_bounds__(256) void k_g2p_fb(SubParams p, float4* rec, const nb_material* __restrict__ mats) {
   __shared__ nb_material s_mats[BATCH ? 1 : NB_MAX_MATERIALS];
@@ -1384,14 +1367,10 @@ __global__ __launch_bounds__(256) void k_g2p_fb(SubParams p, float4* rec, const 
       ro[R_FB] = make_float4(F.m[1][2], F.m[2][0], F.m[2][1], F.m[2][2]);
     }
   }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    if (atomicAdd(&p.fb_ctl[kFbSlots * kFbCtlStride], 1u) == gridDim.x - 1u) {
-      for (uint32_t t = 0; t < kFbSlots; t++) atomicExch(&p.fb_ctl[t * kFbCtlStride], 0u);
-      atomicExch(&p.fb_ctl[kFbSlots * kFbCtlStride], 0u);
-    }
-  }
+}
+
+__global__ __launch_bounds__(64) void k_g2p_fb_reset(uint32_t* __restrict__ fb_ctl) {
+  if (threadIdx.x < kFbSlots) fb_ctl[threadIdx.x * kFbCtlStride] = 0u;
 }
 
 // Grid update and G2P fused, one block per work unit: a unit is up to
@@ -1702,12 +1681,15 @@ void launch_g2p_fused(const DevState& d, const SubParams& p, uint32_t n_tiles, h
 void launch_g2p_fb(const DevState& d, const SubParams& p, float4* rec, hipStream_t s, Prof* pf) {
   if (!p.svd_fp64) return;
   ProfScope ps(pf, 12, s);
-  // a fixed grid, at most one thread per particle, that strides over the queue
-  const uint32_t g = std::min(cdiv(p.n, 256u), 1024u);
+  // a fixed grid, at most one thread per particle, that strides over the
+  // queue: 4096 blocks give every entry its own thread up to a queue of a
+  // million (a block past the queue's end only reads the 64 lengths)
+  const uint32_t g = std::min(cdiv(p.n, 256u), 4096u);
   if (p.n_scenes > 0)
     k_g2p_fb<true><<<g, 256, 0, s>>>(p, rec, d.materials);
   else
     k_g2p_fb<false><<<g, 256, 0, s>>>(p, rec, d.materials);
+  k_g2p_fb_reset<<<1, 64, 0, s>>>(p.fb_ctl);
 }
 
 void launch_g2p(const DevState& d, const SubParams& p, bool use_sorted, hipStream_t s, Prof* pf, float4* rec_out) {

@@ -150,7 +150,7 @@ struct DevState {
   uint32_t* counters;     // CNT_SLOTS
   uint2* units;           // NT + cap / kG2PUnit + 1: (active tile, chunk) work units of k_g2p_tile
   float4* fb_queue;       // (cap + kFbSlots) * kFbEntry: particles G2P leaves to k_g2p_fb (FP64 fallback)
-  uint32_t* fb_ctl;       // kFbSlots * kFbCtlStride + 1: slot lengths, k_g2p_fb blocks done (zero between substeps)
+  uint32_t* fb_ctl;       // kFbSlots * kFbCtlStride + 1: slot lengths (zero between substeps)
   float4* partial;        // cap_tiles * 1000
   float4* grid_node;      // NT*512
   float4* grid_old;       // NT*512
