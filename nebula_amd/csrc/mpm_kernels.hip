@@ -515,10 +515,19 @@ __global__ __launch_bounds__(kP2GThreads, 2) void k_p2g(SubParams p, const float
       };
       // (the stencil's quotient (x - wmin)/h is taken as a product with 1/h:
       // the cell is the bin's, only the weights use it)
+      // (a particle of the cell takes the fixed-piece form; one the bin
+      // placed there from outside the grid, the general one)
       auto stencil = [&](const float4& pp, Src& q) {
-        axis_stencil_f((pp.x - g.wmin[0]) * inv_h, pp.x, g.wmin[0], g.h, inv_h, cx - 1, q.sx);
-        axis_stencil_f((pp.y - g.wmin[1]) * inv_h, pp.y, g.wmin[1], g.h, inv_h, cy - 1, q.sy);
-        axis_stencil_f((pp.z - g.wmin[2]) * inv_h, pp.z, g.wmin[2], g.h, inv_h, cz - 1, q.sz);
+        const float fx = (pp.x - g.wmin[0]) * inv_h, fy = (pp.y - g.wmin[1]) * inv_h;
+        const float fz = (pp.z - g.wmin[2]) * inv_h;
+        bool ok = axis_stencil_cell(fx, pp.x, g.wmin[0], g.h, inv_h, cx, q.sx);
+        ok &= axis_stencil_cell(fy, pp.y, g.wmin[1], g.h, inv_h, cy, q.sy);
+        ok &= axis_stencil_cell(fz, pp.z, g.wmin[2], g.h, inv_h, cz, q.sz);
+        if (!ok) {
+          axis_stencil_f(fx, pp.x, g.wmin[0], g.h, inv_h, cx - 1, q.sx);
+          axis_stencil_f(fy, pp.y, g.wmin[1], g.h, inv_h, cy - 1, q.sy);
+          axis_stencil_f(fz, pp.z, g.wmin[2], g.h, inv_h, cz - 1, q.sz);
+        }
       };
       // keep = 0 scales the particle's contribution to exactly zero
       auto setup = [&](const float4& pp, const float4& vv, const float4& b0, const float4& b1, const float4& b2,
@@ -919,12 +928,18 @@ __device__ __forceinline__ void g2p_gather(const Axis3 (&ax3)[3], bool tiny, Nod
 __device__ __forceinline__ bool g2p_stencil(const SceneGeo& g, float xp, float yp, float zp, int (&base)[3],
                                             Axis3 (&ax3)[3]) {
   const float xs[3] = {xp, yp, zp};
+  float fq[3];
+  bool ok = true;
 #pragma unroll
   for (int a = 0; a < 3; a++) {
-    const float fq = cell_coord(g, xs[a], a);
-    const float f = fminf(fmaxf(fq, -4.f), float(g.grid_res[a]) + 4.f);
+    fq[a] = cell_coord(g, xs[a], a);
+    const float f = fminf(fmaxf(fq[a], -4.f), float(g.grid_res[a]) + 4.f);
     base[a] = int(floorf(f)) - 1;
-    axis_stencil_f(fq, xs[a], g.wmin[a], g.h, g.inv_h, base[a], ax3[a]);
+    ok &= axis_stencil_cell(fq[a], xs[a], g.wmin[a], g.h, g.inv_h, base[a] + 1, ax3[a]);
+  }
+  if (!ok) {  // the first cell layer, or outside the grid
+#pragma unroll
+    for (int a = 0; a < 3; a++) axis_stencil_f(fq[a], xs[a], g.wmin[a], g.h, g.inv_h, base[a], ax3[a]);
   }
   return (min_positive3(ax3[0].w) * min_positive3(ax3[1].w)) * min_positive3(ax3[2].w) < 1e-12f;
 }

@@ -156,6 +156,29 @@ __device__ __forceinline__ void axis_stencil_f(float f, float xp, float wmin, fl
     a.dx[i] = nf * h + wmin - xp;
   }
 }
+// The same stencil for a particle known to lie in cell c (c - 1 the first
+// node): when f is in [c, c+1) and c >= 1, the differences f - (c + i - 0.5)
+// are exact and fall in [0.5, 1.5), [-0.5, 0.5) and [-1.5, -0.5), so each
+// node's piece of the B-spline is fixed and the weights and derivatives come
+// out with the bits axis_stencil_f gives, without its compares and selects
+// (at the ends of the ranges both pieces give the same values: the spline
+// is C1).  Returns false when f is not in that range; a is then not valid.
+__device__ __forceinline__ bool axis_stencil_cell(float f, float xp, float wmin, float h, float inv_h, int c,
+                                                  Axis3& a) {
+  const float n0 = float(c - 1) + 0.5f, n1 = float(c) + 0.5f, n2 = float(c + 1) + 0.5f;
+  const float d0 = f - n0, d1 = f - n1, d2 = f - n2;
+  const float t0 = 1.5f - d0, t2 = 1.5f + d2;  // 1.5 - |d|
+  a.w[0] = 0.5f * t0 * t0;
+  a.g[0] = -t0 * inv_h;
+  a.w[1] = 0.75f - d1 * d1;
+  a.g[1] = (-2.f * d1) * inv_h;
+  a.w[2] = 0.5f * t2 * t2;
+  a.g[2] = t2 * inv_h;
+  a.dx[0] = n0 * h + wmin - xp;
+  a.dx[1] = n1 * h + wmin - xp;
+  a.dx[2] = n2 * h + wmin - xp;
+  return (c >= 1) & (f >= float(c)) & (f < float(c + 1));
+}
 // f = (x - wmin) / h, the IEEE quotient the cell keys use
 __device__ __forceinline__ void axis_stencil(float xp, float wmin, float h, float inv_h, int first_node, Axis3& a) {
   axis_stencil_f((xp - wmin) / h, xp, wmin, h, inv_h, first_node, a);
