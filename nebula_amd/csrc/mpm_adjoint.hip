@@ -77,9 +77,9 @@ __device__ __forceinline__ void node_w(const AxisD& X, const AxisD& Y, const Axi
 // The transfers' weight cut as a bit per stencil node (bit (z 3 + y) 3 + x
 // set: node dropped) for per-axis weights wx, wy, wz: on a particle with a
 // node of 0 < w < limit (the smallest positive per-axis weights bound every
-// w from below), the nodes whose weight, multiplied in the forward kernel's
-// order, is below 1e-12.  Zero for every other particle.
-template <bool XY_FIRST>
+// w from below), the nodes whose weight (wx wy) wz, multiplied in the order
+// of the reference and of the forward kernels, is below 1e-12.  Zero for
+// every other particle.
 __device__ __forceinline__ uint32_t cut_mask(const float wx[3], const float wy[3], const float wz[3], float limit) {
   uint32_t m = 0u;
   if ((min_positive3(wx) * min_positive3(wy)) * min_positive3(wz) < limit) {
@@ -89,7 +89,7 @@ __device__ __forceinline__ uint32_t cut_mask(const float wx[3], const float wy[3
       for (int jy = 0; jy < 3; jy++)
 #pragma unroll
         for (int ix = 0; ix < 3; ix++) {
-          const float w = XY_FIRST ? (wx[ix] * wy[jy]) * wz[kz] : wx[ix] * (wy[jy] * wz[kz]);
+          const float w = (wx[ix] * wy[jy]) * wz[kz];
           if (w < 1e-12f) m |= 1u << ((kz * 3 + jy) * 3 + ix);
         }
   }
@@ -336,7 +336,7 @@ __global__ __launch_bounds__(256) void k_g2p_bwd(SubParams p, const float4* __re
     }
     // G2P's weight cut (shaders/mpm_g2p.comp:93-101) as its forward takes it:
     // on a particle with a node of 0 < w < 1e-12, nodes with (wx wy) wz < 1e-12
-    const uint32_t cutm = cut_mask<true>(ax[0].w, ax[1].w, ax[2].w, 1e-12f);
+    const uint32_t cutm = cut_mask(ax[0].w, ax[1].w, ax[2].w, 1e-12f);
     // grid velocity of node (ix, iy, iz); out-of-grid, massless (mass < 1e-10)
     // and cut nodes read as zero (shaders/mpm_g2p.comp:93-101)
     auto node_v = [&](int ix, int iy, int iz) -> float4 {
@@ -641,7 +641,7 @@ __global__ __launch_bounds__(kAdjThreads) void k_adj_scatter(SubParams p, const 
         //   Q = wy wz M_B[0],  R = wy wz M_G[0]
         // G2P's weight cut as its forward takes it: on a particle with a node
         // of 0 < w < 1e-12, the nodes with (wx wy) wz < 1e-12 get nothing.
-        const uint32_t cutm = cut_mask<true>(X.w, Y.w, Z.w, 1e-12f);
+        const uint32_t cutm = cut_mask(X.w, Y.w, Z.w, 1e-12f);
         float wdx[3];
 #pragma unroll
         for (int ix = 0; ix < 3; ix++) wdx[ix] = X.w[ix] * X.dx[ix];
@@ -831,10 +831,10 @@ __global__ __launch_bounds__(256) void k_p2g_bwd(SubParams p, const float4* __re
     axis_factors(d, ax[a]);
   }
   const uint32_t cube = sg.cube_res;
-  // P2G's weight cut (shaders/mpm_p2g.comp:61) as its forward takes it: a
+  // P2G's weight cut (shaders/mpm_p2g.comp:60-61) as its forward takes it: a
   // particle whose smallest positive per-axis weights multiply below
-  // 1.0001e-12 skips the nodes with wx (wy wz) < 1e-12
-  const uint32_t cutm = cut_mask<false>(ax[0].w, ax[1].w, ax[2].w, 1.0001e-12f);
+  // 1.0001e-12 skips the nodes with (wx wy) wz < 1e-12
+  const uint32_t cutm = cut_mask(ax[0].w, ax[1].w, ax[2].w, 1.0001e-12f);
   // grid adjoint (dL/dp, dL/dm) of node (ix, iy, iz); nodes outside the cube
   // and cut nodes read as zero
   auto node_g = [&](int ix, int iy, int iz) -> float4 {

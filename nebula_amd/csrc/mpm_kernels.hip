@@ -516,18 +516,27 @@ __global__ __launch_bounds__(kP2GThreads, 2) void k_p2g(SubParams p, const float
       // (the stencil's quotient (x - wmin)/h is taken as a product with 1/h:
       // the cell is the bin's, only the weights use it)
       // (a particle of the cell takes the fixed-piece form; one the bin
-      // placed there from outside the grid, the general one)
-      auto stencil = [&](const float4& pp, Src& q) {
+      // placed there from outside the grid, the general one).  Returns
+      // whether the particle may have a node the reference skips, w =
+      // (wx wy) wz < 1e-12 (shaders/mpm_p2g.comp:60-61): the smallest positive
+      // per-axis weights bound every w from below (1.0001e-12 covers the
+      // rounding of the product order).  In the fixed-piece form the middle
+      // weight is >= 0.5 and the first one > 0, so an axis's smallest
+      // positive weight is the smaller outer one (the last counting when > 0).
+      auto stencil = [&](const float4& pp, Src& q) -> bool {
         const float fx = (pp.x - g.wmin[0]) * inv_h, fy = (pp.y - g.wmin[1]) * inv_h;
         const float fz = (pp.z - g.wmin[2]) * inv_h;
         bool ok = axis_stencil_cell(fx, pp.x, g.wmin[0], g.h, inv_h, cx, q.sx);
         ok &= axis_stencil_cell(fy, pp.y, g.wmin[1], g.h, inv_h, cy, q.sy);
         ok &= axis_stencil_cell(fz, pp.z, g.wmin[2], g.h, inv_h, cz, q.sz);
-        if (!ok) {
-          axis_stencil_f(fx, pp.x, g.wmin[0], g.h, inv_h, cx - 1, q.sx);
-          axis_stencil_f(fy, pp.y, g.wmin[1], g.h, inv_h, cy - 1, q.sy);
-          axis_stencil_f(fz, pp.z, g.wmin[2], g.h, inv_h, cz - 1, q.sz);
+        if (ok) {
+          auto mpos = [](const Axis3& a) { return a.w[2] > 0.f ? fminf(a.w[0], a.w[2]) : a.w[0]; };
+          return (mpos(q.sx) * mpos(q.sy)) * mpos(q.sz) < 1.0001e-12f;
         }
+        axis_stencil_f(fx, pp.x, g.wmin[0], g.h, inv_h, cx - 1, q.sx);
+        axis_stencil_f(fy, pp.y, g.wmin[1], g.h, inv_h, cy - 1, q.sy);
+        axis_stencil_f(fz, pp.z, g.wmin[2], g.h, inv_h, cz - 1, q.sz);
+        return (min_positive3(q.sx.w) * min_positive3(q.sy.w)) * min_positive3(q.sz.w) < 1.0001e-12f;
       };
       // keep = 0 scales the particle's contribution to exactly zero
       auto setup = [&](const float4& pp, const float4& vv, const float4& b0, const float4& b1, const float4& b2,
@@ -543,13 +552,6 @@ __global__ __launch_bounds__(kP2GThreads, 2) void k_p2g(SubParams p, const float
         q.t00 = st * tt.x; q.t11 = st * tt.y; q.t22 = st * tt.z;
         q.t01 = st * b0.w; q.t02 = st * b1.w; q.t12 = st * b2.w;
         q.mvx = q.mp * vv.x; q.mvy = q.mp * vv.y; q.mvz = q.mp * vv.z;
-      };
-      // The reference skips nodes with w = wx wy wz < 1e-12
-      // (shaders/mpm_p2g.comp:61).  The smallest positive per-axis weights
-      // bound every w from below (1.0001e-12 covers the rounding of the
-      // product order): particles with no such node take the fast loop.
-      auto has_tiny = [&](const Src& q) {
-        return (min_positive3(q.sx.w) * min_positive3(q.sy.w)) * min_positive3(q.sz.w) < 1.0001e-12f;
       };
 
       // Fast loop.  Node contributions factor as
@@ -583,8 +585,7 @@ __global__ __launch_bounds__(kP2GThreads, 2) void k_p2g(SubParams p, const float
           pref = true;
         }
         Src q;
-        stencil(pp, q);
-        const bool tiny = has_tiny(q);
+        const bool tiny = stencil(pp, q);
         if (tiny) {
           const uint32_t off = k - kbeg;
           if (off >= 0xFFFFu || (tiny_at >> 16) != 0xFFFFu) {
@@ -609,8 +610,7 @@ __global__ __launch_bounds__(kP2GThreads, 2) void k_p2g(SubParams p, const float
           if (!(pref && !tiny_all && s == 0u)) load_rec(idx_of(k));
           Src q;
           const float4 pp = npp;
-          stencil(pp, q);
-          if (!has_tiny(q)) continue;
+          if (!stencil(pp, q)) continue;
           setup(pp, nvv, nb0, nb1, nb2, ntt, 1.f, q);
           NB_P2G_CHAINS(true)
         }
