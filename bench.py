@@ -288,29 +288,45 @@ def diff_batch_leg(k, seg):
     eng.substep(dt, k)
     eng.sync()
     t_fwd = time.perf_counter() - t0
-    eng.setState(init)
-    eng.adjointBegin(k, seg)
-    t0 = time.perf_counter()
-    eng.forwardRecorded(dt)
-    t_rec = time.perf_counter() - t0
-    final = eng.getState()
-    target = final.copy()
-    target[0, :, :3] += np.float32(1e-3)  # loss = MSE of final positions against shifted targets
-    g = np.zeros_like(final)
-    g[0, :, :3] = 2.0 * (final[0, :, :3] - target[0, :, :3]) / n
-    eng.setAdjoint(g)
-    t0 = time.perf_counter()
-    dpar = eng.backward()
-    t_bwd = time.perf_counter() - t0
-    eng.adjointEnd()
-    ratio = (t_rec + t_bwd) / t_fwd
-    log(f"[bench] differentiable batch: fwd {t_fwd / k * 1e3:.3f} ms, recorded fwd {t_rec / k * 1e3:.3f} ms, "
-        f"bwd {t_bwd / k * 1e3:.3f} ms per substep; ratio {ratio:.3f}")
+
+    def run(tapes):
+        eng.setState(init)
+        eng.adjointBegin(k, seg, tapes)
+        n_tapes = eng.adjointTapes()
+        t0 = time.perf_counter()
+        eng.forwardRecorded(dt)
+        t_rec = time.perf_counter() - t0
+        final = eng.getState()
+        target = final.copy()
+        target[0, :, :3] += np.float32(1e-3)  # loss = MSE of final positions against shifted targets
+        g = np.zeros_like(final)
+        g[0, :, :3] = 2.0 * (final[0, :, :3] - target[0, :, :3]) / n
+        eng.setAdjoint(g)
+        t0 = time.perf_counter()
+        dpar = eng.backward()
+        t_bwd = time.perf_counter() - t0
+        grad = eng.getAdjoint()
+        eng.adjointEnd()
+        ratio = (t_rec + t_bwd) / t_fwd
+        log(f"[bench] differentiable batch, {n_tapes} tape(s): fwd {t_fwd / k * 1e3:.3f} ms, recorded fwd "
+            f"{t_rec / k * 1e3:.3f} ms, bwd {t_bwd / k * 1e3:.3f} ms per substep; ratio {ratio:.3f}")
+        return n_tapes, t_rec, t_bwd, ratio, dpar, grad
+
+    # one tape: checkpoints every `seg` substeps, every segment but the last
+    # recomputed; then as many segments' tapes as fit in free HBM (the
+    # gradients must be the same bits)
+    _, t_rec, t_bwd, ratio, dpar, grad = run(1)
+    n_res, r_rec, r_bwd, r_ratio, r_dpar, r_grad = run(0)
+    same = bool(np.array_equal(grad, r_grad) and np.array_equal(np.asarray(dpar), np.asarray(r_dpar)))
     return {"workload": "64 scenes x 32,768 particles on 64^3 grids, elastic layer on Drucker-Prager layer, "
                         "hidden (E, nu, friction angle) per scene (seed 5), proper polar SVD, APIC, dt 1e-4",
             "particles": n, "substeps": k, "segment": seg, "fwd_ms_per_substep": round(t_fwd / k * 1e3, 4),
             "recorded_fwd_ms_per_substep": round(t_rec / k * 1e3, 4), "bwd_ms_per_substep": round(t_bwd / k * 1e3, 4),
-            "ratio": round(ratio, 3), "finite_gradients": bool(np.isfinite(dpar).all())}
+            "ratio": round(ratio, 3), "finite_gradients": bool(np.isfinite(dpar).all()),
+            "resident_tapes": {"tapes": n_res, "segments": -(-k // seg),
+                               "recorded_fwd_ms_per_substep": round(r_rec / k * 1e3, 4),
+                               "bwd_ms_per_substep": round(r_bwd / k * 1e3, 4), "ratio": round(r_ratio, 3),
+                               "same_gradients": same}}
 
 
 def batch_leg(k):

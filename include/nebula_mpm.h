@@ -487,6 +487,17 @@ int nb_mpm_active_tile_count(nb_mpm* h, uint32_t* tiles);
  * and cell counts, kept so that the backward does not bin again or redo P2G
  * and the grid update) + 32 B. */
 int nb_mpm_adjoint_begin(nb_mpm* h, uint32_t n_substeps, uint32_t segment);
+/* nb_mpm_adjoint_begin with `tapes` segments' inner states kept at once
+ * (nb_mpm_adjoint_begin keeps one): the recorded run leaves the last `tapes`
+ * segments in place and the backward recomputes only the others, each from
+ * its checkpoint.  tapes = 0 takes as many as fit in the device memory free
+ * at the call (4 GiB and a tenth of the rest left over); tapes >= the number
+ * of segments recomputes nothing.  Gradients are the same bit for bit
+ * whatever `tapes` is.  Memory: the segment-dependent terms above times
+ * `tapes`. */
+int nb_mpm_adjoint_begin_tapes(nb_mpm* h, uint32_t n_substeps, uint32_t segment, uint32_t tapes);
+/* The number of tapes the current nb_mpm_adjoint_begin* call reserved. */
+int nb_mpm_adjoint_tapes(const nb_mpm* h, uint32_t* tapes);
 int nb_mpm_forward_recorded(nb_mpm* h, float dt_sub);
 int nb_mpm_set_adjoint(nb_mpm* h, int field, const float* in4, uint32_t first, uint32_t count);
 int nb_mpm_get_adjoint(nb_mpm* h, int field, float* out4, uint32_t first, uint32_t count);

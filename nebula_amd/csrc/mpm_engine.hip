@@ -121,21 +121,24 @@ struct nb_mpm {
     float dt = 0.f;
     std::vector<void*> allocs;
     // The state before substep s = sg * seg + j is slot(sg, j): ckpt[sg] for
-    // j = 0, tape[j - 1] otherwise.  The recorded run writes every slot; the
-    // tape then holds the last segment, and the backward recomputes the tape
-    // for each earlier segment from its checkpoint.
+    // j = 0, tape t = sg % n_tapes at j - 1 otherwise.  The recorded run
+    // writes every slot; the tapes then hold the last n_tapes segments, and
+    // the backward recomputes a segment's tape from its checkpoint when the
+    // tape holds another segment.
+    uint32_t n_tapes = 1;
     float4* ckpt = nullptr;   // n_ckpt * cap * kRec
-    float4* tape = nullptr;   // (seg - 1) * cap * kRec
+    float4* tape = nullptr;   // n_tapes * (seg - 1) * cap * kRec
     uint32_t* ckpt_keys = nullptr;  // bin keys of the same states: n_ckpt * cap
-    uint32_t* tape_keys = nullptr;  // (seg - 1) * cap
-    // grids of substep j of the tape's segment: node velocities after the
-    // update at grid_tape[2j], (momentum / mass, mass) before it at [2j + 1]
-    float4* grid_tape = nullptr;    // seg * 2 * NT * 512
-    int tape_seg = -1;        // the segment whose records the tape holds
-    // the bins substep j of the tape's segment built (sorted order, cell
-    // counts and ends, tile lists): the backward reads them instead of
+    uint32_t* tape_keys = nullptr;  // n_tapes * (seg - 1) * cap
+    // grids of substep j of tape t's segment: node velocities after the
+    // update at grid_tape[2 (t seg + j)], (momentum / mass, mass) before it
+    // at [2 (t seg + j) + 1]
+    float4* grid_tape = nullptr;    // n_tapes * seg * 2 * NT * 512
+    std::vector<int> tape_seg;      // n_tapes: the segment whose states each tape holds (-1: none)
+    // the bins substep j of tape t's segment built (sorted order, cell counts
+    // and ends, tile lists) at t seg + j: the backward reads them instead of
     // binning again
-    std::vector<DevState> slot_bins;  // seg
+    std::vector<DevState> slot_bins;  // n_tapes * seg
     float4* adj_a = nullptr;  // cap * kRec
     float4* adj_b = nullptr;
     float4* field[NB_NUM_FIELDS] = {};  // adjoint in the reference layout
@@ -1625,12 +1628,14 @@ int copy_rec(nb_mpm* h, float4* dst, const float4* src) {
   NB_HIP(hipMemcpyAsync(dst, src, size_t(h->n_live) * kRec * sizeof(float4), hipMemcpyDeviceToDevice, h->stream));
   return NB_OK;
 }
-// the records before substep s (see nb_mpm::Adjoint)
+// the tape that holds segment sg's inner states (see nb_mpm::Adjoint)
+uint32_t tape_of(const nb_mpm* h, uint32_t sg) { return sg % h->adj.n_tapes; }
+// the records before substep s
 float4* adj_slot(nb_mpm* h, uint32_t s) {
   const auto& A = h->adj;
   const size_t rec = size_t(h->cap) * kRec;
   const uint32_t sg = s / A.seg, j = s % A.seg;
-  return j == 0 ? A.ckpt + sg * rec : A.tape + (j - 1) * rec;
+  return j == 0 ? A.ckpt + sg * rec : A.tape + (size_t(tape_of(h, sg)) * (A.seg - 1) + (j - 1)) * rec;
 }
 // where substep s writes its result: the next slot, or the engine's records after the last substep
 float4* adj_next(nb_mpm* h, uint32_t s) { return s + 1 == h->adj.n ? h->d.rec : adj_slot(h, s + 1); }
@@ -1638,12 +1643,17 @@ float4* adj_next(nb_mpm* h, uint32_t s) { return s + 1 == h->adj.n ? h->d.rec : 
 uint32_t* key_slot(nb_mpm* h, uint32_t s) {
   const auto& A = h->adj;
   const uint32_t sg = s / A.seg, j = s % A.seg;
-  return j == 0 ? A.ckpt_keys + size_t(sg) * h->cap : A.tape_keys + size_t(j - 1) * h->cap;
+  return j == 0 ? A.ckpt_keys + size_t(sg) * h->cap
+                : A.tape_keys + (size_t(tape_of(h, sg)) * (A.seg - 1) + (j - 1)) * h->cap;
 }
 uint32_t* key_next(nb_mpm* h, uint32_t s) { return s + 1 == h->adj.n ? h->d.keys : key_slot(h, s + 1); }
+// the slot of substep s in its tape (grids and bins)
+size_t tape_slot(const nb_mpm* h, uint32_t s) {
+  return size_t(tape_of(h, s / h->adj.seg)) * h->adj.seg + s % h->adj.seg;
+}
 // the grids kept for substep s
 float4* grid_slot(nb_mpm* h, uint32_t s, int which) {
-  return h->adj.grid_tape + (size_t(s % h->adj.seg) * 2 + which) * h->NT * kTileCells;
+  return h->adj.grid_tape + (tape_slot(h, s) * 2 + which) * h->NT * kTileCells;
 }
 // substep s of the recorded run: slot to slot, keeping its grids
 TapeIO tape_io(nb_mpm* h, uint32_t s, float4* rec_out, uint32_t* keys_out) {
@@ -1654,7 +1664,7 @@ TapeIO tape_io(nb_mpm* h, uint32_t s, float4* rec_out, uint32_t* keys_out) {
   io.keys_out = keys_out;
   io.grid_node = grid_slot(h, s, 0);
   io.grid_pre = grid_slot(h, s, 1);
-  io.bins = &h->adj.slot_bins[s % h->adj.seg];
+  io.bins = &h->adj.slot_bins[tape_slot(h, s)];
   return io;
 }
 }  // namespace
@@ -1662,6 +1672,10 @@ TapeIO tape_io(nb_mpm* h, uint32_t s, float4* rec_out, uint32_t* keys_out) {
 extern "C" {
 
 int nb_mpm_adjoint_begin(nb_mpm* h, uint32_t n_substeps, uint32_t segment) {
+  return nb_mpm_adjoint_begin_tapes(h, n_substeps, segment, 1);
+}
+
+int nb_mpm_adjoint_begin_tapes(nb_mpm* h, uint32_t n_substeps, uint32_t segment, uint32_t tapes) {
   if (!h) return fail(NB_ERR_ARG, "null handle");
   if (n_substeps == 0 || segment == 0) return fail(NB_ERR_ARG, "n_substeps and segment must be > 0");
   if (!h->knobs.corrected_svd)
@@ -1679,17 +1693,32 @@ int nb_mpm_adjoint_begin(nb_mpm* h, uint32_t n_substeps, uint32_t segment) {
   A.seg = std::min(segment, n_substeps);
   A.n_ckpt = (n_substeps + A.seg - 1) / A.seg;
   const size_t rec = size_t(h->cap) * kRec;
+  if (tapes == 0) {
+    // as many segments' tapes as fit in the free device memory, 4 GiB and a
+    // tenth of it left over
+    const size_t cells = size_t(h->NT) * kTileCells;
+    const size_t per_tape = size_t(A.seg - 1) * (rec * sizeof(float4) + size_t(h->cap) * 4) +
+                            size_t(A.seg) * (2 * cells * sizeof(float4) + 2 * cells * 4 + size_t(h->cap) * 8 +
+                                             size_t(h->NT) * 40);
+    size_t free_b = 0, total_b = 0;
+    NB_HIP(hipMemGetInfo(&free_b, &total_b));
+    const size_t fixed = rec * sizeof(float4) * (A.n_ckpt + 2) + size_t(h->cap) * 4 * A.n_ckpt + (size_t(4) << 30);
+    const size_t avail = free_b > fixed ? (free_b - fixed) / 10 * 9 : 0;
+    tapes = uint32_t(std::min<size_t>(A.n_ckpt, std::max<size_t>(1, avail / std::max<size_t>(per_tape, 1))));
+  }
+  A.n_tapes = std::min(tapes, A.n_ckpt);
   const uint32_t n_blocks = (h->cap + 255u) / 256u;  // blocks of the G2P backward
   int rc = NB_OK;
   if (rc == NB_OK) rc = adj_alloc(h, &A.ckpt, rec * A.n_ckpt);
-  if (rc == NB_OK) rc = adj_alloc(h, &A.tape, rec * (A.seg - 1));
+  if (rc == NB_OK) rc = adj_alloc(h, &A.tape, rec * (A.seg - 1) * A.n_tapes);
   float4* kc = nullptr;
   float4* kt = nullptr;
   if (rc == NB_OK) rc = adj_alloc(h, &kc, (size_t(h->cap) * A.n_ckpt + 3) / 4);
-  if (rc == NB_OK) rc = adj_alloc(h, &kt, (size_t(h->cap) * (A.seg - 1) + 3) / 4);
-  if (rc == NB_OK) rc = adj_alloc(h, &A.grid_tape, size_t(A.seg) * 2 * h->NT * kTileCells);
-  A.slot_bins.assign(A.seg, DevState{});
-  for (uint32_t j = 0; j < A.seg && rc == NB_OK; j++) rc = adj_alloc_bins(h, A.slot_bins[j]);
+  if (rc == NB_OK) rc = adj_alloc(h, &kt, (size_t(h->cap) * (A.seg - 1) * A.n_tapes + 3) / 4);
+  if (rc == NB_OK) rc = adj_alloc(h, &A.grid_tape, size_t(A.seg) * A.n_tapes * 2 * h->NT * kTileCells);
+  A.slot_bins.assign(size_t(A.seg) * A.n_tapes, DevState{});
+  for (size_t j = 0; j < A.slot_bins.size() && rc == NB_OK; j++) rc = adj_alloc_bins(h, A.slot_bins[j]);
+  A.tape_seg.assign(A.n_tapes, -1);
   if (rc == NB_OK) rc = adj_alloc(h, &A.adj_a, rec);
   if (rc == NB_OK) rc = adj_alloc(h, &A.adj_b, rec);
   for (int f = 0; f < NB_NUM_FIELDS && rc == NB_OK; f++) rc = adj_alloc(h, &A.field[f], h->cap);
@@ -1721,6 +1750,13 @@ int nb_mpm_adjoint_begin(nb_mpm* h, uint32_t n_substeps, uint32_t segment) {
   return NB_OK;
 }
 
+int nb_mpm_adjoint_tapes(const nb_mpm* h, uint32_t* tapes) {
+  if (!h || !tapes) return fail(NB_ERR_ARG, "null argument");
+  if (!h->adj.on) return fail(NB_ERR_STATE, "call nb_mpm_adjoint_begin first");
+  *tapes = h->adj.n_tapes;
+  return NB_OK;
+}
+
 int nb_mpm_adjoint_end(nb_mpm* h) {
   if (!h) return fail(NB_ERR_ARG, "null handle");
   adj_free(h);
@@ -1743,7 +1779,8 @@ int nb_mpm_forward_recorded(nb_mpm* h, float dt_sub) {
   NB_HIP(hipStreamSynchronize(h->stream));
   A.dt = dt_sub;
   A.recorded = true;
-  A.tape_seg = int(A.n_ckpt) - 1;
+  // each tape holds the last segment the run wrote into it
+  for (uint32_t sg = 0; sg < A.n_ckpt; sg++) A.tape_seg[tape_of(h, sg)] = int(sg);
   return NB_OK;
 }
 
@@ -1783,7 +1820,7 @@ int nb_mpm_backward(nb_mpm* h, float* d_params, uint32_t params_cap) {
   for (int sg = int(A.n_ckpt) - 1; sg >= 0 && rc == NB_OK; sg--) {
     const uint32_t s0 = uint32_t(sg) * A.seg;
     const uint32_t len = std::min(A.seg, A.n - s0);
-    if (A.tape_seg != sg) {
+    if (A.tape_seg[tape_of(h, uint32_t(sg))] != sg) {
       // recompute the segment's records and grids from its checkpoint (its
       // last substep only up to the grid update: its result is the next checkpoint)
       h->keys_valid = true;
@@ -1793,13 +1830,13 @@ int nb_mpm_backward(nb_mpm* h, float* d_params, uint32_t params_cap) {
         io.g2p = !last;
         rc = enqueue_substep(h, A.dt, nullptr, io);
       }
-      A.tape_seg = sg;
+      A.tape_seg[tape_of(h, uint32_t(sg))] = sg;
     }
     for (int j = int(len) - 1; j >= 0 && rc == NB_OK; j--) {
       const uint32_t s = s0 + uint32_t(j);
       SubParams p = h->params(A.dt, false);
       DevState dv = h->dev();
-      use_bins(dv, A.slot_bins[s % A.seg]);  // as the forward substep binned them
+      use_bins(dv, A.slot_bins[tape_slot(h, s)]);  // as the forward substep binned them
       dv.rec = adj_slot(h, s);
       dv.keys = key_slot(h, s);
       dv.grid_node = grid_slot(h, s, 0);

@@ -416,7 +416,7 @@ EXPORTED_SYMBOLS = (
     "nb_mpm_set_live_particle_count", "nb_mpm_run_bin", "nb_mpm_get_bins", "nb_mpm_run_p2g",
     "nb_mpm_run_grid_update", "nb_mpm_run_g2p", "nb_mpm_get_grid", "nb_mpm_set_grid",
     "nb_mpm_profile_substeps", "nb_mpm_calibration_copy", "nb_mpm_active_node_count", "nb_mpm_active_tile_count",
-    "nb_mpm_adjoint_begin", "nb_mpm_forward_recorded", "nb_mpm_set_adjoint", "nb_mpm_get_adjoint",
+    "nb_mpm_adjoint_begin", "nb_mpm_adjoint_begin_tapes", "nb_mpm_adjoint_tapes", "nb_mpm_forward_recorded", "nb_mpm_set_adjoint", "nb_mpm_get_adjoint",
     "nb_mpm_adjoint_device_ptr", "nb_mpm_backward", "nb_mpm_adjoint_end",
     "nb_mpm_create_batch", "nb_mpm_scene_count", "nb_mpm_scene_info",
     "nb_mpm_get_tuning", "nb_mpm_set_tuning", "nb_mpm_autotune", "nb_mpm_graph_substeps",
@@ -654,6 +654,8 @@ def lib():
         L.nb_mpm_calibration_copy.argtypes = [vp, C.c_uint64, C.c_uint32]
         L.nb_mpm_active_tile_count.argtypes = [vp, up]
         L.nb_mpm_adjoint_begin.argtypes = [vp, C.c_uint32, C.c_uint32]
+        L.nb_mpm_adjoint_begin_tapes.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_uint32]
+        L.nb_mpm_adjoint_tapes.argtypes = [vp, C.POINTER(C.c_uint32)]
         L.nb_mpm_forward_recorded.argtypes = [vp, C.c_float]
         L.nb_mpm_set_adjoint.argtypes = [vp, C.c_int, vp, C.c_uint32, C.c_uint32]
         L.nb_mpm_get_adjoint.argtypes = [vp, C.c_int, vp, C.c_uint32, C.c_uint32]
@@ -1090,11 +1092,18 @@ class MPMEngine:
         return t.value
 
     # ---- reverse mode (no counterpart in the reference) ----
-    def adjointBegin(self, n_substeps: int, segment: int):
+    def adjointBegin(self, n_substeps: int, segment: int, tapes: int = 1):
         """Reserve checkpoints for a recorded run of n_substeps, one every
         `segment` substeps (needs corrected_svd = 1; PIC/APIC; elastic, Von
-        Mises, Drucker-Prager)."""
-        _check(lib().nb_mpm_adjoint_begin(self._h, n_substeps, segment))
+        Mises, Drucker-Prager), and `tapes` segments' inner states at once
+        (0: as many as fit in free device memory); the backward recomputes
+        only the segments the tapes do not hold."""
+        _check(lib().nb_mpm_adjoint_begin_tapes(self._h, n_substeps, segment, tapes))
+
+    def adjointTapes(self) -> int:
+        t = C.c_uint32()
+        _check(lib().nb_mpm_adjoint_tapes(self._h, C.byref(t)))
+        return t.value
 
     def forwardRecorded(self, dt_sub: float):
         _check(lib().nb_mpm_forward_recorded(self._h, dt_sub))

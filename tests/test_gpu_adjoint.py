@@ -166,16 +166,19 @@ def test_adjoint_rejects_unsupported_configurations(gpu_lib):
 
 
 def test_adjoint_segmentations_agree_bitwise(gpu_lib):
-    """Checkpoint spacing only changes what is recomputed: every segment
-    length, and a second backward over the same recording (whose tape then
-    holds the first segment), gives the same gradients bit for bit."""
+    """Checkpoint spacing and the number of segments kept at once only change
+    what is recomputed: every segment length, with one tape, two, three or as
+    many as fit (0), and a second backward over the same recording (whose
+    tapes then hold the first segments), gives the same gradients bit for bit."""
     sc = _scene()
     n = sc.n
     W = (np.random.default_rng(2).standard_normal((8, n, 4)) * _mask(n)).astype(np.float32)
     ref = None
-    for seg in (1, 2, 3, N_SUB):
+    for seg, tapes in ((1, 1), (2, 1), (3, 1), (N_SUB, 1), (1, 2), (1, 3), (2, 2), (1, 0)):
         e = make_engine(sc)
-        e.adjointBegin(N_SUB, seg)
+        e.adjointBegin(N_SUB, seg, tapes)
+        n_seg = -(-N_SUB // seg)
+        assert e.adjointTapes() == (min(tapes, n_seg) if tapes else n_seg), (seg, tapes, e.adjointTapes())
         e.forwardRecorded(sc.dt)
         final = e.getState()
         runs = []
@@ -188,8 +191,8 @@ def test_adjoint_segmentations_agree_bitwise(gpu_lib):
         for g, dp in runs:
             if ref is None:
                 ref = (g, dp)
-            assert np.array_equal(g, ref[0]), f"segment {seg}: state gradient differs"
-            assert np.array_equal(dp, ref[1]), f"segment {seg}: parameter gradient differs"
+            assert np.array_equal(g, ref[0]), f"segment {seg}, tapes {tapes}: state gradient differs"
+            assert np.array_equal(dp, ref[1]), f"segment {seg}, tapes {tapes}: parameter gradient differs"
 
 
 def test_adjoint_on_a_differentiable_batch_scene_matches_fp64_finite_differences(gpu_lib):
