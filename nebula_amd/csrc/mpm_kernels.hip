@@ -57,7 +57,10 @@ __global__ __launch_bounds__(512) void k_bin_clear(uint32_t* __restrict__ cell_c
 
 // Key of every particle (computed from the record, or read when the previous
 // G2P wrote it), its rank among equal keys (wave-aggregated atomics: one
-// atomic per distinct key per wave), and the occupied-tile flags.
+// atomic per run of equal keys in consecutive lanes; particles of a cell
+// mostly sit next to each other in storage), and the occupied-tile flags.
+// The ranks within a cell only need to be distinct: k_bin_stabilize orders
+// each cell's particles by index afterwards.
 template <bool FROM_REC, bool BATCH>
 __global__ __launch_bounds__(256) void k_bin_keys(SubParams p, const float4* __restrict__ rec,
                                                   uint32_t* __restrict__ keys, uint32_t* __restrict__ rank,
@@ -74,31 +77,24 @@ __global__ __launch_bounds__(256) void k_bin_keys(SubParams p, const float4* __r
       key = keys[i];
     }
   }
-  // group the wave's lanes by key (ballots only), then one atomic per group,
-  // all groups in the same instruction
+  // runs of equal keys in consecutive lanes (an invalid lane starts a run of
+  // its own); the run's first lane adds its length, the others take their
+  // offset in the run
   const uint32_t lane = threadIdx.x & 63;
-  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  uint64_t pending = __ballot(valid);
-  int my_leader = 0;
-  uint32_t my_pos = 0, my_cnt = 0;
-  while (pending) {
-    const int leader = __ffsll((unsigned long long)pending) - 1;
-    const uint32_t lkey = __shfl(key, leader);
-    const uint64_t peers = __ballot(valid && key == lkey);
-    if (valid && key == lkey) {
-      my_leader = leader;
-      my_pos = uint32_t(__popcll(peers & lt_mask));
-    }
-    if (int(lane) == leader) my_cnt = uint32_t(__popcll(peers));
-    pending &= ~peers;
-  }
+  const uint32_t prev = uint32_t(__shfl_up(int(key), 1));
+  const bool start = !valid || lane == 0u || prev != key;
+  const uint64_t starts = __ballot(start);
+  const uint64_t upto = lane == 63u ? ~0ull : ((2ull << lane) - 1ull);  // lanes 0..lane
+  const int rs = 63 - __clzll((unsigned long long)(starts & upto));     // this lane's run start
+  const uint64_t after = starts & ~upto;                                 // starts above this lane
+  const uint32_t run_end = after ? uint32_t(__ffsll((unsigned long long)after) - 1) : 64u;
   uint32_t base = 0;
-  if (valid && my_pos == 0u) {
-    base = atomicAdd(&cell_count[key], my_cnt);
+  if (valid && start) {
+    base = atomicAdd(&cell_count[key], run_end - lane);
     tile_flag[key >> kTileBits] = 1;
   }
-  base = __shfl(base, my_leader);
-  if (valid) rank[i] = base + my_pos;
+  base = uint32_t(__shfl(int(base), rs));
+  if (valid) rank[i] = base + (lane - uint32_t(rs));
 }
 
 // Per tile: bit0 = occupied, bit1 = some tile in its 3x3x3 neighbourhood is
@@ -445,10 +441,23 @@ __global__ __launch_bounds__(kP2GThreads, 2) void k_p2g(SubParams p, const float
     if (threadIdx.x == 0) s_layers = 0u;
     const uint32_t ca = 2u * threadIdx.x, cb = ca + 1u;
     const uint32_t na_ = cell_count[tbase + ca], nb_ = cell_count[tbase + cb];
+    const uint32_t ea = cell_end[tbase + ca], eb = cell_end[tbase + cb];
+    // the tile's particles are one contiguous range of the bin order: their
+    // indices are staged in LDS with coalesced loads (a tile beyond the stage
+    // reads the bin order directly), loaded here so that they are in flight
+    // during the scan below and stored to LDS after it
+    const uint32_t lo = tile_start[a], tile_n = tile_total[a];
+    const bool staged = tile_n <= kIdxStage;
+    uint32_t stg[kIdxStage / kP2GThreads];
+#pragma unroll
+    for (uint32_t r = 0; r < kIdxStage / kP2GThreads; r++) {
+      const uint32_t e = threadIdx.x + r * kP2GThreads;
+      stg[r] = (staged && e < tile_n) ? sorted[lo + e] : 0u;
+    }
     s_cnt[ca] = na_;
     s_cnt[cb] = nb_;
-    s_end[ca] = cell_end[tbase + ca];
-    s_end[cb] = cell_end[tbase + cb];
+    s_end[ca] = ea;
+    s_end[cb] = eb;
     const uint32_t qa = p2g_chunks(na_, p.p2g_item), qb = p2g_chunks(nb_, p.p2g_item);
     const uint32_t ga = p2g_group(na_, qa), gb = p2g_group(nb_, qb);
     const uint64_t mine = (uint64_t(qa) << (12 * ga)) + (uint64_t(qb) << (12 * gb));
@@ -472,13 +481,11 @@ __global__ __launch_bounds__(kP2GThreads, 2) void k_p2g(SubParams p, const float
     }
     for (uint32_t q = 0; q < qa; q++) s_item[pa + q] = uint16_t(ca | (q << 9));
     for (uint32_t q = 0; q < qb; q++) s_item[pb + q] = uint16_t(cb | (q << 9));
-    // the tile's particles are one contiguous range of the bin order: their
-    // indices are staged in LDS with coalesced loads (a tile beyond the stage
-    // reads the bin order directly)
-    const uint32_t lo = tile_start[a], tile_n = tile_total[a];
-    const bool staged = tile_n <= kIdxStage;
-    if (staged)
-      for (uint32_t e = threadIdx.x; e < tile_n; e += kP2GThreads) s_idx[e] = sorted[lo + e];
+#pragma unroll
+    for (uint32_t r = 0; r < kIdxStage / kP2GThreads; r++) {
+      const uint32_t e = threadIdx.x + r * kP2GThreads;
+      if (staged && e < tile_n) s_idx[e] = stg[r];
+    }
     __syncthreads();
     uint32_t n_items = 0;
 #pragma unroll
