@@ -57,10 +57,7 @@ __global__ __launch_bounds__(512) void k_bin_clear(uint32_t* __restrict__ cell_c
 
 // Key of every particle (computed from the record, or read when the previous
 // G2P wrote it), its rank among equal keys (wave-aggregated atomics: one
-// atomic per run of equal keys in consecutive lanes; particles of a cell
-// mostly sit next to each other in storage), and the occupied-tile flags.
-// The ranks within a cell only need to be distinct: k_bin_stabilize orders
-// each cell's particles by index afterwards.
+// atomic per distinct key per wave), and the occupied-tile flags.
 template <bool FROM_REC, bool BATCH>
 __global__ __launch_bounds__(256) void k_bin_keys(SubParams p, const float4* __restrict__ rec,
                                                   uint32_t* __restrict__ keys, uint32_t* __restrict__ rank,
@@ -77,24 +74,31 @@ __global__ __launch_bounds__(256) void k_bin_keys(SubParams p, const float4* __r
       key = keys[i];
     }
   }
-  // runs of equal keys in consecutive lanes (an invalid lane starts a run of
-  // its own); the run's first lane adds its length, the others take their
-  // offset in the run
+  // group the wave's lanes by key (ballots only), then one atomic per group,
+  // all groups in the same instruction
   const uint32_t lane = threadIdx.x & 63;
-  const uint32_t prev = uint32_t(__shfl_up(int(key), 1));
-  const bool start = !valid || lane == 0u || prev != key;
-  const uint64_t starts = __ballot(start);
-  const uint64_t upto = lane == 63u ? ~0ull : ((2ull << lane) - 1ull);  // lanes 0..lane
-  const int rs = 63 - __clzll((unsigned long long)(starts & upto));     // this lane's run start
-  const uint64_t after = starts & ~upto;                                 // starts above this lane
-  const uint32_t run_end = after ? uint32_t(__ffsll((unsigned long long)after) - 1) : 64u;
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  uint64_t pending = __ballot(valid);
+  int my_leader = 0;
+  uint32_t my_pos = 0, my_cnt = 0;
+  while (pending) {
+    const int leader = __ffsll((unsigned long long)pending) - 1;
+    const uint32_t lkey = __shfl(key, leader);
+    const uint64_t peers = __ballot(valid && key == lkey);
+    if (valid && key == lkey) {
+      my_leader = leader;
+      my_pos = uint32_t(__popcll(peers & lt_mask));
+    }
+    if (int(lane) == leader) my_cnt = uint32_t(__popcll(peers));
+    pending &= ~peers;
+  }
   uint32_t base = 0;
-  if (valid && start) {
-    base = atomicAdd(&cell_count[key], run_end - lane);
+  if (valid && my_pos == 0u) {
+    base = atomicAdd(&cell_count[key], my_cnt);
     tile_flag[key >> kTileBits] = 1;
   }
-  base = uint32_t(__shfl(int(base), rs));
-  if (valid) rank[i] = base + (lane - uint32_t(rs));
+  base = __shfl(base, my_leader);
+  if (valid) rank[i] = base + my_pos;
 }
 
 // Per tile: bit0 = occupied, bit1 = some tile in its 3x3x3 neighbourhood is
