@@ -304,8 +304,7 @@ constexpr int kCoef = 6;  // float4 per particle: A (3), M_B (9), M_G (9)
 template <bool BATCH>
 __global__ __launch_bounds__(256) void k_g2p_bwd(SubParams p, const float4* __restrict__ rec,
                                                  const float4* __restrict__ adj_in, float4* __restrict__ adj_out,
-                                                 float4* __restrict__ coef, uint32_t* __restrict__ pg_mid,
-                                                 float4* __restrict__ pg_val, uint32_t* __restrict__ pg_cnt,
+                                                 float4* __restrict__ coef, float4* __restrict__ pg_acc,
                                                  const nb_material* __restrict__ mats,
                                                  const uint32_t* __restrict__ sorted,
                                                  const float4* __restrict__ grid_node, const float* __restrict__ sdf) {
@@ -525,8 +524,8 @@ __global__ __launch_bounds__(256) void k_g2p_bwd(SubParams p, const float4* __re
     for (int j = 0; j < kCoef; j++) co[j] = make_float4(cf[4 * j], cf[4 * j + 1], cf[4 * j + 2], cf[4 * j + 3]);
   }
   // Material-parameter partials: each wave sums its lanes per material (the
-  // groups in order of their first lane), and the block writes the groups of
-  // its waves, wave by wave, as its entry list.
+  // groups in order of their first lane), and one thread adds the groups of
+  // the block's waves, wave by wave, to the block's row of running sums.
   __shared__ uint32_t s_gmid[4][64];
   __shared__ float4 s_gval[4][64];
   __shared__ uint32_t s_gcnt[4];
@@ -554,15 +553,15 @@ __global__ __launch_bounds__(256) void k_g2p_bwd(SubParams p, const float4* __re
   }
   if (lane == 0) s_gcnt[wv] = n_grp;
   __syncthreads();
-  const uint32_t c0 = s_gcnt[0], c1 = s_gcnt[1], c2 = s_gcnt[2], c3 = s_gcnt[3];
-  const size_t eb = size_t(blockIdx.x) * kPgEntries;
-  for (uint32_t e = threadIdx.x; e < c0 + c1 + c2 + c3; e += blockDim.x) {
-    const uint32_t w = e < c0 ? 0u : (e < c0 + c1 ? 1u : (e < c0 + c1 + c2 ? 2u : 3u));
-    const uint32_t i = e - (w > 0 ? c0 : 0u) - (w > 1 ? c1 : 0u) - (w > 2 ? c2 : 0u);
-    pg_mid[eb + e] = s_gmid[w][i];
-    pg_val[eb + e] = s_gval[w][i];
+  if (threadIdx.x == 0) {
+    float4* row = pg_acc + size_t(blockIdx.x) * p.material_count;
+    for (uint32_t w = 0; w < 4u; w++)
+      for (uint32_t i = 0; i < s_gcnt[w]; i++) {
+        const float4 v = s_gval[w][i];
+        float4& t = row[s_gmid[w][i]];
+        t = make_float4(t.x + v.x, t.y + v.y, t.z + v.z, t.w + v.w);
+      }
   }
-  if (threadIdx.x == 0) pg_cnt[blockIdx.x] = c0 + c1 + c2 + c3;
 }
 
 // ---------------------------------------------------------------------------
@@ -893,32 +892,19 @@ __global__ __launch_bounds__(256) void k_p2g_bwd(SubParams p, const float4* __re
   go[R_T] = make_float4(gT.m[0][0], gT.m[1][1], gT.m[2][2], gt.w);
 }
 
-// Fixed-order sum of the per-block entry lists in two levels.  Level 1: one
-// wave per (chunk of 64 source blocks, material); lane l takes block
-// chunk * 64 + l and adds its entries of that material; a fixed shuffle tree
-// sums the lanes into chunk_part[material][chunk].  Level 2: one block per
-// material adds its chunk sums (thread t: chunks t, t + 256, ..., then a
-// fixed LDS tree) to the running total.
+// Fixed-order sum of the per-block rows in two levels.  Level 1: one wave per
+// (chunk of 64 source blocks, material); lane l takes block chunk * 64 + l; a
+// fixed shuffle tree sums the lanes into chunk_part[material][chunk].  Level
+// 2: one block per material adds its chunk sums (thread t: chunks t, t + 256,
+// ..., then a fixed LDS tree).
 constexpr uint32_t kPgChunk = 64;
-__global__ __launch_bounds__(256) void k_pgrad_chunks(const uint32_t* __restrict__ pg_mid,
-                                                      const float4* __restrict__ pg_val,
-                                                      const uint32_t* __restrict__ pg_cnt, uint32_t n_blocks,
+__global__ __launch_bounds__(256) void k_pgrad_chunks(const float4* __restrict__ pg_acc, uint32_t n_blocks,
                                                       uint32_t n_materials, uint32_t n_chunks,
                                                       float4* __restrict__ chunk_part) {
   const uint32_t chunk = blockIdx.x, m = blockIdx.y * 4u + (threadIdx.x >> 6), lane = threadIdx.x & 63u;
   if (m >= n_materials) return;  // whole waves
   const uint32_t b = chunk * kPgChunk + lane;
-  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-  if (b < n_blocks) {
-    const uint32_t c = pg_cnt[b];
-    for (uint32_t j = 0; j < c; j++) {
-      const size_t e = size_t(b) * kPgEntries + j;
-      if (pg_mid[e] == m) {
-        const float4 v = pg_val[e];
-        acc.x += v.x; acc.y += v.y; acc.z += v.z; acc.w += v.w;
-      }
-    }
-  }
+  float4 acc = b < n_blocks ? pg_acc[size_t(b) * n_materials + m] : make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) {
     acc.x += __shfl_xor(acc.x, o);
@@ -947,10 +933,10 @@ __global__ __launch_bounds__(256) void k_pgrad_total(const float4* __restrict__ 
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    pgrad[4 * m + 0] += s[0].x;
-    pgrad[4 * m + 1] += s[0].y;
-    pgrad[4 * m + 2] += s[0].z;
-    pgrad[4 * m + 3] += s[0].w;
+    pgrad[4 * m + 0] = s[0].x;
+    pgrad[4 * m + 1] = s[0].y;
+    pgrad[4 * m + 2] = s[0].z;
+    pgrad[4 * m + 3] = s[0].w;
   }
 }
 
@@ -967,7 +953,7 @@ template <bool BATCH>
 static void adjoint_kernels(const DevState& d, const AdjState& a, const SubParams& p, uint32_t n_tiles,
                             const float4* adj_in, float4* adj_out, hipStream_t s) {
   const uint32_t nblk = cdiv_a(p.n, 256);
-  k_g2p_bwd<BATCH><<<nblk, 256, 0, s>>>(p, d.rec, adj_in, adj_out, a.coef, a.pg_mid, a.pg_val, a.pg_cnt,
+  k_g2p_bwd<BATCH><<<nblk, 256, 0, s>>>(p, d.rec, adj_in, adj_out, a.coef, a.pg_acc,
                                          d.materials, d.sorted, d.grid_node, d.sdf);
   k_adj_scatter<BATCH><<<tile_grid_a(n_tiles, p.tile_blocks), kAdjThreads, 0, s>>>(p, d.rec, a.coef, d.cell_count, d.cell_end,
                                                                     d.sorted, d.act_tiles, d.counters, a.partial_adj);
@@ -982,15 +968,19 @@ static void adjoint_kernels(const DevState& d, const AdjState& a, const SubParam
 
 void launch_adjoint_substep(const DevState& d, const AdjState& a, const SubParams& p, uint32_t n_tiles,
                             const float4* adj_in, float4* adj_out, hipStream_t s) {
-  const uint32_t nblk = cdiv_a(p.n, 256);
   if (p.n_scenes > 0)
     adjoint_kernels<true>(d, a, p, n_tiles, adj_in, adj_out, s);
   else
     adjoint_kernels<false>(d, a, p, n_tiles, adj_in, adj_out, s);
+}
+
+void launch_adjoint_params(const AdjState& a, uint32_t n_particles, uint32_t n_materials, hipStream_t s) {
+  if (n_particles == 0 || n_materials == 0) return;  // pgrad stays zero
+  const uint32_t nblk = cdiv_a(n_particles, 256);
   const uint32_t n_chunks = cdiv_a(nblk, kPgChunk);
-  k_pgrad_chunks<<<dim3(n_chunks, cdiv_a(p.material_count, 4u)), 256, 0, s>>>(
-      a.pg_mid, a.pg_val, a.pg_cnt, nblk, p.material_count, n_chunks, a.pg_chunk);
-  k_pgrad_total<<<p.material_count, 256, 0, s>>>(a.pg_chunk, n_chunks, a.pgrad);
+  k_pgrad_chunks<<<dim3(n_chunks, cdiv_a(n_materials, 4u)), 256, 0, s>>>(a.pg_acc, nblk, n_materials, n_chunks,
+                                                                          a.pg_chunk);
+  k_pgrad_total<<<n_materials, 256, 0, s>>>(a.pg_chunk, n_chunks, a.pgrad);
 }
 
 }  // namespace nb

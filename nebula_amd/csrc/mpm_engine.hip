@@ -1725,12 +1725,9 @@ int nb_mpm_adjoint_begin_tapes(nb_mpm* h, uint32_t n_substeps, uint32_t segment,
   if (rc == NB_OK) rc = adj_alloc(h, &A.a.coef, size_t(h->cap) * 6);
   if (rc == NB_OK) rc = adj_alloc(h, &A.a.partial_adj, size_t(h->cap_tiles) * kBoxNodes);
   if (rc == NB_OK) rc = adj_alloc(h, &A.a.grid_adj, size_t(h->NT) * kTileCells);
-  float4* pm = nullptr;
-  float4* pc = nullptr;
   float4* pg = nullptr;
-  if (rc == NB_OK) rc = adj_alloc(h, &A.a.pg_val, size_t(n_blocks) * kPgEntries);
-  if (rc == NB_OK) rc = adj_alloc(h, &pm, size_t(n_blocks) * kPgEntries / 4);
-  if (rc == NB_OK) rc = adj_alloc(h, &pc, (n_blocks + 3) / 4);
+  A.a.pg_materials = std::max(h->material_count, 1u);
+  if (rc == NB_OK) rc = adj_alloc(h, &A.a.pg_acc, size_t(n_blocks) * A.a.pg_materials);
   if (rc == NB_OK) rc = adj_alloc(h, &A.a.pg_chunk, size_t(h->max_materials()) * ((n_blocks + 63) / 64));
   if (rc == NB_OK) rc = adj_alloc(h, &pg, h->max_materials());
   if (rc != NB_OK) {
@@ -1739,8 +1736,6 @@ int nb_mpm_adjoint_begin_tapes(nb_mpm* h, uint32_t n_substeps, uint32_t segment,
     g_err = keep;
     return rc;
   }
-  A.a.pg_mid = reinterpret_cast<uint32_t*>(pm);
-  A.a.pg_cnt = reinterpret_cast<uint32_t*>(pc);
   A.ckpt_keys = reinterpret_cast<uint32_t*>(kc);
   A.tape_keys = reinterpret_cast<uint32_t*>(kt);
   A.a.pgrad = reinterpret_cast<float*>(pg);
@@ -1812,7 +1807,11 @@ int nb_mpm_backward(nb_mpm* h, float* d_params, uint32_t params_cap) {
   if (!h) return fail(NB_ERR_ARG, "null handle");
   auto& A = h->adj;
   if (!A.on || !A.recorded) return fail(NB_ERR_STATE, "call nb_mpm_forward_recorded first");
+  if (h->material_count > A.a.pg_materials)
+    return fail(NB_ERR_STATE, "materials were added after nb_mpm_adjoint_begin");
   NB_HIP(hipMemsetAsync(A.a.pgrad, 0, size_t(h->max_materials()) * 4 * sizeof(float), h->stream));
+  NB_HIP(hipMemsetAsync(A.a.pg_acc, 0, size_t((h->n_live + 255) / 256) * h->material_count * sizeof(float4),
+                        h->stream));
   launch_soa_to_aos_ptrs(adj_fields(h), A.adj_a, h->n_live, h->stream);
   float4* in = A.adj_a;
   float4* out = A.adj_b;
@@ -1847,6 +1846,8 @@ int nb_mpm_backward(nb_mpm* h, float* d_params, uint32_t params_cap) {
     }
   }
   if (rc != NB_OK) return rc;
+  launch_adjoint_params(A.a, h->n_live, h->material_count, h->stream);
+  NB_HIP(hipGetLastError());
   launch_aos_to_soa_ptrs(in, adj_fields(h), h->n_live, h->stream);
   // the engine's records still hold the state the recorded run ended in; the
   // bins, keys and grid are those of an earlier substep

@@ -214,20 +214,22 @@ struct AdjState {
   float4* coef;         // capacity * 6: per-particle grid-adjoint coefficients
   float4* partial_adj;  // cap_tiles * kBoxNodes: per-tile grid-adjoint boxes
   float4* grid_adj;     // NT * 512: (dL/dp, dL/dm) per node
-  // material-parameter partials of the G2P backward: per block, a list of
-  // up to kPgEntries (material, d/d(mu, lambda, M, q_c)) entries and its length
-  uint32_t* pg_mid;     // blocks * kPgEntries
-  float4* pg_val;       // blocks * kPgEntries
-  uint32_t* pg_cnt;     // blocks
+  // material-parameter partials of the G2P backward: per block of the G2P
+  // backward and material, d/d(mu, lambda, M, q_c) summed over the substeps of
+  // the backward so far (each block adds to its own row, substep after
+  // substep); launch_adjoint_params sums the rows at the end
+  float4* pg_acc;       // blocks * pg_materials
+  uint32_t pg_materials;
   float4* pg_chunk;     // materials x ceil(blocks / 64): per-chunk sums
-  float* pgrad;         // materials x 4: running d/d(mu, lambda, M, q_c)
+  float* pgrad;         // materials x 4: d/d(mu, lambda, M, q_c)
 };
 // One substep backwards on the recomputed forward of state d.rec (bins, P2G
 // partials and grid velocities already in d): adj_in is the adjoint of the
 // state after the substep, adj_out receives the adjoint of d.rec.
-constexpr uint32_t kPgEntries = 256;  // 4 waves x at most 64 material groups each
 void launch_adjoint_substep(const DevState& d, const AdjState& a, const SubParams& p, uint32_t n_tiles,
                             const float4* adj_in, float4* adj_out, hipStream_t s);
+// pgrad = the sum of pg_acc's rows, per material, in a fixed order
+void launch_adjoint_params(const AdjState& a, uint32_t n_particles, uint32_t n_materials, hipStream_t s);
 
 // Sets the thread-local message nb_last_error() returns; returns code.
 int set_error(int code, const char* fmt, ...);
