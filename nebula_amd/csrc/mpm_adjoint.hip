@@ -16,7 +16,8 @@
 //   k_grid_bwd     per node: the grid update backwards -> dL/dp, dL/dm
 //   k_p2g_bwd      per particle: P2G backwards as a 27-node gather
 //                  -> d/dv, d/dB, d/dtau and the position part from P2G
-//   k_pgrad_chunks / k_pgrad_total  per-block material-parameter partials
+//   k_pgrad_chunks / k_pgrad_total  once per backward: the material-parameter
+//                  partials k_g2p_bwd's blocks accumulated over the substeps,
 //                  summed in a fixed order
 // Positions enter every weight, so both transfers contribute the weight
 // gradients and Hessians.  Everything is deterministic.
