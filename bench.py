@@ -67,6 +67,22 @@ MEASURED_TRAFFIC = {"g2p": 1496.7e6, "p2g": 613.6e6, "grid_update": 120.9e6}
 # average launch time they give the fraction of the VALU issue peak.
 VALU_SOURCE = "profiles/counters_snow_terrain_256.md"
 MEASURED_VALU = {"g2p": 1.823e8, "p2g": 5.586e7, "grid_update": 5.518e6}
+# sha256 (first 16 hex digits) of the forward kernels' sources the two tables
+# above were measured on; the bench line says whether the running build's
+# sources still hash to it (if not, the counter passes are due again)
+COUNTER_SOURCES = ("nebula_amd/csrc/mpm_kernels.hip", "nebula_amd/csrc/nb_device_math.hpp",
+                   "nebula_amd/csrc/nb_kernel_util.hpp")
+COUNTER_SOURCES_SHA = "4da2e77d0c4ec7cb"
+
+
+def counter_sources_sha():
+    import hashlib
+
+    h = hashlib.sha256()
+    for f in COUNTER_SOURCES:
+        with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
 
 
 def log(*a):
@@ -449,7 +465,9 @@ def main():
                 "traffic_source": TRAFFIC_SOURCE if dom in MEASURED_TRAFFIC else None,
                 "valu_frac": round(valu_frac, 4) if valu_frac else None,
                 "valu_insts_per_launch": valu, "valu_source": VALU_SOURCE if valu else None,
-                "bytes_per_launch": dbytes, "avg_launch_ms": round(avg_ms[dom], 5)}
+                "bytes_per_launch": dbytes, "avg_launch_ms": round(avg_ms[dom], 5),
+                "counters_measured_on_sources": COUNTER_SOURCES_SHA,
+                "counters_match_this_build": counter_sources_sha() == COUNTER_SOURCES_SHA}
     substep_rate = b_alg / (ms_per_step * 1e-3) / 1e9
 
     if args.calibration_copy_mb > 0:
