@@ -43,7 +43,7 @@ _record_ids = itertools.count(1)
 
 class _Rollout(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, state, params, engine, n_substeps, dt, segment):
+    def forward(ctx, state, params, engine, n_substeps, dt, segment, tapes):
         if state.dim() != 3 or state.shape[0] != 8 or state.shape[2] != 4 or state.dtype != torch.float32:
             raise ValueError("state must be a (8, n, 4) float32 tensor")
         if not state.is_cuda:
@@ -58,7 +58,7 @@ class _Rollout(torch.autograd.Function):
         n = st.shape[1]
         torch.cuda.current_stream(st.device).synchronize()
         engine.setStateFrom(st.data_ptr(), n)
-        engine.adjointBegin(n_substeps, segment)
+        engine.adjointBegin(n_substeps, segment, tapes)
         engine.forwardRecorded(dt)
         out = torch.empty_like(st)
         engine.getStateTo(out.data_ptr())
@@ -87,17 +87,19 @@ class _Rollout(torch.autograd.Function):
         e._record_id = None
         dtype, device = ctx.params_meta
         g_params = torch.as_tensor(dpar, dtype=dtype, device=device)
-        return g_state, g_params, None, None, None, None
+        return g_state, g_params, None, None, None, None, None
 
 
 def rollout(engine: "mpm.MPMEngine", state: torch.Tensor, params: torch.Tensor, n_substeps: int, dt: float,
-            segment: int = 0) -> torch.Tensor:
+            segment: int = 0, tapes: int = 1) -> torch.Tensor:
     """Final state after ``n_substeps`` substeps of size ``dt`` from ``state``
     with material parameters ``params``; differentiable in both.  segment = 0
     keeps every substep's state (no recompute in the backward); a smaller
-    segment trades memory for one recomputed forward per segment."""
+    segment trades memory for one recomputed forward per segment, except for
+    the ``tapes`` segments whose states are kept at once (0: as many as fit in
+    free device memory).  The gradients do not depend on segment or tapes."""
     seg = n_substeps if segment <= 0 else segment
-    return _Rollout.apply(state, params, engine, int(n_substeps), float(dt), int(seg))
+    return _Rollout.apply(state, params, engine, int(n_substeps), float(dt), int(seg), int(tapes))
 
 
 def material_params(engine: "mpm.MPMEngine", device=None) -> torch.Tensor:

@@ -46,17 +46,18 @@ def rollout_matches_direct_reverse_mode():
     W = (np.random.default_rng(3).standard_normal((8, n, 4)) * _mask(n)).astype(np.float32)
     final, grad, dpar = _direct(sc, W)
 
-    e = make_engine(sc)
-    state = torch.tensor(sc.fields, device="cuda", requires_grad=True)
-    params = material_params(e, "cuda").requires_grad_()
-    out = rollout(e, state, params, N_SUB, sc.dt, SEG)
-    loss = (torch.tensor(W, device="cuda") * out).sum()
-    loss.backward()
-    torch.cuda.synchronize()
-    assert np.array_equal(out.detach().cpu().numpy().view(np.uint32), final.view(np.uint32))
-    assert np.array_equal(state.grad.cpu().numpy(), grad)
-    assert np.array_equal(params.grad.cpu().numpy(), dpar)
-    assert float(state.grad[1, :, 3].abs().max()) == 0.0  # material id lane
+    for tapes in (1, 0):  # one segment's states kept at once, or as many as fit
+        e = make_engine(sc)
+        state = torch.tensor(sc.fields, device="cuda", requires_grad=True)
+        params = material_params(e, "cuda").requires_grad_()
+        out = rollout(e, state, params, N_SUB, sc.dt, SEG, tapes)
+        loss = (torch.tensor(W, device="cuda") * out).sum()
+        loss.backward()
+        torch.cuda.synchronize()
+        assert np.array_equal(out.detach().cpu().numpy().view(np.uint32), final.view(np.uint32))
+        assert np.array_equal(state.grad.cpu().numpy(), grad)
+        assert np.array_equal(params.grad.cpu().numpy(), dpar)
+        assert float(state.grad[1, :, 3].abs().max()) == 0.0  # material id lane
 
 
 def rollout_params_enter_the_forward():
