@@ -360,32 +360,34 @@ __device__ __forceinline__ uint32_t p2g_group(uint32_t count, uint32_t chunks) {
 // walk below.  MASK: wx and wyz are set to zero at each node whose weight
 // (wx wy) wz the reference skips (fma(a, 0, s) leaves s as it was); every
 // other node gets the same bits as without the mask.
-#define NB_P2G_CHAINS(MASK)                                                                          \
+#define NB_P2G_CHAINS(MASK)                                                                           \
   {                                                                                                   \
     const float mp = q.mp, mvx = q.mvx, mvy = q.mvy, mvz = q.mvz;                                     \
     const float c00 = q.c00, c01 = q.c01, c02 = q.c02, c10 = q.c10, c11 = q.c11, c12 = q.c12;         \
     const float c20 = q.c20, c21 = q.c21, c22 = q.c22;                                                \
     const float t00 = q.t00, t11 = q.t11, t22 = q.t22, t01 = q.t01, t02 = q.t02, t12 = q.t12;         \
     const Axis3 &sx = q.sx, &sy = q.sy, &sz = q.sz;                                                   \
-    float mwx[3], Qx[3], Qy[3], Qz[3], Ayx[3], Ayy[3], Ayz[3];                                        \
+    float mwx[3], Qx[3], Qy[3], Qz[3], Rx[3], Ry[3], Rz[3], Sx[3], Sy[3], Sz[3];                      \
     _Pragma("unroll") for (int i = 0; i < 3; i++) {                                                   \
       mwx[i] = mp * sx.w[i];                                                                          \
       const float wdx = sx.w[i] * sx.dx[i];                                                           \
       Qx[i] = fmaf(-sx.g[i], t00, wdx * c00);                                                         \
       Qy[i] = fmaf(-sx.g[i], t01, wdx * c01);                                                         \
       Qz[i] = fmaf(-sx.g[i], t02, wdx * c02);                                                         \
-      Ayx[i] = fmaf(c10, sy.dx[i], mvx);                                                              \
-      Ayy[i] = fmaf(c11, sy.dx[i], mvy);                                                              \
-      Ayz[i] = fmaf(c12, sy.dx[i], mvz);                                                              \
+      Rx[i] = fmaf(sy.w[i], fmaf(c10, sy.dx[i], mvx), -(sy.g[i] * t01));                              \
+      Ry[i] = fmaf(sy.w[i], fmaf(c11, sy.dx[i], mvy), -(sy.g[i] * t11));                              \
+      Rz[i] = fmaf(sy.w[i], fmaf(c12, sy.dx[i], mvz), -(sy.g[i] * t12));                              \
+      const float wdz = sz.w[i] * sz.dx[i];                                                           \
+      Sx[i] = fmaf(-sz.g[i], t02, wdz * c20);                                                         \
+      Sy[i] = fmaf(-sz.g[i], t12, wdz * c21);                                                         \
+      Sz[i] = fmaf(-sz.g[i], t22, wdz * c22);                                                         \
     }                                                                                                 \
     _Pragma("unroll") for (int kz = 0; kz < 3; kz++) {                                                \
       _Pragma("unroll") for (int jy = 0; jy < 3; jy++) {                                              \
         const float wyz = sy.w[jy] * sz.w[kz];                                                        \
-        const float gy_wz = sy.g[jy] * sz.w[kz];                                                      \
-        const float wy_gz = sy.w[jy] * sz.g[kz];                                                      \
-        const float Px = fmaf(-t02, wy_gz, fmaf(wyz, fmaf(c20, sz.dx[kz], Ayx[jy]), -(t01 * gy_wz))); \
-        const float Py = fmaf(-t12, wy_gz, fmaf(wyz, fmaf(c21, sz.dx[kz], Ayy[jy]), -(t11 * gy_wz))); \
-        const float Pz = fmaf(-t22, wy_gz, fmaf(wyz, fmaf(c22, sz.dx[kz], Ayz[jy]), -(t12 * gy_wz))); \
+        const float Px = fmaf(sz.w[kz], Rx[jy], sy.w[jy] * Sx[kz]);                                   \
+        const float Py = fmaf(sz.w[kz], Ry[jy], sy.w[jy] * Sy[kz]);                                   \
+        const float Pz = fmaf(sz.w[kz], Rz[jy], sy.w[jy] * Sz[kz]);                                   \
         _Pragma("unroll") for (int ix = 0; ix < 3; ix++) {                                            \
           const int o = (kz * 3 + jy) * 3 + ix;                                                       \
           float wx = sx.w[ix], wn = wyz;                                                              \
@@ -570,7 +572,10 @@ __global__ __launch_bounds__(kP2GThreads, 2) void k_p2g(SubParams p, const float
       // with P = wyz (m v + C (0, dy, dz)) - tau (0, gy wz, wy gz) and
       // Q = wx dx C e_x - gx tau e_x: 7 FMAs per node (explicit FMA chains;
       // strict evaluation of acc + (a b + c d) costs a multiply, an FMA and an
-      // add).  A particle with a sub-cutoff node is scaled to zero here and
+      // add).  P itself splits into per-axis terms, P = wz R(y) + wy S(z) with
+      // R = wy (m v + dy C e_y) - gy tau e_y and S = wz dz C e_z - gz tau e_z,
+      // so each of the 9 (y, z) pairs costs a multiply and an FMA per
+      // component.  A particle with a sub-cutoff node is scaled to zero here and
       // added by the exact loop below, which visits only those particles:
       // the first two by their offsets from kbeg (16 bits each in tiny_at,
       // 0xFFFF = none), all of the item's particles when there are more (or
