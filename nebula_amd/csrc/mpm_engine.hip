@@ -290,6 +290,7 @@ void use_bins(DevState& dv, const DevState& b) {
   dv.node_tiles = b.node_tiles;
   dv.tile_total = b.tile_total;
   dv.tile_start = b.tile_start;
+  dv.tile_order = b.tile_order;
   dv.counters = b.counters;
 }
 
@@ -633,6 +634,7 @@ int engine_alloc(nb_mpm* h, void* hip_stream) {
   if (rc == NB_OK) rc = h->alloc(&d.all_tiles, h->NT);
   if (rc == NB_OK) rc = h->alloc(&d.tile_total, h->NT);
   if (rc == NB_OK) rc = h->alloc(&d.tile_start, h->NT);
+  if (rc == NB_OK) rc = h->alloc(&d.tile_order, h->NT);
   if (rc == NB_OK) rc = h->alloc(&d.counters, CNT_SLOTS);
   if (rc == NB_OK) rc = h->alloc(&d.partial, size_t(h->cap_tiles) * kBoxNodes);
   if (rc == NB_OK) rc = h->alloc(&d.grid_node, cells);
@@ -661,6 +663,7 @@ int engine_alloc(nb_mpm* h, void* hip_stream) {
   if (hipMemsetAsync(d.tile_act, 0xFF, size_t(h->NT) * 4, h->stream) != hipSuccess)
     return fail(NB_ERR_HIP, "hipMemsetAsync failed");
   launch_fill_identity(d.all_tiles, h->NT, h->stream);
+  launch_fill_identity(d.tile_order, h->NT, h->stream);
   {
     uint32_t all = h->NT;
     if (hipMemcpyAsync(d.counters + CNT_ALL_TILES, &all, 4, hipMemcpyHostToDevice, h->stream) != hipSuccess ||
@@ -1608,6 +1611,7 @@ int adj_alloc_bins(nb_mpm* h, DevState& b) {
   if (rc == NB_OK) rc = adj_alloc(h, &b.node_tiles, h->NT);
   if (rc == NB_OK) rc = adj_alloc(h, &b.tile_total, h->NT);
   if (rc == NB_OK) rc = adj_alloc(h, &b.tile_start, h->NT);
+  if (rc == NB_OK) rc = adj_alloc(h, &b.tile_order, h->NT);
   if (rc == NB_OK) rc = adj_alloc(h, &b.counters, CNT_SLOTS);
   if (rc == NB_OK &&
       (hipMemsetAsync(b.cell_count, 0, cells * 4, h->stream) != hipSuccess ||
@@ -1616,6 +1620,7 @@ int adj_alloc_bins(nb_mpm* h, DevState& b) {
        hipMemsetAsync(b.counters, 0, CNT_SLOTS * 4, h->stream) != hipSuccess ||
        hipMemsetAsync(b.tile_act, 0xFF, size_t(h->NT) * 4, h->stream) != hipSuccess))
     rc = fail(NB_ERR_HIP, "hipMemsetAsync failed");
+  if (rc == NB_OK) launch_fill_identity(b.tile_order, h->NT, h->stream);
   return rc;
 }
 void adj_free(nb_mpm* h) {

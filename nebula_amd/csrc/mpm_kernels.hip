@@ -188,9 +188,10 @@ __global__ __launch_bounds__(512) void k_bin_tile_scan(const uint32_t* __restric
 // together as the low and high halves of one 64-bit sum.
 __global__ __launch_bounds__(1024) void k_bin_tile_offsets(const uint32_t* __restrict__ tile_total,
                                                            uint32_t* __restrict__ tile_start,
+                                                           uint32_t* __restrict__ tile_order,
                                                            uint32_t* __restrict__ counters,
                                                            uint2* __restrict__ units) {
-  __shared__ uint64_t s_wave[1024 / 64 + 1];
+  __shared__ uint64_t s_wave[1024 / 64 + 1], s_wave2[1024 / 64 + 1];
   const uint32_t na = counters[CNT_ACTIVE];
   const uint32_t chunk = (na + 1023u) / 1024u;
   const uint32_t lo = min(na, threadIdx.x * chunk), hi = min(na, lo + chunk);
@@ -210,6 +211,34 @@ __global__ __launch_bounds__(1024) void k_bin_tile_offsets(const uint32_t* __res
     for (uint32_t c = 0, nu = units_of(t); c < nu; c++) units[uo++] = make_uint2(a, c);
   }
   if (threadIdx.x == 0) counters[CNT_UNITS] = uint32_t(tot >> 32);
+  // P2G's order of the tiles: four classes by particle count, largest first,
+  // tile order within a class (a stable counting sort; the class counts are
+  // packed 16 bits each into one 64-bit block scan).  A block's work grows
+  // with its tile's particles, so the long tiles start first and the short
+  // ones fill the end of the launch.  Which block takes which tile does not
+  // change any result.
+  auto cls = [](uint32_t t) { return t >= 2048u ? 0u : (t >= 1024u ? 1u : (t >= 512u ? 2u : 3u)); };
+  uint64_t cnt = 0;
+  if (na < 65536u)
+    for (uint32_t a = lo; a < hi; a++) cnt += uint64_t(1) << (16u * cls(tile_total[a]));
+  uint64_t ctot;
+  const uint64_t cpre = block_exclusive_scan<1024>(cnt, s_wave2, ctot);
+  if (na >= 65536u) {
+    for (uint32_t a = lo; a < hi; a++) tile_order[a] = a;
+  } else {
+    uint32_t base[4];
+    uint32_t acc = 0;
+#pragma unroll
+    for (uint32_t c = 0; c < 4u; c++) {
+      base[c] = acc + (uint32_t(cpre >> (16u * c)) & 0xFFFFu);
+      acc += uint32_t(ctot >> (16u * c)) & 0xFFFFu;
+    }
+    for (uint32_t a = lo; a < hi; a++) {
+      const uint32_t c = cls(tile_total[a]);
+      const uint32_t pos = c == 0u ? base[0]++ : (c == 1u ? base[1]++ : (c == 2u ? base[2]++ : base[3]++));
+      tile_order[pos] = a;
+    }
+  }
 }
 
 // Place every particle at tile start + cell start + its rank.
@@ -412,6 +441,7 @@ __global__ __launch_bounds__(kP2GThreads, 2) void k_p2g(SubParams p, const float
                                                      const uint32_t* __restrict__ cell_end,
                                                      const uint32_t* __restrict__ sorted,
                                                      const uint32_t* __restrict__ act_tiles,
+                                                     const uint32_t* __restrict__ tile_order,
                                                      const uint32_t* __restrict__ tile_start,
                                                      const uint32_t* __restrict__ tile_total,
                                                      const uint32_t* __restrict__ counters,
@@ -428,7 +458,10 @@ __global__ __launch_bounds__(kP2GThreads, 2) void k_p2g(SubParams p, const float
   const bool apic = p.flip_ratio < -0.5f;
   const uint32_t mlast = p.material_count - 1u;
 
-  for (uint32_t a = blockIdx.x; a < na; a += gridDim.x) {
+  for (uint32_t a0 = blockIdx.x; a0 < na; a0 += gridDim.x) {
+    // the tile of place a0 in the largest-first order (its own index if the
+    // order were ever out of range)
+    const uint32_t ao = tile_order[a0], a = ao < na ? ao : a0;
     const uint32_t T = act_tiles[a];
     const SceneGeo g = tile_geo<BATCH>(p, T);
     const float D_inv = 4.f / (g.h * g.h);
@@ -1639,7 +1672,7 @@ void launch_bin(const DevState& d, const SubParams& p, uint32_t n_tiles, bool ke
                                                      d.node_tiles, d.counters, n_tiles);
   }
   { ProfScope ps(pf, 4, s); k_bin_tile_scan<<<tg, 512, 0, s>>>(d.cell_count, d.cell_end, d.act_tiles, d.tile_total, d.counters); }
-  { ProfScope ps(pf, 5, s); k_bin_tile_offsets<<<1, 1024, 0, s>>>(d.tile_total, d.tile_start, d.counters, d.units); }
+  { ProfScope ps(pf, 5, s); k_bin_tile_offsets<<<1, 1024, 0, s>>>(d.tile_total, d.tile_start, d.tile_order, d.counters, d.units); }
   { ProfScope ps(pf, 6, s); k_bin_scatter<<<cdiv(p.n, 256), 256, 0, s>>>(p.n, d.keys, d.rank, d.cell_end, d.tile_act, d.tile_start, d.sorted); }
   { ProfScope ps(pf, 7, s); k_bin_stabilize<<<tg, 512, 0, s>>>(p.n, d.cell_count, d.cell_end, d.act_tiles, d.tile_start, d.sorted, d.counters); }
 }
@@ -1648,12 +1681,12 @@ void launch_p2g(const DevState& d, const SubParams& p, uint32_t n_tiles, hipStre
   ProfScope ps(pf, 8, s);
   if (p.n_scenes > 0)
     k_p2g<true><<<tile_grid(n_tiles, p.tile_blocks), kP2GThreads, 0, s>>>(p, d.rec, d.materials, d.cell_count, d.cell_end, d.sorted,
-                                                           d.act_tiles, d.tile_start, d.tile_total, d.counters,
-                                                           d.partial);
+                                                           d.act_tiles, d.tile_order, d.tile_start, d.tile_total,
+                                                           d.counters, d.partial);
   else
     k_p2g<false><<<tile_grid(n_tiles, p.tile_blocks), kP2GThreads, 0, s>>>(p, d.rec, d.materials, d.cell_count, d.cell_end,
-                                                            d.sorted, d.act_tiles, d.tile_start, d.tile_total,
-                                                            d.counters, d.partial);
+                                                            d.sorted, d.act_tiles, d.tile_order, d.tile_start,
+                                                            d.tile_total, d.counters, d.partial);
 }
 
 void launch_grid(const DevState& d, const SubParams& p, uint32_t n_tiles, bool from_partials, bool update,

@@ -147,6 +147,7 @@ struct DevState {
   uint32_t* all_tiles;    // NT: 0..NT-1 (dense pass-level mode)
   uint32_t* tile_total;   // NT
   uint32_t* tile_start;   // NT
+  uint32_t* tile_order;   // NT: the occupied tiles' indices, most particles first (P2G's order)
   uint32_t* counters;     // CNT_SLOTS
   uint2* units;           // NT + cap / kG2PUnit + 1: (active tile, chunk) work units of k_g2p_tile
   float4* fb_queue;       // (cap + kFbSlots) * kFbEntry: particles G2P leaves to k_g2p_fb (FP64 fallback)
