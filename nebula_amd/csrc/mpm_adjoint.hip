@@ -577,13 +577,16 @@ __global__ __launch_bounds__(kAdjThreads) void k_adj_scatter(SubParams p, const 
                                                              const uint32_t* __restrict__ cell_end,
                                                              const uint32_t* __restrict__ sorted,
                                                              const uint32_t* __restrict__ act_tiles,
+                                                             const uint32_t* __restrict__ tile_order,
                                                              const uint32_t* __restrict__ counters,
                                                              float4* __restrict__ partial_adj) {
   __shared__ float s_x[kBoxNodes], s_y[kBoxNodes], s_z[kBoxNodes];
   __shared__ uint16_t s_cell[kTileCells];
   __shared__ uint32_t s_wave[kAdjThreads / 64 + 1];
   const uint32_t na = counters[CNT_ACTIVE];
-  for (uint32_t a = blockIdx.x; a < na; a += gridDim.x) {
+  for (uint32_t a0 = blockIdx.x; a0 < na; a0 += gridDim.x) {
+    // largest tiles first, as in the forward's P2G
+    const uint32_t ao = tile_order[a0], a = ao < na ? ao : a0;
     const uint32_t T = act_tiles[a];
     const SceneGeo sg = tile_geo<BATCH>(p, T);
     int tx, ty, tz;
@@ -957,7 +960,8 @@ static void adjoint_kernels(const DevState& d, const AdjState& a, const SubParam
   k_g2p_bwd<BATCH><<<nblk, 256, 0, s>>>(p, d.rec, adj_in, adj_out, a.coef, a.pg_acc,
                                          d.materials, d.sorted, d.grid_node, d.sdf);
   k_adj_scatter<BATCH><<<tile_grid_a(n_tiles, p.tile_blocks), kAdjThreads, 0, s>>>(p, d.rec, a.coef, d.cell_count, d.cell_end,
-                                                                    d.sorted, d.act_tiles, d.counters, a.partial_adj);
+                                                                    d.sorted, d.act_tiles, d.tile_order, d.counters,
+                                                                    a.partial_adj);
   if (p.noise_forces)
     k_grid_bwd<BATCH, true><<<tile_grid_a(n_tiles, p.tile_blocks), 512, 0, s>>>(
         p, d.grid_pre, a.partial_adj, d.tile_act, d.node_tiles, d.counters + CNT_NODE_TILES, a.grid_adj, d.sdf, d.forces);
