@@ -61,18 +61,18 @@ SIMDS, CLOCK_HZ, VALU_CYCLES = 1024, 2.4e9, 4
 # counters.  A counter pass cannot run inside the timed bench; the commands and
 # the per-kernel table are in the file named below.
 TRAFFIC_SOURCE = "profiles/hbm_traffic_snow_terrain_256.md"
-MEASURED_TRAFFIC = {"g2p": 1495.7e6, "p2g": 613.5e6, "grid_update": 120.9e6}
+MEASURED_TRAFFIC = {"g2p": 1495.2e6, "p2g": 613.6e6, "grid_update": 120.9e6}
 # VALU instructions (wave64) per launch over the default window, from a
 # rocprofv3 SQ_INSTS_VALU pass of this build (file below): with the live
 # average launch time they give the fraction of the VALU issue peak.
 VALU_SOURCE = "profiles/counters_snow_terrain_256.md"
-MEASURED_VALU = {"g2p": 1.823e8, "p2g": 5.292e7, "grid_update": 5.518e6}
+MEASURED_VALU = {"g2p": 1.823e8, "p2g": 5.299e7, "grid_update": 5.518e6}
 # sha256 (first 16 hex digits) of the forward kernels' sources the two tables
 # above were measured on; the bench line says whether the running build's
 # sources still hash to it (if not, the counter passes are due again)
 COUNTER_SOURCES = ("nebula_amd/csrc/mpm_kernels.hip", "nebula_amd/csrc/nb_device_math.hpp",
                    "nebula_amd/csrc/nb_kernel_util.hpp")
-COUNTER_SOURCES_SHA = "5979138be1366c2e"
+COUNTER_SOURCES_SHA = "01205c9029ebebae"
 
 
 def counter_sources_sha():
